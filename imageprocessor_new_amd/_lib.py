@@ -1,0 +1,124 @@
+"""ctypes binding of libsubpx_hip.so (declarations: include/subpx_hip.h).
+
+The library is built in-tree by ``python -m imageprocessor_new_amd._build``. Nothing here
+falls back to another implementation: a missing or unloadable library raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libsubpx_hip.so"
+
+SPX_OK = 0
+SPX_E_ARG = -1
+SPX_E_NOMEM = -2
+SPX_E_HIP = -3
+SPX_E_UNSUPPORTED = -4
+SPX_E_CAPACITY = -5
+
+FIT_CONVERGED = 0
+FIT_MAX_ITER = 1
+FIT_SINGULAR = 2
+FIT_DIVERGED = 3
+FIT_TOO_FEW = 4
+FIT_OUTSIDE = 5
+FIT_BAD_START = 6
+FIT_STATUS_NAMES = {
+    FIT_CONVERGED: "converged",
+    FIT_MAX_ITER: "max_iter",
+    FIT_SINGULAR: "singular",
+    FIT_DIVERGED: "diverged",
+    FIT_TOO_FEW: "too_few_pixels",
+    FIT_OUTSIDE: "center_outside_patch",
+    FIT_BAD_START: "bad_start",
+}
+
+MODEL_ERF_DISK6 = 0
+MODEL_ERF_ELLIPSE7 = 1
+
+
+class SpxFitOptions(ctypes.Structure):
+    _fields_ = [
+        ("max_iter", ctypes.c_int32),
+        ("step_tol", ctypes.c_float),
+        ("lambda0", ctypes.c_float),
+        ("lambda_up", ctypes.c_float),
+        ("lambda_down", ctypes.c_float),
+        ("init_sigma", ctypes.c_float),
+        ("tile_waves", ctypes.c_int32),
+        ("persistent", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+# name -> (restype, argtypes); every symbol include/subpx_hip.h declares.
+SIGNATURES = {
+    "spx_init": (_I32, [ctypes.c_int]),
+    "spx_release": (None, []),
+    "spx_last_error": (ctypes.c_char_p, []),
+    "spx_version": (ctypes.c_char_p, []),
+    "spx_otsu_u16": (_I32, [_P, _I64, _I64, _I64, _I32, _P, _P]),
+    "spx_threshold_cc": (_I32, [_P, _I64, _I64, _I64, _I32, _I32, _I32, _P, _P, _P, _I32, _P, _P]),
+    "spx_refine_logquad": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _P, _P, _P]),
+    "spx_fit_stats": (_I32, [_P, _P]),
+    "spx_fit_batch": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions),
+                             _I32, _P, _P, _P, _P, _P]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class SpxError(RuntimeError):
+    """A libsubpx_hip call failed; ``code`` is the SPX_E_* value."""
+
+    def __init__(self, code: int, name: str, text: str):
+        super().__init__(f"{name} failed ({code}): {text}")
+        self.code = code
+
+
+def load() -> ctypes.CDLL:
+    """Load the library once and declare every exported signature."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m imageprocessor_new_amd._build` first"
+            )
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error() -> str:
+    return (load().spx_last_error() or b"").decode()
+
+
+def call(name: str, *args) -> int:
+    """Call ``name``; raise ValueError for argument errors and SpxError otherwise."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc == SPX_OK:
+        return rc
+    text = last_error()
+    if rc == SPX_E_ARG:
+        raise ValueError(f"{name}: {text}")
+    raise SpxError(rc, name, text)
+
+
+def fit_options(*, max_iter=30, step_tol=1e-6, lambda0=1e-3, lambda_up=10.0, lambda_down=0.1,
+                init_sigma=1.0, tile_waves=0, persistent=1) -> SpxFitOptions:
+    return SpxFitOptions(int(max_iter), float(step_tol), float(lambda0), float(lambda_up),
+                         float(lambda_down), float(init_sigma), int(tile_waves), int(persistent))

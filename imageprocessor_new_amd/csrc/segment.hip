@@ -1,0 +1,493 @@
+// Integer front end on the GPU: 65,536-bin histogram + Otsu threshold, threshold +
+// connected components with stats in OpenCV's layout.
+//
+// Reference calls replaced: cv2.threshold(g, 0, 255, BINARY|OTSU) (reference
+// subpx/_cpu/centers.py:258, global variant subpx/_gpu/centers.py:1789-1796) and
+// cv2.connectedComponentsWithStats (reference subpx/_cpu/components.py:18-34, GPU twin
+// subpx/_gpu/components.py:64-134 which uses cupyx label + an atomic stats kernel).
+//
+// Layout: HBM-bound byte work. The histogram is privatised per workgroup in LDS as packed
+// 16-bit counters (65,536 bins = 128 KiB) over at most 65,535 pixels per workgroup, then
+// flushed with one global atomic per non-empty bin. Components use block-based union-find:
+// for 8-connectivity every 2x2 pixel block is one node (its foreground pixels are
+// mutually adjacent), which quarters the nodes and makes the smallest root the component's
+// first block in block-raster order, i.e. OpenCV's numbering; 4-connectivity uses pixels.
+// Labels are numbered by an exclusive scan over root flags.
+#include <hipcub/hipcub.hpp>
+
+#include "spx_internal.hpp"
+
+namespace spx {
+namespace {
+
+constexpr int kHistBins = 65536;
+constexpr int kHistThreads = 1024;
+constexpr int kHistPixelsPerThread = 63;  // 1024 * 63 < 65536: a 16-bit counter cannot wrap
+
+__global__ void fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// hist[v] += count of img[y*step][x*step] == v over the strided view (Hs x Ws).
+__global__ __launch_bounds__(kHistThreads) void hist_u16_kernel(const uint16_t* img, int64_t Hs,
+                                                                int64_t Ws, int64_t pitch,
+                                                                int step, uint32_t* hist) {
+    __shared__ uint32_t lds[kHistBins / 2];
+    for (int i = threadIdx.x; i < kHistBins / 2; i += blockDim.x) lds[i] = 0;
+    __syncthreads();
+    const int64_t n = Hs * Ws;
+    const int64_t per_block = (int64_t)kHistThreads * kHistPixelsPerThread;
+    const int64_t base = (int64_t)blockIdx.x * per_block;
+    for (int k = 0; k < kHistPixelsPerThread; ++k) {
+        const int64_t idx = base + (int64_t)k * kHistThreads + threadIdx.x;
+        if (idx < n) {
+            const int64_t y = idx / Ws;
+            const int64_t x = idx - y * Ws;
+            const uint32_t v = img[y * step * pitch + x * step];
+            atomicAdd(&lds[v >> 1], 1u << ((v & 1u) * 16u));
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistBins / 2; i += blockDim.x) {
+        const uint32_t w = lds[i];
+        if (w & 0xFFFFu) atomicAdd(&hist[2 * i], w & 0xFFFFu);
+        if (w >> 16) atomicAdd(&hist[2 * i + 1], w >> 16);
+    }
+}
+
+// Otsu over the histogram, one workgroup of 1024 threads, 64 bins each.
+// With exact integer class counts cnt(i) = sum_{j<=i} h_j and moments, OpenCV's
+// recurrence (q1 += p_i; mu1 = (mu1 q1_prev + i p_i) / q1; skip while min(q1, q2) <
+// FLT_EPSILON) evaluates sigma(i) = q1 q2 (mu1 - mu2)^2 with mu1 = m1 / cnt and
+// mu2 = (S - m1) / (N - cnt), where m1 sums j h_j from the first bin that is not skipped
+// (bins skipped at the start never enter mu1). The first maximum wins.
+__global__ __launch_bounds__(1024) void otsu_kernel(const uint32_t* hist, int32_t* out) {
+    __shared__ unsigned long long s_cnt[1024];
+    __shared__ unsigned long long s_mom[1024];
+    __shared__ double s_best[1024];
+    __shared__ int s_arg[1024];
+    __shared__ int s_first;
+    const int t = threadIdx.x;
+    const int b0 = t * 64;
+    unsigned long long c = 0, m = 0;
+    for (int i = 0; i < 64; ++i) {
+        const unsigned long long h = hist[b0 + i];
+        c += h;
+        m += h * (unsigned long long)(b0 + i);
+    }
+    s_cnt[t] = c;
+    s_mom[t] = m;
+    if (t == 0) s_first = kHistBins;
+    __syncthreads();
+    // inclusive scan over the 1024 chunk totals (Hillis-Steele)
+    for (int off = 1; off < 1024; off <<= 1) {
+        unsigned long long a = 0, bm = 0;
+        if (t >= off) {
+            a = s_cnt[t - off];
+            bm = s_mom[t - off];
+        }
+        __syncthreads();
+        s_cnt[t] += a;
+        s_mom[t] += bm;
+        __syncthreads();
+    }
+    const unsigned long long N = s_cnt[1023];
+    const unsigned long long S = s_mom[1023];
+    const unsigned long long cnt_before = t ? s_cnt[t - 1] : 0ull;
+    const double scale = 1.0 / (double)N;
+    const double eps = (double)__FLT_EPSILON__;
+    // first bin that is not skipped
+    {
+        unsigned long long cc = cnt_before;
+        for (int i = 0; i < 64; ++i) {
+            cc += hist[b0 + i];
+            const double q1 = (double)cc * scale;
+            const double q2 = 1.0 - q1;
+            if (!(fmin(q1, q2) < eps || fmax(q1, q2) > 1.0 - eps)) {
+                atomicMin(&s_first, b0 + i);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    const int i0 = s_first;
+    // moment of the skipped prefix (bins < i0) is excluded from m1
+    unsigned long long mom_skip = 0;
+    if (i0 < kHistBins) {
+        // all threads compute the same prefix moment below i0 from the chunk scan
+        const int tc = i0 / 64;
+        mom_skip = tc ? s_mom[tc - 1] : 0ull;
+        for (int i = tc * 64; i < i0; ++i) mom_skip += (unsigned long long)hist[i] * (unsigned long long)i;
+    }
+    double best = 0.0;
+    int arg = 0x7fffffff;
+    {
+        unsigned long long cc = cnt_before;
+        unsigned long long mm = t ? s_mom[t - 1] : 0ull;
+        for (int i = 0; i < 64; ++i) {
+            const int bin = b0 + i;
+            const unsigned long long h = hist[bin];
+            cc += h;
+            mm += h * (unsigned long long)bin;
+            if (bin < i0) continue;
+            const double q1 = (double)cc * scale;
+            const double q2 = 1.0 - q1;
+            if (fmin(q1, q2) < eps || fmax(q1, q2) > 1.0 - eps) continue;
+            const double m1 = (double)(mm - mom_skip);
+            const double mu1 = m1 / (double)cc;
+            const double mu2 = ((double)S - m1) / (double)(N - cc);
+            const double d = mu1 - mu2;
+            const double sigma = q1 * q2 * d * d;
+            if (sigma > best) {
+                best = sigma;
+                arg = bin;
+            }
+        }
+    }
+    s_best[t] = best;
+    s_arg[t] = arg;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off) {
+            const double ob = s_best[t + off];
+            const int oa = s_arg[t + off];
+            if (ob > s_best[t] || (ob == s_best[t] && oa < s_arg[t])) {
+                s_best[t] = ob;
+                s_arg[t] = oa;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) out[0] = (s_best[0] > 0.0 && s_arg[0] != 0x7fffffff) ? s_arg[0] : 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Connected components. BS = 2: 2x2 pixel blocks (8-connectivity); BS = 1: pixels (4).
+
+struct CCGeom {
+    int64_t H, W, pitch;
+    int64_t NH, NW;  // node grid
+};
+
+__device__ __forceinline__ bool is_fg(const uint16_t* img, const CCGeom& g, int64_t y, int64_t x,
+                                      int32_t thresh, int invert) {
+    if (y < 0 || y >= g.H || x < 0 || x >= g.W) return false;
+    const int32_t v = img[y * g.pitch + x];
+    return invert ? (v <= thresh) : (v > thresh);
+}
+
+// Foreground bits of a node: bit (2*dy + dx) for BS = 2, bit 0 for BS = 1.
+template <int BS>
+__device__ __forceinline__ uint32_t node_bits(const uint16_t* img, const CCGeom& g, int64_t ny,
+                                              int64_t nx, int32_t thresh, int invert) {
+    if (BS == 1) return is_fg(img, g, ny, nx, thresh, invert) ? 1u : 0u;
+    const int64_t y = 2 * ny, x = 2 * nx;
+    return (is_fg(img, g, y, x, thresh, invert) ? 1u : 0u) |
+           (is_fg(img, g, y, x + 1, thresh, invert) ? 2u : 0u) |
+           (is_fg(img, g, y + 1, x, thresh, invert) ? 4u : 0u) |
+           (is_fg(img, g, y + 1, x + 1, thresh, invert) ? 8u : 0u);
+}
+
+template <int BS>
+__global__ void cc_init_kernel(const uint16_t* img, CCGeom g, int32_t thresh, int invert,
+                               int32_t* parent, uint8_t* bits) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.NH * g.NW) return;
+    const int64_t ny = n / g.NW, nx = n - (n / g.NW) * g.NW;
+    const uint32_t b = node_bits<BS>(img, g, ny, nx, thresh, invert);
+    bits[n] = (uint8_t)b;
+    parent[n] = b ? (int32_t)n : -1;
+}
+
+__device__ __forceinline__ int32_t uf_find(const int32_t* parent, int32_t a) {
+    int32_t p = parent[a];
+    while (p != a) {
+        a = p;
+        p = parent[a];
+    }
+    return a;
+}
+
+// Lock-free union: link the larger root under the smaller one with atomicMin; if the
+// larger root was re-linked meanwhile, continue from what it points to.
+__device__ __forceinline__ void uf_union(int32_t* parent, int32_t a, int32_t b) {
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        const int32_t old = atomicMin(&parent[b], a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+template <int BS>
+__global__ void cc_merge_kernel(CCGeom g, int32_t* parent, const uint8_t* bits) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.NH * g.NW) return;
+    const uint32_t b = bits[n];
+    if (!b) return;
+    const int64_t ny = n / g.NW, nx = n - (n / g.NW) * g.NW;
+    if (BS == 1) {
+        if (nx > 0 && bits[n - 1]) uf_union(parent, (int32_t)n, (int32_t)(n - 1));
+        if (ny > 0 && bits[n - g.NW]) uf_union(parent, (int32_t)n, (int32_t)(n - g.NW));
+        return;
+    }
+    // left block: its right column (bits 1, 3) touches our left column (bits 0, 2)
+    if (nx > 0 && (b & 5u)) {
+        const uint32_t l = bits[n - 1];
+        if (l & 10u) uf_union(parent, (int32_t)n, (int32_t)(n - 1));
+    }
+    if (ny > 0) {
+        const int64_t u = n - g.NW;
+        // upper block: its bottom row (bits 4, 8) touches our top row (bits 1, 2)
+        if ((b & 3u) && (bits[u] & 12u)) uf_union(parent, (int32_t)n, (int32_t)u);
+        // upper-left: its bottom-right pixel (bit 8) touches our top-left (bit 1)
+        if (nx > 0 && (b & 1u) && (bits[u - 1] & 8u)) uf_union(parent, (int32_t)n, (int32_t)(u - 1));
+        // upper-right: its bottom-left pixel (bit 4) touches our top-right (bit 2)
+        if (nx + 1 < g.NW && (b & 2u) && (bits[u + 1] & 4u)) uf_union(parent, (int32_t)n, (int32_t)(u + 1));
+    }
+}
+
+__global__ void cc_compress_kernel(int64_t nn, int32_t* parent, int32_t* root_flag) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const int32_t p = parent[n];
+    if (p < 0) {
+        root_flag[n] = 0;
+        return;
+    }
+    const int32_t r = uf_find(parent, (int32_t)n);
+    parent[n] = r;
+    root_flag[n] = (r == (int32_t)n) ? 1 : 0;
+}
+
+struct CCAcc {
+    int32_t xmin, ymin, xmax, ymax;
+    uint32_t area, pad;
+    unsigned long long sx, sy;
+};
+
+__global__ void cc_acc_init_kernel(CCAcc* acc, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    acc[i] = CCAcc{0x7fffffff, 0x7fffffff, -1, -1, 0u, 0u, 0ull, 0ull};
+}
+
+// Per node: write the labels of its pixels and add its pixels to the label's stats
+// (one set of atomics per node). Background pixels go to row 0 through a workgroup
+// reduction.
+template <int BS>
+__global__ __launch_bounds__(256) void cc_label_kernel(CCGeom g, const int32_t* parent,
+                                                       const uint8_t* bits, const int32_t* ids,
+                                                       int32_t* labels, CCAcc* acc, int32_t cap) {
+    __shared__ int32_t s_min[2][256];
+    __shared__ int32_t s_max[2][256];
+    __shared__ unsigned long long s_sum[3][256];
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t bxmin = 0x7fffffff, bymin = 0x7fffffff, bxmax = -1, bymax = -1;
+    unsigned long long bcnt = 0, bsx = 0, bsy = 0;
+    if (n < g.NH * g.NW) {
+        const uint32_t b = bits[n];
+        const int64_t ny = n / g.NW, nx = n - (n / g.NW) * g.NW;
+        int32_t lab = 0;
+        if (b) lab = ids[parent[n]] + 1;
+        int32_t xmin = 0x7fffffff, ymin = 0x7fffffff, xmax = -1, ymax = -1;
+        uint32_t cnt = 0;
+        unsigned long long sx = 0, sy = 0;
+        for (int dy = 0; dy < BS; ++dy) {
+            for (int dx = 0; dx < BS; ++dx) {
+                const int64_t y = ny * BS + dy, x = nx * BS + dx;
+                if (y >= g.H || x >= g.W) continue;
+                const bool fg = (b >> (BS == 2 ? (2 * dy + dx) : 0)) & 1u;
+                labels[y * g.W + x] = fg ? lab : 0;
+                if (fg) {
+                    xmin = min(xmin, (int32_t)x);
+                    ymin = min(ymin, (int32_t)y);
+                    xmax = max(xmax, (int32_t)x);
+                    ymax = max(ymax, (int32_t)y);
+                    ++cnt;
+                    sx += (unsigned long long)x;
+                    sy += (unsigned long long)y;
+                } else {
+                    bxmin = min(bxmin, (int32_t)x);
+                    bymin = min(bymin, (int32_t)y);
+                    bxmax = max(bxmax, (int32_t)x);
+                    bymax = max(bymax, (int32_t)y);
+                    ++bcnt;
+                    bsx += (unsigned long long)x;
+                    bsy += (unsigned long long)y;
+                }
+            }
+        }
+        if (cnt && lab < cap) {
+            CCAcc* a = acc + lab;
+            atomicMin(&a->xmin, xmin);
+            atomicMin(&a->ymin, ymin);
+            atomicMax(&a->xmax, xmax);
+            atomicMax(&a->ymax, ymax);
+            atomicAdd(&a->area, cnt);
+            atomicAdd(&a->sx, sx);
+            atomicAdd(&a->sy, sy);
+        }
+    }
+    const int t = threadIdx.x;
+    s_min[0][t] = bxmin;
+    s_min[1][t] = bymin;
+    s_max[0][t] = bxmax;
+    s_max[1][t] = bymax;
+    s_sum[0][t] = bcnt;
+    s_sum[1][t] = bsx;
+    s_sum[2][t] = bsy;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            s_min[0][t] = min(s_min[0][t], s_min[0][t + off]);
+            s_min[1][t] = min(s_min[1][t], s_min[1][t + off]);
+            s_max[0][t] = max(s_max[0][t], s_max[0][t + off]);
+            s_max[1][t] = max(s_max[1][t], s_max[1][t + off]);
+            s_sum[0][t] += s_sum[0][t + off];
+            s_sum[1][t] += s_sum[1][t + off];
+            s_sum[2][t] += s_sum[2][t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0 && s_sum[0][0] && cap > 0) {
+        atomicMin(&acc[0].xmin, s_min[0][0]);
+        atomicMin(&acc[0].ymin, s_min[1][0]);
+        atomicMax(&acc[0].xmax, s_max[0][0]);
+        atomicMax(&acc[0].ymax, s_max[1][0]);
+        atomicAdd(&acc[0].area, (uint32_t)s_sum[0][0]);
+        atomicAdd(&acc[0].sx, s_sum[1][0]);
+        atomicAdd(&acc[0].sy, s_sum[2][0]);
+    }
+}
+
+__global__ void cc_finalize_kernel(const CCAcc* acc, int32_t rows, int32_t* stats, double* cents) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    const CCAcc a = acc[i];
+    if (a.area == 0) {
+        stats[5 * i + 0] = 0;
+        stats[5 * i + 1] = 0;
+        stats[5 * i + 2] = 0;
+        stats[5 * i + 3] = 0;
+        stats[5 * i + 4] = 0;
+        cents[2 * i] = __builtin_nan("");
+        cents[2 * i + 1] = __builtin_nan("");
+        return;
+    }
+    stats[5 * i + 0] = a.xmin;
+    stats[5 * i + 1] = a.ymin;
+    stats[5 * i + 2] = a.xmax - a.xmin + 1;
+    stats[5 * i + 3] = a.ymax - a.ymin + 1;
+    stats[5 * i + 4] = (int32_t)a.area;
+    cents[2 * i] = (double)a.sx / (double)a.area;
+    cents[2 * i + 1] = (double)a.sy / (double)a.area;
+}
+
+inline int grid_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
+
+template <int BS>
+int threshold_cc_impl(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
+                      int32_t invert, int32_t* labels, int32_t* stats, double* cents, int32_t cap,
+                      int32_t* n_out, hipStream_t stream) {
+    CCGeom g{H, W, pitch, (H + BS - 1) / BS, (W + BS - 1) / BS};
+    const int64_t nn = g.NH * g.NW;
+    SPX_REQUIRE(nn < (int64_t)0x7fffffff, "spx_threshold_cc: image too large for int32 node ids");
+    // workspace: parent i32[nn] | flag i32[nn] | ids i32[nn] | acc[cap] | bits u8[nn] | scan temp
+    size_t scan_bytes = 0;
+    SPX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr,
+                                             (int)nn, stream));
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_parent = 0;
+    const size_t o_flag = o_parent + align(4 * nn);
+    const size_t o_ids = o_flag + align(4 * nn);
+    const size_t o_acc = o_ids + align(4 * nn);
+    const size_t n_acc = (size_t)(cap > 0 ? cap : 1);
+    const size_t o_bits = o_acc + align(sizeof(CCAcc) * n_acc);
+    const size_t o_scan = o_bits + align(nn);
+    const size_t total = o_scan + align(scan_bytes) + 256;
+    void* ws = nullptr;
+    int rc = workspace(total, &ws);
+    if (rc != SPX_OK) return rc;
+    char* base = static_cast<char*>(ws);
+    int32_t* parent = reinterpret_cast<int32_t*>(base + o_parent);
+    int32_t* flag = reinterpret_cast<int32_t*>(base + o_flag);
+    int32_t* ids = reinterpret_cast<int32_t*>(base + o_ids);
+    CCAcc* acc = reinterpret_cast<CCAcc*>(base + o_acc);
+    uint8_t* bits = reinterpret_cast<uint8_t*>(base + o_bits);
+    void* scan_tmp = base + o_scan;
+
+    const int T = 256;
+    cc_init_kernel<BS><<<grid_for(nn, T), T, 0, stream>>>(img, g, thresh, invert, parent, bits);
+    SPX_LAUNCHED();
+    cc_merge_kernel<BS><<<grid_for(nn, T), T, 0, stream>>>(g, parent, bits);
+    SPX_LAUNCHED();
+    cc_compress_kernel<<<grid_for(nn, T), T, 0, stream>>>(nn, parent, flag);
+    SPX_LAUNCHED();
+    SPX_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, flag, ids, (int)nn, stream));
+    cc_acc_init_kernel<<<grid_for((int64_t)n_acc, T), T, 0, stream>>>(acc, (int64_t)n_acc);
+    SPX_LAUNCHED();
+    cc_label_kernel<BS><<<grid_for(nn, T), T, 0, stream>>>(g, parent, bits, ids, labels, acc, cap);
+    SPX_LAUNCHED();
+    // number of components = ids[nn-1] + flag[nn-1]
+    int32_t tail[2] = {0, 0};
+    SPX_HIP(hipMemcpyAsync(&tail[0], ids + nn - 1, 4, hipMemcpyDeviceToHost, stream));
+    SPX_HIP(hipMemcpyAsync(&tail[1], flag + nn - 1, 4, hipMemcpyDeviceToHost, stream));
+    SPX_HIP(hipStreamSynchronize(stream));
+    const int32_t num = tail[0] + tail[1] + 1;
+    *n_out = num;
+    const int32_t rows = num < cap ? num : cap;
+    if (rows > 0) {
+        cc_finalize_kernel<<<grid_for(rows, T), T, 0, stream>>>(acc, rows, stats, cents);
+        SPX_LAUNCHED();
+    }
+    if (num > cap) {
+        set_error("spx_threshold_cc: " + std::to_string(num) + " labels exceed capacity " +
+                  std::to_string(cap));
+        return SPX_E_CAPACITY;
+    }
+    return SPX_OK;
+}
+
+}  // namespace
+
+int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
+                int32_t* thresh_out, hipStream_t stream) {
+    const int64_t Hs = (H + step - 1) / step;
+    const int64_t Ws = (W + step - 1) / step;
+    void* ws = nullptr;
+    int rc = workspace(sizeof(uint32_t) * kHistBins + 256, &ws);
+    if (rc != SPX_OK) return rc;
+    uint32_t* hist = static_cast<uint32_t*>(ws);
+    int32_t* dev_out = reinterpret_cast<int32_t*>(hist + kHistBins);
+    fill_u32<<<64, 256, 0, stream>>>(hist, kHistBins, 0u);
+    SPX_LAUNCHED();
+    const int64_t n = Hs * Ws;
+    const int64_t per_block = (int64_t)kHistThreads * kHistPixelsPerThread;
+    const int64_t blocks = (n + per_block - 1) / per_block;
+    hist_u16_kernel<<<(unsigned)blocks, kHistThreads, 0, stream>>>(img, Hs, Ws, pitch, step, hist);
+    SPX_LAUNCHED();
+    otsu_kernel<<<1, 1024, 0, stream>>>(hist, dev_out);
+    SPX_LAUNCHED();
+    SPX_HIP(hipMemcpyAsync(thresh_out, dev_out, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    SPX_HIP(hipStreamSynchronize(stream));
+    return SPX_OK;
+}
+
+int threshold_cc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
+                        int32_t invert, int32_t connectivity, int32_t* labels, int32_t* stats,
+                        double* cents, int32_t cap, int32_t* n_out, hipStream_t stream) {
+    if (connectivity == 8)
+        return threshold_cc_impl<2>(img, H, W, pitch, thresh, invert, labels, stats, cents, cap, n_out, stream);
+    return threshold_cc_impl<1>(img, H, W, pitch, thresh, invert, labels, stats, cents, cap, n_out, stream);
+}
+
+}  // namespace spx

@@ -1,0 +1,109 @@
+"""Batched erf-disk / erf-ellipse LM fits of one patch per feature on the GPU.
+
+No reference function fits this model; the batched per-feature LM replaces the role of
+the reference's `lm_fit` (subpx/_gpu/deblend.py:55-152) for contact-hole centres, radii
+(CD = 2 r) and edge widths. The kernel is spx_fit_batch (include/subpx_hip.h).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .backends import gpu_device, resolve_backend, stream_handle, to_device_u16
+from .types import ErfFitResult
+
+MODELS = {"disk": _lib.MODEL_ERF_DISK6, "erf_disk": _lib.MODEL_ERF_DISK6,
+          "ellipse": _lib.MODEL_ERF_ELLIPSE7, "erf_ellipse": _lib.MODEL_ERF_ELLIPSE7}
+MAX_HALF = 22
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else int(t.data_ptr())
+
+
+def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor,
+                         model: torch.Tensor, init: torch.Tensor | None = None, *,
+                         options: _lib.SpxFitOptions | None = None, out: dict | None = None,
+                         stream: torch.cuda.Stream | None = None) -> dict:
+    """Enqueue spx_fit_batch on device tensors; no host synchronisation.
+
+    img (H, W) uint16, seeds (K, 2) float32, half (K,) int16, model (K,) uint8,
+    init (K, 7) float32 or None. Returns dict of device tensors params/rms/iters/status.
+    """
+    if img.dim() != 2 or img.dtype != torch.uint16:
+        raise ValueError("img must be a 2-D uint16 tensor")
+    K = int(seeds.shape[0])
+    if seeds.shape != (K, 2) or seeds.dtype != torch.float32:
+        raise ValueError("seeds must be (K, 2) float32")
+    if half.shape != (K,) or half.dtype != torch.int16 or model.shape != (K,) or model.dtype != torch.uint8:
+        raise ValueError("half must be (K,) int16 and model (K,) uint8")
+    if init is not None and (init.shape != (K, 7) or init.dtype != torch.float32):
+        raise ValueError("init must be (K, 7) float32")
+    dev = img.device
+    for t in (seeds, half, model) + ((init,) if init is not None else ()):
+        if t.device != dev or not t.is_contiguous():
+            raise ValueError("all inputs must be contiguous tensors on the image's device")
+    if out is None:
+        out = {
+            "params": torch.empty((K, 7), dtype=torch.float32, device=dev),
+            "rms": torch.empty((K,), dtype=torch.float32, device=dev),
+            "iters": torch.empty((K,), dtype=torch.int16, device=dev),
+            "status": torch.empty((K,), dtype=torch.uint8, device=dev),
+        }
+    opts = options if options is not None else _lib.fit_options()
+    H, W = img.shape
+    pitch = img.stride(0)
+    _lib.call("spx_fit_batch", _ptr(img), H, W, pitch, _ptr(seeds), _ptr(half), _ptr(model),
+              _ptr(init), ctypes.byref(opts), K, _ptr(out["params"]), _ptr(out["rms"]),
+              _ptr(out["iters"]), _ptr(out["status"]), stream_handle(stream))
+    return out
+
+
+def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: int = 30,
+                  step_tol: float = 1e-6, lambda0: float = 1e-3, lambda_up: float = 10.0,
+                  lambda_down: float = 0.1, init_sigma: float = 1.0, backend: str = "gpu",
+                  device: int = 0) -> ErfFitResult:
+    """Fit an erf-edged disk (6 parameters) or ellipse (7) to a patch around each seed.
+
+    ``seeds`` (K, 2) [x, y]; the patch is (2 half + 1)^2 pixels centred on rint(seed)
+    (half per feature or scalar, 2..22). ``model`` is "disk" / "ellipse" or a per-feature
+    array of 0/1. ``init`` (K, 7) [x0, y0, r, sigma, A, B, q] replaces the moment start.
+    Fits stop after ``max_iter`` step attempts or when a step below ``step_tol`` px is
+    taken.
+    """
+    resolve_backend(backend)
+    seeds_np = np.asarray(seeds, dtype=np.float32).reshape(-1, 2)
+    K = seeds_np.shape[0]
+    half_np = np.broadcast_to(np.asarray(half, dtype=np.int16), (K,)).copy()
+    if K and (half_np.min() < 2 or half_np.max() > MAX_HALF):
+        raise ValueError(f"half must lie in [2, {MAX_HALF}]")
+    if isinstance(model, str):
+        if model not in MODELS:
+            raise ValueError(f"model must be one of {sorted(MODELS)}")
+        model_np = np.full(K, MODELS[model], dtype=np.uint8)
+    else:
+        model_np = np.broadcast_to(np.asarray(model, dtype=np.uint8), (K,)).copy()
+    opts = _lib.fit_options(max_iter=max_iter, step_tol=step_tol, lambda0=lambda0,
+                            lambda_up=lambda_up, lambda_down=lambda_down, init_sigma=init_sigma)
+    with gpu_device(device) as dev:
+        img_t = to_device_u16(image, dev)
+        if K == 0:
+            empty = np.zeros((0, 7), np.float32)
+            return ErfFitResult(empty, np.zeros(0, np.float32), np.zeros(0, np.int16),
+                                np.zeros(0, np.uint8), model_np)
+        seeds_t = torch.from_numpy(seeds_np).to(dev)
+        half_t = torch.from_numpy(half_np).to(dev)
+        model_t = torch.from_numpy(model_np).to(dev)
+        init_t = None
+        if init is not None:
+            init_t = torch.from_numpy(np.asarray(init, dtype=np.float32).reshape(K, 7)).to(dev)
+        out = fit_erf_disks_device(img_t, seeds_t, half_t, model_t, init_t, options=opts)
+        torch.cuda.current_stream().synchronize()
+        return ErfFitResult(out["params"].cpu().numpy(), out["rms"].cpu().numpy(),
+                            out["iters"].cpu().numpy(), out["status"].cpu().numpy(), model_np,
+                            backend="gpu",
+                            meta={"max_iter": int(max_iter), "step_tol": float(step_tol)})

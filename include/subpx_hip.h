@@ -1,0 +1,125 @@
+/*
+ * subpx_hip.h -- C-ABI of the MI355X-native per-feature subpixel refinement path.
+ *
+ * Every pointer argument marked "device" is a device pointer (e.g. torch.Tensor.data_ptr()
+ * of a tensor on the current HIP device); "host" pointers are ordinary process memory.
+ * The caller owns every buffer. Calls are ordered on the given stream (NULL = the
+ * device's null stream) and return 0 on success or a negative SPX_E* code; the text
+ * of the last error on the calling thread is available from spx_last_error().
+ * Failed fits never make a call fail: they are reported per feature through a status
+ * byte, as the reference reports NaN + ok=False per feature.
+ *
+ * Image convention (reference subpx/synthetic_arrays.py:22-24): points are [x, y] in
+ * pixels, pixel centres at integer coordinates, images row-major (H, W) uint16 with a
+ * row pitch in elements.
+ */
+#ifndef SUBPX_HIP_H
+#define SUBPX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* spx_stream_t; /* a hipStream_t; NULL = null stream */
+
+/* ---- error classes -------------------------------------------------------------- */
+#define SPX_OK 0
+#define SPX_E_ARG (-1)         /* bad argument: shape, size, null pointer, option     */
+#define SPX_E_NOMEM (-2)       /* device workspace could not be allocated             */
+#define SPX_E_HIP (-3)         /* a HIP runtime call failed                            */
+#define SPX_E_UNSUPPORTED (-4) /* valid request this build does not implement         */
+#define SPX_E_CAPACITY (-5)    /* more components than the caller's capacity          */
+
+/* ---- per-feature fit status (status_out bytes) ----------------------------------- */
+#define SPX_FIT_CONVERGED 0    /* a step below step_tol was taken                      */
+#define SPX_FIT_MAX_ITER 1     /* iteration cap reached; parameters are the best seen  */
+#define SPX_FIT_SINGULAR 2     /* the damped normal matrix stayed singular            */
+#define SPX_FIT_DIVERGED 3     /* non-finite cost or parameters                       */
+#define SPX_FIT_TOO_FEW 4      /* fewer in-image patch pixels than the model needs    */
+#define SPX_FIT_OUTSIDE 5      /* the fitted centre left the patch                    */
+#define SPX_FIT_BAD_START 6    /* moment start impossible (flat patch)                */
+
+/* ---- fit models (model bytes) ------------------------------------------------------ */
+#define SPX_MODEL_ERF_DISK6 0    /* B + A/2 erfc((rho-r)/(sqrt2 s)), rho = |p - c|       */
+#define SPX_MODEL_ERF_ELLIPSE7 1 /* as DISK6 with rho = sqrt(dx^2 + (dy/q)^2), q 7th param*/
+
+/* Options of the batched Levenberg-Marquardt erf fits. */
+typedef struct SpxFitOptions {
+    int32_t max_iter;     /* LM step attempts per feature (30)                         */
+    float step_tol;       /* converged when an accepted step moves x0,y0,r,s < this (1e-6 px) */
+    float lambda0;        /* initial Marquardt damping (1e-3)                          */
+    float lambda_up;      /* damping factor after a rejected step (10)                 */
+    float lambda_down;    /* damping factor after an accepted step (0.1)               */
+    float init_sigma;     /* edge sigma of the moment start (1.0 px)                   */
+    int32_t tile_waves;   /* waves per workgroup, 0 = library default (autotuned)      */
+    int32_t persistent;   /* 1 = persistent work-queue launch, 0 = one wave per feature */
+} SpxFitOptions;
+
+/* ---- lifecycle ---------------------------------------------------------------------- */
+/* Select and initialise a HIP device (reference: subpx/backends.py:55-59 gpu_device). */
+int spx_init(int device);
+/* Free every library-owned workspace on every device (reference: subpx/_gpu/_cuda.py:35-46). */
+void spx_release(void);
+/* Text of the last error raised on this thread ("" if none). */
+const char* spx_last_error(void);
+/* Library version string. */
+const char* spx_version(void);
+
+/* ---- integer front end ---------------------------------------------------------------- */
+/* Otsu threshold of a uint16 image over its 65,536-bin histogram.
+ * Replaces cv2.threshold(g, 0, 255, BINARY|OTSU) at subpx/_cpu/centers.py:258 and the
+ * global Otsu of the tiled path at subpx/_gpu/centers.py:1789-1796 (step > 1 samples the
+ * image as img[::step, ::step]). img: device. thresh_out: host. */
+int spx_otsu_u16(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
+                 int32_t* thresh_out, spx_stream_t stream);
+
+/* Threshold (foreground = invert ? g <= thresh : g > thresh) + connected components with
+ * stats, replacing _segment_binary + connected_components_stats_cpu
+ * (subpx/_cpu/centers.py:247-279, subpx/_cpu/components.py:18-34) and
+ * connected_components_stats_gpu (subpx/_gpu/components.py:64-134).
+ * connectivity 4 or 8. Labels: 0 background, 1..n-1 components numbered in the order of
+ * their first pixel in a raster scan over 2x2 blocks (8-connectivity) or pixels
+ * (4-connectivity). img, labels (H*W int32, row pitch W), stats ((cap,5) int32
+ * [x, y, w, h, area], row 0 = background), centroids ((cap,2) f64): device.
+ * n_out (host) receives the label count including the background; SPX_E_CAPACITY when it
+ * exceeds cap (n_out still holds the true count). */
+int spx_threshold_cc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
+                     int32_t invert, int32_t connectivity, int32_t* labels, int32_t* stats,
+                     double* centroids, int32_t cap, int32_t* n_out, spx_stream_t stream);
+
+/* ---- per-feature refinement ------------------------------------------------------------- */
+/* CPU log-quadratic refinement restated per label (subpx/_cpu/centers.py:67-107 with the
+ * ROI rule of :376-391 and the border median of :31-42). rows: device (K,5) int32
+ * [x, y, w, h, label] (bbox from stats); pad: ROI padding. centers_out (K,2) f64 in image
+ * coordinates, ok_out (K) uint8: device. */
+int spx_refine_logquad(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
+                       const int32_t* labels, const int32_t* rows, int32_t K, int32_t pad,
+                       double* centers_out, uint8_t* ok_out, spx_stream_t stream);
+
+/* Batched erf-disk / erf-ellipse Levenberg-Marquardt fit of one patch per feature (no
+ * reference function: the model comes from the task definition; the wave-per-feature LM
+ * follows the reference's lm_fit kernel, subpx/_gpu/deblend.py:55-152, and _fit_disks,
+ * subpx/_cpu/invariant_cd.py:165-246). Patch = (2*half+1)^2 pixels centred on
+ * rint(seed); pixels outside the image are left out. seeds (K,2) f32 [x, y], half (K)
+ * int16 in [2, 22], model (K) uint8 SPX_MODEL_*, init (K,7) f32 or NULL (moment start),
+ * params_out (K,7) f32 [x0, y0, r, sigma, A, B, q] (q = 1 for disks), rms_out (K) f32,
+ * iters_out (K) int16, status_out (K) uint8: device. opt: host. */
+int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                  const int16_t* half, const uint8_t* model, const float* init,
+                  const SpxFitOptions* opt, int32_t K, float* params_out, float* rms_out,
+                  int16_t* iters_out, uint8_t* status_out, spx_stream_t stream);
+
+/* Work counters of the last spx_fit_batch call issued on this thread (no reference
+ * counterpart; used to state algorithmic FLOP rates): out[0] = sum over features of pixel
+ * passes x patch pixels for 6-parameter fits, out[1] the same for 7-parameter fits,
+ * out[2] = pixel passes, out[3] = LM step attempts. out: host uint64[4]. Synchronises
+ * the stream. */
+int spx_fit_stats(uint64_t* out, spx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUBPX_HIP_H */

@@ -1,0 +1,235 @@
+"""FP64 CPU oracle of the batched erf-disk / erf-ellipse Levenberg-Marquardt fit.
+
+TEST INFRASTRUCTURE ONLY. Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the checker.
+
+The erf-disk model has no reference function (the task defines it); the batched LM it
+follows is the reference's wave-per-cell Gaussian LM (reference
+subpx/_gpu/deblend.py:55-152: normal equations :82-107, solve :113-130, accept/reject
+:141-149) and `_fit_disks` (reference subpx/_cpu/invariant_cd.py:165-246). This file
+states the algorithm the HIP kernel runs, step for step, in float64 with the exact
+``scipy.special.erfc``:
+
+  I(x, y) = B + A * erfc(u) / 2,  u = (rho - r) / (sqrt(2) s),
+  rho = sqrt(dx^2 + dy^2)            (disk, parameters x0 y0 r s A B)
+  rho = sqrt(dx^2 + (dy / q)^2)      (ellipse, 7th parameter q)
+
+Patch: (2h+1)^2 pixels centred on rint(seed), pixels outside the image left out.
+Moment start: B = mean of the patch's outer ring; the in-set is the pixels more than
+half the peak above B; centre = weighted centroid of the in-set; r = sqrt(n_in / pi);
+A = mean of (v - B) over the in-set; s = init_sigma; q = 1.
+LM: Marquardt damping M_aa += lambda * max(M_aa, 1e-12 max_a M_aa); LDL^T solve; a
+trial is refused (lambda *= up) when it leaves |x0|,|y0| <= h + 0.5, r > 0.25,
+0.05 < s < 2h + 1 (and 0.25 < q < 4), otherwise accepted when its cost is at most
+cost * (1 + accept_rtol) (lambda *= down, floor 1e-12) and refused otherwise; the fit
+stops with status "converged" when a step with max(|dx0|, |dy0|, |dr|, |ds|, r |dq|) <
+step_tol was taken or refused, else after max_iter attempts with "max_iter".
+
+Pinned by: recovery of noise-free renders to 1e-8 px, the Jacobian against central
+differences, the lattice-sum identity sum_pixels erfc(u)/2 = pi (r^2 + s^2) for r >> s,
+integer-shift and mirror equivariance (tests/test_oracle_erfdisk.py).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy.special import erfc
+
+CONVERGED, MAX_ITER, SINGULAR, DIVERGED, TOO_FEW, OUTSIDE, BAD_START = range(7)
+SQRT2 = np.sqrt(2.0)
+INV_SQRT2PI = 1.0 / np.sqrt(2.0 * np.pi)
+
+
+def extract_patch(img: np.ndarray, seed_xy, half: int):
+    """Values, validity, patch-local coordinates and the integer centre of one patch."""
+    H, W = img.shape
+    cx = float(np.rint(np.float32(seed_xy[0])))
+    cy = float(np.rint(np.float32(seed_xy[1])))
+    side = 2 * half + 1
+    rows = np.arange(side)[:, None] + (int(cy) - half)
+    cols = np.arange(side)[None, :] + (int(cx) - half)
+    valid = (rows >= 0) & (rows < H) & (cols >= 0) & (cols < W)
+    v = np.zeros((side, side), dtype=np.float64)
+    rr = np.clip(rows, 0, H - 1)
+    cc = np.clip(cols, 0, W - 1)
+    v[:, :] = img[np.broadcast_to(rr, (side, side)), np.broadcast_to(cc, (side, side))]
+    v[~valid] = 0.0
+    fy, fx = np.meshgrid(np.arange(side) - half, np.arange(side) - half, indexing="ij")
+    return v, valid, fx.astype(np.float64), fy.astype(np.float64), cx, cy
+
+
+def moment_start(v, valid, half, init_sigma=1.0, ellipse=False):
+    """Internal-order start [x0, y0, r, s, A, (q), B] or None for a flat patch."""
+    side = 2 * half + 1
+    ring = np.zeros((side, side), dtype=bool)
+    ring[0, :] = ring[-1, :] = ring[:, 0] = ring[:, -1] = True
+    ring &= valid
+    B0 = float(v[ring].mean()) if ring.any() else 0.0
+    w = v - B0
+    vmax = float(w[valid].max())
+    ins = valid & (w > 0.5 * vmax)
+    s0 = float(w[ins].sum())
+    nin = int(ins.sum())
+    if not (vmax > 0) or not (s0 > 0) or nin < 1:
+        return None
+    fy, fx = np.meshgrid(np.arange(side) - half, np.arange(side) - half, indexing="ij")
+    x0 = float((w[ins] * fx[ins]).sum() / s0)
+    y0 = float((w[ins] * fy[ins]).sum() / s0)
+    p = [x0, y0, np.sqrt(nin / np.pi), float(init_sigma), s0 / nin]
+    if ellipse:
+        p.append(1.0)
+    p.append(B0)
+    return np.asarray(p, dtype=np.float64)
+
+
+def model_jac(p, fx, fy, ellipse):
+    """Model values and Jacobian (n, P) in internal order at parameters ``p``."""
+    x0, y0, r, s, A = p[:5]
+    q = p[5] if ellipse else 1.0
+    B = p[-1]
+    dx = fx - x0
+    dy = fy - y0
+    rho = np.sqrt(dx * dx + (dy / q) ** 2)
+    safe = np.where(rho > 0, rho, 1.0)
+    u = (rho - r) / (SQRT2 * s)
+    E = 0.5 * erfc(u)
+    ez = np.exp(-u * u)
+    G = A * INV_SQRT2PI / s * ez
+    cols = [np.where(rho > 0, G * dx / safe, 0.0),
+            np.where(rho > 0, G * dy / (q * q) / safe, 0.0),
+            G,
+            G * u * SQRT2,
+            E]
+    if ellipse:
+        cols.append(np.where(rho > 0, G * dy * dy / (q ** 3) / safe, 0.0))
+    cols.append(np.ones_like(E))
+    return B + A * E, np.stack(cols, axis=-1)
+
+
+def ldl_solve(M, g):
+    """LDL^T solve with the kernel's pivot rule; None when a pivot is not positive."""
+    P = M.shape[0]
+    L = np.zeros((P, P))
+    D = np.zeros(P)
+    for j in range(P):
+        d = M[j, j] - np.sum(L[j, :j] ** 2 * D[:j])
+        if not (d > 1e-13 * M[j, j]) or not (d > 0):
+            return None
+        D[j] = d
+        for i in range(j + 1, P):
+            L[i, j] = (M[i, j] - np.sum(L[i, :j] * L[j, :j] * D[:j])) / d
+    y = np.zeros(P)
+    for i in range(P):
+        y[i] = g[i] - np.sum(L[i, :i] * y[:i])
+    x = np.zeros(P)
+    for i in range(P - 1, -1, -1):
+        x[i] = y[i] / D[i] - np.sum(L[i + 1:, i] * x[i + 1:])
+    return x if np.all(np.isfinite(x)) else None
+
+
+def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, max_iter=30, step_tol=1e-6,
+                 lambda0=1e-3, lambda_up=10.0, lambda_down=0.1, init_sigma=1.0,
+                 accept_rtol=1e-6):
+    """Fit one patch. Returns (internal params or NaN, cost, iterations, status, npix)."""
+    P = 7 if ellipse else 6
+    npix = int(valid.sum())
+    side = 2 * half + 1
+    fy, fx = np.meshgrid(np.arange(side) - half, np.arange(side) - half, indexing="ij")
+    fxv, fyv, vv = fx[valid], fy[valid], v[valid]
+    nanp = np.full(P, np.nan)
+    if npix < P + 2:
+        return nanp, np.nan, 0, TOO_FEW, npix
+    p = moment_start(v, valid, half, init_sigma, ellipse) if init is None else np.asarray(init, float)
+    if p is None:
+        return nanp, np.nan, 0, BAD_START, npix
+
+    def evaluate(pp):
+        mod, J = model_jac(pp, fxv, fyv, ellipse)
+        res = vv - mod
+        return J.T @ J, J.T @ res, float(res @ res)
+
+    JTJ, JTr, cost = evaluate(p)
+    lam = float(lambda0)
+    lim = half + 0.5
+    status = MAX_ITER if np.isfinite(cost) else DIVERGED
+    it = 0
+    any_solve = False
+    while status == MAX_ITER and it < max_iter:
+        it += 1
+        M = JTJ.copy()
+        dg = np.diag(M).copy()
+        floor_d = 1e-12 * dg.max()
+        M[np.diag_indices(P)] += lam * np.maximum(dg, floor_d)
+        delta = ldl_solve(M, JTr)
+        if delta is None:
+            lam *= lambda_up
+            continue
+        any_solve = True
+        tr = p + delta
+        step = max(abs(delta[0]), abs(delta[1]), abs(delta[2]), abs(delta[3]))
+        if ellipse:
+            step = max(step, abs(delta[5]) * abs(p[2]))
+        inside = abs(tr[0]) <= lim and abs(tr[1]) <= lim and tr[2] > 0.25 and 0.05 < tr[3] < side
+        if ellipse:
+            inside = inside and 0.25 < tr[5] < 4.0
+        if not inside:
+            lam *= lambda_up
+            continue
+        JTJ_t, JTr_t, cost_t = evaluate(tr)
+        if cost_t <= cost * (1.0 + accept_rtol):
+            p, JTJ, JTr, cost = tr, JTJ_t, JTr_t, cost_t
+            lam = max(lam * lambda_down, 1e-12)
+            if step < step_tol:
+                status = CONVERGED
+        else:
+            lam *= lambda_up
+            if step < step_tol:
+                status = CONVERGED
+    if not any_solve and status == MAX_ITER:
+        status = SINGULAR
+    if not np.isfinite(cost):
+        status = DIVERGED
+    if status <= MAX_ITER and (abs(p[0]) > half or abs(p[1]) > half):
+        status = OUTSIDE
+    return p, cost, it, status, npix
+
+
+def to_output(p_internal, ellipse, cx, cy):
+    """Internal order -> output row [x0, y0, r, s, A, B, q] in image coordinates."""
+    p = p_internal
+    q = p[5] if ellipse else 1.0
+    return np.array([p[0] + cx, p[1] + cy, p[2], p[3], p[4], p[-1], q], dtype=np.float64)
+
+
+def fit_batch(img, seeds, half, model, init=None, **opts):
+    """Oracle counterpart of spx_fit_batch: (params (K,7), rms, iterations, status)."""
+    img = np.asarray(img)
+    seeds = np.asarray(seeds, dtype=np.float64).reshape(-1, 2)
+    K = seeds.shape[0]
+    half = np.broadcast_to(np.asarray(half, dtype=np.int64), (K,))
+    model = np.broadcast_to(np.asarray(model, dtype=np.int64), (K,))
+    params = np.full((K, 7), np.nan)
+    rms = np.full(K, np.nan)
+    iters = np.zeros(K, dtype=np.int64)
+    status = np.zeros(K, dtype=np.int64)
+    for k in range(K):
+        h = int(half[k])
+        ell = bool(model[k] == 1)
+        if h < 2 or h > 31 or model[k] not in (0, 1):
+            status[k] = TOO_FEW
+            continue
+        v, valid, _, _, cx, cy = extract_patch(img, seeds[k], h)
+        p0 = None
+        if init is not None:
+            row = np.asarray(init[k], dtype=np.float64)
+            p0 = [row[0] - cx, row[1] - cy, row[2], row[3], row[4]]
+            if ell:
+                p0.append(row[6])
+            p0.append(row[5])
+        p, cost, it, st, npix = lm_fit_patch(v, valid, h, ell, p0, **opts)
+        params[k] = to_output(p, ell, cx, cy)
+        P = 7 if ell else 6
+        rms[k] = np.sqrt(cost / max(npix - P, 1)) if st not in (TOO_FEW, BAD_START) else np.nan
+        iters[k] = it
+        status[k] = st
+    return params, rms, iters, status

@@ -1,0 +1,95 @@
+"""GPU parity of the batched erf-disk / erf-ellipse LM against the FP64 oracle.
+
+Bar: centres, r and sigma within 1e-3 px of the oracle; A, B within 1e-4 relative;
+identical status codes; bit-identical results when run twice.
+"""
+
+import numpy as np
+import pytest
+
+from imageprocessor_new_amd import synthetic
+from oracle import erfdisk
+
+pytestmark = pytest.mark.gpu
+
+
+def _fit_gpu(img, seeds, half, model, **kw):
+    from imageprocessor_new_amd.fit import fit_erf_disks
+
+    return fit_erf_disks(img, seeds, half=half, model=model, **kw)
+
+
+def _compare(res, ref_params, ref_status, tol_px=1e-3):
+    st = res.status.astype(np.int64)
+    assert (st == ref_status).mean() >= 0.99, (np.bincount(st), np.bincount(ref_status))
+    both = (st <= 1) & (ref_status <= 1)
+    assert both.mean() > 0.95
+    d = np.abs(res.params[both].astype(np.float64) - ref_params[both])
+    print("max |gpu - fp64 oracle| per output x0 y0 r s A B q:", d.max(axis=0))
+    assert d[:, 0].max() <= tol_px and d[:, 1].max() <= tol_px
+    assert d[:, 2].max() <= tol_px and d[:, 3].max() <= tol_px
+    rel = d[:, 4:6] / np.abs(ref_params[both][:, 4:6])
+    assert rel.max() <= 1e-4
+    assert d[:, 6].max() <= 1e-4
+
+
+def test_erf_disk_matches_fp64_oracle_c1_shape():
+    arr = synthetic.make_config("c1", seed=3, H=512, W=512)
+    seeds = np.rint(arr.centers_xy)
+    res = _fit_gpu(arr.image, seeds, 15, "disk")
+    ref_p, ref_rms, ref_it, ref_st = erfdisk.fit_batch(arr.image, seeds, 15, 0)
+    _compare(res, ref_p, ref_st)
+    ok = res.ok & arr.inside
+    err = np.abs(res.params[ok, :2] - arr.centers_xy[ok])
+    assert np.median(err) < 0.02  # statistical accuracy against the truth
+
+
+def test_noise_free_recovery():
+    c = np.array([[40.3, 41.7], [100.45, 37.2], [70.1, 90.9], [20.6, 110.2]])
+    arr = synthetic.render_hole_array((140, 140), c, radius=[8, 7.3, 9.1, 6.2], sigma=[1.2, 0.9, 1.5, 1.0],
+                                      noise=False)
+    res = _fit_gpu(arr.image, c + 0.4, 15, "disk", step_tol=1e-7)
+    assert (res.status == 0).all(), res.status
+    # integer rounding of the image is the only noise: errors well below 1e-3 px
+    assert np.abs(res.params[:, :2] - c).max() < 2e-4
+    assert np.abs(res.params[:, 2] - arr.radius).max() < 2e-4
+
+
+def test_mixed_disks_and_ellipses():
+    arr = synthetic.make_config("c2", seed=5, H=600, W=600)
+    seeds = np.rint(arr.centers_xy)
+    model = arr.ellipse.astype(np.uint8)
+    half = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int), 7, 22)
+    res = _fit_gpu(arr.image, seeds, half, model)
+    ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, seeds, half, model)
+    _compare(res, ref_p, ref_st)
+
+
+def test_border_and_degenerate_patches():
+    arr = synthetic.make_config("c1", seed=7, H=256, W=256)
+    seeds = np.concatenate([np.rint(arr.centers_xy), [[0.0, 0.0], [255.0, 3.0], [128.0, 255.0]]])
+    res = _fit_gpu(arr.image, seeds, 15, "disk")
+    ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, seeds, 15, 0)
+    assert (res.status.astype(np.int64) == ref_st).mean() >= 0.97
+    flat = np.full((64, 64), 700, dtype=np.uint16)
+    res2 = _fit_gpu(flat, [[32.0, 32.0]], 10, "disk")
+    assert res2.status[0] == 6 and not res2.ok[0]  # flat patch: no start
+
+
+def test_deterministic():
+    arr = synthetic.make_config("c1", seed=11, H=384, W=384)
+    seeds = np.rint(arr.centers_xy)
+    a = _fit_gpu(arr.image, seeds, 15, "disk")
+    b = _fit_gpu(arr.image, seeds, 15, "disk")
+    assert np.array_equal(a.params, b.params, equal_nan=True)
+    assert np.array_equal(a.status, b.status)
+
+
+def test_init_start_is_used():
+    arr = synthetic.make_config("c1", seed=2, H=256, W=256)
+    seeds = np.rint(arr.centers_xy)
+    first = _fit_gpu(arr.image, seeds, 15, "disk")
+    again = _fit_gpu(arr.image, seeds, 15, "disk", init=first.params)
+    ok = first.ok & again.ok
+    assert np.abs(again.params[ok, :2] - first.params[ok, :2]).max() < 1e-3
+    assert again.iterations[ok].mean() < first.iterations[ok].mean()

@@ -1,0 +1,131 @@
+// Host emulation of one wave64 running the fit kernel's device logic
+// (imageprocessor_new_amd/csrc/erf_fit_device.hpp), for CPU tests only.
+//
+// Each lane is a std::thread; every cross-lane exchange (shuffle) passes two barriers, so
+// lanes that take different control paths through a shuffle deadlock here. A watchdog
+// turns that into exit code 3 with the shuffle count of every lane.
+#include <atomic>
+#include <barrier>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <thread>
+#include <vector>
+
+#include "subpx_hip.h"
+
+#define SPX_DEV_INLINE inline
+#define SPX_DEV
+
+using std::fmaxf;
+using std::isfinite;
+
+namespace spx {
+namespace fitdev {
+
+struct WaveState {
+    std::barrier<> bar{64};
+    float fv[64];
+    int iv[64];
+    std::atomic<long> shuffles[64];
+};
+thread_local int t_lane = 0;
+thread_local WaveState* t_wave = nullptr;
+
+inline int spx_lane() { return t_lane; }
+inline float spx_shfl_xor(float v, int off) {
+    WaveState& w = *t_wave;
+    w.shuffles[t_lane]++;
+    w.fv[t_lane] = v;
+    w.bar.arrive_and_wait();
+    float r = w.fv[t_lane ^ off];
+    w.bar.arrive_and_wait();
+    return r;
+}
+inline int spx_shfl(int v, int src) {
+    WaveState& w = *t_wave;
+    w.shuffles[t_lane]++;
+    w.iv[t_lane] = v;
+    w.bar.arrive_and_wait();
+    int r = w.iv[src];
+    w.bar.arrive_and_wait();
+    return r;
+}
+inline int spx_atomic_add(int32_t* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long spx_atomic_add(unsigned long long* p, unsigned long long v) {
+    return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
+}
+inline float hw_rcp(float x) { return 1.0f / x; }
+inline float hw_rsq(float x) { return 1.0f / std::sqrt(x); }
+inline float hw_exp2(float x) { return std::exp2(x); }
+inline double hw_rcp64(double d) { return 1.0 / d; }
+
+}  // namespace fitdev
+}  // namespace spx
+
+#include "erf_fit_device.hpp"
+
+using namespace spx::fitdev;
+
+template <int NPL>
+static void run_wave(const FitArgs& a, int timeout_s) {
+    WaveState w;
+    for (auto& s : w.shuffles) s = 0;
+    std::atomic<int> done{0};
+    std::vector<std::thread> lanes;
+    for (int l = 0; l < 64; ++l) {
+        lanes.emplace_back([&, l] {
+            t_lane = l;
+            t_wave = &w;
+            erf_fit_wave<NPL>(a);
+            done++;
+        });
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    while (done.load() < 64) {
+        std::this_thread::sleep_for(std::chrono::milliseconds(5));
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(timeout_s)) {
+            std::fprintf(stderr, "wave emulation stuck: %d of 64 lanes finished; shuffles per lane:", done.load());
+            for (int l = 0; l < 64; ++l) std::fprintf(stderr, " %ld", w.shuffles[l].load());
+            std::fprintf(stderr, "\n");
+            std::fflush(stderr);
+            std::_Exit(3);
+        }
+    }
+    for (auto& t : lanes) t.join();
+}
+
+extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
+                             const float* seeds, const int16_t* half, const uint8_t* model,
+                             const float* init, const SpxFitOptions* opt, int32_t K, float* params,
+                             float* rms, int16_t* iters, uint8_t* status, int timeout_s) {
+    std::vector<std::vector<int32_t>> order(kBuckets);
+    for (int f = 0; f < K; ++f) {
+        const int hs = half[f];
+        const int md = model[f];
+        if (hs < 2 || hs > kMaxHalf || (md != SPX_MODEL_ERF_DISK6 && md != SPX_MODEL_ERF_ELLIPSE7)) {
+            for (int i = 0; i < 7; ++i) params[(int64_t)f * 7 + i] = NAN;
+            rms[f] = NAN;
+            iters[f] = 0;
+            status[f] = SPX_FIT_TOO_FEW;
+            continue;
+        }
+        order[bucket_of(hs)].push_back(f);
+    }
+    unsigned long long stats[4] = {0, 0, 0, 0};
+    for (int b = 0; b < kBuckets; ++b) {
+        int32_t count = (int32_t)order[b].size();
+        int32_t queue = 0;
+        if (count == 0) continue;
+        FitArgs a{img, H, W, pitch, seeds, half, model, init, *opt, order[b].data(), &count, &queue,
+                  stats, params, rms, iters, status};
+        switch (b) {
+            case 0: run_wave<4>(a, timeout_s); break;
+            case 1: run_wave<8>(a, timeout_s); break;
+            case 2: run_wave<16>(a, timeout_s); break;
+            default: run_wave<32>(a, timeout_s); break;
+        }
+    }
+    return 0;
+}
