@@ -1,0 +1,36 @@
+"""MI355X-native per-feature subpixel refinement for contact-hole images.
+
+Drop-in for the reference ``subpx`` centre path on ``backend="gpu"``: ``detect_centers``,
+``detect_centers_tiled``, ``connected_components_stats``, ``dedupe_centers``,
+``refine_centers_edge_erf_gpu`` and the batched erf-disk fits, all backed by hand-written
+gfx950 HIP kernels in libsubpx_hip.so (C-ABI: include/subpx_hip.h).
+"""
+
+from .types import CenterResult, ComponentStatsResult, ErfFitResult  # noqa: F401
+
+__version__ = "0.1.0"
+
+
+def __getattr__(name):
+    # torch-dependent modules load lazily so that the dataclasses import without a device.
+    if name in ("detect_centers", "dedupe_centers", "filter_centers"):
+        from . import centers
+
+        return getattr(centers, name)
+    if name in ("detect_centers_tiled", "detect_centers_tiled_global_otsu"):
+        from . import legacy
+
+        return getattr(legacy, name)
+    if name == "connected_components_stats":
+        from .components import connected_components_stats
+
+        return connected_components_stats
+    if name in ("fit_erf_disks", "fit_erf_disks_device"):
+        from . import fit
+
+        return getattr(fit, name)
+    if name in ("refine_centers_edge_erf_gpu",):
+        from . import edge_erf
+
+        return getattr(edge_erf, name)
+    raise AttributeError(name)

@@ -1,0 +1,301 @@
+"""Public centre detection API on the GPU.
+
+Keeps the signature and result of the reference ``detect_centers`` (subpx/centers.py:130-220)
+and its GPU path (subpx/_gpu/centers.py:1472-1862): threshold -> connected components ->
+area filter -> per-feature refinement, optionally over horizontal strips with a global
+Otsu threshold, an ownership band and a final de-duplication. Every stage runs on the GPU
+through libsubpx_hip.so; the host only filters component rows and merges strips.
+
+Refinement methods:
+  * "logquad": the reference CPU log-quadratic fit (subpx/_cpu/centers.py:67-107), the
+    parity target for this method;
+  * "none": the component's pixel centroid;
+  * "erf_disk" / "erf_ellipse": batched erf-edge disk / ellipse LM fits seeded at the
+    component centroid (no reference counterpart; see fit.py);
+  * "edge_erf": the reference's edge-erf refinement (subpx/_gpu/centers.py:782-929).
+"""
+
+from __future__ import annotations
+
+from typing import Sequence, Tuple, Union
+
+import numpy as np
+import torch
+
+from . import _lib
+from .backends import gpu_device, resolve_backend, stream_handle, to_device_u16
+from .components import otsu_threshold_device, threshold_components_device
+from .types import CenterResult
+
+_ERF_METHODS = {"erf_disk": _lib.MODEL_ERF_DISK6, "erf_ellipse": _lib.MODEL_ERF_ELLIPSE7}
+_SUPPORTED = {"logquad", "logquadratic", "none", "edge_erf", "edge_gradmoment_erf",
+              "edge_erf_centered"} | set(_ERF_METHODS)
+_VORONOI_METHODS = {"radial_symmetry", "isophote_curvature"}
+
+
+def _refine_group(img_t, labels, rows_np, cents_np, refine, *, pad, fit_half, fit_options):
+    """Refine the kept components of one image / strip. Returns (centers (K,2), ok (K,), extra)."""
+    dev = img_t.device
+    K = rows_np.shape[0]
+    H, W = img_t.shape
+    extra = {}
+    if K == 0:
+        return np.zeros((0, 2)), np.zeros(0, bool), extra
+    if refine in ("logquad", "logquadratic"):
+        rows_t = torch.from_numpy(np.ascontiguousarray(rows_np.astype(np.int32))).to(dev)
+        out = torch.empty((K, 2), dtype=torch.float64, device=dev)
+        ok = torch.empty((K,), dtype=torch.uint8, device=dev)
+        _lib.call("spx_refine_logquad", int(img_t.data_ptr()), H, W, img_t.stride(0),
+                  int(labels.data_ptr()), int(rows_t.data_ptr()), K, int(pad), int(out.data_ptr()),
+                  int(ok.data_ptr()), stream_handle())
+        torch.cuda.current_stream().synchronize()
+        return out.cpu().numpy(), ok.cpu().numpy().astype(bool), extra
+    if refine == "none":
+        return cents_np.copy(), np.isfinite(cents_np).all(axis=1), extra
+    if refine in _ERF_METHODS:
+        from .fit import fit_erf_disks_device
+
+        seeds = torch.from_numpy(cents_np.astype(np.float32)).to(dev)
+        if fit_half is None:
+            half_np = np.clip(np.ceil(0.5 * rows_np[:, 2:4].max(axis=1)).astype(np.int64) + 4, 7, 22)
+        else:
+            half_np = np.broadcast_to(np.asarray(fit_half, dtype=np.int64), (K,))
+        half_t = torch.from_numpy(half_np.astype(np.int16)).to(dev)
+        model_t = torch.full((K,), _ERF_METHODS[refine], dtype=torch.uint8, device=dev)
+        res = fit_erf_disks_device(img_t, seeds, half_t, model_t, options=fit_options)
+        torch.cuda.current_stream().synchronize()
+        params = res["params"].cpu().numpy()
+        status = res["status"].cpu().numpy()
+        extra = {"params": params, "status": status, "rms": res["rms"].cpu().numpy(),
+                 "iterations": res["iters"].cpu().numpy()}
+        ok = (status <= 1) & np.isfinite(params[:, :2]).all(axis=1)
+        return params[:, :2].astype(np.float64), ok, extra
+    if refine in ("edge_erf", "edge_gradmoment_erf", "edge_erf_centered"):
+        from .edge_erf import refine_edge_erf_device
+
+        stats6 = np.concatenate([rows_np[:, :4].astype(np.float32), cents_np.astype(np.float32)], axis=1)
+        c, ok = refine_edge_erf_device(img_t, torch.from_numpy(stats6).to(dev))
+        return c, ok, extra
+    raise ValueError(f"Unsupported refine method: {refine}")
+
+
+def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, refine, connectivity,
+                        pre_threshold, fit_half, fit_options):
+    """Segmentation -> CC -> area filter -> refinement on one device image (or strip view)."""
+    if pre_threshold is None:
+        if threshold != "otsu":
+            raise NotImplementedError(f"threshold={threshold!r}: only 'otsu' is implemented on the GPU")
+        thresh = otsu_threshold_device(img_t)
+    else:
+        thresh = int(np.floor(pre_threshold))
+    num, labels, stats, cents = threshold_components_device(img_t, thresh, invert=invert,
+                                                            connectivity=connectivity)
+    stats_np = stats.cpu().numpy()
+    cents_np = cents.cpu().numpy()
+    area = stats_np[:, 4]
+    keep = np.nonzero((area >= int(area_min)) & (area <= int(area_max)))[0]
+    keep = keep[keep >= 1]
+    rows = np.concatenate([stats_np[keep, :4], keep[:, None]], axis=1).astype(np.int64)
+    centers, ok, extra = _refine_group(img_t, labels, rows, cents_np[keep], refine, pad=pad,
+                                       fit_half=fit_half, fit_options=fit_options)
+    good = ok & np.isfinite(centers).all(axis=1)
+    info = {"threshold_value": thresh, "num_labels": num, "n_candidates": int(keep.size),
+            "n_refined": int(good.sum())}
+    if extra:
+        info["fit"] = {k: v[good] for k, v in extra.items()}
+        st = extra["status"]
+        info["fit_status_counts"] = {int(s): int((st == s).sum()) for s in np.unique(st)}
+    return centers[good], info
+
+
+def detect_centers(
+    image,
+    *,
+    backend: str = "auto",
+    threshold: str = "triangle",
+    invert: bool = False,
+    area_min: int = 1,
+    area_max: int = 50,
+    morph_open: int = 0,
+    morph_close: int = 0,
+    pad: int = 3,
+    refine: str = "logquad",
+    connectivity: int = 8,
+    gpu_batch: int = 4096,
+    device: int = 0,
+    use_float64: bool = True,
+    components_backend: str = "cpu",
+    small_feature_max: float = 12.0,
+    upsample_factor: int = 4,
+    tile_h: int | None = None,
+    overlap: int = 128,
+    threshold_mode: str = "auto",
+    otsu_downsample: int = 4,
+    thr_scale: float = 0.8,
+    dedupe_eps: float = 1.5,
+    background: str | None = None,
+    deblend_iterations: int = 30,
+    fit_half=None,
+    fit_max_iter: int = 30,
+    fit_step_tol: float = 1e-6,
+) -> CenterResult:
+    """Detect and refine centres (reference subpx/centers.py:130-220).
+
+    Same keywords and CenterResult as the reference; ``gpu_batch``, ``use_float64``,
+    ``components_backend``, ``small_feature_max`` and ``upsample_factor`` are accepted for
+    signature compatibility (the kernels need no batching and always refine in FP64 /
+    FP32-Jacobian + FP64 solve). ``fit_half``, ``fit_max_iter`` and ``fit_step_tol``
+    configure the erf fits.
+    """
+    b = resolve_backend(backend)
+    if refine in _VORONOI_METHODS:
+        raise NotImplementedError(f"refine='{refine}' (Voronoi methods) is not implemented")
+    if background not in (None, "model"):
+        raise ValueError("background must be None or 'model'")
+    if background == "model":
+        raise NotImplementedError("background='model' (deblending) is not implemented")
+    if morph_open or morph_close:
+        raise NotImplementedError("morphological open/close is not implemented")
+    if refine == "auto" or refine not in _SUPPORTED:
+        raise ValueError(f"Unsupported refine method: {refine}")
+    if isinstance(image, torch.Tensor):
+        shape = tuple(image.shape)
+    else:
+        shape = np.shape(image)
+    if len(shape) != 2:
+        raise ValueError("Expected 2D grayscale image")
+    fit_options = _lib.fit_options(max_iter=fit_max_iter, step_tol=fit_step_tol)
+    common = dict(threshold=threshold, invert=invert, area_min=area_min, area_max=area_max, pad=pad,
+                  refine=refine, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options)
+    base_meta = {
+        "invert": bool(invert), "area_min": int(area_min), "area_max": int(area_max),
+        "morph_open": int(morph_open), "morph_close": int(morph_close), "pad": int(pad),
+        "connectivity": int(connectivity), "gpu_batch": int(gpu_batch), "device": int(device),
+        "use_float64": bool(use_float64), "components_backend": str(components_backend),
+        "small_feature_max": float(small_feature_max),
+    }
+    with gpu_device(device) as dev:
+        img_t = to_device_u16(image, dev)
+        H, W = img_t.shape
+        if tile_h is None:
+            centers, info = _detect_single_tile(img_t, pre_threshold=None, **common)
+            meta = dict(base_meta)
+            meta.update(info)
+            return CenterResult(centers_xy=centers, method=f"threshold={threshold}, refine={refine}",
+                                backend=b, meta=meta)
+
+        tile_h = int(tile_h)
+        if overlap >= tile_h:
+            raise ValueError("overlap must be < tile_h")
+        pre_threshold = None
+        if threshold_mode in ("global", "auto"):
+            ds = max(1, int(otsu_downsample))
+            pre_threshold = float(otsu_threshold_device(img_t, ds)) * float(thr_scale)
+        step = tile_h - int(overlap)
+        all_centers = []
+        n_tiles = 0
+        y0 = 0
+        while y0 < H:
+            y1 = min(H, y0 + tile_h)
+            tile = img_t[y0:y1]
+            tile_thr = None if threshold_mode == "per_tile" else pre_threshold
+            centers, _ = _detect_single_tile(tile, pre_threshold=tile_thr, **common)
+            n_tiles += 1
+            if centers.shape[0] > 0:
+                centers = centers.copy()
+                centers[:, 1] += y0
+                half = int(overlap) // 2
+                keep_lo = y0 if y0 == 0 else (y0 + half)
+                keep_hi = y1 if y1 == H else (y1 - half)
+                m = (centers[:, 1] >= keep_lo) & (centers[:, 1] < keep_hi)
+                if np.any(m):
+                    all_centers.append(centers[m])
+            if y1 == H:
+                break
+            y0 += step
+    if all_centers:
+        centers = dedupe_centers(np.vstack(all_centers), eps=float(dedupe_eps))
+    else:
+        centers = np.zeros((0, 2), dtype=np.float64)
+    return CenterResult(
+        centers_xy=centers,
+        method=f"tiled_gpu(refine={refine})",
+        backend=b,
+        meta={
+            "tile_h": int(tile_h), "overlap": int(overlap), "threshold_mode": str(threshold_mode),
+            "otsu_downsample": int(otsu_downsample), "thr_scale": float(thr_scale),
+            "dedupe_eps": float(dedupe_eps), "refine": str(refine), "invert": bool(invert),
+            "area_min": int(area_min), "area_max": int(area_max), "device": int(device),
+            "small_feature_max": float(small_feature_max), "upsample_factor": int(upsample_factor),
+            "n_tiles": n_tiles, "threshold_value": pre_threshold,
+        },
+    )
+
+
+def filter_centers(centers_xy, image_shape_hw: Union[np.ndarray, Tuple[int, int], Sequence[int]], *,
+                   margin: float = 5.0) -> np.ndarray:
+    """Keep centres at least ``margin`` px inside the image (reference subpx/centers.py:317-332)."""
+    centers = np.asarray(centers_xy, dtype=np.float64)
+    if centers.size == 0:
+        return centers.reshape(0, 2)
+    H, W = int(image_shape_hw[0]), int(image_shape_hw[1])
+    m = float(margin)
+    x, y = centers[:, 0], centers[:, 1]
+    keep = (x >= m) & (x < (W - m)) & (y >= m) & (y < (H - m))
+    return centers[keep]
+
+
+def dedupe_centers(centers_xy, *, eps: float = 1.5) -> np.ndarray:
+    """Merge detections closer than ``eps`` (reference subpx/centers.py:335-378).
+
+    Same clusters, same cluster order and same means as the reference's O(N^2) loop: the
+    pairs within eps are found with a k-d tree (candidate radius padded, then the
+    reference's squared-distance test), and the reference's union-by-rank / path-halving
+    union-find is replayed over them in the reference's (i, j) order.
+    """
+    pts = np.asarray(centers_xy, dtype=np.float64)
+    n = pts.shape[0]
+    if n == 0:
+        return pts.reshape(0, 2)
+    from scipy.spatial import cKDTree
+
+    eps2 = float(eps) ** 2
+    pairs = cKDTree(pts).query_pairs(r=float(eps) * (1.0 + 1e-9) + 1e-12, output_type="ndarray")
+    parent = np.arange(n, dtype=np.int64)
+    if pairs.size:
+        pairs = np.sort(pairs, axis=1)
+        d2 = np.sum((pts[pairs[:, 1]] - pts[pairs[:, 0]]) ** 2, axis=1)
+        pairs = pairs[d2 <= eps2]
+        order = np.lexsort((pairs[:, 1], pairs[:, 0]))
+        pairs = pairs[order]
+        rank = np.zeros(n, dtype=np.int8)
+        par = parent.tolist()
+        rk = rank.tolist()
+
+        def find(a):
+            while par[a] != a:
+                par[a] = par[par[a]]
+                a = par[a]
+            return a
+
+        for i, j in pairs.tolist():
+            ra, rb = find(i), find(j)
+            if ra == rb:
+                continue
+            if rk[ra] < rk[rb]:
+                par[ra] = rb
+            elif rk[ra] > rk[rb]:
+                par[rb] = ra
+            else:
+                par[rb] = ra
+                rk[ra] += 1
+        roots = np.array([find(i) for i in range(n)], dtype=np.int64)
+    else:
+        roots = parent
+    unique_roots, inv = np.unique(roots, return_inverse=True)
+    K = unique_roots.size
+    sums = np.zeros((K, 2), dtype=np.float64)
+    counts = np.zeros((K,), dtype=np.int64)
+    np.add.at(sums, inv, pts)
+    np.add.at(counts, inv, 1)
+    return sums / np.maximum(counts[:, None], 1)

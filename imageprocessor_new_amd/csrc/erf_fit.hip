@@ -11,8 +11,17 @@ namespace fitdev {
 
 SPX_DEV_INLINE int spx_lane() { return threadIdx.x & 63; }
 SPX_DEV_INLINE float spx_shfl_xor(float v, int off) { return __shfl_xor(v, off, 64); }
-SPX_DEV_INLINE int spx_shfl(int v, int src) { return __shfl(v, src, 64); }
-SPX_DEV_INLINE int spx_atomic_add(int32_t* p, int v) { return atomicAdd(p, v); }
+// readfirstlane: the compiler then knows the value (and every branch on it) is uniform.
+SPX_DEV_INLINE float spx_uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+SPX_DEV_INLINE int spx_uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// Every lane takes part in the atomic (lane 0 adds 1, the others 0; the atomic optimizer
+// merges them into one), and the first lane's return is the wave's slot.
+SPX_DEV_INLINE int spx_queue_grab(int32_t* q) {
+    const int r = atomicAdd(q, spx_lane() == 0 ? 1 : 0);
+    return __builtin_amdgcn_readfirstlane(r);
+}
 SPX_DEV_INLINE unsigned long long spx_atomic_add(unsigned long long* p, unsigned long long v) {
     return atomicAdd(p, v);
 }

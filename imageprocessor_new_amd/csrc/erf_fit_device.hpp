@@ -1,9 +1,15 @@
 // Device-side logic of the batched erf-disk / erf-ellipse LM fit (one wave64 per feature).
 //
 // Included by erf_fit.hip (HIP build) after it defines the wave primitives:
-//   SPX_DEV, SPX_DEV_INLINE, spx_lane(), spx_shfl_xor(float, int), spx_shfl(int, int),
-//   spx_atomic_add(int32_t*, int), spx_atomic_add(unsigned long long*, unsigned long long),
+//   SPX_DEV, SPX_DEV_INLINE, spx_lane(), spx_shfl_xor(float, int),
+//   spx_uniform(float) / spx_uniform_i(int) (the first lane's value, made wave-uniform),
+//   spx_queue_grab(int32_t*) (one atomic increment per wave, result uniform),
+//   spx_atomic_add(unsigned long long*, unsigned long long),
 //   hw_rcp / hw_rsq / hw_exp2 (float) and hw_rcp64 (double).
+// Every value that steers control flow is wave-uniform by construction (reductions and
+// the queue head pass through spx_uniform / readfirstlane), so the compiler sees uniform
+// branches only: a lane-dependent branch around a shuffle inside the persistent loop
+// lets it give lanes different loop headers, and the wave then never reconverges.
 // The same text compiles against a host emulation of a wave (64 threads exchanging through
 // barriers) in the tests, which checks on the CPU that every lane takes the same path.
 //
@@ -55,13 +61,13 @@ constexpr float kAcceptRtol = 1.0e-6f;
 SPX_DEV_INLINE float wave_sum(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += spx_shfl_xor(v, off);
-    return v;
+    return spx_uniform(v);
 }
 
 SPX_DEV_INLINE float wave_max(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, spx_shfl_xor(v, off));
-    return v;
+    return spx_uniform(v);
 }
 
 // Sum layout for P parameters with the background B last (column of ones):
@@ -257,13 +263,13 @@ SPX_DEV void fit_feature(const FitArgs& a, int f, int hs, const float (&v)[NPL],
         bool start_ok = true;
         if (a.init != nullptr) {
             const float* in = a.init + (int64_t)f * 7;
-            cur.p[0] = (double)in[0] - cx;
-            cur.p[1] = (double)in[1] - cy;
-            cur.p[2] = in[2];
-            cur.p[3] = in[3];
-            cur.p[4] = in[4];
-            cur.p[P - 1] = in[5];
-            if constexpr (P == 7) cur.p[5] = in[6];
+            cur.p[0] = (double)spx_uniform(in[0]) - cx;
+            cur.p[1] = (double)spx_uniform(in[1]) - cy;
+            cur.p[2] = spx_uniform(in[2]);
+            cur.p[3] = spx_uniform(in[3]);
+            cur.p[4] = spx_uniform(in[4]);
+            cur.p[P - 1] = spx_uniform(in[5]);
+            if constexpr (P == 7) cur.p[5] = spx_uniform(in[6]);
         } else {
             // Moment start: background = mean of the patch's outer ring; in-set = pixels
             // above half the peak; centre = weighted centroid of the in-set; radius from
@@ -404,16 +410,15 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
     const int lane = spx_lane();
     const int count = *a.count;
     for (;;) {
-        int slot = 0;
-        if (lane == 0) slot = spx_atomic_add(a.queue, 1);
-        slot = spx_shfl(slot, 0);
+        const int slot = spx_queue_grab(a.queue);
         if (slot >= count) break;
-        const int f = a.order[slot];
-        const int hs = a.half[f];
+        const int f = spx_uniform_i(a.order[slot]);
+        const int hs = spx_uniform_i(a.half[f]);
+        const int model = spx_uniform_i(a.model[f]);
         const int side = 2 * hs + 1;
         const int npx = side * side;
-        const float cx = rintf(a.seeds[2 * (int64_t)f]);
-        const float cy = rintf(a.seeds[2 * (int64_t)f + 1]);
+        const float cx = spx_uniform(rintf(a.seeds[2 * (int64_t)f]));
+        const float cy = spx_uniform(rintf(a.seeds[2 * (int64_t)f + 1]));
         const int64_t ox = (int64_t)cx - hs;
         const int64_t oy = (int64_t)cy - hs;
 
@@ -436,7 +441,7 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
             cnt += m[n];
         }
         const float npix = wave_sum(cnt);
-        if (a.model[f] == SPX_MODEL_ERF_ELLIPSE7)
+        if (model == SPX_MODEL_ERF_ELLIPSE7)
             fit_feature<NPL, 7>(a, f, hs, v, fx, fy, m, npix, cx, cy);
         else
             fit_feature<NPL, 6>(a, f, hs, v, fx, fy, m, npix, cx, cy);

@@ -18,20 +18,28 @@
 namespace spx {
 namespace {
 
+// Wave reductions return the first lane's total through readfirstlane, so every branch
+// taken on them is a uniform (scalar) branch.
+__device__ __forceinline__ double uniform_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    return uniform_d(v);
 }
 __device__ __forceinline__ int wave_sum_i(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    return __builtin_amdgcn_readfirstlane(v);
 }
 __device__ __forceinline__ double wave_max_d(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
+    return uniform_d(v);
 }
 
 struct Roi {
@@ -94,8 +102,11 @@ __global__ __launch_bounds__(256) void logquad_kernel(const uint16_t* img, int64
     const int64_t f = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (f >= K) return;
     const int32_t* row = rows + 5 * f;
-    const int64_t x = row[0], y = row[1], w = row[2], h = row[3];
-    const int32_t lab = row[4];
+    const int64_t x = __builtin_amdgcn_readfirstlane(row[0]);
+    const int64_t y = __builtin_amdgcn_readfirstlane(row[1]);
+    const int64_t w = __builtin_amdgcn_readfirstlane(row[2]);
+    const int64_t h = __builtin_amdgcn_readfirstlane(row[3]);
+    const int32_t lab = __builtin_amdgcn_readfirstlane(row[4]);
     Roi r;
     r.x0 = x - pad > 0 ? x - pad : 0;
     r.y0 = y - pad > 0 ? y - pad : 0;

@@ -1,0 +1,68 @@
+"""The fit kernel's device logic (erf_fit_device.hpp) run on the CPU by a wave emulator.
+
+Each of the 64 lanes is a thread and every shuffle is a barrier exchange, so lanes that
+take different paths through a shuffle deadlock (the emulator then exits with code 3).
+Checks uniform control flow and agreement with the FP64 oracle without a GPU.
+"""
+
+import ctypes
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+SRC = ROOT / "tests" / "wave_emu" / "fit_emu.cpp"
+OUT = ROOT / "tests" / "wave_emu" / "_build" / "fit_emu.so"
+
+
+def _build():
+    OUT.parent.mkdir(exist_ok=True)
+    if OUT.exists() and OUT.stat().st_mtime > max(SRC.stat().st_mtime,
+            (ROOT / "imageprocessor_new_amd/csrc/erf_fit_device.hpp").stat().st_mtime):
+        return
+    cmd = ["g++", "-O2", "-std=c++20", "-pthread", "-shared", "-fPIC", "-Wno-unknown-pragmas",
+           f"-I{ROOT / 'include'}", f"-I{ROOT / 'imageprocessor_new_amd/csrc'}", str(SRC), "-o", str(OUT)]
+    subprocess.run(cmd, check=True)
+
+
+RUNNER = r"""
+import ctypes, sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+from imageprocessor_new_amd import synthetic, _lib
+from oracle import erfdisk
+lib = ctypes.CDLL(sys.argv[2])
+arr = synthetic.make_config("c2", seed=21, H=160, W=160)
+sel = np.arange(min(6, arr.centers_xy.shape[0]))
+seeds = np.rint(arr.centers_xy[sel]).astype(np.float32)
+seeds = np.concatenate([seeds, [[1.0, 80.0]]]).astype(np.float32)  # clipped border patch
+K = seeds.shape[0]
+half = np.clip(np.ceil(np.r_[arr.radius[sel], 6.0] + 3 * np.r_[arr.sigma[sel], 1.0]), 7, 22).astype(np.int16)
+model = np.r_[arr.ellipse[sel].astype(np.uint8), 0].astype(np.uint8)
+model[1] = 1
+params = np.zeros((K, 7), np.float32); rms = np.zeros(K, np.float32)
+iters = np.zeros(K, np.int16); st = np.zeros(K, np.uint8)
+opts = _lib.fit_options()
+img = np.ascontiguousarray(arr.image)
+P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+rc = lib.emu_fit_batch(P(img), ctypes.c_int64(160), ctypes.c_int64(160), ctypes.c_int64(160), P(seeds),
+                       P(half), P(model), None, ctypes.byref(opts), ctypes.c_int32(K), P(params), P(rms),
+                       P(iters), P(st), ctypes.c_int(240))
+ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, seeds, half, model)
+assert (st == ref_st).all(), (st, ref_st)
+ok = st <= 1
+err = np.abs(params[ok].astype(np.float64) - ref_p[ok]).max(axis=0)
+assert err[0] < 1e-3 and err[1] < 1e-3 and err[2] < 1e-3 and err[3] < 1e-3, err
+print("emulated", K, "fits; max |emu - oracle|:", err)
+"""
+
+
+def test_fit_device_logic_under_wave_emulation():
+    _build()
+    proc = subprocess.run([sys.executable, "-c", RUNNER, str(ROOT), str(OUT)], capture_output=True,
+                          text=True, timeout=600)
+    assert proc.returncode != 3, "lanes diverged in the wave emulation:\n" + proc.stderr
+    assert proc.returncode == 0, proc.stdout + proc.stderr

@@ -1,0 +1,138 @@
+"""GPU parity of Otsu, threshold + connected components, log-quadratic refinement and the
+detect_centers pipeline against the CPU oracle (bit-exact for integer outputs; 1e-8 px for
+the FP64 log-quadratic centres)."""
+
+import numpy as np
+import pytest
+import torch
+
+from imageprocessor_new_amd import synthetic
+from oracle import segment, tiling
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    from imageprocessor_new_amd.backends import gpu_device
+
+    return gpu_device(0)
+
+
+@pytest.mark.parametrize("seed,size,step", [(0, 512, 1), (1, 640, 1), (2, 700, 4), (3, 333, 3)])
+def test_otsu_matches_oracle(seed, size, step):
+    from imageprocessor_new_amd.components import otsu_threshold_device
+
+    arr = synthetic.make_config("c0", seed=seed, H=size, W=size + 17)
+    with _dev() as dev:
+        t = otsu_threshold_device(torch.from_numpy(arr.image).to(dev), step)
+    assert t == segment.otsu_u16(arr.image, step)
+
+
+def test_otsu_small_and_constant():
+    from imageprocessor_new_amd.components import otsu_threshold_device
+
+    rng = np.random.default_rng(5)
+    for shape in [(1, 1), (3, 7), (31, 2)]:
+        img = rng.integers(0, 65535, size=shape).astype(np.uint16)
+        with _dev() as dev:
+            assert otsu_threshold_device(torch.from_numpy(img).to(dev)) == segment.otsu_u16(img)
+    with _dev() as dev:
+        assert otsu_threshold_device(torch.full((9, 9), 77, dtype=torch.uint16, device=dev)) == 0
+
+
+@pytest.mark.parametrize("conn", [8, 4])
+@pytest.mark.parametrize("seed,shape,frac", [(0, (57, 93), 0.45), (1, (64, 64), 0.3), (2, (301, 257), 0.55),
+                                             (3, (1, 40), 0.5), (4, (40, 1), 0.5)])
+def test_components_match_oracle(conn, seed, shape, frac):
+    from imageprocessor_new_amd.components import threshold_components_device
+
+    rng = np.random.default_rng(seed)
+    img = (rng.random(shape) < frac).astype(np.uint16) * 1000
+    with _dev() as dev:
+        num, labels, stats, cents = threshold_components_device(torch.from_numpy(img).to(dev), 500,
+                                                                connectivity=conn)
+        labels, stats, cents = labels.cpu().numpy(), stats.cpu().numpy(), cents.cpu().numpy()
+    rn, rl, rs, rc = segment.connected_components_fast(img > 500, conn)
+    assert num == rn
+    assert np.array_equal(labels, rl)
+    assert np.array_equal(stats, rs)
+    np.testing.assert_array_equal(cents, rc)
+
+
+def test_components_invert_and_capacity_retry():
+    from imageprocessor_new_amd.components import threshold_components_device
+
+    rng = np.random.default_rng(9)
+    img = rng.integers(0, 1000, size=(120, 130)).astype(np.uint16)
+    with _dev() as dev:
+        num, labels, stats, cents = threshold_components_device(torch.from_numpy(img).to(dev), 300,
+                                                                invert=True, capacity=4)
+        labels = labels.cpu().numpy()
+    rn, rl, _, _ = segment.connected_components_fast(img <= 300, 8)
+    assert num == rn and np.array_equal(labels, rl)
+
+
+def test_logquad_and_pipeline_match_oracle_c0():
+    from imageprocessor_new_amd.centers import detect_centers
+
+    arr = synthetic.make_config("c0", seed=4, H=768, W=640)
+    res = detect_centers(arr.image, backend="gpu", threshold="otsu", refine="logquad", area_min=20,
+                         area_max=2000, pad=3)
+    ref, rows, t = segment.detect_centers_logquad(arr.image, area_min=20, area_max=2000, pad=3)
+    assert res.meta["threshold_value"] == t
+    assert res.centers_xy.shape == ref.shape
+    np.testing.assert_allclose(res.centers_xy, ref, rtol=0, atol=1e-8)
+    # statistical accuracy against the truth (nearest true centre)
+    d = np.hypot(*(res.centers_xy[:, None, :] - arr.centers_xy[None, :, :]).transpose(2, 0, 1)).min(axis=1)
+    assert np.median(d) < 0.2
+
+
+def test_logquad_border_and_small_rois():
+    from imageprocessor_new_amd.centers import detect_centers
+
+    arr = synthetic.make_config("c0", seed=6, H=200, W=200)
+    # holes cut by the border and a tiny speck; area filters off
+    img = arr.image.copy()
+    img[5:7, 100:102] = 4000
+    res = detect_centers(img, backend="gpu", threshold="otsu", refine="logquad", area_min=1,
+                         area_max=5000, pad=3)
+    ref, _, _ = segment.detect_centers_logquad(img, area_min=1, area_max=5000, pad=3)
+    assert res.centers_xy.shape == ref.shape
+    np.testing.assert_allclose(res.centers_xy, ref, rtol=0, atol=1e-8)
+
+
+def test_empty_image_gives_no_centers():
+    from imageprocessor_new_amd.centers import detect_centers
+
+    res = detect_centers(np.full((64, 64), 500, np.uint16), backend="gpu", threshold="otsu")
+    assert res.centers_xy.shape == (0, 2)
+
+
+def test_tiled_matches_oracle():
+    from imageprocessor_new_amd.legacy import detect_centers_tiled
+
+    arr = synthetic.make_config("c0", seed=8, H=1024, W=384)
+    with pytest.warns(DeprecationWarning):
+        res = detect_centers_tiled(arr.image, tile_h=300, overlap=64, backend="gpu", threshold="otsu",
+                                   refine="logquad", area_min=20, area_max=2000)
+    ref = tiling.detect_centers_tiled_logquad(arr.image, tile_h=300, overlap=64, area_min=20, area_max=2000)
+    assert res.centers_xy.shape == ref.shape
+    np.testing.assert_allclose(res.centers_xy, ref, rtol=0, atol=1e-8)
+
+
+def test_tiled_overlap_error():
+    from imageprocessor_new_amd.centers import detect_centers
+
+    with pytest.raises(ValueError, match="overlap must be < tile_h"):
+        detect_centers(np.zeros((64, 64), np.uint16), backend="gpu", tile_h=64, overlap=64)
+
+
+def test_erf_disk_refine_through_detect_centers():
+    from imageprocessor_new_amd.centers import detect_centers
+
+    arr = synthetic.make_config("c1", seed=12, H=384, W=384)
+    res = detect_centers(arr.image, backend="gpu", threshold="otsu", refine="erf_disk", area_min=20,
+                         area_max=2000, fit_half=15)
+    d = np.hypot(*(res.centers_xy[:, None, :] - arr.centers_xy[None, :, :]).transpose(2, 0, 1)).min(axis=1)
+    assert res.centers_xy.shape[0] >= 0.9 * arr.centers_xy.shape[0]
+    assert np.median(d) < 0.02

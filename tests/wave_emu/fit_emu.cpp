@@ -52,7 +52,20 @@ inline int spx_shfl(int v, int src) {
     w.bar.arrive_and_wait();
     return r;
 }
-inline int spx_atomic_add(int32_t* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline float spx_uniform(float v) {
+    WaveState& w = *t_wave;
+    w.shuffles[t_lane]++;
+    w.fv[t_lane] = v;
+    w.bar.arrive_and_wait();
+    float r = w.fv[0];
+    w.bar.arrive_and_wait();
+    return r;
+}
+inline int spx_uniform_i(int v) { return spx_shfl(v, 0); }
+inline int spx_queue_grab(int32_t* q) {
+    const int mine = t_lane == 0 ? __atomic_fetch_add(q, 1, __ATOMIC_SEQ_CST) : 0;
+    return spx_shfl(mine, 0);
+}
 inline unsigned long long spx_atomic_add(unsigned long long* p, unsigned long long v) {
     return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
 }
