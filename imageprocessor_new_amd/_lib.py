@@ -67,6 +67,14 @@ SIGNATURES = {
     "spx_threshold_cc": (_I32, [_P, _I64, _I64, _I64, _I32, _I32, _I32, _P, _P, _P, _I32, _P, _P]),
     "spx_refine_logquad": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _P, _P, _P]),
     "spx_fit_stats": (_I32, [_P, _P]),
+    "spx_cnn_param_count": (_I64, []),
+    "spx_gemm_bf16": (_I32, [_P, _I64, _I64, _P, _I64, _I64, _P, _I32, _I32, _I32, _P]),
+    "spx_cnn_prepare": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _P, _P, _P]),
+    "spx_cnn_encode": (_I32, [_P, _P, _I32, _P, _P]),
+    "spx_cnn_forward": (_I32, [_P, _I32, _P, _P, _P, _P, _P]),
+    "spx_cnn_train_step": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _I32, ctypes.c_float, _P, _P]),
+    "spx_cnn_to_bf16": (_I32, [_P, _I64, _P, _P]),
+    "spx_cnn_grad": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _P]),
     "spx_fit_batch": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions),
                              _I32, _P, _P, _P, _P, _P]),
 }

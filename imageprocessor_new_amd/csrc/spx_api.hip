@@ -64,9 +64,84 @@ int logquad_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, con
                    const int32_t* rows, int32_t K, int32_t pad, double* centers, uint8_t* ok,
                    hipStream_t stream);
 
+int64_t cnn_param_count();
+int gemm_bf16_launch(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn,
+                     float* C, int M, int N, int K, hipStream_t stream);
+int cnn_prepare_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                       const int16_t* half, int K, uint16_t* x_out, float* norm, hipStream_t stream);
+int cnn_encode_launch(const float* params, const float* norm, int K, float* t, hipStream_t stream);
+int cnn_forward_launch(const uint16_t* x, int K, const float* theta, const uint16_t* theta_bf, const float* norm,
+                       float* out, hipStream_t stream);
+int cnn_train_step_launch(const uint16_t* x, const float* t, const float* w, int Bn, float* theta,
+                          uint16_t* theta_bf, float* adam_m, float* adam_v, int step, float lr, float* loss_out,
+                          float* grad_out, hipStream_t stream);
+int cnn_to_bf16_launch(const float* theta, int64_t n, uint16_t* theta_bf, hipStream_t stream);
+
 }  // namespace spx
 
 extern "C" {
+
+int64_t spx_cnn_param_count(void) { return spx::cnn_param_count(); }
+
+int spx_gemm_bf16(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn,
+                  float* C, int32_t M, int32_t N, int32_t K, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(A != nullptr && B != nullptr && C != nullptr, "spx_gemm_bf16: null pointer");
+    SPX_REQUIRE(M > 0 && N > 0 && K > 0, "spx_gemm_bf16: empty shape");
+    return spx::gemm_bf16_launch(A, sam, sak, B, sbk, sbn, C, M, N, K, spx::as_stream(stream));
+}
+
+int spx_cnn_prepare(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                    const int16_t* half, int32_t K, uint16_t* x_out, float* norm_out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(K >= 0, "spx_cnn_prepare: K < 0");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(img != nullptr && seeds != nullptr && half != nullptr && x_out != nullptr && norm_out != nullptr,
+                "spx_cnn_prepare: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_cnn_prepare: bad image shape or pitch");
+    return spx::cnn_prepare_launch(img, H, W, pitch, seeds, half, K, x_out, norm_out, spx::as_stream(stream));
+}
+
+int spx_cnn_encode(const float* params, const float* norm, int32_t K, float* t_out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(K >= 0 && (K == 0 || (params && norm && t_out)), "spx_cnn_encode: bad arguments");
+    return spx::cnn_encode_launch(params, norm, K, t_out, spx::as_stream(stream));
+}
+
+int spx_cnn_forward(const uint16_t* x, int32_t K, const float* theta, const uint16_t* theta_bf,
+                    const float* norm, float* out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(K >= 0, "spx_cnn_forward: K < 0");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(x != nullptr && theta != nullptr && theta_bf != nullptr && out != nullptr,
+                "spx_cnn_forward: null pointer");
+    return spx::cnn_forward_launch(x, K, theta, theta_bf, norm, out, spx::as_stream(stream));
+}
+
+int spx_cnn_train_step(const uint16_t* x, const float* target, const float* weight, int32_t B, float* theta,
+                       uint16_t* theta_bf, float* adam_m, float* adam_v, int32_t step, float lr, float* loss_out,
+                       spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(B > 0 && step >= 1 && lr > 0.0f, "spx_cnn_train_step: bad batch, step or lr");
+    SPX_REQUIRE(x && target && theta && theta_bf && adam_m && adam_v, "spx_cnn_train_step: null pointer");
+    return spx::cnn_train_step_launch(x, target, weight, B, theta, theta_bf, adam_m, adam_v, step, lr, loss_out,
+                                      nullptr, spx::as_stream(stream));
+}
+
+int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, int32_t B, float* theta,
+                 uint16_t* theta_bf, float* grad_out, float* loss_out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(B > 0, "spx_cnn_grad: B <= 0");
+    SPX_REQUIRE(x && target && theta && theta_bf && grad_out, "spx_cnn_grad: null pointer");
+    return spx::cnn_train_step_launch(x, target, weight, B, theta, theta_bf, nullptr, nullptr, 1, 1.0f, loss_out,
+                                      grad_out, spx::as_stream(stream));
+}
+
+int spx_cnn_to_bf16(const float* theta, int64_t n, uint16_t* theta_bf, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(theta && theta_bf && n >= 0, "spx_cnn_to_bf16: bad arguments");
+    return spx::cnn_to_bf16_launch(theta, n, theta_bf, spx::as_stream(stream));
+}
 
 int spx_fit_stats(uint64_t* out, spx_stream_t stream) {
     spx::clear_error();

@@ -118,6 +118,48 @@ int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, cons
  * the stream. */
 int spx_fit_stats(uint64_t* out, spx_stream_t stream);
 
+/* ---- learned initial guess (no reference counterpart; task-defined CNN) ------------------ */
+/* Parameter count of the CNN: conv3x3 1->16->32->64->64 (stride 2) + fc 256->128->7, each
+ * layer a (9 Cin or fan-in) x Cout weight matrix followed by Cout biases, FP32. */
+int64_t spx_cnn_param_count(void);
+
+/* C (M x N, row-major FP32) = A B with A(m, k) = A[m sam + k sak] and B(k, n) = B[k sbk +
+ * n sbn] BF16, FP32 accumulation on the matrix cores (the GEMM every CNN layer uses).
+ * A, B, C: device. */
+int spx_gemm_bf16(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn,
+                  float* C, int32_t M, int32_t N, int32_t K, spx_stream_t stream);
+
+/* Network input of each feature: its (2 half + 1)^2 fit patch resampled to 32 x 32 and
+ * normalised by the ring mean B0 and A0 = max - B0. x_out (K, 32, 32) BF16, norm_out (K, 5)
+ * f32 {cx, cy, side, B0, A0}: device. */
+int spx_cnn_prepare(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                    const int16_t* half, int32_t K, uint16_t* x_out, float* norm_out, spx_stream_t stream);
+
+/* Fit parameters (K, 7) -> normalised network targets (K, 7) for the given norms. Device. */
+int spx_cnn_encode(const float* params, const float* norm, int32_t K, float* t_out, spx_stream_t stream);
+
+/* Forward pass. theta (FP32, biases read from it) and theta_bf (BF16 copy) hold the
+ * parameters; out (K, 7): decoded fit parameters [x0, y0, r, s, A, B, q] when norm is given,
+ * raw outputs when norm is NULL. Device. */
+int spx_cnn_forward(const uint16_t* x, int32_t K, const float* theta, const uint16_t* theta_bf,
+                    const float* norm, float* out, spx_stream_t stream);
+
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8; step counts from 1) on a batch of B inputs
+ * against targets (B, 7) with optional per-output weights (B, 7): mean squared error,
+ * backward through every layer, update of theta, theta_bf, adam_m, adam_v (all device).
+ * loss_out: host float or NULL (NULL keeps the call asynchronous). */
+int spx_cnn_train_step(const uint16_t* x, const float* target, const float* weight, int32_t B, float* theta,
+                       uint16_t* theta_bf, float* adam_m, float* adam_v, int32_t step, float lr, float* loss_out,
+                       spx_stream_t stream);
+
+/* Loss gradient of a batch without updating (the gradient spx_cnn_train_step applies):
+ * grad_out (spx_cnn_param_count() floats, device), loss_out host float or NULL. */
+int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, int32_t B, float* theta,
+                 uint16_t* theta_bf, float* grad_out, float* loss_out, spx_stream_t stream);
+
+/* theta_bf = BF16(theta), n values, device. */
+int spx_cnn_to_bf16(const float* theta, int64_t n, uint16_t* theta_bf, spx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
