@@ -69,6 +69,9 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=8192, help="image side (8192 = full workload)")
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-start leg")
+    ap.add_argument("--train-tiles", type=int, default=1 << 18)
+    ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -146,6 +149,41 @@ def main() -> int:
         status = out["status"].cpu().numpy()
         iters = out["iters"].cpu().numpy()
 
+        learned_leg = None
+        if not args.no_learned:
+            from imageprocessor_new_amd import learned
+
+            log("learned start: rendering, labelling and training the CNN on the GPU")
+            net, tinfo = learned.train_learned_start(
+                n_tiles=args.train_tiles, epochs=args.train_epochs, batch=4096, lr=1e-3, seed=10_000, device=dev,
+                mix=[(learned.SPEC_SQUARE, 0.5), (learned.SPEC_STAGGERED, 0.5)])
+            log(f"trained on {tinfo['samples']} tiles: data {tinfo['data_seconds']:.1f}s, "
+                f"train {tinfo['train_seconds']:.1f}s, loss {tinfo['losses'][0]:.4f} -> {tinfo['losses'][-1]:.4f}")
+            for _ in range(max(1, args.warmup)):
+                learned.fit_learned(net, img_t, seeds_t, half_t, model_t, lm_iters=4)
+            times, parts = [], []
+            for _ in range(args.steps):
+                res_l, tm = learned.fit_learned(net, img_t, seeds_t, half_t, model_t, lm_iters=4)
+                times.append(tm.seconds)
+                parts.append((tm.propose_s, tm.refine_s, tm.fallback_s, tm.n_fallback))
+            lp = res_l["params"].cpu().numpy()
+            ls = res_l["status"].cpu().numpy()
+            li = res_l["iters"].cpu().numpy()
+            both = (ls <= 1) & (status <= 1)
+            agree = float(np.abs(lp[both, :2].astype(np.float64) - params[both, :2]).max()) if both.any() else None
+            learned_leg = {
+                "ms_per_step": 1e3 * float(np.mean(times)),
+                "propose_ms": 1e3 * float(np.mean([p[0] for p in parts])),
+                "refine_ms": 1e3 * float(np.mean([p[1] for p in parts])),
+                "fallback_ms": 1e3 * float(np.mean([p[2] for p in parts])),
+                "fallback_features": int(parts[-1][3]),
+                "mean_iterations": float(li.mean()),
+                "max_centre_diff_vs_plain_px": agree,
+                "train": {k: tinfo[k] for k in ("tiles", "samples", "data_seconds", "train_seconds", "epochs")},
+                "status_ok": float((ls <= 1).mean()),
+            }
+            log(f"learned leg: {learned_leg}")
+
     ms_per_step = 1e3 * wall / args.steps
     flops = float(stats[0]) * FLOP_PER_PIXEL_PASS[6] + float(stats[1]) * FLOP_PER_PIXEL_PASS[7]
     achieved = flops / (kernel_ms * 1e-3) / 1e12
@@ -166,6 +204,11 @@ def main() -> int:
         log(f"cpu baseline: {cpu}")
         acc_frac = cpu["frac_within_1e-3px"] * float(ok.mean())
     fitted_per_s = K * acc_frac / (ms_per_step * 1e-3)
+    learned_per_s = None
+    if learned_leg is not None:
+        learned_per_s = K * acc_frac * learned_leg["status_ok"] / (learned_leg["ms_per_step"] * 1e-3)
+        learned_leg["features_per_s"] = learned_per_s
+        learned_leg["ratio_vs_plain"] = learned_per_s / fitted_per_s
 
     result = {
         "metric": "features fitted/sec at <=1e-3 px vs FP64 oracle; learned+batched vs plain LM",
@@ -190,6 +233,8 @@ def main() -> int:
             "counted_fraction": acc_frac,
             "mean_iterations": float(iters.mean()),
             "front_end_ms": front_ms,
+            "plain_lm_features_per_s": fitted_per_s,
+            "learned_start": learned_leg,
             "median_err_vs_truth_px": float(np.median(d_true)),
         },
         "roofline": {

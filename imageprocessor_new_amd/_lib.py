@@ -53,6 +53,23 @@ class SpxFitOptions(ctypes.Structure):
     ]
 
 
+class SpxTileSpec(ctypes.Structure):
+    _fields_ = [
+        ("layout", ctypes.c_int32),
+        ("cols", ctypes.c_int32),
+        ("pitch", ctypes.c_float),
+        ("jitter", ctypes.c_float),
+        ("r_lo", ctypes.c_float),
+        ("r_hi", ctypes.c_float),
+        ("s_lo", ctypes.c_float),
+        ("s_hi", ctypes.c_float),
+        ("ellipse_frac", ctypes.c_float),
+        ("amplitude", ctypes.c_float),
+        ("background", ctypes.c_float),
+        ("read_sigma", ctypes.c_float),
+    ]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -75,6 +92,7 @@ SIGNATURES = {
     "spx_cnn_train_step": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _I32, ctypes.c_float, _P, _P]),
     "spx_cnn_to_bf16": (_I32, [_P, _I64, _P, _P]),
     "spx_cnn_grad": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _P]),
+    "spx_render_tiles": (_I32, [ctypes.c_uint64, _I32, _I32, ctypes.POINTER(SpxTileSpec), _P, _I64, _P, _P]),
     "spx_fit_batch": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions),
                              _I32, _P, _P, _P, _P, _P]),
 }

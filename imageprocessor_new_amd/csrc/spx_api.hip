@@ -76,6 +76,8 @@ int cnn_train_step_launch(const uint16_t* x, const float* t, const float* w, int
                           uint16_t* theta_bf, float* adam_m, float* adam_v, int step, float lr, float* loss_out,
                           float* grad_out, hipStream_t stream);
 int cnn_to_bf16_launch(const float* theta, int64_t n, uint16_t* theta_bf, hipStream_t stream);
+int render_tiles_launch(uint64_t seed, int n, int T, const SpxTileSpec& spec, uint16_t* img, int64_t pitch,
+                        float* truth, hipStream_t stream);
 
 }  // namespace spx
 
@@ -135,6 +137,17 @@ int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, in
     SPX_REQUIRE(x && target && theta && theta_bf && grad_out, "spx_cnn_grad: null pointer");
     return spx::cnn_train_step_launch(x, target, weight, B, theta, theta_bf, nullptr, nullptr, 1, 1.0f, loss_out,
                                       grad_out, spx::as_stream(stream));
+}
+
+int spx_render_tiles(uint64_t seed, int32_t n, int32_t tile, const SpxTileSpec* spec, uint16_t* img,
+                     int64_t pitch, float* truth_out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(spec != nullptr && img != nullptr && truth_out != nullptr, "spx_render_tiles: null pointer");
+    SPX_REQUIRE(n >= 0 && tile >= 8 && tile <= 256 && spec->cols >= 1, "spx_render_tiles: bad sizes");
+    SPX_REQUIRE(pitch >= (int64_t)tile * spec->cols, "spx_render_tiles: pitch smaller than the mosaic width");
+    SPX_REQUIRE(spec->r_lo > 0 && spec->r_hi >= spec->r_lo && spec->s_lo > 0 && spec->s_hi >= spec->s_lo,
+                "spx_render_tiles: bad radius or sigma range");
+    return spx::render_tiles_launch(seed, n, tile, *spec, img, pitch, truth_out, spx::as_stream(stream));
 }
 
 int spx_cnn_to_bf16(const float* theta, int64_t n, uint16_t* theta_bf, spx_stream_t stream) {

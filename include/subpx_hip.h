@@ -57,6 +57,20 @@ typedef struct SpxFitOptions {
     int32_t persistent;   /* 1 = persistent work-queue launch, 0 = one wave per feature */
 } SpxFitOptions;
 
+/* Synthetic training tiles (spx_render_tiles). */
+typedef struct SpxTileSpec {
+    int32_t layout;       /* 0 = square array (8 neighbours), 1 = staggered (6)           */
+    int32_t cols;         /* tiles per mosaic row                                         */
+    float pitch;          /* neighbour distance, px                                       */
+    float jitter;         /* sigma of the centre jitter per axis, px                      */
+    float r_lo, r_hi;     /* radius ~ U[r_lo, r_hi]                                       */
+    float s_lo, s_hi;     /* edge sigma ~ U[s_lo, s_hi]                                   */
+    float ellipse_frac;   /* probability of an ellipse, q ~ U[0.8, 1]                     */
+    float amplitude;      /* A                                                            */
+    float background;     /* B                                                            */
+    float read_sigma;     /* Gaussian read noise, counts                                  */
+} SpxTileSpec;
+
 /* ---- lifecycle ---------------------------------------------------------------------- */
 /* Select and initialise a HIP device (reference: subpx/backends.py:55-59 gpu_device). */
 int spx_init(int device);
@@ -156,6 +170,14 @@ int spx_cnn_train_step(const uint16_t* x, const float* target, const float* weig
  * grad_out (spx_cnn_param_count() floats, device), loss_out host float or NULL. */
 int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, int32_t B, float* theta,
                  uint16_t* theta_bf, float* grad_out, float* loss_out, spx_stream_t stream);
+
+/* Render n synthetic tiles of side `tile` into a mosaic (tile t at column t % cols, row
+ * t / cols): one hole at the tile centre plus its array neighbours, erf-disk / ellipse
+ * model, photon noise in its normal limit + read noise, uint16. truth_out (n, 7)
+ * [x0, y0, r, s, A, B, q] of the centre hole in mosaic coordinates. img, truth: device.
+ * No reference counterpart (training data for the learned start). */
+int spx_render_tiles(uint64_t seed, int32_t n, int32_t tile, const SpxTileSpec* spec, uint16_t* img,
+                     int64_t pitch, float* truth_out, spx_stream_t stream);
 
 /* theta_bf = BF16(theta), n values, device. */
 int spx_cnn_to_bf16(const float* theta, int64_t n, uint16_t* theta_bf, spx_stream_t stream);

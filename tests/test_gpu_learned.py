@@ -1,0 +1,54 @@
+"""Learned start on the GPU: tile rendering, training on plain-LM labels, and the gain of
+the trained proposal over the moment start at equal accuracy."""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    from imageprocessor_new_amd.backends import gpu_device
+
+    return gpu_device(0)
+
+
+def test_tiles_fit_back_to_truth():
+    from imageprocessor_new_amd import learned
+
+    with _dev() as dev:
+        tiles = learned.render_tiles(2048, 7, learned.SPEC_SQUARE, dev)
+        res = learned.plain_fit(tiles.img, tiles.seeds, tiles.half, tiles.model)
+        st = res["status"].cpu().numpy()
+        p = res["params"].cpu().numpy()
+        tr = tiles.truth.cpu().numpy()
+    ok = st == 0
+    assert ok.mean() > 0.98
+    err = np.hypot(p[ok, 0] - tr[ok, 0], p[ok, 1] - tr[ok, 1])
+    assert np.median(err) < 0.02
+    assert np.median(np.abs(p[ok, 2] - tr[ok, 2])) < 0.02
+
+
+def test_training_gains_over_moment_start():
+    from imageprocessor_new_amd import learned
+
+    with _dev() as dev:
+        net, info = learned.train_learned_start(n_tiles=1 << 15, epochs=6, batch=1024, lr=2e-3, seed=20_000,
+                                                device=dev, mix=[(learned.SPEC_SQUARE, 0.5),
+                                                                 (learned.SPEC_STAGGERED, 0.5)])
+        assert info["losses"][-1] < info["losses"][0]
+        ev = learned.render_tiles(8192, 3, learned.SPEC_SQUARE, dev, fixed_half=15)
+        plain = learned.plain_fit(ev.img, ev.seeds, ev.half, ev.model)
+        res, timing = learned.fit_learned(net, ev.img, ev.seeds, ev.half, ev.model, lm_iters=4)
+        p0, s0, i0 = plain["params"].cpu().numpy(), plain["status"].cpu().numpy(), plain["iters"].cpu().numpy()
+        p1, s1, i1 = res["params"].cpu().numpy(), res["status"].cpu().numpy(), res["iters"].cpu().numpy()
+    both = (s0 == 0) & (s1 == 0)
+    assert both.mean() > 0.97
+    # same minimum: centres agree to 1e-3 px
+    assert np.abs(p0[both, :2] - p1[both, :2]).max() < 1e-3
+    # the trained proposal needs fewer LM steps than the moment start
+    print("mean iterations: moment start", i0.mean(), "learned", i1.mean(), "fallback", timing.n_fallback)
+    assert i1[both].mean() < i0[both].mean()
+    # after this short training a part of the features still needs the plain fallback
+    assert timing.n_fallback < 0.9 * len(s1)
