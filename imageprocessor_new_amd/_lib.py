@@ -53,6 +53,29 @@ class SpxFitOptions(ctypes.Structure):
     ]
 
 
+class SpxEdgeErfOptions(ctypes.Structure):
+    _fields_ = [
+        ("band_rad", ctypes.c_int32),
+        ("edge_rad", ctypes.c_int32),
+        ("smooth_passes", ctypes.c_int32),
+        ("loc_rad", ctypes.c_int32),
+        ("grad_power", ctypes.c_float),
+        ("iters", ctypes.c_int32),
+        ("erf_fit_rad", ctypes.c_int32),
+        ("erf_gn_iters", ctypes.c_int32),
+        ("erf_init_sigma", ctypes.c_float),
+        ("erf_sigma_min", ctypes.c_float),
+        ("erf_sigma_max", ctypes.c_float),
+        ("erf_max_shift", ctypes.c_float),
+        ("erf_damp", ctypes.c_float),
+        ("erf_cond_max", ctypes.c_float),
+    ]
+
+
+DTYPE_U16 = 0
+DTYPE_F32 = 1
+
+
 class SpxTileSpec(ctypes.Structure):
     _fields_ = [
         ("layout", ctypes.c_int32),
@@ -92,6 +115,8 @@ SIGNATURES = {
     "spx_cnn_train_step": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _I32, ctypes.c_float, _P, _P]),
     "spx_cnn_to_bf16": (_I32, [_P, _I64, _P, _P]),
     "spx_cnn_grad": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _P]),
+    "spx_edge_erf": (_I32, [_P, _I32, _I64, _I64, _I64, _P, _I32, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
+    "spx_edge_profile_erf": (_I32, [_P, _I32, _I32, _P, _P, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
     "spx_render_tiles": (_I32, [ctypes.c_uint64, _I32, _I32, ctypes.POINTER(SpxTileSpec), _P, _I64, _P, _P]),
     "spx_fit_batch": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions),
                              _I32, _P, _P, _P, _P, _P]),

@@ -76,6 +76,10 @@ int cnn_train_step_launch(const uint16_t* x, const float* t, const float* w, int
                           uint16_t* theta_bf, float* adam_m, float* adam_v, int step, float lr, float* loss_out,
                           float* grad_out, hipStream_t stream);
 int cnn_to_bf16_launch(const float* theta, int64_t n, uint16_t* theta_bf, hipStream_t stream);
+int edge_erf_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const float* rows, int K,
+                    const SpxEdgeErfOptions& opt, double* centers, uint8_t* ok, hipStream_t stream);
+int edge_profile_launch(const double* prof, int K, int L, const double* base, const double* seed,
+                        const SpxEdgeErfOptions& opt, double* x, uint8_t* ok, hipStream_t stream);
 int render_tiles_launch(uint64_t seed, int n, int T, const SpxTileSpec& spec, uint16_t* img, int64_t pitch,
                         float* truth, hipStream_t stream);
 
@@ -137,6 +141,28 @@ int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, in
     SPX_REQUIRE(x && target && theta && theta_bf && grad_out, "spx_cnn_grad: null pointer");
     return spx::cnn_train_step_launch(x, target, weight, B, theta, theta_bf, nullptr, nullptr, 1, 1.0f, loss_out,
                                       grad_out, spx::as_stream(stream));
+}
+
+int spx_edge_erf(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t pitch, const float* rows,
+                 int32_t K, const SpxEdgeErfOptions* opt, double* centers_out, uint8_t* ok_out,
+                 spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(opt != nullptr && K >= 0, "spx_edge_erf: bad arguments");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(img && rows && centers_out && ok_out, "spx_edge_erf: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_edge_erf: bad image shape or pitch");
+    SPX_REQUIRE(opt->iters >= 0 && opt->erf_gn_iters >= 0 && opt->smooth_passes >= 0 && opt->loc_rad >= 1,
+                "spx_edge_erf: bad iteration or window options");
+    return spx::edge_erf_launch(img, dtype, H, W, pitch, rows, K, *opt, centers_out, ok_out, spx::as_stream(stream));
+}
+
+int spx_edge_profile_erf(const double* prof, int32_t K, int32_t L, const double* base, const double* seed,
+                         const SpxEdgeErfOptions* opt, double* x_out, uint8_t* ok_out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(opt != nullptr && K >= 0, "spx_edge_profile_erf: bad arguments");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(prof && base && seed && x_out && ok_out, "spx_edge_profile_erf: null pointer");
+    return spx::edge_profile_launch(prof, K, L, base, seed, *opt, x_out, ok_out, spx::as_stream(stream));
 }
 
 int spx_render_tiles(uint64_t seed, int32_t n, int32_t tile, const SpxTileSpec* spec, uint16_t* img,

@@ -57,6 +57,28 @@ typedef struct SpxFitOptions {
     int32_t persistent;   /* 1 = persistent work-queue launch, 0 = one wave per feature */
 } SpxFitOptions;
 
+/* Options of the edge-erf refinement (defaults of the reference's
+ * refine_centers_edge_erf_gpu, subpx/_gpu/centers.py:882-900). */
+typedef struct SpxEdgeErfOptions {
+    int32_t band_rad;       /* 4: half width of the averaged band                        */
+    int32_t edge_rad;       /* 10: profile half length (2..15)                            */
+    int32_t smooth_passes;  /* 2: [1 4 6 4 1]/16 passes                                   */
+    int32_t loc_rad;        /* 3: grad-moment window; the erf polarity uses loc_rad - 1  */
+    float grad_power;       /* 8                                                          */
+    int32_t iters;          /* 2: outer passes                                            */
+    int32_t erf_fit_rad;    /* 5                                                          */
+    int32_t erf_gn_iters;   /* 5                                                          */
+    float erf_init_sigma;   /* 1.25                                                       */
+    float erf_sigma_min;    /* 0.35                                                       */
+    float erf_sigma_max;    /* 6                                                          */
+    float erf_max_shift;    /* 2.5                                                        */
+    float erf_damp;         /* 1e-4                                                       */
+    float erf_cond_max;     /* 1e8                                                        */
+} SpxEdgeErfOptions;
+
+#define SPX_DTYPE_U16 0
+#define SPX_DTYPE_F32 1
+
 /* Synthetic training tiles (spx_render_tiles). */
 typedef struct SpxTileSpec {
     int32_t layout;       /* 0 = square array (8 neighbours), 1 = staggered (6)           */
@@ -111,6 +133,23 @@ int spx_threshold_cc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, i
 int spx_refine_logquad(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
                        const int32_t* labels, const int32_t* rows, int32_t K, int32_t pad,
                        double* centers_out, uint8_t* ok_out, spx_stream_t stream);
+
+/* Edge-erf centre refinement, replacing _refine_edge_erf_core / refine_centers_edge_erf_gpu
+ * (subpx/_gpu/centers.py:782-929): left/right/top/bottom edge profiles (band means, binomial
+ * smoothing), gradient-moment seeds, 5-parameter erf-edge Gauss-Newton fits, centre =
+ * midpoints, two passes. img (H, W) uint16 (dtype SPX_DTYPE_U16) or float32 (SPX_DTYPE_F32)
+ * with a row pitch in elements; rows (K, 6) f32 [x, y, w, h, cx, cy]; centers_out (K, 2) f64;
+ * ok_out (K) uint8: device. opt: host. */
+int spx_edge_erf(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t pitch, const float* rows,
+                 int32_t K, const SpxEdgeErfOptions* opt, double* centers_out, uint8_t* ok_out,
+                 spx_stream_t stream);
+
+/* One erf-edge fit per profile (the reference's _edge_from_profile_erf_gpu,
+ * subpx/_gpu/centers.py:665-779; polarity window opt->loc_rad): prof (K, L) f64, base (K)
+ * f64 (position of sample 0), seed (K) f64; x_out (K) f64 (NaN when refused), ok_out (K)
+ * uint8. All device. */
+int spx_edge_profile_erf(const double* prof, int32_t K, int32_t L, const double* base, const double* seed,
+                         const SpxEdgeErfOptions* opt, double* x_out, uint8_t* ok_out, spx_stream_t stream);
 
 /* Batched erf-disk / erf-ellipse Levenberg-Marquardt fit of one patch per feature (no
  * reference function: the model comes from the task definition; the wave-per-feature LM

@@ -1,0 +1,75 @@
+"""Pins of the FP64 erf-disk / erf-ellipse LM oracle (oracle/erfdisk.py) by closed forms."""
+
+import numpy as np
+from scipy.special import erfc
+
+from imageprocessor_new_amd import synthetic
+from oracle import erfdisk
+
+
+def test_noise_free_recovery_disks_and_ellipse():
+    c = np.array([[40.3, 41.7], [100.45, 37.2], [70.1, 90.9]])
+    r = np.array([8.0, 7.3, 9.1])
+    s = np.array([1.2, 0.9, 1.5])
+    q = np.array([1.0, 0.85, 1.0])
+    clean = synthetic.render_clean((140, 140), c, r, s, q, 3000.0, 500.0)
+    p, rms, it, st = erfdisk.fit_batch(clean, c + 0.4, 15, [0, 1, 0], max_iter=60, step_tol=1e-10)
+    truth = np.c_[c, r, s, np.full(3, 3000.0), np.full(3, 500.0), q]
+    assert (st == 0).all()
+    assert np.abs(p - truth).max(axis=0)[:4].max() < 1e-8
+    assert rms.max() < 1e-8
+
+
+def test_jacobian_matches_central_differences():
+    rng = np.random.default_rng(0)
+    fy, fx = np.meshgrid(np.arange(-12, 13.0), np.arange(-12, 13.0), indexing="ij")
+    fx, fy = fx.ravel(), fy.ravel()
+    for ellipse in (False, True):
+        p = np.array([0.31, -0.27, 7.4, 1.1, 2800.0] + ([0.9] if ellipse else []) + [480.0])
+        _, J = erfdisk.model_jac(p, fx, fy, ellipse)
+        for k in range(p.size):
+            h = 1e-6 * max(1.0, abs(p[k]))
+            pp, pm = p.copy(), p.copy()
+            pp[k] += h
+            pm[k] -= h
+            fd = (erfdisk.model_jac(pp, fx, fy, ellipse)[0] - erfdisk.model_jac(pm, fx, fy, ellipse)[0]) / (2 * h)
+            assert np.abs(fd - J[:, k]).max() <= 1e-5 * max(1.0, np.abs(fd).max()), (ellipse, k)
+
+
+def test_lattice_sum_is_disk_area():
+    # sum over the integer lattice of erfc((rho - r) / (sqrt2 s)) / 2 equals the integral
+    # pi (r^2 + s^2) up to terms exp(-r^2 / 2 s^2) and aliasing exp(-2 pi^2 s^2): a wrong
+    # sqrt(2) in u would change the s^2 term by a factor 2.
+    for r, s in [(8.0, 1.2), (10.0, 1.5), (6.0, 1.0)]:
+        yy, xx = np.mgrid[-40:41, -40:41].astype(np.float64)
+        rho = np.hypot(xx - 0.37, yy + 0.21)
+        total = (0.5 * erfc((rho - r) / (np.sqrt(2.0) * s))).sum()
+        assert abs(total - np.pi * (r * r + s * s)) < 1e-6 * np.pi * r * r
+
+
+def test_integer_shift_and_mirror_equivariance():
+    arr = synthetic.make_config("c1", seed=9, H=160, W=160)
+    img = arr.image
+    seeds = np.rint(arr.centers_xy[:6])
+    p0, _, _, s0 = erfdisk.fit_batch(img, seeds, 12, 0)
+    shifted = np.zeros((160 + 7, 160 + 5), dtype=img.dtype)
+    shifted[7:, 5:] = img
+    p1, _, _, s1 = erfdisk.fit_batch(shifted, seeds + [5, 7], 12, 0)
+    ok = (s0 == 0) & (s1 == 0)
+    assert ok.any()
+    np.testing.assert_allclose(p1[ok, :2] - [5, 7], p0[ok, :2], atol=1e-9)
+    np.testing.assert_allclose(p1[ok, 2:], p0[ok, 2:], rtol=1e-9)
+    mirrored = img[:, ::-1]
+    seeds_m = np.c_[159 - seeds[:, 0], seeds[:, 1]]
+    p2, _, _, s2 = erfdisk.fit_batch(mirrored, seeds_m, 12, 0)
+    ok = (s0 == 0) & (s2 == 0)
+    np.testing.assert_allclose(159 - p2[ok, 0], p0[ok, 0], atol=1e-8)
+    np.testing.assert_allclose(p2[ok, 1], p0[ok, 1], atol=1e-8)
+
+
+def test_statuses_for_degenerate_patches():
+    flat = np.full((40, 40), 600, dtype=np.uint16)
+    _, _, _, st = erfdisk.fit_batch(flat, [[20, 20]], 8, 0)
+    assert st[0] == erfdisk.BAD_START
+    _, _, _, st = erfdisk.fit_batch(flat, [[-20, -20]], 8, 0)
+    assert st[0] == erfdisk.TOO_FEW
