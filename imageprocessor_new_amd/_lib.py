@@ -76,6 +76,25 @@ DTYPE_U16 = 0
 DTYPE_F32 = 1
 
 
+class SpxLogquadWindowOptions(ctypes.Structure):
+    _fields_ = [
+        ("win_rad", ctypes.c_int32),
+        ("bg_win_pad", ctypes.c_int32),
+        ("bg_dilate", ctypes.c_int32),
+        ("fit_dilate", ctypes.c_int32),
+        ("min_bg_pts", ctypes.c_int32),
+        ("min_plane_pts", ctypes.c_int32),
+        ("plane_bg", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("core_frac", ctypes.c_double),
+        ("weight_power", ctypes.c_double),
+        ("bg_tail_frac", ctypes.c_double),
+        ("max_cond", ctypes.c_double),
+        ("negdef_eps", ctypes.c_double),
+        ("eps", ctypes.c_double),
+    ]
+
+
 class SpxTileSpec(ctypes.Structure):
     _fields_ = [
         ("layout", ctypes.c_int32),
@@ -115,6 +134,10 @@ SIGNATURES = {
     "spx_cnn_train_step": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _I32, ctypes.c_float, _P, _P]),
     "spx_cnn_to_bf16": (_I32, [_P, _I64, _P, _P]),
     "spx_cnn_grad": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _P]),
+    "spx_refine_logquad_window": (_I32, [_P, _I32, _I64, _I64, _I64, _P, _P, _P, _I32, _I32,
+                                         ctypes.POINTER(SpxLogquadWindowOptions), _P, _P, _P]),
+    "spx_refine_weighted": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _P, _P, _P]),
+    "spx_edge_moment": (_I32, [_P, _I32, _I64, _I64, _I64, _P, _I32, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
     "spx_edge_erf": (_I32, [_P, _I32, _I64, _I64, _I64, _P, _I32, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
     "spx_edge_profile_erf": (_I32, [_P, _I32, _I32, _P, _P, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
     "spx_render_tiles": (_I32, [ctypes.c_uint64, _I32, _I32, ctypes.POINTER(SpxTileSpec), _P, _I64, _P, _P]),

@@ -6,13 +6,23 @@ area filter -> per-feature refinement, optionally over horizontal strips with a 
 Otsu threshold, an ownership band and a final de-duplication. Every stage runs on the GPU
 through libsubpx_hip.so; the host only filters component rows and merges strips.
 
-Refinement methods:
-  * "logquad": the reference CPU log-quadratic fit (subpx/_cpu/centers.py:67-107), the
-    parity target for this method;
-  * "none": the component's pixel centroid;
+Refinement methods (reference _detect_single_tile_gpu, subpx/_gpu/centers.py:1632-1710):
+  * "logquad": the GPU backend's window log-quadratic fit with plane background
+    (refine.py, reference :288-517); ``logquad_variant="cpu"`` selects the CPU backend's
+    per-label fit instead (subpx/_cpu/centers.py:67-107);
+  * "edge_gradmoment" (aliases "edge_moment", "edges_centered"): gradient-moment edges
+    (reference :547-636);
+  * "edge_erf" (aliases "edge_gradmoment_erf", "edge_erf_centered"): erf-edge fits
+    (reference :782-929);
+  * "weighted" and "none": the background-subtracted weighted centroid (reference
+    :1687-1710, both through _weighted_from_batch_gpu);
+  * "auto": per component, "logquad" when max(w, h) <= small_feature_max, else
+    "edge_gradmoment" (reference subpx/_cpu/centers.py:22-30);
   * "erf_disk" / "erf_ellipse": batched erf-edge disk / ellipse LM fits seeded at the
-    component centroid (no reference counterpart; see fit.py);
-  * "edge_erf": the reference's edge-erf refinement (subpx/_gpu/centers.py:782-929).
+    component centroid (no reference counterpart; see fit.py).
+Centres are emitted method group by method group in the reference's order (logquad,
+edge_gradmoment, edge_erf, weighted, none), each group in component order, keeping only
+accepted, finite centres.
 """
 
 from __future__ import annotations
@@ -23,36 +33,58 @@ import numpy as np
 import torch
 
 from . import _lib
-from .backends import gpu_device, resolve_backend, stream_handle, to_device_u16
+from .backends import gpu_device, resolve_backend, to_device_u16
 from .components import otsu_threshold_device, threshold_components_device
 from .types import CenterResult
 
 _ERF_METHODS = {"erf_disk": _lib.MODEL_ERF_DISK6, "erf_ellipse": _lib.MODEL_ERF_ELLIPSE7}
-_SUPPORTED = {"logquad", "logquadratic", "none", "edge_erf", "edge_gradmoment_erf",
-              "edge_erf_centered"} | set(_ERF_METHODS)
+_ALIASES = {"logquadratic": "logquad", "edge_moment": "edge_gradmoment", "edges_centered": "edge_gradmoment",
+            "edge_gradmoment_erf": "edge_erf", "edge_erf_centered": "edge_erf"}
+_GROUP_ORDER = ("logquad", "edge_gradmoment", "edge_erf", "weighted", "none") + tuple(_ERF_METHODS)
+_SUPPORTED = set(_GROUP_ORDER) | set(_ALIASES) | {"auto"}
 _VORONOI_METHODS = {"radial_symmetry", "isophote_curvature"}
 
 
-def _refine_group(img_t, labels, rows_np, cents_np, refine, *, pad, fit_half, fit_options):
-    """Refine the kept components of one image / strip. Returns (centers (K,2), ok (K,), extra)."""
+def choose_refine_method_for_bbox(width: float, height: float, *, small_feature_max: float = 12.0) -> str:
+    """Reference subpx/_cpu/centers.py:22-30."""
+    return "logquad" if float(max(width, height)) <= float(small_feature_max) else "edge_gradmoment"
+
+
+def _row_methods(refine, rows_np, small_feature_max):
+    K = rows_np.shape[0]
+    if refine == "auto":
+        size = np.maximum(rows_np[:, 2], rows_np[:, 3]).astype(np.float64)
+        return np.where(size <= float(small_feature_max), "logquad", "edge_gradmoment")
+    return np.full(K, _ALIASES.get(refine, refine), dtype=object)
+
+
+def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit_options, logquad_variant):
+    """Refine components of one method on one image / strip. rows_np (K, 5) [x, y, w, h, label],
+    cents_np (K, 2) CC centroids. Returns (centers (K, 2), ok (K,), extra)."""
+    from . import edge_erf, refine as rf
+
     dev = img_t.device
     K = rows_np.shape[0]
-    H, W = img_t.shape
     extra = {}
-    if K == 0:
-        return np.zeros((0, 2)), np.zeros(0, bool), extra
-    if refine in ("logquad", "logquadratic"):
-        rows_t = torch.from_numpy(np.ascontiguousarray(rows_np.astype(np.int32))).to(dev)
-        out = torch.empty((K, 2), dtype=torch.float64, device=dev)
-        ok = torch.empty((K,), dtype=torch.uint8, device=dev)
-        _lib.call("spx_refine_logquad", int(img_t.data_ptr()), H, W, img_t.stride(0),
-                  int(labels.data_ptr()), int(rows_t.data_ptr()), K, int(pad), int(out.data_ptr()),
-                  int(ok.data_ptr()), stream_handle())
-        torch.cuda.current_stream().synchronize()
-        return out.cpu().numpy(), ok.cpu().numpy().astype(bool), extra
-    if refine == "none":
-        return cents_np.copy(), np.isfinite(cents_np).all(axis=1), extra
-    if refine in _ERF_METHODS:
+    stats6 = np.concatenate([rows_np[:, :4].astype(np.float32), cents_np.astype(np.float32)], axis=1)
+    if method == "logquad":
+        if logquad_variant == "cpu":
+            c, ok = rf.refine_logquad_rows_device(img_t, labels, rows_np, pad=pad)
+        else:
+            c, ok = rf.refine_logquad_window_device(img_t, labels, torch.from_numpy(stats6).to(dev),
+                                                    torch.from_numpy(rows_np[:, 4].astype(np.int32)).to(dev),
+                                                    pad=pad)
+        return c, ok, extra
+    if method in ("weighted", "none"):
+        c, ok = rf.refine_weighted_rows_device(img_t, labels, rows_np, pad=pad)
+        return c, ok, extra
+    if method == "edge_gradmoment":
+        c, ok = edge_erf.refine_edge_moment_device(img_t, torch.from_numpy(stats6).to(dev))
+        return c, ok, extra
+    if method == "edge_erf":
+        c, ok = edge_erf.refine_edge_erf_device(img_t, torch.from_numpy(stats6).to(dev))
+        return c, ok, extra
+    if method in _ERF_METHODS:
         from .fit import fit_erf_disks_device
 
         seeds = torch.from_numpy(cents_np.astype(np.float32)).to(dev)
@@ -61,7 +93,7 @@ def _refine_group(img_t, labels, rows_np, cents_np, refine, *, pad, fit_half, fi
         else:
             half_np = np.broadcast_to(np.asarray(fit_half, dtype=np.int64), (K,))
         half_t = torch.from_numpy(half_np.astype(np.int16)).to(dev)
-        model_t = torch.full((K,), _ERF_METHODS[refine], dtype=torch.uint8, device=dev)
+        model_t = torch.full((K,), _ERF_METHODS[method], dtype=torch.uint8, device=dev)
         res = fit_erf_disks_device(img_t, seeds, half_t, model_t, options=fit_options)
         torch.cuda.current_stream().synchronize()
         params = res["params"].cpu().numpy()
@@ -70,17 +102,11 @@ def _refine_group(img_t, labels, rows_np, cents_np, refine, *, pad, fit_half, fi
                  "iterations": res["iters"].cpu().numpy()}
         ok = (status <= 1) & np.isfinite(params[:, :2]).all(axis=1)
         return params[:, :2].astype(np.float64), ok, extra
-    if refine in ("edge_erf", "edge_gradmoment_erf", "edge_erf_centered"):
-        from .edge_erf import refine_edge_erf_device
-
-        stats6 = np.concatenate([rows_np[:, :4].astype(np.float32), cents_np.astype(np.float32)], axis=1)
-        c, ok = refine_edge_erf_device(img_t, torch.from_numpy(stats6).to(dev))
-        return c, ok, extra
-    raise ValueError(f"Unsupported refine method: {refine}")
+    raise ValueError(f"Unsupported refine method: {method}")
 
 
 def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, refine, connectivity,
-                        pre_threshold, fit_half, fit_options):
+                        pre_threshold, fit_half, fit_options, small_feature_max=12.0, logquad_variant="gpu"):
     """Segmentation -> CC -> area filter -> refinement on one device image (or strip view)."""
     if pre_threshold is None:
         if threshold != "otsu":
@@ -96,16 +122,27 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
     keep = np.nonzero((area >= int(area_min)) & (area <= int(area_max)))[0]
     keep = keep[keep >= 1]
     rows = np.concatenate([stats_np[keep, :4], keep[:, None]], axis=1).astype(np.int64)
-    centers, ok, extra = _refine_group(img_t, labels, rows, cents_np[keep], refine, pad=pad,
-                                       fit_half=fit_half, fit_options=fit_options)
-    good = ok & np.isfinite(centers).all(axis=1)
-    info = {"threshold_value": thresh, "num_labels": num, "n_candidates": int(keep.size),
-            "n_refined": int(good.sum())}
-    if extra:
-        info["fit"] = {k: v[good] for k, v in extra.items()}
-        st = extra["status"]
-        info["fit_status_counts"] = {int(s): int((st == s).sum()) for s in np.unique(st)}
-    return centers[good], info
+    methods = _row_methods(refine, rows, small_feature_max)
+    out = []
+    info = {"threshold_value": thresh, "num_labels": num, "n_candidates": int(keep.size)}
+    n_refined = 0
+    for method in _GROUP_ORDER:
+        idx = np.nonzero(methods == method)[0]
+        if idx.size == 0:
+            continue
+        centers, ok, extra = _refine_rows(img_t, labels, rows[idx], cents_np[keep][idx], method, pad=pad,
+                                          fit_half=fit_half, fit_options=fit_options,
+                                          logquad_variant=logquad_variant)
+        good = ok & np.isfinite(centers).all(axis=1)
+        out.append(centers[good])
+        n_refined += int(good.sum())
+        if extra:
+            info["fit"] = {k: v[good] for k, v in extra.items()}
+            st = extra["status"]
+            info["fit_status_counts"] = {int(s): int((st == s).sum()) for s in np.unique(st)}
+    info["n_refined"] = n_refined
+    centers = np.concatenate(out, axis=0) if out else np.zeros((0, 2), dtype=np.float64)
+    return centers, info
 
 
 def detect_centers(
@@ -138,16 +175,21 @@ def detect_centers(
     fit_half=None,
     fit_max_iter: int = 30,
     fit_step_tol: float = 1e-6,
+    logquad_variant: str = "gpu",
 ) -> CenterResult:
     """Detect and refine centres (reference subpx/centers.py:130-220).
 
     Same keywords and CenterResult as the reference; ``gpu_batch``, ``use_float64``,
-    ``components_backend``, ``small_feature_max`` and ``upsample_factor`` are accepted for
-    signature compatibility (the kernels need no batching and always refine in FP64 /
-    FP32-Jacobian + FP64 solve). ``fit_half``, ``fit_max_iter`` and ``fit_step_tol``
-    configure the erf fits.
+    ``components_backend`` and ``upsample_factor`` are accepted for signature compatibility
+    (the kernels need no batching and always refine in FP64 / FP32-Jacobian + FP64 solve;
+    components always come from the GPU kernel, numbered as OpenCV numbers them).
+    ``small_feature_max`` splits ``refine="auto"``. ``fit_half``, ``fit_max_iter`` and
+    ``fit_step_tol`` configure the erf fits; ``logquad_variant`` picks the GPU backend's
+    ("gpu", default) or the CPU backend's ("cpu") log-quadratic algorithm.
     """
     b = resolve_backend(backend)
+    if logquad_variant not in ("gpu", "cpu"):
+        raise ValueError("logquad_variant must be 'gpu' or 'cpu'")
     if refine in _VORONOI_METHODS:
         raise NotImplementedError(f"refine='{refine}' (Voronoi methods) is not implemented")
     if background not in (None, "model"):
@@ -156,7 +198,7 @@ def detect_centers(
         raise NotImplementedError("background='model' (deblending) is not implemented")
     if morph_open or morph_close:
         raise NotImplementedError("morphological open/close is not implemented")
-    if refine == "auto" or refine not in _SUPPORTED:
+    if refine not in _SUPPORTED:
         raise ValueError(f"Unsupported refine method: {refine}")
     if isinstance(image, torch.Tensor):
         shape = tuple(image.shape)
@@ -166,13 +208,14 @@ def detect_centers(
         raise ValueError("Expected 2D grayscale image")
     fit_options = _lib.fit_options(max_iter=fit_max_iter, step_tol=fit_step_tol)
     common = dict(threshold=threshold, invert=invert, area_min=area_min, area_max=area_max, pad=pad,
-                  refine=refine, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options)
+                  refine=refine, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options,
+                  small_feature_max=small_feature_max, logquad_variant=logquad_variant)
     base_meta = {
         "invert": bool(invert), "area_min": int(area_min), "area_max": int(area_max),
         "morph_open": int(morph_open), "morph_close": int(morph_close), "pad": int(pad),
         "connectivity": int(connectivity), "gpu_batch": int(gpu_batch), "device": int(device),
         "use_float64": bool(use_float64), "components_backend": str(components_backend),
-        "small_feature_max": float(small_feature_max),
+        "small_feature_max": float(small_feature_max), "logquad_variant": str(logquad_variant),
     }
     with gpu_device(device) as dev:
         img_t = to_device_u16(image, dev)

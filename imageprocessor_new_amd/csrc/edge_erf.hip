@@ -18,7 +18,11 @@ namespace {
 constexpr int kLMax = 31;  // edge_rad <= 15
 constexpr int kBandMax = 15;
 
+// Profiles are formed with the same IEEE operations as the FP64 restatement (no FMA
+// contraction, band mean as a division), so exact ties of |diff| in the argmax break the
+// same way on both sides.
 __device__ inline void smooth5(double* x, int L) {
+#pragma clang fp contract(off)
     double y[kLMax];
     for (int j = 0; j < L; ++j) {
         const double xm2 = x[max(j - 2, 0)], xm1 = x[max(j - 1, 0)];
@@ -29,6 +33,7 @@ __device__ inline void smooth5(double* x, int L) {
 }
 
 __device__ inline bool gradmoment(const double* prof, int L, double base, int loc, double power, double& x_edge) {
+#pragma clang fp contract(off)
     const int Lm1 = L - 1;
     int ip = 0;
     double best = -1.0;
@@ -255,9 +260,12 @@ __device__ inline int64_t toint(double v) {
     return isfinite(r) && fabs(r) < 9.0e18 ? (int64_t)r : 0;
 }
 
-template <typename T>
+// kErf: gradient-moment seeds refined by erf fits (_refine_edge_erf_core); otherwise the
+// seeds are the edges (_refine_edge_moment_core, :547-604, no xR > xL / yB > yT test).
+template <typename T, bool kErf>
 __global__ void edge_erf_kernel(const T* img, int64_t H, int64_t W, int64_t pitch, const float* rows, int K,
                                 SpxEdgeErfOptions opt, ErfOpts o, double* centers, uint8_t* ok_out) {
+#pragma clang fp contract(off)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const float* row = rows + 6 * (int64_t)k;
@@ -265,7 +273,7 @@ __global__ void edge_erf_kernel(const T* img, int64_t H, int64_t W, int64_t pitc
     double xc = (double)row[4], yc = (double)row[5];
     const int br = opt.band_rad, er = opt.edge_rad;
     const int L = 2 * er + 1;
-    const double inv_band = 1.0 / (double)(2 * br + 1);
+    const double nband = (double)(2 * br + 1);
     bool ok_all = true;
     for (int pass = 0; pass < opt.iters; ++pass) {
         const int64_t e0[4] = {toint(xc - 0.5 * (w - 1.0)), toint(xc + 0.5 * (w - 1.0)),
@@ -288,18 +296,23 @@ __global__ void edge_erf_kernel(const T* img, int64_t H, int64_t W, int64_t pitc
                     }
                     s += (double)img[y * pitch + x];
                 }
-                prof[j] = s * inv_band;
+                prof[j] = s / nband;
             }
             for (int sp = 0; sp < opt.smooth_passes; ++sp) smooth5(prof, L);
             const double base = (double)e0[side] - (double)er;
             double seed;
             const bool ok0 = gradmoment(prof, L, base, opt.loc_rad, (double)opt.grad_power, seed);
-            double e;
-            const bool ok1 = erf_profile(prof, L, base, seed, o, e);
-            edge[side] = e;
-            oks = oks && ok0 && ok1;
+            if (kErf) {
+                double e;
+                const bool ok1 = erf_profile(prof, L, base, seed, o, e);
+                edge[side] = e;
+                oks = oks && ok0 && ok1;
+            } else {
+                edge[side] = seed;
+                oks = oks && ok0;
+            }
         }
-        oks = oks && (edge[1] > edge[0]) && (edge[3] > edge[2]);
+        if (kErf) oks = oks && (edge[1] > edge[0]) && (edge[3] > edge[2]);
         ok_all = ok_all && oks;
         xc = 0.5 * (edge[0] + edge[1]);
         yc = 0.5 * (edge[2] + edge[3]);
@@ -317,20 +330,40 @@ ErfOpts erf_opts(const SpxEdgeErfOptions& opt) {
 
 }  // namespace
 
-int edge_erf_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const float* rows, int K,
-                    const SpxEdgeErfOptions& opt, double* centers, uint8_t* ok, hipStream_t stream) {
+template <bool kErf>
+int edge_launch(const char* name, const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const float* rows,
+                int K, const SpxEdgeErfOptions& opt, double* centers, uint8_t* ok, hipStream_t stream) {
     if (K == 0) return SPX_OK;
-    SPX_REQUIRE(opt.edge_rad >= 2 && opt.edge_rad <= (kLMax - 1) / 2, "spx_edge_erf: edge_rad must lie in [2, 15]");
-    SPX_REQUIRE(opt.band_rad >= 0 && opt.band_rad <= kBandMax, "spx_edge_erf: band_rad must lie in [0, 15]");
-    SPX_REQUIRE(dtype == SPX_DTYPE_U16 || dtype == SPX_DTYPE_F32, "spx_edge_erf: dtype must be U16 or F32");
+    if (!(opt.edge_rad >= 2 && opt.edge_rad <= (kLMax - 1) / 2)) {
+        set_error(std::string(name) + ": edge_rad must lie in [2, 15]");
+        return SPX_E_ARG;
+    }
+    if (!(opt.band_rad >= 0 && opt.band_rad <= kBandMax)) {
+        set_error(std::string(name) + ": band_rad must lie in [0, 15]");
+        return SPX_E_ARG;
+    }
+    if (dtype != SPX_DTYPE_U16 && dtype != SPX_DTYPE_F32) {
+        set_error(std::string(name) + ": dtype must be U16 or F32");
+        return SPX_E_ARG;
+    }
     if (dtype == SPX_DTYPE_U16)
-        edge_erf_kernel<uint16_t><<<(K + 127) / 128, 128, 0, stream>>>(static_cast<const uint16_t*>(img), H, W, pitch,
-                                                                        rows, K, opt, erf_opts(opt), centers, ok);
+        edge_erf_kernel<uint16_t, kErf><<<(K + 127) / 128, 128, 0, stream>>>(
+            static_cast<const uint16_t*>(img), H, W, pitch, rows, K, opt, erf_opts(opt), centers, ok);
     else
-        edge_erf_kernel<float><<<(K + 127) / 128, 128, 0, stream>>>(static_cast<const float*>(img), H, W, pitch, rows,
-                                                                     K, opt, erf_opts(opt), centers, ok);
+        edge_erf_kernel<float, kErf><<<(K + 127) / 128, 128, 0, stream>>>(
+            static_cast<const float*>(img), H, W, pitch, rows, K, opt, erf_opts(opt), centers, ok);
     SPX_LAUNCHED();
     return SPX_OK;
+}
+
+int edge_erf_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const float* rows, int K,
+                    const SpxEdgeErfOptions& opt, double* centers, uint8_t* ok, hipStream_t stream) {
+    return edge_launch<true>("spx_edge_erf", img, dtype, H, W, pitch, rows, K, opt, centers, ok, stream);
+}
+
+int edge_moment_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const float* rows, int K,
+                       const SpxEdgeErfOptions& opt, double* centers, uint8_t* ok, hipStream_t stream) {
+    return edge_launch<false>("spx_edge_moment", img, dtype, H, W, pitch, rows, K, opt, centers, ok, stream);
 }
 
 int edge_profile_launch(const double* prof, int K, int L, const double* base, const double* seed,

@@ -14,33 +14,10 @@
 // uses normal equations in coordinates centred on the ROI and Jacobi-scaled, solved by
 // LDL^T; the centre and cond(H) do not depend on the shift.
 #include "spx_internal.hpp"
+#include "wave64.hpp"
 
 namespace spx {
 namespace {
-
-// Wave reductions return the first lane's total through readfirstlane, so every branch
-// taken on them is a uniform (scalar) branch.
-__device__ __forceinline__ double uniform_d(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
-    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return uniform_d(v);
-}
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return __builtin_amdgcn_readfirstlane(v);
-}
-__device__ __forceinline__ double wave_max_d(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-    return uniform_d(v);
-}
 
 struct Roi {
     int64_t x0, y0;
@@ -85,13 +62,89 @@ __device__ int ring_kth(const uint16_t* img, int64_t pitch, const Roi& r, bool w
             ring_pos(r, whole, j, ry, rx);
             c += (int)img[(r.y0 + ry) * pitch + r.x0 + rx] <= mid;
         }
-        c = wave_sum_i(c);
+        c = wave::sum_i(c);
         if (c > k)
             hi = mid;
         else
             lo = mid + 1;
     }
     return lo;
+}
+
+// np.median of the ring (mean of the middle pair for an even count).
+__device__ double border_median(const uint16_t* img, int64_t pitch, const Roi& r) {
+    const bool whole = (r.rh <= 4 || r.rw <= 4);
+    const int n = whole ? r.rw * r.rh : 4 * r.rw + 4 * r.rh;
+    if (n & 1) return (double)ring_kth(img, pitch, r, whole, n, n >> 1);
+    const int a = ring_kth(img, pitch, r, whole, n, (n >> 1) - 1);
+    const int b = ring_kth(img, pitch, r, whole, n, n >> 1);
+    return 0.5 * ((double)a + (double)b);
+}
+
+// ROI = bbox +- pad clipped to the image (reference subpx/_cpu/centers.py:381-384); false when empty.
+__device__ __forceinline__ bool make_roi(const int32_t* row, int32_t pad, int64_t H, int64_t W, Roi& r,
+                                         int32_t& lab) {
+    const int64_t x = __builtin_amdgcn_readfirstlane(row[0]);
+    const int64_t y = __builtin_amdgcn_readfirstlane(row[1]);
+    const int64_t w = __builtin_amdgcn_readfirstlane(row[2]);
+    const int64_t h = __builtin_amdgcn_readfirstlane(row[3]);
+    lab = __builtin_amdgcn_readfirstlane(row[4]);
+    r.x0 = x - pad > 0 ? x - pad : 0;
+    r.y0 = y - pad > 0 ? y - pad : 0;
+    const int64_t x1 = (x + w + pad) < W ? (x + w + pad) : W;
+    const int64_t y1 = (y + h + pad) < H ? (y + h + pad) : H;
+    r.rw = (int)(x1 - r.x0);
+    r.rh = (int)(y1 - r.y0);
+    return x1 > r.x0 && y1 > r.y0;
+}
+
+// Weighted centroid of the GPU detect path's "weighted" / "none" methods (reference
+// subpx/_gpu/centers.py:43-66 via _refine_batch_gpu :931-955, background from the CPU
+// border median :33-46): w = max(g - bg, 0) on the mask; centroid sum(w x) / max(sum w, 1e-12);
+// the mask mean when sum w <= 0; NaN unless inside [0, rw) x [0, rh) of the ROI.
+__global__ __launch_bounds__(256) void weighted_kernel(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
+                                                       const int32_t* labels, const int32_t* rows, int32_t K,
+                                                       int32_t pad, double* centers, uint8_t* ok_out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t f = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (f >= K) return;
+    Roi r;
+    int32_t lab;
+    bool good = make_roi(rows + 5 * f, pad, H, W, r, lab);
+    double cx = __builtin_nan(""), cy = __builtin_nan("");
+    if (good) {
+        const double bg = border_median(img, pitch, r);
+        double s[6] = {0, 0, 0, 0, 0, 0};  // sum w, sum w x, sum w y, count, sum x, sum y
+        const int npx = r.rw * r.rh;
+        for (int j = lane; j < npx; j += 64) {
+            const int ry = j / r.rw, rx = j - (j / r.rw) * r.rw;
+            const int64_t gy = r.y0 + ry, gx = r.x0 + rx;
+            if (labels[gy * W + gx] != lab) continue;
+            const double wt = fmax((double)img[gy * pitch + gx] - bg, 0.0);
+            s[0] += wt;
+            s[1] += wt * (double)rx;
+            s[2] += wt * (double)ry;
+            s[3] += 1.0;
+            s[4] += (double)rx;
+            s[5] += (double)ry;
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s[q] = wave::sum_d(s[q]);
+        cx = s[1] / fmax(s[0], 1e-12);
+        cy = s[2] / fmax(s[0], 1e-12);
+        if (!(s[0] > 0.0) || !isfinite(cx) || !isfinite(cy)) {
+            cx = s[4] / fmax(s[3], 1e-12);
+            cy = s[5] / fmax(s[3], 1e-12);
+        }
+        good = cx >= 0.0 && cy >= 0.0 && cx < (double)r.rw && cy < (double)r.rh;
+        cx += (double)r.x0;
+        cy += (double)r.y0;
+    }
+    if (lane == 0) {
+        centers[2 * f] = good ? cx : __builtin_nan("");
+        centers[2 * f + 1] = good ? cy : __builtin_nan("");
+        ok_out[f] = good ? 1 : 0;
+    }
 }
 
 __global__ __launch_bounds__(256) void logquad_kernel(const uint16_t* img, int64_t H, int64_t W,
@@ -101,32 +154,12 @@ __global__ __launch_bounds__(256) void logquad_kernel(const uint16_t* img, int64
     const int lane = threadIdx.x & 63;
     const int64_t f = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (f >= K) return;
-    const int32_t* row = rows + 5 * f;
-    const int64_t x = __builtin_amdgcn_readfirstlane(row[0]);
-    const int64_t y = __builtin_amdgcn_readfirstlane(row[1]);
-    const int64_t w = __builtin_amdgcn_readfirstlane(row[2]);
-    const int64_t h = __builtin_amdgcn_readfirstlane(row[3]);
-    const int32_t lab = __builtin_amdgcn_readfirstlane(row[4]);
     Roi r;
-    r.x0 = x - pad > 0 ? x - pad : 0;
-    r.y0 = y - pad > 0 ? y - pad : 0;
-    const int64_t x1 = (x + w + pad) < W ? (x + w + pad) : W;
-    const int64_t y1 = (y + h + pad) < H ? (y + h + pad) : H;
-    bool good = x1 > r.x0 && y1 > r.y0;
-    r.rw = (int)(x1 - r.x0);
-    r.rh = (int)(y1 - r.y0);
+    int32_t lab;
+    bool good = make_roi(rows + 5 * f, pad, H, W, r, lab);
     double cx = __builtin_nan(""), cy = __builtin_nan("");
     if (good) {
-        const bool whole = (r.rh <= 4 || r.rw <= 4);
-        const int n = whole ? r.rw * r.rh : 4 * r.rw + 4 * r.rh;
-        double bg;
-        if (n & 1) {
-            bg = (double)ring_kth(img, pitch, r, whole, n, n >> 1);
-        } else {
-            const int a = ring_kth(img, pitch, r, whole, n, (n >> 1) - 1);
-            const int b = ring_kth(img, pitch, r, whole, n, n >> 1);
-            bg = 0.5 * ((double)a + (double)b);
-        }
+        const double bg = border_median(img, pitch, r);
         // mask statistics and normal equations in centred coordinates
         const double xc = 0.5 * (double)(r.rw - 1);
         const double yc = 0.5 * (double)(r.rh - 1);
@@ -141,8 +174,8 @@ __global__ __launch_bounds__(256) void logquad_kernel(const uint16_t* img, int64
                 vmax = fmax(vmax, (double)img[gy * pitch + gx] - bg);
             }
         }
-        cnt = wave_sum_i(cnt);
-        vmax = wave_max_d(vmax);
+        cnt = wave::sum_i(cnt);
+        vmax = wave::max_d(vmax);
         if (cnt < 6 || !(vmax > 0.0)) good = false;
         if (good) {
             double A[21];
@@ -168,9 +201,9 @@ __global__ __launch_bounds__(256) void logquad_kernel(const uint16_t* img, int64
                 }
             }
 #pragma unroll
-            for (int i = 0; i < 21; ++i) A[i] = wave_sum_d(A[i]);
+            for (int i = 0; i < 21; ++i) A[i] = wave::sum_d(A[i]);
 #pragma unroll
-            for (int i = 0; i < 6; ++i) bz[i] = wave_sum_d(bz[i]);
+            for (int i = 0; i < 6; ++i) bz[i] = wave::sum_d(bz[i]);
             // Jacobi scaling + LDL^T
             double M[6][6], s[6], g6[6];
             int idx = 0;
@@ -266,6 +299,17 @@ int logquad_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, con
     const int blocks = (K + waves_per_block - 1) / waves_per_block;
     logquad_kernel<<<blocks, 64 * waves_per_block, 0, stream>>>(img, H, W, pitch, labels, rows, K, pad,
                                                                 centers, ok);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int weighted_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
+                    const int32_t* rows, int32_t K, int32_t pad, double* centers, uint8_t* ok, hipStream_t stream) {
+    if (K == 0) return SPX_OK;
+    const int waves_per_block = 4;
+    const int blocks = (K + waves_per_block - 1) / waves_per_block;
+    weighted_kernel<<<blocks, 64 * waves_per_block, 0, stream>>>(img, H, W, pitch, labels, rows, K, pad, centers,
+                                                                 ok);
     SPX_LAUNCHED();
     return SPX_OK;
 }

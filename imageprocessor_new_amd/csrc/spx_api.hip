@@ -76,6 +76,13 @@ int cnn_train_step_launch(const uint16_t* x, const float* t, const float* w, int
                           uint16_t* theta_bf, float* adam_m, float* adam_v, int step, float lr, float* loss_out,
                           float* grad_out, hipStream_t stream);
 int cnn_to_bf16_launch(const float* theta, int64_t n, uint16_t* theta_bf, hipStream_t stream);
+int logquad_window_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
+                          const float* rows, const int32_t* lab_ids, int32_t K, int32_t pad,
+                          const SpxLogquadWindowOptions& opt, double* centers, uint8_t* ok, hipStream_t stream);
+int weighted_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
+                    const int32_t* rows, int32_t K, int32_t pad, double* centers, uint8_t* ok, hipStream_t stream);
+int edge_moment_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const float* rows, int K,
+                       const SpxEdgeErfOptions& opt, double* centers, uint8_t* ok, hipStream_t stream);
 int edge_erf_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const float* rows, int K,
                     const SpxEdgeErfOptions& opt, double* centers, uint8_t* ok, hipStream_t stream);
 int edge_profile_launch(const double* prof, int K, int L, const double* base, const double* seed,
@@ -154,6 +161,47 @@ int spx_edge_erf(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t p
     SPX_REQUIRE(opt->iters >= 0 && opt->erf_gn_iters >= 0 && opt->smooth_passes >= 0 && opt->loc_rad >= 1,
                 "spx_edge_erf: bad iteration or window options");
     return spx::edge_erf_launch(img, dtype, H, W, pitch, rows, K, *opt, centers_out, ok_out, spx::as_stream(stream));
+}
+
+int spx_edge_moment(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t pitch, const float* rows,
+                    int32_t K, const SpxEdgeErfOptions* opt, double* centers_out, uint8_t* ok_out,
+                    spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(opt != nullptr && K >= 0, "spx_edge_moment: bad arguments");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(img && rows && centers_out && ok_out, "spx_edge_moment: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_edge_moment: bad image shape or pitch");
+    SPX_REQUIRE(opt->iters >= 0 && opt->smooth_passes >= 0 && opt->loc_rad >= 0,
+                "spx_edge_moment: bad iteration or window options");
+    return spx::edge_moment_launch(img, dtype, H, W, pitch, rows, K, *opt, centers_out, ok_out,
+                                   spx::as_stream(stream));
+}
+
+int spx_refine_logquad_window(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t pitch,
+                              const int32_t* labels, const float* rows, const int32_t* lab_ids, int32_t K,
+                              int32_t pad, const SpxLogquadWindowOptions* opt, double* centers_out,
+                              uint8_t* ok_out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(opt != nullptr && K >= 0, "spx_refine_logquad_window: bad arguments");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(img && labels && rows && lab_ids && centers_out && ok_out, "spx_refine_logquad_window: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_refine_logquad_window: bad image shape or pitch");
+    SPX_REQUIRE(pad >= 0, "spx_refine_logquad_window: pad < 0");
+    return spx::logquad_window_launch(img, dtype, H, W, pitch, labels, rows, lab_ids, K, pad, *opt, centers_out,
+                                      ok_out, spx::as_stream(stream));
+}
+
+int spx_refine_weighted(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
+                        const int32_t* rows, int32_t K, int32_t pad, double* centers_out, uint8_t* ok_out,
+                        spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(K >= 0, "spx_refine_weighted: K < 0");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(img && labels && rows && centers_out && ok_out, "spx_refine_weighted: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_refine_weighted: bad image shape or pitch");
+    SPX_REQUIRE(pad >= 0, "spx_refine_weighted: pad < 0");
+    return spx::weighted_launch(static_cast<const uint16_t*>(img), H, W, pitch, labels, rows, K, pad, centers_out,
+                                ok_out, spx::as_stream(stream));
 }
 
 int spx_edge_profile_erf(const double* prof, int32_t K, int32_t L, const double* base, const double* seed,

@@ -1,10 +1,11 @@
-"""Edge-erf centre refinement on the GPU (reference subpx/_gpu/centers.py:518-929).
+"""Edge-profile centre refinements on the GPU (reference subpx/_gpu/centers.py:518-929).
 
-``refine_centers_edge_erf_gpu`` keeps the reference's signature, defaults and return
-value: ``(centers (N, 2) float64, ok (N,) bool)`` for rows ``[x, y, w, h, cx, cy]``. The work
-runs in one HIP kernel, one thread per feature, through ``spx_edge_erf``; the reference's
-per-profile fit ``_edge_from_profile_erf_gpu`` (:665-779) is exposed as
-``edge_from_profile_erf`` through ``spx_edge_profile_erf``.
+``refine_centers_edge_erf_gpu`` and ``refine_centers_edge_moment_gpu`` keep the reference's
+signatures, defaults and return value: ``(centers (N, 2) float64, ok (N,) bool)`` for rows
+``[x, y, w, h, cx, cy]``. Each runs in one HIP kernel, one thread per feature
+(``spx_edge_erf``, ``spx_edge_moment``); the reference's per-profile fit
+``_edge_from_profile_erf_gpu`` (:665-779) is exposed as ``edge_from_profile_erf`` through
+``spx_edge_profile_erf``.
 """
 
 from __future__ import annotations
@@ -38,6 +39,15 @@ def edge_options(**kw) -> _lib.SpxEdgeErfOptions:
 def refine_edge_erf_device(img_t: torch.Tensor, stats6_t: torch.Tensor, **kw):
     """Refine on a device image (uint16 or float32, (H, W), unit column stride) and device rows
     (K, 6) float32. Returns NumPy (centers (K, 2) float64, ok (K,) bool)."""
+    return _edge_call("spx_edge_erf", img_t, stats6_t, **kw)
+
+
+def refine_edge_moment_device(img_t: torch.Tensor, stats6_t: torch.Tensor, **kw):
+    """Gradient-moment edges only (no erf fits); arguments and result as refine_edge_erf_device."""
+    return _edge_call("spx_edge_moment", img_t, stats6_t, **kw)
+
+
+def _edge_call(name, img_t, stats6_t, **kw):
     if img_t.dim() != 2 or img_t.stride(1) != 1:
         raise ValueError("image must be a 2-D device tensor with unit column stride")
     if img_t.dtype == torch.uint16:
@@ -54,7 +64,7 @@ def refine_edge_erf_device(img_t: torch.Tensor, stats6_t: torch.Tensor, **kw):
     if K:
         opt = edge_options(**kw)
         H, W = img_t.shape
-        _lib.call("spx_edge_erf", img_t.data_ptr(), dtype, int(H), int(W), int(img_t.stride(0)), rows.data_ptr(), K,
+        _lib.call(name, img_t.data_ptr(), dtype, int(H), int(W), int(img_t.stride(0)), rows.data_ptr(), K,
                   ctypes.byref(opt), centers.data_ptr(), ok.data_ptr(), stream_handle())
     torch.cuda.current_stream().synchronize()
     return centers.cpu().numpy(), ok.cpu().numpy().astype(bool)
@@ -84,6 +94,29 @@ def refine_centers_edge_erf_gpu(gray, stats_xywh_cc, *, band_rad: int = 4, edge_
             grad_power=grad_power, iters=iters, erf_fit_rad=erf_fit_rad, erf_gn_iters=erf_gn_iters,
             erf_init_sigma=erf_init_sigma, erf_sigma_min=erf_sigma_min, erf_sigma_max=erf_sigma_max,
             erf_max_shift=erf_max_shift, erf_damp=erf_damp, erf_cond_max=erf_cond_max)
+
+
+def refine_centers_edge_moment_gpu(gray, stats_xywh_cc, *, band_rad: int = 4, edge_rad: int = 10,
+                                   smooth_passes: int = 2, loc_rad: int = 3, grad_power: float = 8.0, iters: int = 2,
+                                   device: int = 0):
+    """Reference refine_centers_edge_moment_gpu (subpx/_gpu/centers.py:607-636): centre =
+    midpoints of the gradient-moment edges of the four edge profiles, two passes. The
+    reference computes the profiles and moments in float32; this kernel uses float64."""
+    resolve_backend("gpu")
+    g_np = np.asarray(gray, dtype=np.float32)
+    stats_np = np.asarray(stats_xywh_cc, dtype=np.float32)
+    if g_np.ndim != 2:
+        raise ValueError("gray must be a 2D grayscale image")
+    if stats_np.size == 0:
+        return np.zeros((0, 2), dtype=np.float64), np.zeros((0,), dtype=bool)
+    if stats_np.ndim != 2 or stats_np.shape[1] < 6:
+        raise ValueError("stats_xywh_cc must have shape (N, >=6) with columns [x, y, w, h, cx, cy]")
+    with gpu_device(device) as dev:
+        img_t = torch.from_numpy(np.ascontiguousarray(g_np)).to(dev)
+        rows_t = torch.from_numpy(np.ascontiguousarray(stats_np[:, :6])).to(dev)
+        return refine_edge_moment_device(img_t, rows_t, band_rad=band_rad, edge_rad=edge_rad,
+                                         smooth_passes=smooth_passes, loc_rad=loc_rad, grad_power=grad_power,
+                                         iters=iters)
 
 
 def edge_from_profile_erf(prof, base_x, x_seed, *, fit_rad: int = 5, gn_iters: int = 5, init_sigma: float = 1.25,

@@ -79,6 +79,25 @@ typedef struct SpxEdgeErfOptions {
 #define SPX_DTYPE_U16 0
 #define SPX_DTYPE_F32 1
 
+/* Options of the window log-quadratic refinement (defaults of the reference's
+ * refine_centers_logquad_gpu_match_cpu, subpx/_gpu/centers.py:288-309). */
+typedef struct SpxLogquadWindowOptions {
+    int32_t win_rad;        /* 4: fit window (2 win_rad + 1)^2                              */
+    int32_t bg_win_pad;     /* 2: outer window radius win_rad + bg_win_pad (at most 10)    */
+    int32_t bg_dilate;      /* 2: blob dilation before the background median              */
+    int32_t fit_dilate;     /* 1: blob dilation of the fit mask                            */
+    int32_t min_bg_pts;     /* 12                                                          */
+    int32_t min_plane_pts;  /* 20                                                          */
+    int32_t plane_bg;       /* 1: plane background, 0: scalar median                       */
+    int32_t reserved;       /* 0                                                           */
+    double core_frac;       /* 0.2                                                         */
+    double weight_power;    /* 1                                                           */
+    double bg_tail_frac;    /* 0.05                                                        */
+    double max_cond;        /* 1e10                                                        */
+    double negdef_eps;      /* 1e-12                                                       */
+    double eps;             /* 1e-12                                                       */
+} SpxLogquadWindowOptions;
+
 /* Synthetic training tiles (spx_render_tiles). */
 typedef struct SpxTileSpec {
     int32_t layout;       /* 0 = square array (8 neighbours), 1 = staggered (6)           */
@@ -133,6 +152,34 @@ int spx_threshold_cc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, i
 int spx_refine_logquad(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
                        const int32_t* labels, const int32_t* rows, int32_t K, int32_t pad,
                        double* centers_out, uint8_t* ok_out, spx_stream_t stream);
+
+/* Window log-quadratic refinement, the GPU backend's "logquad" method, replacing
+ * refine_centers_logquad_gpu_match_cpu (subpx/_gpu/centers.py:288-517): plane background
+ * from a lower median outside the dilated blob, weighted log-quadratic fit in a
+ * (2 win_rad + 1)^2 window at rint(centroid). img (H, W) uint16 / float32 (dtype) with a row
+ * pitch in elements; labels (H, W) int32 contiguous; rows (K, 6) f32 [x, y, w, h, cx, cy];
+ * lab_ids (K) int32; centers_out (K, 2) f64 (the centroid where refused); ok_out (K) uint8:
+ * device. opt: host. */
+int spx_refine_logquad_window(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t pitch,
+                              const int32_t* labels, const float* rows, const int32_t* lab_ids, int32_t K,
+                              int32_t pad, const SpxLogquadWindowOptions* opt, double* centers_out,
+                              uint8_t* ok_out, spx_stream_t stream);
+
+/* Weighted centroid of the GPU detect path's "weighted" and "none" methods
+ * (subpx/_gpu/centers.py:1687-1710 -> _refine_batch_gpu :931-955 -> _weighted_from_batch_gpu
+ * :43-66, border median of subpx/_cpu/centers.py:31-42). Same arguments as
+ * spx_refine_logquad; NaN and ok 0 where the centroid leaves the ROI. */
+int spx_refine_weighted(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
+                        const int32_t* rows, int32_t K, int32_t pad, double* centers_out, uint8_t* ok_out,
+                        spx_stream_t stream);
+
+/* Edge gradient-moment centre refinement, replacing _refine_edge_moment_core /
+ * refine_centers_edge_moment_gpu (subpx/_gpu/centers.py:547-636): the profiles and seeds of
+ * spx_edge_erf without the erf fits (band_rad, edge_rad, smooth_passes, loc_rad, grad_power
+ * and iters of opt are used). Same arguments as spx_edge_erf. */
+int spx_edge_moment(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t pitch, const float* rows,
+                    int32_t K, const SpxEdgeErfOptions* opt, double* centers_out, uint8_t* ok_out,
+                    spx_stream_t stream);
 
 /* Edge-erf centre refinement, replacing _refine_edge_erf_core / refine_centers_edge_erf_gpu
  * (subpx/_gpu/centers.py:782-929): left/right/top/bottom edge profiles (band means, binomial

@@ -12,6 +12,8 @@ Restates, step for step, in float64 NumPy:
   s = clip(exp(lsig), s_min, s_max); weights 1 within fit_rad plus 0.15 within fit_rad + 1;
   Marquardt damping damp * diag; LU solve; clamps; cond gate on the last damped normal
   matrix; rms gate);
+* ``refine_edge_moment``: :547-604 (the same profiles and seeds, centre = midpoints of the
+  grad-moment edges; ok needs the four seeds' weight sums > 1e-12, no ordering test);
 * ``refine_edge_erf``: :782-879 (two passes of left/right profiles along x from a 9-row
   band mean and top/bottom along y, two binomial passes, grad-moment seeds, erf fits,
   centre = midpoints; ok needs all eight sub-fits and xR > xL, yB > yT).
@@ -140,7 +142,12 @@ def edge_erf(prof, base_x, x_seed, *, fit_rad=5, gn_iters=5, init_sigma=1.25, si
     return np.where(ok, x0, np.nan), ok
 
 
-def refine_edge_erf(gray, stats6, **kw):
+def refine_edge_moment(gray, stats6, **kw):
+    """(centers (K, 2), ok (K,)) of the gradient-moment edges for rows [x, y, w, h, cx, cy]."""
+    return refine_edge_erf(gray, stats6, _erf=False, **kw)
+
+
+def refine_edge_erf(gray, stats6, _erf=True, **kw):
     """(centers (K, 2), ok (K,)) for rows [x, y, w, h, cx, cy]."""
     o = dict(DEFAULTS)
     o.update(kw)
@@ -190,12 +197,17 @@ def refine_edge_erf(gray, stats6, **kw):
         oks = np.ones(K, dtype=bool)
         for prof, base in (prof_x(xL0), prof_x(xR0), prof_y(yT0), prof_y(yB0)):
             seed, ok0 = edge_gradmoment(prof, base, grad_power=o["grad_power"], loc_rad=int(o["loc_rad"]))
-            e, ok1 = edge_erf(prof, base, seed, **erf_kw)
-            res.append(e)
-            oks &= ok0 & ok1
+            if _erf:
+                e, ok1 = edge_erf(prof, base, seed, **erf_kw)
+                res.append(e)
+                oks &= ok0 & ok1
+            else:
+                res.append(seed)
+                oks &= ok0
         xL, xR, yT, yB = res
-        with np.errstate(invalid="ignore"):
-            oks &= (xR > xL) & (yB > yT)
+        if _erf:
+            with np.errstate(invalid="ignore"):
+                oks &= (xR > xL) & (yB > yT)
         ok_all &= oks
         xc = 0.5 * (xL + xR)
         yc = 0.5 * (yT + yB)

@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import segment
+from . import pipeline, segment
 
 
 def dedupe_centers(centers_xy, eps: float = 1.5) -> np.ndarray:
@@ -61,7 +61,7 @@ def dedupe_centers(centers_xy, eps: float = 1.5) -> np.ndarray:
 
 def detect_centers_tiled_logquad(img, *, tile_h, overlap=128, otsu_downsample=4, thr_scale=0.8,
                                  area_min=1, area_max=50, pad=3, connectivity=8, invert=False,
-                                 dedupe_eps=1.5):
+                                 dedupe_eps=1.5, refine="logquad", logquad_variant="cpu"):
     g = np.asarray(img)
     H = g.shape[0]
     ds = max(1, int(otsu_downsample))
@@ -72,9 +72,9 @@ def detect_centers_tiled_logquad(img, *, tile_h, overlap=128, otsu_downsample=4,
     y0 = 0
     while y0 < H:
         y1 = min(H, y0 + tile_h)
-        c, _, _ = segment.detect_centers_logquad(g[y0:y1], threshold_value=t, invert=invert,
-                                                 area_min=area_min, area_max=area_max, pad=pad,
-                                                 connectivity=connectivity)
+        c, _ = pipeline.detect_centers(g[y0:y1], threshold_value=t, invert=invert, area_min=area_min,
+                                       area_max=area_max, pad=pad, connectivity=connectivity, refine=refine,
+                                       logquad_variant=logquad_variant)
         if c.shape[0]:
             c = c.copy()
             c[:, 1] += y0

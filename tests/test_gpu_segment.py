@@ -77,7 +77,7 @@ def test_logquad_and_pipeline_match_oracle_c0():
 
     arr = synthetic.make_config("c0", seed=4, H=768, W=640)
     res = detect_centers(arr.image, backend="gpu", threshold="otsu", refine="logquad", area_min=20,
-                         area_max=2000, pad=3)
+                         area_max=2000, pad=3, logquad_variant="cpu")
     ref, rows, t = segment.detect_centers_logquad(arr.image, area_min=20, area_max=2000, pad=3)
     assert res.meta["threshold_value"] == t
     assert res.centers_xy.shape == ref.shape
@@ -95,7 +95,7 @@ def test_logquad_border_and_small_rois():
     img = arr.image.copy()
     img[5:7, 100:102] = 4000
     res = detect_centers(img, backend="gpu", threshold="otsu", refine="logquad", area_min=1,
-                         area_max=5000, pad=3)
+                         area_max=5000, pad=3, logquad_variant="cpu")
     ref, _, _ = segment.detect_centers_logquad(img, area_min=1, area_max=5000, pad=3)
     assert res.centers_xy.shape == ref.shape
     np.testing.assert_allclose(res.centers_xy, ref, rtol=0, atol=1e-8)
@@ -114,7 +114,7 @@ def test_tiled_matches_oracle():
     arr = synthetic.make_config("c0", seed=8, H=1024, W=384)
     with pytest.warns(DeprecationWarning):
         res = detect_centers_tiled(arr.image, tile_h=300, overlap=64, backend="gpu", threshold="otsu",
-                                   refine="logquad", area_min=20, area_max=2000)
+                                   refine="logquad", area_min=20, area_max=2000, logquad_variant="cpu")
     ref = tiling.detect_centers_tiled_logquad(arr.image, tile_h=300, overlap=64, area_min=20, area_max=2000)
     assert res.centers_xy.shape == ref.shape
     np.testing.assert_allclose(res.centers_xy, ref, rtol=0, atol=1e-8)

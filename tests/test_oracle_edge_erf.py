@@ -69,3 +69,26 @@ def test_high_amplitude_edges_fail_the_cond_gate():
     arr = _page(amp=3000.0, bg=500.0, read=30.0)
     c, ok = edge_erf.refine_edge_erf(arr.image.astype(np.float32), _rows(arr.image))
     assert not ok.any()
+
+
+def test_edge_moment_of_symmetric_step_is_its_centre():
+    # the |diff|^8 moment of a step symmetric about sample 10.5 is exactly 10.5 (+ base)
+    k = np.arange(21.0)
+    prof = 40.0 + 90.0 * 0.5 * (1 + erf((k - 10.5) / (np.sqrt(2) * 1.3)))
+    x, ok = edge_erf.edge_gradmoment(prof[None], np.array([100.0]))
+    assert ok[0]
+    assert abs(x[0] - 110.5) < 1e-12
+
+
+@pytest.mark.parametrize("refine", ["moment", "erf"])
+def test_mirror_symmetric_disk_is_centred_exactly(refine):
+    # a disk symmetric about (40.5, 37.0): mirrored profiles give mirrored edges, so both
+    # refinements return the centre to rounding; a wrong half-sample or base offset fails
+    yy, xx = np.mgrid[0:80, 0:84].astype(np.float64)
+    rho = np.hypot(xx - 40.5, yy - 37.0)
+    img = (30.0 + 150.0 * 0.5 * (1 - erf((rho - 8.0) / (np.sqrt(2) * 1.2)))).astype(np.float32)
+    rows = np.array([[32, 29, 18, 17, 40.5, 37.0]], np.float32)
+    fn = edge_erf.refine_edge_moment if refine == "moment" else edge_erf.refine_edge_erf
+    c, ok = fn(img, rows)
+    assert ok[0]
+    np.testing.assert_allclose(c[0], [40.5, 37.0], atol=1e-9)
