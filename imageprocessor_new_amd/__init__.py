@@ -2,7 +2,8 @@
 
 Drop-in for the reference ``subpx`` centre path on ``backend="gpu"``: ``detect_centers``,
 ``detect_centers_tiled``, ``connected_components_stats``, ``dedupe_centers``,
-``refine_centers_edge_erf_gpu`` and the batched erf-disk fits, all backed by hand-written
+``refine_centers_edge_erf_gpu``, ``refine_centers_edge_moment_gpu``,
+``refine_centers_logquad_gpu_match_cpu`` and the batched erf-disk fits, all backed by hand-written
 gfx950 HIP kernels in libsubpx_hip.so (C-ABI: include/subpx_hip.h).
 """
 
@@ -13,7 +14,7 @@ __version__ = "0.1.0"
 
 def __getattr__(name):
     # torch-dependent modules load lazily so that the dataclasses import without a device.
-    if name in ("detect_centers", "dedupe_centers", "filter_centers"):
+    if name in ("detect_centers", "dedupe_centers", "filter_centers", "choose_refine_method_for_bbox"):
         from . import centers
 
         return getattr(centers, name)
@@ -29,8 +30,12 @@ def __getattr__(name):
         from . import fit
 
         return getattr(fit, name)
-    if name in ("refine_centers_edge_erf_gpu",):
+    if name in ("refine_centers_edge_erf_gpu", "refine_centers_edge_moment_gpu"):
         from . import edge_erf
 
         return getattr(edge_erf, name)
+    if name == "refine_centers_logquad_gpu_match_cpu":
+        from . import refine
+
+        return refine.refine_centers_logquad_gpu_match_cpu
     raise AttributeError(name)
