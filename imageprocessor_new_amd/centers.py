@@ -34,7 +34,8 @@ import torch
 
 from . import _lib
 from .backends import gpu_device, resolve_backend, to_device_u16
-from .components import otsu_threshold_device, threshold_components_device
+from .components import (binary_mask_device, otsu_threshold_device, threshold_components_device,
+                         triangle_threshold_device)
 from .types import CenterResult
 
 _ERF_METHODS = {"erf_disk": _lib.MODEL_ERF_DISK6, "erf_ellipse": _lib.MODEL_ERF_ELLIPSE7}
@@ -106,16 +107,24 @@ def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit
 
 
 def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, refine, connectivity,
-                        pre_threshold, fit_half, fit_options, small_feature_max=12.0, logquad_variant="gpu"):
+                        pre_threshold, fit_half, fit_options, small_feature_max=12.0, logquad_variant="gpu",
+                        morph_open=0, morph_close=0):
     """Segmentation -> CC -> area filter -> refinement on one device image (or strip view)."""
     if pre_threshold is None:
-        if threshold != "otsu":
-            raise NotImplementedError(f"threshold={threshold!r}: only 'otsu' is implemented on the GPU")
-        thresh = otsu_threshold_device(img_t)
+        if threshold == "otsu":
+            thresh = otsu_threshold_device(img_t)
+        elif threshold == "triangle":
+            thresh = triangle_threshold_device(img_t)
+        else:
+            raise ValueError(f"Unsupported threshold method: {threshold!r}. Supported: ['otsu', 'triangle']")
     else:
         thresh = int(np.floor(pre_threshold))
-    num, labels, stats, cents = threshold_components_device(img_t, thresh, invert=invert,
-                                                            connectivity=connectivity)
+    if morph_open > 0 or morph_close > 0:
+        mask = binary_mask_device(img_t, thresh, invert=invert, morph_open=morph_open, morph_close=morph_close)
+        num, labels, stats, cents = threshold_components_device(mask, 0, connectivity=connectivity)
+    else:
+        num, labels, stats, cents = threshold_components_device(img_t, thresh, invert=invert,
+                                                                connectivity=connectivity)
     stats_np = stats.cpu().numpy()
     cents_np = cents.cpu().numpy()
     area = stats_np[:, 4]
@@ -196,8 +205,6 @@ def detect_centers(
         raise ValueError("background must be None or 'model'")
     if background == "model":
         raise NotImplementedError("background='model' (deblending) is not implemented")
-    if morph_open or morph_close:
-        raise NotImplementedError("morphological open/close is not implemented")
     if refine not in _SUPPORTED:
         raise ValueError(f"Unsupported refine method: {refine}")
     if isinstance(image, torch.Tensor):
@@ -209,7 +216,8 @@ def detect_centers(
     fit_options = _lib.fit_options(max_iter=fit_max_iter, step_tol=fit_step_tol)
     common = dict(threshold=threshold, invert=invert, area_min=area_min, area_max=area_max, pad=pad,
                   refine=refine, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options,
-                  small_feature_max=small_feature_max, logquad_variant=logquad_variant)
+                  small_feature_max=small_feature_max, logquad_variant=logquad_variant,
+                  morph_open=int(morph_open), morph_close=int(morph_close))
     base_meta = {
         "invert": bool(invert), "area_min": int(area_min), "area_max": int(area_max),
         "morph_open": int(morph_open), "morph_close": int(morph_close), "pad": int(pad),

@@ -2,8 +2,9 @@
 
 Replaces ``connected_components_stats`` (reference subpx/components.py:15,
 subpx/_cpu/components.py:18-34, subpx/_gpu/components.py:64-134) and the Otsu threshold
-of ``_segment_binary`` (reference subpx/_cpu/centers.py:247-266) with spx_otsu_u16 /
-spx_threshold_cc (include/subpx_hip.h).
+and triangle thresholds and the morphology of ``_segment_binary`` (reference
+subpx/_cpu/centers.py:247-266) with spx_otsu_u16 / spx_triangle_u8 / spx_threshold_mask /
+spx_morph_cross / spx_threshold_cc (include/subpx_hip.h).
 """
 
 from __future__ import annotations
@@ -29,6 +30,37 @@ def otsu_threshold_device(img: torch.Tensor, step: int = 1) -> int:
     _lib.call("spx_otsu_u16", int(img.data_ptr()), H, W, img.stride(0), int(step),
               ctypes.byref(out), stream_handle())
     return int(out.value)
+
+
+def triangle_threshold_device(img: torch.Tensor, step: int = 1) -> int:
+    """Triangle threshold of a (H, W) uint16 device image holding 8-bit grey levels (OpenCV's
+    THRESH_TRIANGLE is 8-bit only; grey levels above 255 raise ValueError)."""
+    if img.dim() != 2 or img.dtype != torch.uint16:
+        raise ValueError("img must be a 2-D uint16 tensor")
+    out = ctypes.c_int32(0)
+    H, W = img.shape
+    _lib.call("spx_triangle_u8", int(img.data_ptr()), H, W, img.stride(0), int(step), ctypes.byref(out),
+              stream_handle())
+    return int(out.value)
+
+
+def binary_mask_device(img: torch.Tensor, thresh: int, *, invert: bool = False, morph_open: int = 0,
+                       morph_close: int = 0) -> torch.Tensor:
+    """Foreground mask (uint16 0 / 1) of ``_segment_binary`` (reference subpx/_cpu/centers.py:257-265):
+    binarise, then ``morph_open`` erosions followed by as many dilations, then ``morph_close``
+    dilations followed by as many erosions, all with the 3x3 cross."""
+    if img.dim() != 2 or img.dtype != torch.uint16 or img.stride(1) != 1:
+        raise ValueError("img must be a 2-D uint16 tensor with unit column stride")
+    H, W = img.shape
+    a = torch.empty((H, W), dtype=torch.uint16, device=img.device)
+    b = torch.empty_like(a)
+    _lib.call("spx_threshold_mask", int(img.data_ptr()), H, W, img.stride(0), int(thresh), int(bool(invert)),
+              int(a.data_ptr()), stream_handle())
+    ops = [0] * int(morph_open) + [1] * int(morph_open) + [1] * int(morph_close) + [0] * int(morph_close)
+    for op in ops:
+        _lib.call("spx_morph_cross", int(a.data_ptr()), H, W, W, op, int(b.data_ptr()), stream_handle())
+        a, b = b, a
+    return a
 
 
 def threshold_components_device(img: torch.Tensor, thresh: int, *, invert: bool = False,

@@ -482,6 +482,144 @@ int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_
     return SPX_OK;
 }
 
+namespace {
+
+// Triangle threshold of a 256-bin histogram (OpenCV's THRESH_TRIANGLE, 8-bit images only):
+// bounds = first / last non-empty bins widened by one, peak = first maximum; the longer
+// side of the peak is taken (the histogram mirrored when the right side is longer); the
+// threshold is the bin of largest distance to the line from (left bound, 0) to the peak,
+// minus one, mirrored back. out = -1 when a grey level above 255 is present.
+__global__ __launch_bounds__(1024) void triangle_kernel(const uint32_t* hist, int32_t* out) {
+    __shared__ int s_wide;
+    const int t = threadIdx.x;
+    if (t == 0) s_wide = 0;
+    __syncthreads();
+    int wide = 0;
+    for (int i = 256 + t; i < kHistBins; i += 1024) wide |= hist[i] != 0;
+    if (wide) atomicOr(&s_wide, 1);
+    __syncthreads();
+    if (t != 0) return;
+    if (s_wide) {
+        out[0] = -1;
+        return;
+    }
+    constexpr int N = 256;
+    int lo = 0, hi = 0, peak = 0;
+    uint32_t hmax = 0;
+    for (int i = 0; i < N; ++i)
+        if (hist[i] > 0) {
+            lo = i;
+            break;
+        }
+    if (lo > 0) --lo;
+    for (int i = N - 1; i > 0; --i)
+        if (hist[i] > 0) {
+            hi = i;
+            break;
+        }
+    if (hi < N - 1) ++hi;
+    for (int i = 0; i < N; ++i)
+        if (hist[i] > hmax) {
+            hmax = hist[i];
+            peak = i;
+        }
+    const bool flip = peak - lo < hi - peak;
+    if (flip) {
+        lo = N - 1 - hi;
+        peak = N - 1 - peak;
+    }
+    // distance to the line, up to a positive factor and a constant: hmax * i + (lo - peak) * h(i)
+    const double a = (double)hmax, b = (double)(lo - peak);
+    double best = 0.0;
+    int th = lo;
+    for (int i = lo + 1; i <= peak; ++i) {
+        const double hv = (double)hist[flip ? N - 1 - i : i];
+        const double d = a * (double)i + b * hv;
+        if (d > best) {
+            best = d;
+            th = i;
+        }
+    }
+    --th;
+    out[0] = flip ? N - 1 - th : th;
+}
+
+// mask = (invert ? v <= t : v > t) as 0 / 1.
+__global__ void threshold_mask_kernel(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
+                                      int32_t invert, uint16_t* out) {
+    const int64_t n = H * W;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t y = i / W, x = i - (i / W) * W;
+        const int32_t v = img[y * pitch + x];
+        out[i] = (invert ? v <= thresh : v > thresh) ? 1 : 0;
+    }
+}
+
+// One erosion (op 0) or dilation (op 1) of a 0 / non-zero mask by the 3x3 cross (OpenCV's
+// 3x3 MORPH_ELLIPSE). Pixels outside the image do not take part (OpenCV's default border:
+// +inf for erosion, -inf for dilation).
+__global__ void morph_cross_kernel(const uint16_t* in, int64_t H, int64_t W, int64_t pitch, int32_t op,
+                                   uint16_t* out) {
+    const int64_t n = H * W;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t y = i / W, x = i - (i / W) * W;
+        const bool c = in[y * pitch + x] != 0;
+        const bool l = x > 0 ? in[y * pitch + x - 1] != 0 : !op;
+        const bool r = x + 1 < W ? in[y * pitch + x + 1] != 0 : !op;
+        const bool u = y > 0 ? in[(y - 1) * pitch + x] != 0 : !op;
+        const bool d = y + 1 < H ? in[(y + 1) * pitch + x] != 0 : !op;
+        const bool v = op ? (c || l || r || u || d) : (c && l && r && u && d);
+        out[i] = v ? 1 : 0;
+    }
+}
+
+}  // namespace
+
+int triangle_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step, int32_t* thresh_out,
+                    hipStream_t stream) {
+    const int64_t Hs = (H + step - 1) / step;
+    const int64_t Ws = (W + step - 1) / step;
+    void* ws = nullptr;
+    int rc = workspace(sizeof(uint32_t) * kHistBins + 256, &ws);
+    if (rc != SPX_OK) return rc;
+    uint32_t* hist = static_cast<uint32_t*>(ws);
+    int32_t* dev_out = reinterpret_cast<int32_t*>(hist + kHistBins);
+    fill_u32<<<64, 256, 0, stream>>>(hist, kHistBins, 0u);
+    SPX_LAUNCHED();
+    const int64_t n = Hs * Ws;
+    const int64_t per_block = (int64_t)kHistThreads * kHistPixelsPerThread;
+    const int64_t blocks = (n + per_block - 1) / per_block;
+    hist_u16_kernel<<<(unsigned)blocks, kHistThreads, 0, stream>>>(img, Hs, Ws, pitch, step, hist);
+    SPX_LAUNCHED();
+    triangle_kernel<<<1, 1024, 0, stream>>>(hist, dev_out);
+    SPX_LAUNCHED();
+    SPX_HIP(hipMemcpyAsync(thresh_out, dev_out, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    SPX_HIP(hipStreamSynchronize(stream));
+    if (*thresh_out < 0) {
+        set_error("spx_triangle_u8: grey levels above 255; the triangle method is defined for 8-bit images");
+        return SPX_E_ARG;
+    }
+    return SPX_OK;
+}
+
+int threshold_mask_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh, int32_t invert,
+                          uint16_t* out, hipStream_t stream) {
+    const int64_t n = H * W;
+    const unsigned blocks = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    threshold_mask_kernel<<<blocks, 256, 0, stream>>>(img, H, W, pitch, thresh, invert, out);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int morph_cross_launch(const uint16_t* in, int64_t H, int64_t W, int64_t pitch, int32_t op, uint16_t* out,
+                       hipStream_t stream) {
+    const int64_t n = H * W;
+    const unsigned blocks = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    morph_cross_kernel<<<blocks, 256, 0, stream>>>(in, H, W, pitch, op, out);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
 int threshold_cc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
                         int32_t invert, int32_t connectivity, int32_t* labels, int32_t* stats,
                         double* cents, int32_t cap, int32_t* n_out, hipStream_t stream) {

@@ -76,6 +76,12 @@ int cnn_train_step_launch(const uint16_t* x, const float* t, const float* w, int
                           uint16_t* theta_bf, float* adam_m, float* adam_v, int step, float lr, float* loss_out,
                           float* grad_out, hipStream_t stream);
 int cnn_to_bf16_launch(const float* theta, int64_t n, uint16_t* theta_bf, hipStream_t stream);
+int triangle_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step, int32_t* thresh_out,
+                    hipStream_t stream);
+int threshold_mask_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh, int32_t invert,
+                          uint16_t* out, hipStream_t stream);
+int morph_cross_launch(const uint16_t* in, int64_t H, int64_t W, int64_t pitch, int32_t op, uint16_t* out,
+                       hipStream_t stream);
 int logquad_window_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
                           const float* rows, const int32_t* lab_ids, int32_t K, int32_t pad,
                           const SpxLogquadWindowOptions& opt, double* centers, uint8_t* ok, hipStream_t stream);
@@ -161,6 +167,31 @@ int spx_edge_erf(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t p
     SPX_REQUIRE(opt->iters >= 0 && opt->erf_gn_iters >= 0 && opt->smooth_passes >= 0 && opt->loc_rad >= 1,
                 "spx_edge_erf: bad iteration or window options");
     return spx::edge_erf_launch(img, dtype, H, W, pitch, rows, K, *opt, centers_out, ok_out, spx::as_stream(stream));
+}
+
+int spx_triangle_u8(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step, int32_t* thresh_out,
+                    spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(img != nullptr && thresh_out != nullptr, "spx_triangle_u8: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W && step >= 1, "spx_triangle_u8: bad image shape, pitch or step");
+    return spx::triangle_launch(img, H, W, pitch, step, thresh_out, spx::as_stream(stream));
+}
+
+int spx_threshold_mask(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh, int32_t invert,
+                       uint16_t* mask_out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(img != nullptr && mask_out != nullptr, "spx_threshold_mask: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_threshold_mask: bad image shape or pitch");
+    return spx::threshold_mask_launch(img, H, W, pitch, thresh, invert, mask_out, spx::as_stream(stream));
+}
+
+int spx_morph_cross(const uint16_t* in, int64_t H, int64_t W, int64_t pitch, int32_t op, uint16_t* out,
+                    spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(in != nullptr && out != nullptr && in != out, "spx_morph_cross: null or aliased buffers");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_morph_cross: bad image shape or pitch");
+    SPX_REQUIRE(op == 0 || op == 1, "spx_morph_cross: op must be 0 (erode) or 1 (dilate)");
+    return spx::morph_cross_launch(in, H, W, pitch, op, out, spx::as_stream(stream));
 }
 
 int spx_edge_moment(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t pitch, const float* rows,

@@ -140,6 +140,23 @@ int spx_otsu_u16(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32
  * [x, y, w, h, area], row 0 = background), centroids ((cap,2) f64): device.
  * n_out (host) receives the label count including the background; SPX_E_CAPACITY when it
  * exceeds cap (n_out still holds the true count). */
+/* Triangle threshold, replacing cv2.threshold(..., THRESH_TRIANGLE) at
+ * subpx/_cpu/centers.py:258 (OpenCV defines it for 8-bit images only): img values must be
+ * 0..255 (SPX_E_ARG otherwise). thresh_out: host. */
+int spx_triangle_u8(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step, int32_t* thresh_out,
+                    spx_stream_t stream);
+
+/* Binary mask (invert ? v <= thresh : v > thresh) as uint16 0 / 1, contiguous (H, W); the
+ * binarisation of cv2.threshold(..., THRESH_BINARY[_INV]) (subpx/_cpu/centers.py:257-258). */
+int spx_threshold_mask(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh, int32_t invert,
+                       uint16_t* mask_out, spx_stream_t stream);
+
+/* One erosion (op 0) or dilation (op 1) of a 0 / non-zero uint16 mask by the 3x3 cross,
+ * pixels outside the image ignored: one iteration of cv2.morphologyEx with the 3x3
+ * MORPH_ELLIPSE element (subpx/_cpu/centers.py:260-265). out: contiguous (H, W), not in. */
+int spx_morph_cross(const uint16_t* in, int64_t H, int64_t W, int64_t pitch, int32_t op, uint16_t* out,
+                    spx_stream_t stream);
+
 int spx_threshold_cc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
                      int32_t invert, int32_t connectivity, int32_t* labels, int32_t* stats,
                      double* centroids, int32_t cap, int32_t* n_out, spx_stream_t stream);

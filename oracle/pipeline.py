@@ -21,9 +21,14 @@ ALIASES = {"logquadratic": "logquad", "edge_moment": "edge_gradmoment", "edges_c
 ORDER = ("logquad", "edge_gradmoment", "edge_erf", "weighted", "none")
 
 
-def component_rows(img, *, threshold_value=None, invert=False, area_min=1, area_max=50, connectivity=8):
-    t = segment.otsu_u16(img) if threshold_value is None else threshold_value
-    num, labels, stats, cents = segment.connected_components_fast(segment.threshold(img, t, invert), connectivity)
+def component_rows(img, *, threshold_value=None, invert=False, area_min=1, area_max=50, connectivity=8,
+                   threshold="otsu", morph_open=0, morph_close=0):
+    if threshold_value is not None:
+        t = threshold_value
+    else:
+        t = segment.otsu_u16(img) if threshold == "otsu" else segment.triangle_u8(img)
+    mask = segment.binary_mask(img, t, invert, morph_open, morph_close)
+    num, labels, stats, cents = segment.connected_components_fast(mask, connectivity)
     keep = np.array([k for k in range(1, num) if area_min <= stats[k, 4] <= area_max], dtype=np.int64)
     rows5 = np.concatenate([stats[keep, :4], keep[:, None]], axis=1).reshape(-1, 5).astype(np.int64)
     rows6 = np.concatenate([stats[keep, :4], cents[keep]], axis=1).reshape(-1, 6).astype(np.float32)
@@ -46,10 +51,12 @@ def refine_method(img, labels, rows5, rows6, method, *, pad=3, logquad_variant="
 
 
 def detect_centers(img, *, threshold_value=None, invert=False, area_min=1, area_max=50, pad=3, connectivity=8,
-                   refine="logquad", small_feature_max=12.0, logquad_variant="gpu"):
+                   refine="logquad", small_feature_max=12.0, logquad_variant="gpu", threshold="otsu",
+                   morph_open=0, morph_close=0):
     """(centers (N, 2), threshold) of the GPU detect path on one image / strip."""
     t, labels, rows5, rows6 = component_rows(img, threshold_value=threshold_value, invert=invert,
-                                             area_min=area_min, area_max=area_max, connectivity=connectivity)
+                                             area_min=area_min, area_max=area_max, connectivity=connectivity,
+                                             threshold=threshold, morph_open=morph_open, morph_close=morph_close)
     if refine == "auto":
         size = np.maximum(rows5[:, 2], rows5[:, 3]).astype(np.float64)
         methods = np.where(size <= float(small_feature_max), "logquad", "edge_gradmoment")

@@ -97,6 +97,57 @@ def otsu_brute_force(img: np.ndarray) -> int:
     return best_t
 
 
+def triangle_u8(img: np.ndarray) -> int:
+    """OpenCV's THRESH_TRIANGLE on 8-bit grey levels (reference call subpx/_cpu/centers.py:258;
+    OpenCV defines it for CV_8UC1 only). Bounds = first / last non-empty bins widened by one;
+    peak = first maximum; mirrored when the right side of the peak is longer; threshold = the
+    bin farthest from the line (left bound, 0)-(peak, h_max) (first maximum of the
+    unnormalised distance), minus one, mirrored back."""
+    g = np.asarray(img)
+    if g.size and int(g.max()) > 255:
+        raise ValueError("triangle threshold needs 8-bit grey levels")
+    h = np.bincount(g.ravel().astype(np.int64), minlength=256)[:256].astype(np.int64)
+    nz = np.nonzero(h)[0]
+    lo = int(nz[0]) if nz.size else 0
+    hi = int(nz[-1]) if nz.size and nz[-1] > 0 else 0
+    lo = lo - 1 if lo > 0 else lo
+    hi = hi + 1 if hi < 255 else hi
+    peak = int(np.argmax(h))
+    flip = peak - lo < hi - peak
+    if flip:
+        h = h[::-1]
+        lo, peak = 255 - hi, 255 - peak
+    a, b = float(h[peak]), float(lo - peak)
+    best, th = 0.0, lo
+    for i in range(lo + 1, peak + 1):
+        d = a * i + b * float(h[i])
+        if d > best:
+            best, th = d, i
+    th -= 1
+    return 255 - th if flip else th
+
+
+def morph_cross(mask: np.ndarray, op: str) -> np.ndarray:
+    """One erosion / dilation by the 3x3 cross (cv2 3x3 MORPH_ELLIPSE); pixels outside the
+    image do not take part (OpenCV's default morphology border)."""
+    m = np.asarray(mask, dtype=bool)
+    fill = op == "erode"
+    p = np.pad(m, 1, constant_values=fill)
+    nb = [p[1:-1, 1:-1], p[:-2, 1:-1], p[2:, 1:-1], p[1:-1, :-2], p[1:-1, 2:]]
+    if op == "erode":
+        return np.logical_and.reduce(nb)
+    return np.logical_or.reduce(nb)
+
+
+def binary_mask(img: np.ndarray, t: float, invert: bool = False, morph_open: int = 0, morph_close: int = 0):
+    """``_segment_binary``'s mask (reference subpx/_cpu/centers.py:257-265): threshold, then
+    open (erode^n, dilate^n) and close (dilate^n, erode^n) with the 3x3 cross."""
+    m = threshold(img, t, invert)
+    for op in ["erode"] * morph_open + ["dilate"] * morph_open + ["dilate"] * morph_close + ["erode"] * morph_close:
+        m = morph_cross(m, op)
+    return m
+
+
 def threshold(img: np.ndarray, t: float, invert: bool = False) -> np.ndarray:
     g = np.asarray(img)
     return (g <= t) if invert else (g > t)

@@ -136,3 +136,61 @@ def test_erf_disk_refine_through_detect_centers():
     d = np.hypot(*(res.centers_xy[:, None, :] - arr.centers_xy[None, :, :]).transpose(2, 0, 1)).min(axis=1)
     assert res.centers_xy.shape[0] >= 0.9 * arr.centers_xy.shape[0]
     assert np.median(d) < 0.02
+
+
+def _page8(seed, shape=(300, 340)):
+    rng = np.random.default_rng(seed)
+    lat = synthetic.square_lattice(shape[0], shape[1], 24.0)
+    c = lat + rng.normal(0, 0.3, lat.shape)
+    arr = synthetic.render_hole_array(shape, c, radius=rng.uniform(3, 7, c.shape[0]), sigma=1.2, amplitude=150.0,
+                                      background=30.0 + 40.0 * (seed % 2), read_sigma=3.0, seed=seed)
+    arr.image = np.minimum(arr.image, 255).astype(np.uint16)  # 8-bit grey levels
+    return arr
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_triangle_matches_oracle(seed):
+    from imageprocessor_new_amd.components import triangle_threshold_device
+
+    arr = _page8(seed)
+    img = arr.image if seed < 3 else (255 - np.minimum(arr.image, 255)).astype(np.uint16)
+    assert img.max() <= 255
+    with _dev() as dev:
+        t = triangle_threshold_device(torch.from_numpy(np.ascontiguousarray(img)).to(dev))
+    assert t == segment.triangle_u8(img)
+
+
+def test_triangle_rejects_16_bit_images():
+    from imageprocessor_new_amd.components import triangle_threshold_device
+
+    arr = synthetic.make_config("c0", seed=1, H=128, W=128)
+    with _dev() as dev, pytest.raises(ValueError, match="8-bit"):
+        triangle_threshold_device(torch.from_numpy(arr.image).to(dev))
+
+
+@pytest.mark.parametrize("mo,mc,invert", [(1, 0, False), (0, 2, False), (2, 1, True)])
+def test_morphology_matches_oracle(mo, mc, invert):
+    from imageprocessor_new_amd.components import binary_mask_device
+
+    arr = _page8(5)
+    t = segment.otsu_u16(arr.image)
+    with _dev() as dev:
+        m = binary_mask_device(torch.from_numpy(arr.image).to(dev), t, invert=invert, morph_open=mo,
+                               morph_close=mc).cpu().numpy()
+    np.testing.assert_array_equal(m.astype(bool), segment.binary_mask(arr.image, t, invert, mo, mc))
+
+
+def test_detect_centers_triangle_and_morphology_match_oracle():
+    from imageprocessor_new_amd.centers import detect_centers
+    from oracle import pipeline
+
+    arr = _page8(6)
+    res = detect_centers(arr.image, backend="gpu", threshold="triangle", morph_open=1, morph_close=1,
+                         refine="weighted", area_min=5, area_max=4000)
+    want, t = pipeline.detect_centers(arr.image, threshold="triangle", morph_open=1, morph_close=1,
+                                      refine="weighted", area_min=5, area_max=4000)
+    assert res.meta["threshold_value"] == t
+    assert res.centers_xy.shape == want.shape and want.shape[0] > 50
+    np.testing.assert_allclose(res.centers_xy, want, atol=1e-9, rtol=0)
+    with pytest.raises(ValueError, match="Unsupported threshold"):
+        detect_centers(arr.image, backend="gpu", threshold="mean")

@@ -132,3 +132,53 @@ def test_logquad_translation_equivariant_and_gates():
     tiny = np.zeros((10, 10), bool)
     tiny[4, 4:8] = True
     assert np.isnan(segment.refine_logquadratic(g[:10, :10], tiny)[0])  # < 6 pixels
+
+
+def test_triangle_is_the_farthest_bin_from_the_peak_line():
+    # geometric restatement: perpendicular distance from (i, h_i) to the line through
+    # (lo, 0) and (peak, h_peak), maximised over lo < i <= peak; OpenCV then steps one bin
+    # towards the tail. Random long-tailed histograms on both sides of the peak.
+    rng = np.random.default_rng(3)
+    for trial in range(40):
+        n = 20000
+        vals = np.concatenate([rng.normal(200 if trial % 2 else 40, 8, n), rng.uniform(0, 255, n // 4)])
+        img = np.clip(np.rint(vals), 0, 255).astype(np.uint8).reshape(-1, 50)
+        t = segment.triangle_u8(img)
+        h = np.bincount(img.ravel(), minlength=256).astype(np.float64)
+        nz = np.nonzero(h)[0]
+        lo, hi, peak = max(nz[0] - 1, 0), min(nz[-1] + 1, 255), int(np.argmax(h))
+        flip = peak - lo < hi - peak
+        if flip:
+            h, lo, peak = h[::-1], 255 - hi, 255 - peak
+        i = np.arange(lo + 1, peak + 1)
+        p0, p1 = np.array([lo, 0.0]), np.array([peak, h[peak]])
+        dvec = (p1 - p0) / np.hypot(*(p1 - p0))
+        rel = np.stack([i - p0[0], h[i] - p0[1]], axis=1)
+        dist = rel[:, 0] * dvec[1] - rel[:, 1] * dvec[0]
+        want = int(i[np.argmax(dist)]) - 1
+        assert t == (255 - want if flip else want)
+
+
+def test_triangle_rejects_16_bit_levels():
+    with pytest.raises(ValueError):
+        segment.triangle_u8(np.array([[0, 300]], np.uint16))
+
+
+def test_morphology_matches_scipy():
+    from scipy import ndimage
+
+    cross = ndimage.generate_binary_structure(2, 1)
+    rng = np.random.default_rng(4)
+    m = rng.random((37, 41)) < 0.55
+    np.testing.assert_array_equal(segment.morph_cross(m, "erode"),
+                                  ndimage.binary_erosion(m, cross, border_value=1))
+    np.testing.assert_array_equal(segment.morph_cross(m, "dilate"),
+                                  ndimage.binary_dilation(m, cross, border_value=0))
+    img = (rng.random((37, 41)) * 100).astype(np.uint16)
+    got = segment.binary_mask(img, 40, morph_open=2, morph_close=1)
+    ref = img > 40
+    ref = ndimage.binary_erosion(ref, cross, iterations=2, border_value=1)
+    ref = ndimage.binary_dilation(ref, cross, iterations=2, border_value=0)
+    ref = ndimage.binary_dilation(ref, cross, iterations=1, border_value=0)
+    ref = ndimage.binary_erosion(ref, cross, iterations=1, border_value=1)
+    np.testing.assert_array_equal(got, ref)
