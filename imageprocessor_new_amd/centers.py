@@ -59,7 +59,8 @@ def _row_methods(refine, rows_np, small_feature_max):
     return np.full(K, _ALIASES.get(refine, refine), dtype=object)
 
 
-def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit_options, logquad_variant):
+def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit_options, logquad_variant,
+                 fit_cells=True):
     """Refine components of one method on one image / strip. rows_np (K, 5) [x, y, w, h, label],
     cents_np (K, 2) CC centroids. Returns (centers (K, 2), ok (K,), extra)."""
     from . import edge_erf, refine as rf
@@ -86,7 +87,7 @@ def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit
         c, ok = edge_erf.refine_edge_erf_device(img_t, torch.from_numpy(stats6).to(dev))
         return c, ok, extra
     if method in _ERF_METHODS:
-        from .fit import fit_erf_disks_device
+        from .fit import cell_radius, fit_erf_disks_device, seed_neighbours_device
 
         seeds = torch.from_numpy(cents_np.astype(np.float32)).to(dev)
         if fit_half is None:
@@ -95,7 +96,10 @@ def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit
             half_np = np.broadcast_to(np.asarray(fit_half, dtype=np.int64), (K,))
         half_t = torch.from_numpy(half_np.astype(np.int16)).to(dev)
         model_t = torch.full((K,), _ERF_METHODS[method], dtype=torch.uint8, device=dev)
-        res = fit_erf_disks_device(img_t, seeds, half_t, model_t, options=fit_options)
+        nbr = None
+        if fit_cells:
+            nbr = seed_neighbours_device(seeds, img_t.shape, cell_radius(int(half_np.max())))
+        res = fit_erf_disks_device(img_t, seeds, half_t, model_t, options=fit_options, neighbours=nbr)
         torch.cuda.current_stream().synchronize()
         params = res["params"].cpu().numpy()
         status = res["status"].cpu().numpy()
@@ -108,7 +112,7 @@ def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit
 
 def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, refine, connectivity,
                         pre_threshold, fit_half, fit_options, small_feature_max=12.0, logquad_variant="gpu",
-                        morph_open=0, morph_close=0):
+                        morph_open=0, morph_close=0, fit_cells=True):
     """Segmentation -> CC -> area filter -> refinement on one device image (or strip view)."""
     if pre_threshold is None:
         if threshold == "otsu":
@@ -141,7 +145,7 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
             continue
         centers, ok, extra = _refine_rows(img_t, labels, rows[idx], cents_np[keep][idx], method, pad=pad,
                                           fit_half=fit_half, fit_options=fit_options,
-                                          logquad_variant=logquad_variant)
+                                          logquad_variant=logquad_variant, fit_cells=fit_cells)
         good = ok & np.isfinite(centers).all(axis=1)
         out.append(centers[good])
         n_refined += int(good.sum())
@@ -184,6 +188,7 @@ def detect_centers(
     fit_half=None,
     fit_max_iter: int = 30,
     fit_step_tol: float = 1e-6,
+    fit_cells: bool = True,
     logquad_variant: str = "gpu",
 ) -> CenterResult:
     """Detect and refine centres (reference subpx/centers.py:130-220).
@@ -192,8 +197,9 @@ def detect_centers(
     ``components_backend`` and ``upsample_factor`` are accepted for signature compatibility
     (the kernels need no batching and always refine in FP64 / FP32-Jacobian + FP64 solve;
     components always come from the GPU kernel, numbered as OpenCV numbers them).
-    ``small_feature_max`` splits ``refine="auto"``. ``fit_half``, ``fit_max_iter`` and
-    ``fit_step_tol`` configure the erf fits; ``logquad_variant`` picks the GPU backend's
+    ``small_feature_max`` splits ``refine="auto"``. ``fit_half``, ``fit_max_iter``,
+    ``fit_step_tol`` and ``fit_cells`` (Voronoi cell mask against neighbouring components,
+    see fit.py) configure the erf fits; ``logquad_variant`` picks the GPU backend's
     ("gpu", default) or the CPU backend's ("cpu") log-quadratic algorithm.
     """
     b = resolve_backend(backend)
@@ -217,7 +223,7 @@ def detect_centers(
     common = dict(threshold=threshold, invert=invert, area_min=area_min, area_max=area_max, pad=pad,
                   refine=refine, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options,
                   small_feature_max=small_feature_max, logquad_variant=logquad_variant,
-                  morph_open=int(morph_open), morph_close=int(morph_close))
+                  morph_open=int(morph_open), morph_close=int(morph_close), fit_cells=bool(fit_cells))
     base_meta = {
         "invert": bool(invert), "area_min": int(area_min), "area_max": int(area_max),
         "morph_open": int(morph_open), "morph_close": int(morph_close), "pad": int(pad),

@@ -188,8 +188,8 @@ int resident_blocks(int dev, int b, int threads) {
 
 int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                      const int16_t* half, const uint8_t* model, const float* init,
-                     const SpxFitOptions& opt, int32_t K, float* params, float* rms, int16_t* iters,
-                     uint8_t* status, hipStream_t stream) {
+                     const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
+                     float* rms, int16_t* iters, uint8_t* status, hipStream_t stream) {
     if (K == 0) return SPX_OK;
     // Workspace: stats u64[4] | counts[10] | queues[10] | order[10][K]
     void* ws = nullptr;
@@ -214,7 +214,7 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
     const int waves_per_block = opt.tile_waves > 0 ? (opt.tile_waves < 4 ? opt.tile_waves : 4) : 4;
     const int threads = waves_per_block * fitdev::kWave;
     FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, nullptr, nullptr, nullptr,
-              stats, params, rms, iters, status};
+              stats, params, rms, iters, status, nbr, nb, K};
     for (int b = 0; b < kKernBuckets; ++b) {
         a.order = order + (int64_t)b * K;
         a.count = counts + b;

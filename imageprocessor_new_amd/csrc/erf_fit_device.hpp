@@ -263,12 +263,48 @@ struct FitArgs {
     float* rms;
     int16_t* iters;
     uint8_t* status;
+    const int32_t* nbr;  // (K, nb) neighbour feature ids, -1 = none; null = no cell mask
+    int32_t nb;
+    int32_t nfeat;       // K: ids outside [0, K) are ignored
 };
 
-// Fit of one feature by the whole wave. The patch registers are filled by the caller.
+// Voronoi cell of a feature against its listed neighbour seeds. A pixel belongs to the cell
+// when it is strictly nearer to the own seed than to every neighbour, ties going to the
+// lower feature index (reference subpx/_cpu/cells.py:8-33). Squared distances are FP64
+// without contraction, so the decision is the same IEEE computation as the oracle's.
+// Neighbour seeds are re-read with plain loads (no wave operations: the test runs under
+// per-pixel conditions) instead of being held in registers, so the cell costs the LM loop
+// no registers.
+constexpr int kMaxNb = 8;
+
+SPX_DEV_INLINE bool nearer_than(double px, double py, double sx, double sy, int self, double nx, double ny,
+                                int id) {
+#pragma clang fp contract(off)
+    const double dx = px - sx, dy = py - sy;
+    const double ex = px - nx, ey = py - ny;
+    const double d0 = dx * dx + dy * dy;
+    const double dj = ex * ex + ey * ey;
+    return dj > d0 || (dj == d0 && self < id);
+}
+
+SPX_DEV_INLINE bool in_cell(const FitArgs& a, int f, double px, double py) {
+    const double sx = (double)a.seeds[2 * (int64_t)f], sy = (double)a.seeds[2 * (int64_t)f + 1];
+    bool in = true;
+    for (int j = 0; j < a.nb; ++j) {
+        const int id = a.nbr[(int64_t)f * a.nb + j];
+        if (id >= 0 && id < a.nfeat) {
+            const double nx = (double)a.seeds[2 * (int64_t)id], ny = (double)a.seeds[2 * (int64_t)id + 1];
+            in = in && nearer_than(px, py, sx, sy, f, nx, ny, id);
+        }
+    }
+    return in;
+}
+
+// Fit of one feature by the whole wave. The patch registers are filled by the caller;
+// `masked`: the patch is restricted to the feature's Voronoi cell.
 template <int NPL, int P>
 SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>& pt,
-                                float cx, float cy) {
+                                float cx, float cy, bool masked) {
     using L = Layout<P>;
     const int lane = spx_lane();
     const float npix = pt.npix;
@@ -293,10 +329,13 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
             cur.p[P - 1] = spx_uniform(in[5]);
             if constexpr (P == 7) cur.p[5] = spx_uniform(in[6]);
         } else {
-            // Moment start: background = mean of the patch's outer ring; in-set = pixels
-            // above half the peak; centre = weighted centroid of the in-set; radius from
-            // its area; amplitude = its mean; edge sigma = init_sigma; q = 1.
+            // Moment start: background = mean of the patch's outer ring (with a cell mask:
+            // also the cell's boundary pixels, those with a 4-neighbour outside the cell,
+            // as the reference's _extract_voronoi_rois takes, subpx/_cpu/cells.py:72-80);
+            // in-set = pixels above half the peak; centre = weighted centroid of the in-set;
+            // radius from its area; amplitude = its mean; edge sigma = init_sigma; q = 1.
             const int side = 2 * hs + 1;
+            const double ox = (double)cx - (double)hs, oy = (double)cy - (double)hs;
             float bsum = 0.0f, bcnt = 0.0f, vmax = -3.0e38f;
 #pragma unroll
             for (int n = 0; n < NPL; ++n) {
@@ -305,7 +344,13 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
                     const uint32_t w = pt.px[n];
                     const int col = (int)(w & 0xffu), row = (int)((w >> 8) & 0xffu);
                     const float val = (float)(w >> 16);
-                    if (row == 0 || row == side - 1 || col == 0 || col == side - 1) {
+                    bool ring = row == 0 || row == side - 1 || col == 0 || col == side - 1;
+                    if (!ring && masked) {
+                        const double px = ox + (double)col, py = oy + (double)row;
+                        ring = !in_cell(a, f, px - 1.0, py) || !in_cell(a, f, px + 1.0, py) ||
+                               !in_cell(a, f, px, py - 1.0) || !in_cell(a, f, px, py + 1.0);
+                    }
+                    if (ring) {
                         bsum += val;
                         bcnt += 1.0f;
                     }
@@ -458,11 +503,18 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
         const int64_t oy = (int64_t)cy - hs;
         const bool clipped = ox < 0 || oy < 0 || ox + side > a.W || oy + side > a.H;
 
+        bool masked = false;
+        if (a.nbr != nullptr)
+            for (int j = 0; j < a.nb; ++j) {
+                const int id = spx_uniform_i(a.nbr[(int64_t)f * a.nb + j]);
+                masked = masked || (id >= 0 && id < a.nfeat);
+            }
+
         // Gather: lane-strided over the flattened patch, so each load instruction reads a
         // few contiguous row segments of the image.
         Patch<NPL> pt;
         pt.nslots = (npx + kWave - 1) / kWave;
-        pt.nfull = clipped ? 0 : npx / kWave;
+        pt.nfull = (clipped || masked) ? 0 : npx / kWave;
         pt.valid = 0ull;
         float cnt = 0.0f;
 #pragma unroll
@@ -472,14 +524,15 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
             const int col = pix - row * side;
             const int64_t gx = ox + col;
             const int64_t gy = oy + row;
-            const bool ok = pix < npx && gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
+            bool ok = pix < npx && gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
+            if (masked) ok = ok && in_cell(a, f, (double)gx, (double)gy);
             const uint32_t v = ok ? (uint32_t)a.img[gy * a.pitch + gx] : 0u;
             pt.px[n] = pack_px(ok ? col : 0, ok ? row : 0, v);
             pt.valid |= ok ? (1ull << n) : 0ull;
             cnt += ok ? 1.0f : 0.0f;
         }
         pt.npix = spx_wave_sum(cnt);
-        fit_feature<NPL, P>(a, f, hs, pt, cx, cy);
+        fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked);
     }
 }
 

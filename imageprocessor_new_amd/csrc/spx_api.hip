@@ -51,8 +51,8 @@ int workspace(size_t bytes, void** out) {
 
 int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                      const int16_t* half, const uint8_t* model, const float* init,
-                     const SpxFitOptions& opt, int32_t K, float* params, float* rms, int16_t* iters,
-                     uint8_t* status, hipStream_t stream);
+                     const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
+                     float* rms, int16_t* iters, uint8_t* status, hipStream_t stream);
 
 int fit_stats_copy(uint64_t* out, hipStream_t stream);
 int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
@@ -76,6 +76,8 @@ int cnn_train_step_launch(const uint16_t* x, const float* t, const float* w, int
                           uint16_t* theta_bf, float* adam_m, float* adam_v, int step, float lr, float* loss_out,
                           float* grad_out, hipStream_t stream);
 int cnn_to_bf16_launch(const float* theta, int64_t n, uint16_t* theta_bf, hipStream_t stream);
+int seed_neighbours_launch(const float* seeds, int32_t K, int64_t H, int64_t W, float radius, int32_t nb,
+                           int32_t* out, hipStream_t stream);
 int triangle_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step, int32_t* thresh_out,
                     hipStream_t stream);
 int threshold_mask_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh, int32_t invert,
@@ -167,6 +169,17 @@ int spx_edge_erf(const void* img, int32_t dtype, int64_t H, int64_t W, int64_t p
     SPX_REQUIRE(opt->iters >= 0 && opt->erf_gn_iters >= 0 && opt->smooth_passes >= 0 && opt->loc_rad >= 1,
                 "spx_edge_erf: bad iteration or window options");
     return spx::edge_erf_launch(img, dtype, H, W, pitch, rows, K, *opt, centers_out, ok_out, spx::as_stream(stream));
+}
+
+int spx_seed_neighbours(const float* seeds, int32_t K, int64_t H, int64_t W, float radius, int32_t nb,
+                        int32_t* out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(K >= 0, "spx_seed_neighbours: K < 0");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(seeds != nullptr && out != nullptr, "spx_seed_neighbours: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0, "spx_seed_neighbours: bad image shape");
+    SPX_REQUIRE(radius > 0.0f && nb >= 1 && nb <= 8, "spx_seed_neighbours: need radius > 0 and nb in [1, 8]");
+    return spx::seed_neighbours_launch(seeds, K, H, W, radius, nb, out, spx::as_stream(stream));
 }
 
 int spx_triangle_u8(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step, int32_t* thresh_out,
@@ -301,6 +314,14 @@ int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, cons
                   const int16_t* half, const uint8_t* model, const float* init,
                   const SpxFitOptions* opt, int32_t K, float* params_out, float* rms_out,
                   int16_t* iters_out, uint8_t* status_out, spx_stream_t stream) {
+    return spx_fit_batch_cells(img, H, W, pitch, seeds, half, model, init, opt, K, nullptr, 0, params_out,
+                               rms_out, iters_out, status_out, stream);
+}
+
+int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                        const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
+                        int32_t K, const int32_t* neighbours, int32_t nb, float* params_out, float* rms_out,
+                        int16_t* iters_out, uint8_t* status_out, spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(opt != nullptr, "spx_fit_batch: opt is NULL");
     SPX_REQUIRE(K >= 0, "spx_fit_batch: K < 0");
@@ -317,8 +338,10 @@ int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, cons
                     opt->lambda_down < 1.0f,
                 "spx_fit_batch: damping options out of range");
     SPX_REQUIRE(opt->init_sigma > 0.0f, "spx_fit_batch: init_sigma <= 0");
-    return spx::fit_batch_launch(img, H, W, pitch, seeds, half, model, init, *opt, K, params_out,
-                                 rms_out, iters_out, status_out, spx::as_stream(stream));
+    SPX_REQUIRE(neighbours == nullptr || (nb >= 1 && nb <= 8), "spx_fit_batch_cells: nb must lie in [1, 8]");
+    return spx::fit_batch_launch(img, H, W, pitch, seeds, half, model, init, *opt, K, neighbours,
+                                 neighbours ? nb : 0, params_out, rms_out, iters_out, status_out,
+                                 spx::as_stream(stream));
 }
 
 int spx_otsu_u16(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,

@@ -3,6 +3,13 @@
 No reference function fits this model; the batched per-feature LM replaces the role of
 the reference's `lm_fit` (subpx/_gpu/deblend.py:55-152) for contact-hole centres, radii
 (CD = 2 r) and edge widths. The kernel is spx_fit_batch (include/subpx_hip.h).
+
+Neighbour policy. Where a patch is larger than the hole pitch (the 15-63 px patches of a
+staggered array at ~39 px pitch), neighbouring holes fall inside it. With ``cells=True`` a
+patch pixel takes part only when it lies in the feature's Voronoi cell against its 8
+nearest seeds (spx_seed_neighbours + spx_fit_batch_cells), the cell rule of the reference's
+Voronoi refinements (subpx/_cpu/cells.py:8-84). A patch that lies inside its cell is fitted
+exactly as without the mask.
 """
 
 from __future__ import annotations
@@ -25,14 +32,40 @@ def _ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None else int(t.data_ptr())
 
 
+NEIGHBOURS = 8
+
+
+def seed_neighbours_device(seeds: torch.Tensor, shape, radius: float, nb: int = NEIGHBOURS,
+                           stream: torch.cuda.Stream | None = None) -> torch.Tensor:
+    """(K, nb) int32 device tensor of the nb nearest other seeds within ``radius`` px
+    (-1 pads), ordered by distance then index (spx_seed_neighbours)."""
+    K = int(seeds.shape[0])
+    if seeds.shape != (K, 2) or seeds.dtype != torch.float32 or not seeds.is_contiguous():
+        raise ValueError("seeds must be a contiguous (K, 2) float32 tensor")
+    out = torch.empty((K, nb), dtype=torch.int32, device=seeds.device)
+    if K:
+        H, W = int(shape[0]), int(shape[1])
+        _lib.call("spx_seed_neighbours", _ptr(seeds), K, H, W, float(radius), int(nb), _ptr(out),
+                  stream_handle(stream))
+    return out
+
+
+def cell_radius(half_max: int) -> float:
+    """Seeds farther than this cannot move the bisector into a patch of half side half_max."""
+    return 2.0 * float(np.sqrt(2.0)) * float(half_max) + 1.0
+
+
 def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor,
                          model: torch.Tensor, init: torch.Tensor | None = None, *,
                          options: _lib.SpxFitOptions | None = None, out: dict | None = None,
-                         stream: torch.cuda.Stream | None = None) -> dict:
-    """Enqueue spx_fit_batch on device tensors; no host synchronisation.
+                         stream: torch.cuda.Stream | None = None,
+                         neighbours: torch.Tensor | None = None) -> dict:
+    """Enqueue spx_fit_batch (spx_fit_batch_cells with ``neighbours``) on device tensors; no
+    host synchronisation.
 
     img (H, W) uint16, seeds (K, 2) float32, half (K,) int16, model (K,) uint8,
-    init (K, 7) float32 or None. Returns dict of device tensors params/rms/iters/status.
+    init (K, 7) float32 or None, neighbours (K, nb) int32 or None. Returns dict of device
+    tensors params/rms/iters/status.
     """
     if img.dim() != 2 or img.dtype != torch.uint16:
         raise ValueError("img must be a 2-D uint16 tensor")
@@ -57,23 +90,33 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
     opts = options if options is not None else _lib.fit_options()
     H, W = img.shape
     pitch = img.stride(0)
-    _lib.call("spx_fit_batch", _ptr(img), H, W, pitch, _ptr(seeds), _ptr(half), _ptr(model),
-              _ptr(init), ctypes.byref(opts), K, _ptr(out["params"]), _ptr(out["rms"]),
-              _ptr(out["iters"]), _ptr(out["status"]), stream_handle(stream))
+    if neighbours is None:
+        _lib.call("spx_fit_batch", _ptr(img), H, W, pitch, _ptr(seeds), _ptr(half), _ptr(model),
+                  _ptr(init), ctypes.byref(opts), K, _ptr(out["params"]), _ptr(out["rms"]),
+                  _ptr(out["iters"]), _ptr(out["status"]), stream_handle(stream))
+        return out
+    if (neighbours.dim() != 2 or neighbours.shape[0] != K or neighbours.dtype != torch.int32
+            or neighbours.device != dev or not neighbours.is_contiguous()):
+        raise ValueError("neighbours must be a contiguous (K, nb) int32 tensor on the image's device")
+    _lib.call("spx_fit_batch_cells", _ptr(img), H, W, pitch, _ptr(seeds), _ptr(half), _ptr(model),
+              _ptr(init), ctypes.byref(opts), K, _ptr(neighbours), int(neighbours.shape[1]),
+              _ptr(out["params"]), _ptr(out["rms"]), _ptr(out["iters"]), _ptr(out["status"]),
+              stream_handle(stream))
     return out
 
 
 def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: int = 30,
                   step_tol: float = 1e-6, lambda0: float = 1e-3, lambda_up: float = 10.0,
-                  lambda_down: float = 0.1, init_sigma: float = 1.0, backend: str = "gpu",
-                  device: int = 0) -> ErfFitResult:
+                  lambda_down: float = 0.1, init_sigma: float = 1.0, cells: bool = False,
+                  backend: str = "gpu", device: int = 0) -> ErfFitResult:
     """Fit an erf-edged disk (6 parameters) or ellipse (7) to a patch around each seed.
 
     ``seeds`` (K, 2) [x, y]; the patch is (2 half + 1)^2 pixels centred on rint(seed)
     (half per feature or scalar, 2..31). ``model`` is "disk" / "ellipse" or a per-feature
     array of 0/1. ``init`` (K, 7) [x0, y0, r, sigma, A, B, q] replaces the moment start.
     Fits stop after ``max_iter`` step attempts or when a step below ``step_tol`` px is
-    taken.
+    taken. ``cells=True`` restricts every patch to the feature's Voronoi cell among the
+    seeds (the neighbour policy above).
     """
     resolve_backend(backend)
     seeds_np = np.asarray(seeds, dtype=np.float32).reshape(-1, 2)
@@ -101,9 +144,12 @@ def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: i
         init_t = None
         if init is not None:
             init_t = torch.from_numpy(np.asarray(init, dtype=np.float32).reshape(K, 7)).to(dev)
-        out = fit_erf_disks_device(img_t, seeds_t, half_t, model_t, init_t, options=opts)
+        nbr = None
+        if cells:
+            nbr = seed_neighbours_device(seeds_t, img_t.shape, cell_radius(int(half_np.max())))
+        out = fit_erf_disks_device(img_t, seeds_t, half_t, model_t, init_t, options=opts, neighbours=nbr)
         torch.cuda.current_stream().synchronize()
         return ErfFitResult(out["params"].cpu().numpy(), out["rms"].cpu().numpy(),
                             out["iters"].cpu().numpy(), out["status"].cpu().numpy(), model_np,
                             backend="gpu",
-                            meta={"max_iter": int(max_iter), "step_tol": float(step_tol)})
+                            meta={"max_iter": int(max_iter), "step_tol": float(step_tol), "cells": bool(cells)})

@@ -228,6 +228,25 @@ int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, cons
                   const SpxFitOptions* opt, int32_t K, float* params_out, float* rms_out,
                   int16_t* iters_out, uint8_t* status_out, spx_stream_t stream);
 
+/* spx_fit_batch with a neighbour (Voronoi cell) mask: a patch pixel takes part only when it
+ * is nearer to the feature's own seed than to each of its listed neighbour seeds (ties to
+ * the lower feature index), and the moment start's background ring adds the cell's
+ * boundary pixels. neighbours: device (K, nb) int32 feature ids (ids outside [0, K), e.g. -1,
+ * mean none), nb in [1, 8];
+ * null = no mask (spx_fit_batch). No reference counterpart: the neighbour policy of
+ * patches larger than the hole pitch, after the reference's Voronoi cells
+ * (subpx/_cpu/cells.py:8-84). */
+int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                        const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
+                        int32_t K, const int32_t* neighbours, int32_t nb, float* params_out, float* rms_out,
+                        int16_t* iters_out, uint8_t* status_out, spx_stream_t stream);
+
+/* The nb nearest other seeds of every seed within `radius` px (ordered by distance, ties by
+ * index; -1 where fewer exist), by a uniform-grid hash on the device. seeds (K, 2) f32 in an
+ * H x W image, out (K, nb) int32: device. nb in [1, 8]. */
+int spx_seed_neighbours(const float* seeds, int32_t K, int64_t H, int64_t W, float radius, int32_t nb,
+                        int32_t* out, spx_stream_t stream);
+
 /* Work counters of the last spx_fit_batch call issued on this thread (no reference
  * counterpart; used to state algorithmic FLOP rates): out[0] = sum over features of pixel
  * passes x patch pixels for 6-parameter fits, out[1] the same for 7-parameter fits,

@@ -25,6 +25,13 @@ cost * (1 + accept_rtol) (lambda *= down, floor 1e-12) and refused otherwise; th
 stops with status "converged" when a step with max(|dx0|, |dy0|, |dr|, |ds|, r |dq|) <
 step_tol was taken or refused, else after max_iter attempts with "max_iter".
 
+Neighbour (Voronoi cell) mask, when neighbour lists are given: a patch pixel takes part
+only when it is strictly nearer to the own seed than to each listed neighbour seed, ties
+going to the lower feature index (after reference subpx/_cpu/cells.py:8-33), squared
+distances in FP64; the start's background ring then also takes the cell's boundary pixels
+(in the cell with a 4-neighbour outside it, reference subpx/_cpu/cells.py:72-80).
+``seed_neighbours`` lists the nb nearest other seeds within a radius by brute force.
+
 Pinned by: recovery of noise-free renders to 1e-8 px, the Jacobian against central
 differences, the lattice-sum identity sum_pixels erfc(u)/2 = pi (r^2 + s^2) for r >> s,
 integer-shift and mirror equivariance (tests/test_oracle_erfdisk.py).
@@ -38,6 +45,34 @@ from scipy.special import erfc
 CONVERGED, MAX_ITER, SINGULAR, DIVERGED, TOO_FEW, OUTSIDE, BAD_START = range(7)
 SQRT2 = np.sqrt(2.0)
 INV_SQRT2PI = 1.0 / np.sqrt(2.0 * np.pi)
+
+
+def seed_neighbours(seeds, radius, nb):
+    """(K, nb) ids of the nb nearest other seeds within radius, by (squared distance, index); -1 pads."""
+    s = np.asarray(seeds, dtype=np.float32).astype(np.float64).reshape(-1, 2)
+    K = s.shape[0]
+    out = np.full((K, nb), -1, dtype=np.int32)
+    r2 = float(np.float32(radius)) ** 2
+    for i in range(K):
+        d = (s[:, 0] - s[i, 0]) ** 2 + (s[:, 1] - s[i, 1]) ** 2
+        d[i] = np.inf
+        cand = np.nonzero(d <= r2)[0]
+        order = cand[np.lexsort((cand, d[cand]))][:nb]
+        out[i, :order.size] = order
+    return out
+
+
+def cell_mask(px, py, seeds, f, nbr_row):
+    """True where pixel (px, py) lies in feature f's cell against the listed neighbours."""
+    s = np.asarray(seeds, dtype=np.float32).astype(np.float64).reshape(-1, 2)
+    d0 = (px - s[f, 0]) * (px - s[f, 0]) + (py - s[f, 1]) * (py - s[f, 1])
+    inside = np.ones(np.shape(px), dtype=bool)
+    for j in nbr_row:
+        if j < 0:
+            continue
+        dj = (px - s[j, 0]) * (px - s[j, 0]) + (py - s[j, 1]) * (py - s[j, 1])
+        inside &= (dj > d0) | ((dj == d0) & (f < j))
+    return inside
 
 
 def extract_patch(img: np.ndarray, seed_xy, half: int):
@@ -58,11 +93,14 @@ def extract_patch(img: np.ndarray, seed_xy, half: int):
     return v, valid, fx.astype(np.float64), fy.astype(np.float64), cx, cy
 
 
-def moment_start(v, valid, half, init_sigma=1.0, ellipse=False):
-    """Internal-order start [x0, y0, r, s, A, (q), B] or None for a flat patch."""
+def moment_start(v, valid, half, init_sigma=1.0, ellipse=False, cell_edge=None):
+    """Internal-order start [x0, y0, r, s, A, (q), B] or None for a flat patch. cell_edge:
+    patch pixels with a 4-neighbour outside the feature's cell (added to the ring)."""
     side = 2 * half + 1
     ring = np.zeros((side, side), dtype=bool)
     ring[0, :] = ring[-1, :] = ring[:, 0] = ring[:, -1] = True
+    if cell_edge is not None:
+        ring |= cell_edge
     ring &= valid
     B0 = float(v[ring].mean()) if ring.any() else 0.0
     w = v - B0
@@ -127,7 +165,7 @@ def ldl_solve(M, g):
     return x if np.all(np.isfinite(x)) else None
 
 
-def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, max_iter=30, step_tol=1e-6,
+def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, cell_edge=None, max_iter=30, step_tol=1e-6,
                  lambda0=1e-3, lambda_up=10.0, lambda_down=0.1, init_sigma=1.0,
                  accept_rtol=1e-6):
     """Fit one patch. Returns (internal params or NaN, cost, iterations, status, npix)."""
@@ -139,7 +177,7 @@ def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, max_iter=30, step_
     nanp = np.full(P, np.nan)
     if npix < P + 2:
         return nanp, np.nan, 0, TOO_FEW, npix
-    p = moment_start(v, valid, half, init_sigma, ellipse) if init is None else np.asarray(init, float)
+    p = moment_start(v, valid, half, init_sigma, ellipse, cell_edge) if init is None else np.asarray(init, float)
     if p is None:
         return nanp, np.nan, 0, BAD_START, npix
 
@@ -202,8 +240,9 @@ def to_output(p_internal, ellipse, cx, cy):
     return np.array([p[0] + cx, p[1] + cy, p[2], p[3], p[4], p[-1], q], dtype=np.float64)
 
 
-def fit_batch(img, seeds, half, model, init=None, **opts):
-    """Oracle counterpart of spx_fit_batch: (params (K,7), rms, iterations, status)."""
+def fit_batch(img, seeds, half, model, init=None, neighbours=None, **opts):
+    """Oracle counterpart of spx_fit_batch (spx_fit_batch_cells with ``neighbours`` (K, nb)):
+    (params (K,7), rms, iterations, status)."""
     img = np.asarray(img)
     seeds = np.asarray(seeds, dtype=np.float64).reshape(-1, 2)
     K = seeds.shape[0]
@@ -220,6 +259,17 @@ def fit_batch(img, seeds, half, model, init=None, **opts):
             status[k] = TOO_FEW
             continue
         v, valid, _, _, cx, cy = extract_patch(img, seeds[k], h)
+        cell_edge = None
+        if neighbours is not None and (np.asarray(neighbours[k]) >= 0).any():
+            side = 2 * h + 1
+            gy, gx = np.meshgrid(np.arange(side) + (cy - h), np.arange(side) + (cx - h), indexing="ij")
+            inside = cell_mask(gx, gy, seeds, k, neighbours[k])
+            valid = valid & inside
+            out = np.zeros((side, side), dtype=bool)
+            for ddx, ddy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                out |= ~cell_mask(gx + ddx, gy + ddy, seeds, k, neighbours[k])
+            cell_edge = inside & out
+            v = np.where(valid, v, 0.0)
         p0 = None
         if init is not None:
             row = np.asarray(init[k], dtype=np.float64)
@@ -227,7 +277,7 @@ def fit_batch(img, seeds, half, model, init=None, **opts):
             if ell:
                 p0.append(row[6])
             p0.append(row[5])
-        p, cost, it, st, npix = lm_fit_patch(v, valid, h, ell, p0, **opts)
+        p, cost, it, st, npix = lm_fit_patch(v, valid, h, ell, p0, cell_edge=cell_edge, **opts)
         params[k] = to_output(p, ell, cx, cy)
         P = 7 if ell else 6
         rms[k] = np.sqrt(cost / max(npix - P, 1)) if st not in (TOO_FEW, BAD_START) else np.nan

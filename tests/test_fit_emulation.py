@@ -57,6 +57,29 @@ ok = st <= 1
 err = np.abs(params[ok].astype(np.float64) - ref_p[ok]).max(axis=0)
 assert err[0] < 1e-3 and err[1] < 1e-3 and err[2] < 1e-3 and err[3] < 1e-3, err
 print("emulated", K, "fits; max |emu - oracle|:", err)
+
+# the same seeds with the Voronoi cell mask (all the page's holes as neighbours)
+allseeds = np.concatenate([np.rint(arr.centers_xy).astype(np.float32), seeds[-1:]]).astype(np.float32)
+Ka = allseeds.shape[0]
+sel2 = np.r_[sel, Ka - 1]
+nbr_all = erfdisk.seed_neighbours(allseeds, 2 * np.sqrt(2) * 22 + 1, 8)
+half_a = np.full(Ka, 7, np.int16); half_a[sel2] = half
+model_a = np.zeros(Ka, np.uint8); model_a[sel2] = model
+only = np.zeros(Ka, bool); only[sel2] = True
+half_a[~only] = 0  # rejected by the kernel's range check: only the selected holes are fitted
+params = np.zeros((Ka, 7), np.float32); rms = np.zeros(Ka, np.float32)
+iters = np.zeros(Ka, np.int16); st = np.zeros(Ka, np.uint8)
+rc = lib.emu_fit_batch_cells(P(img), ctypes.c_int64(160), ctypes.c_int64(160), ctypes.c_int64(160), P(allseeds),
+                             P(half_a), P(model_a), None, ctypes.byref(opts), ctypes.c_int32(Ka),
+                             P(np.ascontiguousarray(nbr_all)), ctypes.c_int32(8), P(params), P(rms), P(iters), P(st),
+                             ctypes.c_int(240))
+ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, allseeds, half_a, model_a, neighbours=nbr_all)
+assert (st[only] == ref_st[only]).all(), (st[only], ref_st[only])
+ok = only & (st <= 1)
+assert ok.sum() >= 4
+err = np.abs(params[ok].astype(np.float64) - ref_p[ok]).max(axis=0)
+assert err[0] < 1e-3 and err[1] < 1e-3 and err[2] < 1e-3 and err[3] < 1e-3, err
+print("emulated", int(only.sum()), "cell-masked fits; max |emu - oracle|:", err)
 """
 
 

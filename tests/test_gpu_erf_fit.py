@@ -101,3 +101,65 @@ def test_init_start_is_used():
     ok = first.ok & again.ok
     assert np.abs(again.params[ok, :2] - first.params[ok, :2]).max() < 1e-3
     assert again.iterations[ok].mean() < first.iterations[ok].mean()
+
+
+def test_seed_neighbours_match_oracle():
+    import torch
+
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.fit import seed_neighbours_device
+
+    rng = np.random.default_rng(11)
+    seeds = rng.uniform(-3, 2003, (4000, 2)).astype(np.float32)
+    seeds[:40] = seeds[40:80] + np.float32(0.5)  # close pairs
+    for radius, nb in [(60.0, 8), (25.0, 3), (200.0, 1)]:
+        with gpu_device(0) as dev:
+            got = seed_neighbours_device(torch.from_numpy(seeds).to(dev), (2000, 2000), radius, nb).cpu().numpy()
+        np.testing.assert_array_equal(got, erfdisk.seed_neighbours(seeds, radius, nb))
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_cell_masked_fits_match_oracle_c2_shape(seed):
+    # staggered array, r 4-20 px, patches 15-63 px at a 39 px pitch, 10 % ellipses: every
+    # patch restricted to its Voronoi cell among the 8 nearest seeds
+    from imageprocessor_new_amd.fit import cell_radius
+
+    arr = synthetic.make_config("c2", seed=seed, H=500, W=500)
+    seeds = np.rint(arr.centers_xy).astype(np.float32)
+    model = arr.ellipse.astype(np.uint8)
+    half = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int) + 2, 7, 31)
+    res = _fit_gpu(arr.image, seeds, half, model, cells=True)
+    nbr = erfdisk.seed_neighbours(seeds, cell_radius(int(half.max())), 8)
+    ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, seeds, half, model, neighbours=nbr)
+    _compare(res, ref_p, ref_st, status_agree=0.97)
+    # the cell removes the neighbours' light: more fits converge, and the radius (the CD)
+    # is no longer pulled by it (centres of a symmetric array are little biased either way)
+    plain = _fit_gpu(arr.image, seeds, half, model)
+    inside = arr.inside
+    m_ok = (res.status == 0) & inside
+    p_ok = (plain.status == 0) & inside
+    both = m_ok & p_ok
+    m_r = np.median(np.abs(res.params[both, 2] - arr.radius[both]))
+    p_r = np.median(np.abs(plain.params[both, 2] - arr.radius[both]))
+    print("converged masked / plain:", m_ok.sum(), p_ok.sum(), "median radius error:", m_r, p_r)
+    assert m_ok.sum() >= p_ok.sum()
+    assert m_r < p_r
+
+
+def test_cells_are_a_no_op_when_patches_sit_inside_them():
+    # 31 x 31 patches on the 32 px square array lie inside their cells, except where two
+    # rounded seeds end up 30 px apart (a bisector row of ties goes to the lower index)
+    from imageprocessor_new_amd.fit import cell_radius
+
+    arr = synthetic.make_config("c1", seed=4, H=320, W=320)
+    seeds = np.rint(arr.centers_xy).astype(np.float32)
+    nbr = erfdisk.seed_neighbours(seeds, cell_radius(15), 8)
+    inside = np.zeros(len(seeds), bool)
+    for k in range(len(seeds)):
+        gy, gx = np.meshgrid(np.arange(31) + (seeds[k, 1] - 15), np.arange(31) + (seeds[k, 0] - 15), indexing="ij")
+        inside[k] = erfdisk.cell_mask(gx, gy, seeds, k, nbr[k]).all()
+    assert inside.mean() > 0.95
+    a = _fit_gpu(arr.image, seeds, 15, "disk")
+    b = _fit_gpu(arr.image, seeds, 15, "disk", cells=True)
+    np.testing.assert_array_equal(a.status[inside], b.status[inside])
+    np.testing.assert_array_equal(a.params[inside], b.params[inside])

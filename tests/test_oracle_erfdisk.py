@@ -73,3 +73,48 @@ def test_statuses_for_degenerate_patches():
     assert st[0] == erfdisk.BAD_START
     _, _, _, st = erfdisk.fit_batch(flat, [[-20, -20]], 8, 0)
     assert st[0] == erfdisk.TOO_FEW
+
+
+def test_seed_neighbours_match_kdtree():
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(5)
+    seeds = rng.uniform(0, 300, (400, 2)).astype(np.float32)
+    nb = erfdisk.seed_neighbours(seeds, 40.0, 6)
+    d, idx = cKDTree(seeds.astype(np.float64)).query(seeds.astype(np.float64), k=7, distance_upper_bound=40.0)
+    for i in range(len(seeds)):
+        want = [j for j, dd in zip(idx[i, 1:], d[i, 1:]) if np.isfinite(dd)]
+        got = [j for j in nb[i] if j >= 0]
+        assert got == want
+
+
+def test_cell_mask_is_the_nearest_seed_partition():
+    # with every other seed listed, the cell is exactly the pixels whose nearest seed is f
+    # (ties to the lowest index, reference subpx/_cpu/cells.py:8-33), computed here by cKDTree
+    from scipy.spatial import cKDTree
+
+    seeds = np.array([[20.0, 20.0], [36.0, 21.5], [28.0, 35.0], [12.0, 33.0], [40.0, 40.0], [28.0, 8.0]], np.float32)
+    gy, gx = np.mgrid[0:50, 0:50].astype(np.float64)
+    pts = np.c_[gx.ravel(), gy.ravel()]
+    d, idx = cKDTree(seeds.astype(np.float64)).query(pts, k=2)
+    lab = idx[:, 0].copy()
+    tie = d[:, 0] == d[:, 1]
+    lab[tie] = np.minimum(idx[tie, 0], idx[tie, 1])
+    lab = lab.reshape(50, 50)
+    for f in range(len(seeds)):
+        others = [j for j in range(len(seeds)) if j != f]
+        np.testing.assert_array_equal(erfdisk.cell_mask(gx, gy, seeds, f, others), lab == f)
+
+
+def test_cell_mask_removes_the_neighbours_bias():
+    # two noise-free holes 26 px apart in 41x41 patches: unmasked, the neighbour's disk biases
+    # the fit; restricted to the Voronoi cells both centres come back to 1e-3 px
+    c = np.array([[40.3, 40.7], [66.1, 40.2]])
+    clean = synthetic.render_clean((81, 107), c, np.array([7.0, 7.0]), np.array([1.0, 1.0]), np.ones(2),
+                                   3000.0, 500.0)
+    nbr = erfdisk.seed_neighbours(c.astype(np.float32), 60.0, 8)
+    p0, _, _, s0 = erfdisk.fit_batch(clean, c, 20, 0)
+    p1, _, _, s1 = erfdisk.fit_batch(clean, c, 20, 0, neighbours=nbr)
+    assert (s1 <= 1).all()
+    assert np.abs(p1[:, :2] - c).max() < 1e-3
+    assert np.abs(p0[:, :2] - c).max() > 10 * np.abs(p1[:, :2] - c).max()

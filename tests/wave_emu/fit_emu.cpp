@@ -128,10 +128,11 @@ static void run_wave(const FitArgs& a, int timeout_s) {
     for (auto& t : lanes) t.join();
 }
 
-extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
-                             const float* seeds, const int16_t* half, const uint8_t* model,
-                             const float* init, const SpxFitOptions* opt, int32_t K, float* params,
-                             float* rms, int16_t* iters, uint8_t* status, int timeout_s) {
+extern "C" int emu_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
+                                   const float* seeds, const int16_t* half, const uint8_t* model,
+                                   const float* init, const SpxFitOptions* opt, int32_t K, const int32_t* nbr,
+                                   int32_t nb, float* params, float* rms, int16_t* iters, uint8_t* status,
+                                   int timeout_s) {
     constexpr int kKB = 2 * kBuckets;
     std::vector<std::vector<int32_t>> order(kKB);
     for (int f = 0; f < K; ++f) {
@@ -152,7 +153,7 @@ extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t 
         int32_t queue = 0;
         if (count == 0) continue;
         FitArgs a{img, H, W, pitch, seeds, half, model, init, *opt, order[b].data(), &count, &queue,
-                  stats, params, rms, iters, status};
+                  stats, params, rms, iters, status, nbr, nb, K};
         switch (b) {
             case 0: run_wave<4, 6>(a, timeout_s); break;
             case 1: run_wave<4, 7>(a, timeout_s); break;
@@ -167,4 +168,12 @@ extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t 
         }
     }
     return 0;
+}
+
+extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
+                             const float* seeds, const int16_t* half, const uint8_t* model,
+                             const float* init, const SpxFitOptions* opt, int32_t K, float* params,
+                             float* rms, int16_t* iters, uint8_t* status, int timeout_s) {
+    return emu_fit_batch_cells(img, H, W, pitch, seeds, half, model, init, opt, K, nullptr, 0, params, rms, iters,
+                               status, timeout_s);
 }
