@@ -59,8 +59,8 @@ SPX_DEV_INLINE float spx_wave_max(float v) {
 
 // Every lane takes part in the atomic (lane 0 adds 1, the others 0; the atomic optimizer
 // merges them into one), and the first lane's return is the wave's slot.
-SPX_DEV_INLINE int spx_queue_grab(int32_t* q) {
-    const int r = atomicAdd(q, spx_lane() == 0 ? 1 : 0);
+SPX_DEV_INLINE int spx_queue_grab(int32_t* q, int n) {
+    const int r = atomicAdd(q, spx_lane() == 0 ? n : 0);
     return __builtin_amdgcn_readfirstlane(r);
 }
 SPX_DEV_INLINE unsigned long long spx_atomic_add(unsigned long long* p, unsigned long long v) {

@@ -4,7 +4,7 @@
 //   SPX_DEV, SPX_DEV_INLINE, spx_lane(),
 //   spx_wave_sum(float) / spx_wave_max(float): reductions over the wave, result uniform,
 //   spx_uniform(float) / spx_uniform_i(int) / spx_uniform_d(double): the first lane's value,
-//   spx_queue_grab(int32_t*): one atomic increment per wave, result uniform,
+//   spx_queue_grab(int32_t*, int n): one atomic add of n per wave, old value uniform,
 //   spx_atomic_add(unsigned long long*, unsigned long long),
 //   hw_rcp / hw_rsq / hw_exp2 (float) and hw_rcp64 (double).
 // Every value that steers control flow is wave-uniform by construction, so the compiler
@@ -57,6 +57,8 @@ constexpr float kSqrt2 = 1.4142135623730951f;
 constexpr float kInvSqrt2 = 0.7071067811865476f;
 constexpr float kInvSqrt2Pi = 0.3989422804014327f;
 constexpr float kAcceptRtol = 1.0e-6f;
+constexpr int kGrab = 4;        // features claimed per work-queue atomic
+constexpr int kUnrollMax = 16;  // pixel loops of up to this many slots per lane are unrolled
 
 constexpr int kBuckets = 5;  // NPL = 4, 8, 16, 32, 64 pixels per lane
 constexpr int kMaxHalf = 31;
@@ -121,53 +123,59 @@ SPX_DEV_INLINE void pixel_pass(const Patch<NPL>& pt, int hs, const Params<P>& pp
 
 #pragma unroll
     for (int n = 0; n < NPL; ++n) {
-        if (n >= pt.nslots) break;
-        const uint32_t w = pt.px[n];
-        const float dx = (float)(w & 0xffu) - xo;
-        const float dy = (float)((w >> 8) & 0xffu) - yo;
-        const float val = (float)(w >> 16);
-        const float dyq = kEll ? dy * iq2 : dy;
-        float rho2 = fmaf(dy, dyq, dx * dx);
-        rho2 = fmaxf(rho2, 1e-20f);
-        const float rs = hw_rsq(rho2);
-        const float rho = rho2 * rs;
-        const float u = fmaf(rho, k, -rk);
-        const float t = hw_rcp(fmaf(kAsP, fabsf(u), 1.0f));
-        const float ez = hw_exp2(-(u * u) * kLog2e);
-        const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, kAs5, kAs4), kAs3), kAs2), kAs1);
-        const float hq = 0.5f * poly * ez;
-        float E = (u >= 0.0f) ? hq : 1.0f - hq;
-        float res = val - fmaf(A, E, B);
-        float G = Ac * ez;
-        if (n >= pt.nfull) {
-            // partly valid slot: zero the residual and the Jacobian of empty lanes
-            const float wm = ((pt.valid >> n) & 1ull) ? 1.0f : 0.0f;
-            res *= wm;
-            E *= wm;
-            G *= wm;
+        // NPL <= 16: a uniform guard keeps the loop fully unrolled with px[] in VGPRs;
+        // larger patches keep the loop rolled (px[] in scratch) rather than spill the LM state.
+        if constexpr (NPL > kUnrollMax) {
+            if (n >= pt.nslots) break;
         }
-        const float Gr = G * rs;
-        float J[P - 1];
-        J[0] = Gr * dx;
-        J[1] = Gr * dyq;
-        J[2] = G;
-        J[3] = G * u * kSqrt2;
-        J[4] = E;
-        if constexpr (kEll) J[5] = Gr * dy * dyq * iq;
-        int idx = 0;
-#pragma unroll
-        for (int a = 0; a < L::kNB; ++a)
-#pragma unroll
-            for (int b = a; b < L::kNB; ++b) {
-                acc[idx] = fmaf(J[a], J[b], acc[idx]);
-                ++idx;
+        if (n < pt.nslots) {
+            const uint32_t w = pt.px[n];
+            const float dx = (float)(w & 0xffu) - xo;
+            const float dy = (float)((w >> 8) & 0xffu) - yo;
+            const float val = (float)(w >> 16);
+            const float dyq = kEll ? dy * iq2 : dy;
+            float rho2 = fmaf(dy, dyq, dx * dx);
+            rho2 = fmaxf(rho2, 1e-20f);
+            const float rs = hw_rsq(rho2);
+            const float rho = rho2 * rs;
+            const float u = fmaf(rho, k, -rk);
+            const float t = hw_rcp(fmaf(kAsP, fabsf(u), 1.0f));
+            const float ez = hw_exp2(-(u * u) * kLog2e);
+            const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, kAs5, kAs4), kAs3), kAs2), kAs1);
+            const float hq = 0.5f * poly * ez;
+            float E = (u >= 0.0f) ? hq : 1.0f - hq;
+            float res = val - fmaf(A, E, B);
+            float G = Ac * ez;
+            if (n >= pt.nfull) {
+                // partly valid slot: zero the residual and the Jacobian of empty lanes
+                const float wm = ((pt.valid >> n) & 1ull) ? 1.0f : 0.0f;
+                res *= wm;
+                E *= wm;
+                G *= wm;
             }
-#pragma unroll
-        for (int a = 0; a < L::kNB; ++a) acc[L::kColB + a] += J[a];
-#pragma unroll
-        for (int a = 0; a < L::kNB; ++a) acc[L::kRhs + a] = fmaf(J[a], res, acc[L::kRhs + a]);
-        acc[L::kRhs + P - 1] += res;
-        acc[L::kCost] = fmaf(res, res, acc[L::kCost]);
+            const float Gr = G * rs;
+            float J[P - 1];
+            J[0] = Gr * dx;
+            J[1] = Gr * dyq;
+            J[2] = G;
+            J[3] = G * u * kSqrt2;
+            J[4] = E;
+            if constexpr (kEll) J[5] = Gr * dy * dyq * iq;
+            int idx = 0;
+    #pragma unroll
+            for (int a = 0; a < L::kNB; ++a)
+    #pragma unroll
+                for (int b = a; b < L::kNB; ++b) {
+                    acc[idx] = fmaf(J[a], J[b], acc[idx]);
+                    ++idx;
+                }
+    #pragma unroll
+            for (int a = 0; a < L::kNB; ++a) acc[L::kColB + a] += J[a];
+    #pragma unroll
+            for (int a = 0; a < L::kNB; ++a) acc[L::kRhs + a] = fmaf(J[a], res, acc[L::kRhs + a]);
+            acc[L::kRhs + P - 1] += res;
+            acc[L::kCost] = fmaf(res, res, acc[L::kCost]);
+        }
     }
 }
 
@@ -300,11 +308,19 @@ SPX_DEV_INLINE bool in_cell(const FitArgs& a, int f, double px, double py) {
     return in;
 }
 
+// Work counters of one wave, flushed to FitArgs::stats once when the wave leaves its
+// queue (one atomic per counter per wave instead of per feature).
+struct WaveStats {
+    unsigned long long evals;   // pixel passes x patch pixels
+    unsigned long long passes;  // pixel passes
+    unsigned long long steps;   // LM step attempts
+};
+
 // Fit of one feature by the whole wave. The patch registers are filled by the caller;
 // `masked`: the patch is restricted to the feature's Voronoi cell.
 template <int NPL, int P>
 SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>& pt,
-                                float cx, float cy, bool masked) {
+                                float cx, float cy, bool masked, WaveStats& ws) {
     using L = Layout<P>;
     const int lane = spx_lane();
     const float npix = pt.npix;
@@ -339,22 +355,28 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
             float bsum = 0.0f, bcnt = 0.0f, vmax = -3.0e38f;
 #pragma unroll
             for (int n = 0; n < NPL; ++n) {
-                if (n >= pt.nslots) break;
-                if ((pt.valid >> n) & 1ull) {
-                    const uint32_t w = pt.px[n];
-                    const int col = (int)(w & 0xffu), row = (int)((w >> 8) & 0xffu);
-                    const float val = (float)(w >> 16);
-                    bool ring = row == 0 || row == side - 1 || col == 0 || col == side - 1;
-                    if (!ring && masked) {
-                        const double px = ox + (double)col, py = oy + (double)row;
-                        ring = !in_cell(a, f, px - 1.0, py) || !in_cell(a, f, px + 1.0, py) ||
-                               !in_cell(a, f, px, py - 1.0) || !in_cell(a, f, px, py + 1.0);
+                // NPL <= 16: a uniform guard keeps the loop fully unrolled with px[] in VGPRs;
+        // larger patches keep the loop rolled (px[] in scratch) rather than spill the LM state.
+        if constexpr (NPL > kUnrollMax) {
+            if (n >= pt.nslots) break;
+        }
+        if (n < pt.nslots) {
+                    if ((pt.valid >> n) & 1ull) {
+                        const uint32_t w = pt.px[n];
+                        const int col = (int)(w & 0xffu), row = (int)((w >> 8) & 0xffu);
+                        const float val = (float)(w >> 16);
+                        bool ring = row == 0 || row == side - 1 || col == 0 || col == side - 1;
+                        if (!ring && masked) {
+                            const double px = ox + (double)col, py = oy + (double)row;
+                            ring = !in_cell(a, f, px - 1.0, py) || !in_cell(a, f, px + 1.0, py) ||
+                                   !in_cell(a, f, px, py - 1.0) || !in_cell(a, f, px, py + 1.0);
+                        }
+                        if (ring) {
+                            bsum += val;
+                            bcnt += 1.0f;
+                        }
+                        vmax = fmaxf(vmax, val);
                     }
-                    if (ring) {
-                        bsum += val;
-                        bcnt += 1.0f;
-                    }
-                    vmax = fmaxf(vmax, val);
                 }
             }
             bsum = spx_wave_sum(bsum);
@@ -365,14 +387,20 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
             float s0 = 0.0f, sx = 0.0f, sy = 0.0f, nin = 0.0f;
 #pragma unroll
             for (int n = 0; n < NPL; ++n) {
-                if (n >= pt.nslots) break;
-                const uint32_t w = pt.px[n];
-                const float wv = (float)(w >> 16) - B0;
-                if (((pt.valid >> n) & 1ull) && wv > half_peak) {
-                    s0 += wv;
-                    sx = fmaf(wv, (float)((int)(w & 0xffu) - hs), sx);
-                    sy = fmaf(wv, (float)((int)((w >> 8) & 0xffu) - hs), sy);
-                    nin += 1.0f;
+                // NPL <= 16: a uniform guard keeps the loop fully unrolled with px[] in VGPRs;
+        // larger patches keep the loop rolled (px[] in scratch) rather than spill the LM state.
+        if constexpr (NPL > kUnrollMax) {
+            if (n >= pt.nslots) break;
+        }
+        if (n < pt.nslots) {
+                    const uint32_t w = pt.px[n];
+                    const float wv = (float)(w >> 16) - B0;
+                    if (((pt.valid >> n) & 1ull) && wv > half_peak) {
+                        s0 += wv;
+                        sx = fmaf(wv, (float)((int)(w & 0xffu) - hs), sx);
+                        sy = fmaf(wv, (float)((int)((w >> 8) & 0xffu) - hs), sy);
+                        nin += 1.0f;
+                    }
                 }
             }
             s0 = spx_wave_sum(s0);
@@ -478,9 +506,9 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
         a.iters[f] = (int16_t)it;
         a.status[f] = status;
     }
-    if (lane == 8) spx_atomic_add(&a.stats[P == 7 ? 1 : 0], (unsigned long long)passes * (unsigned long long)npix);
-    if (lane == 9) spx_atomic_add(&a.stats[2], (unsigned long long)passes);
-    if (lane == 10) spx_atomic_add(&a.stats[3], (unsigned long long)it);
+    ws.evals += (unsigned long long)passes * (unsigned long long)npix;
+    ws.passes += (unsigned long long)passes;
+    ws.steps += (unsigned long long)it;
 }
 
 // Body of the persistent fit kernel for one wave: pull features of one (patch size,
@@ -490,8 +518,16 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
     const int lane = spx_lane();
     const int count = spx_uniform_i(*a.count);
     if (count == 0) return;
+    WaveStats ws{0ull, 0ull, 0ull};
+    // Features are claimed kGrab at a time: one queue atomic per kGrab features.
+    int base = 0, left = 0;
     for (;;) {
-        const int slot = spx_queue_grab(a.queue);
+        if (left == 0) {
+            base = spx_queue_grab(a.queue, kGrab);
+            left = kGrab;
+        }
+        const int slot = base + (kGrab - left);
+        --left;
         if (slot >= count) break;
         const int f = spx_uniform_i(a.order[slot]);
         const int hs = spx_uniform_i(a.half[f]);
@@ -532,8 +568,11 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
             cnt += ok ? 1.0f : 0.0f;
         }
         pt.npix = spx_wave_sum(cnt);
-        fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked);
+        fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked, ws);
     }
+    if (lane == 8) spx_atomic_add(&a.stats[P == 7 ? 1 : 0], ws.evals);
+    if (lane == 9) spx_atomic_add(&a.stats[2], ws.passes);
+    if (lane == 10) spx_atomic_add(&a.stats[3], ws.steps);
 }
 
 // Patch-size bucket of a half side: NPL = 4, 8, 16, 32, 64 pixels per lane.

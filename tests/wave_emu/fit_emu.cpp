@@ -81,8 +81,8 @@ inline float spx_wave_max(float v) {
     for (int off = 32; off >= 1; off >>= 1) v = std::max(v, spx_shfl_xor(v, off));
     return spx_uniform(v);
 }
-inline int spx_queue_grab(int32_t* q) {
-    const int mine = t_lane == 0 ? __atomic_fetch_add(q, 1, __ATOMIC_SEQ_CST) : 0;
+inline int spx_queue_grab(int32_t* q, int n) {
+    const int mine = t_lane == 0 ? __atomic_fetch_add(q, n, __ATOMIC_SEQ_CST) : 0;
     return spx_shfl(mine, 0);
 }
 inline unsigned long long spx_atomic_add(unsigned long long* p, unsigned long long v) {
