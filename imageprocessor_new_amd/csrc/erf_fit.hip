@@ -57,6 +57,35 @@ SPX_DEV_INLINE float spx_wave_max(float v) {
     return fmaxf(fmaxf(__int_as_float(r16), __int_as_float(r32)), fmaxf(__int_as_float(r48), __int_as_float(r64)));
 }
 
+// Butterfly steps of the many-value wave sum (wave_sum_many in erf_fit_device.hpp). Each
+// pair step adds two per-lane values over lanes that differ in one index bit and keeps the
+// first value's partial sum in lanes with that bit clear, the second's in lanes with it set.
+// Bit 5 and bit 4: v_permlane32_swap / v_permlane16_swap exchange half-waves / rows of two
+// registers, so the pair costs one swap and one add. Bits 3..0: row_mirror,
+// row_half_mirror and quad_perm DPP partners (two selects and one DPP add).
+SPX_DEV_INLINE float spx_pair_sum32(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+SPX_DEV_INLINE float spx_pair_sum16(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+template <int CTRL, int BIT>
+SPX_DEV_INLINE float spx_pair_dpp(float a, float b) {
+    const bool hi = (spx_lane() >> BIT) & 1;
+    const float send = hi ? a : b;
+    const float keep = hi ? b : a;
+    return keep + dpp_f<CTRL, 0xF>(send);
+}
+template <int CTRL>
+SPX_DEV_INLINE float spx_self_dpp(float a) {
+    return a + dpp_f<CTRL, 0xF>(a);
+}
+SPX_DEV_INLINE float spx_readlane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 // Every lane takes part in the atomic (lane 0 adds 1, the others 0; the atomic optimizer
 // merges them into one), and the first lane's return is the wave's slot.
 SPX_DEV_INLINE int spx_queue_grab(int32_t* q, int n) {

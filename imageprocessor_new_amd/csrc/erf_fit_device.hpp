@@ -6,6 +6,8 @@
 //   spx_uniform(float) / spx_uniform_i(int) / spx_uniform_d(double): the first lane's value,
 //   spx_queue_grab(int32_t*, int n): one atomic add of n per wave, old value uniform,
 //   spx_atomic_add(unsigned long long*, unsigned long long),
+//   spx_pair_sum32 / spx_pair_sum16 / spx_pair_dpp<CTRL, BIT> / spx_self_dpp<CTRL> /
+//   spx_readlane: the butterfly steps of wave_sum_many,
 //   hw_rcp / hw_rsq / hw_exp2 (float) and hw_rcp64 (double).
 // Every value that steers control flow is wave-uniform by construction, so the compiler
 // sees uniform (scalar) branches only. (A lane-dependent branch around a shuffle inside the
@@ -179,11 +181,52 @@ SPX_DEV_INLINE void pixel_pass(const Patch<NPL>& pt, int hs, const Params<P>& pp
     }
 }
 
+// Wave totals of M per-lane values at once (a transposed butterfly). Six pair steps, one per
+// lane-index bit from 5 down to 0, each add two registers over partner lanes and halve the
+// register count; a register left without a partner is added to itself across the bit. After
+// the sixth step every value's wave total sits in the lane whose index is the 6-bit reversal
+// of the value's index, read out with readlane. For M = 26 this takes ~60 VALU operations plus
+// 26 readlanes, against ~10 per value for separate reductions.
+SPX_DEV_INLINE constexpr int bitrev6(int i) {
+    return ((i & 1) << 5) | ((i & 2) << 3) | ((i & 4) << 1) | ((i & 8) >> 1) | ((i & 16) >> 3) | ((i & 32) >> 5);
+}
+
+template <int M>
+SPX_DEV_INLINE void wave_sum_many(const float (&v)[M], float (&out)[M]) {
+    static_assert(M >= 1 && M <= 64, "wave_sum_many: 1..64 values");
+    constexpr int M1 = (M + 1) / 2, M2 = (M1 + 1) / 2, M3 = (M2 + 1) / 2;
+    constexpr int M4 = (M3 + 1) / 2, M5 = (M4 + 1) / 2, M6 = (M5 + 1) / 2;
+    float r1[M1], r2[M2], r3[M3], r4[M4], r5[M5], r6[M6];
+#pragma unroll
+    for (int j = 0; j < M1; ++j) r1[j] = (2 * j + 1 < M) ? spx_pair_sum32(v[2 * j], v[2 * j + 1]) : spx_pair_sum32(v[2 * j], v[2 * j]);
+#pragma unroll
+    for (int j = 0; j < M2; ++j) r2[j] = (2 * j + 1 < M1) ? spx_pair_sum16(r1[2 * j], r1[2 * j + 1]) : spx_pair_sum16(r1[2 * j], r1[2 * j]);
+#pragma unroll
+    for (int j = 0; j < M3; ++j)
+        r3[j] = (2 * j + 1 < M2) ? spx_pair_dpp<0x140, 3>(r2[2 * j], r2[2 * j + 1]) : spx_self_dpp<0x140>(r2[2 * j]);
+#pragma unroll
+    for (int j = 0; j < M4; ++j)
+        r4[j] = (2 * j + 1 < M3) ? spx_pair_dpp<0x141, 2>(r3[2 * j], r3[2 * j + 1]) : spx_self_dpp<0x141>(r3[2 * j]);
+#pragma unroll
+    for (int j = 0; j < M5; ++j)
+        r5[j] = (2 * j + 1 < M4) ? spx_pair_dpp<0x4E, 1>(r4[2 * j], r4[2 * j + 1]) : spx_self_dpp<0x4E>(r4[2 * j]);
+#pragma unroll
+    for (int j = 0; j < M6; ++j)
+        r6[j] = (2 * j + 1 < M5) ? spx_pair_dpp<0xB1, 0>(r5[2 * j], r5[2 * j + 1]) : spx_self_dpp<0xB1>(r5[2 * j]);
+#pragma unroll
+    for (int i = 0; i < M; ++i) out[i] = spx_readlane(r6[0], bitrev6(i));
+}
+
 // Wave totals of every partial sum except the cost (which the caller reduces first).
 template <int P>
 SPX_DEV_INLINE void reduce_sums(const float (&acc)[Layout<P>::kN], float (&S)[Layout<P>::kN]) {
+    constexpr int M = Layout<P>::kCost;
+    float in[M], out[M];
 #pragma unroll
-    for (int i = 0; i < Layout<P>::kCost; ++i) S[i] = spx_wave_sum(acc[i]);
+    for (int i = 0; i < M; ++i) in[i] = acc[i];
+    wave_sum_many<M>(in, out);
+#pragma unroll
+    for (int i = 0; i < M; ++i) S[i] = out[i];
 }
 
 // Solves (JtJ + lambda * diag) delta = J^T r in FP64 by an in-place LDL^T on the packed

@@ -73,6 +73,41 @@ inline double spx_uniform_d(double v) {
     w.bar.arrive_and_wait();
     return r;
 }
+// float read from an arbitrary lane (every lane must call it, like a hardware shuffle)
+inline float spx_shfl_f(float v, int src) {
+    WaveState& w = *t_wave;
+    w.shuffles[t_lane]++;
+    w.fv[t_lane] = v;
+    w.bar.arrive_and_wait();
+    float r = w.fv[src];
+    w.bar.arrive_and_wait();
+    return r;
+}
+// Butterfly steps of wave_sum_many: the first value's partial sum stays in lanes with the
+// step's bit clear, the second's in lanes with it set (partner lanes as the hardware pairs them).
+inline float emu_pair(float a, float b, int bit, int partner) {
+    const bool hi = (t_lane >> bit) & 1;
+    const float send = hi ? a : b;
+    const float keep = hi ? b : a;
+    return keep + spx_shfl_f(send, partner);
+}
+inline float spx_pair_sum32(float a, float b) { return emu_pair(a, b, 5, t_lane ^ 32); }
+inline float spx_pair_sum16(float a, float b) { return emu_pair(a, b, 4, t_lane ^ 16); }
+inline int dpp_partner(int ctrl) {
+    const int l = t_lane;
+    switch (ctrl) {
+        case 0x140: return (l & ~15) | (15 - (l & 15));  // row_mirror
+        case 0x141: return (l & ~7) | (7 - (l & 7));     // row_half_mirror
+        case 0x4E: return l ^ 2;                         // quad_perm [2,3,0,1]
+        default: return l ^ 1;                           // quad_perm [1,0,3,2]
+    }
+}
+template <int CTRL, int BIT>
+inline float spx_pair_dpp(float a, float b) { return emu_pair(a, b, BIT, dpp_partner(CTRL)); }
+template <int CTRL>
+inline float spx_self_dpp(float a) { return a + spx_shfl_f(a, dpp_partner(CTRL)); }
+inline float spx_readlane(float v, int lane) { return spx_shfl_f(v, lane); }
+
 inline float spx_wave_sum(float v) {
     for (int off = 32; off >= 1; off >>= 1) v += spx_shfl_xor(v, off);
     return spx_uniform(v);
