@@ -90,22 +90,29 @@ SPX_DEV_INLINE uint32_t pack_px(int col, int row, uint32_t v) {
 }
 
 // Work of one patch in registers: NPL words per lane, lane l holding flattened pixels
-// l, l + 64, ...; slots below `nfull` are valid in every lane, slots up to `nslots` are
-// checked against `valid` (bit n = slot n), later slots are skipped.
+// l, l + 64, ...; slots up to `nslots` are weighted by `valid` (bit n = slot n), later
+// slots are skipped.
 template <int NPL>
 struct Patch {
     uint32_t px[NPL];
     uint64_t valid;
-    int nfull;
     int nslots;
     float npix;
 };
 
 // One pixel pass at parameters `pp`: this lane's partial sums of J^T J, J^T r and the
-// cost over its pixels (not yet reduced over the wave).
+// cost over its pixels (not yet reduced over the wave), with the Jacobian columns of x0, y0,
+// r, sigma (and q) accumulated without their constant factors, which `sc` returns: the
+// caller scales the reduced sums once per pass instead of every pixel once per pass.
+// Per pixel: w = (rho - r) sqrt(log2 e) / (sqrt2 s), so that e^{-u^2} = 2^{-w^2} with
+// u = (rho - r) / (sqrt2 s); erfc(|u|) / 2 = t p(t) 2^{-w^2} with the A&S coefficients halved
+// and t = 1 / (1 + p |u|); empty lanes of a partly valid slot are zeroed by multiplying
+// the residual, E and 2^{-w^2} by the slot's valid bit.
+constexpr float kSqrtLog2e = 1.2011224087864498f;
+
 template <int NPL, int P>
 SPX_DEV_INLINE void pixel_pass(const Patch<NPL>& pt, int hs, const Params<P>& pp,
-                               float (&acc)[Layout<P>::kN]) {
+                               float (&acc)[Layout<P>::kN], float (&sc)[Layout<P>::kNB]) {
     using L = Layout<P>;
     constexpr bool kEll = (P == 7);
     // dx = col - hs - x0 = col - xo
@@ -113,12 +120,22 @@ SPX_DEV_INLINE void pixel_pass(const Patch<NPL>& pt, int hs, const Params<P>& pp
     const float r = (float)pp.p[2];
     const float s = (float)pp.p[3], A = (float)pp.p[4], B = (float)pp.p[P - 1];
     const float q = kEll ? (float)pp.p[5] : 1.0f;
-    const float inv_s = 1.0f / s;
-    const float k = kInvSqrt2 * inv_s;
-    const float rk = r * k;
-    const float Ac = A * kInvSqrt2Pi * inv_s;
-    const float iq = 1.0f / q;
+    const float inv_s = hw_rcp(s);
+    const float kw = (kInvSqrt2 * kSqrtLog2e) * inv_s;
+    const float rkw = r * kw;
+    const float iq = kEll ? hw_rcp(q) : 1.0f;
     const float iq2 = iq * iq;
+    // true columns: x0, y0, r: Ac * (2^{-w^2} rs dx, 2^{-w^2} rs dy q^-2, 2^{-w^2});
+    // sigma: Ac sqrt2 / sqrt(log2 e) * 2^{-w^2} w; q: Ac * 2^{-w^2} rs dy^2 q^-3
+    const float Ac = A * kInvSqrt2Pi * inv_s;
+    sc[0] = Ac;
+    sc[1] = Ac;
+    sc[2] = Ac;
+    sc[3] = Ac * (kSqrt2 / kSqrtLog2e);
+    sc[4] = 1.0f;
+    if constexpr (kEll) sc[5] = Ac;
+    constexpr float kP = kAsP / kSqrtLog2e;
+    constexpr float h1 = 0.5f * kAs1, h2 = 0.5f * kAs2, h3 = 0.5f * kAs3, h4 = 0.5f * kAs4, h5 = 0.5f * kAs5;
 
 #pragma unroll
     for (int i = 0; i < L::kN; ++i) acc[i] = 0.0f;
@@ -131,49 +148,40 @@ SPX_DEV_INLINE void pixel_pass(const Patch<NPL>& pt, int hs, const Params<P>& pp
             if (n >= pt.nslots) break;
         }
         if (n < pt.nslots) {
-            const uint32_t w = pt.px[n];
-            const float dx = (float)(w & 0xffu) - xo;
-            const float dy = (float)((w >> 8) & 0xffu) - yo;
-            const float val = (float)(w >> 16);
+            const uint32_t wd = pt.px[n];
+            const float dx = (float)(wd & 0xffu) - xo;
+            const float dy = (float)((wd >> 8) & 0xffu) - yo;
+            const float val = (float)(wd >> 16);
             const float dyq = kEll ? dy * iq2 : dy;
-            float rho2 = fmaf(dy, dyq, dx * dx);
-            rho2 = fmaxf(rho2, 1e-20f);
+            const float rho2 = fmaf(dy, dyq, fmaf(dx, dx, 1e-20f));
             const float rs = hw_rsq(rho2);
             const float rho = rho2 * rs;
-            const float u = fmaf(rho, k, -rk);
-            const float t = hw_rcp(fmaf(kAsP, fabsf(u), 1.0f));
-            const float ez = hw_exp2(-(u * u) * kLog2e);
-            const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, kAs5, kAs4), kAs3), kAs2), kAs1);
-            const float hq = 0.5f * poly * ez;
-            float E = (u >= 0.0f) ? hq : 1.0f - hq;
-            float res = val - fmaf(A, E, B);
-            float G = Ac * ez;
-            if (n >= pt.nfull) {
-                // partly valid slot: zero the residual and the Jacobian of empty lanes
-                const float wm = ((pt.valid >> n) & 1ull) ? 1.0f : 0.0f;
-                res *= wm;
-                E *= wm;
-                G *= wm;
-            }
-            const float Gr = G * rs;
+            const float w = fmaf(rho, kw, -rkw);
+            const float t = hw_rcp(fmaf(kP, fabsf(w), 1.0f));
+            const float wm = (float)((uint32_t)(pt.valid >> n) & 1u);
+            const float ez = hw_exp2(-(w * w)) * wm;
+            const float hq = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, h5, h4), h3), h2), h1) * ez;
+            const float E = (w >= 0.0f) ? hq : wm - hq;
+            const float res = fmaf(val, wm, -fmaf(A, E, B * wm));
+            const float Gr = ez * rs;
             float J[P - 1];
             J[0] = Gr * dx;
             J[1] = Gr * dyq;
-            J[2] = G;
-            J[3] = G * u * kSqrt2;
+            J[2] = ez;
+            J[3] = ez * w;
             J[4] = E;
             if constexpr (kEll) J[5] = Gr * dy * dyq * iq;
             int idx = 0;
-    #pragma unroll
+#pragma unroll
             for (int a = 0; a < L::kNB; ++a)
-    #pragma unroll
+#pragma unroll
                 for (int b = a; b < L::kNB; ++b) {
                     acc[idx] = fmaf(J[a], J[b], acc[idx]);
                     ++idx;
                 }
-    #pragma unroll
+#pragma unroll
             for (int a = 0; a < L::kNB; ++a) acc[L::kColB + a] += J[a];
-    #pragma unroll
+#pragma unroll
             for (int a = 0; a < L::kNB; ++a) acc[L::kRhs + a] = fmaf(J[a], res, acc[L::kRhs + a]);
             acc[L::kRhs + P - 1] += res;
             acc[L::kCost] = fmaf(res, res, acc[L::kCost]);
@@ -217,16 +225,30 @@ SPX_DEV_INLINE void wave_sum_many(const float (&v)[M], float (&out)[M]) {
     for (int i = 0; i < M; ++i) out[i] = spx_readlane(r6[0], bitrev6(i));
 }
 
-// Wave totals of every partial sum except the cost (which the caller reduces first).
+// Wave totals of every partial sum except the cost (which the caller reduces first), with
+// the constant column factors `sc` of the pass applied.
 template <int P>
-SPX_DEV_INLINE void reduce_sums(const float (&acc)[Layout<P>::kN], float (&S)[Layout<P>::kN]) {
-    constexpr int M = Layout<P>::kCost;
+SPX_DEV_INLINE void reduce_sums(const float (&acc)[Layout<P>::kN], const float (&sc)[Layout<P>::kNB],
+                                float (&S)[Layout<P>::kN]) {
+    using L = Layout<P>;
+    constexpr int M = L::kCost;
     float in[M], out[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) in[i] = acc[i];
     wave_sum_many<M>(in, out);
+    int idx = 0;
 #pragma unroll
-    for (int i = 0; i < M; ++i) S[i] = out[i];
+    for (int a = 0; a < L::kNB; ++a)
+#pragma unroll
+        for (int b = a; b < L::kNB; ++b) {
+            S[idx] = out[idx] * (sc[a] * sc[b]);
+            ++idx;
+        }
+#pragma unroll
+    for (int a = 0; a < L::kNB; ++a) S[L::kColB + a] = out[L::kColB + a] * sc[a];
+#pragma unroll
+    for (int a = 0; a < L::kNB; ++a) S[L::kRhs + a] = out[L::kRhs + a] * sc[a];
+    S[L::kRhs + P - 1] = out[L::kRhs + P - 1];
 }
 
 // Solves (JtJ + lambda * diag) delta = J^T r in FP64 by an in-place LDL^T on the packed
@@ -468,10 +490,10 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
         } else {
             float S[L::kN];
             {
-                float acc[L::kN];
-                pixel_pass<NPL, P>(pt, hs, cur, acc);
+                float acc[L::kN], sc[L::kNB];
+                pixel_pass<NPL, P>(pt, hs, cur, acc, sc);
                 S[L::kCost] = spx_wave_sum(acc[L::kCost]);
-                reduce_sums<P>(acc, S);
+                reduce_sums<P>(acc, sc, S);
             }
             ++passes;
             float cost = S[L::kCost];
@@ -502,8 +524,8 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
                     lambda *= (double)a.opt.lambda_up;
                     continue;
                 }
-                float acc[L::kN];
-                pixel_pass<NPL, P>(pt, hs, tr, acc);
+                float acc[L::kN], sc[L::kNB];
+                pixel_pass<NPL, P>(pt, hs, tr, acc, sc);
                 ++passes;
                 const float cost_t = spx_wave_sum(acc[L::kCost]);
                 // A trial within the FP32 rounding floor of the cost (1e-6 relative) is
@@ -511,7 +533,7 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
                 // differences drown in rounding. Only an accepted trial's sums are reduced.
                 if (cost_t <= cost * (1.0f + kAcceptRtol)) {
                     cur = tr;
-                    reduce_sums<P>(acc, S);
+                    reduce_sums<P>(acc, sc, S);
                     S[L::kCost] = cost_t;
                     cost = cost_t;
                     lambda = fmax(lambda * (double)a.opt.lambda_down, 1e-12);
@@ -580,7 +602,6 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
         const float cy = spx_uniform(rintf(a.seeds[2 * (int64_t)f + 1]));
         const int64_t ox = (int64_t)cx - hs;
         const int64_t oy = (int64_t)cy - hs;
-        const bool clipped = ox < 0 || oy < 0 || ox + side > a.W || oy + side > a.H;
 
         bool masked = false;
         if (a.nbr != nullptr)
@@ -593,7 +614,6 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
         // few contiguous row segments of the image.
         Patch<NPL> pt;
         pt.nslots = (npx + kWave - 1) / kWave;
-        pt.nfull = (clipped || masked) ? 0 : npx / kWave;
         pt.valid = 0ull;
         float cnt = 0.0f;
 #pragma unroll
