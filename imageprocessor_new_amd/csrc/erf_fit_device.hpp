@@ -524,6 +524,14 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
                     lambda *= (double)a.opt.lambda_up;
                     continue;
                 }
+                // A step below the tolerance ends the fit and is applied without a trial
+                // pass: taken or refused, it moves the result by less than step_tol, so the
+                // pass would decide nothing (the rms stays that of the point before it).
+                if (step < (double)a.opt.step_tol) {
+                    cur = tr;
+                    status = SPX_FIT_CONVERGED;
+                    break;
+                }
                 float acc[L::kN], sc[L::kNB];
                 pixel_pass<NPL, P>(pt, hs, tr, acc, sc);
                 ++passes;
@@ -537,10 +545,8 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
                     S[L::kCost] = cost_t;
                     cost = cost_t;
                     lambda = fmax(lambda * (double)a.opt.lambda_down, 1e-12);
-                    if (step < (double)a.opt.step_tol) status = SPX_FIT_CONVERGED;
                 } else {
                     lambda *= (double)a.opt.lambda_up;
-                    if (step < (double)a.opt.step_tol) status = SPX_FIT_CONVERGED;
                 }
             }
             if (!any_solve && status == SPX_FIT_MAX_ITER) status = SPX_FIT_SINGULAR;

@@ -33,7 +33,7 @@ typedef void* spx_stream_t; /* a hipStream_t; NULL = null stream */
 #define SPX_E_CAPACITY (-5)    /* more components than the caller's capacity          */
 
 /* ---- per-feature fit status (status_out bytes) ----------------------------------- */
-#define SPX_FIT_CONVERGED 0    /* a step below step_tol was taken                      */
+#define SPX_FIT_CONVERGED 0    /* a step below step_tol was reached (and applied)      */
 #define SPX_FIT_MAX_ITER 1     /* iteration cap reached; parameters are the best seen  */
 #define SPX_FIT_SINGULAR 2     /* the damped normal matrix stayed singular            */
 #define SPX_FIT_DIVERGED 3     /* non-finite cost or parameters                       */
@@ -48,7 +48,8 @@ typedef void* spx_stream_t; /* a hipStream_t; NULL = null stream */
 /* Options of the batched Levenberg-Marquardt erf fits. */
 typedef struct SpxFitOptions {
     int32_t max_iter;     /* LM step attempts per feature (30)                         */
-    float step_tol;       /* converged when an accepted step moves x0,y0,r,s < this (1e-6 px) */
+    float step_tol;       /* converged when a solved step moves x0,y0,r,s < this (1e-6 px);
+                             that step is applied without a trial pass */
     float lambda0;        /* initial Marquardt damping (1e-3)                          */
     float lambda_up;      /* damping factor after a rejected step (10)                 */
     float lambda_down;    /* damping factor after an accepted step (0.1)               */

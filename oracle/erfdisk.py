@@ -20,10 +20,12 @@ half the peak above B; centre = weighted centroid of the in-set; r = sqrt(n_in /
 A = mean of (v - B) over the in-set; s = init_sigma; q = 1.
 LM: Marquardt damping M_aa += lambda * max(M_aa, 1e-12 max_a M_aa); LDL^T solve; a
 trial is refused (lambda *= up) when it leaves |x0|,|y0| <= h + 0.5,
-0.25 < r < 2h + 1, 0.05 < s < h + 0.5 (and 0.25 < q < 4), otherwise accepted when its cost is at most
-cost * (1 + accept_rtol) (lambda *= down, floor 1e-12) and refused otherwise; the fit
-stops with status "converged" when a step with max(|dx0|, |dy0|, |dr|, |ds|, r |dq|) <
-step_tol was taken or refused, else after max_iter attempts with "max_iter".
+0.25 < r < 2h + 1, 0.05 < s < h + 0.5 (and 0.25 < q < 4); a step with
+max(|dx0|, |dy0|, |dr|, |ds|, r |dq|) < step_tol ends the fit with status "converged" and is
+applied without a trial evaluation (the reported rms is that of the point before it);
+otherwise the trial is accepted when its cost is at most cost * (1 + accept_rtol)
+(lambda *= down, floor 1e-12) and refused otherwise; after max_iter attempts the status is
+"max_iter".
 
 Neighbour (Voronoi cell) mask, when neighbour lists are given: a patch pixel takes part
 only when it is strictly nearer to the own seed than to each listed neighbour seed, ties
@@ -214,16 +216,18 @@ def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, cell_edge=None, ma
         if not inside:
             lam *= lambda_up
             continue
+        if step < step_tol:
+            # a step below the tolerance ends the fit: it is applied without a trial
+            # evaluation (the cost, hence the rms, stays that of the point before it)
+            p = tr
+            status = CONVERGED
+            break
         JTJ_t, JTr_t, cost_t = evaluate(tr)
         if cost_t <= cost * (1.0 + accept_rtol):
             p, JTJ, JTr, cost = tr, JTJ_t, JTr_t, cost_t
             lam = max(lam * lambda_down, 1e-12)
-            if step < step_tol:
-                status = CONVERGED
         else:
             lam *= lambda_up
-            if step < step_tol:
-                status = CONVERGED
     if not any_solve and status == MAX_ITER:
         status = SINGULAR
     if not np.isfinite(cost):
