@@ -29,6 +29,20 @@ if ROOT not in sys.path:
 FLOP_PER_PIXEL_PASS = {6: 90.0, 7: 110.0}  # model + Jacobian + J^T J + J^T r + cost
 PEAK_FP32_TFLOPS = 157.3                    # MI355X vendor FP32 peak (packed)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "erf_fit_c1_kernel_stats.md")
+
+
+def measured_traffic():
+    """HBM bytes per fit launch of the full workload from the committed rocprofv3 counter
+    summary (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated against a 256 MiB copy), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def log(msg: str) -> None:
@@ -244,7 +258,8 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic() if args.size == 8192 else None,
+            "algorithmic_bytes_per_launch": float(K) * (31 * 31 * 2 + 32),
             "kernel_ms": kernel_ms,
             "algorithmic_flop_per_launch": flops,
         },
