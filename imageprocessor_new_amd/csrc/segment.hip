@@ -280,27 +280,57 @@ __global__ void cc_acc_init_kernel(CCAcc* acc, int64_t n) {
     acc[i] = CCAcc{0x7fffffff, 0x7fffffff, -1, -1, 0u, 0u, 0ull, 0ull};
 }
 
-// Per node: write the labels of its pixels and add its pixels to the label's stats
-// (one set of atomics per node). Background pixels go to row 0 through a workgroup
-// reduction.
+// Wave reductions for the per-label aggregation (every lane takes part; lanes outside the
+// current label contribute the identity). The result is uniform (first lane's value).
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, off, 64), hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Labels and statistics. A block covers 16 x 16 nodes, each wave an 8 x 8 node tile
+// (16 x 16 pixels for 8-connectivity), so a wave holds few distinct components. Each node
+// writes its pixels' labels and forms its partial stats; the wave then loops over its
+// distinct labels (leader = first pending lane), reduces the members' partials and the
+// leader issues one set of atomics per label: a few per tile instead of seven per node.
+// Background pixels (label 0) are reduced per block into bg_part[block] (no atomics);
+// cc_bg_reduce_kernel folds those into row 0.
+constexpr int kLabTile = 16;  // nodes per block side
+
 template <int BS>
 __global__ __launch_bounds__(256) void cc_label_kernel(CCGeom g, const int32_t* parent,
                                                        const uint8_t* bits, const int32_t* ids,
-                                                       int32_t* labels, CCAcc* acc, int32_t cap) {
-    __shared__ int32_t s_min[2][256];
-    __shared__ int32_t s_max[2][256];
-    __shared__ unsigned long long s_sum[3][256];
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                       int32_t* labels, CCAcc* acc, int32_t cap,
+                                                       CCAcc* bg_part, int64_t tiles_x) {
+    __shared__ CCAcc s_bg[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t bx = (int64_t)blockIdx.x % tiles_x, by = (int64_t)blockIdx.x / tiles_x;
+    const int64_t nx = bx * kLabTile + (wv & 1) * 8 + (lane & 7);
+    const int64_t ny = by * kLabTile + (wv >> 1) * 8 + (lane >> 3);
     int32_t bxmin = 0x7fffffff, bymin = 0x7fffffff, bxmax = -1, bymax = -1;
     unsigned long long bcnt = 0, bsx = 0, bsy = 0;
-    if (n < g.NH * g.NW) {
+    int32_t lab = 0;
+    int32_t xmin = 0x7fffffff, ymin = 0x7fffffff, xmax = -1, ymax = -1;
+    uint32_t cnt = 0;
+    unsigned long long sx = 0, sy = 0;
+    if (nx < g.NW && ny < g.NH) {
+        const int64_t n = ny * g.NW + nx;
         const uint32_t b = bits[n];
-        const int64_t ny = n / g.NW, nx = n - (n / g.NW) * g.NW;
-        int32_t lab = 0;
         if (b) lab = ids[parent[n]] + 1;
-        int32_t xmin = 0x7fffffff, ymin = 0x7fffffff, xmax = -1, ymax = -1;
-        uint32_t cnt = 0;
-        unsigned long long sx = 0, sy = 0;
         for (int dy = 0; dy < BS; ++dy) {
             for (int dx = 0; dx < BS; ++dx) {
                 const int64_t y = ny * BS + dy, x = nx * BS + dx;
@@ -326,47 +356,93 @@ __global__ __launch_bounds__(256) void cc_label_kernel(CCGeom g, const int32_t* 
                 }
             }
         }
-        if (cnt && lab < cap) {
-            CCAcc* a = acc + lab;
-            atomicMin(&a->xmin, xmin);
-            atomicMin(&a->ymin, ymin);
-            atomicMax(&a->xmax, xmax);
-            atomicMax(&a->ymax, ymax);
-            atomicAdd(&a->area, cnt);
-            atomicAdd(&a->sx, sx);
-            atomicAdd(&a->sy, sy);
-        }
     }
-    const int t = threadIdx.x;
-    s_min[0][t] = bxmin;
-    s_min[1][t] = bymin;
-    s_max[0][t] = bxmax;
-    s_max[1][t] = bymax;
-    s_sum[0][t] = bcnt;
-    s_sum[1][t] = bsx;
-    s_sum[2][t] = bsy;
+    // foreground: one aggregation round per distinct label of the wave
+    bool pending = cnt > 0 && lab < cap;
+    for (;;) {
+        const unsigned long long m = __ballot(pending);
+        if (m == 0ull) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const int32_t L = __builtin_amdgcn_readlane(lab, leader);
+        const bool mine = pending && lab == L;
+        const int32_t rxmin = wave_min_i32(mine ? xmin : 0x7fffffff);
+        const int32_t rymin = wave_min_i32(mine ? ymin : 0x7fffffff);
+        const int32_t rxmax = wave_max_i32(mine ? xmax : -1);
+        const int32_t rymax = wave_max_i32(mine ? ymax : -1);
+        const unsigned long long rcnt = wave_sum_u64(mine ? (unsigned long long)cnt : 0ull);
+        const unsigned long long rsx = wave_sum_u64(mine ? sx : 0ull);
+        const unsigned long long rsy = wave_sum_u64(mine ? sy : 0ull);
+        if (lane == leader) {
+            CCAcc* a = acc + L;
+            atomicMin(&a->xmin, rxmin);
+            atomicMin(&a->ymin, rymin);
+            atomicMax(&a->xmax, rxmax);
+            atomicMax(&a->ymax, rymax);
+            atomicAdd(&a->area, (uint32_t)rcnt);
+            atomicAdd(&a->sx, rsx);
+            atomicAdd(&a->sy, rsy);
+        }
+        pending = pending && !mine;
+    }
+    // background: per-wave reduction, then the block's four waves into bg_part[block]
+    const int32_t wxmin = wave_min_i32(bxmin), wymin = wave_min_i32(bymin);
+    const int32_t wxmax = wave_max_i32(bxmax), wymax = wave_max_i32(bymax);
+    const unsigned long long wcnt = wave_sum_u64(bcnt), wsx = wave_sum_u64(bsx), wsy = wave_sum_u64(bsy);
+    if (lane == 0) s_bg[wv] = CCAcc{wxmin, wymin, wxmax, wymax, (uint32_t)wcnt, 0u, wsx, wsy};
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x == 0) {
+        CCAcc r = s_bg[0];
+        for (int k = 1; k < 4; ++k) {
+            r.xmin = min(r.xmin, s_bg[k].xmin);
+            r.ymin = min(r.ymin, s_bg[k].ymin);
+            r.xmax = max(r.xmax, s_bg[k].xmax);
+            r.ymax = max(r.ymax, s_bg[k].ymax);
+            r.area += s_bg[k].area;
+            r.sx += s_bg[k].sx;
+            r.sy += s_bg[k].sy;
+        }
+        bg_part[blockIdx.x] = r;
+    }
+}
+
+// Row 0 (background) from the per-block partials: one block, grid-stride over the parts.
+__global__ __launch_bounds__(1024) void cc_bg_reduce_kernel(const CCAcc* part, int64_t n, CCAcc* acc0) {
+    __shared__ int32_t s_i[4][1024];
+    __shared__ unsigned long long s_u[3][1024];
+    const int t = threadIdx.x;
+    int32_t xmin = 0x7fffffff, ymin = 0x7fffffff, xmax = -1, ymax = -1;
+    unsigned long long area = 0, sx = 0, sy = 0;
+    for (int64_t i = t; i < n; i += 1024) {
+        const CCAcc p = part[i];
+        xmin = min(xmin, p.xmin);
+        ymin = min(ymin, p.ymin);
+        xmax = max(xmax, p.xmax);
+        ymax = max(ymax, p.ymax);
+        area += p.area;
+        sx += p.sx;
+        sy += p.sy;
+    }
+    s_i[0][t] = xmin;
+    s_i[1][t] = ymin;
+    s_i[2][t] = xmax;
+    s_i[3][t] = ymax;
+    s_u[0][t] = area;
+    s_u[1][t] = sx;
+    s_u[2][t] = sy;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
         if (t < off) {
-            s_min[0][t] = min(s_min[0][t], s_min[0][t + off]);
-            s_min[1][t] = min(s_min[1][t], s_min[1][t + off]);
-            s_max[0][t] = max(s_max[0][t], s_max[0][t + off]);
-            s_max[1][t] = max(s_max[1][t], s_max[1][t + off]);
-            s_sum[0][t] += s_sum[0][t + off];
-            s_sum[1][t] += s_sum[1][t + off];
-            s_sum[2][t] += s_sum[2][t + off];
+            s_i[0][t] = min(s_i[0][t], s_i[0][t + off]);
+            s_i[1][t] = min(s_i[1][t], s_i[1][t + off]);
+            s_i[2][t] = max(s_i[2][t], s_i[2][t + off]);
+            s_i[3][t] = max(s_i[3][t], s_i[3][t + off]);
+            s_u[0][t] += s_u[0][t + off];
+            s_u[1][t] += s_u[1][t + off];
+            s_u[2][t] += s_u[2][t + off];
         }
         __syncthreads();
     }
-    if (t == 0 && s_sum[0][0] && cap > 0) {
-        atomicMin(&acc[0].xmin, s_min[0][0]);
-        atomicMin(&acc[0].ymin, s_min[1][0]);
-        atomicMax(&acc[0].xmax, s_max[0][0]);
-        atomicMax(&acc[0].ymax, s_max[1][0]);
-        atomicAdd(&acc[0].area, (uint32_t)s_sum[0][0]);
-        atomicAdd(&acc[0].sx, s_sum[1][0]);
-        atomicAdd(&acc[0].sy, s_sum[2][0]);
-    }
+    if (t == 0) *acc0 = CCAcc{s_i[0][0], s_i[1][0], s_i[2][0], s_i[3][0], (uint32_t)s_u[0][0], 0u, s_u[1][0], s_u[2][0]};
 }
 
 __global__ void cc_finalize_kernel(const CCAcc* acc, int32_t rows, int32_t* stats, double* cents) {
@@ -412,7 +488,10 @@ int threshold_cc_impl(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, 
     const size_t o_acc = o_ids + align(4 * nn);
     const size_t n_acc = (size_t)(cap > 0 ? cap : 1);
     const size_t o_bits = o_acc + align(sizeof(CCAcc) * n_acc);
-    const size_t o_scan = o_bits + align(nn);
+    const int64_t tiles_x = (g.NW + kLabTile - 1) / kLabTile;
+    const int64_t tiles = tiles_x * ((g.NH + kLabTile - 1) / kLabTile);
+    const size_t o_bg = o_bits + align(nn);
+    const size_t o_scan = o_bg + align(sizeof(CCAcc) * tiles);
     const size_t total = o_scan + align(scan_bytes) + 256;
     void* ws = nullptr;
     int rc = workspace(total, &ws);
@@ -423,6 +502,7 @@ int threshold_cc_impl(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, 
     int32_t* ids = reinterpret_cast<int32_t*>(base + o_ids);
     CCAcc* acc = reinterpret_cast<CCAcc*>(base + o_acc);
     uint8_t* bits = reinterpret_cast<uint8_t*>(base + o_bits);
+    CCAcc* bg_part = reinterpret_cast<CCAcc*>(base + o_bg);
     void* scan_tmp = base + o_scan;
 
     const int T = 256;
@@ -435,8 +515,13 @@ int threshold_cc_impl(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, 
     SPX_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, flag, ids, (int)nn, stream));
     cc_acc_init_kernel<<<grid_for((int64_t)n_acc, T), T, 0, stream>>>(acc, (int64_t)n_acc);
     SPX_LAUNCHED();
-    cc_label_kernel<BS><<<grid_for(nn, T), T, 0, stream>>>(g, parent, bits, ids, labels, acc, cap);
+    SPX_REQUIRE(tiles < (int64_t)0x7fffffff, "spx_threshold_cc: image too large");
+    cc_label_kernel<BS><<<(unsigned)tiles, T, 0, stream>>>(g, parent, bits, ids, labels, acc, cap, bg_part, tiles_x);
     SPX_LAUNCHED();
+    if (cap > 0) {
+        cc_bg_reduce_kernel<<<1, 1024, 0, stream>>>(bg_part, tiles, acc);
+        SPX_LAUNCHED();
+    }
     // number of components = ids[nn-1] + flag[nn-1]
     int32_t tail[2] = {0, 0};
     SPX_HIP(hipMemcpyAsync(&tail[0], ids + nn - 1, 4, hipMemcpyDeviceToHost, stream));
