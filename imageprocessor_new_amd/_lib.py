@@ -146,6 +146,8 @@ SIGNATURES = {
     "spx_edge_moment": (_I32, [_P, _I32, _I64, _I64, _I64, _P, _I32, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
     "spx_edge_erf": (_I32, [_P, _I32, _I64, _I64, _I64, _P, _I32, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
     "spx_edge_profile_erf": (_I32, [_P, _I32, _I32, _P, _P, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
+    "spx_render_page": (_I32, [ctypes.c_uint64, _P, _I32, _I64, _I64, ctypes.c_float, ctypes.c_float, _P, _I64,
+                               _P]),
     "spx_render_tiles": (_I32, [ctypes.c_uint64, _I32, _I32, ctypes.POINTER(SpxTileSpec), _P, _I64, _P, _P]),
     "spx_fit_batch": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions),
                              _I32, _P, _P, _P, _P, _P]),

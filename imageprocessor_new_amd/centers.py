@@ -27,6 +27,7 @@ accepted, finite centres.
 
 from __future__ import annotations
 
+import time
 from typing import Sequence, Tuple, Union
 
 import numpy as np
@@ -113,7 +114,9 @@ def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit
 def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, refine, connectivity,
                         pre_threshold, fit_half, fit_options, small_feature_max=12.0, logquad_variant="gpu",
                         morph_open=0, morph_close=0, fit_cells=True):
-    """Segmentation -> CC -> area filter -> refinement on one device image (or strip view)."""
+    """Segmentation -> CC -> area filter -> refinement on one device image (or strip view).
+    info["stage_ms"] holds wall times of the stages (each ends with its results on the host)."""
+    t0 = time.perf_counter()
     if pre_threshold is None:
         if threshold == "otsu":
             thresh = otsu_threshold_device(img_t)
@@ -123,14 +126,18 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
             raise ValueError(f"Unsupported threshold method: {threshold!r}. Supported: ['otsu', 'triangle']")
     else:
         thresh = int(np.floor(pre_threshold))
+    t1 = time.perf_counter()
+    # capacity for a component per 256 px up front: one labelling pass on hole arrays
+    cap = int(min(max(65536, img_t.shape[0] * img_t.shape[1] // 256), 1 << 24))
     if morph_open > 0 or morph_close > 0:
         mask = binary_mask_device(img_t, thresh, invert=invert, morph_open=morph_open, morph_close=morph_close)
-        num, labels, stats, cents = threshold_components_device(mask, 0, connectivity=connectivity)
+        num, labels, stats, cents = threshold_components_device(mask, 0, connectivity=connectivity, capacity=cap)
     else:
         num, labels, stats, cents = threshold_components_device(img_t, thresh, invert=invert,
-                                                                connectivity=connectivity)
+                                                                connectivity=connectivity, capacity=cap)
     stats_np = stats.cpu().numpy()
     cents_np = cents.cpu().numpy()
+    t2 = time.perf_counter()
     area = stats_np[:, 4]
     keep = np.nonzero((area >= int(area_min)) & (area <= int(area_max)))[0]
     keep = keep[keep >= 1]
@@ -138,6 +145,7 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
     methods = _row_methods(refine, rows, small_feature_max)
     out = []
     info = {"threshold_value": thresh, "num_labels": num, "n_candidates": int(keep.size)}
+    t3 = time.perf_counter()
     n_refined = 0
     for method in _GROUP_ORDER:
         idx = np.nonzero(methods == method)[0]
@@ -155,6 +163,9 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
             info["fit_status_counts"] = {int(s): int((st == s).sum()) for s in np.unique(st)}
     info["n_refined"] = n_refined
     centers = np.concatenate(out, axis=0) if out else np.zeros((0, 2), dtype=np.float64)
+    t4 = time.perf_counter()
+    info["stage_ms"] = {"threshold": 1e3 * (t1 - t0), "components": 1e3 * (t2 - t1), "rows": 1e3 * (t3 - t2),
+                        "refine": 1e3 * (t4 - t3)}
     return centers, info
 
 
@@ -245,9 +256,13 @@ def detect_centers(
         if overlap >= tile_h:
             raise ValueError("overlap must be < tile_h")
         pre_threshold = None
+        stage_ms = {"global_threshold": 0.0, "threshold": 0.0, "components": 0.0, "rows": 0.0, "refine": 0.0,
+                    "band": 0.0, "dedupe": 0.0}
+        t0 = time.perf_counter()
         if threshold_mode in ("global", "auto"):
             ds = max(1, int(otsu_downsample))
             pre_threshold = float(otsu_threshold_device(img_t, ds)) * float(thr_scale)
+        stage_ms["global_threshold"] = 1e3 * (time.perf_counter() - t0)
         step = tile_h - int(overlap)
         all_centers = []
         n_tiles = 0
@@ -256,7 +271,10 @@ def detect_centers(
             y1 = min(H, y0 + tile_h)
             tile = img_t[y0:y1]
             tile_thr = None if threshold_mode == "per_tile" else pre_threshold
-            centers, _ = _detect_single_tile(tile, pre_threshold=tile_thr, **common)
+            centers, tinfo = _detect_single_tile(tile, pre_threshold=tile_thr, **common)
+            for k, v in tinfo["stage_ms"].items():
+                stage_ms[k] += v
+            tb = time.perf_counter()
             n_tiles += 1
             if centers.shape[0] > 0:
                 centers = centers.copy()
@@ -267,13 +285,20 @@ def detect_centers(
                 m = (centers[:, 1] >= keep_lo) & (centers[:, 1] < keep_hi)
                 if np.any(m):
                     all_centers.append(centers[m])
+            stage_ms["band"] += 1e3 * (time.perf_counter() - tb)
             if y1 == H:
                 break
             y0 += step
-    if all_centers:
-        centers = dedupe_centers(np.vstack(all_centers), eps=float(dedupe_eps))
-    else:
-        centers = np.zeros((0, 2), dtype=np.float64)
+        t0 = time.perf_counter()
+        if all_centers:
+            pts = np.vstack(all_centers)
+            pairs = _candidate_pairs_device(pts, float(dedupe_eps), (H, W))
+            if pairs is None:
+                pairs = _candidate_pairs_kdtree(pts, float(dedupe_eps))
+            centers = _merge_pairs(pts, pairs, float(dedupe_eps))
+        else:
+            centers = np.zeros((0, 2), dtype=np.float64)
+        stage_ms["dedupe"] = 1e3 * (time.perf_counter() - t0)
     return CenterResult(
         centers_xy=centers,
         method=f"tiled_gpu(refine={refine})",
@@ -284,7 +309,7 @@ def detect_centers(
             "dedupe_eps": float(dedupe_eps), "refine": str(refine), "invert": bool(invert),
             "area_min": int(area_min), "area_max": int(area_max), "device": int(device),
             "small_feature_max": float(small_feature_max), "upsample_factor": int(upsample_factor),
-            "n_tiles": n_tiles, "threshold_value": pre_threshold,
+            "n_tiles": n_tiles, "threshold_value": pre_threshold, "stage_ms": stage_ms,
         },
     )
 
@@ -311,44 +336,69 @@ def dedupe_centers(centers_xy, *, eps: float = 1.5) -> np.ndarray:
     union-find is replayed over them in the reference's (i, j) order.
     """
     pts = np.asarray(centers_xy, dtype=np.float64)
-    n = pts.shape[0]
-    if n == 0:
+    if pts.shape[0] == 0:
         return pts.reshape(0, 2)
+    return _merge_pairs(pts, _candidate_pairs_kdtree(pts, eps), eps)
+
+
+def _candidate_pairs_kdtree(pts: np.ndarray, eps: float) -> np.ndarray:
     from scipy.spatial import cKDTree
 
+    return cKDTree(pts).query_pairs(r=float(eps) * (1.0 + 1e-9) + 1e-12, output_type="ndarray")
+
+
+def _candidate_pairs_device(pts: np.ndarray, eps: float, shape) -> np.ndarray | None:
+    """Candidate pairs (i < j) from the device neighbour grid (spx_seed_neighbours) on float32
+    coordinates, with the radius padded past their rounding (< 0.01 px below 2^16 px); None
+    when a neighbour list is full (it may have dropped a candidate)."""
+    from .fit import seed_neighbours_device
+
+    nb = 8
+    dev = torch.device("cuda", torch.cuda.current_device())
+    s = torch.from_numpy(np.ascontiguousarray(pts, dtype=np.float32)).to(dev)
+    nbr = seed_neighbours_device(s, shape, float(eps) + 0.02, nb).cpu().numpy()
+    if (nbr[:, nb - 1] >= 0).any():
+        return None
+    i, k = np.nonzero(nbr >= 0)
+    j = nbr[i, k].astype(np.int64)
+    keep = j > i
+    return np.stack([i[keep].astype(np.int64), j[keep]], axis=1).reshape(-1, 2)
+
+
+def _merge_pairs(pts: np.ndarray, pairs: np.ndarray, eps: float) -> np.ndarray:
+    """The reference's union-find over the exact pairs within eps, then cluster means."""
+    n = pts.shape[0]
     eps2 = float(eps) ** 2
-    pairs = cKDTree(pts).query_pairs(r=float(eps) * (1.0 + 1e-9) + 1e-12, output_type="ndarray")
     parent = np.arange(n, dtype=np.int64)
     if pairs.size:
         pairs = np.sort(pairs, axis=1)
         d2 = np.sum((pts[pairs[:, 1]] - pts[pairs[:, 0]]) ** 2, axis=1)
         pairs = pairs[d2 <= eps2]
-        order = np.lexsort((pairs[:, 1], pairs[:, 0]))
-        pairs = pairs[order]
-        rank = np.zeros(n, dtype=np.int8)
-        par = parent.tolist()
-        rk = rank.tolist()
+    if pairs.size == 0:
+        return pts.copy()
+    order = np.lexsort((pairs[:, 1], pairs[:, 0]))
+    pairs = pairs[order]
+    par = parent.tolist()
+    rk = [0] * n
 
-        def find(a):
-            while par[a] != a:
-                par[a] = par[par[a]]
-                a = par[a]
-            return a
+    def find(a):
+        while par[a] != a:
+            par[a] = par[par[a]]
+            a = par[a]
+        return a
 
-        for i, j in pairs.tolist():
-            ra, rb = find(i), find(j)
-            if ra == rb:
-                continue
-            if rk[ra] < rk[rb]:
-                par[ra] = rb
-            elif rk[ra] > rk[rb]:
-                par[rb] = ra
-            else:
-                par[rb] = ra
-                rk[ra] += 1
-        roots = np.array([find(i) for i in range(n)], dtype=np.int64)
-    else:
-        roots = parent
+    for i, j in pairs.tolist():
+        ra, rb = find(i), find(j)
+        if ra == rb:
+            continue
+        if rk[ra] < rk[rb]:
+            par[ra] = rb
+        elif rk[ra] > rk[rb]:
+            par[rb] = ra
+        else:
+            par[rb] = ra
+            rk[ra] += 1
+    roots = np.array([find(i) for i in range(n)], dtype=np.int64)
     unique_roots, inv = np.unique(roots, return_inverse=True)
     K = unique_roots.size
     sums = np.zeros((K, 2), dtype=np.float64)

@@ -341,36 +341,65 @@ struct FitArgs {
     int32_t nfeat;       // K: ids outside [0, K) are ignored
 };
 
-// Voronoi cell of a feature against its listed neighbour seeds. A pixel belongs to the cell
-// when it is strictly nearer to the own seed than to every neighbour, ties going to the
-// lower feature index (reference subpx/_cpu/cells.py:8-33). Squared distances are FP64
-// without contraction, so the decision is the same IEEE computation as the oracle's.
-// Neighbour seeds are re-read with plain loads (no wave operations: the test runs under
-// per-pixel conditions) instead of being held in registers, so the cell costs the LM loop
-// no registers.
+// Voronoi cell of a feature against its listed neighbour seeds. A pixel q belongs to the
+// cell when it is strictly nearer to the own seed s0 than to every neighbour s_j, ties going
+// to the lower feature index (reference subpx/_cpu/cells.py:8-33). The test uses the linear
+// form f_j(q) = |q - s_j|^2 - |q - s0|^2 = (a_j qx + b_j qy) + c_j with a_j = 2 (s0x - s_jx),
+// b_j = 2 (s0y - s_jy), c_j = (s_jx^2 + s_jy^2) - (s0x^2 + s0y^2), in FP64 without contraction
+// and in the oracle's operation order, so both take the same decisions. A pixel of the cell is
+// on its boundary (a 4-neighbour lies outside) when some g_j = f_j(q) - max(|a_j|, |b_j|) is
+// negative, or zero towards a lower index: the 4-neighbours shift f_j by +-a_j or +-b_j.
+// Absent neighbours get a = b = 0, c = 1e300 (never decisive).
 constexpr int kMaxNb = 8;
 
-SPX_DEV_INLINE bool nearer_than(double px, double py, double sx, double sy, int self, double nx, double ny,
-                                int id) {
+struct CellForm {
+    double a[kMaxNb], b[kMaxNb], c[kMaxNb], m[kMaxNb];
+    bool lower[kMaxNb];
+};
+
+SPX_DEV_INLINE void load_cell(const FitArgs& a, int f, CellForm& cf) {
 #pragma clang fp contract(off)
-    const double dx = px - sx, dy = py - sy;
-    const double ex = px - nx, ey = py - ny;
-    const double d0 = dx * dx + dy * dy;
-    const double dj = ex * ex + ey * ey;
-    return dj > d0 || (dj == d0 && self < id);
+    const double x0 = (double)spx_uniform(a.seeds[2 * (int64_t)f]);
+    const double y0 = (double)spx_uniform(a.seeds[2 * (int64_t)f + 1]);
+    const double n0 = x0 * x0 + y0 * y0;
+#pragma unroll
+    for (int j = 0; j < kMaxNb; ++j) {
+        int id = -1;
+        if (j < a.nb) id = spx_uniform_i(a.nbr[(int64_t)f * a.nb + j]);
+        const bool act = id >= 0 && id < a.nfeat;
+        double xj = 0.0, yj = 0.0;
+        if (act) {
+            xj = (double)spx_uniform(a.seeds[2 * (int64_t)id]);
+            yj = (double)spx_uniform(a.seeds[2 * (int64_t)id + 1]);
+        }
+        cf.a[j] = act ? 2.0 * (x0 - xj) : 0.0;
+        cf.b[j] = act ? 2.0 * (y0 - yj) : 0.0;
+        cf.c[j] = act ? (xj * xj + yj * yj) - n0 : 1e300;
+        cf.m[j] = fmax(fabs(cf.a[j]), fabs(cf.b[j]));
+        cf.lower[j] = act && id < f;
+    }
 }
 
-SPX_DEV_INLINE bool in_cell(const FitArgs& a, int f, double px, double py) {
-    const double sx = (double)a.seeds[2 * (int64_t)f], sy = (double)a.seeds[2 * (int64_t)f + 1];
+SPX_DEV_INLINE bool cell_in(const CellForm& cf, double qx, double qy) {
+#pragma clang fp contract(off)
     bool in = true;
-    for (int j = 0; j < a.nb; ++j) {
-        const int id = a.nbr[(int64_t)f * a.nb + j];
-        if (id >= 0 && id < a.nfeat) {
-            const double nx = (double)a.seeds[2 * (int64_t)id], ny = (double)a.seeds[2 * (int64_t)id + 1];
-            in = in && nearer_than(px, py, sx, sy, f, nx, ny, id);
-        }
+#pragma unroll
+    for (int j = 0; j < kMaxNb; ++j) {
+        const double fj = (cf.a[j] * qx + cf.b[j] * qy) + cf.c[j];
+        in = in && (fj > 0.0 || (fj == 0.0 && !cf.lower[j]));
     }
     return in;
+}
+
+SPX_DEV_INLINE bool cell_edge(const CellForm& cf, double qx, double qy) {
+#pragma clang fp contract(off)
+    bool e = false;
+#pragma unroll
+    for (int j = 0; j < kMaxNb; ++j) {
+        const double gj = ((cf.a[j] * qx + cf.b[j] * qy) + cf.c[j]) - cf.m[j];
+        e = e || gj < 0.0 || (gj == 0.0 && cf.lower[j]);
+    }
+    return e;
 }
 
 // Work counters of one wave, flushed to FitArgs::stats once when the wave leaves its
@@ -418,24 +447,20 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
             const int side = 2 * hs + 1;
             const double ox = (double)cx - (double)hs, oy = (double)cy - (double)hs;
             float bsum = 0.0f, bcnt = 0.0f, vmax = -3.0e38f;
+            CellForm cell;
+            if (masked) load_cell(a, f, cell);
 #pragma unroll
             for (int n = 0; n < NPL; ++n) {
-                // NPL <= 16: a uniform guard keeps the loop fully unrolled with px[] in VGPRs;
-        // larger patches keep the loop rolled (px[] in scratch) rather than spill the LM state.
-        if constexpr (NPL > kUnrollMax) {
-            if (n >= pt.nslots) break;
-        }
-        if (n < pt.nslots) {
+                if constexpr (NPL > kUnrollMax) {
+                    if (n >= pt.nslots) break;
+                }
+                if (n < pt.nslots) {
                     if ((pt.valid >> n) & 1ull) {
                         const uint32_t w = pt.px[n];
                         const int col = (int)(w & 0xffu), row = (int)((w >> 8) & 0xffu);
                         const float val = (float)(w >> 16);
                         bool ring = row == 0 || row == side - 1 || col == 0 || col == side - 1;
-                        if (!ring && masked) {
-                            const double px = ox + (double)col, py = oy + (double)row;
-                            ring = !in_cell(a, f, px - 1.0, py) || !in_cell(a, f, px + 1.0, py) ||
-                                   !in_cell(a, f, px, py - 1.0) || !in_cell(a, f, px, py + 1.0);
-                        }
+                        if (!ring && masked) ring = cell_edge(cell, ox + (double)col, oy + (double)row);
                         if (ring) {
                             bsum += val;
                             bcnt += 1.0f;
@@ -452,12 +477,10 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
             float s0 = 0.0f, sx = 0.0f, sy = 0.0f, nin = 0.0f;
 #pragma unroll
             for (int n = 0; n < NPL; ++n) {
-                // NPL <= 16: a uniform guard keeps the loop fully unrolled with px[] in VGPRs;
-        // larger patches keep the loop rolled (px[] in scratch) rather than spill the LM state.
-        if constexpr (NPL > kUnrollMax) {
-            if (n >= pt.nslots) break;
-        }
-        if (n < pt.nslots) {
+                if constexpr (NPL > kUnrollMax) {
+                    if (n >= pt.nslots) break;
+                }
+                if (n < pt.nslots) {
                     const uint32_t w = pt.px[n];
                     const float wv = (float)(w >> 16) - B0;
                     if (((pt.valid >> n) & 1ull) && wv > half_peak) {
@@ -615,7 +638,6 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
                 const int id = spx_uniform_i(a.nbr[(int64_t)f * a.nb + j]);
                 masked = masked || (id >= 0 && id < a.nfeat);
             }
-
         // Gather: lane-strided over the flattened patch, so each load instruction reads a
         // few contiguous row segments of the image.
         Patch<NPL> pt;
@@ -629,12 +651,26 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
             const int col = pix - row * side;
             const int64_t gx = ox + col;
             const int64_t gy = oy + row;
-            bool ok = pix < npx && gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
-            if (masked) ok = ok && in_cell(a, f, (double)gx, (double)gy);
+            const bool ok = pix < npx && gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
             const uint32_t v = ok ? (uint32_t)a.img[gy * a.pitch + gx] : 0u;
             pt.px[n] = pack_px(ok ? col : 0, ok ? row : 0, v);
             pt.valid |= ok ? (1ull << n) : 0ull;
             cnt += ok ? 1.0f : 0.0f;
+        }
+        if (masked) {
+            // restrict to the Voronoi cell (the cell's 64 FP64 registers live only here)
+            CellForm cell;
+            load_cell(a, f, cell);
+            cnt = 0.0f;
+#pragma unroll
+            for (int n = 0; n < NPL; ++n) {
+                const int pix = lane + kWave * n;
+                const int row = pix / side;
+                const int col = pix - row * side;
+                const bool in = ((pt.valid >> n) & 1ull) && cell_in(cell, (double)(ox + col), (double)(oy + row));
+                if (!in) pt.valid &= ~(1ull << n);
+                cnt += in ? 1.0f : 0.0f;
+            }
         }
         pt.npix = spx_wave_sum(cnt);
         fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked, ws);

@@ -1,5 +1,6 @@
 // Nearest-seed lists on the device: the nb nearest other seeds of every seed within a radius,
-// by a uniform-grid hash (cell side = radius, so a 3 x 3 block of cells covers the disk).
+// by a uniform-grid hash (cell side >= radius, so a 3 x 3 block of cells covers the disk; the
+// side grows past the radius when the radius is small against the image, to bound the grid).
 //
 // Counting sort of the seeds into cells (atomic counts, exclusive scan, atomic scatter),
 // then one thread per seed scans its 3 x 3 cells and keeps the nb nearest in a sorted
@@ -89,14 +90,12 @@ int seed_neighbours_launch(const float* seeds, int32_t K, int64_t H, int64_t W, 
                            int32_t* out, hipStream_t stream) {
     if (K == 0) return SPX_OK;
     Grid g;
-    g.inv = 1.0f / radius;
-    g.gw = (int)(W / radius) + 1;
-    g.gh = (int)(H / radius) + 1;
+    const double min_cell = sqrt((double)H * (double)W / (double)(1 << 24));
+    const float cell = (float)fmax((double)radius, min_cell);
+    g.inv = 1.0f / cell;
+    g.gw = (int)(W / cell) + 1;
+    g.gh = (int)(H / cell) + 1;
     const int64_t ncell = (int64_t)g.gw * g.gh;
-    if (ncell > (1 << 26)) {
-        set_error("spx_seed_neighbours: radius too small for the image (more than 2^26 grid cells)");
-        return SPX_E_ARG;
-    }
     size_t scan_bytes = 0;
     SPX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr, (int)ncell,
                                              stream));

@@ -10,6 +10,8 @@
 // N(0, read_sigma^2), rounded half-even and clipped to uint16.
 // Draws come from a counter-based generator (splitmix64 of seed, tile, hole, field), so
 // a tile is the same whatever else is rendered with it.
+#include <hipcub/hipcub.hpp>
+
 #include "spx_internal.hpp"
 
 namespace spx {
@@ -111,7 +113,95 @@ __global__ void tile_pixels_kernel(uint64_t seed, int n, int T, SpxTileSpec spec
     img[gy * pitch + gx] = (uint16_t)v;
 }
 
+// Whole pages: a hole list (x, y, r, s, q, A per hole, image coordinates) binned into a grid
+// of kPageCell-px cells by a counting sort; each pixel adds the holes of its 3 x 3 cells
+// whose support (r + 8 s, where erfc/2 < 1e-15) reaches it, then the tile renderer's noise.
+constexpr int kPageCell = 64;
+
+__global__ void page_count_kernel(const float* holes, int n, int gw, int gh, int32_t* cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cx = min(max((int)floorf(holes[6 * i] / kPageCell), 0), gw - 1);
+    const int cy = min(max((int)floorf(holes[6 * i + 1] / kPageCell), 0), gh - 1);
+    atomicAdd(&cnt[cy * gw + cx], 1);
+}
+
+__global__ void page_scatter_kernel(const float* holes, int n, int gw, int gh, int32_t* cursor, int32_t* ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cx = min(max((int)floorf(holes[6 * i] / kPageCell), 0), gw - 1);
+    const int cy = min(max((int)floorf(holes[6 * i + 1] / kPageCell), 0), gh - 1);
+    ids[atomicAdd(&cursor[cy * gw + cx], 1)] = i;
+}
+
+__global__ void page_pixels_kernel(uint64_t seed, const float* holes, const int32_t* start, const int32_t* cnt,
+                                   const int32_t* ids, int gw, int gh, int64_t H, int64_t W, double background,
+                                   double read_sigma, uint16_t* img, int64_t pitch) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * W) return;
+    const int64_t py = i / W, px = i - (i / W) * W;
+    const int cx = (int)(px / kPageCell), cy = (int)(py / kPageCell);
+    double I = background;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = cy + dy;
+        if (yy < 0 || yy >= gh) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = cx + dx;
+            if (xx < 0 || xx >= gw) continue;
+            const int c = yy * gw + xx;
+            const int s0 = start[c], s1 = s0 + cnt[c];
+            for (int k = s0; k < s1; ++k) {
+                const float* o = holes + 6 * (int64_t)ids[k];
+                const double ex = (double)px - o[0], ey = ((double)py - o[1]) / o[4];
+                const double rho = sqrt(ex * ex + ey * ey);
+                if (rho > (double)o[2] + 8.0 * (double)o[3]) continue;
+                I += (double)o[5] * 0.5 * erfc((rho - o[2]) / (1.4142135623730951 * o[3]));
+            }
+        }
+    }
+    double v = I + sqrt(fmax(I, 0.0)) * normal(seed ^ 0x5bd1e995ull, (uint64_t)py, (uint64_t)px, 0);
+    if (read_sigma > 0.0) v += read_sigma * normal(seed ^ 0x1b873593ull, (uint64_t)py, (uint64_t)px, 0);
+    v = rint(v);
+    v = fmin(fmax(v, 0.0), 65535.0);
+    img[py * pitch + px] = (uint16_t)v;
+}
+
 }  // namespace
+
+int render_page_launch(uint64_t seed, const float* holes, int n, int64_t H, int64_t W, float background,
+                       float read_sigma, uint16_t* img, int64_t pitch, hipStream_t stream) {
+    const int gw = (int)((W + kPageCell - 1) / kPageCell), gh = (int)((H + kPageCell - 1) / kPageCell);
+    const int64_t ncell = (int64_t)gw * gh;
+    size_t scan_bytes = 0;
+    SPX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr, (int)ncell,
+                                             stream));
+    const size_t head = sizeof(int32_t) * (3 * (size_t)ncell + (size_t)(n > 0 ? n : 1));
+    const size_t head_al = (head + 255) & ~(size_t)255;
+    void* ws = nullptr;
+    int rc = workspace(head_al + scan_bytes, &ws);
+    if (rc != SPX_OK) return rc;
+    int32_t* cnt = static_cast<int32_t*>(ws);
+    int32_t* start = cnt + ncell;
+    int32_t* cursor = start + ncell;
+    int32_t* ids = cursor + ncell;
+    SPX_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * ncell, stream));
+    if (n > 0) {
+        page_count_kernel<<<(n + 255) / 256, 256, 0, stream>>>(holes, n, gw, gh, cnt);
+        SPX_LAUNCHED();
+    }
+    SPX_HIP(hipcub::DeviceScan::ExclusiveSum(static_cast<char*>(ws) + head_al, scan_bytes, cnt, start, (int)ncell,
+                                             stream));
+    SPX_HIP(hipMemcpyAsync(cursor, start, sizeof(int32_t) * ncell, hipMemcpyDeviceToDevice, stream));
+    if (n > 0) {
+        page_scatter_kernel<<<(n + 255) / 256, 256, 0, stream>>>(holes, n, gw, gh, cursor, ids);
+        SPX_LAUNCHED();
+    }
+    const int64_t total = H * W;
+    page_pixels_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(
+        seed, holes, start, cnt, ids, gw, gh, H, W, (double)background, (double)read_sigma, img, pitch);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
 
 int render_tiles_launch(uint64_t seed, int n, int T, const SpxTileSpec& spec, uint16_t* img, int64_t pitch,
                         float* truth, hipStream_t stream) {

@@ -95,6 +95,8 @@ int edge_erf_launch(const void* img, int dtype, int64_t H, int64_t W, int64_t pi
                     const SpxEdgeErfOptions& opt, double* centers, uint8_t* ok, hipStream_t stream);
 int edge_profile_launch(const double* prof, int K, int L, const double* base, const double* seed,
                         const SpxEdgeErfOptions& opt, double* x, uint8_t* ok, hipStream_t stream);
+int render_page_launch(uint64_t seed, const float* holes, int n, int64_t H, int64_t W, float background,
+                       float read_sigma, uint16_t* img, int64_t pitch, hipStream_t stream);
 int render_tiles_launch(uint64_t seed, int n, int T, const SpxTileSpec& spec, uint16_t* img, int64_t pitch,
                         float* truth, hipStream_t stream);
 
@@ -255,6 +257,15 @@ int spx_edge_profile_erf(const double* prof, int32_t K, int32_t L, const double*
     if (K == 0) return SPX_OK;
     SPX_REQUIRE(prof && base && seed && x_out && ok_out, "spx_edge_profile_erf: null pointer");
     return spx::edge_profile_launch(prof, K, L, base, seed, *opt, x_out, ok_out, spx::as_stream(stream));
+}
+
+int spx_render_page(uint64_t seed, const float* holes, int32_t n, int64_t H, int64_t W, float background,
+                    float read_sigma, uint16_t* img, int64_t pitch, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(n >= 0 && H > 0 && W > 0 && pitch >= W, "spx_render_page: bad sizes");
+    SPX_REQUIRE(img != nullptr && (n == 0 || holes != nullptr), "spx_render_page: null pointer");
+    SPX_REQUIRE((W + 63) / 64 * ((H + 63) / 64) < (int64_t)0x7fffffff, "spx_render_page: image too large");
+    return spx::render_page_launch(seed, holes, n, H, W, background, read_sigma, img, pitch, spx::as_stream(stream));
 }
 
 int spx_render_tiles(uint64_t seed, int32_t n, int32_t tile, const SpxTileSpec* spec, uint16_t* img,

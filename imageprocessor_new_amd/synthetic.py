@@ -165,3 +165,55 @@ def make_config(name: str, seed: int = 0, *, H: int | None = None, W: int | None
     arr.ellipse = ell
     arr.meta.update(config=name, seed=seed, H=H, W=W, pitch=cfg["pitch"], layout=cfg["layout"])
     return arr
+
+
+# ---------------------------------------------------------------------------------------
+# Large pages (32768^2, ~1.1M holes): hole lists drawn here, pixels rendered on the GPU.
+
+def mixed_page_holes(seed: int = 0, H: int = 32768, W: int = 32768, region: int = 2048,
+                     pitch=(24.0, 44.0), r_frac=(0.2, 0.3), sigma=(0.8, 1.6), jitter: float = 0.3,
+                     amplitude: float = 3000.0):
+    """Hole list of a page tiled into ``region`` x ``region`` blocks, each a square or a
+    staggered array with its own pitch ~ U[pitch] (the mixed large-page workload). Holes keep
+    half a pitch from their block's edges; r ~ U[r_frac] * pitch, sigma ~ U[sigma], centre
+    jitter N(0, jitter^2). Returns (holes (N, 6) float32 [x, y, r, sigma, q = 1, A],
+    layouts (nby, nbx) 0 square / 1 staggered, pitches (nby, nbx))."""
+    rng = np.random.default_rng(seed)
+    nby, nbx = -(-H // region), -(-W // region)
+    layouts = rng.integers(0, 2, size=(nby, nbx))
+    pitches = rng.uniform(pitch[0], pitch[1], size=(nby, nbx))
+    out = []
+    for by in range(nby):
+        for bx in range(nbx):
+            p = float(pitches[by, bx])
+            h = min(region, H - by * region)
+            w = min(region, W - bx * region)
+            lat = square_lattice(h, w, p) if layouts[by, bx] == 0 else hex_lattice(h, w, p)
+            keep = ((lat[:, 0] >= 0.5 * p) & (lat[:, 0] <= w - 0.5 * p) & (lat[:, 1] >= 0.5 * p)
+                    & (lat[:, 1] <= h - 0.5 * p))
+            lat = lat[keep] + np.array([bx * region, by * region], dtype=np.float64)
+            n = lat.shape[0]
+            c = lat + rng.normal(0.0, jitter, size=(n, 2))
+            r = rng.uniform(r_frac[0], r_frac[1], size=n) * p
+            s = rng.uniform(sigma[0], sigma[1], size=n)
+            out.append(np.column_stack([c, r, s, np.ones(n), np.full(n, amplitude)]))
+    holes = np.concatenate(out, axis=0).astype(np.float32)
+    return holes, layouts, pitches
+
+
+def render_page(holes, H: int, W: int, *, background: float = 500.0, read_sigma: float = 30.0,
+                seed: int = 0, device: int = 0):
+    """Render a hole list (N, 6) [x, y, r, sigma, q, A] into an (H, W) uint16 device tensor
+    (spx_render_page: point-sampled erf disks, normal-limit photon noise + read noise)."""
+    import torch
+
+    from . import _lib
+    from .backends import gpu_device, stream_handle
+
+    with gpu_device(device) as dev:
+        h = torch.from_numpy(np.ascontiguousarray(holes, dtype=np.float32)).to(dev)
+        img = torch.empty((int(H), int(W)), dtype=torch.uint16, device=dev)
+        _lib.call("spx_render_page", int(seed) & 0xFFFFFFFFFFFFFFFF, int(h.data_ptr()), int(h.shape[0]), int(H),
+                  int(W), float(background), float(read_sigma), int(img.data_ptr()), int(W), stream_handle())
+        torch.cuda.current_stream().synchronize()
+        return img

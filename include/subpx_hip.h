@@ -299,6 +299,14 @@ int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, in
  * model, photon noise in its normal limit + read noise, uint16. truth_out (n, 7)
  * [x0, y0, r, s, A, B, q] of the centre hole in mosaic coordinates. img, truth: device.
  * No reference counterpart (training data for the learned start). */
+/* Whole synthetic page: holes (n, 6) f32 [x, y, r, s, q, A] in image coordinates (device),
+ * each I += A erfc((rho - r) / (sqrt2 s)) / 2 with rho = sqrt(dx^2 + (dy / q)^2), on a constant
+ * background, then N(I, I) photon noise (normal limit) + N(0, read_sigma^2), rounded and
+ * clipped to uint16 (device image, row pitch in elements). No reference counterpart: the
+ * generator of the large benchmark pages, rendered on the device. */
+int spx_render_page(uint64_t seed, const float* holes, int32_t n, int64_t H, int64_t W, float background,
+                    float read_sigma, uint16_t* img, int64_t pitch, spx_stream_t stream);
+
 int spx_render_tiles(uint64_t seed, int32_t n, int32_t tile, const SpxTileSpec* spec, uint16_t* img,
                      int64_t pitch, float* truth_out, spx_stream_t stream);
 

@@ -29,9 +29,12 @@ otherwise the trial is accepted when its cost is at most cost * (1 + accept_rtol
 
 Neighbour (Voronoi cell) mask, when neighbour lists are given: a patch pixel takes part
 only when it is strictly nearer to the own seed than to each listed neighbour seed, ties
-going to the lower feature index (after reference subpx/_cpu/cells.py:8-33), squared
-distances in FP64; the start's background ring then also takes the cell's boundary pixels
-(in the cell with a 4-neighbour outside it, reference subpx/_cpu/cells.py:72-80).
+going to the lower feature index (after reference subpx/_cpu/cells.py:8-33), decided on the
+FP64 linear form |q - s_j|^2 - |q - s_0|^2 = (a_j qx + b_j qy) + c_j in the kernel's operation
+order; the start's background ring then also takes the cell's boundary pixels (in the cell
+with a 4-neighbour outside it, reference subpx/_cpu/cells.py:72-80), found as
+f_j(q) - max(|a_j|, |b_j|) < 0 (or = 0 towards a lower index), the minimum of f_j over the
+4-neighbours.
 ``seed_neighbours`` lists the nb nearest other seeds within a radius by brute force.
 
 Pinned by: recovery of noise-free renders to 1e-8 px, the Jacobian against central
@@ -64,17 +67,41 @@ def seed_neighbours(seeds, radius, nb):
     return out
 
 
-def cell_mask(px, py, seeds, f, nbr_row):
-    """True where pixel (px, py) lies in feature f's cell against the listed neighbours."""
+def _cell_forms(seeds, f, nbr_row):
+    """(a, b, c, m, lower) per listed neighbour: f_j(q) = |q - s_j|^2 - |q - s_f|^2 =
+    (a qx + b qy) + c, m = max(|a|, |b|), lower = neighbour index below f."""
     s = np.asarray(seeds, dtype=np.float32).astype(np.float64).reshape(-1, 2)
-    d0 = (px - s[f, 0]) * (px - s[f, 0]) + (py - s[f, 1]) * (py - s[f, 1])
-    inside = np.ones(np.shape(px), dtype=bool)
+    x0, y0 = s[f]
+    n0 = x0 * x0 + y0 * y0
+    out = []
     for j in nbr_row:
-        if j < 0:
+        j = int(j)
+        if j < 0 or j >= s.shape[0]:
             continue
-        dj = (px - s[j, 0]) * (px - s[j, 0]) + (py - s[j, 1]) * (py - s[j, 1])
-        inside &= (dj > d0) | ((dj == d0) & (f < j))
+        xj, yj = s[j]
+        a, b = 2.0 * (x0 - xj), 2.0 * (y0 - yj)
+        out.append((a, b, (xj * xj + yj * yj) - n0, max(abs(a), abs(b)), j < f))
+    return out
+
+
+def cell_mask(px, py, seeds, f, nbr_row):
+    """True where pixel (px, py) lies in feature f's cell against the listed neighbours:
+    every f_j(q) > 0, or = 0 towards a higher index."""
+    inside = np.ones(np.shape(px), dtype=bool)
+    for a, b, c, _, lower in _cell_forms(seeds, f, nbr_row):
+        fj = (a * px + b * py) + c
+        inside &= (fj > 0) | ((fj == 0) & (not lower))
     return inside
+
+
+def cell_edge(px, py, seeds, f, nbr_row):
+    """True where some 4-neighbour of (px, py) lies outside the cell: some
+    g_j = f_j(q) - max(|a_j|, |b_j|) < 0, or = 0 towards a lower index."""
+    edge = np.zeros(np.shape(px), dtype=bool)
+    for a, b, c, m, lower in _cell_forms(seeds, f, nbr_row):
+        gj = ((a * px + b * py) + c) - m
+        edge |= (gj < 0) | ((gj == 0) & lower)
+    return edge
 
 
 def extract_patch(img: np.ndarray, seed_xy, half: int):
@@ -263,16 +290,13 @@ def fit_batch(img, seeds, half, model, init=None, neighbours=None, **opts):
             status[k] = TOO_FEW
             continue
         v, valid, _, _, cx, cy = extract_patch(img, seeds[k], h)
-        cell_edge = None
+        edge_px = None
         if neighbours is not None and (np.asarray(neighbours[k]) >= 0).any():
             side = 2 * h + 1
             gy, gx = np.meshgrid(np.arange(side) + (cy - h), np.arange(side) + (cx - h), indexing="ij")
             inside = cell_mask(gx, gy, seeds, k, neighbours[k])
             valid = valid & inside
-            out = np.zeros((side, side), dtype=bool)
-            for ddx, ddy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
-                out |= ~cell_mask(gx + ddx, gy + ddy, seeds, k, neighbours[k])
-            cell_edge = inside & out
+            edge_px = inside & cell_edge(gx, gy, seeds, k, neighbours[k])
             v = np.where(valid, v, 0.0)
         p0 = None
         if init is not None:
@@ -281,7 +305,7 @@ def fit_batch(img, seeds, half, model, init=None, neighbours=None, **opts):
             if ell:
                 p0.append(row[6])
             p0.append(row[5])
-        p, cost, it, st, npix = lm_fit_patch(v, valid, h, ell, p0, cell_edge=cell_edge, **opts)
+        p, cost, it, st, npix = lm_fit_patch(v, valid, h, ell, p0, cell_edge=edge_px, **opts)
         params[k] = to_output(p, ell, cx, cy)
         P = 7 if ell else 6
         rms[k] = np.sqrt(cost / max(npix - P, 1)) if st not in (TOO_FEW, BAD_START) else np.nan

@@ -118,3 +118,18 @@ def test_cell_mask_removes_the_neighbours_bias():
     assert (s1 <= 1).all()
     assert np.abs(p1[:, :2] - c).max() < 1e-3
     assert np.abs(p0[:, :2] - c).max() > 10 * np.abs(p1[:, :2] - c).max()
+
+
+def test_cell_edge_is_a_four_neighbour_outside():
+    # the one-subtraction boundary test equals evaluating the cell at the 4 neighbours
+    rng = np.random.default_rng(8)
+    for trial in range(20):
+        seeds = (rng.uniform(0, 60, (7, 2)) * 4).round() / 4  # quarter-pixel seeds: exact FP64 forms
+        f = int(rng.integers(0, 7))
+        others = [j for j in range(7) if j != f]
+        gy, gx = np.mgrid[0:64, 0:64].astype(np.float64)
+        inside = erfdisk.cell_mask(gx, gy, seeds, f, others)
+        out = np.zeros_like(inside)
+        for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+            out |= ~erfdisk.cell_mask(gx + dx, gy + dy, seeds, f, others)
+        np.testing.assert_array_equal(inside & erfdisk.cell_edge(gx, gy, seeds, f, others), inside & out)
