@@ -53,11 +53,13 @@ def choose_refine_method_for_bbox(width: float, height: float, *, small_feature_
 
 
 def _row_methods(refine, rows_np, small_feature_max):
+    """Method of every row as an index into _GROUP_ORDER (int8)."""
     K = rows_np.shape[0]
     if refine == "auto":
-        size = np.maximum(rows_np[:, 2], rows_np[:, 3]).astype(np.float64)
-        return np.where(size <= float(small_feature_max), "logquad", "edge_gradmoment")
-    return np.full(K, _ALIASES.get(refine, refine), dtype=object)
+        size = np.maximum(rows_np[:, 2], rows_np[:, 3])
+        lq, em = _GROUP_ORDER.index("logquad"), _GROUP_ORDER.index("edge_gradmoment")
+        return np.where(size <= float(small_feature_max), lq, em).astype(np.int8)
+    return np.full(K, _GROUP_ORDER.index(_ALIASES.get(refine, refine)), dtype=np.int8)
 
 
 def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit_options, logquad_variant,
@@ -147,10 +149,11 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
     info = {"threshold_value": thresh, "num_labels": num, "n_candidates": int(keep.size)}
     t3 = time.perf_counter()
     n_refined = 0
-    for method in _GROUP_ORDER:
-        idx = np.nonzero(methods == method)[0]
-        if idx.size == 0:
+    counts = np.bincount(methods, minlength=len(_GROUP_ORDER))
+    for code, method in enumerate(_GROUP_ORDER):
+        if counts[code] == 0:
             continue
+        idx = np.arange(methods.size) if counts[code] == methods.size else np.nonzero(methods == code)[0]
         centers, ok, extra = _refine_rows(img_t, labels, rows[idx], cents_np[keep][idx], method, pad=pad,
                                           fit_half=fit_half, fit_options=fit_options,
                                           logquad_variant=logquad_variant, fit_cells=fit_cells)

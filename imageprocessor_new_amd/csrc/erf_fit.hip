@@ -118,7 +118,9 @@ using namespace fitdev;
 
 template <int NPL, int P>
 __global__ __launch_bounds__(256) void erf_fit_kernel(FitArgs a) {
-    erf_fit_wave<NPL, P>(a);
+    constexpr bool kLds = Patch<NPL>::kLds;
+    __shared__ uint32_t s_words[kLds ? 4 * NPL * spx::fitdev::kWave : 1];
+    erf_fit_wave<NPL, P>(a, kLds ? s_words + (threadIdx.x >> 6) * NPL * spx::fitdev::kWave : nullptr);
 }
 
 // Bucket id: patch-size class x model (disk 6 / ellipse 7 parameters).

@@ -89,15 +89,27 @@ SPX_DEV_INLINE uint32_t pack_px(int col, int row, uint32_t v) {
     return (uint32_t)col | ((uint32_t)row << 8) | (v << 16);
 }
 
-// Work of one patch in registers: NPL words per lane, lane l holding flattened pixels
-// l, l + 64, ...; slots up to `nslots` are weighted by `valid` (bit n = slot n), later
-// slots are skipped.
+// Work of one patch: NPL words per lane, lane l holding flattened pixels l, l + 64, ...;
+// slots up to `nslots` are weighted by `valid` (bit n = slot n), later slots are skipped.
+// Patches of up to kUnrollMax slots per lane keep their words in VGPRs (fully unrolled
+// loops); larger ones in the wave's LDS slab (word n of lane l at n * 64 + l, conflict
+// free), so the LM state keeps its registers.
 template <int NPL>
 struct Patch {
-    uint32_t px[NPL];
+    static constexpr bool kLds = NPL > kUnrollMax;
+    uint32_t px[kLds ? 1 : NPL];
+    uint32_t* lds;
     uint64_t valid;
     int nslots;
     float npix;
+    SPX_DEV_INLINE uint32_t word(int n) const {
+        if constexpr (kLds) return lds[n * kWave + spx_lane()];
+        else return px[n];
+    }
+    SPX_DEV_INLINE void set(int n, uint32_t w) {
+        if constexpr (kLds) lds[n * kWave + spx_lane()] = w;
+        else px[n] = w;
+    }
 };
 
 // One pixel pass at parameters `pp`: this lane's partial sums of J^T J, J^T r and the
@@ -140,15 +152,15 @@ SPX_DEV_INLINE void pixel_pass(const Patch<NPL>& pt, int hs, const Params<P>& pp
 #pragma unroll
     for (int i = 0; i < L::kN; ++i) acc[i] = 0.0f;
 
-#pragma unroll
+    // Register patches: a uniform guard keeps the loop fully unrolled with px[] in VGPRs;
+    // LDS patches run a rolled loop over their slots.
+#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
     for (int n = 0; n < NPL; ++n) {
-        // NPL <= 16: a uniform guard keeps the loop fully unrolled with px[] in VGPRs;
-        // larger patches keep the loop rolled (px[] in scratch) rather than spill the LM state.
-        if constexpr (NPL > kUnrollMax) {
+        if constexpr (Patch<NPL>::kLds) {
             if (n >= pt.nslots) break;
         }
         if (n < pt.nslots) {
-            const uint32_t wd = pt.px[n];
+            const uint32_t wd = pt.word(n);
             const float dx = (float)(wd & 0xffu) - xo;
             const float dy = (float)((wd >> 8) & 0xffu) - yo;
             const float val = (float)(wd >> 16);
@@ -449,14 +461,14 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
             float bsum = 0.0f, bcnt = 0.0f, vmax = -3.0e38f;
             CellForm cell;
             if (masked) load_cell(a, f, cell);
-#pragma unroll
+#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
             for (int n = 0; n < NPL; ++n) {
-                if constexpr (NPL > kUnrollMax) {
+                if constexpr (Patch<NPL>::kLds) {
                     if (n >= pt.nslots) break;
                 }
                 if (n < pt.nslots) {
                     if ((pt.valid >> n) & 1ull) {
-                        const uint32_t w = pt.px[n];
+                        const uint32_t w = pt.word(n);
                         const int col = (int)(w & 0xffu), row = (int)((w >> 8) & 0xffu);
                         const float val = (float)(w >> 16);
                         bool ring = row == 0 || row == side - 1 || col == 0 || col == side - 1;
@@ -475,13 +487,13 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
             vmax = spx_wave_max(vmax) - B0;
             const float half_peak = 0.5f * vmax;
             float s0 = 0.0f, sx = 0.0f, sy = 0.0f, nin = 0.0f;
-#pragma unroll
+#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
             for (int n = 0; n < NPL; ++n) {
-                if constexpr (NPL > kUnrollMax) {
+                if constexpr (Patch<NPL>::kLds) {
                     if (n >= pt.nslots) break;
                 }
                 if (n < pt.nslots) {
-                    const uint32_t w = pt.px[n];
+                    const uint32_t w = pt.word(n);
                     const float wv = (float)(w >> 16) - B0;
                     if (((pt.valid >> n) & 1ull) && wv > half_peak) {
                         s0 += wv;
@@ -606,9 +618,10 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
 }
 
 // Body of the persistent fit kernel for one wave: pull features of one (patch size,
-// model) bucket from the work queue until it is empty.
+// model) bucket from the work queue until it is empty. lds_words: the wave's NPL x 64-word
+// LDS slab for patches held in LDS (unused otherwise).
 template <int NPL, int P>
-SPX_DEV void erf_fit_wave(const FitArgs& a) {
+SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
     const int lane = spx_lane();
     const int count = spx_uniform_i(*a.count);
     if (count == 0) return;
@@ -641,10 +654,11 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
         // Gather: lane-strided over the flattened patch, so each load instruction reads a
         // few contiguous row segments of the image.
         Patch<NPL> pt;
+        pt.lds = lds_words;
         pt.nslots = (npx + kWave - 1) / kWave;
         pt.valid = 0ull;
         float cnt = 0.0f;
-#pragma unroll
+#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
         for (int n = 0; n < NPL; ++n) {
             const int pix = lane + kWave * n;
             const int row = pix / side;
@@ -653,7 +667,7 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
             const int64_t gy = oy + row;
             const bool ok = pix < npx && gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
             const uint32_t v = ok ? (uint32_t)a.img[gy * a.pitch + gx] : 0u;
-            pt.px[n] = pack_px(ok ? col : 0, ok ? row : 0, v);
+            pt.set(n, pack_px(ok ? col : 0, ok ? row : 0, v));
             pt.valid |= ok ? (1ull << n) : 0ull;
             cnt += ok ? 1.0f : 0.0f;
         }
@@ -662,7 +676,7 @@ SPX_DEV void erf_fit_wave(const FitArgs& a) {
             CellForm cell;
             load_cell(a, f, cell);
             cnt = 0.0f;
-#pragma unroll
+#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
             for (int n = 0; n < NPL; ++n) {
                 const int pix = lane + kWave * n;
                 const int row = pix / side;

@@ -139,13 +139,14 @@ template <int NPL, int P>
 static void run_wave(const FitArgs& a, int timeout_s) {
     WaveState w;
     for (auto& s : w.shuffles) s = 0;
+    std::vector<uint32_t> slab((size_t)NPL * 64);  // the wave's LDS slab
     std::atomic<int> done{0};
     std::vector<std::thread> lanes;
     for (int l = 0; l < 64; ++l) {
         lanes.emplace_back([&, l] {
             t_lane = l;
             t_wave = &w;
-            erf_fit_wave<NPL, P>(a);
+            erf_fit_wave<NPL, P>(a, slab.data());
             done++;
         });
     }
