@@ -95,6 +95,20 @@ class SpxLogquadWindowOptions(ctypes.Structure):
     ]
 
 
+class SpxHoleStepOptions(ctypes.Structure):
+    _fields_ = [
+        ("nv0", ctypes.c_double),
+        ("nv1", ctypes.c_double),
+        ("rscale", ctypes.c_double),
+        ("max_move", ctypes.c_double),
+        ("max_delta", ctypes.c_double),
+        ("a_med", ctypes.c_double),
+        ("relax", ctypes.c_double),
+        ("size", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class SpxTileSpec(ctypes.Structure):
     _fields_ = [
         ("layout", ctypes.c_int32),
@@ -148,6 +162,13 @@ SIGNATURES = {
     "spx_edge_profile_erf": (_I32, [_P, _I32, _I32, _P, _P, ctypes.POINTER(SpxEdgeErfOptions), _P, _P, _P]),
     "spx_render_page": (_I32, [ctypes.c_uint64, _P, _I32, _I64, _I64, ctypes.c_float, ctypes.c_float, _P, _I64,
                                _P]),
+    "spx_hole_templates": (_I32, [ctypes.c_double, ctypes.c_double, _I32, _I32, ctypes.c_double, ctypes.c_double,
+                                  _I32, _P, _P, _P]),
+    "spx_hole_step": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _I32, _P, _I32, _P, _P, _P, _P, _P,
+                             ctypes.POINTER(SpxHoleStepOptions), _P, _P]),
+    "spx_hole_render": (_I32, [_P, _I64, _I64, _P, _P, _I32, _I32, _I32, ctypes.c_double, _P, _I32, _P, _P, _P,
+                               _P, _P]),
+    "spx_far_splat": (_I32, [_P, _P, _P, ctypes.c_double, _I32, _I32, _I32, _I32, _I32, _P, _P]),
     "spx_render_tiles": (_I32, [ctypes.c_uint64, _I32, _I32, ctypes.POINTER(SpxTileSpec), _P, _I64, _P, _P]),
     "spx_fit_batch": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions),
                              _I32, _P, _P, _P, _P, _P]),

@@ -1,0 +1,416 @@
+// Outline hole stage on gfx950: every hole's centre, brightness, level and edge offset by one
+// Gauss-Newton step per sweep against the complete page model, one colour class at a time.
+//
+//   page model  P(p) = sum_i a_i [T0 + d_i T1 + d_i^2 T2](p - c_i) + far light   (FP64, H x W)
+//   hole i      data(p) = P(p) + level_i on its (2R+1)^2 window
+//
+// Follows the reference's GPU hole pass (subpx/_gpu/outline_onepass.py:252-397, 399-427):
+// the same normal equations, weights, step bounds and render rules. Layout and work split are
+// this build's:
+//   - templates are one interleaved FP32 grid of 8 floats per fine point,
+//     [T0, T1, T2, Tx, Ty, 0, 0, 0], so a tap is one 16-byte load (+ one dword for Ty);
+//   - `hole_step_kernel` gives each hole a group of G lanes (16, 32 or 64 of a wave64). The
+//     lanes stride over the window, keep FP32 partial sums of J^T W J (as the reference's
+//     per-row sums) and FP64 sums of J^T W r, reduce them in FP64 over the group with
+//     cross-lane swaps in a fixed order (deterministic, no atomics), then solve the 5 x 5
+//     (4 x 4 without the edge offset) system by FP64 Cholesky and apply the bounded step:
+//     the reference's hole_normal + hole_update in one launch;
+//   - `hole_render_kernel` is one thread per (hole, render offset): old light out, new light in,
+//     by FP64 atomics into the page model, as the reference.
+#include "spx_internal.hpp"
+
+namespace spx {
+namespace {
+
+// Keys cubic (a = -0.5) weights of the taps at -1, 0, 1, 2 for a fraction t in [0, 1).
+__device__ __forceinline__ void keys4(float t, float* w) {
+    const float x0 = 1.0f + t, x1 = t, x2 = 1.0f - t, x3 = 2.0f - t;
+    w[0] = ((-0.5f * x0 + 2.5f) * x0 - 4.0f) * x0 + 2.0f;
+    w[1] = (1.5f * x1 - 2.5f) * x1 * x1 + 1.0f;
+    w[2] = (1.5f * x2 - 2.5f) * x2 * x2 + 1.0f;
+    w[3] = ((-0.5f * x3 + 2.5f) * x3 - 4.0f) * x3 + 2.0f;
+}
+__device__ __forceinline__ void keys4d(double t, double* w) {
+    const double x0 = 1.0 + t, x1 = t, x2 = 1.0 - t, x3 = 2.0 - t;
+    w[0] = ((-0.5 * x0 + 2.5) * x0 - 4.0) * x0 + 2.0;
+    w[1] = (1.5 * x1 - 2.5) * x1 * x1 + 1.0;
+    w[2] = (1.5 * x2 - 2.5) * x2 * x2 + 1.0;
+    w[3] = ((-0.5 * x3 + 2.5) * x3 - 4.0) * x3 + 2.0;
+}
+
+// Cubic sample of the template grid at fine coordinates (sx, sy); taps off the grid read 0.
+// NG = 3: (T0, T1, T2); NG = 5: (T0, T1, T2, Tx, Ty).
+template <int NG>
+__device__ __forceinline__ void sample_tmpl(const float* __restrict__ tmpl, int N, double sx, double sy, float* v) {
+    const double fx = floor(sx), fy = floor(sy);
+    const int ix = (int)fx, iy = (int)fy;
+    float wx[4], wy[4];
+    keys4((float)(sx - fx), wx);
+    keys4((float)(sy - fy), wy);
+#pragma unroll
+    for (int g = 0; g < NG; ++g) v[g] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int yy = iy + k - 1;
+        if (yy < 0 || yy >= N) continue;
+        float row[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) row[g] = 0.0f;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const int xx = ix + l - 1;
+            if (xx < 0 || xx >= N) continue;
+            const float* p = tmpl + ((int64_t)yy * N + xx) * 8;
+            const float4 q = *reinterpret_cast<const float4*>(p);
+            row[0] += wx[l] * q.x;
+            row[1] += wx[l] * q.y;
+            row[2] += wx[l] * q.z;
+            if constexpr (NG == 5) {
+                row[3] += wx[l] * q.w;
+                row[4] += wx[l] * p[4];
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < NG; ++g) v[g] += wy[k] * row[g];
+    }
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+struct HoleArgs {
+    const float* image;
+    int64_t H, W, pitch;
+    const double* page;
+    const float* tmpl;
+    int N, U, R;
+    const int32_t* idx;
+    int n;
+    double* c;
+    double* a;
+    double* level;
+    double* delta;
+    double* c_old;
+    double nv0, nv1, rscale;
+    int size;
+    double max_move, max_delta, a_med, relax;
+    double* stat;
+};
+
+template <int G, int K>
+__global__ __launch_bounds__(256) void hole_step_kernel(HoleArgs A) {
+    const int slot = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const int gl = threadIdx.x & (G - 1);
+    const bool live = slot < A.n;
+    const int h = live ? A.idx[slot] : 0;
+    const double x = live ? A.c[2 * (int64_t)h] : 0.0, y = live ? A.c[2 * (int64_t)h + 1] : 0.0;
+    const double aa = live ? A.a[h] : 0.0, lev = live ? A.level[h] : 0.0, de = live ? A.delta[h] : 0.0;
+    const int64_t rx = (int64_t)rint(x), ry = (int64_t)rint(y);
+    const int side = 2 * A.R + 1;
+    const int npx = live ? side * side : 0;
+    const float fa = (float)aa, fde = (float)de;
+    const double half_n = (double)(A.N / 2);
+
+    float m[15];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) m[q] = 0.0f;
+    double rhs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double rss = 0.0, cnt = 0.0;
+    for (int p = gl; p < npx; p += G) {
+        const int oy = p / side - A.R, ox = p - (p / side) * side - A.R;
+        const int64_t px = rx + ox, py = ry + oy;
+        if (px < 0 || px >= A.W || py < 0 || py >= A.H) continue;
+        float v[5];
+        sample_tmpl<5>(A.tmpl, A.N, ((double)px - x) * A.U + half_n, ((double)py - y) * A.U + half_n, v);
+        const double model = A.page[py * A.W + px] + lev;
+        const double resid = (double)A.image[py * A.pitch + px] - model;
+        double wgt = 1.0 / (A.nv0 + A.nv1 * fmax(model, 0.0));
+        rss += resid * resid;
+        cnt += 1.0;
+        if (A.rscale > 0.0) {
+            const double q = resid / A.rscale;
+            wgt = wgt / (1.0 + q * q);
+        }
+        float J[5];
+        J[0] = -fa * v[3];
+        J[1] = -fa * v[4];
+        J[2] = v[0] + fde * (v[1] + fde * v[2]);
+        J[3] = 1.0f;
+        J[4] = (K == 5) ? fa * (v[1] + 2.0f * fde * v[2]) : 0.0f;
+        const float fw = (float)wgt;
+        const double wr = wgt * resid;
+        int q = 0;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const float jw = J[r] * fw;
+#pragma unroll
+            for (int s = r; s < 5; ++s) m[q++] += jw * J[s];
+            rhs[r] += (double)J[r] * wr;
+        }
+    }
+    // group totals (every lane of the group ends with them)
+    double S[22];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) S[q] = group_sum<G>((double)m[q]);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) S[15 + r] = group_sum<G>(rhs[r]);
+    S[20] = group_sum<G>(rss);
+    S[21] = group_sum<G>(cnt);
+    if (!live) return;
+
+    // K x K system (K = 5 with the edge offset, 4 without) with the 1e-9 trace ridge, FP64 Cholesky
+    constexpr int k = K;
+    double L[5][5];
+    {
+        int q = 0;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+            for (int s = r; s < 5; ++s) {
+                L[s][r] = S[q];
+                ++q;
+            }
+    }
+    double tr = 0.0;
+    for (int r = 0; r < k; ++r) tr += L[r][r];
+    const double ridge = 1e-9 * tr / k;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        if (j >= k) break;
+        double d = L[j][j] + ridge;
+#pragma unroll
+        for (int t = 0; t < j; ++t) d -= L[j][t] * L[j][t];
+        if (!(d > 0.0)) ok = false;
+        const double lj = sqrt(fmax(d, 1e-300));
+        L[j][j] = lj;
+#pragma unroll
+        for (int i = j + 1; i < 5; ++i) {
+            if (i >= k) break;
+            double e = L[i][j];
+#pragma unroll
+            for (int t = 0; t < j; ++t) e -= L[i][t] * L[j][t];
+            L[i][j] = e / lj;
+        }
+    }
+    double step[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (ok) {
+        double z[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            if (i >= k) break;
+            double e = S[15 + i];
+#pragma unroll
+            for (int t = 0; t < i; ++t) e -= L[i][t] * z[t];
+            z[i] = e / L[i][i];
+        }
+#pragma unroll
+        for (int i = 4; i >= 0; --i) {
+            if (i >= k) continue;
+            double e = z[i];
+#pragma unroll
+            for (int t = i + 1; t < 5; ++t)
+                if (t < k) e -= L[t][i] * step[t];
+            step[i] = e / L[i][i];
+        }
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+            if (!(step[r] == step[r]) || fabs(step[r]) > 1e30) step[r] = 0.0;
+    }
+    step[2] *= A.relax;
+    step[3] *= A.relax;
+    step[4] *= A.relax;
+    if (gl != 0) return;
+    double* st = A.stat + 4 * (int64_t)slot;
+    st[2] = sqrt(S[20] / (S[21] > 0.0 ? S[21] : 1.0));
+    // (A^-1)_44 = 1 / L_44^2 for the last index of a Cholesky factor
+    st[3] = (ok && K == 5) ? sqrt(fmax(1.0 / (L[4][4] * L[4][4]), 0.0)) : __builtin_nan("");
+    A.c_old[2 * (int64_t)slot] = x;
+    A.c_old[2 * (int64_t)slot + 1] = y;
+    const double mx = fmin(fmax(step[0], -A.max_move), A.max_move);
+    const double my = fmin(fmax(step[1], -A.max_move), A.max_move);
+    A.c[2 * (int64_t)h] = x + mx;
+    A.c[2 * (int64_t)h + 1] = y + my;
+    st[0] = fmax(fabs(mx), fabs(my));
+    const double an = fmin(fmax(aa + step[2], 0.5 * aa), 2.0 * aa);
+    A.a[h] = fmin(fmax(an, 1e-3), 5.0 * A.a_med);
+    A.level[h] = lev + step[3];
+    st[1] = 0.0;
+    if (K == 5) {
+        const double dn = fmin(fmax(de + step[4], -A.max_delta), A.max_delta);
+        st[1] = fabs(dn - de);
+        A.delta[h] = dn;
+    }
+}
+
+struct RenderArgs {
+    double* page;
+    int64_t H, W;
+    float* cache;
+    const float* tmpl;
+    int N, U, Rr;
+    double rl2;
+    const int32_t* idx;
+    int n;
+    const double* c_old;
+    const double* c;
+    const double* a;
+    const double* delta;
+};
+
+__global__ __launch_bounds__(256) void hole_render_kernel(RenderArgs A) {
+    const int side = 2 * A.Rr + 1;
+    const int64_t P = (int64_t)side * side;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)A.n * P) return;
+    const int i = (int)(t / P);
+    const int p = (int)(t - (int64_t)i * P);
+    const int oy = p / side - A.Rr, ox = p - (p / side) * side - A.Rr;
+    if ((double)(ox * ox + oy * oy) > A.rl2) return;  // no near light there (nor when cached)
+    const int h = A.idx[i];
+    const float old = A.cache[t];
+    if (old != 0.0f) {
+        const int64_t pxo = (int64_t)rint(A.c_old[2 * (int64_t)i]) + ox;
+        const int64_t pyo = (int64_t)rint(A.c_old[2 * (int64_t)i + 1]) + oy;
+        atomicAdd(A.page + pyo * A.W + pxo, -(double)old);
+    }
+    const double x = A.c[2 * (int64_t)h], y = A.c[2 * (int64_t)h + 1];
+    const int64_t px = (int64_t)rint(x) + ox, py = (int64_t)rint(y) + oy;
+    float light = 0.0f;
+    if (px >= 0 && px < A.W && py >= 0 && py < A.H) {
+        float v[3];
+        const double half_n = (double)(A.N / 2);
+        sample_tmpl<3>(A.tmpl, A.N, ((double)px - x) * A.U + half_n, ((double)py - y) * A.U + half_n, v);
+        const double de = A.delta[h];
+        light = (float)(A.a[h] * ((double)v[0] + de * ((double)v[1] + de * (double)v[2])));
+        atomicAdd(A.page + py * A.W + px, (double)light);
+    }
+    A.cache[t] = light;
+}
+
+// Keys splat of every hole's far-light amplitude a (1 + 4 delta / cd0) on the padded pixel grid.
+__global__ void far_splat_kernel(const double* c, const double* a, const double* delta, double cd0, int n, int mx,
+                                 int my, int hp, int wp, float* grid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = c[2 * (int64_t)i] + mx, y = c[2 * (int64_t)i + 1] + my;
+    const double fx = floor(x), fy = floor(y);
+    const int ix = (int)fx, iy = (int)fy;
+    if (ix < 1 || ix + 2 >= wp || iy < 1 || iy + 2 >= hp) return;
+    double wx[4], wy[4];
+    keys4d(x - fx, wx);
+    keys4d(y - fy, wy);
+    const double am = a[i] * (1.0 + 4.0 * delta[i] / cd0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+            atomicAdd(grid + (int64_t)(iy + q - 1) * wp + (ix + p - 1), (float)(am * wy[q] * wx[p]));
+}
+
+// Analytic near templates of an erf-edged disk 1/2 erfc((rho - r - d) / (sqrt2 s)) and its far kernel.
+__global__ void disk_templates_kernel(double radius, double sigma, int N, int U, double r0, double ramp, int B,
+                                      float* tmpl, float* far) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double kPi = 3.14159265358979323846;
+    const double s2 = 1.4142135623730951 * sigma;
+    const int64_t nn = (int64_t)N * N;
+    if (t < nn) {
+        const int iy = (int)(t / N), ix = (int)(t - (int64_t)iy * N);
+        const double X = (double)(ix - N / 2) / U, Y = (double)(iy - N / 2) / U;
+        const double rho = sqrt(X * X + Y * Y);
+        const double u = (rho - radius) / s2;
+        const double t0 = 0.5 * erfc(u);
+        const double g = exp(-u * u) / (2.5066282746310002 * sigma);
+        const double t2 = g * u / s2;
+        const double sr = fmin(fmax((rho - r0) / ramp, 0.0), 1.0);
+        const double w = 0.5 * (1.0 + cos(kPi * sr));
+        const double dw = (rho > r0 && rho < r0 + ramp) ? -0.5 * kPi * sin(kPi * sr) / ramp : 0.0;
+        const double drho = -g * w + t0 * dw;
+        const double ux = rho > 0.0 ? X / rho : 0.0, uy = rho > 0.0 ? Y / rho : 0.0;
+        float* o = tmpl + t * 8;
+        o[0] = (float)(t0 * w);
+        o[1] = (float)(g * w);
+        o[2] = (float)(t2 * w);
+        o[3] = (float)(drho * ux);
+        o[4] = (float)(drho * uy);
+        o[5] = 0.0f;
+        o[6] = 0.0f;
+        o[7] = 0.0f;
+    } else if (far != nullptr && t < nn + (int64_t)B * B) {
+        const int64_t q = t - nn;
+        const int iy = (int)(q / B), ix = (int)(q - (int64_t)iy * B);
+        const double X = (double)(ix - B / 2), Y = (double)(iy - B / 2);
+        const double rho = sqrt(X * X + Y * Y);
+        const double t0 = 0.5 * erfc((rho - radius) / s2);
+        const double wf = 1.0 - 0.5 * (1.0 + cos(kPi * fmin(fmax((rho - r0) / ramp, 0.0), 1.0)));
+        far[q] = (float)(t0 * wf);
+    }
+}
+
+}  // namespace
+
+int hole_step_launch(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page, const float* tmpl,
+                     int N, int U, int R, const int32_t* idx, int n, double* c, double* a, double* level,
+                     double* delta, double* c_old, double nv0, double nv1, double rscale, int size, double max_move,
+                     double max_delta, double a_med, double relax, double* stat, hipStream_t stream) {
+    if (n == 0) return SPX_OK;
+    HoleArgs A{image, H, W, pitch, page, tmpl, N, U, R, idx, n, c, a, level, delta, c_old, nv0, nv1, rscale, size,
+               max_move, max_delta, a_med, relax, stat};
+    const int npx = (2 * R + 1) * (2 * R + 1);
+    const int G = npx < 512 ? 16 : (npx < 2048 ? 32 : 64);
+    const int64_t threads = (int64_t)n * G;
+    const int blocks = (int)((threads + 255) / 256);
+    if (size) {
+        if (G == 16)
+            hole_step_kernel<16, 5><<<blocks, 256, 0, stream>>>(A);
+        else if (G == 32)
+            hole_step_kernel<32, 5><<<blocks, 256, 0, stream>>>(A);
+        else
+            hole_step_kernel<64, 5><<<blocks, 256, 0, stream>>>(A);
+    } else {
+        if (G == 16)
+            hole_step_kernel<16, 4><<<blocks, 256, 0, stream>>>(A);
+        else if (G == 32)
+            hole_step_kernel<32, 4><<<blocks, 256, 0, stream>>>(A);
+        else
+            hole_step_kernel<64, 4><<<blocks, 256, 0, stream>>>(A);
+    }
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int N, int U, int Rr,
+                       double rl2, const int32_t* idx, int n, const double* c_old, const double* c, const double* a,
+                       const double* delta, hipStream_t stream) {
+    if (n == 0) return SPX_OK;
+    RenderArgs A{page, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta};
+    const int64_t P = (int64_t)(2 * Rr + 1) * (2 * Rr + 1);
+    const int64_t total = (int64_t)n * P;
+    hole_render_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int far_splat_launch(const double* c, const double* a, const double* delta, double cd0, int n, int mx, int my, int hp,
+                     int wp, float* grid, hipStream_t stream) {
+    if (n == 0) return SPX_OK;
+    far_splat_kernel<<<(n + 255) / 256, 256, 0, stream>>>(c, a, delta, cd0, n, mx, my, hp, wp, grid);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int disk_templates_launch(double radius, double sigma, int N, int U, double r0, double ramp, int B, float* tmpl,
+                          float* far, hipStream_t stream) {
+    const int64_t total = (int64_t)N * N + (far ? (int64_t)B * B : 0);
+    disk_templates_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(radius, sigma, N, U, r0, ramp, B, tmpl,
+                                                                                far);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+}  // namespace spx

@@ -99,6 +99,17 @@ int render_page_launch(uint64_t seed, const float* holes, int n, int64_t H, int6
                        float read_sigma, uint16_t* img, int64_t pitch, hipStream_t stream);
 int render_tiles_launch(uint64_t seed, int n, int T, const SpxTileSpec& spec, uint16_t* img, int64_t pitch,
                         float* truth, hipStream_t stream);
+int hole_step_launch(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page, const float* tmpl,
+                     int N, int U, int R, const int32_t* idx, int n, double* c, double* a, double* level,
+                     double* delta, double* c_old, double nv0, double nv1, double rscale, int size, double max_move,
+                     double max_delta, double a_med, double relax, double* stat, hipStream_t stream);
+int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int N, int U, int Rr,
+                       double rl2, const int32_t* idx, int n, const double* c_old, const double* c, const double* a,
+                       const double* delta, hipStream_t stream);
+int far_splat_launch(const double* c, const double* a, const double* delta, double cd0, int n, int mx, int my, int hp,
+                     int wp, float* grid, hipStream_t stream);
+int disk_templates_launch(double radius, double sigma, int N, int U, double r0, double ramp, int B, float* tmpl,
+                          float* far, hipStream_t stream);
 
 }  // namespace spx
 
@@ -390,6 +401,50 @@ int spx_refine_logquad(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
     SPX_REQUIRE(pad >= 0, "spx_refine_logquad: pad < 0");
     return spx::logquad_launch(img, H, W, pitch, labels, rows, K, pad, centers_out, ok_out,
                                spx::as_stream(stream));
+}
+
+int spx_hole_templates(double radius, double sigma, int32_t N, int32_t U, double r0, double ramp, int32_t far_box,
+                       float* tmpl, float* far, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(tmpl != nullptr, "spx_hole_templates: null pointer");
+    SPX_REQUIRE(N >= 8 && N <= 8192 && (N % 2) == 0 && U >= 1, "spx_hole_templates: bad grid");
+    SPX_REQUIRE(radius > 0 && sigma > 0 && ramp > 0, "spx_hole_templates: radius, sigma and ramp must be > 0");
+    SPX_REQUIRE(far == nullptr || (far_box >= 2 && far_box <= 4096), "spx_hole_templates: bad far box");
+    return spx::disk_templates_launch(radius, sigma, N, U, r0, ramp, far_box, tmpl, far, spx::as_stream(stream));
+}
+
+int spx_hole_step(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page, const float* tmpl,
+                  int32_t N, int32_t U, int32_t R, const int32_t* idx, int32_t n, double* c, double* a, double* level,
+                  double* delta, double* c_old, const SpxHoleStepOptions* opt, double* stat, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(opt != nullptr, "spx_hole_step: null options");
+    SPX_REQUIRE(n >= 0 && H > 0 && W > 0 && pitch >= W, "spx_hole_step: bad sizes");
+    SPX_REQUIRE(R >= 1 && R <= 255 && N >= 8 && U >= 1, "spx_hole_step: bad window or grid");
+    SPX_REQUIRE(n == 0 || (image && page && tmpl && idx && c && a && level && delta && c_old && stat),
+                "spx_hole_step: null pointer");
+    SPX_REQUIRE(opt->nv0 > 0 && opt->nv1 >= 0, "spx_hole_step: noise variance line must be positive");
+    return spx::hole_step_launch(image, H, W, pitch, page, tmpl, N, U, R, idx, n, c, a, level, delta, c_old, opt->nv0,
+                                 opt->nv1, opt->rscale, opt->size ? 1 : 0, opt->max_move, opt->max_delta, opt->a_med,
+                                 opt->relax, stat, spx::as_stream(stream));
+}
+
+int spx_hole_render(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int32_t N, int32_t U,
+                    int32_t Rr, double rl2, const int32_t* idx, int32_t n, const double* c_old, const double* c,
+                    const double* a, const double* delta, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(n >= 0 && H > 0 && W > 0 && Rr >= 0 && Rr <= 255 && N >= 8 && U >= 1, "spx_hole_render: bad sizes");
+    SPX_REQUIRE(n == 0 || (page && cache && tmpl && idx && c_old && c && a && delta), "spx_hole_render: null pointer");
+    SPX_REQUIRE((int64_t)n * (2 * Rr + 1) * (2 * Rr + 1) < ((int64_t)1 << 40), "spx_hole_render: too many offsets");
+    return spx::hole_render_launch(page, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta,
+                                   spx::as_stream(stream));
+}
+
+int spx_far_splat(const double* c, const double* a, const double* delta, double cd0, int32_t n, int32_t mx, int32_t my,
+                  int32_t hp, int32_t wp, float* grid, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(n >= 0 && hp > 0 && wp > 0 && cd0 > 0, "spx_far_splat: bad sizes");
+    SPX_REQUIRE(n == 0 || (c && a && delta && grid), "spx_far_splat: null pointer");
+    return spx::far_splat_launch(c, a, delta, cd0, n, mx, my, hp, wp, grid, spx::as_stream(stream));
 }
 
 }  // extern "C"

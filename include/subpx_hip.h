@@ -294,11 +294,6 @@ int spx_cnn_train_step(const uint16_t* x, const float* target, const float* weig
 int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, int32_t B, float* theta,
                  uint16_t* theta_bf, float* grad_out, float* loss_out, spx_stream_t stream);
 
-/* Render n synthetic tiles of side `tile` into a mosaic (tile t at column t % cols, row
- * t / cols): one hole at the tile centre plus its array neighbours, erf-disk / ellipse
- * model, photon noise in its normal limit + read noise, uint16. truth_out (n, 7)
- * [x0, y0, r, s, A, B, q] of the centre hole in mosaic coordinates. img, truth: device.
- * No reference counterpart (training data for the learned start). */
 /* Whole synthetic page: holes (n, 6) f32 [x, y, r, s, q, A] in image coordinates (device),
  * each I += A erfc((rho - r) / (sqrt2 s)) / 2 with rho = sqrt(dx^2 + (dy / q)^2), on a constant
  * background, then N(I, I) photon noise (normal limit) + N(0, read_sigma^2), rounded and
@@ -307,11 +302,66 @@ int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, in
 int spx_render_page(uint64_t seed, const float* holes, int32_t n, int64_t H, int64_t W, float background,
                     float read_sigma, uint16_t* img, int64_t pitch, spx_stream_t stream);
 
+/* Render n synthetic tiles of side `tile` into a mosaic (tile t at column t % cols, row
+ * t / cols): one hole at the tile centre plus its array neighbours, erf-disk / ellipse
+ * model, photon noise in its normal limit + read noise, uint16. truth_out (n, 7)
+ * [x0, y0, r, s, A, B, q] of the centre hole in mosaic coordinates. img, truth: device.
+ * No reference counterpart (training data for the learned start). */
 int spx_render_tiles(uint64_t seed, int32_t n, int32_t tile, const SpxTileSpec* spec, uint16_t* img,
                      int64_t pitch, float* truth_out, spx_stream_t stream);
 
 /* theta_bf = BF16(theta), n values, device. */
 int spx_cnn_to_bf16(const float* theta, int64_t n, uint16_t* theta_bf, spx_stream_t stream);
+
+/* ---- Outline hole stage (reference subpx/_gpu/outline_onepass.py:252-427, the GPU form of
+ * PageFit.fit_holes, subpx/_outline/page.py:587-729) ----
+ * Page model P(p) = sum_i a_i [T0 + d_i T1 + d_i^2 T2](p - c_i) + far light, FP64 (H, W)
+ * contiguous; hole i fits data(p) = P(p) + level_i on its (2R+1)^2 window. Hole state arrays
+ * c (n, 2), a, level, delta (n) are FP64 device arrays indexed by hole. */
+
+/* Template grid of an erf-edged disk 1/2 erfc((rho - r - d) / (sqrt2 s)): N x N fine points,
+ * U per pixel, centre at N/2, 8 floats per point [T0, T1, T2, Tx, Ty, 0, 0, 0] with
+ * T1 = dT/dd, T2 = (1/2) d2T/dd2 and Tx, Ty the spatial derivatives of T0 w, each times the
+ * near window w(rho) = (1 + cos(pi clip((rho - r0) / ramp, 0, 1))) / 2. far (far_box^2, or
+ * NULL) = T0 (1 - w) at whole-pixel offsets, centre at far_box / 2. Stands in for the
+ * outline templates of set_template (outline_onepass.py:730-793) with a disk outline. */
+int spx_hole_templates(double radius, double sigma, int32_t N, int32_t U, double r0, double ramp,
+                       int32_t far_box, float* tmpl, float* far, spx_stream_t stream);
+
+typedef struct SpxHoleStepOptions {
+    double nv0, nv1;   /* pixel weight 1 / (nv0 + nv1 max(model, 0)) */
+    double rscale;     /* Cauchy weight 1 / (1 + (r / rscale)^2); 0 = off */
+    double max_move;   /* centre step clip, px */
+    double max_delta;  /* edge offset bound, px */
+    double a_med;      /* brightness bound 5 a_med */
+    double relax;      /* factor on the brightness, level and edge-offset steps */
+    int32_t size;      /* 1: the edge offset is fitted (5 parameters), 0: held (4) */
+    int32_t reserved;
+} SpxHoleStepOptions;
+
+/* One Gauss-Newton step of the holes idx[0..n) (one colour class; windows must not overlap
+ * another hole of the class being written) against the page model: normal equations of
+ * (x, y, a, level[, delta]) with noise x Cauchy weights, 1e-9 trace ridge, bounded step
+ * (outline_onepass.py:252-363: hole_normal + hole_update). image FP32 (H, pitch). Writes
+ * c_old (n, 2) = centres before the step and stat (n, 4) = [centre move, edge-offset move,
+ * leftover rms, edge-offset standard error]. */
+int spx_hole_step(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,
+                  const float* tmpl, int32_t N, int32_t U, int32_t R, const int32_t* idx, int32_t n,
+                  double* c, double* a, double* level, double* delta, double* c_old,
+                  const SpxHoleStepOptions* opt, double* stat, spx_stream_t stream);
+
+/* Take the holes' cached light (cache (n, (2Rr+1)^2) FP32, at offsets around rint(c_old))
+ * out of the page model and put their light at the current state in; offsets with
+ * ox^2 + oy^2 > rl2 are skipped (outline_onepass.py:365-397). */
+int spx_hole_render(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int32_t N,
+                    int32_t U, int32_t Rr, double rl2, const int32_t* idx, int32_t n, const double* c_old,
+                    const double* c, const double* a, const double* delta, spx_stream_t stream);
+
+/* Keys splat of a_i (1 + 4 delta_i / cd0) at (c_i + (mx, my)) on the padded FP32 grid
+ * (hp, wp) (outline_onepass.py:399-427); the far light is this grid convolved with the far
+ * kernel. */
+int spx_far_splat(const double* c, const double* a, const double* delta, double cd0, int32_t n, int32_t mx,
+                  int32_t my, int32_t hp, int32_t wp, float* grid, spx_stream_t stream);
 
 #ifdef __cplusplus
 }
