@@ -106,6 +106,8 @@ int hole_step_launch(const float* image, int64_t H, int64_t W, int64_t pitch, co
 int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int N, int U, int Rr,
                        double rl2, const int32_t* idx, int n, const double* c_old, const double* c, const double* a,
                        const double* delta, hipStream_t stream);
+int select_kth_launch(const double* x, int64_t n, int64_t stride, const int64_t* ks, int nk, double* out,
+                      hipStream_t stream);
 int far_splat_launch(const double* c, const double* a, const double* delta, double cd0, int n, int mx, int my, int hp,
                      int wp, float* grid, hipStream_t stream);
 int disk_templates_launch(double radius, double sigma, int N, int U, double r0, double ramp, int B, float* tmpl,
@@ -437,6 +439,45 @@ int spx_hole_render(double* page, int64_t H, int64_t W, float* cache, const floa
     SPX_REQUIRE((int64_t)n * (2 * Rr + 1) * (2 * Rr + 1) < ((int64_t)1 << 40), "spx_hole_render: too many offsets");
     return spx::hole_render_launch(page, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta,
                                    spx::as_stream(stream));
+}
+
+int spx_hole_sweep(const float* image, int64_t H, int64_t W, int64_t pitch, double* page, const float* tmpl, int32_t N,
+                   int32_t U, int32_t R, int32_t Rr, double rl2, const int32_t* order, const int32_t* start,
+                   int32_t ncolour, double* c, double* a, double* level, double* delta, double* c_old, float* cache,
+                   const SpxHoleStepOptions* opt, double* stat, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(opt != nullptr && start != nullptr, "spx_hole_sweep: null options or offsets");
+    SPX_REQUIRE(ncolour >= 0 && H > 0 && W > 0 && pitch >= W, "spx_hole_sweep: bad sizes");
+    SPX_REQUIRE(R >= 1 && R <= 255 && Rr >= 0 && Rr <= 255 && N >= 8 && U >= 1, "spx_hole_sweep: bad window or grid");
+    SPX_REQUIRE(opt->nv0 > 0 && opt->nv1 >= 0, "spx_hole_sweep: noise variance line must be positive");
+    SPX_REQUIRE(start[0] == 0, "spx_hole_sweep: offsets must start at 0");
+    for (int j = 0; j < ncolour; ++j) SPX_REQUIRE(start[j + 1] >= start[j], "spx_hole_sweep: offsets must not decrease");
+    const int32_t n = start[ncolour];
+    SPX_REQUIRE(n == 0 || (image && page && tmpl && order && c && a && level && delta && c_old && cache && stat),
+                "spx_hole_sweep: null pointer");
+    const int64_t P = (int64_t)(2 * Rr + 1) * (2 * Rr + 1);
+    hipStream_t s = spx::as_stream(stream);
+    for (int j = 0; j < ncolour; ++j) {
+        const int32_t b = start[j], m = start[j + 1] - start[j];
+        if (m == 0) continue;
+        int rc = spx::hole_step_launch(image, H, W, pitch, page, tmpl, N, U, R, order + b, m, c, a, level, delta,
+                                       c_old + 2 * (int64_t)b, opt->nv0, opt->nv1, opt->rscale, opt->size ? 1 : 0,
+                                       opt->max_move, opt->max_delta, opt->a_med, opt->relax, stat + 4 * (int64_t)b, s);
+        if (rc != SPX_OK) return rc;
+        rc = spx::hole_render_launch(page, H, W, cache + P * b, tmpl, N, U, Rr, rl2, order + b, m,
+                                     c_old + 2 * (int64_t)b, c, a, delta, s);
+        if (rc != SPX_OK) return rc;
+    }
+    return SPX_OK;
+}
+
+int spx_select_kth(const double* x, int64_t n, int64_t stride, const int64_t* ks, int32_t nk, double* out,
+                   spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(ks != nullptr && nk >= 0 && nk <= 8, "spx_select_kth: 0..8 ranks");
+    SPX_REQUIRE(n > 0 && stride >= 1 && x != nullptr && out != nullptr, "spx_select_kth: bad input");
+    for (int j = 0; j < nk; ++j) SPX_REQUIRE(ks[j] >= 0 && ks[j] < n, "spx_select_kth: rank out of range");
+    return spx::select_kth_launch(x, n, stride, ks, nk, out, spx::as_stream(stream));
 }
 
 int spx_far_splat(const double* c, const double* a, const double* delta, double cd0, int32_t n, int32_t mx, int32_t my,

@@ -230,50 +230,68 @@ class HolePage:
         rl2 = float((self.r_light + 0.75) ** 2)
         sh = stream_handle
         with gpu_device(self.device) as dev:
+            # holes in colour order: order[start[j]:start[j+1]] is colour j; c_old, the light cache
+            # and the statistics follow the same order
+            order = torch.cat(chunks).contiguous()
+            start_np = np.cumsum([0] + [int(ix.shape[0]) for ix in chunks]).astype(np.int32)
+            start = (ctypes.c_int32 * len(start_np))(*start_np.tolist())
+            ncol = len(chunks)
             page = torch.zeros(H * W, dtype=torch.float64, device=dev)
-            caches = [torch.zeros(int(ix.shape[0]) * P, dtype=torch.float32, device=dev) for ix in chunks]
-            c_olds = []
-            for ix, cache in zip(chunks, caches):
-                c_old = self.c[ix.long()].contiguous()
-                _lib.call("spx_hole_render", _ptr(page), H, W, _ptr(cache), _ptr(self.tmpl), N, U, Rr, rl2, _ptr(ix),
-                          int(ix.shape[0]), _ptr(c_old), _ptr(self.c), _ptr(self.a), _ptr(self.delta), sh())
-                c_olds.append(c_old)
+            cache = torch.zeros(self.n * P, dtype=torch.float32, device=dev)
+            c_old = self.c[order.long()].contiguous()
+            # first light of every hole (the cache is empty, so nothing is taken out)
+            _lib.call("spx_hole_render", _ptr(page), H, W, _ptr(cache), _ptr(self.tmpl), N, U, Rr, rl2, _ptr(order),
+                      self.n, _ptr(c_old), _ptr(self.c), _ptr(self.a), _ptr(self.delta), sh())
             far_l = self.far_field().reshape(-1) if far else None
             if far:
                 page += far_l
             rms_typ = self.noise
             a_med = float(np.median(self.a.cpu().numpy()))
             stat = torch.zeros((self.n, 4), dtype=torch.float64, device=dev)
-            offs = np.cumsum([0] + [int(ix.shape[0]) for ix in chunks])
             opt = _lib.SpxHoleStepOptions(nv0=self.noise ** 2, nv1=0.0, rscale=0.0, max_move=float(max_move),
                                           max_delta=float(max_delta), a_med=a_med, relax=float(relax),
                                           size=int(bool(size)), reserved=0)
             log = []
             for it in range(int(sweeps)):
                 opt.rscale = float(robust_c) * max(rms_typ, 1e-6) if robust_c else 0.0
-                for j, ix in enumerate(chunks):
-                    n = int(ix.shape[0])
-                    st = stat[offs[j]:offs[j + 1]]
-                    _lib.call("spx_hole_step", _ptr(self.img), H, W, W, _ptr(page), _ptr(self.tmpl), N, U, R,
-                              _ptr(ix), n, _ptr(self.c), _ptr(self.a), _ptr(self.level), _ptr(self.delta),
-                              _ptr(c_olds[j]), ctypes.byref(opt), _ptr(st), sh())
-                    _lib.call("spx_hole_render", _ptr(page), H, W, _ptr(caches[j]), _ptr(self.tmpl), N, U, Rr, rl2,
-                              _ptr(ix), n, _ptr(c_olds[j]), _ptr(self.c), _ptr(self.a), _ptr(self.delta), sh())
+                _lib.call("spx_hole_sweep", _ptr(self.img), H, W, W, _ptr(page), _ptr(self.tmpl), N, U, R, Rr, rl2,
+                          _ptr(order), start, ncol, _ptr(self.c), _ptr(self.a), _ptr(self.level), _ptr(self.delta),
+                          _ptr(c_old), _ptr(cache), ctypes.byref(opt), _ptr(stat), sh())
                 if far and it + 1 < sweeps:
                     new_far = self.far_field().reshape(-1)
                     page += new_far - far_l
                     far_l = new_far
-                st_h = stat.cpu().numpy()
-                rms_typ = float(np.median(st_h[:, 2]))
-                q = np.percentile(st_h[:, :2], 99, axis=0)
-                log.append(dict(sweep=it + 1, move99=float(q[0]), delta99=float(q[1]), rms=rms_typ))
-            order = torch.cat([ix.long() for ix in chunks])
+                rms_typ, m99, d99 = self._sweep_stats(stat)
+                log.append(dict(sweep=it + 1, move99=m99, delta99=d99, rms=rms_typ))
             self.hole_rms = torch.empty(self.n, dtype=torch.float64, device=dev)
-            self.hole_rms[order] = stat[:, 2]
+            self.hole_rms[order.long()] = stat[:, 2]
             self.stat = stat
             if keep is not None:
                 keep["light"] = page.reshape(H, W)
         return log
+
+    def _sweep_stats(self, stat: torch.Tensor):
+        """(median leftover rms, 99th percentile centre move, 99th percentile edge-offset move)
+        of a sweep, by NumPy's rules (median: mean of the middle pair; percentile: linear
+        interpolation), from six exact order statistics selected on the device."""
+        n = self.n
+        pos = 0.99 * (n - 1)
+        lo, hi = int(math.floor(pos)), int(math.ceil(pos))
+        t = pos - lo
+        out = torch.empty(6, dtype=torch.float64, device=stat.device)
+        ks = (ctypes.c_int64 * 2)((n - 1) // 2, n // 2)
+        _lib.call("spx_select_kth", int(stat.data_ptr()) + 2 * 8, n, 4, ks, 2, _ptr(out), stream_handle())
+        ks = (ctypes.c_int64 * 2)(lo, hi)
+        for col in (0, 1):
+            _lib.call("spx_select_kth", int(stat.data_ptr()) + col * 8, n, 4, ks, 2, int(out.data_ptr()) + 8 * (2 + 2 * col),
+                      stream_handle())
+        v = out.cpu().tolist()
+
+        def lerp(a, b):  # numpy's _lerp
+            d = b - a
+            return b - d * (1.0 - t) if t >= 0.5 else a + d * t
+
+        return 0.5 * (v[0] + v[1]), lerp(v[2], v[3]), lerp(v[4], v[5])
 
     def result(self, log=None) -> HolePassResult:
         return HolePassResult(

@@ -357,6 +357,22 @@ int spx_hole_render(double* page, int64_t H, int64_t W, float* cache, const floa
                     int32_t U, int32_t Rr, double rl2, const int32_t* idx, int32_t n, const double* c_old,
                     const double* c, const double* a, const double* delta, spx_stream_t stream);
 
+/* One Gauss-Seidel sweep over the colour classes: for j = 0 .. ncolour-1, spx_hole_step then
+ * spx_hole_render of the holes order[start[j] .. start[j+1]) (start: host array of ncolour + 1
+ * offsets; order, c_old (n, 2), cache (n, (2Rr+1)^2) and stat (n, 4): device, in colour order).
+ * The reference's per-colour loop of fit_holes (outline_onepass.py:855-870) in one call. */
+int spx_hole_sweep(const float* image, int64_t H, int64_t W, int64_t pitch, double* page, const float* tmpl,
+                   int32_t N, int32_t U, int32_t R, int32_t Rr, double rl2, const int32_t* order,
+                   const int32_t* start, int32_t ncolour, double* c, double* a, double* level, double* delta,
+                   double* c_old, float* cache, const SpxHoleStepOptions* opt, double* stat, spx_stream_t stream);
+
+/* Order statistics: out[j] = the ks[j]-th smallest (0-based) of x[i * stride], i < n, for
+ * j < nk <= 8 (ks: host array; x, out: device; no NaNs in x). Exact selection by an 8-round
+ * radix select over the doubles' order-preserving 64-bit keys. Serves the hole pass's median
+ * leftover and 99th-percentile moves (cp.median / cp.percentile in outline_onepass.py:871-873). */
+int spx_select_kth(const double* x, int64_t n, int64_t stride, const int64_t* ks, int32_t nk, double* out,
+                   spx_stream_t stream);
+
 /* Keys splat of a_i (1 + 4 delta_i / cd0) at (c_i + (mx, my)) on the padded FP32 grid
  * (hp, wp) (outline_onepass.py:399-427); the far light is this grid convolved with the far
  * kernel. */

@@ -150,3 +150,38 @@ def test_bad_arguments_raise():
     import ctypes
     with pytest.raises(ValueError):
         _lib.call("spx_hole_step", 1, 10, 10, 10, 1, 1, 256, 8, 3, 1, 1, 1, 1, 1, 1, 1, ctypes.byref(opt), 1, None)
+
+
+def test_select_kth_matches_sorting():
+    import ctypes
+
+    from imageprocessor_new_amd import _lib
+    from imageprocessor_new_amd.backends import gpu_device, stream_handle
+
+    rng = np.random.default_rng(21)
+    x = rng.normal(size=(5001, 4))
+    x[::7, 1] = 0.5
+    x[::3, 2] = -0.0
+    x[1::3, 2] = 0.0
+    x[:, 3] = np.abs(x[:, 3]) * 1e-300
+    with gpu_device(0) as dev:
+        xt = torch.from_numpy(x).to(dev)
+        out = torch.empty(8, dtype=torch.float64, device=dev)
+        for col in range(4):
+            ks = [0, 2500, 5000, 1234, 4950, 17, 2499, 3]
+            _lib.call("spx_select_kth", int(xt.data_ptr()) + 8 * col, 5001, 4, (ctypes.c_int64 * 8)(*ks), 8,
+                      out.data_ptr(), stream_handle())
+            got = out.cpu().numpy()
+            want = np.sort(x[:, col])[ks]
+            assert np.array_equal(got, want), (col, got, want)
+    pg_stats = None
+    from imageprocessor_new_amd import holes
+
+    pg = holes.HolePage.__new__(holes.HolePage)
+    pg.n = 5001
+    with gpu_device(0) as dev:
+        st = torch.from_numpy(np.abs(x)).to(dev)
+        pg_stats = pg._sweep_stats(st)
+    assert pg_stats[0] == np.median(np.abs(x[:, 2]))
+    assert pg_stats[1] == np.percentile(np.abs(x[:, 0]), 99)
+    assert pg_stats[2] == np.percentile(np.abs(x[:, 1]), 99)
