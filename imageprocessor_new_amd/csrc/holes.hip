@@ -15,8 +15,9 @@
 //     cross-lane swaps in a fixed order (deterministic, no atomics), then solve the 5 x 5
 //     (4 x 4 without the edge offset) system by FP64 Cholesky and apply the bounded step:
 //     the reference's hole_normal + hole_update in one launch;
-//   - `hole_render_kernel` is one thread per (hole, render offset): old light out, new light in,
-//     by FP64 atomics into the page model, as the reference.
+//   - `hole_render_kernel` is one thread per (hole, render offset inside the light's reach):
+//     old light out, new light in, by FP64 atomics into the page model (one atomic when the
+//     rounded centre did not move), as the reference.
 #include "spx_internal.hpp"
 
 namespace spx {
@@ -268,17 +269,47 @@ struct RenderArgs {
     const double* delta;
 };
 
-__global__ __launch_bounds__(256) void hole_render_kernel(RenderArgs A) {
+// Offsets of the render window inside the light's reach, ox^2 + oy^2 <= rl2, row by row: row
+// oy holds ox in [-w, w] with w the largest such |ox| (none when oy^2 > rl2).
+__host__ __device__ inline int reach_half_width(int oy, int Rr, double rl2) {
+    if ((double)(oy * oy) > rl2) return -1;
+    int w = 0;
+    while (w < Rr && (double)((w + 1) * (w + 1) + oy * oy) <= rl2) ++w;
+    return w;
+}
+
+// One thread per (hole, offset inside the reach); the block builds the row table in LDS.
+__global__ __launch_bounds__(256) void hole_render_kernel(RenderArgs A, int Pn) {
+    __shared__ int row_start[512];
+    __shared__ int row_w[512];
     const int side = 2 * A.Rr + 1;
-    const int64_t P = (int64_t)side * side;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)A.n * P) return;
-    const int i = (int)(t / P);
-    const int p = (int)(t - (int64_t)i * P);
-    const int oy = p / side - A.Rr, ox = p - (p / side) * side - A.Rr;
-    if ((double)(ox * ox + oy * oy) > A.rl2) return;  // no near light there (nor when cached)
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int r = 0; r < side; ++r) {
+            const int w = reach_half_width(r - A.Rr, A.Rr, A.rl2);
+            row_start[r] = acc;
+            row_w[r] = w;
+            acc += 2 * w + 1;
+        }
+        row_start[side] = acc;
+    }
+    __syncthreads();
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)A.n * (unsigned)Pn) return;
+    const int i = (int)(t / (unsigned)Pn);
+    const int q = (int)(t - (unsigned)i * (unsigned)Pn);
+    int lo = 0, hi = side;  // row r with row_start[r] <= q < row_start[r + 1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (row_start[mid] <= q)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int oy = lo - A.Rr, ox = q - row_start[lo] - row_w[lo];
+    const int64_t slot = (int64_t)i * side * side + (int64_t)(oy + A.Rr) * side + (ox + A.Rr);
     const int h = A.idx[i];
-    const float old = A.cache[t];
+    const float old = A.cache[slot];
     const int64_t pxo = (int64_t)rint(A.c_old[2 * (int64_t)i]) + ox;
     const int64_t pyo = (int64_t)rint(A.c_old[2 * (int64_t)i + 1]) + oy;
     const double x = A.c[2 * (int64_t)h], y = A.c[2 * (int64_t)h + 1];
@@ -299,7 +330,7 @@ __global__ __launch_bounds__(256) void hole_render_kernel(RenderArgs A) {
         if (old != 0.0f) atomicAdd(A.page + pyo * A.W + pxo, -(double)old);
         if (inside) atomicAdd(A.page + py * A.W + px, (double)light);
     }
-    A.cache[t] = light;
+    A.cache[slot] = light;
 }
 
 // Keys splat of every hole's far-light amplitude a (1 + 4 delta / cd0) on the padded pixel grid.
@@ -375,6 +406,7 @@ __device__ __forceinline__ double from_key(uint64_t k) {
 
 struct KthList {
     int64_t k[8];
+    int64_t col[8];
 };
 
 __global__ __launch_bounds__(1024) void select_kth_kernel(const double* x, int64_t n, int64_t stride, KthList ks,
@@ -382,6 +414,8 @@ __global__ __launch_bounds__(1024) void select_kth_kernel(const double* x, int64
     __shared__ unsigned int hist[256];
     __shared__ uint64_t s_prefix;
     __shared__ int64_t s_k;
+    const double* xc = x + ks.col[blockIdx.x];
+    const int lane = threadIdx.x & 63;
     if (threadIdx.x == 0) {
         s_prefix = 0;
         s_k = ks.k[blockIdx.x];
@@ -391,9 +425,27 @@ __global__ __launch_bounds__(1024) void select_kth_kernel(const double* x, int64
         for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
         __syncthreads();
         const uint64_t prefix = s_prefix;
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint64_t key = order_key(x[i * stride]);
-            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        for (int64_t base = 0; base < n; base += blockDim.x) {
+            const int64_t i = base + threadIdx.x;
+            bool part = false;
+            unsigned int d = 0;
+            if (i < n) {
+                const uint64_t key = order_key(xc[i * stride]);
+                if ((key & mask) == prefix) {
+                    part = true;
+                    d = (unsigned int)((key >> shift) & 255u);
+                }
+            }
+            // one LDS atomic per distinct digit of the wave (the high digits are mostly equal)
+            uint64_t pending = __ballot(part);
+            while (pending) {
+                const int leader = __ffsll((unsigned long long)pending) - 1;
+                const unsigned int dl = __shfl(d, leader, 64);
+                const uint64_t same = __ballot(part && d == dl);
+                if (lane == leader) atomicAdd(&hist[dl], (unsigned int)__popcll(same));
+                if (part && d == dl) part = false;
+                pending &= ~same;
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -414,11 +466,14 @@ __global__ __launch_bounds__(1024) void select_kth_kernel(const double* x, int64
 
 }  // namespace
 
-int select_kth_launch(const double* x, int64_t n, int64_t stride, const int64_t* ks, int nk, double* out,
-                      hipStream_t stream) {
+int select_kth_launch(const double* x, int64_t n, int64_t stride, const int64_t* cols, const int64_t* ks, int nk,
+                      double* out, hipStream_t stream) {
     if (nk == 0) return SPX_OK;
     KthList kl{};
-    for (int i = 0; i < nk; ++i) kl.k[i] = ks[i];
+    for (int i = 0; i < nk; ++i) {
+        kl.k[i] = ks[i];
+        kl.col[i] = cols ? cols[i] : 0;
+    }
     select_kth_kernel<<<nk, 1024, 0, stream>>>(x, n, stride, kl, out);
     SPX_LAUNCHED();
     return SPX_OK;
@@ -448,9 +503,15 @@ int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const f
                        const double* delta, hipStream_t stream) {
     if (n == 0) return SPX_OK;
     RenderArgs A{page, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta};
-    const int64_t P = (int64_t)(2 * Rr + 1) * (2 * Rr + 1);
-    const int64_t total = (int64_t)n * P;
-    hole_render_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A);
+    int Pn = 0;
+    for (int oy = -Rr; oy <= Rr; ++oy) Pn += 2 * reach_half_width(oy, Rr, rl2) + 1;
+    if (Pn == 0) return SPX_OK;
+    const int64_t total = (int64_t)n * Pn;
+    if (total >= ((int64_t)1 << 32)) {
+        set_error("hole_render: too many (hole, offset) pairs for one launch");
+        return SPX_E_ARG;
+    }
+    hole_render_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A, Pn);
     SPX_LAUNCHED();
     return SPX_OK;
 }

@@ -106,8 +106,8 @@ int hole_step_launch(const float* image, int64_t H, int64_t W, int64_t pitch, co
 int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int N, int U, int Rr,
                        double rl2, const int32_t* idx, int n, const double* c_old, const double* c, const double* a,
                        const double* delta, hipStream_t stream);
-int select_kth_launch(const double* x, int64_t n, int64_t stride, const int64_t* ks, int nk, double* out,
-                      hipStream_t stream);
+int select_kth_launch(const double* x, int64_t n, int64_t stride, const int64_t* cols, const int64_t* ks, int nk,
+                      double* out, hipStream_t stream);
 int far_splat_launch(const double* c, const double* a, const double* delta, double cd0, int n, int mx, int my, int hp,
                      int wp, float* grid, hipStream_t stream);
 int disk_templates_launch(double radius, double sigma, int N, int U, double r0, double ramp, int B, float* tmpl,
@@ -471,13 +471,16 @@ int spx_hole_sweep(const float* image, int64_t H, int64_t W, int64_t pitch, doub
     return SPX_OK;
 }
 
-int spx_select_kth(const double* x, int64_t n, int64_t stride, const int64_t* ks, int32_t nk, double* out,
-                   spx_stream_t stream) {
+int spx_select_kth(const double* x, int64_t n, int64_t stride, const int64_t* cols, const int64_t* ks, int32_t nk,
+                   double* out, spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(ks != nullptr && nk >= 0 && nk <= 8, "spx_select_kth: 0..8 ranks");
     SPX_REQUIRE(n > 0 && stride >= 1 && x != nullptr && out != nullptr, "spx_select_kth: bad input");
-    for (int j = 0; j < nk; ++j) SPX_REQUIRE(ks[j] >= 0 && ks[j] < n, "spx_select_kth: rank out of range");
-    return spx::select_kth_launch(x, n, stride, ks, nk, out, spx::as_stream(stream));
+    for (int j = 0; j < nk; ++j) {
+        SPX_REQUIRE(ks[j] >= 0 && ks[j] < n, "spx_select_kth: rank out of range");
+        SPX_REQUIRE(cols == nullptr || cols[j] >= 0, "spx_select_kth: negative column offset");
+    }
+    return spx::select_kth_launch(x, n, stride, cols, ks, nk, out, spx::as_stream(stream));
 }
 
 int spx_far_splat(const double* c, const double* a, const double* delta, double cd0, int32_t n, int32_t mx, int32_t my,

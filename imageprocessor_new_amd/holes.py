@@ -215,10 +215,12 @@ class HolePage:
 
     # ------------------------------------------------------------------------------ hole pass
     def fit_holes(self, sweeps: int = 2, size: bool = False, max_move: float = 0.25, robust_c: float = 3.0,
-                  relax: float = 1.0, keep: dict | None = None, far: bool = True) -> list:
+                  relax: float = 1.0, keep: dict | None = None, far: bool = True,
+                  timing: dict | None = None) -> list:
         """Hole stage for a fixed number of sweeps (outline_onepass.py:828-887): centre, brightness
         and level of every hole (``size``: its edge offset too). Returns per-sweep statistics;
-        ``keep`` receives ``light`` (the page model the last sweep ended with, device (H, W))."""
+        ``keep`` receives ``light`` (the page model the last sweep ended with, device (H, W));
+        ``timing`` receives ``sweep_ms``, the device time of each sweep's step + render launches."""
         if self.tmpl is None:
             raise RuntimeError("set_template must run before fit_holes")
         if int(sweeps) != sweeps or sweeps < 1:
@@ -254,15 +256,24 @@ class HolePage:
             log = []
             for it in range(int(sweeps)):
                 opt.rscale = float(robust_c) * max(rms_typ, 1e-6) if robust_c else 0.0
+                if timing is not None:
+                    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                    ev[0].record()
                 _lib.call("spx_hole_sweep", _ptr(self.img), H, W, W, _ptr(page), _ptr(self.tmpl), N, U, R, Rr, rl2,
                           _ptr(order), start, ncol, _ptr(self.c), _ptr(self.a), _ptr(self.level), _ptr(self.delta),
                           _ptr(c_old), _ptr(cache), ctypes.byref(opt), _ptr(stat), sh())
+                if timing is not None:
+                    ev[1].record()
+                    timing.setdefault("events", []).append(ev)
                 if far and it + 1 < sweeps:
                     new_far = self.far_field().reshape(-1)
                     page += new_far - far_l
                     far_l = new_far
                 rms_typ, m99, d99 = self._sweep_stats(stat)
                 log.append(dict(sweep=it + 1, move99=m99, delta99=d99, rms=rms_typ))
+            if timing is not None:
+                torch.cuda.current_stream().synchronize()
+                timing["sweep_ms"] = [e0.elapsed_time(e1) for e0, e1 in timing.pop("events", [])]
             self.hole_rms = torch.empty(self.n, dtype=torch.float64, device=dev)
             self.hole_rms[order.long()] = stat[:, 2]
             self.stat = stat
@@ -279,12 +290,9 @@ class HolePage:
         lo, hi = int(math.floor(pos)), int(math.ceil(pos))
         t = pos - lo
         out = torch.empty(6, dtype=torch.float64, device=stat.device)
-        ks = (ctypes.c_int64 * 2)((n - 1) // 2, n // 2)
-        _lib.call("spx_select_kth", int(stat.data_ptr()) + 2 * 8, n, 4, ks, 2, _ptr(out), stream_handle())
-        ks = (ctypes.c_int64 * 2)(lo, hi)
-        for col in (0, 1):
-            _lib.call("spx_select_kth", int(stat.data_ptr()) + col * 8, n, 4, ks, 2, int(out.data_ptr()) + 8 * (2 + 2 * col),
-                      stream_handle())
+        cols = (ctypes.c_int64 * 6)(2, 2, 0, 0, 1, 1)
+        ks = (ctypes.c_int64 * 6)((n - 1) // 2, n // 2, lo, hi, lo, hi)
+        _lib.call("spx_select_kth", _ptr(stat), n, 4, cols, ks, 6, _ptr(out), stream_handle())
         v = out.cpu().tolist()
 
         def lerp(a, b):  # numpy's _lerp
@@ -313,3 +321,33 @@ def fit_holes(image, seeds, *, radius: float, sigma: float, noise: float, bright
     pg.set_template(radius, sigma)
     log = pg.fit_holes(sweeps=sweeps, size=size, max_move=max_move, robust_c=robust_c, relax=relax, far=far)
     return pg.result(log)
+
+
+def render_model_page(centers, brightness, *, radius: float, sigma: float, shape, background: float = 500.0,
+                      noise: float = 30.0, edge_offset=None, seed: int = 0, device: int = 0) -> torch.Tensor:
+    """A synthetic page of the hole-stage model on the device: every hole's near light
+    (spx_hole_render from an empty cache) and far light, plus a constant background and
+    N(0, noise^2) pixel noise. Returns a float32 (H, W) device tensor. Synthetic input for
+    tests and benchmarks."""
+    h, w = int(shape[0]), int(shape[1])
+    c = np.asarray(centers, dtype=np.float64).reshape(-1, 2)
+    pg = HolePage(torch.zeros((h, w), dtype=torch.float32, device=torch.device("cuda", int(device))), c,
+                  noise=max(float(noise), 1e-3), brightness=brightness, device=device)
+    pg.set_template(radius, sigma)
+    if edge_offset is not None:
+        pg.delta.copy_(torch.as_tensor(np.asarray(edge_offset, dtype=np.float64)))
+    H, W, N, U, Rr = pg.h, pg.w, pg.tN, pg.tU, pg.Rr
+    rl2 = float((pg.r_light + 0.75) ** 2)
+    with gpu_device(pg.device) as dev:
+        order = torch.arange(pg.n, dtype=torch.int32, device=dev)
+        page = torch.zeros(H * W, dtype=torch.float64, device=dev)
+        cache = torch.zeros(pg.n * (2 * Rr + 1) ** 2, dtype=torch.float32, device=dev)
+        _lib.call("spx_hole_render", _ptr(page), H, W, _ptr(cache), _ptr(pg.tmpl), N, U, Rr, rl2, _ptr(order), pg.n,
+                  _ptr(pg.c), _ptr(pg.c), _ptr(pg.a), _ptr(pg.delta), stream_handle())
+        page = page.reshape(H, W) + pg.far_field() + float(background)
+        if noise > 0:
+            g = torch.Generator(device=dev)
+            g.manual_seed(int(seed))
+            page = page + float(noise) * torch.randn((H, W), dtype=torch.float64, device=dev, generator=g)
+        return page.to(torch.float32)
+

@@ -366,12 +366,13 @@ int spx_hole_sweep(const float* image, int64_t H, int64_t W, int64_t pitch, doub
                    const int32_t* start, int32_t ncolour, double* c, double* a, double* level, double* delta,
                    double* c_old, float* cache, const SpxHoleStepOptions* opt, double* stat, spx_stream_t stream);
 
-/* Order statistics: out[j] = the ks[j]-th smallest (0-based) of x[i * stride], i < n, for
- * j < nk <= 8 (ks: host array; x, out: device; no NaNs in x). Exact selection by an 8-round
- * radix select over the doubles' order-preserving 64-bit keys. Serves the hole pass's median
- * leftover and 99th-percentile moves (cp.median / cp.percentile in outline_onepass.py:871-873). */
-int spx_select_kth(const double* x, int64_t n, int64_t stride, const int64_t* ks, int32_t nk, double* out,
-                   spx_stream_t stream);
+/* Order statistics: out[j] = the ks[j]-th smallest (0-based) of x[cols[j] + i * stride], i < n,
+ * for j < nk <= 8 (cols, ks: host arrays, cols may be NULL for 0; x, out: device; no NaNs).
+ * Exact selection by an 8-round radix select over the doubles' order-preserving 64-bit keys,
+ * one block per rank. Serves the hole pass's median leftover and 99th-percentile moves
+ * (cp.median / cp.percentile in outline_onepass.py:871-873). */
+int spx_select_kth(const double* x, int64_t n, int64_t stride, const int64_t* cols, const int64_t* ks, int32_t nk,
+                   double* out, spx_stream_t stream);
 
 /* Keys splat of a_i (1 + 4 delta_i / cd0) at (c_i + (mx, my)) on the padded FP32 grid
  * (hp, wp) (outline_onepass.py:399-427); the far light is this grid convolved with the far

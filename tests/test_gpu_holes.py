@@ -169,8 +169,8 @@ def test_select_kth_matches_sorting():
         out = torch.empty(8, dtype=torch.float64, device=dev)
         for col in range(4):
             ks = [0, 2500, 5000, 1234, 4950, 17, 2499, 3]
-            _lib.call("spx_select_kth", int(xt.data_ptr()) + 8 * col, 5001, 4, (ctypes.c_int64 * 8)(*ks), 8,
-                      out.data_ptr(), stream_handle())
+            _lib.call("spx_select_kth", int(xt.data_ptr()), 5001, 4, (ctypes.c_int64 * 8)(*([col] * 8)),
+                      (ctypes.c_int64 * 8)(*ks), 8, out.data_ptr(), stream_handle())
             got = out.cpu().numpy()
             want = np.sort(x[:, col])[ks]
             assert np.array_equal(got, want), (col, got, want)
