@@ -288,6 +288,17 @@ __global__ __launch_bounds__(256) void prepare_kernel(const uint16_t* img, int64
 // Network outputs t -> fit parameters [x0, y0, r, s, A, B, q] and back:
 //   x0 = cx + side t0 / 8, y0 = cy + side t1 / 8, r = side t2, s = side exp(t3) / 16,
 //   A = A0 t4, B = B0 + A0 t5, q = 1 + t6 / 8.
+__device__ __forceinline__ void decode_one(const float* o, const float* n, float* p) {
+    const float side = n[2];
+    p[0] = n[0] + side * o[0] * 0.125f;
+    p[1] = n[1] + side * o[1] * 0.125f;
+    p[2] = side * fmaxf(o[2], 0.01f);
+    p[3] = side * expf(fminf(fmaxf(o[3], -6.0f), 3.0f)) * 0.0625f;
+    p[4] = n[4] * o[4];
+    p[5] = n[3] + n[4] * o[5];
+    p[6] = 1.0f + o[6] * 0.125f;
+}
+
 __global__ void decode_kernel(const float* t, const float* norm, int K, float* params) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
@@ -334,17 +345,18 @@ struct Layer {
     bool relu;
 };
 constexpr int kLayers = 6;
-const Layer kNet[kLayers] = {
+constexpr Layer kNet[kLayers] = {
     {9 * 1, 16, 32, 32, 1, true, true},   {9 * 16, 32, 16, 16, 16, true, true},
     {9 * 32, 64, 8, 8, 32, true, true},   {9 * 64, 64, 4, 4, 64, true, true},
     {256, 128, 0, 0, 0, false, true},     {128, 7, 0, 0, 0, false, false},
 };
 
-int64_t param_offset(int l) {  // start of layer l's weights (K x N), bias follows
+__host__ __device__ constexpr int64_t param_offset_dev(int l) {  // start of layer l's weights (K x N)
     int64_t o = 0;
     for (int i = 0; i < l; ++i) o += (int64_t)kNet[i].kd * kNet[i].n + kNet[i].n;
     return o;
 }
+int64_t param_offset(int l) { return param_offset_dev(l); }  // bias follows the K x N weights
 int64_t param_total() { return param_offset(kLayers); }
 
 int64_t rows_of(int l, int Bn) {  // GEMM M of layer l
@@ -476,6 +488,164 @@ int forward(const uint16_t* x, int Bn, const uint16_t* wbf, const float* theta, 
     return SPX_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// Fused inference: one workgroup (4 waves) takes kIP patches through all six layers, the
+// activations staying in LDS (NHWC, BF16), the weights read as MFMA B fragments from a
+// transposed, K-padded BF16 copy (Wt[n][k], k padded to a multiple of 16, zeros beyond K).
+// Each layer is an implicit GEMM on v_mfma_f32_16x16x16bf16_1k tiles: rows = (patch, output
+// pixel), columns = output channels, k = (ky, kx, c). A wave owns one column block and a group
+// of up to kGM row blocks, so one B fragment feeds kGM MFMAs. Bias + ReLU + BF16 rounding in
+// the epilogue, as the GEMM path. The 128 -> 7 head runs on the VALU in FP32.
+typedef short bf16x4s __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kIP = 4;   // patches per workgroup
+constexpr int kGM = 8;   // row blocks per wave item (B fragment reuse)
+
+struct InferWeights {
+    const uint16_t* wt;      // transposed, K-padded BF16 weights, layers 0..4
+    int64_t off[5];          // element offset of layer l in wt
+    int kp[5];               // padded K of layer l
+    const float* theta;      // FP32 parameters (biases; K x N layout)
+    const uint16_t* wbf;     // BF16 K x N weights (head)
+};
+
+__global__ void transpose_weights_kernel(const uint16_t* wbf, InferWeights iw, uint16_t* wt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t base = 0;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+        const int K = kNet[l].kd, N = kNet[l].n, Kp = iw.kp[l];
+        const int64_t cnt = (int64_t)N * Kp;
+        if (t >= base && t < base + cnt) {
+            const int64_t q = t - base;
+            const int n = (int)(q / Kp), k = (int)(q - (int64_t)n * Kp);
+            wt[iw.off[l] + q] = k < K ? wbf[param_offset_dev(l) + (int64_t)k * N + n] : (uint16_t)0;
+        }
+        base += cnt;
+    }
+}
+
+// One conv layer (3x3, stride 2, pad 1) or the first fc layer (as a 1x1 "conv" on the NHWC
+// flatten: Hout = 1, Cin = 256) from LDS `in` to LDS `out`.
+template <int CIN, int COUT, int HIN, bool FC>
+__device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                            const uint16_t* __restrict__ wt, int kp, const float* __restrict__ bias,
+                                            int wave, int lane) {
+    constexpr int HOUT = FC ? 1 : HIN / 2;
+    constexpr int PIX = HOUT * HOUT;
+    constexpr int M = kIP * PIX;
+    constexpr int MB = (M + 15) / 16;
+    constexpr int NB = COUT / 16;
+    constexpr int MG = (MB + kGM - 1) / kGM;
+    constexpr int KIN = FC ? CIN : 9 * CIN;
+    const int kc_n = kp / 16;
+    const int li = lane & 15, lq = lane >> 4;
+    for (int item = wave; item < NB * MG; item += 4) {
+        const int nb = item % NB, mg = item / NB;
+        const int mb0 = mg * kGM;
+        const int nmb = MB - mb0 < kGM ? MB - mb0 : kGM;
+        f32x4 acc[kGM];
+#pragma unroll
+        for (int g = 0; g < kGM; ++g) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const uint16_t* wcol = wt + (int64_t)(nb * 16 + li) * kp;
+        for (int kc = 0; kc < kc_n; ++kc) {
+            const int k0 = kc * 16 + 4 * lq;
+            const bf16x4s b = *reinterpret_cast<const bf16x4s*>(wcol + k0);
+#pragma unroll
+            for (int g = 0; g < kGM; ++g) {
+                if (g >= nmb) break;
+                const int row = (mb0 + g) * 16 + li;
+                bf16x4s a = bf16x4s{0, 0, 0, 0};
+                if (row < M) {
+                    const int pt = row / PIX, pix = row - pt * PIX;
+                    if constexpr (FC) {
+                        a = *reinterpret_cast<const bf16x4s*>(in + pt * CIN + k0);
+                    } else if constexpr (CIN >= 4) {
+                        const int oy = pix / HOUT, ox = pix - oy * HOUT;
+                        const int tap = k0 / CIN, c0 = k0 - tap * CIN;
+                        const int iy = 2 * oy + tap / 3 - 1, ix = 2 * ox + tap % 3 - 1;
+                        if (k0 < KIN && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN)
+                            a = *reinterpret_cast<const bf16x4s*>(in + ((pt * HIN + iy) * HIN + ix) * CIN + c0);
+                    } else {
+                        const int oy = pix / HOUT, ox = pix - oy * HOUT;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int k = k0 + t;  // CIN == 1: k = tap
+                            const int iy = 2 * oy + k / 3 - 1, ix = 2 * ox + k % 3 - 1;
+                            if (k < KIN && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN)
+                                a[t] = (short)in[(pt * HIN + iy) * HIN + ix];
+                        }
+                    }
+                }
+                acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[g], 0, 0, 0);
+            }
+        }
+        // D(i, j): i = 4 (lane / 16) + r, j = lane % 16
+        const int n = nb * 16 + li;
+        const float bn = bias[n];
+#pragma unroll
+        for (int g = 0; g < kGM; ++g) {
+            if (g >= nmb) break;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = (mb0 + g) * 16 + 4 * lq + r;
+                if (row < M) out[row * COUT + n] = f2bf(fmaxf(acc[g][r] + bn, 0.0f));
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void cnn_infer_kernel(const uint16_t* __restrict__ x, int K, InferWeights iw,
+                                                        const float* __restrict__ norm, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint16_t bufA[kIP * 16 * 16 * 16];  // L1 out, L3 out, fc1 out
+    __shared__ __attribute__((aligned(16))) uint16_t bufB[kIP * 8 * 8 * 32];    // input, L2 out, L4 out
+    __shared__ float head[kIP * 8];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int p0 = blockIdx.x * kIP;
+    // input patches (2 KB each) -> bufB, 16 bytes per thread per step; missing patches read 0
+    for (int q = tid; q < kIP * 128; q += 256) {
+        const int pt = q >> 7, e = (q & 127) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (p0 + pt < K) v = *reinterpret_cast<const uint4*>(x + (int64_t)(p0 + pt) * 1024 + e);
+        *reinterpret_cast<uint4*>(bufB + pt * 1024 + e) = v;
+    }
+    __syncthreads();
+    const float* th = iw.theta;
+    infer_layer<1, 16, 32, false>(bufB, bufA, iw.wt + iw.off[0], iw.kp[0], th + param_offset_dev(0) + 9 * 16, wave,
+                                  lane);
+    __syncthreads();
+    infer_layer<16, 32, 16, false>(bufA, bufB, iw.wt + iw.off[1], iw.kp[1], th + param_offset_dev(1) + 144 * 32,
+                                   wave, lane);
+    __syncthreads();
+    infer_layer<32, 64, 8, false>(bufB, bufA, iw.wt + iw.off[2], iw.kp[2], th + param_offset_dev(2) + 288 * 64,
+                                  wave, lane);
+    __syncthreads();
+    infer_layer<64, 64, 4, false>(bufA, bufB, iw.wt + iw.off[3], iw.kp[3], th + param_offset_dev(3) + 576 * 64,
+                                  wave, lane);
+    __syncthreads();
+    infer_layer<256, 128, 1, true>(bufB, bufA, iw.wt + iw.off[4], iw.kp[4], th + param_offset_dev(4) + 256 * 128,
+                                   wave, lane);
+    __syncthreads();
+    // head 128 -> 7 (FP32 on the VALU): thread (patch, output)
+    if (tid < kIP * 7) {
+        const int pt = tid / 7, j = tid - pt * 7;
+        const int64_t wo = param_offset_dev(5);
+        float acc = 0.0f;
+        for (int k = 0; k < 128; ++k) acc = fmaf(bf2f(bufA[pt * 128 + k]), bf2f(iw.wbf[wo + k * 7 + j]), acc);
+        head[pt * 8 + j] = acc + th[wo + 128 * 7 + j];
+    }
+    __syncthreads();
+    if (tid < kIP && p0 + tid < K) {
+        const int64_t k = p0 + tid;
+        if (norm) {
+            decode_one(head + tid * 8, norm + 5 * k, out + 7 * k);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 7; ++j) out[7 * k + j] = head[tid * 8 + j];
+        }
+    }
+}
+
 }  // namespace
 
 int64_t cnn_param_count() { return param_total(); }
@@ -501,26 +671,30 @@ int cnn_encode_launch(const float* params, const float* norm, int K, float* t, h
     return SPX_OK;
 }
 
-// Forward in chunks of at most `chunk` patches; out = decoded parameters (K x 7) when
-// norm != NULL, raw network outputs otherwise.
+// Inference: out = decoded parameters (K x 7) when norm != NULL, raw network outputs
+// otherwise. The fused kernel reads a transposed, K-padded copy of the BF16 weights made in
+// the library workspace first.
 int cnn_forward_launch(const uint16_t* x, int K, const float* theta, const uint16_t* theta_bf, const float* norm,
                        float* out, hipStream_t stream) {
-    const int chunk = 16384;
-    for (int k0 = 0; k0 < K; k0 += chunk) {
-        const int Bn = std::min(chunk, K - k0);
-        CnnBuffers b;
-        int rc = cnn_buffers(Bn, false, b);
-        if (rc != SPX_OK) return rc;
-        rc = forward(x + (int64_t)k0 * 1024, Bn, theta_bf, theta, b, stream);
-        if (rc != SPX_OK) return rc;
-        if (norm) {
-            decode_kernel<<<grid1(Bn), 256, 0, stream>>>(b.out, norm + 5 * (int64_t)k0, Bn, out + 7 * (int64_t)k0);
-        } else {
-            SPX_HIP(hipMemcpyAsync(out + 7 * (int64_t)k0, b.out, sizeof(float) * 7 * Bn, hipMemcpyDeviceToDevice,
-                                   stream));
-        }
-        SPX_LAUNCHED();
+    if (K == 0) return SPX_OK;
+    InferWeights iw{};
+    int64_t total = 0;
+    for (int l = 0; l < 5; ++l) {
+        iw.kp[l] = (kNet[l].kd + 15) / 16 * 16;
+        iw.off[l] = total;
+        total += (int64_t)kNet[l].n * iw.kp[l];
     }
+    void* ws = nullptr;
+    int rc = workspace(sizeof(uint16_t) * (size_t)total, &ws);
+    if (rc != SPX_OK) return rc;
+    uint16_t* wt = static_cast<uint16_t*>(ws);
+    iw.wt = wt;
+    iw.theta = theta;
+    iw.wbf = theta_bf;
+    transpose_weights_kernel<<<grid1(total), 256, 0, stream>>>(theta_bf, iw, wt);
+    SPX_LAUNCHED();
+    cnn_infer_kernel<<<(K + kIP - 1) / kIP, 256, 0, stream>>>(x, K, iw, norm, out);
+    SPX_LAUNCHED();
     return SPX_OK;
 }
 
