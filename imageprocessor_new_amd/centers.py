@@ -172,6 +172,63 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
     return centers, info
 
 
+def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity, fit_half, fit_options,
+                      model_code, fit_cells, morph_open=0, morph_close=0, events=None):
+    """Segmentation -> CC -> area filter -> erf fit of one strip, kept on the device: the rows of
+    _detect_single_tile's erf group (label order, same seeds, half sides, neighbour lists and
+    fit), with every component fitted in one launch and the filtered-out rows (background,
+    area outside the limits) given half side 0, which the fit skips. Returns (centres (num, 2)
+    float64, ok (num,) bool) device tensors; `events` collects (stage, start, end) CUDA events.
+    The only host synchronisation is the component count."""
+    from .fit import cell_radius, fit_erf_disks_device, seed_neighbours_device
+
+    def mark(name, e0):
+        if events is not None:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record()
+            events.append((name, e0, e1))
+            return e1
+        return None
+
+    e0 = None
+    if events is not None:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+    cap = int(min(max(65536, img_t.shape[0] * img_t.shape[1] // 256), 1 << 24))
+    if morph_open > 0 or morph_close > 0:
+        mask = binary_mask_device(img_t, thresh, invert=invert, morph_open=morph_open, morph_close=morph_close)
+        num, labels, stats, cents = threshold_components_device(mask, 0, connectivity=connectivity, capacity=cap)
+    else:
+        num, labels, stats, cents = threshold_components_device(img_t, thresh, invert=invert,
+                                                                connectivity=connectivity, capacity=cap)
+    e0 = mark("components", e0)
+    dev = img_t.device
+    area = stats[:, 4]
+    valid = (area >= int(area_min)) & (area <= int(area_max))
+    valid[0] = False
+    # seeds of filtered-out rows are NaN: the neighbour lists skip them
+    seeds = torch.where(valid[:, None], cents.to(torch.float32),
+                        torch.full_like(cents, float("nan"), dtype=torch.float32))
+    seeds = seeds.contiguous()
+    if fit_half is None:
+        size = torch.maximum(stats[:, 2], stats[:, 3])
+        half = torch.clamp((size + 1) // 2 + 4, 7, 31)
+    else:
+        half = torch.full((num,), int(fit_half), dtype=torch.int32, device=dev)
+    half = torch.where(valid, half, torch.zeros_like(half)).to(torch.int16).contiguous()
+    model = torch.full((num,), int(model_code), dtype=torch.uint8, device=dev)
+    e0 = mark("rows", e0)
+    nbr = None
+    if fit_cells:
+        hmax = int(half.max().item()) if num else 0
+        nbr = seed_neighbours_device(seeds, img_t.shape, cell_radius(max(hmax, 7)))
+    res = fit_erf_disks_device(img_t, seeds, half, model, options=fit_options, neighbours=nbr)
+    params = res["params"]
+    ok = valid & (res["status"] <= 1) & torch.isfinite(params[:, :2]).all(dim=1)
+    mark("refine", e0)
+    return params[:, :2].to(torch.float64), ok
+
+
 def detect_centers(
     image,
     *,
@@ -270,7 +327,38 @@ def detect_centers(
         all_centers = []
         n_tiles = 0
         y0 = 0
-        while y0 < H:
+        device_path = refine in _ERF_METHODS and pre_threshold is not None
+        pts_dev = None
+        if device_path:
+            # erf fits keep every strip on the device; one host copy of the centres at the end
+            events = []
+            dev_centers = []
+            tb = time.perf_counter()
+            while y0 < H:
+                y1 = min(H, y0 + tile_h)
+                c_t, ok_t = _erf_strip_device(
+                    img_t[y0:y1], int(np.floor(pre_threshold)), invert=invert, area_min=area_min,
+                    area_max=area_max, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options,
+                    model_code=_ERF_METHODS[refine], fit_cells=bool(fit_cells), morph_open=int(morph_open),
+                    morph_close=int(morph_close), events=events)
+                n_tiles += 1
+                half = int(overlap) // 2
+                keep_lo = y0 if y0 == 0 else (y0 + half)
+                keep_hi = y1 if y1 == H else (y1 - half)
+                yy = c_t[:, 1] + y0
+                m = ok_t & (yy >= keep_lo) & (yy < keep_hi)
+                dev_centers.append(torch.stack([c_t[:, 0], yy], dim=1)[m])
+                if y1 == H:
+                    break
+                y0 += step
+            pts_t = torch.cat(dev_centers) if dev_centers else torch.zeros((0, 2), dtype=torch.float64)
+            pts = pts_t.cpu().numpy()
+            pts_dev = pts_t if pts_t.is_cuda else None
+            stage_ms["strips_wall"] = 1e3 * (time.perf_counter() - tb)
+            for name, e0, e1 in events:
+                stage_ms[name] = stage_ms.get(name, 0.0) + e0.elapsed_time(e1)
+            all_centers = [pts] if len(pts) else []
+        while not device_path and y0 < H:
             y1 = min(H, y0 + tile_h)
             tile = img_t[y0:y1]
             tile_thr = None if threshold_mode == "per_tile" else pre_threshold
@@ -295,7 +383,7 @@ def detect_centers(
         t0 = time.perf_counter()
         if all_centers:
             pts = np.vstack(all_centers)
-            pairs = _candidate_pairs_device(pts, float(dedupe_eps), (H, W))
+            pairs = _candidate_pairs_device(pts if pts_dev is None else pts_dev, float(dedupe_eps), (H, W))
             if pairs is None:
                 pairs = _candidate_pairs_kdtree(pts, float(dedupe_eps))
             centers = _merge_pairs(pts, pairs, float(dedupe_eps))
@@ -350,22 +438,27 @@ def _candidate_pairs_kdtree(pts: np.ndarray, eps: float) -> np.ndarray:
     return cKDTree(pts).query_pairs(r=float(eps) * (1.0 + 1e-9) + 1e-12, output_type="ndarray")
 
 
-def _candidate_pairs_device(pts: np.ndarray, eps: float, shape) -> np.ndarray | None:
+def _candidate_pairs_device(pts, eps: float, shape) -> np.ndarray | None:
     """Candidate pairs (i < j) from the device neighbour grid (spx_seed_neighbours) on float32
     coordinates, with the radius padded past their rounding (< 0.01 px below 2^16 px); None
-    when a neighbour list is full (it may have dropped a candidate)."""
+    when a neighbour list is full (it may have dropped a candidate). ``pts`` is a NumPy array
+    or a device tensor; only the pairs come back to the host."""
     from .fit import seed_neighbours_device
 
     nb = 8
     dev = torch.device("cuda", torch.cuda.current_device())
-    s = torch.from_numpy(np.ascontiguousarray(pts, dtype=np.float32)).to(dev)
-    nbr = seed_neighbours_device(s, shape, float(eps) + 0.02, nb).cpu().numpy()
-    if (nbr[:, nb - 1] >= 0).any():
+    if isinstance(pts, torch.Tensor):
+        s = pts.to(device=dev, dtype=torch.float32).contiguous()
+    else:
+        s = torch.from_numpy(np.ascontiguousarray(pts, dtype=np.float32)).to(dev)
+    nbr = seed_neighbours_device(s, shape, float(eps) + 0.02, nb)
+    if bool((nbr[:, nb - 1] >= 0).any()):
         return None
-    i, k = np.nonzero(nbr >= 0)
-    j = nbr[i, k].astype(np.int64)
-    keep = j > i
-    return np.stack([i[keep].astype(np.int64), j[keep]], axis=1).reshape(-1, 2)
+    ids = torch.arange(nbr.shape[0], device=dev, dtype=torch.int32)[:, None]
+    i, k = torch.nonzero(nbr > ids, as_tuple=True)
+    if i.numel() == 0:
+        return np.zeros((0, 2), dtype=np.int64)
+    return torch.stack([i.to(torch.int64), nbr[i, k].to(torch.int64)], dim=1).cpu().numpy()
 
 
 def _merge_pairs(pts: np.ndarray, pairs: np.ndarray, eps: float) -> np.ndarray:
@@ -401,7 +494,9 @@ def _merge_pairs(pts: np.ndarray, pairs: np.ndarray, eps: float) -> np.ndarray:
         else:
             par[rb] = ra
             rk[ra] += 1
-    roots = np.array([find(i) for i in range(n)], dtype=np.int64)
+    roots = np.arange(n, dtype=np.int64)  # holes in no pair are their own roots
+    for i in np.unique(pairs).tolist():
+        roots[i] = find(i)
     unique_roots, inv = np.unique(roots, return_inverse=True)
     K = unique_roots.size
     sums = np.zeros((K, 2), dtype=np.float64)

@@ -31,14 +31,19 @@ __global__ void zero_cells(int32_t* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
+// Seeds with a non-finite coordinate take no part: not binned, no neighbours of their own.
+__device__ __forceinline__ bool seed_ok(const float* seeds, int i) {
+    return isfinite(seeds[2 * i]) && isfinite(seeds[2 * i + 1]);
+}
+
 __global__ void count_cells(const float* seeds, int K, Grid g, int32_t* cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < K) atomicAdd(&cnt[cell_of(g, seeds[2 * i], seeds[2 * i + 1])], 1);
+    if (i < K && seed_ok(seeds, i)) atomicAdd(&cnt[cell_of(g, seeds[2 * i], seeds[2 * i + 1])], 1);
 }
 
 __global__ void scatter_cells(const float* seeds, int K, Grid g, int32_t* cursor, int32_t* ids) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < K) ids[atomicAdd(&cursor[cell_of(g, seeds[2 * i], seeds[2 * i + 1])], 1)] = i;
+    if (i < K && seed_ok(seeds, i)) ids[atomicAdd(&cursor[cell_of(g, seeds[2 * i], seeds[2 * i + 1])], 1)] = i;
 }
 
 __global__ void query_cells(const float* seeds, int K, Grid g, const int32_t* start, const int32_t* cnt,
@@ -46,6 +51,10 @@ __global__ void query_cells(const float* seeds, int K, Grid g, const int32_t* st
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K) return;
+    if (!seed_ok(seeds, i)) {
+        for (int q = 0; q < nb; ++q) out[(int64_t)i * nb + q] = -1;
+        return;
+    }
     const double xi = (double)seeds[2 * i], yi = (double)seeds[2 * i + 1];
     const int c = cell_of(g, seeds[2 * i], seeds[2 * i + 1]);
     const int cx = c % g.gw, cy = c / g.gw;

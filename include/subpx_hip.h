@@ -244,7 +244,8 @@ int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch
 
 /* The nb nearest other seeds of every seed within `radius` px (ordered by distance, ties by
  * index; -1 where fewer exist), by a uniform-grid hash on the device. seeds (K, 2) f32 in an
- * H x W image, out (K, nb) int32: device. nb in [1, 8]. */
+ * H x W image, out (K, nb) int32: device. nb in [1, 8]. A seed with a non-finite coordinate
+ * takes no part (it is nobody's neighbour and its own row is all -1). */
 int spx_seed_neighbours(const float* seeds, int32_t K, int64_t H, int64_t W, float radius, int32_t nb,
                         int32_t* out, spx_stream_t stream);
 

@@ -112,10 +112,13 @@ def test_seed_neighbours_match_oracle():
     rng = np.random.default_rng(11)
     seeds = rng.uniform(-3, 2003, (4000, 2)).astype(np.float32)
     seeds[:40] = seeds[40:80] + np.float32(0.5)  # close pairs
+    seeds[100:130] = np.nan  # seeds that take no part
+    seeds[130:140, 1] = np.inf
     for radius, nb in [(60.0, 8), (25.0, 3), (200.0, 1)]:
         with gpu_device(0) as dev:
             got = seed_neighbours_device(torch.from_numpy(seeds).to(dev), (2000, 2000), radius, nb).cpu().numpy()
         np.testing.assert_array_equal(got, erfdisk.seed_neighbours(seeds, radius, nb))
+        assert np.all(got[100:140] == -1) and not np.isin(np.arange(100, 140), got).any()
 
 
 @pytest.mark.parametrize("seed", [5, 6])

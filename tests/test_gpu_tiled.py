@@ -57,3 +57,30 @@ def test_device_pairs_give_the_kdtree_dedupe():
     b = _merge_pairs(pts, _candidate_pairs_kdtree(pts, 1.5), 1.5)
     np.testing.assert_array_equal(a, b)
     assert a.shape[0] < pts.shape[0]
+
+
+def test_device_strip_path_matches_the_row_path():
+    """The device-resident erf strip (one strip, global threshold) gives the centres of the
+    row-by-row path (_detect_single_tile's erf group) on the same threshold."""
+    import torch
+
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.centers import _detect_single_tile, _erf_strip_device
+    from imageprocessor_new_amd import _lib
+    from imageprocessor_new_amd.components import otsu_threshold_device
+
+    H = W = 1024
+    holes, _, _ = synthetic.mixed_page_holes(seed=4, H=H, W=W, region=512)
+    img_t = synthetic.render_page(holes, H, W, seed=4)
+    opts = _lib.fit_options()
+    with gpu_device(0):
+        thr = int(otsu_threshold_device(img_t))
+        c_dev, ok = _erf_strip_device(img_t, thr, invert=False, area_min=20, area_max=5000, connectivity=8,
+                                      fit_half=None, fit_options=opts, model_code=_lib.MODEL_ERF_DISK6,
+                                      fit_cells=True)
+        got = c_dev[ok].cpu().numpy()
+        ref, _ = _detect_single_tile(img_t, threshold="otsu", invert=False, area_min=20, area_max=5000, pad=3,
+                                     refine="erf_disk", connectivity=8, pre_threshold=float(thr), fit_half=None,
+                                     fit_options=opts, fit_cells=True)
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got, ref)
