@@ -189,17 +189,6 @@ __device__ __forceinline__ uint32_t node_bits(const uint16_t* img, const CCGeom&
            (is_fg(img, g, y + 1, x + 1, thresh, invert) ? 8u : 0u);
 }
 
-template <int BS>
-__global__ void cc_init_kernel(const uint16_t* img, CCGeom g, int32_t thresh, int invert,
-                               int32_t* parent, uint8_t* bits) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.NH * g.NW) return;
-    const int64_t ny = n / g.NW, nx = n - (n / g.NW) * g.NW;
-    const uint32_t b = node_bits<BS>(img, g, ny, nx, thresh, invert);
-    bits[n] = (uint8_t)b;
-    parent[n] = b ? (int32_t)n : -1;
-}
-
 __device__ __forceinline__ int32_t uf_find(const int32_t* parent, int32_t a) {
     int32_t p = parent[a];
     while (p != a) {
@@ -227,32 +216,116 @@ __device__ __forceinline__ void uf_union(int32_t* parent, int32_t a, int32_t b) 
     }
 }
 
+// Union-find inside one 16 x 16-node tile in LDS (same lock-free rule as the global one).
+constexpr int kUfTile = 16;
+
+__device__ __forceinline__ int uf_find_lds(const int32_t* lp, int a) {
+    int p = lp[a];
+    while (p != a) {
+        a = p;
+        p = lp[a];
+    }
+    return a;
+}
+__device__ __forceinline__ void uf_union_lds(int32_t* lp, int a, int b) {
+    for (;;) {
+        a = uf_find_lds(lp, a);
+        b = uf_find_lds(lp, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicMin(&lp[b], a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+// Which of a node's four backward neighbours (left, up, up-left, up-right) are adjacent to it:
+// bit 0 left, bit 1 up, bit 2 up-left, bit 3 up-right, from the nodes' foreground bits.
+template <int BS>
+__device__ __forceinline__ uint32_t cc_links(uint32_t b, uint32_t l, uint32_t u, uint32_t ul, uint32_t ur) {
+    if (BS == 1) return (l ? 1u : 0u) | (u ? 2u : 0u);
+    uint32_t m = 0;
+    // left block: its right column (bits 1, 3) touches our left column (bits 0, 2)
+    if ((b & 5u) && (l & 10u)) m |= 1u;
+    // upper block: its bottom row (bits 4, 8) touches our top row (bits 1, 2)
+    if ((b & 3u) && (u & 12u)) m |= 2u;
+    // upper-left: its bottom-right pixel (bit 8) touches our top-left (bit 1)
+    if ((b & 1u) && (ul & 8u)) m |= 4u;
+    // upper-right: its bottom-left pixel (bit 4) touches our top-right (bit 2)
+    if ((b & 2u) && (ur & 4u)) m |= 8u;
+    return m;
+}
+
+// Foreground bits of every node, then the union-find of the edges inside each 16 x 16-node
+// tile in LDS: parent[n] = the node index of n's tile-local root (the smallest node index of
+// its component within the tile), -1 for background. Only edges between tiles are left for
+// cc_merge_kernel, so the global atomics run on tile borders only.
+template <int BS>
+__global__ __launch_bounds__(256) void cc_local_kernel(const uint16_t* img, CCGeom g, int32_t thresh, int invert,
+                                                       int32_t* parent, uint8_t* bits, int64_t tiles_x) {
+    __shared__ int32_t lp[kUfTile * kUfTile];
+    __shared__ uint8_t lb[kUfTile * kUfTile];
+    const int64_t ty = blockIdx.x / tiles_x, tx = blockIdx.x - (blockIdx.x / tiles_x) * tiles_x;
+    const int i = threadIdx.x, ly = i / kUfTile, lx = i % kUfTile;
+    const int64_t ny = ty * kUfTile + ly, nx = tx * kUfTile + lx;
+    const bool in = ny < g.NH && nx < g.NW;
+    const int64_t n = ny * g.NW + nx;
+    const uint32_t b = in ? node_bits<BS>(img, g, ny, nx, thresh, invert) : 0u;
+    if (in) bits[n] = (uint8_t)b;
+    lb[i] = (uint8_t)b;
+    lp[i] = b ? i : -1;
+    __syncthreads();
+    if (b) {
+        const uint32_t l = lx > 0 ? lb[i - 1] : 0u;
+        const uint32_t u = ly > 0 ? lb[i - kUfTile] : 0u;
+        const uint32_t ul = (lx > 0 && ly > 0) ? lb[i - kUfTile - 1] : 0u;
+        const uint32_t ur = (lx + 1 < kUfTile && ly > 0) ? lb[i - kUfTile + 1] : 0u;
+        const uint32_t m = cc_links<BS>(b, l, u, ul, ur);
+        if (m & 1u) uf_union_lds(lp, i, i - 1);
+        if (m & 2u) uf_union_lds(lp, i, i - kUfTile);
+        if (m & 4u) uf_union_lds(lp, i, i - kUfTile - 1);
+        if (m & 8u) uf_union_lds(lp, i, i - kUfTile + 1);
+    }
+    __syncthreads();
+    if (in) {
+        if (b) {
+            const int r = uf_find_lds(lp, i);
+            parent[n] = (int32_t)((ty * kUfTile + r / kUfTile) * g.NW + tx * kUfTile + r % kUfTile);
+        } else {
+            parent[n] = -1;
+        }
+    }
+}
+
+// The edges between tiles, by the global lock-free union-find.
 template <int BS>
 __global__ void cc_merge_kernel(CCGeom g, int32_t* parent, const uint8_t* bits) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.NH * g.NW) return;
+    const int64_t ny = n / g.NW, nx = n - (n / g.NW) * g.NW;
+    const int lx = (int)(nx % kUfTile), ly = (int)(ny % kUfTile);
+    if (lx != 0 && ly != 0 && lx != kUfTile - 1) return;  // every neighbour in the same tile
     const uint32_t b = bits[n];
     if (!b) return;
-    const int64_t ny = n / g.NW, nx = n - (n / g.NW) * g.NW;
-    if (BS == 1) {
-        if (nx > 0 && bits[n - 1]) uf_union(parent, (int32_t)n, (int32_t)(n - 1));
-        if (ny > 0 && bits[n - g.NW]) uf_union(parent, (int32_t)n, (int32_t)(n - g.NW));
-        return;
-    }
-    // left block: its right column (bits 1, 3) touches our left column (bits 0, 2)
-    if (nx > 0 && (b & 5u)) {
-        const uint32_t l = bits[n - 1];
-        if (l & 10u) uf_union(parent, (int32_t)n, (int32_t)(n - 1));
-    }
-    if (ny > 0) {
-        const int64_t u = n - g.NW;
-        // upper block: its bottom row (bits 4, 8) touches our top row (bits 1, 2)
-        if ((b & 3u) && (bits[u] & 12u)) uf_union(parent, (int32_t)n, (int32_t)u);
-        // upper-left: its bottom-right pixel (bit 8) touches our top-left (bit 1)
-        if (nx > 0 && (b & 1u) && (bits[u - 1] & 8u)) uf_union(parent, (int32_t)n, (int32_t)(u - 1));
-        // upper-right: its bottom-left pixel (bit 4) touches our top-right (bit 2)
-        if (nx + 1 < g.NW && (b & 2u) && (bits[u + 1] & 4u)) uf_union(parent, (int32_t)n, (int32_t)(u + 1));
-    }
+    const int64_t u = n - g.NW;
+    const uint32_t l = nx > 0 ? bits[n - 1] : 0u;
+    const uint32_t up = ny > 0 ? bits[u] : 0u;
+    const uint32_t ul = (nx > 0 && ny > 0) ? bits[u - 1] : 0u;
+    const uint32_t ur = (nx + 1 < g.NW && ny > 0) ? bits[u + 1] : 0u;
+    uint32_t m = cc_links<BS>(b, l, up, ul, ur);
+    // keep the edges that leave the tile
+    if (lx != 0) m &= ~1u;
+    if (ly != 0) m &= ~2u;
+    if (lx != 0 && ly != 0) m &= ~4u;
+    if (lx != kUfTile - 1 && ly != 0) m &= ~8u;
+    if (m & 1u) uf_union(parent, (int32_t)n, (int32_t)(n - 1));
+    if (m & 2u) uf_union(parent, (int32_t)n, (int32_t)u);
+    if (m & 4u) uf_union(parent, (int32_t)n, (int32_t)(u - 1));
+    if (m & 8u) uf_union(parent, (int32_t)n, (int32_t)(u + 1));
 }
 
 __global__ void cc_compress_kernel(int64_t nn, int32_t* parent, int32_t* root_flag) {
@@ -506,7 +579,10 @@ int threshold_cc_impl(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, 
     void* scan_tmp = base + o_scan;
 
     const int T = 256;
-    cc_init_kernel<BS><<<grid_for(nn, T), T, 0, stream>>>(img, g, thresh, invert, parent, bits);
+    const int64_t uf_tiles_x = (g.NW + kUfTile - 1) / kUfTile;
+    const int64_t uf_tiles = uf_tiles_x * ((g.NH + kUfTile - 1) / kUfTile);
+    cc_local_kernel<BS><<<(unsigned)uf_tiles, kUfTile * kUfTile, 0, stream>>>(img, g, thresh, invert, parent, bits,
+                                                                              uf_tiles_x);
     SPX_LAUNCHED();
     cc_merge_kernel<BS><<<grid_for(nn, T), T, 0, stream>>>(g, parent, bits);
     SPX_LAUNCHED();

@@ -42,7 +42,8 @@ def test_otsu_small_and_constant():
 
 @pytest.mark.parametrize("conn", [8, 4])
 @pytest.mark.parametrize("seed,shape,frac", [(0, (57, 93), 0.45), (1, (64, 64), 0.3), (2, (301, 257), 0.55),
-                                             (3, (1, 40), 0.5), (4, (40, 1), 0.5)])
+                                             (3, (1, 40), 0.5), (4, (40, 1), 0.5), (5, (600, 700), 0.6),
+                                             (6, (512, 512), 0.35)])
 def test_components_match_oracle(conn, seed, shape, frac):
     from imageprocessor_new_amd.components import threshold_components_device
 
