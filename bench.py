@@ -93,6 +93,8 @@ def main() -> int:
                          "detect_centers over strips of a 32768^2 mixed page (erf-disk refinement); "
                          "holes: outline hole-stage sweeps on a 2048x1378 page of 39,042 holes")
     ap.add_argument("--page-size", type=int, default=32768, help="side of the tiled-workload page")
+    ap.add_argument("--learned", action="store_true",
+                    help="tiled workload: also run the learned start (CNN proposal, <= 4 LM steps, fallback)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -319,6 +321,38 @@ def run_tiled(args) -> int:
             torch.cuda.synchronize()
             walls.append(time.perf_counter() - t)
             metas.append(res.meta["stage_ms"])
+        learned_leg = None
+        if args.learned:
+            from imageprocessor_new_amd import learned
+
+            log("learned start: rendering, labelling and training the CNN on the GPU")
+            net, tinfo = learned.train_learned_start(
+                n_tiles=args.train_tiles, epochs=args.train_epochs, batch=4096, lr=1e-3, seed=10_000,
+                device=torch.device("cuda", 0), mix=[(learned.SPEC_SQUARE, 0.5), (learned.SPEC_STAGGERED, 0.5)])
+            kl = dict(kw, learned_start=net, learned_iters=4)
+            detect_centers(img_t, **kl)
+            lwalls, lmetas = [], []
+            for _ in range(steps):
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                res_l = detect_centers(img_t, **kl)
+                torch.cuda.synchronize()
+                lwalls.append(time.perf_counter() - t)
+                lmetas.append(res_l.meta["stage_ms"])
+            lw = float(np.median(lwalls))
+            dl, _ = cKDTree(res_l.centers_xy).query(holes[:, :2].astype(np.float64))
+            dp, _ = cKDTree(res.centers_xy).query(res_l.centers_xy)
+            learned_leg = {
+                "ms_per_step": 1e3 * lw,
+                "features_per_s": res_l.centers_xy.shape[0] / lw,
+                "stage_ms": {k: float(np.median([m[k] for m in lmetas])) for k in lmetas[0]},
+                "fallback_features": res_l.meta["learned_fallbacks"],
+                "detected": int(res_l.centers_xy.shape[0]),
+                "found_within_0.5px": float((dl <= 0.5).mean()),
+                "max_centre_diff_vs_plain_px": float(dp.max()) if dp.size else None,
+                "train": {k: tinfo[k] for k in ("tiles", "samples", "data_seconds", "train_seconds", "epochs")},
+            }
+            log(f"learned leg: {learned_leg}")
     wall = float(np.median(walls))
     found = res.centers_xy
     d, _ = cKDTree(found).query(holes[:, :2].astype(np.float64))
@@ -355,6 +389,7 @@ def run_tiled(args) -> int:
             "strips": int(res.meta["n_tiles"]),
             "stage_ms": stage,
             "front_end_GBps": gbps,
+            "learned_start": learned_leg,
         },
         "roofline": None,
         "cpu_baseline": None,

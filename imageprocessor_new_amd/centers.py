@@ -173,13 +173,17 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
 
 
 def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity, fit_half, fit_options,
-                      model_code, fit_cells, morph_open=0, morph_close=0, events=None):
+                      model_code, fit_cells, morph_open=0, morph_close=0, events=None, net=None, lm_iters=4,
+                      fallbacks=None):
     """Segmentation -> CC -> area filter -> erf fit of one strip, kept on the device: the rows of
     _detect_single_tile's erf group (label order, same seeds, half sides, neighbour lists and
     fit), with every component fitted in one launch and the filtered-out rows (background,
     area outside the limits) given half side 0, which the fit skips. Returns (centres (num, 2)
     float64, ok (num,) bool) device tensors; `events` collects (stage, start, end) CUDA events.
-    The only host synchronisation is the component count."""
+    The only host synchronisation is the component count. With ``net`` (a cnn.LearnedStart)
+    the fit starts from the network's proposal for at most ``lm_iters`` steps, and the rows not
+    converged then are fitted again from the moment start (``fallbacks`` collects their count
+    per strip as device tensors)."""
     from .fit import cell_radius, fit_erf_disks_device, seed_neighbours_device
 
     def mark(name, e0):
@@ -222,9 +226,25 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
     if fit_cells:
         hmax = int(half.max().item()) if num else 0
         nbr = seed_neighbours_device(seeds, img_t.shape, cell_radius(max(hmax, 7)))
-    res = fit_erf_disks_device(img_t, seeds, half, model, options=fit_options, neighbours=nbr)
-    params = res["params"]
-    ok = valid & (res["status"] <= 1) & torch.isfinite(params[:, :2]).all(dim=1)
+    if net is None:
+        res = fit_erf_disks_device(img_t, seeds, half, model, options=fit_options, neighbours=nbr)
+        params, status = res["params"], res["status"]
+    else:
+        init = net.propose(img_t, torch.nan_to_num(seeds, nan=0.0).contiguous(), half)
+        e0 = mark("propose", e0)
+        o = fit_options
+        short = _lib.fit_options(max_iter=int(lm_iters), step_tol=o.step_tol, lambda0=o.lambda0, lambda_up=o.lambda_up,
+                                 lambda_down=o.lambda_down, init_sigma=o.init_sigma, tile_waves=o.tile_waves,
+                                 persistent=o.persistent)
+        res = fit_erf_disks_device(img_t, seeds, half, model, init=init, options=short, neighbours=nbr)
+        redo = valid & (res["status"] != _lib.FIT_CONVERGED)
+        again = fit_erf_disks_device(img_t, seeds, torch.where(redo, half, torch.zeros_like(half)).contiguous(),
+                                     model, options=fit_options, neighbours=nbr)
+        params = torch.where(redo[:, None], again["params"], res["params"])
+        status = torch.where(redo, again["status"], res["status"])
+        if fallbacks is not None:
+            fallbacks.append(redo.sum())
+    ok = valid & (status <= 1) & torch.isfinite(params[:, :2]).all(dim=1)
     mark("refine", e0)
     return params[:, :2].to(torch.float64), ok
 
@@ -261,6 +281,8 @@ def detect_centers(
     fit_step_tol: float = 1e-6,
     fit_cells: bool = True,
     logquad_variant: str = "gpu",
+    learned_start=None,
+    learned_iters: int = 4,
 ) -> CenterResult:
     """Detect and refine centres (reference subpx/centers.py:130-220).
 
@@ -272,6 +294,9 @@ def detect_centers(
     ``fit_step_tol`` and ``fit_cells`` (Voronoi cell mask against neighbouring components,
     see fit.py) configure the erf fits; ``logquad_variant`` picks the GPU backend's
     ("gpu", default) or the CPU backend's ("cpu") log-quadratic algorithm.
+    ``learned_start`` (a trained cnn.LearnedStart, erf methods only) starts every fit from the
+    network's proposal for at most ``learned_iters`` LM steps and refits the features not
+    converged then from the moment start.
     """
     b = resolve_backend(backend)
     if logquad_variant not in ("gpu", "cpu"):
@@ -305,6 +330,20 @@ def detect_centers(
     with gpu_device(device) as dev:
         img_t = to_device_u16(image, dev)
         H, W = img_t.shape
+        if learned_start is not None and refine not in _ERF_METHODS:
+            raise ValueError("learned_start applies to the erf refinements (erf_disk, erf_ellipse)")
+        if tile_h is None and learned_start is not None:
+            thresh = otsu_threshold_device(img_t) if threshold == "otsu" else triangle_threshold_device(img_t)
+            fb = []
+            c_t, ok_t = _erf_strip_device(
+                img_t, int(thresh), invert=invert, area_min=area_min, area_max=area_max, connectivity=connectivity,
+                fit_half=fit_half, fit_options=fit_options, model_code=_ERF_METHODS[refine], fit_cells=bool(fit_cells),
+                morph_open=int(morph_open), morph_close=int(morph_close), net=learned_start,
+                lm_iters=int(learned_iters), fallbacks=fb)
+            meta = dict(base_meta)
+            meta.update({"threshold_value": int(thresh), "learned_fallbacks": int(sum(int(v) for v in fb))})
+            return CenterResult(centers_xy=c_t[ok_t].cpu().numpy(),
+                                method=f"threshold={threshold}, refine={refine}, learned start", backend=b, meta=meta)
         if tile_h is None:
             centers, info = _detect_single_tile(img_t, pre_threshold=None, **common)
             meta = dict(base_meta)
@@ -328,11 +367,15 @@ def detect_centers(
         n_tiles = 0
         y0 = 0
         device_path = refine in _ERF_METHODS and pre_threshold is not None
+        learned_fallbacks = None
+        if learned_start is not None and not device_path:
+            raise ValueError("learned_start needs a global threshold (threshold_mode 'global' or 'auto')")
         pts_dev = None
         if device_path:
             # erf fits keep every strip on the device; one host copy of the centres at the end
             events = []
             dev_centers = []
+            fallbacks = []
             tb = time.perf_counter()
             while y0 < H:
                 y1 = min(H, y0 + tile_h)
@@ -340,7 +383,8 @@ def detect_centers(
                     img_t[y0:y1], int(np.floor(pre_threshold)), invert=invert, area_min=area_min,
                     area_max=area_max, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options,
                     model_code=_ERF_METHODS[refine], fit_cells=bool(fit_cells), morph_open=int(morph_open),
-                    morph_close=int(morph_close), events=events)
+                    morph_close=int(morph_close), events=events, net=learned_start, lm_iters=int(learned_iters),
+                    fallbacks=fallbacks)
                 n_tiles += 1
                 half = int(overlap) // 2
                 keep_lo = y0 if y0 == 0 else (y0 + half)
@@ -357,6 +401,8 @@ def detect_centers(
             stage_ms["strips_wall"] = 1e3 * (time.perf_counter() - tb)
             for name, e0, e1 in events:
                 stage_ms[name] = stage_ms.get(name, 0.0) + e0.elapsed_time(e1)
+            if fallbacks:
+                learned_fallbacks = int(torch.stack(fallbacks).sum())
             all_centers = [pts] if len(pts) else []
         while not device_path and y0 < H:
             y1 = min(H, y0 + tile_h)
@@ -401,6 +447,7 @@ def detect_centers(
             "area_min": int(area_min), "area_max": int(area_max), "device": int(device),
             "small_feature_max": float(small_feature_max), "upsample_factor": int(upsample_factor),
             "n_tiles": n_tiles, "threshold_value": pre_threshold, "stage_ms": stage_ms,
+            "learned_fallbacks": learned_fallbacks,
         },
     )
 

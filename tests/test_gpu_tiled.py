@@ -84,3 +84,34 @@ def test_device_strip_path_matches_the_row_path():
                                      fit_options=opts, fit_cells=True)
     assert got.shape == ref.shape
     np.testing.assert_array_equal(got, ref)
+
+
+def test_learned_start_in_the_device_strip_path():
+    """With a (barely trained) network the strip path still returns the plain fit's centres:
+    every fit either converges from the proposal to the same minimum or is refitted."""
+    import torch
+
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.centers import detect_centers
+    from imageprocessor_new_amd import learned
+
+    H = W = 1024
+    holes, _, _ = synthetic.mixed_page_holes(seed=5, H=H, W=W, region=512)
+    img_t = synthetic.render_page(holes, H, W, seed=5)
+    kw = dict(backend="gpu", threshold="otsu", tile_h=512, overlap=128, refine="erf_disk", area_min=20,
+              area_max=5000, fit_cells=True)
+    with gpu_device(0) as dev:
+        net, _ = learned.train_learned_start(n_tiles=4096, epochs=1, batch=1024, device=dev)
+        plain = detect_centers(img_t, **kw)
+        got = detect_centers(img_t, learned_start=net, **kw)
+        single = detect_centers(img_t, **dict(kw, tile_h=None), learned_start=net)
+    from scipy.spatial import cKDTree
+
+    assert got.centers_xy.shape == plain.centers_xy.shape
+    d, _ = cKDTree(plain.centers_xy).query(got.centers_xy)
+    assert d.max() < 1e-3
+    assert got.meta["learned_fallbacks"] is not None
+    d1, _ = cKDTree(holes[:, :2].astype(np.float64)).query(single.centers_xy)
+    assert (d1 < 0.5).mean() > 0.97
+    with pytest.raises(ValueError):
+        detect_centers(img_t, learned_start=net, **dict(kw, refine="logquad"))
