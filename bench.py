@@ -45,6 +45,32 @@ def measured_traffic():
     return None
 
 
+def measured_valu():
+    """(VALU wave instructions, quarter-rate transcendental instructions) per fit launch of the
+    full workload from the committed rocprofv3 counter summary, or (None, None)."""
+    out = {}
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                for key in ("valu_instructions_per_launch:", "valu_trans_per_launch:"):
+                    if line.startswith(key):
+                        out[key] = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return out.get("valu_instructions_per_launch:"), out.get("valu_trans_per_launch:")
+
+
+def valu_issue_frac(kernel_ms: float):
+    """Fraction of the VALU issue cycles the fit launch used: its wave instructions (quarter-rate
+    transcendentals counted four times) from the committed counter summary, over 1,024 SIMDs x
+    2.4 GHz / 4 cycles per wave64 instruction for the measured launch time."""
+    valu, trans = measured_valu()
+    if valu is None or not kernel_ms:
+        return None
+    capacity = 1024 * 2.4e9 / 4.0 * kernel_ms * 1e-3
+    return (valu + 3.0 * (trans or 0.0)) / capacity
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -274,6 +300,7 @@ def main() -> int:
             "algorithmic_bytes_per_launch": float(K) * (31 * 31 * 2 + 32),
             "kernel_ms": kernel_ms,
             "algorithmic_flop_per_launch": flops,
+            "valu_issue_frac": valu_issue_frac(kernel_ms) if args.size == 8192 else None,
         },
         "cpu_baseline": None if cpu is None else {
             "value": cpu["features_per_s"],
