@@ -301,14 +301,29 @@ __global__ __launch_bounds__(256) void cc_local_kernel(const uint16_t* img, CCGe
     }
 }
 
-// The edges between tiles, by the global lock-free union-find.
+// The edges between tiles, by the global lock-free union-find: one block of 64 threads per
+// tile, thread t taking one of the tile's 46 border nodes (left column, top row without its
+// first node, right column without its first node); interior nodes have no such edge.
 template <int BS>
-__global__ void cc_merge_kernel(CCGeom g, int32_t* parent, const uint8_t* bits) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.NH * g.NW) return;
-    const int64_t ny = n / g.NW, nx = n - (n / g.NW) * g.NW;
-    const int lx = (int)(nx % kUfTile), ly = (int)(ny % kUfTile);
-    if (lx != 0 && ly != 0 && lx != kUfTile - 1) return;  // every neighbour in the same tile
+__global__ __launch_bounds__(64) void cc_merge_kernel(CCGeom g, int32_t* parent, const uint8_t* bits,
+                                                      int64_t tiles_x) {
+    const int t = threadIdx.x;
+    if (t >= 3 * kUfTile - 2) return;
+    int lx, ly;
+    if (t < kUfTile) {
+        ly = t;
+        lx = 0;
+    } else if (t < 2 * kUfTile - 1) {
+        ly = 0;
+        lx = t - kUfTile + 1;
+    } else {
+        ly = t - 2 * kUfTile + 2;
+        lx = kUfTile - 1;
+    }
+    const int64_t ty = blockIdx.x / tiles_x, tx = blockIdx.x - (blockIdx.x / tiles_x) * tiles_x;
+    const int64_t ny = ty * kUfTile + ly, nx = tx * kUfTile + lx;
+    if (ny >= g.NH || nx >= g.NW) return;
+    const int64_t n = ny * g.NW + nx;
     const uint32_t b = bits[n];
     if (!b) return;
     const int64_t u = n - g.NW;
@@ -584,7 +599,7 @@ int threshold_cc_impl(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, 
     cc_local_kernel<BS><<<(unsigned)uf_tiles, kUfTile * kUfTile, 0, stream>>>(img, g, thresh, invert, parent, bits,
                                                                               uf_tiles_x);
     SPX_LAUNCHED();
-    cc_merge_kernel<BS><<<grid_for(nn, T), T, 0, stream>>>(g, parent, bits);
+    cc_merge_kernel<BS><<<(unsigned)uf_tiles, 64, 0, stream>>>(g, parent, bits, uf_tiles_x);
     SPX_LAUNCHED();
     cc_compress_kernel<<<grid_for(nn, T), T, 0, stream>>>(nn, parent, flag);
     SPX_LAUNCHED();
