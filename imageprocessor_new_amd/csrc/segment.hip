@@ -369,32 +369,42 @@ __global__ void cc_acc_init_kernel(CCAcc* acc, int64_t n) {
 }
 
 // Wave reductions for the per-label aggregation (every lane takes part; lanes outside the
-// current label contribute the identity). The result is uniform (first lane's value).
+// current label contribute the identity). Within each row of 16 lanes by four DPP steps
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror: every lane ends with
+// its row's total), then the four row totals read into scalars: no LDS traffic. The result is
+// uniform.
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_i32(int32_t v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
 __device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return __builtin_amdgcn_readfirstlane(v);
+    v = min(v, dpp_i32<0xB1>(v));
+    v = min(v, dpp_i32<0x4E>(v));
+    v = min(v, dpp_i32<0x141>(v));
+    v = min(v, dpp_i32<0x140>(v));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 __device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return __builtin_amdgcn_readfirstlane(v);
+    v = max(v, dpp_i32<0xB1>(v));
+    v = max(v, dpp_i32<0x4E>(v));
+    v = max(v, dpp_i32<0x141>(v));
+    v = max(v, dpp_i32<0x140>(v));
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const uint32_t lo = __shfl_xor((uint32_t)v, off, 64), hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
-        v += ((unsigned long long)hi << 32) | lo;
-    }
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((unsigned long long)hi << 32) | lo;
+// Sum of per-lane values whose wave total fits 32 bits (pixel counts and coordinate sums of
+// one wave's 8 x 8 nodes: below 2^32 for images narrower and shorter than 2^24 px).
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    int32_t w = (int32_t)v;
+    w += dpp_i32<0xB1>(w);
+    w += dpp_i32<0x4E>(w);
+    w += dpp_i32<0x141>(w);
+    w += dpp_i32<0x140>(w);
+    return (uint32_t)__builtin_amdgcn_readlane(w, 0) + (uint32_t)__builtin_amdgcn_readlane(w, 16) +
+           (uint32_t)__builtin_amdgcn_readlane(w, 32) + (uint32_t)__builtin_amdgcn_readlane(w, 48);
 }
 
-// Labels and statistics. A block covers 16 x 16 nodes, each wave an 8 x 8 node tile
-// (16 x 16 pixels for 8-connectivity), so a wave holds few distinct components. Each node
-// writes its pixels' labels and forms its partial stats; the wave then loops over its
-// distinct labels (leader = first pending lane), reduces the members' partials and the
-// leader issues one set of atomics per label: a few per tile instead of seven per node.
 // Background pixels (label 0) are reduced per block into bg_part[block] (no atomics);
 // cc_bg_reduce_kernel folds those into row 0.
 constexpr int kLabTile = 16;  // nodes per block side
@@ -410,11 +420,10 @@ __global__ __launch_bounds__(256) void cc_label_kernel(CCGeom g, const int32_t* 
     const int64_t nx = bx * kLabTile + (wv & 1) * 8 + (lane & 7);
     const int64_t ny = by * kLabTile + (wv >> 1) * 8 + (lane >> 3);
     int32_t bxmin = 0x7fffffff, bymin = 0x7fffffff, bxmax = -1, bymax = -1;
-    unsigned long long bcnt = 0, bsx = 0, bsy = 0;
+    uint32_t bcnt = 0, bsx = 0, bsy = 0;
     int32_t lab = 0;
     int32_t xmin = 0x7fffffff, ymin = 0x7fffffff, xmax = -1, ymax = -1;
-    uint32_t cnt = 0;
-    unsigned long long sx = 0, sy = 0;
+    uint32_t cnt = 0, sx = 0, sy = 0;
     if (nx < g.NW && ny < g.NH) {
         const int64_t n = ny * g.NW + nx;
         const uint32_t b = bits[n];
@@ -431,16 +440,16 @@ __global__ __launch_bounds__(256) void cc_label_kernel(CCGeom g, const int32_t* 
                     xmax = max(xmax, (int32_t)x);
                     ymax = max(ymax, (int32_t)y);
                     ++cnt;
-                    sx += (unsigned long long)x;
-                    sy += (unsigned long long)y;
+                    sx += (uint32_t)x;
+                    sy += (uint32_t)y;
                 } else {
                     bxmin = min(bxmin, (int32_t)x);
                     bymin = min(bymin, (int32_t)y);
                     bxmax = max(bxmax, (int32_t)x);
                     bymax = max(bymax, (int32_t)y);
                     ++bcnt;
-                    bsx += (unsigned long long)x;
-                    bsy += (unsigned long long)y;
+                    bsx += (uint32_t)x;
+                    bsy += (uint32_t)y;
                 }
             }
         }
@@ -457,26 +466,26 @@ __global__ __launch_bounds__(256) void cc_label_kernel(CCGeom g, const int32_t* 
         const int32_t rymin = wave_min_i32(mine ? ymin : 0x7fffffff);
         const int32_t rxmax = wave_max_i32(mine ? xmax : -1);
         const int32_t rymax = wave_max_i32(mine ? ymax : -1);
-        const unsigned long long rcnt = wave_sum_u64(mine ? (unsigned long long)cnt : 0ull);
-        const unsigned long long rsx = wave_sum_u64(mine ? sx : 0ull);
-        const unsigned long long rsy = wave_sum_u64(mine ? sy : 0ull);
+        const uint32_t rcnt = wave_sum_u32(mine ? cnt : 0u);
+        const uint32_t rsx = wave_sum_u32(mine ? sx : 0u);
+        const uint32_t rsy = wave_sum_u32(mine ? sy : 0u);
         if (lane == leader) {
             CCAcc* a = acc + L;
             atomicMin(&a->xmin, rxmin);
             atomicMin(&a->ymin, rymin);
             atomicMax(&a->xmax, rxmax);
             atomicMax(&a->ymax, rymax);
-            atomicAdd(&a->area, (uint32_t)rcnt);
-            atomicAdd(&a->sx, rsx);
-            atomicAdd(&a->sy, rsy);
+            atomicAdd(&a->area, rcnt);
+            atomicAdd(&a->sx, (unsigned long long)rsx);
+            atomicAdd(&a->sy, (unsigned long long)rsy);
         }
         pending = pending && !mine;
     }
     // background: per-wave reduction, then the block's four waves into bg_part[block]
     const int32_t wxmin = wave_min_i32(bxmin), wymin = wave_min_i32(bymin);
     const int32_t wxmax = wave_max_i32(bxmax), wymax = wave_max_i32(bymax);
-    const unsigned long long wcnt = wave_sum_u64(bcnt), wsx = wave_sum_u64(bsx), wsy = wave_sum_u64(bsy);
-    if (lane == 0) s_bg[wv] = CCAcc{wxmin, wymin, wxmax, wymax, (uint32_t)wcnt, 0u, wsx, wsy};
+    const uint32_t wcnt = wave_sum_u32(bcnt), wsx = wave_sum_u32(bsx), wsy = wave_sum_u32(bsy);
+    if (lane == 0) s_bg[wv] = CCAcc{wxmin, wymin, wxmax, wymax, wcnt, 0u, wsx, wsy};
     __syncthreads();
     if (threadIdx.x == 0) {
         CCAcc r = s_bg[0];

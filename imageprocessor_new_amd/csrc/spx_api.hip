@@ -384,6 +384,7 @@ int spx_threshold_cc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, i
     SPX_REQUIRE(img != nullptr && labels != nullptr && n_out != nullptr, "spx_threshold_cc: null pointer");
     SPX_REQUIRE(cap == 0 || (stats != nullptr && centroids != nullptr), "spx_threshold_cc: null stats");
     SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_threshold_cc: bad image shape or pitch");
+    SPX_REQUIRE(H < ((int64_t)1 << 24) && W < ((int64_t)1 << 24), "spx_threshold_cc: image side of 2^24 px or more");
     SPX_REQUIRE(connectivity == 4 || connectivity == 8, "spx_threshold_cc: connectivity must be 4 or 8");
     SPX_REQUIRE(cap >= 0, "spx_threshold_cc: cap < 0");
     return spx::threshold_cc_launch(img, H, W, pitch, thresh, invert, connectivity, labels, stats,

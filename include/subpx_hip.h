@@ -131,16 +131,6 @@ const char* spx_version(void);
 int spx_otsu_u16(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
                  int32_t* thresh_out, spx_stream_t stream);
 
-/* Threshold (foreground = invert ? g <= thresh : g > thresh) + connected components with
- * stats, replacing _segment_binary + connected_components_stats_cpu
- * (subpx/_cpu/centers.py:247-279, subpx/_cpu/components.py:18-34) and
- * connected_components_stats_gpu (subpx/_gpu/components.py:64-134).
- * connectivity 4 or 8. Labels: 0 background, 1..n-1 components numbered in the order of
- * their first pixel in a raster scan over 2x2 blocks (8-connectivity) or pixels
- * (4-connectivity). img, labels (H*W int32, row pitch W), stats ((cap,5) int32
- * [x, y, w, h, area], row 0 = background), centroids ((cap,2) f64): device.
- * n_out (host) receives the label count including the background; SPX_E_CAPACITY when it
- * exceeds cap (n_out still holds the true count). */
 /* Triangle threshold, replacing cv2.threshold(..., THRESH_TRIANGLE) at
  * subpx/_cpu/centers.py:258 (OpenCV defines it for 8-bit images only): img values must be
  * 0..255 (SPX_E_ARG otherwise). thresh_out: host. */
@@ -158,6 +148,16 @@ int spx_threshold_mask(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
 int spx_morph_cross(const uint16_t* in, int64_t H, int64_t W, int64_t pitch, int32_t op, uint16_t* out,
                     spx_stream_t stream);
 
+/* Threshold (foreground = invert ? g <= thresh : g > thresh) + connected components with
+ * stats, replacing _segment_binary + connected_components_stats_cpu
+ * (subpx/_cpu/centers.py:247-279, subpx/_cpu/components.py:18-34) and
+ * connected_components_stats_gpu (subpx/_gpu/components.py:64-134).
+ * connectivity 4 or 8. Labels: 0 background, 1..n-1 components numbered in the order of
+ * their first pixel in a raster scan over 2x2 blocks (8-connectivity) or pixels
+ * (4-connectivity). img, labels (H*W int32, row pitch W), stats ((cap,5) int32
+ * [x, y, w, h, area], row 0 = background), centroids ((cap,2) f64): device.
+ * n_out (host) receives the label count including the background; SPX_E_CAPACITY when it
+ * exceeds cap (n_out still holds the true count). H and W below 2^24. */
 int spx_threshold_cc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
                      int32_t invert, int32_t connectivity, int32_t* labels, int32_t* stats,
                      double* centroids, int32_t cap, int32_t* n_out, spx_stream_t stream);
