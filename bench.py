@@ -139,7 +139,7 @@ def main() -> int:
     from imageprocessor_new_amd import _lib, synthetic
     from imageprocessor_new_amd.backends import gpu_device
     from imageprocessor_new_amd.components import otsu_threshold_device, threshold_components_device
-    from imageprocessor_new_amd.fit import fit_erf_disks_device
+    from imageprocessor_new_amd.fit import autotune_launch, fit_erf_disks_device
 
     if not torch.cuda.is_available():
         print("no HIP device", file=sys.stderr)
@@ -176,7 +176,10 @@ def main() -> int:
         seeds_t = torch.from_numpy(seeds_np).to(dev)
         half_t = torch.full((K,), 15, dtype=torch.int16, device=dev)
         model_t = torch.zeros((K,), dtype=torch.uint8, device=dev)
-        opts = _lib.fit_options(max_iter=30, step_tol=1e-6)
+        # launch configuration (waves per workgroup, persistent grid) chosen by timing
+        tune = autotune_launch(img_t, seeds_t, half_t, model_t, repeats=3)
+        log(f"launch autotune: {tune}")
+        opts = _lib.fit_options(max_iter=30, step_tol=1e-6, tile_waves=0)  # 0: the tuned configuration
         out = fit_erf_disks_device(img_t, seeds_t, half_t, model_t, options=opts)
         stream = torch.cuda.current_stream()
         for _ in range(args.warmup):
@@ -285,6 +288,8 @@ def main() -> int:
             "counted_fraction": acc_frac,
             "mean_iterations": float(iters.mean()),
             "front_end_ms": front_ms,
+            "launch_autotune": {"best_waves_per_workgroup": tune["best"][0],
+                                "best_persistent": bool(tune["best"][1]), "ms": tune["ms"]},
             "plain_lm_features_per_s": fitted_per_s,
             "learned_start": learned_leg,
             "median_err_vs_truth_px": float(np.median(d_true)),

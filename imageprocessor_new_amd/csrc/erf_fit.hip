@@ -252,8 +252,9 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
         a.queue = queues + b;
         // Persistent grid: the resident capacity of the chip, never more than features.
         // Waves of an empty bucket read its count and exit.
+        // (persistent = 0: one wave per feature, the queue then hands each wave its chunk)
         const int64_t want = ((int64_t)K + waves_per_block - 1) / waves_per_block;
-        const int64_t cap = (int64_t)cus * resident_blocks(dev, b, threads);
+        const int64_t cap = opt.persistent ? (int64_t)cus * resident_blocks(dev, b, threads) : want;
         const int blocks = (int)(want < cap ? want : cap);
         hipLaunchKernelGGL(kKernels[b], dim3(blocks), dim3(threads), 0, stream, a);
         SPX_LAUNCHED();

@@ -55,6 +55,49 @@ def cell_radius(half_max: int) -> float:
     return 2.0 * float(np.sqrt(2.0)) * float(half_max) + 1.0
 
 
+# Launch configurations chosen by autotune_launch, per (device, K class).
+_TUNED: dict = {}
+LAUNCH_CANDIDATES = ((4, 1), (2, 1), (1, 1), (4, 0))  # (waves per workgroup, persistent)
+
+
+def _tune_key(dev: torch.device, K: int):
+    return (dev.index if dev.index is not None else torch.cuda.current_device(), max(int(K), 1).bit_length())
+
+
+def _with_launch(o: _lib.SpxFitOptions, tile_waves: int, persistent: int) -> _lib.SpxFitOptions:
+    return _lib.fit_options(max_iter=o.max_iter, step_tol=o.step_tol, lambda0=o.lambda0, lambda_up=o.lambda_up,
+                            lambda_down=o.lambda_down, init_sigma=o.init_sigma, tile_waves=int(tile_waves),
+                            persistent=int(persistent))
+
+
+def autotune_launch(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor, model: torch.Tensor, *,
+                    options: _lib.SpxFitOptions | None = None, neighbours: torch.Tensor | None = None,
+                    candidates=LAUNCH_CANDIDATES, repeats: int = 3) -> dict:
+    """Time the fit's launch configurations (waves per workgroup, persistent grid or one wave
+    per feature) on these features with HIP events and remember the fastest for this device
+    and batch size class: fit_erf_disks_device uses it when options.tile_waves is 0. Results
+    do not depend on the configuration. Returns {"best": (waves, persistent), "ms": {...}}."""
+    base = options if options is not None else _lib.fit_options()
+    K = int(seeds.shape[0])
+    out = None
+    ms = {}
+    for cand in candidates:
+        o = _with_launch(base, *cand)
+        out = fit_erf_disks_device(img, seeds, half, model, options=o, out=out, neighbours=neighbours)
+        times = []
+        for _ in range(max(1, int(repeats))):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fit_erf_disks_device(img, seeds, half, model, options=o, out=out, neighbours=neighbours)
+            e1.record()
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1))
+        ms[f"{cand[0]}w_{'persistent' if cand[1] else 'per_feature'}"] = float(np.median(times))
+    best = min(zip(candidates, ms.values()), key=lambda t: t[1])[0]
+    _TUNED[_tune_key(img.device, K)] = tuple(best)
+    return {"best": tuple(best), "ms": ms}
+
+
 def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor,
                          model: torch.Tensor, init: torch.Tensor | None = None, *,
                          options: _lib.SpxFitOptions | None = None, out: dict | None = None,
@@ -88,6 +131,10 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
             "status": torch.empty((K,), dtype=torch.uint8, device=dev),
         }
     opts = options if options is not None else _lib.fit_options()
+    if opts.tile_waves == 0:
+        tuned = _TUNED.get(_tune_key(dev, K))
+        if tuned is not None:
+            opts = _with_launch(opts, *tuned)
     H, W = img.shape
     pitch = img.stride(0)
     if neighbours is None:

@@ -54,8 +54,10 @@ typedef struct SpxFitOptions {
     float lambda_up;      /* damping factor after a rejected step (10)                 */
     float lambda_down;    /* damping factor after an accepted step (0.1)               */
     float init_sigma;     /* edge sigma of the moment start (1.0 px)                   */
-    int32_t tile_waves;   /* waves per workgroup, 0 = library default (autotuned)      */
-    int32_t persistent;   /* 1 = persistent work-queue launch, 0 = one wave per feature */
+    int32_t tile_waves;   /* waves per workgroup (1..4), 0 = library default (4); the
+                             Python layer's autotuner (fit.autotune_launch) picks one   */
+    int32_t persistent;   /* 1 = grid of the chip's resident capacity pulling from the
+                             work queue, 0 = one wave per feature                        */
 } SpxFitOptions;
 
 /* Options of the edge-erf refinement (defaults of the reference's

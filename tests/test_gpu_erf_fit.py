@@ -166,3 +166,32 @@ def test_cells_are_a_no_op_when_patches_sit_inside_them():
     b = _fit_gpu(arr.image, seeds, 15, "disk", cells=True)
     np.testing.assert_array_equal(a.status[inside], b.status[inside])
     np.testing.assert_array_equal(a.params[inside], b.params[inside])
+
+
+def test_launch_configurations_give_identical_fits_and_autotune_picks_one():
+    import torch
+
+    from imageprocessor_new_amd import _lib
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.fit import LAUNCH_CANDIDATES, _TUNED, autotune_launch, fit_erf_disks_device
+
+    arr = synthetic.make_config("c2", seed=9, H=600, W=600)
+    K = arr.centers_xy.shape[0]
+    hs = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int) + 2, 7, 31).astype(np.int16)
+    with gpu_device(0) as dev:
+        img = torch.from_numpy(arr.image).to(dev)
+        seeds = torch.from_numpy(np.rint(arr.centers_xy).astype(np.float32)).to(dev)
+        half = torch.from_numpy(hs).to(dev)
+        model = torch.from_numpy(arr.ellipse.astype(np.uint8)).to(dev)
+        outs = []
+        for tw, pers in LAUNCH_CANDIDATES:
+            o = _lib.fit_options(tile_waves=tw, persistent=pers)
+            r = fit_erf_disks_device(img, seeds, half, model, options=o)
+            outs.append({k: v.cpu().numpy() for k, v in r.items()})
+        tuned = autotune_launch(img, seeds, half, model, repeats=2)
+        assert tuned["best"] in LAUNCH_CANDIDATES and len(tuned["ms"]) == len(LAUNCH_CANDIDATES)
+        assert _TUNED
+        again = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(img, seeds, half, model).items()}
+    for o in outs[1:] + [again]:
+        for k in ("params", "rms", "iters", "status"):
+            np.testing.assert_array_equal(o[k], outs[0][k])
