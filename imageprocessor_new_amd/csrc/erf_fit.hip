@@ -116,10 +116,12 @@ namespace {
 
 using namespace fitdev;
 
+// LDS patches take NPL x 64 words per wave from dynamic LDS sized at launch for the
+// workgroup's waves, so small workgroups do not reserve LDS for absent waves.
 template <int NPL, int P>
 __global__ __launch_bounds__(256) void erf_fit_kernel(FitArgs a) {
     constexpr bool kLds = Patch<NPL>::kLds;
-    __shared__ uint32_t s_words[kLds ? 4 * NPL * spx::fitdev::kWave : 1];
+    extern __shared__ uint32_t s_words[];
     erf_fit_wave<NPL, P>(a, kLds ? s_words + (threadIdx.x >> 6) * NPL * spx::fitdev::kWave : nullptr);
 }
 
@@ -198,6 +200,12 @@ constexpr KernelFn kKernels[kKernBuckets] = {kernel_of<0>(), kernel_of<1>(), ker
 
 // Resident workgroups per CU of each bucket kernel at a given workgroup size (cached per
 // device): the persistent grid is sized to what the chip holds at once.
+// Dynamic LDS of bucket b's kernel at `threads` per workgroup (the LDS slab of large patches).
+size_t lds_bytes(int b, int threads) {
+    const int npl = 4 << (b / 2);
+    return npl > fitdev::kUnrollMax ? (size_t)(threads / fitdev::kWave) * npl * fitdev::kWave * sizeof(uint32_t) : 0;
+}
+
 int resident_blocks(int dev, int b, int threads) {
     static std::mutex mu;
     static std::map<std::tuple<int, int, int>, int> cache;
@@ -207,7 +215,8 @@ int resident_blocks(int dev, int b, int threads) {
     if (it != cache.end()) return it->second;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kKernels[b]),
-                                                     threads, 0) != hipSuccess || blocks < 1) {
+                                                     threads, lds_bytes(b, threads)) != hipSuccess ||
+        blocks < 1) {
         (void)hipGetLastError();
         blocks = 1;
     }
@@ -256,7 +265,7 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
         const int64_t want = ((int64_t)K + waves_per_block - 1) / waves_per_block;
         const int64_t cap = opt.persistent ? (int64_t)cus * resident_blocks(dev, b, threads) : want;
         const int blocks = (int)(want < cap ? want : cap);
-        hipLaunchKernelGGL(kKernels[b], dim3(blocks), dim3(threads), 0, stream, a);
+        hipLaunchKernelGGL(kKernels[b], dim3(blocks), dim3(threads), lds_bytes(b, threads), stream, a);
         SPX_LAUNCHED();
     }
     return SPX_OK;
