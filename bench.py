@@ -177,9 +177,13 @@ def main() -> int:
         half_t = torch.full((K,), 15, dtype=torch.int16, device=dev)
         model_t = torch.zeros((K,), dtype=torch.uint8, device=dev)
         # launch configuration (waves per workgroup, persistent grid) chosen by timing
-        tune = autotune_launch(img_t, seeds_t, half_t, model_t, repeats=3)
+        tune = autotune_launch(img_t, seeds_t, half_t, model_t, repeats=3,
+                               options=_lib.fit_options(bucket_mask=_lib.bucket_mask_of(half_t.cpu().numpy(),
+                                                                                         model_t.cpu().numpy())))
         log(f"launch autotune: {tune}")
-        opts = _lib.fit_options(max_iter=30, step_tol=1e-6, tile_waves=0)  # 0: the tuned configuration
+        # 0 waves: the tuned configuration; the bucket mask launches only the patch classes present
+        opts = _lib.fit_options(max_iter=30, step_tol=1e-6, tile_waves=0,
+                                bucket_mask=_lib.bucket_mask_of(half_t.cpu().numpy(), model_t.cpu().numpy()))
         out = fit_erf_disks_device(img_t, seeds_t, half_t, model_t, options=opts)
         stream = torch.cuda.current_stream()
         for _ in range(args.warmup):

@@ -50,6 +50,8 @@ class SpxFitOptions(ctypes.Structure):
         ("init_sigma", ctypes.c_float),
         ("tile_waves", ctypes.c_int32),
         ("persistent", ctypes.c_int32),
+        ("bucket_mask", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -225,6 +227,21 @@ def call(name: str, *args) -> int:
 
 
 def fit_options(*, max_iter=30, step_tol=1e-6, lambda0=1e-3, lambda_up=10.0, lambda_down=0.1,
-                init_sigma=1.0, tile_waves=0, persistent=1) -> SpxFitOptions:
+                init_sigma=1.0, tile_waves=0, persistent=1, bucket_mask=0) -> SpxFitOptions:
     return SpxFitOptions(int(max_iter), float(step_tol), float(lambda0), float(lambda_up),
-                         float(lambda_down), float(init_sigma), int(tile_waves), int(persistent))
+                         float(lambda_down), float(init_sigma), int(tile_waves), int(persistent),
+                         int(bucket_mask), 0)
+
+
+def bucket_mask_of(half, model) -> int:
+    """Bucket mask (SpxFitOptions.bucket_mask) covering every (patch class, model) present in
+    host arrays of half sides and models."""
+    import numpy as np
+
+    h = np.asarray(half, dtype=np.int64).ravel()
+    m = np.asarray(model, dtype=np.int64).ravel()
+    ok = (h >= 2) & (h <= 31) & ((m == MODEL_ERF_DISK6) | (m == MODEL_ERF_ELLIPSE7))
+    npl = ((2 * h[ok] + 1) ** 2 + 63) // 64
+    cls = np.select([npl <= 4, npl <= 8, npl <= 16, npl <= 32], [0, 1, 2, 3], 4)
+    bits = np.unique(2 * cls + (m[ok] == MODEL_ERF_ELLIPSE7))
+    return int(np.bitwise_or.reduce(1 << bits)) if bits.size else 0

@@ -235,7 +235,7 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
         o = fit_options
         short = _lib.fit_options(max_iter=int(lm_iters), step_tol=o.step_tol, lambda0=o.lambda0, lambda_up=o.lambda_up,
                                  lambda_down=o.lambda_down, init_sigma=o.init_sigma, tile_waves=o.tile_waves,
-                                 persistent=o.persistent)
+                                 persistent=o.persistent, bucket_mask=o.bucket_mask)
         res = fit_erf_disks_device(img_t, seeds, half, model, init=init, options=short, neighbours=nbr)
         redo = valid & (res["status"] != _lib.FIT_CONVERGED)
         again = fit_erf_disks_device(img_t, seeds, torch.where(redo, half, torch.zeros_like(half)).contiguous(),

@@ -139,7 +139,8 @@ __global__ void zero_i32(int32_t* p, int n) {
 // see K / 256 atomics instead of K.
 __global__ __launch_bounds__(256) void bucket_kernel(const int16_t* half, const uint8_t* model, int K,
                                                      int32_t* counts, int32_t* order, float* params,
-                                                     float* rms, int16_t* iters, uint8_t* status) {
+                                                     float* rms, int16_t* iters, uint8_t* status,
+                                                     int32_t mask) {
     __shared__ int32_t s_cnt[kKernBuckets];
     __shared__ int32_t s_base[kKernBuckets];
     if (threadIdx.x < kKernBuckets) s_cnt[threadIdx.x] = 0;
@@ -150,13 +151,15 @@ __global__ __launch_bounds__(256) void bucket_kernel(const int16_t* half, const 
     if (f < K) {
         const int hs = half[f];
         const int md = model[f];
-        if (hs < 2 || hs > kMaxHalf || (md != SPX_MODEL_ERF_DISK6 && md != SPX_MODEL_ERF_ELLIPSE7)) {
+        const bool known = !(hs < 2 || hs > kMaxHalf || (md != SPX_MODEL_ERF_DISK6 && md != SPX_MODEL_ERF_ELLIPSE7));
+        const int bk = known ? 2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0) : 0;
+        if (!known || (mask != 0 && !((mask >> bk) & 1))) {
             for (int i = 0; i < 7; ++i) params[(int64_t)f * 7 + i] = __builtin_nanf("");
             rms[f] = __builtin_nanf("");
             iters[f] = 0;
             status[f] = SPX_FIT_TOO_FEW;
         } else {
-            b = 2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0);
+            b = bk;
             local = atomicAdd(&s_cnt[b], 1);
         }
     }
@@ -244,7 +247,7 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
     zero_i32<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(stats), 8 + 2 * kKernBuckets);
     SPX_LAUNCHED();
     bucket_kernel<<<(K + 255) / 256, 256, 0, stream>>>(half, model, K, counts, order, params, rms,
-                                                        iters, status);
+                                                        iters, status, opt.bucket_mask);
     SPX_LAUNCHED();
 
     int dev = 0;
@@ -256,6 +259,7 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
     FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, nullptr, nullptr, nullptr,
               stats, params, rms, iters, status, nbr, nb, K};
     for (int b = 0; b < kKernBuckets; ++b) {
+        if (opt.bucket_mask != 0 && !((opt.bucket_mask >> b) & 1)) continue;
         a.order = order + (int64_t)b * K;
         a.count = counts + b;
         a.queue = queues + b;

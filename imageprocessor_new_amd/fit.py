@@ -67,7 +67,7 @@ def _tune_key(dev: torch.device, K: int):
 def _with_launch(o: _lib.SpxFitOptions, tile_waves: int, persistent: int) -> _lib.SpxFitOptions:
     return _lib.fit_options(max_iter=o.max_iter, step_tol=o.step_tol, lambda0=o.lambda0, lambda_up=o.lambda_up,
                             lambda_down=o.lambda_down, init_sigma=o.init_sigma, tile_waves=int(tile_waves),
-                            persistent=int(persistent))
+                            persistent=int(persistent), bucket_mask=o.bucket_mask)
 
 
 def autotune_launch(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor, model: torch.Tensor, *,
@@ -178,7 +178,8 @@ def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: i
     else:
         model_np = np.broadcast_to(np.asarray(model, dtype=np.uint8), (K,)).copy()
     opts = _lib.fit_options(max_iter=max_iter, step_tol=step_tol, lambda0=lambda0,
-                            lambda_up=lambda_up, lambda_down=lambda_down, init_sigma=init_sigma)
+                            lambda_up=lambda_up, lambda_down=lambda_down, init_sigma=init_sigma,
+                            bucket_mask=_lib.bucket_mask_of(half_np, model_np))
     with gpu_device(device) as dev:
         img_t = to_device_u16(image, dev)
         if K == 0:

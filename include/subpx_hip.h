@@ -58,6 +58,12 @@ typedef struct SpxFitOptions {
                              Python layer's autotuner (fit.autotune_launch) picks one   */
     int32_t persistent;   /* 1 = grid of the chip's resident capacity pulling from the
                              work queue, 0 = one wave per feature                        */
+    int32_t bucket_mask;  /* bit 2 c + m: launch the (patch class c, model m) kernel;
+                             c = 0..4 for 4, 8, 16, 32, 64 pixels per lane ((2 half + 1)^2
+                             / 64 rounded up), m = 1 for ellipses. Features of buckets not
+                             launched are not fitted (NaN, status too few). 0 = all ten,
+                             so callers that know their patch sizes skip empty launches  */
+    int32_t reserved;
 } SpxFitOptions;
 
 /* Options of the edge-erf refinement (defaults of the reference's

@@ -195,3 +195,32 @@ def test_launch_configurations_give_identical_fits_and_autotune_picks_one():
     for o in outs[1:] + [again]:
         for k in ("params", "rms", "iters", "status"):
             np.testing.assert_array_equal(o[k], outs[0][k])
+
+
+def test_bucket_mask_skips_only_the_masked_classes():
+    import torch
+
+    from imageprocessor_new_amd import _lib
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.fit import fit_erf_disks_device
+
+    arr = synthetic.make_config("c2", seed=10, H=600, W=600)
+    hs = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int) + 2, 7, 31).astype(np.int16)
+    md = arr.ellipse.astype(np.uint8)
+    full = _lib.bucket_mask_of(hs, md)
+    assert full != 0
+    with gpu_device(0) as dev:
+        args = (torch.from_numpy(arr.image).to(dev), torch.from_numpy(np.rint(arr.centers_xy).astype(np.float32)).to(dev),
+                torch.from_numpy(hs).to(dev), torch.from_numpy(md).to(dev))
+        ref = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(*args).items()}
+        got = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(*args, options=_lib.fit_options(bucket_mask=full)).items()}
+        low = full & -full  # only the lowest present bucket
+        part = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(*args, options=_lib.fit_options(bucket_mask=low)).items()}
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k])
+    inside = np.array([_lib.bucket_mask_of([h], [m]) == low for h, m in zip(hs, md)])
+    assert inside.any() and (~inside).any()
+    for k in ref:
+        np.testing.assert_array_equal(part[k][inside], ref[k][inside])
+    assert np.all(part["status"][~inside] == _lib.FIT_TOO_FEW)
+    assert np.all(np.isnan(part["params"][~inside]))
