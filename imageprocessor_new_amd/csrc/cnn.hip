@@ -548,6 +548,39 @@ __device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ in, uin
 #pragma unroll
         for (int g = 0; g < kGM; ++g) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         const uint16_t* wcol = wt + (int64_t)(nb * 16 + li) * kp;
+        if constexpr (!FC && CIN >= 16) {
+            // tap-major: the row geometry once per item, the tap offset and bounds per tap
+            constexpr int CPT = CIN / 16;  // 16-wide k chunks per tap
+            int base[kGM], y0[kGM], x0[kGM];
+#pragma unroll
+            for (int g = 0; g < kGM; ++g) {
+                const int row = (mb0 + g) * 16 + li;
+                const int pt = row / PIX, pix = row - pt * PIX;
+                const int oy = pix / HOUT, ox = pix - (pix / HOUT) * HOUT;
+                const bool live = g < nmb && row < M;
+                base[g] = pt * HIN * HIN * CIN;
+                y0[g] = live ? 2 * oy - 1 : -1000;  // an absent row fails every bounds test
+                x0[g] = 2 * ox - 1;
+            }
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) {
+                const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+                for (int cc = 0; cc < CPT; ++cc) {
+                    const int c0 = cc * 16 + 4 * lq;
+                    const bf16x4s b = *reinterpret_cast<const bf16x4s*>(wcol + (tap * CPT + cc) * 16 + 4 * lq);
+#pragma unroll
+                    for (int g = 0; g < kGM; ++g) {
+                        if (g >= nmb) break;
+                        const int iy = y0[g] + ky, ix = x0[g] + kx;
+                        bf16x4s a = bf16x4s{0, 0, 0, 0};
+                        if ((unsigned)iy < (unsigned)HIN && (unsigned)ix < (unsigned)HIN)
+                            a = *reinterpret_cast<const bf16x4s*>(in + base[g] + (iy * HIN + ix) * CIN + c0);
+                        acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[g], 0, 0, 0);
+                    }
+                }
+            }
+        } else
         for (int kc = 0; kc < kc_n; ++kc) {
             const int k0 = kc * 16 + 4 * lq;
             const bf16x4s b = *reinterpret_cast<const bf16x4s*>(wcol + k0);
