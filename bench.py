@@ -114,10 +114,11 @@ def main() -> int:
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM counters)")
-    ap.add_argument("--workload", choices=("fit", "tiled", "holes"), default="fit",
+    ap.add_argument("--workload", choices=("fit", "tiled", "holes", "cnn"), default="fit",
                     help="fit: the batched fit of the 8192^2 array (default); tiled: end-to-end "
                          "detect_centers over strips of a 32768^2 mixed page (erf-disk refinement); "
-                         "holes: outline hole-stage sweeps on a 2048x1378 page of 39,042 holes")
+                         "holes: outline hole-stage sweeps on a 2048x1378 page of 39,042 holes; "
+                         "cnn: the fused CNN proposal on the 65,536 patches of the 8192^2 array")
     ap.add_argument("--page-size", type=int, default=32768, help="side of the tiled-workload page")
     ap.add_argument("--learned", action="store_true",
                     help="tiled workload: also run the learned start (CNN proposal, <= 4 LM steps, fallback)")
@@ -135,6 +136,8 @@ def main() -> int:
         return run_tiled(args)
     if args.workload == "holes":
         return run_holes(args)
+    if args.workload == "cnn":
+        return run_cnn(args)
 
     from imageprocessor_new_amd import _lib, synthetic
     from imageprocessor_new_amd.backends import gpu_device
@@ -535,6 +538,71 @@ def holes_cpu_baseline(image, start, a0, truth, radius, sigma, noise, pg):
     dt = time.perf_counter() - t
     return {"value": len(c) / dt, "unit": "hole-sweeps/s", "cores": 1, "kind": "port",
             "sample": f"one FP64 NumPy oracle sweep over {len(c)} holes of a {S}x{S} crop ({dt:.1f} s)"}
+
+
+CNN_FLOP_PER_PATCH = 2.0 * (256 * 16 * 9 + 64 * 32 * 144 + 16 * 64 * 288 + 4 * 64 * 576 + 256 * 128 + 128 * 7)
+PEAK_BF16_DENSE_TFLOPS = 2500.0
+
+
+def run_cnn(args) -> int:
+    """The learned start's network alone: 65,536 32 x 32 BF16 patches prepared from the c1 array
+    (one per hole, 31 x 31 fit patch resampled), then the fused inference kernel; one step =
+    one forward over all patches (weights: the initialisation, which does not change the work)."""
+    import torch
+
+    from imageprocessor_new_amd import synthetic
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.cnn import LearnedStart
+
+    arr = synthetic.make_config("c1", seed=args.seed, H=args.size, W=args.size)
+    K = arr.centers_xy.shape[0]
+    steps = max(1, min(args.steps, 50))
+    with gpu_device(0) as dev:
+        net = LearnedStart(device=dev)
+        img = torch.from_numpy(arr.image).to(dev)
+        seeds = torch.from_numpy(np.rint(arr.centers_xy).astype(np.float32)).to(dev)
+        half = torch.full((K,), 15, dtype=torch.int16, device=dev)
+        x, norm = net.prepare(img, seeds, half)
+        for _ in range(max(1, args.warmup)):
+            net.forward_raw(x)
+        times = []
+        for _ in range(steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            net.forward_raw(x)
+            e1.record()
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        net.prepare(img, seeds, half)
+        e1.record()
+        e1.synchronize()
+        prep_ms = e0.elapsed_time(e1)
+    ms = float(np.median(times))
+    achieved = CNN_FLOP_PER_PATCH * K / (ms * 1e-3) / 1e12
+    log(f"cnn: {K} patches, forward {ms:.3f} ms ({achieved:.1f} TFLOP/s), prepare {prep_ms:.3f} ms")
+    result = {
+        "metric": "learned-start CNN proposals per second (32x32 BF16 patches, fused MFMA inference)",
+        "value": K / (ms * 1e-3),
+        "unit": "patches/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(1, args.warmup),
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": f"CNN forward over {K} patches of the {args.size}^2 c1 array (conv 16-32-64-64 s2, "
+                               "fc 128 -> 7)", "patches": K, "prepare_ms": prep_ms},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_DENSE_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_BF16_DENSE_TFLOPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
 
 
 if __name__ == "__main__":
