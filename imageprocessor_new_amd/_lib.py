@@ -146,6 +146,7 @@ SIGNATURES = {
     "spx_gemm_bf16": (_I32, [_P, _I64, _I64, _P, _I64, _I64, _P, _I32, _I32, _I32, _P]),
     "spx_cnn_prepare": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _P, _P, _P]),
     "spx_cnn_encode": (_I32, [_P, _P, _I32, _P, _P]),
+    "spx_cnn_propose": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _P, _P, _P, _P]),
     "spx_cnn_forward": (_I32, [_P, _I32, _P, _P, _P, _P, _P]),
     "spx_cnn_train_step": (_I32, [_P, _P, _P, _I32, _P, _P, _P, _P, _I32, ctypes.c_float, _P, _P]),
     "spx_cnn_to_bf16": (_I32, [_P, _I64, _P, _P]),

@@ -97,7 +97,17 @@ class LearnedStart:
         return out
 
     def propose(self, img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor) -> torch.Tensor:
-        """Start parameters (K, 7) [x0, y0, r, sigma, A, B, q] for each feature (device)."""
+        """Start parameters (K, 7) [x0, y0, r, sigma, A, B, q] for each feature (device): patch
+        preparation, network and decoding in one launch (spx_cnn_propose)."""
+        K = int(seeds.shape[0])
+        out = torch.empty((K, 7), dtype=torch.float32, device=self.device)
+        H, W = img.shape
+        _lib.call("spx_cnn_propose", _p(img), H, W, img.stride(0), _p(seeds), _p(half), K, _p(self.theta),
+                  _p(self.theta_bf), _p(out), stream_handle())
+        return out
+
+    def propose_two_step(self, img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor) -> torch.Tensor:
+        """The same proposal by spx_cnn_prepare + spx_cnn_forward (the training path's steps)."""
         x, norm = self.prepare(img, seeds, half)
         K = int(seeds.shape[0])
         out = torch.empty((K, 7), dtype=torch.float32, device=self.device)

@@ -229,25 +229,22 @@ __global__ void to_bf16_kernel(const float* x, int64_t n, uint16_t* y) {
 constexpr int kPrepWaves = 4;
 constexpr int kMaxSide = 63;
 
-__global__ __launch_bounds__(256) void prepare_kernel(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
-                                                      const float* seeds, const int16_t* half, int K,
-                                                      uint16_t* x_out, float* norm) {
-    __shared__ float patch[kPrepWaves][kMaxSide * kMaxSide];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int f = blockIdx.x * kPrepWaves + wave;
-    if (f >= K) return;
-    const int hs = __builtin_amdgcn_readfirstlane(max(1, min((int)half[f], (kMaxSide - 1) / 2)));
+// One wave prepares one patch: gather into the u16 staging area P (side^2 values), ring
+// mean and maximum, bilinear resampling to 32 x 32 written as BF16 to xdst (global memory or
+// LDS). Returns {B0, A0} through b0 / a0 (every lane).
+__device__ __forceinline__ void prep_patch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, float cx,
+                                           float cy, int hs, int lane, uint16_t* P, uint16_t* xdst, float& b0,
+                                           float& a0) {
     const int side = 2 * hs + 1;
-    const float cx = rintf(seeds[2 * (int64_t)f]), cy = rintf(seeds[2 * (int64_t)f + 1]);
     const int64_t ox = (int64_t)cx - hs, oy = (int64_t)cy - hs;
     float rsum = 0.0f, rcnt = 0.0f, vmax = -3.0e38f;
-    float* P = patch[wave];
     for (int p = lane; p < side * side; p += 64) {
         const int row = p / side, col = p - (p / side) * side;
         const int64_t gx = min(max(ox + col, (int64_t)0), W - 1);
         const int64_t gy = min(max(oy + row, (int64_t)0), H - 1);
-        const float v = (float)img[gy * pitch + gx];
-        P[p] = v;
+        const uint16_t raw = img[gy * pitch + gx];
+        const float v = (float)raw;
+        P[p] = raw;
         if (row == 0 || row == side - 1 || col == 0 || col == side - 1) {
             rsum += v;
             rcnt += 1.0f;
@@ -272,14 +269,29 @@ __global__ __launch_bounds__(256) void prepare_kernel(const uint16_t* img, int64
         const int x0 = min((int)px, side - 2 >= 0 ? side - 2 : 0), y0 = min((int)py, side - 2 >= 0 ? side - 2 : 0);
         const float fx = px - (float)x0, fy = py - (float)y0;
         const int x1 = min(x0 + 1, side - 1), y1 = min(y0 + 1, side - 1);
-        const float v = (1.0f - fy) * ((1.0f - fx) * P[y0 * side + x0] + fx * P[y0 * side + x1]) +
-                        fy * ((1.0f - fx) * P[y1 * side + x0] + fx * P[y1 * side + x1]);
-        x_out[(int64_t)f * 1024 + o] = f2bf((v - B0) / A0);
+        const float v = (1.0f - fy) * ((1.0f - fx) * (float)P[y0 * side + x0] + fx * (float)P[y0 * side + x1]) +
+                        fy * ((1.0f - fx) * (float)P[y1 * side + x0] + fx * (float)P[y1 * side + x1]);
+        xdst[o] = f2bf((v - B0) / A0);
     }
+    b0 = B0;
+    a0 = A0;
+}
+
+__global__ __launch_bounds__(256) void prepare_kernel(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
+                                                      const float* seeds, const int16_t* half, int K,
+                                                      uint16_t* x_out, float* norm) {
+    __shared__ uint16_t patch[kPrepWaves][kMaxSide * kMaxSide];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int f = blockIdx.x * kPrepWaves + wave;
+    if (f >= K) return;
+    const int hs = __builtin_amdgcn_readfirstlane(max(1, min((int)half[f], (kMaxSide - 1) / 2)));
+    const float cx = rintf(seeds[2 * (int64_t)f]), cy = rintf(seeds[2 * (int64_t)f + 1]);
+    float B0, A0;
+    prep_patch(img, H, W, pitch, cx, cy, hs, lane, patch[wave], x_out + (int64_t)f * 1024, B0, A0);
     if (lane == 0) {
         norm[5 * (int64_t)f + 0] = cx;
         norm[5 * (int64_t)f + 1] = cy;
-        norm[5 * (int64_t)f + 2] = (float)side;
+        norm[5 * (int64_t)f + 2] = (float)(2 * hs + 1);
         norm[5 * (int64_t)f + 3] = B0;
         norm[5 * (int64_t)f + 4] = A0;
     }
@@ -628,21 +640,12 @@ __device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ in, uin
     }
 }
 
-__global__ __launch_bounds__(256) void cnn_infer_kernel(const uint16_t* __restrict__ x, int K, InferWeights iw,
-                                                        const float* __restrict__ norm, float* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) uint16_t bufA[kIP * 16 * 16 * 16];  // L1 out, L3 out, fc1 out
-    __shared__ __attribute__((aligned(16))) uint16_t bufB[kIP * 8 * 8 * 32];    // input, L2 out, L4 out
-    __shared__ float head[kIP * 8];
+// Layers 1-5 on the patches in bufB (NHWC BF16, 32 x 32 x 1 each), the head, then the
+// outputs of patches p0 .. p0 + kIP - 1 (< K): decoded with nrm[patch][5] when `decode`, raw
+// otherwise.
+__device__ __forceinline__ void infer_tail(uint16_t* bufA, uint16_t* bufB, float* head, const InferWeights& iw,
+                                           int K, int p0, bool decode, const float* nrm, float* out) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int p0 = blockIdx.x * kIP;
-    // input patches (2 KB each) -> bufB, 16 bytes per thread per step; missing patches read 0
-    for (int q = tid; q < kIP * 128; q += 256) {
-        const int pt = q >> 7, e = (q & 127) * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (p0 + pt < K) v = *reinterpret_cast<const uint4*>(x + (int64_t)(p0 + pt) * 1024 + e);
-        *reinterpret_cast<uint4*>(bufB + pt * 1024 + e) = v;
-    }
-    __syncthreads();
     const float* th = iw.theta;
     infer_layer<1, 16, 32, false>(bufB, bufA, iw.wt + iw.off[0], iw.kp[0], th + param_offset_dev(0) + 9 * 16, wave,
                                   lane);
@@ -670,13 +673,66 @@ __global__ __launch_bounds__(256) void cnn_infer_kernel(const uint16_t* __restri
     __syncthreads();
     if (tid < kIP && p0 + tid < K) {
         const int64_t k = p0 + tid;
-        if (norm) {
-            decode_one(head + tid * 8, norm + 5 * k, out + 7 * k);
+        if (decode) {
+            decode_one(head + tid * 8, nrm + 5 * tid, out + 7 * k);
         } else {
 #pragma unroll
             for (int j = 0; j < 7; ++j) out[7 * k + j] = head[tid * 8 + j];
         }
     }
+}
+
+__global__ __launch_bounds__(256) void cnn_infer_kernel(const uint16_t* __restrict__ x, int K, InferWeights iw,
+                                                        const float* __restrict__ norm, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint16_t bufA[kIP * 16 * 16 * 16];  // L1 out, L3 out, fc1 out
+    __shared__ __attribute__((aligned(16))) uint16_t bufB[kIP * 8 * 8 * 32];    // input, L2 out, L4 out
+    __shared__ float head[kIP * 8];
+    const int tid = threadIdx.x;
+    const int p0 = blockIdx.x * kIP;
+    // input patches (2 KB each) -> bufB, 16 bytes per thread per step; missing patches read 0
+    for (int q = tid; q < kIP * 128; q += 256) {
+        const int pt = q >> 7, e = (q & 127) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (p0 + pt < K) v = *reinterpret_cast<const uint4*>(x + (int64_t)(p0 + pt) * 1024 + e);
+        *reinterpret_cast<uint4*>(bufB + pt * 1024 + e) = v;
+    }
+    __syncthreads();
+    infer_tail(bufA, bufB, head, iw, K, p0, norm != nullptr, norm ? norm + 5 * (int64_t)p0 : nullptr, out);
+}
+
+// Proposal in one launch: each wave prepares its patch straight into LDS (the u16 staging
+// area aliases bufA, free until the first layer writes it), then the network and the decode.
+static_assert(kIP * kMaxSide * kMaxSide <= kIP * 16 * 16 * 16, "patch staging must fit in bufA");
+static_assert(kIP == 256 / 64, "one wave per patch");
+
+__global__ __launch_bounds__(256) void cnn_propose_kernel(const uint16_t* __restrict__ img, int64_t H, int64_t W,
+                                                          int64_t pitch, const float* __restrict__ seeds,
+                                                          const int16_t* __restrict__ half, int K, InferWeights iw,
+                                                          float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint16_t bufA[kIP * 16 * 16 * 16];
+    __shared__ __attribute__((aligned(16))) uint16_t bufB[kIP * 8 * 8 * 32];
+    __shared__ float head[kIP * 8];
+    __shared__ float s_norm[kIP * 5];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p0 = blockIdx.x * kIP;
+    const int f = p0 + wave;
+    if (f < K) {
+        const int hs = __builtin_amdgcn_readfirstlane(max(1, min((int)half[f], (kMaxSide - 1) / 2)));
+        const float cx = rintf(seeds[2 * (int64_t)f]), cy = rintf(seeds[2 * (int64_t)f + 1]);
+        float B0, A0;
+        prep_patch(img, H, W, pitch, cx, cy, hs, lane, bufA + wave * kMaxSide * kMaxSide, bufB + wave * 1024, B0, A0);
+        if (lane == 0) {
+            s_norm[5 * wave + 0] = cx;
+            s_norm[5 * wave + 1] = cy;
+            s_norm[5 * wave + 2] = (float)(2 * hs + 1);
+            s_norm[5 * wave + 3] = B0;
+            s_norm[5 * wave + 4] = A0;
+        }
+    } else {
+        for (int o = lane; o < 1024; o += 64) bufB[wave * 1024 + o] = 0;
+    }
+    __syncthreads();
+    infer_tail(bufA, bufB, head, iw, K, p0, true, s_norm, out);
 }
 
 }  // namespace
@@ -707,10 +763,10 @@ int cnn_encode_launch(const float* params, const float* norm, int K, float* t, h
 // Inference: out = decoded parameters (K x 7) when norm != NULL, raw network outputs
 // otherwise. The fused kernel reads a transposed, K-padded copy of the BF16 weights made in
 // the library workspace first.
-int cnn_forward_launch(const uint16_t* x, int K, const float* theta, const uint16_t* theta_bf, const float* norm,
-                       float* out, hipStream_t stream) {
-    if (K == 0) return SPX_OK;
-    InferWeights iw{};
+namespace {
+// The transposed, K-padded BF16 weight copy of the fused kernels, in the library workspace.
+int infer_weights(const float* theta, const uint16_t* theta_bf, InferWeights& iw, hipStream_t stream) {
+    iw = InferWeights{};
     int64_t total = 0;
     for (int l = 0; l < 5; ++l) {
         iw.kp[l] = (kNet[l].kd + 15) / 16 * 16;
@@ -726,7 +782,30 @@ int cnn_forward_launch(const uint16_t* x, int K, const float* theta, const uint1
     iw.wbf = theta_bf;
     transpose_weights_kernel<<<grid1(total), 256, 0, stream>>>(theta_bf, iw, wt);
     SPX_LAUNCHED();
+    return SPX_OK;
+}
+}  // namespace
+
+int cnn_forward_launch(const uint16_t* x, int K, const float* theta, const uint16_t* theta_bf, const float* norm,
+                       float* out, hipStream_t stream) {
+    if (K == 0) return SPX_OK;
+    InferWeights iw;
+    int rc = infer_weights(theta, theta_bf, iw, stream);
+    if (rc != SPX_OK) return rc;
     cnn_infer_kernel<<<(K + kIP - 1) / kIP, 256, 0, stream>>>(x, K, iw, norm, out);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+// Patch preparation + inference + decode in one launch: params (K x 7).
+int cnn_propose_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                       const int16_t* half, int K, const float* theta, const uint16_t* theta_bf, float* params,
+                       hipStream_t stream) {
+    if (K == 0) return SPX_OK;
+    InferWeights iw;
+    int rc = infer_weights(theta, theta_bf, iw, stream);
+    if (rc != SPX_OK) return rc;
+    cnn_propose_kernel<<<(K + kIP - 1) / kIP, 256, 0, stream>>>(img, H, W, pitch, seeds, half, K, iw, params);
     SPX_LAUNCHED();
     return SPX_OK;
 }

@@ -70,6 +70,9 @@ int gemm_bf16_launch(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t
 int cnn_prepare_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                        const int16_t* half, int K, uint16_t* x_out, float* norm, hipStream_t stream);
 int cnn_encode_launch(const float* params, const float* norm, int K, float* t, hipStream_t stream);
+int cnn_propose_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                       const int16_t* half, int K, const float* theta, const uint16_t* theta_bf, float* params,
+                       hipStream_t stream);
 int cnn_forward_launch(const uint16_t* x, int K, const float* theta, const uint16_t* theta_bf, const float* norm,
                        float* out, hipStream_t stream);
 int cnn_train_step_launch(const uint16_t* x, const float* t, const float* w, int Bn, float* theta,
@@ -142,6 +145,16 @@ int spx_cnn_encode(const float* params, const float* norm, int32_t K, float* t_o
     spx::clear_error();
     SPX_REQUIRE(K >= 0 && (K == 0 || (params && norm && t_out)), "spx_cnn_encode: bad arguments");
     return spx::cnn_encode_launch(params, norm, K, t_out, spx::as_stream(stream));
+}
+
+int spx_cnn_propose(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds, const int16_t* half,
+                    int32_t K, const float* theta, const uint16_t* theta_bf, float* params_out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(K >= 0, "spx_cnn_propose: K < 0");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_cnn_propose: bad image shape or pitch");
+    SPX_REQUIRE(K == 0 || (img && seeds && half && theta && theta_bf && params_out), "spx_cnn_propose: null pointer");
+    return spx::cnn_propose_launch(img, H, W, pitch, seeds, half, K, theta, theta_bf, params_out,
+                                   spx::as_stream(stream));
 }
 
 int spx_cnn_forward(const uint16_t* x, int32_t K, const float* theta, const uint16_t* theta_bf,

@@ -284,6 +284,13 @@ int spx_cnn_prepare(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, co
 /* Fit parameters (K, 7) -> normalised network targets (K, 7) for the given norms. Device. */
 int spx_cnn_encode(const float* params, const float* norm, int32_t K, float* t_out, spx_stream_t stream);
 
+/* The learned start in one launch: spx_cnn_prepare + spx_cnn_forward with decoding, the
+ * patch resampled straight into LDS (same values as the two-step path). img (H, pitch)
+ * uint16, seeds (K, 2) f32, half (K) int16, params_out (K, 7) f32 [x0, y0, r, s, A, B, q]:
+ * device. No reference counterpart (the learned start is task-defined). */
+int spx_cnn_propose(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds, const int16_t* half,
+                    int32_t K, const float* theta, const uint16_t* theta_bf, float* params_out, spx_stream_t stream);
+
 /* Forward pass. theta (FP32, biases read from it) and theta_bf (BF16 copy) hold the
  * parameters; out (K, 7): decoded fit parameters [x0, y0, r, s, A, B, q] when norm is given,
  * raw outputs when norm is NULL. Device. */

@@ -159,3 +159,18 @@ def test_training_reduces_loss():
         log = net.train(xb, tt, epochs=15, batch=512, lr=3e-3)
         _, l1 = net.grad(xb, tt)
     assert l1 < 0.5 * l0, (l0, l1, log.losses)
+
+
+def test_fused_proposal_equals_prepare_then_forward():
+    from imageprocessor_new_amd.cnn import LearnedStart
+
+    arr = synthetic.make_config("c2", seed=32, H=300, W=300)
+    seeds = np.concatenate([arr.centers_xy[:35], [[2.0, 3.0], [297.0, 150.0]]]).astype(np.float32)  # K = 37
+    half = np.resize(np.array([5, 7, 15, 20, 31, 12, 3, 9], np.int16), len(seeds))
+    with _dev() as dev:
+        net = LearnedStart(_theta(6), device=dev)
+        args = (torch.from_numpy(arr.image).to(dev), torch.from_numpy(seeds).to(dev), torch.from_numpy(half).to(dev))
+        fused = net.propose(*args).cpu().numpy()
+        two = net.propose_two_step(*args).cpu().numpy()
+    assert np.isfinite(fused).all()
+    np.testing.assert_array_equal(fused, two)
