@@ -3,11 +3,12 @@
 Drop-in for the reference ``subpx`` centre path on ``backend="gpu"``: ``detect_centers``,
 ``detect_centers_tiled``, ``connected_components_stats``, ``dedupe_centers``,
 ``refine_centers_edge_erf_gpu``, ``refine_centers_edge_moment_gpu``,
-``refine_centers_logquad_gpu_match_cpu`` and the batched erf-disk fits, all backed by hand-written
+``refine_centers_logquad_gpu_match_cpu``, the batched erf-disk fits and the page outline fit
+(``fit_outline``), all backed by hand-written
 gfx950 HIP kernels in libsubpx_hip.so (C-ABI: include/subpx_hip.h).
 """
 
-from .types import CenterResult, ComponentStatsResult, ErfFitResult  # noqa: F401
+from .types import CenterResult, ComponentStatsResult, ErfFitResult, OutlineFitResult  # noqa: F401
 
 __version__ = "0.1.0"
 
@@ -34,6 +35,14 @@ def __getattr__(name):
         from . import edge_erf
 
         return getattr(edge_erf, name)
+    if name in ("fit_outline", "ellipse_widths", "outline_rays"):
+        from . import outline
+
+        return getattr(outline, name)
+    if name in ("fit_holes", "HolePage"):
+        from . import holes
+
+        return getattr(holes, name)
     if name == "refine_centers_logquad_gpu_match_cpu":
         from . import refine
 

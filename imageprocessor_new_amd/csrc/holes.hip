@@ -340,8 +340,9 @@ __global__ void far_splat_kernel(const double* c, const double* a, const double*
     if (i >= n) return;
     const double x = c[2 * (int64_t)i] + mx, y = c[2 * (int64_t)i + 1] + my;
     const double fx = floor(x), fy = floor(y);
+    // all 4 x 4 taps inside the grid (compared in FP64, so a non-finite centre is dropped too)
+    if (!(fx >= 1.0 && fx + 2.0 < (double)wp && fy >= 1.0 && fy + 2.0 < (double)hp)) return;
     const int ix = (int)fx, iy = (int)fy;
-    if (ix < 1 || ix + 2 >= wp || iy < 1 || iy + 2 >= hp) return;
     double wx[4], wy[4];
     keys4d(x - fx, wx);
     keys4d(y - fy, wy);
@@ -391,6 +392,114 @@ __global__ void disk_templates_kernel(double radius, double sigma, int N, int U,
         const double wf = 1.0 - 0.5 * (1.0 + cos(kPi * fmin(fmax((rho - r0) / ramp, 0.0), 1.0)));
         far[q] = (float)(t0 * wf);
     }
+}
+
+// Page outline fit (subpx/_outline/page.py, model.py): the outline's pixel coverage on the
+// template's fine grid, per-window gathers for the stacks, noise line and hole starts, and the
+// far light's splat of a, a d, a d^2.
+
+// Coverage of fine pixel (iy, ix): the mean over its aa x aa sub-samples of a one-sample
+// linear ramp across the edge, clamp(0.5 + (edge[idx] - rho) * scale, 0, 1), with the
+// sub-samples' radius rho and angle-table index idx precomputed (model.py:149-159).
+__global__ void outline_coverage_kernel(const double* __restrict__ edge, const double* __restrict__ rho,
+                                        const int32_t* __restrict__ ang, int N, int aa, double scale,
+                                        double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * N) return;
+    const int iy = (int)(t / N), ix = (int)(t - (int64_t)iy * N);
+    const int64_t M = (int64_t)N * aa;
+    double acc = 0.0;
+    for (int q = 0; q < aa; ++q) {
+        const int64_t row = ((int64_t)iy * aa + q) * M + (int64_t)ix * aa;
+        for (int p = 0; p < aa; ++p) {
+            const double v = 0.5 + (edge[ang[row + p]] - rho[row + p]) * scale;
+            acc += fmin(fmax(v, 0.0), 1.0);
+        }
+    }
+    out[t] = acc / (double)(aa * aa);
+}
+
+struct WindowArgs {
+    const float* image;
+    int64_t H, W, pitch;
+    const double* page;  // null: 0
+    const float* tmpl;   // null: no template values
+    int N, U, R;
+    const int32_t* idx;
+    int n;
+    const double* c;
+    const double* a;
+    const double* level;  // null: 0
+    const double* delta;
+    double radius_fine;  // > 0: bins farther than this (fine units) from the template centre get -1
+    int32_t* bin;        // outputs, (n, (2R+1)^2) each; any may be null
+    uint8_t* valid;
+    double* data;
+    double* model;
+    double* own;
+    double* v0;
+};
+
+// One thread per (hole, window pixel): the pixel's grey level, the page model plus the hole's
+// level, the hole's own near light a (T0 + d T1 + d^2 T2), T0 alone, and the stack bin
+// rint(sx), rint(sy) of the pixel's fine offset from the hole (page.py:206-216, 294-299, 488-505,
+// 732-766). Pixels off the frame have valid 0, data and model at the clamped pixel, bin -1.
+__global__ __launch_bounds__(256) void outline_window_kernel(WindowArgs A) {
+    const int side = 2 * A.R + 1;
+    const int64_t P = (int64_t)side * side;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)A.n * P) return;
+    const int i = (int)(t / P);
+    const int q = (int)(t - (int64_t)i * P);
+    const int h = A.idx[i];
+    const double x = A.c[2 * (int64_t)h], y = A.c[2 * (int64_t)h + 1];
+    const int64_t px = (int64_t)rint(x) + (q % side) - A.R, py = (int64_t)rint(y) + (q / side) - A.R;
+    const bool ok = px >= 0 && px < A.W && py >= 0 && py < A.H;
+    const int64_t cx = px < 0 ? 0 : (px >= A.W ? A.W - 1 : px), cy = py < 0 ? 0 : (py >= A.H ? A.H - 1 : py);
+    if (A.valid) A.valid[t] = ok ? 1 : 0;
+    if (A.data) A.data[t] = (double)A.image[cy * A.pitch + cx];
+    if (A.model) A.model[t] = (A.page ? A.page[cy * A.W + cx] : 0.0) + (A.level ? A.level[h] : 0.0);
+    const double half_n = (double)(A.N / 2);
+    const double sx = ((double)px - x) * A.U + half_n, sy = ((double)py - y) * A.U + half_n;
+    if (A.tmpl && (A.own || A.v0)) {
+        float v[3];
+        sample_tmpl<3>(A.tmpl, A.N, sx, sy, v);
+        const double de = A.delta[h];
+        if (A.v0) A.v0[t] = (double)v[0];
+        if (A.own) A.own[t] = A.a[h] * ((double)v[0] + de * ((double)v[1] + de * (double)v[2]));
+    }
+    if (A.bin) {
+        const double bx = rint(sx), by = rint(sy);
+        bool in = ok && bx >= 0.0 && bx < (double)A.N && by >= 0.0 && by < (double)A.N;
+        if (in && A.radius_fine > 0.0) in = hypot(sx - half_n, sy - half_n) <= A.radius_fine;
+        A.bin[t] = in ? (int32_t)by * A.N + (int32_t)bx : -1;
+    }
+}
+
+// Keys splat of a, a d and a d^2 of every hole on three padded pixel grids (FP64, grid k at
+// grid + k hp wp); a hole whose 4 x 4 taps leave the grid is dropped (page.py:371-397).
+__global__ void outline_far_splat_kernel(const double* c, const double* a, const double* delta, int n, int mx, int my,
+                                         int hp, int wp, double* grid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = c[2 * (int64_t)i] + mx, y = c[2 * (int64_t)i + 1] + my;
+    const double fx = floor(x), fy = floor(y);
+    if (!(fx >= 1.0 && fx + 2.0 < (double)wp && fy >= 1.0 && fy + 2.0 < (double)hp)) return;
+    const int ix = (int)fx, iy = (int)fy;
+    double wx[4], wy[4];
+    keys4d(x - fx, wx);
+    keys4d(y - fy, wy);
+    const double m[3] = {a[i], a[i] * delta[i], a[i] * delta[i] * delta[i]};
+    const int64_t plane = (int64_t)hp * wp;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const double w = wy[q] * wx[p];
+            const int64_t o = (int64_t)(iy + q - 1) * wp + (ix + p - 1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) atomicAdd(grid + k * plane + o, m[k] * w);
+        }
 }
 
 // k-th smallest of x[i * stride], i < n (no NaNs), by an 8-round radix select over the 64-bit
@@ -465,6 +574,40 @@ __global__ __launch_bounds__(1024) void select_kth_kernel(const double* x, int64
 }
 
 }  // namespace
+
+int outline_coverage_launch(const double* edge, const double* rho, const int32_t* ang, int N, int aa, double scale,
+                            double* out, hipStream_t stream) {
+    const int64_t nn = (int64_t)N * N;
+    outline_coverage_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, stream>>>(edge, rho, ang, N, aa, scale, out);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int outline_window_launch(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,
+                          const float* tmpl, int N, int U, int R, const int32_t* idx, int n, const double* c,
+                          const double* a, const double* level, const double* delta, double radius_fine,
+                          int32_t* bin, uint8_t* valid, double* data, double* model, double* own, double* v0,
+                          hipStream_t stream) {
+    if (n == 0) return SPX_OK;
+    const int64_t total = (int64_t)n * (2 * R + 1) * (2 * R + 1);
+    if ((total + 255) / 256 >= ((int64_t)1 << 32)) {
+        set_error("outline_window: too many (hole, pixel) pairs for one launch");
+        return SPX_E_ARG;
+    }
+    WindowArgs A{image, H, W, pitch, page, tmpl, N, U, R, idx, n, c, a, level, delta, radius_fine,
+                 bin, valid, data, model, own, v0};
+    outline_window_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int outline_far_splat_launch(const double* c, const double* a, const double* delta, int n, int mx, int my, int hp,
+                             int wp, double* grid, hipStream_t stream) {
+    if (n == 0) return SPX_OK;
+    outline_far_splat_kernel<<<(n + 255) / 256, 256, 0, stream>>>(c, a, delta, n, mx, my, hp, wp, grid);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
 
 int select_kth_launch(const double* x, int64_t n, int64_t stride, const int64_t* cols, const int64_t* ks, int nk,
                       double* out, hipStream_t stream) {

@@ -115,6 +115,15 @@ int far_splat_launch(const double* c, const double* a, const double* delta, doub
                      int wp, float* grid, hipStream_t stream);
 int disk_templates_launch(double radius, double sigma, int N, int U, double r0, double ramp, int B, float* tmpl,
                           float* far, hipStream_t stream);
+int outline_coverage_launch(const double* edge, const double* rho, const int32_t* ang, int N, int aa, double scale,
+                            double* out, hipStream_t stream);
+int outline_window_launch(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,
+                          const float* tmpl, int N, int U, int R, const int32_t* idx, int n, const double* c,
+                          const double* a, const double* level, const double* delta, double radius_fine,
+                          int32_t* bin, uint8_t* valid, double* data, double* model, double* own, double* v0,
+                          hipStream_t stream);
+int outline_far_splat_launch(const double* c, const double* a, const double* delta, int n, int mx, int my, int hp,
+                             int wp, double* grid, hipStream_t stream);
 
 }  // namespace spx
 
@@ -503,6 +512,37 @@ int spx_far_splat(const double* c, const double* a, const double* delta, double 
     SPX_REQUIRE(n >= 0 && hp > 0 && wp > 0 && cd0 > 0, "spx_far_splat: bad sizes");
     SPX_REQUIRE(n == 0 || (c && a && delta && grid), "spx_far_splat: null pointer");
     return spx::far_splat_launch(c, a, delta, cd0, n, mx, my, hp, wp, grid, spx::as_stream(stream));
+}
+
+int spx_outline_coverage(const double* edge, const double* rho, const int32_t* ang, int32_t N, int32_t aa,
+                         double scale, double* out, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(N >= 8 && N <= 4096 && aa >= 1 && aa <= 16, "spx_outline_coverage: bad grid");
+    SPX_REQUIRE(edge && rho && ang && out, "spx_outline_coverage: null pointer");
+    return spx::outline_coverage_launch(edge, rho, ang, N, aa, scale, out, spx::as_stream(stream));
+}
+
+int spx_outline_windows(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,
+                        const float* tmpl, int32_t N, int32_t U, int32_t R, const int32_t* idx, int32_t n,
+                        const double* c, const double* a, const double* level, const double* delta,
+                        double radius_fine, int32_t* bin, uint8_t* valid, double* data, double* model, double* own,
+                        double* v0, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(n >= 0 && H > 0 && W > 0 && pitch >= W, "spx_outline_windows: bad sizes");
+    SPX_REQUIRE(R >= 0 && R <= 255 && N >= 8 && N <= 4096 && U >= 1, "spx_outline_windows: bad window or grid");
+    SPX_REQUIRE(n == 0 || (image && idx && c), "spx_outline_windows: null pointer");
+    SPX_REQUIRE(!(own || v0) || tmpl, "spx_outline_windows: template values need the template");
+    SPX_REQUIRE(!own || (a && delta), "spx_outline_windows: own light needs brightness and edge offset");
+    return spx::outline_window_launch(image, H, W, pitch, page, tmpl, N, U, R, idx, n, c, a, level, delta,
+                                      radius_fine, bin, valid, data, model, own, v0, spx::as_stream(stream));
+}
+
+int spx_outline_far_splat(const double* c, const double* a, const double* delta, int32_t n, int32_t mx, int32_t my,
+                          int32_t hp, int32_t wp, double* grid, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(n >= 0 && hp > 0 && wp > 0, "spx_outline_far_splat: bad sizes");
+    SPX_REQUIRE(n == 0 || (c && a && delta && grid), "spx_outline_far_splat: null pointer");
+    return spx::outline_far_splat_launch(c, a, delta, n, mx, my, hp, wp, grid, spx::as_stream(stream));
 }
 
 }  // extern "C"

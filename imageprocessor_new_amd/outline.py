@@ -1,0 +1,935 @@
+"""Page outline fit on the GPU: ``fit_outline`` (reference subpx/outline.py:34-170).
+
+One sharp outline (a superellipse turned by phi, times 1 + harmonics) convolved with one blur
+(two centred anisotropic Gaussians, or a Gaussian times a circular pupil's transfer function)
+is shared by every hole of a page; each hole has its own centre, brightness, level and edge
+offset (a move of the outline along its normal). The fit alternates
+
+- the hole stage: Gauss-Newton of (x, y, a, level[, delta]) per hole against the complete page
+  model, one colour class (chunk) at a time: the HIP kernels of the hole pass
+  (``spx_hole_step`` / ``spx_hole_render`` / ``spx_hole_sweep``) with this fit's templates,
+  a far light of three kernels (``spx_outline_far_splat`` + FFT) and square render windows;
+- the shape stage: the neighbour-cleaned stacked average hole (``spx_outline_windows`` gathers,
+  one FP64 scatter-add) fitted by bounded least squares over the outline and blur parameters
+  (SciPy's trust-region solver on the host, as the reference; every model evaluation renders
+  the template on the device: ``spx_outline_coverage`` + rocFFT).
+
+Templates (subpx/_outline/model.py:25-267): T = pixel box * blur * coverage(outline) on an
+N x N grid of 1/U px, rendered spectrally; T1, T2 (first and second edge-offset terms) by central
+differences of 0.1 px; Tx, Ty by spectral derivatives. The driver follows PageFit
+(subpx/_outline/page.py:78-937): greedy colours in chunks of 2000 holes, a noise-variance line
+from the leftovers, Cauchy weights at 3 typical leftovers, the staged release round -> super ->
+full, relaxation 0.5, box doubling when the light outgrows it, results from the exact area of
+the offset outline.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .backends import gpu_device, resolve_backend, stream_handle
+from .holes import _even_fast
+from .types import OutlineFitResult
+
+BLUR_FAMILIES = {"gauss2": 24, "lens": 48}  # blur family -> first template box (px)
+MIN_HOLES = 10
+UPSAMPLE = 8
+
+
+def _ptr(t):
+    return None if t is None else int(t.data_ptr())
+
+
+def _off(t, elements: int) -> int:
+    """Device address of element `elements` of a contiguous tensor."""
+    return int(t.data_ptr()) + int(elements) * t.element_size()
+
+
+def _np_median(x: torch.Tensor) -> float:
+    """NumPy's median (mean of the middle pair) of a 1-D device tensor."""
+    v = torch.sort(x.reshape(-1)).values
+    n = v.numel()
+    return float(0.5 * (v[(n - 1) // 2] + v[n // 2]))
+
+
+# ------------------------------------------------------------------------------------- template
+class OutlineTemplate:
+    """The page's hole template on the device (model.py:25-267). theta = [log a, log b,
+    log(p - 1), phi, (c_k, s_k) per harmonic k, blur parameters]; lengths in px."""
+
+    N_TABLE = 8192
+
+    def __init__(self, upsample=8, size_px=24, harmonics=(1, 3), blur="gauss2", aa=4, device=None):
+        if blur not in ("gauss2", "lens", "gauss1"):
+            raise ValueError("blur must be 'gauss2', 'lens' or 'gauss1'")
+        self.U, self.N, self.aa = int(upsample), int(size_px * upsample), int(aa)
+        self.ks, self.blur = tuple(int(k) for k in harmonics), str(blur)
+        dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        n, u, a = self.N, self.U, self.aa
+        # the aa x aa sub-samples of every fine pixel: radius and angle-table index (fixed per grid)
+        m = n * a
+        xa = (np.arange(m) - m // 2 + 0.5) / (u * a) - 0.5 / u
+        X, Y = np.meshgrid(xa, xa)
+        self.rho_d = torch.from_numpy(np.hypot(X, Y)).to(dev)
+        ang = np.mod(np.rint((np.arctan2(Y, X) + np.pi) / (2.0 * np.pi) * self.N_TABLE), self.N_TABLE)
+        self.ang_d = torch.from_numpy(ang.astype(np.int32)).to(dev)
+        self.table_angles = -np.pi + 2.0 * np.pi * np.arange(self.N_TABLE) / self.N_TABLE
+        self.angles = np.linspace(-np.pi, np.pi, 4096, endpoint=False)
+        x = (np.arange(n) - n // 2) / u
+        self.xx, self.yy = np.meshgrid(x, x)
+        self.xx_d, self.yy_d = torch.from_numpy(self.xx).to(dev), torch.from_numpy(self.yy).to(dev)
+        f = torch.from_numpy(np.fft.fftfreq(n, d=1.0 / u)).to(dev)
+        self.fx, self.fy = f[None, :], f[:, None]
+        self.nu = torch.hypot(self.fx, self.fy)
+        self.pixel_otf = torch.sinc(self.fx) * torch.sinc(self.fy)
+        self.edge_fix = 1.0 / (torch.sinc(self.fx / u) * torch.sinc(self.fy / u) * torch.sinc(self.nu / (u * a)))
+
+    # parameters
+    @property
+    def n_shape(self):
+        return 4 + 2 * len(self.ks)
+
+    @property
+    def n_blur(self):
+        return {"gauss2": 5, "lens": 3, "gauss1": 2}[self.blur]
+
+    @property
+    def n_params(self):
+        return self.n_shape + self.n_blur
+
+    def pack(self, a, b, p, phi, coeffs, blur_params):
+        return np.concatenate([[math.log(a), math.log(b), math.log(p - 1.0), phi],
+                               np.asarray(coeffs, float).ravel(), np.asarray(blur_params, float)])
+
+    def blur_pack(self, **kw):
+        if self.blur == "gauss2":
+            w2 = kw.get("w2", 0.05)
+            return [math.log(kw.get("s1x", 1.0)), math.log(kw.get("s1y", 1.0)), math.log(kw.get("s2x", 2.0)),
+                    math.log(kw.get("s2y", 2.0)), math.log(w2 / (1.0 - w2))]
+        sxy = [math.log(kw.get("sx", 1.0)), math.log(kw.get("sy", 1.0))]
+        return sxy + [math.log(kw.get("cutoff", 0.45))] if self.blur == "lens" else sxy
+
+    def unpack(self, theta):
+        th = np.asarray(theta, float)
+        return (math.exp(th[0]), math.exp(th[1]), 1.0 + math.exp(th[2]), th[3], th[4:self.n_shape].reshape(-1, 2),
+                th[self.n_shape:])
+
+    def start(self, half_width_px, blur_sigma_px=1.0):
+        s = blur_sigma_px
+        return self.pack(half_width_px, half_width_px, 2.0, 0.0, np.zeros((len(self.ks), 2)),
+                         self.blur_pack(s1x=s, s1y=s, s2x=2 * s, s2y=2 * s, sx=s, sy=s))
+
+    def bounds(self):
+        lo, hi = np.full(self.n_params, -np.inf), np.full(self.n_params, np.inf)
+        lo[0:2], hi[0:2] = math.log(0.2), math.log(8.0)
+        lo[2], hi[2] = math.log(0.3), math.log(30.0)
+        lo[3], hi[3] = -np.pi / 4, np.pi / 4
+        lo[4:self.n_shape], hi[4:self.n_shape] = -0.3, 0.3
+        j = self.n_shape
+        if self.blur == "gauss2":
+            lo[j:j + 4], hi[j:j + 4] = math.log(0.15), math.log(4.0)
+            lo[j + 4], hi[j + 4] = math.log(0.005 / 0.995), math.log(0.6 / 0.4)
+        elif self.blur == "lens":
+            lo[j:j + 2], hi[j:j + 2] = math.log(0.1), math.log(4.0)
+            lo[j + 2], hi[j + 2] = math.log(0.2), math.log(8.0)
+        else:
+            lo[j:j + 2], hi[j:j + 2] = math.log(0.15), math.log(4.0)
+        return lo, hi
+
+    # outline (host: a few thousand angles)
+    def radius_at(self, theta, angles):
+        """r(angle) and dr / d angle (px) of the outline (model.py:130-147)."""
+        a, b, p, phi, coeffs, _ = self.unpack(theta)
+        ang = np.asarray(angles, float)
+        ct, st = np.cos(ang - phi), np.sin(ang - phi)
+        qa, qb = np.abs(ct / a), np.abs(st / b)
+        base = (qa ** p + qb ** p) ** (-1.0 / p)
+        dbase = -base ** (1 + p) * (qa ** (p - 1) * np.sign(ct) * (-st / a) + qb ** (p - 1) * np.sign(st) * (ct / b))
+        mod, dmod = np.ones_like(ang), np.zeros_like(ang)
+        for i, k in enumerate(self.ks):
+            ck, sk = np.cos(k * ang), np.sin(k * ang)
+            mod = mod + coeffs[i, 0] * ck + coeffs[i, 1] * sk
+            dmod = dmod + k * (coeffs[i, 1] * ck - coeffs[i, 0] * sk)
+        return base * mod, dbase * mod + base * dmod
+
+    def indicator(self, theta, delta=0.0):
+        """Fine-pixel coverage of the outline moved `delta` px along its normal (device, N x N)."""
+        r, dr = self.radius_at(theta, self.table_angles)
+        edge = r + delta * np.hypot(r, dr) / r if delta else r
+        edge_d = torch.from_numpy(np.ascontiguousarray(edge)).to(self.device)
+        out = torch.empty((self.N, self.N), dtype=torch.float64, device=self.device)
+        _lib.call("spx_outline_coverage", _ptr(edge_d), _ptr(self.rho_d), _ptr(self.ang_d), self.N, self.aa,
+                  float(self.U * self.aa), _ptr(out), stream_handle())
+        return out
+
+    # blur
+    def otf(self, theta):
+        bp = self.unpack(theta)[5]
+        fx2, fy2 = self.fx * self.fx, self.fy * self.fy
+        k = -2.0 * math.pi ** 2
+        if self.blur == "gauss2":
+            s1x, s1y, s2x, s2y = np.exp(bp[:4])
+            w2 = 1.0 / (1.0 + math.exp(-bp[4]))
+            g1 = torch.exp(k * (s1x ** 2 * fx2 + s1y ** 2 * fy2))
+            g2 = torch.exp(k * (s2x ** 2 * fx2 + s2y ** 2 * fy2))
+            return (1.0 - w2) * g1 + w2 * g2
+        sx, sy = np.exp(bp[:2])
+        g = torch.exp(k * (sx ** 2 * fx2 + sy ** 2 * fy2))
+        if self.blur == "lens":
+            s = torch.clamp(self.nu / math.exp(bp[2]), 0.0, 1.0)
+            g = g * (2.0 / math.pi) * (torch.acos(s) - s * torch.sqrt(1.0 - s * s))
+        return g
+
+    def blur_kernel(self, theta) -> np.ndarray:
+        k = torch.fft.fftshift(torch.fft.ifft2(self.otf(theta)).real).cpu().numpy()
+        return k / k.sum()
+
+    def blur_fwhm_px(self, theta):
+        k = self.blur_kernel(theta)
+        c = self.N // 2
+        out = []
+        for prof in (k[c], k[:, c]):
+            half = prof[c] / 2
+            j = c
+            while j + 1 < len(prof) and prof[j + 1] > half:
+                j += 1
+            out.append(2 * (j - c + (prof[j] - half) / max(prof[j] - prof[j + 1], 1e-12)) / self.U)
+        return tuple(out)
+
+    def blur_sigma_px(self, theta):
+        k = self.blur_kernel(theta)
+        return math.sqrt((k * self.xx ** 2).sum()), math.sqrt((k * self.yy ** 2).sum())
+
+    # template
+    def render(self, theta, delta=0.0, box_px=0.0):
+        """T on the device (N x N, FP64): coverage -> spectrum x sharp-edge correction x blur x
+        pixel box [x an extra box of box_px] -> back (model.py:200-205)."""
+        spec = torch.fft.fft2(torch.fft.ifftshift(self.indicator(theta, delta)))
+        spec = spec * (self.edge_fix * self.otf(theta) * self.pixel_otf)
+        if box_px > 0:
+            spec = spec * (torch.sinc(self.fx * box_px) * torch.sinc(self.fy * box_px))
+        return torch.fft.fftshift(torch.fft.ifft2(spec).real)
+
+    def render_quad(self, theta, h=0.1):
+        t0, tp, tm = self.render(theta), self.render(theta, h), self.render(theta, -h)
+        return t0, (tp - tm) / (2 * h), (tp + tm - 2 * t0) / (2 * h * h)
+
+    def derivatives_xy(self, t):
+        spec = torch.fft.fft2(torch.fft.ifftshift(t))
+        w = 2j * math.pi
+        return (torch.fft.fftshift(torch.fft.ifft2(spec * (w * self.fx)).real),
+                torch.fft.fftshift(torch.fft.ifft2(spec * (w * self.fy)).real))
+
+    # readings
+    def outline_area(self, theta, delta=0.0):
+        r, dr = self.radius_at(theta, self.angles)
+        if delta:
+            r = r + delta * np.hypot(r, dr) / r
+        return float(np.pi * np.mean(r * r))
+
+    def cd_px(self, theta, delta=0.0):
+        return 2.0 * math.sqrt(self.outline_area(theta, delta) / math.pi)
+
+    def extents_px(self, theta):
+        r, _ = self.radius_at(theta, np.array([0.0, math.pi, math.pi / 2, -math.pi / 2]))
+        return float(r[0] + r[1]), float(r[2] + r[3])
+
+    def centroid_px(self, theta):
+        ind = self.indicator(theta)
+        s = ind.sum()
+        return float((ind * self.xx_d).sum() / s), float((ind * self.yy_d).sum() / s)
+
+    def describe(self, theta, pixel_size=1.0):
+        a, b, p, phi, coeffs, _ = self.unpack(theta)
+        fw, (ex, ey) = self.blur_fwhm_px(theta), self.extents_px(theta)
+        return dict(cd_nm=self.cd_px(theta) * pixel_size, extent_x_nm=ex * pixel_size, extent_y_nm=ey * pixel_size,
+                    superellipse=dict(a_nm=a * pixel_size, b_nm=b * pixel_size, p=p, phi_deg=math.degrees(phi)),
+                    harmonics={f"k{k}": [float(coeffs[i, 0]), float(coeffs[i, 1])] for i, k in enumerate(self.ks)},
+                    blur_fwhm_nm=[v * pixel_size for v in fw],
+                    blur_rms_nm=[v * pixel_size for v in self.blur_sigma_px(theta)], blur_family=self.blur)
+
+
+# ---------------------------------------------------------------------------------- shape stage
+def stage_map(t, stage):
+    """Free parameters and ties of a stage (model.py:270-290): round outline with isotropic
+    blur; superellipse with the full blur; everything."""
+    j = t.n_shape
+    if stage == "round":
+        free, ties = [0, j], {1: 0}
+        if t.blur == "gauss2":
+            free += [j + 2, j + 4]
+            ties.update({j + 1: j, j + 3: j + 2})
+        elif t.blur == "lens":
+            free += [j + 2]
+            ties.update({j + 1: j})
+        else:
+            ties.update({j + 1: j})
+        return free, ties
+    if stage == "super":
+        return [0, 1, 2, 3] + list(range(j, t.n_params)), {}
+    return list(range(t.n_params)), {}
+
+
+def fit_stages(t, render_for, y, sw, theta0, stages=("round", "super", "full"), max_nfev=80, grid=True, log=None):
+    """Bounded least squares of scale * render_for(theta) + const on y (device, sqrt-weights sw)
+    in stages, scale and const solved linearly per theta (model.py:293-368). render_for
+    returns the device model values; the solver (SciPy 'trf') runs on the host."""
+    from scipy.optimize import least_squares
+
+    lo_all, hi_all = t.bounds()
+    theta = np.clip(np.asarray(theta0, float), lo_all + 1e-6, hi_all - 1e-6)
+    w = sw * sw
+    wsum = w.sum()
+    yw = (w * y).sum() / wsum
+
+    def resid(th):
+        m = render_for(th)
+        mw = (w * m).sum() / wsum
+        dm = m - mw
+        var = (w * dm * dm).sum()
+        cov = (w * dm * (y - yw)).sum()
+        sc = torch.where(var > 0, cov / torch.where(var > 0, var, torch.ones_like(var)), torch.ones_like(var))
+        co = yw - sc * mw
+        r = sw * (sc * m + co - y)
+        host = torch.cat([r, sc.reshape(1), co.reshape(1)]).cpu().numpy()
+        return host[:-2], float(host[-2]), float(host[-1])
+
+    chi2, scale, const = float("nan"), 1.0, 0.0
+    for stage in stages:
+        free, ties = stage_map(t, stage)
+        base = theta.copy()
+        if stage == "round":
+            base[1], base[2], base[3] = base[0], 0.0, 0.0
+            base[4:t.n_shape] = 0.0
+            if grid:
+                j, best = t.n_shape, None
+                a0 = math.exp(base[0])
+                for a in a0 * np.linspace(0.4, 1.8, 15):
+                    for s in np.linspace(0.45, 2.2, 12):
+                        q = base.copy()
+                        q[0] = q[1] = math.log(a)
+                        q[j] = q[j + 1] = math.log(s)
+                        if t.blur == "gauss2":
+                            q[j + 2] = q[j + 3] = math.log(2 * s)
+                            q[j + 4] = math.log(0.05 / 0.95)
+                        elif t.blur == "lens":
+                            q[j + 2] = math.log(0.5)
+                        q = np.clip(q, lo_all + 1e-6, hi_all - 1e-6)
+                        chi = float(np.mean(resid(q)[0] ** 2))
+                        if best is None or chi < best[0]:
+                            best = (chi, q)
+                base = best[1]
+
+        def expand(q, base=base, free=free, ties=ties):
+            th = base.copy()
+            th[free] = q
+            for k, src in ties.items():
+                th[k] = th[src]
+            return th
+
+        lo, hi = lo_all[free], hi_all[free]
+        r = least_squares(lambda q: resid(expand(q))[0], np.clip(base[free], lo + 1e-6, hi - 1e-6), bounds=(lo, hi),
+                          method="trf", x_scale="jac", xtol=1e-9, ftol=1e-10, max_nfev=max_nfev)
+        theta = expand(r.x)
+        _, scale, const = resid(theta)
+        chi2 = float(np.mean(r.fun ** 2))
+        if log is not None:
+            log(f"    shape stage {stage}: chi2 {chi2:.3f}, CD {t.cd_px(theta):.3f} px, scale {scale:.4f}, "
+                f"evaluations {r.nfev}")
+    return theta, scale, const, chi2
+
+
+# -------------------------------------------------------------------------------------- readings
+def ellipse_widths(t, theta, delta):
+    """x / y diameters of the equal-area ellipse with the x / y spread of the outline moved
+    delta along its normal (outline.py:196-217): area and moments are polynomials in delta."""
+    delta = np.asarray(delta, float)
+    r, dr = t.radius_at(theta, t.angles)
+    g = np.hypot(r, dr) / r
+    cs, sn = np.cos(t.angles), np.sin(t.angles)
+
+    def moment(k, weight):
+        # integral over the angle of (r + delta g)^k weight
+        return sum(math.comb(k, m) * 2.0 * np.pi * np.mean(r ** (k - m) * g ** m * weight) * delta ** m
+                   for m in range(k + 1))
+
+    area = moment(2, 1.0) / 2
+    mx, my = moment(3, cs) / 3, moment(3, sn) / 3
+    var_x = moment(4, cs * cs) / 4 / area - (mx / area) ** 2
+    var_y = moment(4, sn * sn) / 4 / area - (my / area) ** 2
+    cd = 2.0 * np.sqrt(np.maximum(area, 1e-12) / np.pi)
+    ratio = np.sqrt(np.sqrt(np.maximum(var_x, 1e-12) / np.maximum(var_y, 1e-12)))
+    return cd * ratio, cd / ratio
+
+
+def outline_rays(t, theta, delta, n_rays=128):
+    """Each hole's outline as its distance from the outline's centroid at angles 2 pi j / n
+    (outline.py:220-251): the page outline solved per ray by Newton on the polar angle, the
+    offset delta added along the normal (stretch 1 / cos of the ray-normal angle)."""
+    cx, cy = t.centroid_px(theta)
+    phi = 2.0 * np.pi * np.arange(int(n_rays)) / int(n_rays)
+    ex, ey = np.cos(phi), np.sin(phi)
+    ang = phi.copy()
+    for _ in range(30):
+        r, dr = t.radius_at(theta, ang)
+        u, v = np.cos(ang), np.sin(ang)
+        f = ex * (r * v - cy) - ey * (r * u - cx)
+        fp = ex * (dr * v + r * u) - ey * (dr * u - r * v)
+        step = f / np.where(np.abs(fp) > 1e-12, fp, 1e-12)
+        ang = ang - np.clip(step, -0.5, 0.5)
+        if np.max(np.abs(step)) < 1e-13:
+            break
+    r, dr = t.radius_at(theta, ang)
+    u, v = np.cos(ang), np.sin(ang)
+    rho0 = ex * (r * u - cx) + ey * (r * v - cy)
+    nx, ny = r * u + dr * v, r * v - dr * u
+    stretch = np.hypot(nx, ny) / (ex * nx + ey * ny)
+    return rho0[None, :] + np.asarray(delta, float)[:, None] * stretch[None, :]
+
+
+# ------------------------------------------------------------------------------------------ page
+def greedy_colours(c: np.ndarray, reach: float) -> np.ndarray:
+    """Holes within `reach` of each other get different colours; holes in index order take the
+    smallest colour no neighbour holds (page.py:185-199)."""
+    from scipy.spatial import cKDTree
+
+    nbrs = cKDTree(c).query_ball_point(c, reach)
+    colour = np.full(len(c), -1, dtype=np.int64)
+    for i, nb in enumerate(nbrs):
+        used = set(colour[nb].tolist())
+        k = 0
+        while k in used:
+            k += 1
+        colour[i] = k
+    return colour
+
+
+class OutlinePage:
+    """One page through the outline fit (PageFit, page.py:78-937): a free level per hole, the
+    far light on, hole state FP64 on the device."""
+
+    def __init__(self, image, seeds, template: OutlineTemplate, *, chunk=2000, far_ramp=4.0, far_box=128,
+                 far_upsample=2, start_sigma=1.0, device=None):
+        from scipy.spatial import cKDTree
+
+        dev = template.device if device is None else device
+        self.device = dev
+        img = torch.as_tensor(np.asarray(image, dtype=np.float64) if not isinstance(image, torch.Tensor) else image)
+        self.img = img.to(device=dev, dtype=torch.float32).contiguous()
+        self.h, self.w = (int(v) for v in self.img.shape)
+        c = np.array(seeds, dtype=np.float64, copy=True).reshape(-1, 2)
+        self.n = len(c)
+        self.template = template
+        self.chunk = int(chunk)
+        d, _ = cKDTree(c).query(c, k=2)
+        self.spacing = float(np.median(d[:, 1]))
+        self.R = max(3, int(math.ceil(0.75 * self.spacing)))
+        colour = greedy_colours(c, 1.5 * self.spacing)
+        chunks = []
+        for k in range(int(colour.max()) + 1):
+            idx = np.flatnonzero(colour == k)
+            chunks += [idx[s:s + self.chunk] for s in range(0, len(idx), self.chunk)]
+        self.n_colours = int(colour.max()) + 1
+        self.order = torch.from_numpy(np.concatenate(chunks).astype(np.int32)).to(dev)
+        self.starts = np.cumsum([0] + [len(x) for x in chunks]).astype(np.int32)
+        self.all_idx = torch.arange(self.n, dtype=torch.int32, device=dev)
+        self.far_ramp, self.far_box, self.far_upsample = float(far_ramp), int(far_box), int(far_upsample)
+        B = self.far_box
+        out_y = max(0.0, -c[:, 1].min(), c[:, 1].max() - (self.h - 1))
+        out_x = max(0.0, -c[:, 0].min(), c[:, 0].max() - (self.w - 1))
+        self.my, self.mx = B // 2 + int(math.ceil(out_y)) + 4, B // 2 + int(math.ceil(out_x)) + 4
+        self.hp, self.wp = _even_fast(self.h + 2 * self.my), _even_fast(self.w + 2 * self.mx)
+        self.c = torch.from_numpy(c).to(dev)
+        z = lambda: torch.zeros(self.n, dtype=torch.float64, device=dev)  # noqa: E731
+        self.a, self.level, self.delta = z(), z(), z()
+        self.delta_sd = torch.full((self.n,), float("nan"), dtype=torch.float64, device=dev)
+        self.size_on, self.clip_px, self.var_line, self.rms_typ = True, None, (1.0, 0.0), None
+        self.theta, self.start_sigma, self.history, self.converged = None, float(start_sigma), [], False
+        self.log, self.timings = [], {}
+        self._far_template = None
+
+    # ---------------------------------------------------------------------------------- gathers
+    def windows(self, idx=None, R=None, *, page=None, level=True, outputs=("valid", "data", "model"),
+                radius_fine=0.0):
+        """Per-window values of the holes idx (device int32; all by default), (n, (2R+1)^2)."""
+        idx = self.all_idx if idx is None else idx
+        R = self.R if R is None else int(R)
+        n, P = int(idx.numel()), (2 * R + 1) ** 2
+        dev = self.device
+        out = {}
+        for name in outputs:
+            dt = {"valid": torch.uint8, "bin": torch.int32}.get(name, torch.float64)
+            out[name] = torch.empty((n, P), dtype=dt, device=dev)
+        t = self.template
+        tm = self.tmpl if ("own" in out or "v0" in out) else None
+        _lib.call("spx_outline_windows", _ptr(self.img), self.h, self.w, self.w, _ptr(page), _ptr(tm), t.N, t.U, R,
+                  _ptr(idx), n, _ptr(self.c), _ptr(self.a), _ptr(self.level) if level else None, _ptr(self.delta),
+                  float(radius_fine), _ptr(out.get("bin")), _ptr(out.get("valid")), _ptr(out.get("data")),
+                  _ptr(out.get("model")), _ptr(out.get("own")), _ptr(out.get("v0")), stream_handle())
+        if "valid" in out:
+            out["valid"] = out["valid"].bool()
+        return out
+
+    def _ramp(self, r, r0):
+        s = torch.clamp((r - r0) / self.far_ramp, 0.0, 1.0)
+        return 0.5 * (1.0 + torch.cos(math.pi * s))
+
+    # --------------------------------------------------------------------------------- template
+    def set_theta(self, theta):
+        """Templates of theta: near parts (inside r0, cosine ramp over far_ramp) for the per-hole
+        render, far kernels for the FFT light, box doubling (page.py:301-369)."""
+        self.theta = np.asarray(theta, float)
+        t = self.template
+        t0, td, t2 = t.render_quad(self.theta)
+        self.rms_typ = None
+        prof = t0[t.N // 2].abs().cpu().numpy()
+        above = np.flatnonzero(prof > 1e-3 * prof.max())
+        reach = (above[-1] - above[0] + 1) / (2 * t.U)
+        cap = t.N // (2 * t.U) - 2
+        if reach > cap and t.N // t.U < 96:
+            self.log.append(f"  template box {t.N // t.U} -> {2 * (t.N // t.U)} px: the hole's light reaches "
+                            f"{reach:.1f} px, past the {cap} px the box can render")
+            self.template = OutlineTemplate(upsample=t.U, size_px=2 * (t.N // t.U), harmonics=t.ks, blur=t.blur,
+                                            aa=t.aa, device=self.device)
+            return self.set_theta(theta)
+        r0 = float(min(int(math.ceil(reach)) + 1, cap, self.R + 3))
+        self.t0_full = t0
+        self.far_r0 = r0
+        wn = self._ramp(torch.hypot(t.xx_d, t.yy_d), r0)
+        t0n, tdn, t2n = t0 * wn, td * wn, t2 * wn
+        tx, ty = t.derivatives_xy(t0n)
+        tm = torch.zeros((t.N, t.N, 8), dtype=torch.float32, device=self.device)
+        for k, g in enumerate((t0n, tdn, t2n, tx, ty)):
+            tm[..., k] = g.to(torch.float32)
+        self.tmpl = tm.contiguous()
+        self.Rr = min(int(math.ceil(r0 + self.far_ramp)) + 1, cap)
+        if self._far_template is None:
+            self._far_template = OutlineTemplate(upsample=self.far_upsample, size_px=self.far_box, harmonics=t.ks,
+                                                 blur=t.blur, aa=t.aa, device=self.device)
+        ft, B = self._far_template, self.far_box
+        k = torch.arange(B, device=self.device) * ft.U
+        off = torch.arange(B, device=self.device, dtype=torch.float64) - B // 2
+        wf = 1.0 - self._ramp(torch.hypot(off[None, :], off[:, None]), r0)
+        self.far_fft = []
+        for g in ft.render_quad(self.theta):
+            pad = torch.zeros((self.hp, self.wp), dtype=torch.float64, device=self.device)
+            pad[:B, :B] = g[k][:, k] * wf
+            self.far_fft.append(torch.fft.rfft2(pad))
+
+    def far_field(self) -> torch.Tensor:
+        """The holes' far light (H, W): a, a d, a d^2 splatted and convolved with the far kernels."""
+        grid = torch.zeros((3, self.hp, self.wp), dtype=torch.float64, device=self.device)
+        _lib.call("spx_outline_far_splat", _ptr(self.c), _ptr(self.a), _ptr(self.delta), self.n, self.mx, self.my,
+                  self.hp, self.wp, _ptr(grid), stream_handle())
+        spec = sum(torch.fft.rfft2(grid[k]) * self.far_fft[k] for k in range(3))
+        far = torch.fft.irfft2(spec, s=(self.hp, self.wp))
+        oy, ox = self.my + self.far_box // 2, self.mx + self.far_box // 2
+        return far[oy:oy + self.h, ox:ox + self.w]
+
+    def _render_near(self, page, order, cache, c_old):
+        t = self.template
+        rl2 = float(4 * (self.Rr + 1) ** 2)  # the whole square window
+        _lib.call("spx_hole_render", _ptr(page), self.h, self.w, _ptr(cache), _ptr(self.tmpl), t.N, t.U, self.Rr,
+                  rl2, _ptr(order), int(order.numel()), _ptr(c_old), _ptr(self.c), _ptr(self.a), _ptr(self.delta),
+                  stream_handle())
+
+    def render_page(self) -> torch.Tensor:
+        """Every hole's near light over its (2Rr+1)^2 window plus the far light (H, W) FP64."""
+        page = torch.zeros(self.h * self.w, dtype=torch.float64, device=self.device)
+        cache = torch.zeros(self.n * (2 * self.Rr + 1) ** 2, dtype=torch.float32, device=self.device)
+        self._render_near(page, self.all_idx, cache, self.c.clone())
+        return page.reshape(self.h, self.w) + self.far_field()
+
+    # ------------------------------------------------------------------------------------ start
+    def plain_stack(self) -> np.ndarray:
+        t = self.template
+        win = self.windows(outputs=("data", "bin"))
+        b = win["bin"].reshape(-1).long()
+        ok = b >= 0
+        nn = t.N * t.N
+        sums = torch.zeros(nn, dtype=torch.float64, device=self.device).index_add_(0, b[ok], win["data"].reshape(-1)[ok])
+        cnt = torch.zeros(nn, dtype=torch.float64, device=self.device).index_add_(
+            0, b[ok], torch.ones_like(win["data"].reshape(-1)[ok]))
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return (sums.cpu().numpy() / cnt.cpu().numpy()).reshape(t.N, t.N)
+
+    def initialise(self):
+        """Round start from the plain stack's half height; brightness and level by a 2 x 2 least
+        squares per hole, twice, the second against the others' light (page.py:420-486)."""
+        if self.theta is None:
+            stack = self.plain_stack()
+            mid = stack.shape[0] // 2
+            prof = stack[mid]
+            with np.errstate(invalid="ignore"):
+                base = np.nanmin(prof)
+            half = base + 0.5 * (prof[mid] - base)
+            lft = rgt = mid
+            while lft > 0 and np.isfinite(prof[lft - 1]) and prof[lft - 1] >= half:
+                lft -= 1
+            while rgt + 1 < len(prof) and np.isfinite(prof[rgt + 1]) and prof[rgt + 1] >= half:
+                rgt += 1
+            fw = (rgt - lft + 1) / self.template.U
+            hw = max(0.5 * math.sqrt(max(fw ** 2 - (2.355 * self.start_sigma) ** 2, 0.25)), 0.6)
+            self.set_theta(self.template.start(hw, self.start_sigma))
+        page = torch.zeros((self.h, self.w), dtype=torch.float64, device=self.device)
+        for _ in range(2):
+            win = self.windows(page=page, level=False, outputs=("valid", "data", "model", "v0"))
+            wv = win["valid"].to(torch.float64)
+            v0 = win["v0"]
+            y = torch.where(win["valid"], win["data"] - (win["model"] - self.a[:, None] * v0), 0.0)
+            s11, s12, s22 = (wv * v0 * v0).sum(1), (wv * v0).sum(1), wv.sum(1)
+            r1, r2 = (wv * v0 * y).sum(1), (wv * y).sum(1)
+            det = s11 * s22 - s12 * s12
+            self.a = torch.clamp((r1 * s22 - r2 * s12) / det, min=1e-3)
+            self.level = (s11 * r2 - s12 * r1) / det
+            page = self.render_page()
+
+    # ------------------------------------------------------------------------------- noise line
+    def fit_noise(self, page):
+        """Variance of the leftovers against the model level as a straight line, from the first
+        three chunks of holes (page.py:507-535)."""
+        n3 = min(self.n, 3 * self.chunk)
+        win = self.windows(self.all_idx[:n3], page=page, outputs=("valid", "data", "model"))
+        ok = win["valid"]
+        res = (win["data"] - win["model"])[ok].cpu().numpy()
+        lev = win["model"][ok].cpu().numpy()
+        edges = np.percentile(lev, np.linspace(0, 100, 13))
+        xs, ys = [], []
+        for lo, hi in zip(edges[:-1], edges[1:]):
+            s = (lev >= lo) & (lev < hi)
+            if s.sum() > 200:
+                r = res[s]
+                xs.append(lev[s].mean())
+                ys.append((1.4826 * np.median(np.abs(r - np.median(r)))) ** 2)
+        xs, ys = np.asarray(xs), np.asarray(ys)
+        if len(xs) >= 3:
+            v1, v0 = np.polyfit(xs, ys, 1)
+            v1, v0 = max(v1, 0.0), max(v0, 0.05 * ys.mean())
+        else:
+            v0, v1 = max(float(ys.mean()) if len(ys) else 1e-3, 1e-3), 0.0
+        self.var_line = (float(v0), float(v1))
+
+    def _opt(self, **kw):
+        v0, v1 = self.var_line
+        base = dict(nv0=v0, nv1=v1, rscale=0.0, max_move=0.25, max_delta=0.0, a_med=1.0, relax=1.0, size=0,
+                    reserved=0)
+        base.update(kw)
+        return _lib.SpxHoleStepOptions(**base)
+
+    def delta_information(self):
+        """Per-hole standard error of the edge offset from the normal matrix at the current state
+        (no step, no robust weights); (median, outline radius) (page.py:553-585)."""
+        t = self.template
+        page = self.render_page().reshape(-1).contiguous()
+        stat = torch.zeros((self.n, 4), dtype=torch.float64, device=self.device)
+        c_old = torch.empty((self.n, 2), dtype=torch.float64, device=self.device)
+        opt = self._opt(max_move=0.0, max_delta=1e300, a_med=1e300, relax=0.0, size=1)
+        _lib.call("spx_hole_step", _ptr(self.img), self.h, self.w, self.w, _ptr(page), _ptr(self.tmpl), t.N, t.U,
+                  self.R, _ptr(self.all_idx), self.n, _ptr(self.c), _ptr(self.a), _ptr(self.level),
+                  _ptr(self.delta), _ptr(c_old), ctypes.byref(opt), _ptr(stat), stream_handle())
+        self.delta_sd = stat[:, 3].clone()
+        return float(np.nanmedian(self.delta_sd.cpu().numpy())), 0.5 * t.cd_px(self.theta)
+
+    # ------------------------------------------------------------------------------- hole stage
+    def _sweep_stats(self, stat, size):
+        """(median leftover rms, 99th-percentile centre move, 99th-percentile offset move) by
+        NumPy's rules, from exact order statistics selected on the device."""
+        n = self.n
+        pos = 0.99 * (n - 1)
+        lo, hi = int(math.floor(pos)), int(math.ceil(pos))
+        frac = pos - lo
+        out = torch.empty(6, dtype=torch.float64, device=self.device)
+        cols = (ctypes.c_int64 * 6)(2, 2, 0, 0, 1, 1)
+        ks = (ctypes.c_int64 * 6)((n - 1) // 2, n // 2, lo, hi, lo, hi)
+        _lib.call("spx_select_kth", _ptr(stat), n, 4, cols, ks, 6, _ptr(out), stream_handle())
+        v = out.cpu().tolist()
+
+        def lerp(a, b):
+            d = b - a
+            return b - d * (1.0 - frac) if frac >= 0.5 else a + d * frac
+
+        return 0.5 * (v[0] + v[1]), lerp(v[2], v[3]), lerp(v[4], v[5]) if size else 0.0
+
+    def fit_holes(self, iterations=5, size=True, max_move=0.25, stop_px=0.002, robust_c=3.0):
+        """Block Gauss-Seidel over the colour chunks (page.py:587-729): every chunk's step against
+        the complete page model, its light re-rendered before the next chunk; the far light
+        refreshed after every sweep; stops once 99 % of the holes move less than stop_px."""
+        t = self.template
+        max_delta = 0.3 * 0.5 * t.cd_px(self.theta)
+        self.clip_px = float(max_delta)
+        dev, n = self.device, self.n
+        P = (2 * self.Rr + 1) ** 2
+        page = torch.zeros(self.h * self.w, dtype=torch.float64, device=dev)
+        cache = torch.zeros(n * P, dtype=torch.float32, device=dev)
+        c_old = self.c[self.order.long()].contiguous()
+        self._render_near(page, self.order, cache, c_old)
+        far = self.far_field().reshape(-1)
+        page += far
+        if not self.history:
+            self.fit_noise(page.reshape(self.h, self.w))
+        a_med = _np_median(self.a)
+        stat = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+        opt = self._opt(max_move=float(max_move), max_delta=float(max_delta), a_med=a_med, size=int(bool(size)))
+        rl2 = float(4 * (self.Rr + 1) ** 2)
+        starts = self.starts
+        nchunk = len(starts) - 1
+        sweeps = 0
+        for _ in range(int(iterations)):
+            sweeps += 1
+            if self.rms_typ is None:
+                # the first sweep with a new template: each chunk's robust scale is its own median
+                # leftover on the page as it stands when the chunk's turn comes
+                for j in range(nchunk):
+                    b, m = int(starts[j]), int(starts[j + 1] - starts[j])
+                    if m == 0:
+                        continue
+                    idx = self.order[b:b + m]
+                    if robust_c:
+                        win = self.windows(idx, page=page, outputs=("valid", "data", "model"))
+                        res = torch.where(win["valid"], win["data"] - win["model"], 0.0)
+                        cnt = win["valid"].sum(1).clamp(min=1).to(torch.float64)
+                        s = _np_median(torch.sqrt((res * res).sum(1) / cnt))
+                        opt.rscale = float(robust_c) * max(s, 1e-6)
+                    else:
+                        opt.rscale = 0.0
+                    _lib.call("spx_hole_step", _ptr(self.img), self.h, self.w, self.w, _ptr(page), _ptr(self.tmpl),
+                              t.N, t.U, self.R, _ptr(idx), m, _ptr(self.c), _ptr(self.a), _ptr(self.level),
+                              _ptr(self.delta), _off(c_old, 2 * b), ctypes.byref(opt), _off(stat, 4 * b),
+                              stream_handle())
+                    _lib.call("spx_hole_render", _ptr(page), self.h, self.w, _off(cache, P * b), _ptr(self.tmpl), t.N,
+                              t.U, self.Rr, rl2, _ptr(idx), m, _off(c_old, 2 * b), _ptr(self.c), _ptr(self.a),
+                              _ptr(self.delta), stream_handle())
+            else:
+                opt.rscale = float(robust_c) * max(self.rms_typ, 1e-6) if robust_c else 0.0
+                start = (ctypes.c_int32 * len(starts))(*starts.tolist())
+                _lib.call("spx_hole_sweep", _ptr(self.img), self.h, self.w, self.w, _ptr(page), _ptr(self.tmpl), t.N,
+                          t.U, self.R, self.Rr, rl2, _ptr(self.order), start, nchunk, _ptr(self.c), _ptr(self.a),
+                          _ptr(self.level), _ptr(self.delta), _ptr(c_old), _ptr(cache), ctypes.byref(opt),
+                          _ptr(stat), stream_handle())
+            new_far = self.far_field().reshape(-1)
+            page += new_far - far
+            far = new_far
+            if size:
+                self.delta_sd[self.order.long()] = stat[:, 3]
+            self.rms_typ, m99, d99 = self._sweep_stats(stat, size)
+            self.log.append(f"    holes it {sweeps}: 99% move {m99:.4f} px, delta 99% {d99:.4f}, typical leftover "
+                            f"rms {self.rms_typ:.2f}")
+            if m99 < stop_px and d99 < stop_px:
+                break
+        self.sweeps_run = getattr(self, "sweeps_run", 0) + sweeps
+        return page.reshape(self.h, self.w)
+
+    # ------------------------------------------------------------------------------ shape stage
+    def inside(self) -> np.ndarray:
+        c = self.c.cpu().numpy()
+        return ((c[:, 0] >= self.R + 1) & (c[:, 0] <= self.w - 2 - self.R) & (c[:, 1] >= self.R + 1)
+                & (c[:, 1] <= self.h - 2 - self.R))
+
+    def good_holes(self, page):
+        """Window inside the page, usual brightness and edge offset, usual leftover (page.py:784-803)."""
+        inside = self.inside()
+        a, delta = self.a.cpu().numpy(), self.delta.cpu().numpy()
+        med = np.median(a[inside]) if inside.any() else np.median(a)
+        dmed = np.median(delta[inside] if inside.any() else delta)
+        usual = (a > 0.4 * med) & (a < 2.5 * med) & (np.abs(delta - dmed) < 0.6)
+        win = self.windows(page=page, outputs=("valid", "data", "model"))
+        res = torch.where(win["valid"], win["data"] - win["model"], 0.0)
+        cnt = win["valid"].sum(1).clamp(min=1).to(torch.float64)
+        rms = torch.sqrt((res * res).sum(1) / cnt).cpu().numpy()
+        typical = np.median(rms[inside]) if inside.any() else np.median(rms)
+        return inside & usual & (rms < 2.0 * typical), rms
+
+    def cleaned_stack(self, page, good, radius_px=None):
+        """(data - page + own - level) / a binned at each pixel's fine offset from its hole,
+        weighted by a^2 / variance; (stack, weight) (N, N) host arrays (page.py:732-766)."""
+        t = self.template
+        win = self.windows(page=page, outputs=("data", "model", "own", "bin"),
+                           radius_fine=0.0 if radius_px is None else float(radius_px) * t.U)
+        a = self.a[:, None]
+        v0, v1 = self.var_line
+        cleaned = (win["data"] - win["model"] + win["own"]) / a
+        wgt = a * a / (v0 + v1 * torch.clamp(win["model"], min=0.0))
+        g = torch.from_numpy(np.asarray(good, dtype=bool)).to(self.device)
+        sel = (win["bin"] >= 0) & g[:, None]
+        b = win["bin"][sel].long()
+        nn = t.N * t.N
+        sums = torch.zeros(nn, dtype=torch.float64, device=self.device).index_add_(0, b, (cleaned * wgt)[sel])
+        wsum = torch.zeros(nn, dtype=torch.float64, device=self.device).index_add_(0, b, wgt[sel])
+        sums, wsum = sums.cpu().numpy(), wsum.cpu().numpy()
+        with np.errstate(invalid="ignore", divide="ignore"):
+            stack = np.where(wsum > 0, sums / wsum, np.nan)
+        return stack.reshape(t.N, t.N), wsum.reshape(t.N, t.N)
+
+    def fit_shape(self, stack, weight, stages):
+        t = self.template
+        ok = np.isfinite(stack) & (weight > 0)
+        okd = torch.from_numpy(ok).to(self.device)
+        y = torch.from_numpy(stack[ok]).to(self.device)
+        sw = torch.from_numpy(np.sqrt(weight[ok])).to(self.device)
+        return fit_stages(t, lambda th: t.render(th, box_px=1.0 / t.U)[okd], y, sw, self.theta, stages=stages,
+                          log=self.log.append)
+
+    # -------------------------------------------------------------------------------------- run
+    def run(self, rounds=14, iterations=5, log=None, relax=0.5, tol_px=0.08 / 21.0, release_px=0.3 / 21.0,
+            per_hole_size="auto"):
+        """Alternate the hole and shape stages (page.py:806-896)."""
+        started = time.perf_counter()
+        say = log if log is not None else (lambda s: None)
+        self.initialise()
+        all_stages = ("round", "super", "full")
+        level, chi2_prev = 0, None
+        self.size_on = per_hole_size is True
+        for rnd in range(int(rounds)):
+            t_round = time.perf_counter()
+            first = rnd == 0
+            if rnd == 1 and per_hole_size == "auto":
+                sd, radius = self.delta_information()
+                self.size_on = sd < 0.3 * radius
+                say(f"  per-hole edge offset: median standard error {sd:.3f} px against the outline radius "
+                    f"{radius:.2f} px -> per-hole size {'on' if self.size_on else 'off'}")
+            page = self.fit_holes(iterations=iterations + 3 if rnd < 3 else max(3, iterations // 2 + 1),
+                                  size=self.size_on and not first)
+            self.fit_noise(page)
+            good, rms = self.good_holes(page)
+            stack, weight = self.cleaned_stack(page, good, radius_px=0.55 * self.spacing if first else None)
+            stages = all_stages[:level + 1]
+            proposal, scale, const, chi2 = self.fit_shape(stack, weight, stages)
+            t = self.template
+            change_px = t.cd_px(proposal) - t.cd_px(self.theta)
+            kappa = 1.0 if first else relax
+            self.a = self.a * scale ** kappa
+            self.set_theta(self.theta + kappa * (proposal - self.theta))
+            t = self.template
+            self.history.append(dict(round=rnd + 1, stages=list(stages), outline_cd_px=t.cd_px(self.theta), chi2=chi2,
+                                     proposal_change_px=change_px, const=const, good=int(np.sum(good)),
+                                     var_line=self.var_line, mean_delta_px=float(self.delta.mean()),
+                                     seconds=time.perf_counter() - t_round))
+            say(f"  round {rnd + 1} ({stages[-1]}): outline CD {t.cd_px(self.theta):.4f} px, proposal "
+                f"{change_px:+.4f} px, stack chi2 {chi2:.3f}, good {int(np.sum(good))}")
+            for line in self.log:
+                say(line)
+            self.log = []
+            settled = chi2_prev is not None and abs(chi2 - chi2_prev) < 0.05 * chi2_prev
+            chi2_prev = chi2
+            if not first and settled and abs(change_px) < release_px and level < len(all_stages) - 1:
+                level += 1
+                chi2_prev = None
+                say(f"  {all_stages[level]} stage released")
+                continue
+            if level == len(all_stages) - 1 and settled and abs(change_px) < tol_px:
+                self.converged = True
+                say(f"  converged after round {rnd + 1}")
+                break
+        page = self.fit_holes(iterations=2 * iterations, size=self.size_on)
+        if not self.size_on:
+            self.delta_information()
+        good, rms = self.good_holes(page)
+        stack, weight = self.cleaned_stack(page, good)
+        self.timings["total"] = time.perf_counter() - started
+        out = self.results(good, rms)
+        t = self.template
+        out.update(stack=stack, stack_weight=weight, template=self.t0_full.cpu().numpy(),
+                   template_box=t.render(self.theta, box_px=1.0 / t.U).cpu().numpy())
+        return out
+
+    def results(self, good, rms):
+        """Per-hole results (page.py:898-937): centres moved to the outline's centroid, edge
+        offsets from the good holes' mean, CD of the offset outline from its exact area."""
+        t = self.template
+        cx0, cy0 = t.centroid_px(self.theta)
+        g = np.asarray(good, dtype=bool)
+        delta_raw = self.delta.cpu().numpy()
+        delta_mean = float(np.mean(delta_raw[g])) if g.any() else 0.0
+        delta = delta_raw - delta_mean
+        r, dr = t.radius_at(self.theta, t.angles)
+        gr = np.hypot(r, dr) / r
+        area0, perim, quad = math.pi * np.mean(r * r), 2 * math.pi * np.mean(r * gr), math.pi * np.mean(gr * gr)
+        area = area0 + perim * delta + quad * delta ** 2
+        delta_sd = self.delta_sd.cpu().numpy()
+        return dict(
+            centers_xy=self.c.cpu().numpy() + np.array([cx0, cy0]), brightness=self.a.cpu().numpy(),
+            level=self.level.cpu().numpy(), edge_offset_px=delta,
+            cd_px=2.0 * np.sqrt(np.maximum(area, 1e-6) / math.pi), cd_outline_px=t.cd_px(self.theta),
+            theta=self.theta.copy(), delta_sd_px=delta_sd,
+            cd_sd_px=delta_sd * (perim + 2 * quad * delta) / np.sqrt(math.pi * np.maximum(area, 1e-6)),
+            per_hole_size=self.size_on, edge_offset_mean_px=delta_mean, clip_px=self.clip_px,
+            at_clip=np.abs(delta_raw) >= 0.99 * (self.clip_px or np.inf),
+            describe=t.describe(self.theta, 1.0), good=g, rms=rms, var_line=self.var_line, history=self.history,
+            seconds=self.timings.get("total"), spacing_px=self.spacing, window_px=self.R, converged=self.converged)
+
+
+# ------------------------------------------------------------------------------------ public API
+def fit_outline(image, seeds_xy, *, pixel_size: float = 1.0, unit: str = "px", blur: str = "gauss2",
+                rounds: int = 14, invert: bool = False, n_rays: int = 128, backend: str = "auto", device: int = 0,
+                log=None) -> OutlineFitResult:
+    """Fit one outline and one blur to the whole page and every hole's position, brightness,
+    level and size (reference subpx/outline.py:34-96, same arguments and checks). Dark holes on a
+    bright page (``invert``) are reflected as max - image. Returns NumPy arrays in pixels."""
+    img = np.asarray(image) if not isinstance(image, torch.Tensor) else image.detach().cpu().numpy()
+    if img.ndim != 2 or img.size == 0 or not np.issubdtype(img.dtype, np.number) or not np.isfinite(img).all():
+        raise ValueError("image must be a nonempty 2D array of finite intensities")
+    seeds = np.asarray(seeds_xy, dtype=np.float64).reshape(-1, 2)
+    if len(seeds) < MIN_HOLES or not np.isfinite(seeds).all():
+        raise ValueError(f"the outline fit needs at least {MIN_HOLES} finite seeds")
+    if blur not in BLUR_FAMILIES:
+        raise ValueError("blur must be one of " + ", ".join(BLUR_FAMILIES))
+    if int(rounds) != rounds or rounds < 1:
+        raise ValueError("rounds must be a positive integer")
+    if int(n_rays) != n_rays or n_rays < 16:
+        raise ValueError("n_rays must be an integer of at least 16")
+    if not (np.isfinite(pixel_size) and pixel_size > 0):
+        raise ValueError("pixel_size must be positive")
+    resolve_backend(backend)
+    signal = img.astype(np.float64)
+    top = float(signal.max())
+    if invert:
+        signal = top - signal
+    with gpu_device(device) as dev:
+        template = OutlineTemplate(upsample=UPSAMPLE, size_px=BLUR_FAMILIES[blur], blur=blur, device=dev)
+        page = OutlinePage(signal, seeds, template, device=dev)
+        res = page.run(rounds=int(rounds), iterations=5, log=log)
+        template = page.template
+        theta = np.asarray(res["theta"], dtype=float)
+        delta = np.asarray(res["edge_offset_px"], dtype=float)
+        cd_x, cd_y = ellipse_widths(template, theta, delta)
+        rays = outline_rays(template, theta, delta, int(n_rays))
+        d = res["describe"]
+    level = np.asarray(res["level"], dtype=float)
+    if invert:
+        level = top - level
+    fwhm = tuple(float(v) for v in d["blur_fwhm_nm"])
+    summary = {
+        "blur_family": blur, "rounds_max": int(rounds), "rounds_run": len(page.history),
+        "converged": bool(res["converged"]), "per_hole_size": bool(res["per_hole_size"]),
+        "outline_cd_px": float(res["cd_outline_px"]), "outline_extent_x_px": float(d["extent_x_nm"]),
+        "outline_extent_y_px": float(d["extent_y_nm"]), "superellipse_p": float(d["superellipse"]["p"]),
+        "outline_turn_deg": float(d["superellipse"]["phi_deg"]),
+        "harmonics": {k: [float(v) for v in pair] for k, pair in d["harmonics"].items()},
+        "blur_fwhm_px": list(fwhm), "blur_rms_px": [float(v) for v in d["blur_rms_nm"]],
+        "holes_fitted": int(len(seeds)), "good_holes": int(np.sum(res["good"])),
+        "spacing_px": float(res["spacing_px"]), "window_px": int(res["window_px"]),
+        "template_box_px": int(template.N // template.U), "noise_variance": [float(v) for v in res["var_line"]],
+        "edge_offset_clip_px": None if res["clip_px"] is None else float(res["clip_px"]),
+        "edge_offset_mean_px": float(res["edge_offset_mean_px"]),
+        "edge_offset_sd_median_px": float(np.nanmedian(res["delta_sd_px"])) if np.isfinite(
+            res["delta_sd_px"]).any() else float("nan"),
+        "at_clip": int(np.sum(res["at_clip"])), "seconds": float(res["seconds"]), "backend": "gpu",
+        "theta": [float(v) for v in theta], "lengths": "px",
+    }
+    return OutlineFitResult(
+        centers_xy=np.asarray(res["centers_xy"], dtype=float), cd_px=np.asarray(res["cd_px"], dtype=float),
+        cd_x_px=cd_x, cd_y_px=cd_y, cd_se_px=np.asarray(res["cd_sd_px"], dtype=float), edge_offset_px=delta,
+        brightness=np.asarray(res["brightness"], dtype=float), level=level, rays_px=rays,
+        at_clip=np.asarray(res["at_clip"], dtype=bool), good=np.asarray(res["good"], dtype=bool),
+        rms=np.asarray(res["rms"], dtype=float), outline_cd_px=float(res["cd_outline_px"]), blur_fwhm_px=fwhm,
+        per_hole_size=bool(res["per_hole_size"]), converged=bool(res["converged"]), rounds=len(page.history),
+        blur=blur, theta=theta, template=np.asarray(res["template"], dtype=float),
+        stack=np.asarray(res["stack"], dtype=float), upsample=int(template.U), summary=summary,
+        pixel_size=float(pixel_size), unit=str(unit),
+    )

@@ -59,3 +59,42 @@ class ErfFitResult:
     def cd(self) -> np.ndarray:
         """Hole diameter along x (2 r)."""
         return 2.0 * self.params[:, 2].astype(np.float64)
+
+
+@dataclass
+class OutlineFitResult:
+    """Outline fit of one page (``fit_outline``; reference subpx/types.py:156-201): one sharp
+    outline and one blur for the page, every hole at its own position, brightness, local level
+    and edge offset. Per hole, in seed order: ``centers_xy``, ``cd_px`` (area-equivalent diameter
+    of the hole's own outline), ``cd_x_px`` / ``cd_y_px`` (x / y diameters of the equal-area
+    ellipse with that outline's spread), ``cd_se_px``, ``edge_offset_px`` (from the good holes'
+    mean), ``brightness``, ``level`` (image grey levels), ``rays_px`` (N, n), ``at_clip``, ``good``,
+    ``rms``. Per page: ``outline_cd_px``, ``blur_fwhm_px``, ``per_hole_size``, ``converged``,
+    ``rounds``, ``blur``, ``theta``, ``template`` and ``stack`` (fine grid, ``upsample`` bins per
+    pixel), ``summary``. NumPy arrays, lengths in pixels."""
+
+    centers_xy: np.ndarray
+    cd_px: np.ndarray
+    cd_x_px: np.ndarray
+    cd_y_px: np.ndarray
+    cd_se_px: np.ndarray
+    edge_offset_px: np.ndarray
+    brightness: np.ndarray
+    level: np.ndarray
+    rays_px: np.ndarray
+    at_clip: np.ndarray
+    good: np.ndarray
+    rms: np.ndarray
+    outline_cd_px: float
+    blur_fwhm_px: tuple[float, float]
+    per_hole_size: bool
+    converged: bool
+    rounds: int
+    blur: str
+    theta: np.ndarray
+    template: np.ndarray
+    stack: np.ndarray
+    upsample: int
+    summary: dict[str, Any] = field(default_factory=dict)
+    pixel_size: float = 1.0
+    unit: str = "px"

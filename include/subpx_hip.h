@@ -396,6 +396,36 @@ int spx_select_kth(const double* x, int64_t n, int64_t stride, const int64_t* co
 int spx_far_splat(const double* c, const double* a, const double* delta, double cd0, int32_t n, int32_t mx,
                   int32_t my, int32_t hp, int32_t wp, float* grid, spx_stream_t stream);
 
+/* ---- page outline fit (fit_outline: subpx/outline.py:34-170, subpx/_outline/page.py) ----
+ * The template T = pixel box * blur * coverage(outline) lives on an N x N fine grid, U points
+ * per pixel; the page model and the hole state are those of the hole stage above. */
+
+/* Pixel coverage of the outline (model.py:149-159): out (N, N) FP64, out[iy][ix] = mean over
+ * the aa x aa sub-samples s of clamp(0.5 + (edge[ang[s]] - rho[s]) * scale, 0, 1), with the
+ * sub-samples' radius rho and angle-table index ang (N aa x N aa, row-major) precomputed and
+ * edge the outline's radius on the angle table (scale = U aa: a one-sample ramp). */
+int spx_outline_coverage(const double* edge, const double* rho, const int32_t* ang, int32_t N, int32_t aa,
+                         double scale, double* out, spx_stream_t stream);
+
+/* Per-window gather of the holes idx[0..n) (page.py:206-216, 294-299, 420-540, 732-766): for
+ * window pixel q of hole i (row-major over (2R+1)^2 offsets around rint(c)), at t = i P + q:
+ * valid[t] (inside the frame), data[t] (grey level at the clamped pixel), model[t] = page +
+ * level (page / level NULL: 0), v0[t] = T0 and own[t] = a (T0 + d T1 + d^2 T2) sampled from
+ * tmpl (the hole stage's 8-float layout) at the pixel's fine offset, bin[t] = rint(sy) N +
+ * rint(sx) of that offset when valid, inside the grid and (radius_fine > 0) within radius_fine
+ * fine units of the centre, else -1. Any output may be NULL. */
+int spx_outline_windows(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,
+                        const float* tmpl, int32_t N, int32_t U, int32_t R, const int32_t* idx, int32_t n,
+                        const double* c, const double* a, const double* level, const double* delta,
+                        double radius_fine, int32_t* bin, uint8_t* valid, double* data, double* model, double* own,
+                        double* v0, spx_stream_t stream);
+
+/* Keys splat of a_i, a_i d_i and a_i d_i^2 at c_i + (mx, my) on three padded FP64 grids
+ * (grid + k hp wp, k = 0, 1, 2; holes whose taps leave the grid are dropped): the far light is
+ * the sum of the three convolved with the far kernels of T0, T1 and T2 (page.py:341-397). */
+int spx_outline_far_splat(const double* c, const double* a, const double* delta, int32_t n, int32_t mx,
+                          int32_t my, int32_t hp, int32_t wp, double* grid, spx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,239 @@
+"""Page outline fit on the GPU (imageprocessor_new_amd.outline) against the FP64 oracle
+(oracle/outline.py) on the same inputs.
+
+Tolerances: templates are FP64 on both sides (coverage sums in another order, rocFFT against
+NumPy's FFT: 1e-12 of the peak); window gathers sample the FP32 copy of the templates (2e-6 of
+the light); far light 1e-9 of its peak. The whole fit uses FP32 templates and FP32 normal-matrix
+sums in the hole stage against the oracle's FP64 ones, and the same SciPy solver for the shape
+stage; after three rounds the outline CD agrees to 1e-4 px, hole CDs to 2e-3 px and centres to
+5e-4 px (the reference's own GPU-against-CPU check of fit_outline allows 1e-4, 1e-3 and 2e-4 px
+after four rounds, tests/test_outline_fit.py:139-156)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import outline as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _theta(t, a, b, p=2.0, phi=0.0, coeffs=None, **blur):
+    coeffs = np.zeros((len(t.ks), 2)) if coeffs is None else np.asarray(coeffs, float)
+    return t.pack(a, b, p, phi, coeffs, t.blur_pack(**(blur or dict(s1x=1.0, s1y=1.0))))
+
+
+def _lattice_page(seed=5, noise=4.0):
+    rng = np.random.default_rng(seed)
+    t = O.Template(upsample=8, size_px=24)
+    th = t.pack(2.4, 2.2, 2.6, 0.1, [[0.03, -0.02], [0.01, 0.0]],
+                t.blur_pack(s1x=1.2, s1y=1.1, s2x=2.5, s2y=2.5, w2=0.08))
+    X, Y = np.meshgrid(np.arange(6, 170, 10.0), np.arange(6, 120, 10.0))
+    c = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.4, 0.4, (X.size, 2))
+    a = rng.uniform(900, 1100, len(c))
+    de = rng.normal(0, 0.08, len(c))
+    page = O.render_model_page(t, th, c, a, 100.0, de, (120, 170), noise=noise, seed=seed)
+    page = page.astype(np.float32).astype(np.float64)  # the device holds the page in FP32
+    return dict(t=t, theta=th, c=c, a=a, delta=de, page=page, seeds=c + rng.uniform(-0.5, 0.5, c.shape))
+
+
+@pytest.mark.parametrize("blur", ["gauss2", "lens"])
+def test_template_matches_oracle(blur):
+    from imageprocessor_new_amd import outline as G
+    from imageprocessor_new_amd.backends import gpu_device
+
+    with gpu_device(0) as dev:
+        size = 24 if blur == "gauss2" else 48
+        tg = G.OutlineTemplate(upsample=8, size_px=size, blur=blur, device=dev)
+        to = O.Template(upsample=8, size_px=size, blur=blur)
+        kw = dict(s1x=1.2, s1y=0.9, s2x=2.3, s2y=2.1, w2=0.07) if blur == "gauss2" else dict(sx=1.0, sy=0.8,
+                                                                                          cutoff=0.6)
+        th = _theta(to, 3.1, 2.4, p=3.0, phi=0.25, coeffs=[[0.05, -0.03], [0.02, 0.01]], **kw)
+        ind = tg.indicator(th).cpu().numpy()
+        assert np.abs(ind - to.indicator(th)).max() < 1e-12
+        ind = tg.indicator(th, 0.17).cpu().numpy()
+        assert np.abs(ind - to.indicator(th, 0.17)).max() < 1e-12
+        for mine, ref in zip(tg.render_quad(th), to.render_quad(th)):
+            assert np.abs(mine.cpu().numpy() - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
+        t0 = to.render(th, box_px=1 / 8)
+        assert np.abs(tg.render(th, box_px=1 / 8).cpu().numpy() - t0).max() < 1e-12
+        dx, dy = tg.derivatives_xy(torch.from_numpy(t0).to(dev))
+        rx, ry = to.derivatives_xy(t0)
+        assert np.abs(dx.cpu().numpy() - rx).max() < 1e-11 and np.abs(dy.cpu().numpy() - ry).max() < 1e-11
+        assert tg.centroid_px(th) == pytest.approx(to.centroid_px(th), abs=1e-12)
+        assert tg.blur_fwhm_px(th) == pytest.approx(to.blur_fwhm_px(th), abs=1e-9)
+        assert tg.cd_px(th, 0.1) == pytest.approx(to.cd_px(th, 0.1), rel=1e-12)
+        np.testing.assert_allclose(G.ellipse_widths(tg, th, np.array([0.0, 0.2])),
+                                   O.ellipse_widths(to, th, np.array([0.0, 0.2])), rtol=1e-12)
+        np.testing.assert_allclose(G.outline_rays(tg, th, np.array([0.0, 0.1]), 32),
+                                   O.outline_rays(to, th, np.array([0.0, 0.1]), 32), atol=1e-9)
+
+
+def _page_pair(K, theta=None):
+    """Product and oracle pages at the same state: theta, brightness, level, edge offsets."""
+    from imageprocessor_new_amd import outline as G
+
+    rng = np.random.default_rng(9)
+    to = O.Template(upsample=8, size_px=24)
+    tg = G.OutlineTemplate(upsample=8, size_px=24, device=torch.device("cuda", 0))
+    po = O.Page(K["page"], K["seeds"], to)
+    pg = G.OutlinePage(K["page"], K["seeds"], tg)
+    th = K["theta"] if theta is None else theta
+    n = len(K["seeds"])
+    a = rng.uniform(800, 1200, n)
+    lev = rng.uniform(90, 110, n)
+    de = rng.normal(0, 0.05, n)
+    po.set_theta(th)
+    pg.set_theta(th)
+    po.a, po.level, po.delta = a.copy(), lev.copy(), de.copy()
+    pg.a = torch.from_numpy(a.copy()).cuda()
+    pg.level = torch.from_numpy(lev.copy()).cuda()
+    pg.delta = torch.from_numpy(de.copy()).cuda()
+    return pg, po
+
+
+def test_page_geometry_colours_and_far_light_match_oracle():
+    from imageprocessor_new_amd.backends import gpu_device
+
+    K = _lattice_page()
+    with gpu_device(0):
+        pg, po = _page_pair(K)
+        assert pg.spacing == pytest.approx(po.spacing, abs=1e-12) and pg.R == po.R
+        assert pg.Rr == po.Rr and pg.far_r0 == po.far_r0
+        order = pg.order.cpu().numpy()
+        assert np.array_equal(order, np.concatenate(po.chunks))
+        far = pg.far_field().cpu().numpy()
+        ref = po.far_field()
+        assert np.abs(far - ref).max() < 1e-9 * max(1.0, np.abs(ref).max())
+        page = pg.render_page().cpu().numpy()
+        ref = po.render_page()
+        assert np.abs(page - ref).max() < 2e-6 * np.abs(ref).max()
+
+
+def test_window_gathers_and_stacks_match_oracle():
+    from oracle import holes as H
+
+    from imageprocessor_new_amd.backends import gpu_device
+
+    K = _lattice_page()
+    with gpu_device(0):
+        pg, po = _page_pair(K)
+        page = po.render_page()
+        pt = torch.from_numpy(page).cuda()
+        win = pg.windows(page=pt, outputs=("valid", "data", "model", "own", "v0", "bin"))
+        idx = np.arange(po.n)
+        flat, valid, px, py = po._windows(idx, po.R)
+        assert np.array_equal(win["valid"].cpu().numpy(), valid)
+        assert np.array_equal(win["data"].cpu().numpy()[valid], np.float32(po.image.ravel()[flat][valid]))
+        np.testing.assert_allclose(win["model"].cpu().numpy()[valid], (page.ravel()[flat] + po.level[:, None])[valid],
+                                   rtol=0, atol=1e-9)
+        sx, sy = po._fine(px, py, idx)
+        v = H.sample(po.g3, sx, sy)
+        own = po.a[:, None] * (v[..., 0] + po.delta[:, None] * (v[..., 1] + po.delta[:, None] * v[..., 2]))
+        assert np.abs(win["own"].cpu().numpy() - own)[valid].max() < 2e-6 * np.abs(own).max()
+        N = po.template.N
+        bx, by = np.rint(sx).astype(np.int64), np.rint(sy).astype(np.int64)
+        inb = valid & (bx >= 0) & (bx < N) & (by >= 0) & (by < N)
+        assert np.array_equal(win["bin"].cpu().numpy(), np.where(inb, by * N + bx, -1))
+        good, _ = po.good_holes(page)
+        s1, w1 = pg.cleaned_stack(pt, good, radius_px=5.5)
+        s0, w0 = po.cleaned_stack(page, good, radius_px=5.5)
+        assert np.array_equal(np.isfinite(s1), np.isfinite(s0))
+        ok = np.isfinite(s0)
+        assert np.abs(s1[ok] - s0[ok]).max() < 1e-5 and np.abs(w1 - w0).max() < 1e-9 * w0.max()
+        g1, r1 = pg.good_holes(pt)
+        np.testing.assert_allclose(r1, po.window_rms(page), rtol=1e-9)
+        pg.fit_noise(pt)
+        po.fit_noise(page)
+        assert pg.var_line == pytest.approx(po.var_line, rel=1e-9)
+
+
+def test_hole_stage_matches_oracle():
+    from imageprocessor_new_amd.backends import gpu_device
+
+    K = _lattice_page()
+    with gpu_device(0):
+        pg, po = _page_pair(K)
+        for p in (pg, po):
+            p.var_line = (20.0, 0.0)
+            p.history = [None]  # the noise line is given
+        page_g = pg.fit_holes(iterations=3, size=True).cpu().numpy()
+        page_o = po.fit_holes(iterations=3, size=True)
+        c_g = pg.c.cpu().numpy()
+        print(f"hole stage: max |dc| {np.abs(c_g - po.c).max():.2e} px, |dd| "
+              f"{np.abs(pg.delta.cpu().numpy() - po.delta).max():.2e} px")
+        assert np.abs(c_g - po.c).max() < 1e-4
+        assert np.abs(pg.delta.cpu().numpy() - po.delta).max() < 1e-4
+        assert np.abs(pg.a.cpu().numpy() / po.a - 1).max() < 1e-5
+        assert np.abs(page_g - page_o).max() < 1e-3
+        sd = pg.delta_sd.cpu().numpy()
+        assert np.allclose(sd, po.delta_sd, rtol=1e-4)
+        assert pg.rms_typ == pytest.approx(po.rms_typ, rel=1e-5)
+        med_g, rad_g = pg.delta_information()
+        med_o, rad_o = po.delta_information()
+        assert med_g == pytest.approx(med_o, rel=1e-4) and rad_g == pytest.approx(rad_o, rel=1e-12)
+
+
+def test_shape_stage_matches_oracle():
+    from imageprocessor_new_amd import outline as G
+    from imageprocessor_new_amd.backends import gpu_device
+
+    to = O.Template(upsample=4, size_px=16)
+    truth = _theta(to, 2.6, 2.4, p=2.5, phi=0.1, s1x=1.1, s1y=1.0, s2x=2.2, s2y=2.0, w2=0.05)
+    rng = np.random.default_rng(2)
+    stack = 7.0 * to.render(truth, box_px=1 / 4) + 0.3 + rng.normal(0, 0.01, (to.N, to.N))
+    wgt = rng.uniform(0.5, 2.0, stack.shape)
+    ok = np.ones(stack.shape, bool)
+    start = to.start(2.0, 1.0)
+    ref = O.fit_stages(to, lambda th: to.render(th, box_px=1 / 4)[ok], stack[ok], np.sqrt(wgt[ok]), start,
+                       stages=("round", "super"))
+    with gpu_device(0) as dev:
+        tg = G.OutlineTemplate(upsample=4, size_px=16, device=dev)
+        okd = torch.from_numpy(ok).to(dev)
+        got = G.fit_stages(tg, lambda th: tg.render(th, box_px=1 / 4)[okd], torch.from_numpy(stack[ok]).to(dev),
+                           torch.from_numpy(np.sqrt(wgt[ok])).to(dev), start, stages=("round", "super"))
+    assert to.cd_px(got[0]) == pytest.approx(to.cd_px(ref[0]), abs=1e-6)
+    np.testing.assert_allclose(got[0], ref[0], atol=1e-5)
+    assert got[1] == pytest.approx(ref[1], rel=1e-6) and got[3] == pytest.approx(ref[3], rel=1e-5)
+
+
+def test_fit_outline_matches_the_oracle_fit():
+    from imageprocessor_new_amd import fit_outline
+
+    K = _lattice_page()
+    ref = O.fit_outline(K["page"], K["seeds"], rounds=3)
+    got = fit_outline(K["page"], K["seeds"], rounds=3)
+    print(f"outline CD {got.outline_cd_px:.6f} vs {ref['cd_outline_px']:.6f}; hole CD max "
+          f"{np.abs(got.cd_px - ref['cd_px']).max():.2e}; centres max {np.abs(got.centers_xy - ref['centers_xy']).max():.2e}")
+    assert got.rounds == ref["rounds"] and got.per_hole_size == ref["per_hole_size"]
+    assert got.outline_cd_px == pytest.approx(ref["cd_outline_px"], abs=1e-4)
+    np.testing.assert_allclose(got.cd_px, ref["cd_px"], atol=2e-3)
+    np.testing.assert_allclose(got.centers_xy, ref["centers_xy"], atol=5e-4)
+    assert np.mean(got.good == ref["good"]) > 0.97
+    assert got.rays_px.shape == (len(K["seeds"]), 128) and got.template.shape == (192, 192)
+
+
+def test_fit_outline_reads_the_size_at_slope_one():
+    """Known answer (as tests/test_outline_fit.py:76-98 of the reference): a noisy page drawn
+    from the model at known sizes; the fit follows them."""
+    from imageprocessor_new_amd import fit_outline
+
+    K = _lattice_page(seed=7, noise=6.0)
+    t, th = K["t"], K["theta"]
+    before = K["seeds"].copy()
+    fit = fit_outline(K["page"], K["seeds"], rounds=6)
+    np.testing.assert_array_equal(K["seeds"], before)
+    ok = fit.good & ~fit.at_clip
+    assert ok.mean() > 0.5
+    true_cd = np.array([t.cd_px(th, float(d)) for d in K["delta"]])
+    assert abs(fit.cd_px[ok].mean() - true_cd[ok].mean()) < 0.05
+    assert 0.85 < np.polyfit(true_cd[ok], fit.cd_px[ok], 1)[0] < 1.15
+    truth_xy = K["c"] + np.array(t.centroid_px(th))
+    assert np.sqrt(np.mean(np.sum((fit.centers_xy[ok] - truth_xy[ok]) ** 2, axis=1))) < 0.03
+    assert fit.outline_cd_px == pytest.approx(t.cd_px(th), abs=0.05)
+    assert 0.0 < np.nanmedian(fit.cd_se_px) < 0.2
+    # invert: dark holes on a bright page are reflected as max - image and read the same outline
+    page0 = K["page"] - K["page"].min()
+    inv = fit_outline(page0.max() - page0, K["seeds"], rounds=2, invert=True)
+    ref = fit_outline(page0, K["seeds"], rounds=2)
+    assert inv.outline_cd_px == pytest.approx(ref.outline_cd_px, abs=1e-4)
+    np.testing.assert_allclose(inv.level, page0.max() - ref.level, atol=1e-3)
