@@ -158,11 +158,32 @@ class OutlineTemplate:
             dmod = dmod + k * (coeffs[i, 1] * ck - coeffs[i, 0] * sk)
         return base * mod, dbase * mod + base * dmod
 
+    def edge_table(self, theta, delta=0.0) -> torch.Tensor:
+        """The outline's radius, moved `delta` px along the normal, on the 8192-entry angle table
+        (device FP64; the same formulas as radius_at, evaluated where the coverage reads them)."""
+        a, b, p, phi, coeffs, _ = self.unpack(theta)
+        if not hasattr(self, "_table_d"):
+            self._table_d = torch.from_numpy(self.table_angles).to(self.device)
+        ang = self._table_d
+        ct, st = torch.cos(ang - phi), torch.sin(ang - phi)
+        qa, qb = torch.abs(ct / a), torch.abs(st / b)
+        base = (qa ** p + qb ** p) ** (-1.0 / p)
+        mod = torch.ones_like(ang)
+        for i, k in enumerate(self.ks):
+            mod = mod + coeffs[i, 0] * torch.cos(k * ang) + coeffs[i, 1] * torch.sin(k * ang)
+        r = base * mod
+        if not delta:
+            return r
+        dbase = -base ** (1 + p) * (qa ** (p - 1) * torch.sign(ct) * (-st / a) + qb ** (p - 1) * torch.sign(st) * (ct / b))
+        dmod = torch.zeros_like(ang)
+        for i, k in enumerate(self.ks):
+            dmod = dmod + k * (coeffs[i, 1] * torch.cos(k * ang) - coeffs[i, 0] * torch.sin(k * ang))
+        dr = dbase * mod + base * dmod
+        return r + delta * torch.hypot(r, dr) / r
+
     def indicator(self, theta, delta=0.0):
         """Fine-pixel coverage of the outline moved `delta` px along its normal (device, N x N)."""
-        r, dr = self.radius_at(theta, self.table_angles)
-        edge = r + delta * np.hypot(r, dr) / r if delta else r
-        edge_d = torch.from_numpy(np.ascontiguousarray(edge)).to(self.device)
+        edge_d = self.edge_table(theta, delta).contiguous()
         out = torch.empty((self.N, self.N), dtype=torch.float64, device=self.device)
         _lib.call("spx_outline_coverage", _ptr(edge_d), _ptr(self.rho_d), _ptr(self.ang_d), self.N, self.aa,
                   float(self.U * self.aa), _ptr(out), stream_handle())
@@ -253,6 +274,32 @@ class OutlineTemplate:
                     harmonics={f"k{k}": [float(coeffs[i, 0]), float(coeffs[i, 1])] for i, k in enumerate(self.ks)},
                     blur_fwhm_nm=[v * pixel_size for v in fw],
                     blur_rms_nm=[v * pixel_size for v in self.blur_sigma_px(theta)], blur_family=self.blur)
+
+
+def _level_bins(res: torch.Tensor, lev: torch.Tensor):
+    """Twelve level bins between the 0, 8.3, ..., 100th percentiles of lev (lo <= lev < hi);
+    per bin with more than 200 pixels the mean level and the squared robust spread
+    (1.4826 x median absolute deviation) of res, by NumPy's interpolation rules (page.py:515-524)."""
+    lev_sorted, perm = torch.sort(lev)
+    res_by_lev = res[perm]
+    m = lev_sorted.numel()
+    pos = np.linspace(0, 100, 13) / 100.0 * (m - 1)
+    lo_i, hi_i = np.floor(pos).astype(np.int64), np.ceil(pos).astype(np.int64)
+    frac = torch.from_numpy(pos - lo_i).to(lev.device)
+    vlo, vhi = lev_sorted[torch.from_numpy(lo_i).to(lev.device)], lev_sorted[torch.from_numpy(hi_i).to(lev.device)]
+    d = vhi - vlo
+    edges = torch.where(frac >= 0.5, vhi - d * (1.0 - frac), vlo + d * frac)
+    # bin j is a contiguous run of the sorted levels
+    bounds = torch.searchsorted(lev_sorted, edges).cpu().numpy()
+    xs, ys = [], []
+    for j in range(12):
+        b0, b1 = int(bounds[j]), int(bounds[j + 1])
+        if b1 - b0 > 200:
+            r = res_by_lev[b0:b1]
+            xs.append(float(lev_sorted[b0:b1].mean()))
+            mad = torch.quantile(torch.abs(r - torch.quantile(r, 0.5)), 0.5)
+            ys.append(float((1.4826 * mad) ** 2))
+    return xs, ys
 
 
 # ---------------------------------------------------------------------------------- shape stage
@@ -597,16 +644,9 @@ class OutlinePage:
         n3 = min(self.n, 3 * self.chunk)
         win = self.windows(self.all_idx[:n3], page=page, outputs=("valid", "data", "model"))
         ok = win["valid"]
-        res = (win["data"] - win["model"])[ok].cpu().numpy()
-        lev = win["model"][ok].cpu().numpy()
-        edges = np.percentile(lev, np.linspace(0, 100, 13))
-        xs, ys = [], []
-        for lo, hi in zip(edges[:-1], edges[1:]):
-            s = (lev >= lo) & (lev < hi)
-            if s.sum() > 200:
-                r = res[s]
-                xs.append(lev[s].mean())
-                ys.append((1.4826 * np.median(np.abs(r - np.median(r)))) ** 2)
+        res = (win["data"] - win["model"])[ok]
+        lev = win["model"][ok]
+        xs, ys = _level_bins(res, lev)
         xs, ys = np.asarray(xs), np.asarray(ys)
         if len(xs) >= 3:
             v1, v0 = np.polyfit(xs, ys, 1)
@@ -933,3 +973,26 @@ def fit_outline(image, seeds_xy, *, pixel_size: float = 1.0, unit: str = "px", b
         stack=np.asarray(res["stack"], dtype=float), upsample=int(template.U), summary=summary,
         pixel_size=float(pixel_size), unit=str(unit),
     )
+
+
+def render_outline_page(theta, centers, brightness, shape, *, level: float = 500.0, noise: float = 30.0,
+                        edge_offset=None, blur: str = "gauss2", seed: int = 0, device: int = 0) -> torch.Tensor:
+    """A synthetic page of the outline model on the device: every hole's light
+    a (T0 + d T1 + d^2 T2) of the outline and blur `theta` (near windows + far light, as the fit's
+    page model), a constant level and N(0, noise^2) pixel noise. (H, W) FP64 device tensor.
+    Synthetic input for tests and benchmarks."""
+    h, w = int(shape[0]), int(shape[1])
+    c = np.asarray(centers, dtype=np.float64).reshape(-1, 2)
+    with gpu_device(device) as dev:
+        t = OutlineTemplate(upsample=UPSAMPLE, size_px=BLUR_FAMILIES[blur], blur=blur, device=dev)
+        pg = OutlinePage(torch.zeros((h, w), dtype=torch.float32, device=dev), c, t, device=dev)
+        pg.set_theta(np.asarray(theta, float))
+        pg.a = torch.as_tensor(np.asarray(brightness, dtype=np.float64)).to(dev).clone()
+        if edge_offset is not None:
+            pg.delta = torch.as_tensor(np.asarray(edge_offset, dtype=np.float64)).to(dev).clone()
+        page = pg.render_page() + float(level)
+        if noise:
+            g = torch.Generator(device=dev)
+            g.manual_seed(int(seed))
+            page += float(noise) * torch.randn(page.shape, generator=g, dtype=torch.float64, device=dev)
+        return page
