@@ -175,7 +175,7 @@ SIGNATURES = {
                               _P, _P, _P, _P, _P, _P, ctypes.POINTER(SpxHoleStepOptions), _P, _P]),
     "spx_select_kth": (_I32, [_P, _I64, _I64, _P, _P, _I32, _P, _P]),
     "spx_far_splat": (_I32, [_P, _P, _P, ctypes.c_double, _I32, _I32, _I32, _I32, _I32, _P, _P]),
-    "spx_outline_coverage": (_I32, [_P, _P, _P, _I32, _I32, ctypes.c_double, _P, _P]),
+    "spx_outline_coverage": (_I32, [_P, _P, _P, _I32, _I32, ctypes.c_double, _I32, _P, _P]),
     "spx_outline_windows": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _I32, _P, _I32, _P, _P, _P, _P,
                                    ctypes.c_double, _P, _P, _P, _P, _P, _P, _P]),
     "spx_outline_far_splat": (_I32, [_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P]),

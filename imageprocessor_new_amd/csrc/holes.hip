@@ -406,6 +406,9 @@ __global__ void outline_coverage_kernel(const double* __restrict__ edge, const d
                                         double* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)N * N) return;
+    // blockIdx.y: outline of the batch (its table of 8192 radii, its output grid)
+    edge += (int64_t)blockIdx.y * 8192;
+    out += (int64_t)blockIdx.y * N * N;
     const int iy = (int)(t / N), ix = (int)(t - (int64_t)iy * N);
     const int64_t M = (int64_t)N * aa;
     double acc = 0.0;
@@ -576,9 +579,11 @@ __global__ __launch_bounds__(1024) void select_kth_kernel(const double* x, int64
 }  // namespace
 
 int outline_coverage_launch(const double* edge, const double* rho, const int32_t* ang, int N, int aa, double scale,
-                            double* out, hipStream_t stream) {
+                            int nbatch, double* out, hipStream_t stream) {
     const int64_t nn = (int64_t)N * N;
-    outline_coverage_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, stream>>>(edge, rho, ang, N, aa, scale, out);
+    if (nbatch == 0) return SPX_OK;
+    dim3 grid((unsigned)((nn + 255) / 256), (unsigned)nbatch);
+    outline_coverage_kernel<<<grid, 256, 0, stream>>>(edge, rho, ang, N, aa, scale, out);
     SPX_LAUNCHED();
     return SPX_OK;
 }

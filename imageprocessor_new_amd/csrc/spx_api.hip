@@ -116,7 +116,7 @@ int far_splat_launch(const double* c, const double* a, const double* delta, doub
 int disk_templates_launch(double radius, double sigma, int N, int U, double r0, double ramp, int B, float* tmpl,
                           float* far, hipStream_t stream);
 int outline_coverage_launch(const double* edge, const double* rho, const int32_t* ang, int N, int aa, double scale,
-                            double* out, hipStream_t stream);
+                            int nbatch, double* out, hipStream_t stream);
 int outline_window_launch(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,
                           const float* tmpl, int N, int U, int R, const int32_t* idx, int n, const double* c,
                           const double* a, const double* level, const double* delta, double radius_fine,
@@ -515,11 +515,12 @@ int spx_far_splat(const double* c, const double* a, const double* delta, double 
 }
 
 int spx_outline_coverage(const double* edge, const double* rho, const int32_t* ang, int32_t N, int32_t aa,
-                         double scale, double* out, spx_stream_t stream) {
+                         double scale, int32_t nbatch, double* out, spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(N >= 8 && N <= 4096 && aa >= 1 && aa <= 16, "spx_outline_coverage: bad grid");
-    SPX_REQUIRE(edge && rho && ang && out, "spx_outline_coverage: null pointer");
-    return spx::outline_coverage_launch(edge, rho, ang, N, aa, scale, out, spx::as_stream(stream));
+    SPX_REQUIRE(nbatch >= 0 && nbatch <= 65535, "spx_outline_coverage: 0..65535 outlines per call");
+    SPX_REQUIRE(nbatch == 0 || (edge && rho && ang && out), "spx_outline_coverage: null pointer");
+    return spx::outline_coverage_launch(edge, rho, ang, N, aa, scale, nbatch, out, spx::as_stream(stream));
 }
 
 int spx_outline_windows(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,

@@ -158,54 +158,79 @@ class OutlineTemplate:
             dmod = dmod + k * (coeffs[i, 1] * ck - coeffs[i, 0] * sk)
         return base * mod, dbase * mod + base * dmod
 
-    def edge_table(self, theta, delta=0.0) -> torch.Tensor:
-        """The outline's radius, moved `delta` px along the normal, on the 8192-entry angle table
-        (device FP64; the same formulas as radius_at, evaluated where the coverage reads them)."""
-        a, b, p, phi, coeffs, _ = self.unpack(theta)
+    def _columns(self, thetas):
+        """Per-outline parameters of a batch of thetas (B, n_params) as (B, 1) device columns."""
+        th = np.atleast_2d(np.asarray(thetas, float))
+        dev = self.device
+        col = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)[:, None]  # noqa: E731
+        coeffs = th[:, 4:self.n_shape].reshape(th.shape[0], -1, 2)
+        return dict(a=col(np.exp(th[:, 0])), b=col(np.exp(th[:, 1])), p=col(1.0 + np.exp(th[:, 2])), phi=col(th[:, 3]),
+                    c=[col(coeffs[:, i, 0]) for i in range(len(self.ks))],
+                    s=[col(coeffs[:, i, 1]) for i in range(len(self.ks))], blur=th[:, self.n_shape:])
+
+    def edge_tables(self, thetas, delta=0.0) -> torch.Tensor:
+        """The outlines' radii, moved `delta` px along the normal, on the 8192-entry angle table
+        (B, 8192) FP64 on the device: the formulas of radius_at, evaluated where the coverage
+        reads them."""
+        q = self._columns(thetas)
         if not hasattr(self, "_table_d"):
-            self._table_d = torch.from_numpy(self.table_angles).to(self.device)
+            self._table_d = torch.from_numpy(self.table_angles).to(self.device)[None, :]
         ang = self._table_d
-        ct, st = torch.cos(ang - phi), torch.sin(ang - phi)
+        a, b, p = q["a"], q["b"], q["p"]
+        ct, st = torch.cos(ang - q["phi"]), torch.sin(ang - q["phi"])
         qa, qb = torch.abs(ct / a), torch.abs(st / b)
         base = (qa ** p + qb ** p) ** (-1.0 / p)
-        mod = torch.ones_like(ang)
+        mod = torch.ones_like(base)
         for i, k in enumerate(self.ks):
-            mod = mod + coeffs[i, 0] * torch.cos(k * ang) + coeffs[i, 1] * torch.sin(k * ang)
+            mod = mod + q["c"][i] * torch.cos(k * ang) + q["s"][i] * torch.sin(k * ang)
         r = base * mod
         if not delta:
             return r
         dbase = -base ** (1 + p) * (qa ** (p - 1) * torch.sign(ct) * (-st / a) + qb ** (p - 1) * torch.sign(st) * (ct / b))
-        dmod = torch.zeros_like(ang)
+        dmod = torch.zeros_like(base)
         for i, k in enumerate(self.ks):
-            dmod = dmod + k * (coeffs[i, 1] * torch.cos(k * ang) - coeffs[i, 0] * torch.sin(k * ang))
+            dmod = dmod + k * (q["s"][i] * torch.cos(k * ang) - q["c"][i] * torch.sin(k * ang))
         dr = dbase * mod + base * dmod
         return r + delta * torch.hypot(r, dr) / r
 
-    def indicator(self, theta, delta=0.0):
-        """Fine-pixel coverage of the outline moved `delta` px along its normal (device, N x N)."""
-        edge_d = self.edge_table(theta, delta).contiguous()
-        out = torch.empty((self.N, self.N), dtype=torch.float64, device=self.device)
-        _lib.call("spx_outline_coverage", _ptr(edge_d), _ptr(self.rho_d), _ptr(self.ang_d), self.N, self.aa,
-                  float(self.U * self.aa), _ptr(out), stream_handle())
+    def edge_table(self, theta, delta=0.0) -> torch.Tensor:
+        return self.edge_tables(theta, delta)[0]
+
+    def indicator_batch(self, thetas, delta=0.0) -> torch.Tensor:
+        """Fine-pixel coverage of each outline moved `delta` px along its normal (B, N, N)."""
+        edge = self.edge_tables(thetas, delta).contiguous()
+        nb = int(edge.shape[0])
+        out = torch.empty((nb, self.N, self.N), dtype=torch.float64, device=self.device)
+        _lib.call("spx_outline_coverage", _ptr(edge), _ptr(self.rho_d), _ptr(self.ang_d), self.N, self.aa,
+                  float(self.U * self.aa), nb, _ptr(out), stream_handle())
         return out
 
+    def indicator(self, theta, delta=0.0):
+        """Fine-pixel coverage of the outline moved `delta` px along its normal (device, N x N)."""
+        return self.indicator_batch(theta, delta)[0]
+
     # blur
-    def otf(self, theta):
-        bp = self.unpack(theta)[5]
-        fx2, fy2 = self.fx * self.fx, self.fy * self.fy
+    def otf_batch(self, thetas) -> torch.Tensor:
+        """Blur transfer functions (B, N, N) of a batch of thetas."""
+        bp = np.atleast_2d(np.asarray(thetas, float))[:, self.n_shape:]
+        col = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(self.device)[:, None, None]  # noqa: E731
+        fx2, fy2 = (self.fx * self.fx)[None], (self.fy * self.fy)[None]
         k = -2.0 * math.pi ** 2
         if self.blur == "gauss2":
-            s1x, s1y, s2x, s2y = np.exp(bp[:4])
-            w2 = 1.0 / (1.0 + math.exp(-bp[4]))
-            g1 = torch.exp(k * (s1x ** 2 * fx2 + s1y ** 2 * fy2))
-            g2 = torch.exp(k * (s2x ** 2 * fx2 + s2y ** 2 * fy2))
+            s = np.exp(bp[:, :4])
+            w2 = col(1.0 / (1.0 + np.exp(-bp[:, 4])))
+            g1 = torch.exp(k * (col(s[:, 0] ** 2) * fx2 + col(s[:, 1] ** 2) * fy2))
+            g2 = torch.exp(k * (col(s[:, 2] ** 2) * fx2 + col(s[:, 3] ** 2) * fy2))
             return (1.0 - w2) * g1 + w2 * g2
-        sx, sy = np.exp(bp[:2])
-        g = torch.exp(k * (sx ** 2 * fx2 + sy ** 2 * fy2))
+        s = np.exp(bp[:, :2])
+        g = torch.exp(k * (col(s[:, 0] ** 2) * fx2 + col(s[:, 1] ** 2) * fy2))
         if self.blur == "lens":
-            s = torch.clamp(self.nu / math.exp(bp[2]), 0.0, 1.0)
-            g = g * (2.0 / math.pi) * (torch.acos(s) - s * torch.sqrt(1.0 - s * s))
+            sc = torch.clamp(self.nu[None] / col(np.exp(bp[:, 2])), 0.0, 1.0)
+            g = g * (2.0 / math.pi) * (torch.acos(sc) - sc * torch.sqrt(1.0 - sc * sc))
         return g
+
+    def otf(self, theta):
+        return self.otf_batch(theta)[0]
 
     def blur_kernel(self, theta) -> np.ndarray:
         k = torch.fft.fftshift(torch.fft.ifft2(self.otf(theta)).real).cpu().numpy()
@@ -228,14 +253,18 @@ class OutlineTemplate:
         return math.sqrt((k * self.xx ** 2).sum()), math.sqrt((k * self.yy ** 2).sum())
 
     # template
-    def render(self, theta, delta=0.0, box_px=0.0):
-        """T on the device (N x N, FP64): coverage -> spectrum x sharp-edge correction x blur x
-        pixel box [x an extra box of box_px] -> back (model.py:200-205)."""
-        spec = torch.fft.fft2(torch.fft.ifftshift(self.indicator(theta, delta)))
-        spec = spec * (self.edge_fix * self.otf(theta) * self.pixel_otf)
+    def render_batch(self, thetas, delta=0.0, box_px=0.0) -> torch.Tensor:
+        """T of each theta on the device (B, N, N) FP64: coverage -> spectrum x sharp-edge
+        correction x blur x pixel box [x an extra box of box_px] -> back (model.py:200-205)."""
+        dims = (-2, -1)
+        spec = torch.fft.fft2(torch.fft.ifftshift(self.indicator_batch(thetas, delta), dim=dims))
+        factor = (self.edge_fix * self.pixel_otf)[None] * self.otf_batch(thetas)
         if box_px > 0:
-            spec = spec * (torch.sinc(self.fx * box_px) * torch.sinc(self.fy * box_px))
-        return torch.fft.fftshift(torch.fft.ifft2(spec).real)
+            factor = factor * (torch.sinc(self.fx * box_px) * torch.sinc(self.fy * box_px))[None]
+        return torch.fft.fftshift(torch.fft.ifft2(spec * factor).real, dim=dims)
+
+    def render(self, theta, delta=0.0, box_px=0.0):
+        return self.render_batch(theta, delta, box_px)[0]
 
     def render_quad(self, theta, h=0.1):
         t0, tp, tm = self.render(theta), self.render(theta, h), self.render(theta, -h)
@@ -323,10 +352,23 @@ def stage_map(t, stage):
     return list(range(t.n_params)), {}
 
 
-def fit_stages(t, render_for, y, sw, theta0, stages=("round", "super", "full"), max_nfev=80, grid=True, log=None):
+def _fd_steps(x0, f0, lo, hi):
+    """Forward-difference steps of SciPy's '2-point' Jacobian (its own step and bound rules,
+    scipy.optimize._numdiff), so that a batched Jacobian has the columns SciPy would form."""
+    from scipy.optimize import _numdiff
+
+    h = _numdiff._compute_absolute_step(None, x0, f0, "2-point")
+    h, _ = _numdiff._adjust_scheme_to_bounds(x0, h, 1, "1-sided", lo, hi)
+    return h
+
+
+def fit_stages(t, render_for, y, sw, theta0, stages=("round", "super", "full"), max_nfev=80, grid=True, log=None,
+               batch=48):
     """Bounded least squares of scale * render_for(theta) + const on y (device, sqrt-weights sw)
-    in stages, scale and const solved linearly per theta (model.py:293-368). render_for
-    returns the device model values; the solver (SciPy 'trf') runs on the host."""
+    in stages, scale and const solved linearly per theta (model.py:293-368). render_for maps a
+    batch of thetas (B, n_params) to the device model values (B, K). The solver (SciPy 'trf')
+    runs on the host; the start grid and every finite-difference Jacobian are rendered as one
+    batch each (forward steps by SciPy's own rules), a residual vector is one copy to the host."""
     from scipy.optimize import least_squares
 
     lo_all, hi_all = t.bounds()
@@ -335,17 +377,20 @@ def fit_stages(t, render_for, y, sw, theta0, stages=("round", "super", "full"), 
     wsum = w.sum()
     yw = (w * y).sum() / wsum
 
-    def resid(th):
-        m = render_for(th)
-        mw = (w * m).sum() / wsum
-        dm = m - mw
-        var = (w * dm * dm).sum()
-        cov = (w * dm * (y - yw)).sum()
-        sc = torch.where(var > 0, cov / torch.where(var > 0, var, torch.ones_like(var)), torch.ones_like(var))
-        co = yw - sc * mw
-        r = sw * (sc * m + co - y)
-        host = torch.cat([r, sc.reshape(1), co.reshape(1)]).cpu().numpy()
-        return host[:-2], float(host[-2]), float(host[-1])
+    def resid_batch(ths):
+        ths = np.atleast_2d(ths)
+        rows = []
+        for s0 in range(0, len(ths), batch):
+            m = render_for(ths[s0:s0 + batch])
+            mw = (w * m).sum(1, keepdim=True) / wsum
+            dm = m - mw
+            var = (w * dm * dm).sum(1, keepdim=True)
+            cov = (w * dm * (y - yw)).sum(1, keepdim=True)
+            sc = torch.where(var > 0, cov / torch.where(var > 0, var, torch.ones_like(var)), torch.ones_like(var))
+            co = yw - sc * mw
+            rows.append(torch.cat([sw * (sc * m + co - y), sc, co], dim=1))
+        host = torch.cat(rows).cpu().numpy()
+        return host[:, :-2], host[:, -2], host[:, -1]
 
     chi2, scale, const = float("nan"), 1.0, 0.0
     for stage in stages:
@@ -355,8 +400,9 @@ def fit_stages(t, render_for, y, sw, theta0, stages=("round", "super", "full"), 
             base[1], base[2], base[3] = base[0], 0.0, 0.0
             base[4:t.n_shape] = 0.0
             if grid:
-                j, best = t.n_shape, None
+                j = t.n_shape
                 a0 = math.exp(base[0])
+                cands = []
                 for a in a0 * np.linspace(0.4, 1.8, 15):
                     for s in np.linspace(0.45, 2.2, 12):
                         q = base.copy()
@@ -367,24 +413,39 @@ def fit_stages(t, render_for, y, sw, theta0, stages=("round", "super", "full"), 
                             q[j + 4] = math.log(0.05 / 0.95)
                         elif t.blur == "lens":
                             q[j + 2] = math.log(0.5)
-                        q = np.clip(q, lo_all + 1e-6, hi_all - 1e-6)
-                        chi = float(np.mean(resid(q)[0] ** 2))
-                        if best is None or chi < best[0]:
-                            best = (chi, q)
-                base = best[1]
+                        cands.append(np.clip(q, lo_all + 1e-6, hi_all - 1e-6))
+                r = resid_batch(np.stack(cands))[0]
+                chi = np.mean(r * r, axis=1)
+                base = cands[int(np.argmin(chi))]
 
         def expand(q, base=base, free=free, ties=ties):
-            th = base.copy()
-            th[free] = q
+            th = np.array(base, copy=True) if q.ndim == 1 else np.repeat(base[None], len(q), axis=0)
+            th[..., free] = q
             for k, src in ties.items():
-                th[k] = th[src]
+                th[..., k] = th[..., src]
             return th
 
         lo, hi = lo_all[free], hi_all[free]
-        r = least_squares(lambda q: resid(expand(q))[0], np.clip(base[free], lo + 1e-6, hi - 1e-6), bounds=(lo, hi),
-                          method="trf", x_scale="jac", xtol=1e-9, ftol=1e-10, max_nfev=max_nfev)
+        last = {}
+
+        def fun(q):
+            r = resid_batch(expand(q))[0][0]
+            last["q"], last["f"] = q.copy(), r
+            return r
+
+        def jac(q):
+            f0 = last["f"] if "q" in last and np.array_equal(last["q"], q) else fun(q)
+            h = _fd_steps(q, f0, lo, hi)
+            xs = q[None, :] + np.diag(h)
+            dx = np.diag(xs) - q
+            fx = resid_batch(expand(xs))[0]
+            return ((fx - f0[None, :]) / dx[:, None]).T
+
+        r = least_squares(fun, np.clip(base[free], lo + 1e-6, hi - 1e-6), jac=jac, bounds=(lo, hi), method="trf",
+                          x_scale="jac", xtol=1e-9, ftol=1e-10, max_nfev=max_nfev)
         theta = expand(r.x)
-        _, scale, const = resid(theta)
+        _, sc, co = resid_batch(theta)
+        scale, const = float(sc[0]), float(co[0])
         chi2 = float(np.mean(r.fun ** 2))
         if log is not None:
             log(f"    shape stage {stage}: chi2 {chi2:.3f}, CD {t.cd_px(theta):.3f} px, scale {scale:.4f}, "
@@ -812,8 +873,8 @@ class OutlinePage:
         okd = torch.from_numpy(ok).to(self.device)
         y = torch.from_numpy(stack[ok]).to(self.device)
         sw = torch.from_numpy(np.sqrt(weight[ok])).to(self.device)
-        return fit_stages(t, lambda th: t.render(th, box_px=1.0 / t.U)[okd], y, sw, self.theta, stages=stages,
-                          log=self.log.append)
+        return fit_stages(t, lambda ths: t.render_batch(ths, box_px=1.0 / t.U)[:, okd], y, sw, self.theta,
+                          stages=stages, log=self.log.append)
 
     # -------------------------------------------------------------------------------------- run
     def run(self, rounds=14, iterations=5, log=None, relax=0.5, tol_px=0.08 / 21.0, release_px=0.3 / 21.0,

@@ -400,12 +400,13 @@ int spx_far_splat(const double* c, const double* a, const double* delta, double 
  * The template T = pixel box * blur * coverage(outline) lives on an N x N fine grid, U points
  * per pixel; the page model and the hole state are those of the hole stage above. */
 
-/* Pixel coverage of the outline (model.py:149-159): out (N, N) FP64, out[iy][ix] = mean over
- * the aa x aa sub-samples s of clamp(0.5 + (edge[ang[s]] - rho[s]) * scale, 0, 1), with the
- * sub-samples' radius rho and angle-table index ang (N aa x N aa, row-major) precomputed and
- * edge the outline's radius on the angle table (scale = U aa: a one-sample ramp). */
+/* Pixel coverage of nbatch outlines (model.py:149-159): out (nbatch, N, N) FP64, out[b][iy][ix] =
+ * mean over the aa x aa sub-samples s of clamp(0.5 + (edge[b][ang[s]] - rho[s]) * scale, 0, 1),
+ * with the sub-samples' radius rho and angle-table index ang (N aa x N aa, row-major)
+ * precomputed and edge[b] (nbatch x 8192) outline b's radius on the angle table (scale = U aa:
+ * a one-sample ramp). */
 int spx_outline_coverage(const double* edge, const double* rho, const int32_t* ang, int32_t N, int32_t aa,
-                         double scale, double* out, spx_stream_t stream);
+                         double scale, int32_t nbatch, double* out, spx_stream_t stream);
 
 /* Per-window gather of the holes idx[0..n) (page.py:206-216, 294-299, 420-540, 732-766): for
  * window pixel q of hole i (row-major over (2R+1)^2 offsets around rint(c)), at t = i P + q:

@@ -189,7 +189,7 @@ def test_shape_stage_matches_oracle():
     with gpu_device(0) as dev:
         tg = G.OutlineTemplate(upsample=4, size_px=16, device=dev)
         okd = torch.from_numpy(ok).to(dev)
-        got = G.fit_stages(tg, lambda th: tg.render(th, box_px=1 / 4)[okd], torch.from_numpy(stack[ok]).to(dev),
+        got = G.fit_stages(tg, lambda ths: tg.render_batch(ths, box_px=1 / 4)[:, okd], torch.from_numpy(stack[ok]).to(dev),
                            torch.from_numpy(np.sqrt(wgt[ok])).to(dev), start, stages=("round", "super"))
     assert to.cd_px(got[0]) == pytest.approx(to.cd_px(ref[0]), abs=1e-6)
     np.testing.assert_allclose(got[0], ref[0], atol=1e-5)
