@@ -179,6 +179,8 @@ SIGNATURES = {
     "spx_outline_windows": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _I32, _P, _I32, _P, _P, _P, _P,
                                    ctypes.c_double, _P, _P, _P, _P, _P, _P, _P]),
     "spx_outline_far_splat": (_I32, [_P, _P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P]),
+    "spx_outline_stack": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _I32, _P, _I32, _P, _P, _P, _P, _P,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, _I32, _P, _P, _P]),
     "spx_render_tiles": (_I32, [ctypes.c_uint64, _I32, _I32, ctypes.POINTER(SpxTileSpec), _P, _I64, _P, _P]),
     "spx_fit_batch": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions),
                              _I32, _P, _P, _P, _P, _P]),

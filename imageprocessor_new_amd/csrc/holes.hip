@@ -479,6 +479,66 @@ __global__ __launch_bounds__(256) void outline_window_kernel(WindowArgs A) {
     }
 }
 
+// Stacked average hole on the fine grid (page.py:488-505, 732-766), one thread per (hole, window
+// pixel) of the holes idx with good[h] (null: all): the pixel's bin rint(sx), rint(sy) (inside
+// the grid, and within radius_fine of the centre when radius_fine > 0) receives
+//   cleaned = 0: the grey level, weight 1 (the plain stack);
+//   cleaned = 1: (data - page - level + own) / a with weight a^2 / (nv0 + nv1 max(page + level, 0)),
+//                own = a (T0 + d T1 + d^2 T2) (the neighbour-cleaned stack),
+// as FP64 atomic sums into sums[bin] and wsum[bin].
+struct StackArgs {
+    const float* image;
+    int64_t H, W, pitch;
+    const double* page;
+    const float* tmpl;
+    int N, U, R;
+    const int32_t* idx;
+    int n;
+    const uint8_t* good;
+    const double* c;
+    const double* a;
+    const double* level;
+    const double* delta;
+    double nv0, nv1, radius_fine;
+    int cleaned;
+    double* sums;
+    double* wsum;
+};
+
+__global__ __launch_bounds__(256) void outline_stack_kernel(StackArgs A) {
+    const int side = 2 * A.R + 1;
+    const int64_t P = (int64_t)side * side;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)A.n * P) return;
+    const int i = (int)(t / P);
+    const int q = (int)(t - (int64_t)i * P);
+    const int h = A.idx[i];
+    if (A.good && !A.good[h]) return;
+    const double x = A.c[2 * (int64_t)h], y = A.c[2 * (int64_t)h + 1];
+    const int64_t px = (int64_t)rint(x) + (q % side) - A.R, py = (int64_t)rint(y) + (q / side) - A.R;
+    if (px < 0 || px >= A.W || py < 0 || py >= A.H) return;
+    const double half_n = (double)(A.N / 2);
+    const double sx = ((double)px - x) * A.U + half_n, sy = ((double)py - y) * A.U + half_n;
+    const double bx = rint(sx), by = rint(sy);
+    if (!(bx >= 0.0 && bx < (double)A.N && by >= 0.0 && by < (double)A.N)) return;
+    if (A.radius_fine > 0.0 && !(hypot(sx - half_n, sy - half_n) <= A.radius_fine)) return;
+    const int64_t b = (int64_t)by * A.N + (int64_t)bx;
+    const double data = (double)A.image[py * A.pitch + px];
+    if (!A.cleaned) {
+        atomicAdd(A.sums + b, data);
+        atomicAdd(A.wsum + b, 1.0);
+        return;
+    }
+    float v[3];
+    sample_tmpl<3>(A.tmpl, A.N, sx, sy, v);
+    const double ah = A.a[h], de = A.delta[h];
+    const double own = ah * ((double)v[0] + de * ((double)v[1] + de * (double)v[2]));
+    const double model = A.page[py * A.W + px] + A.level[h];
+    const double wgt = ah * ah / (A.nv0 + A.nv1 * fmax(model, 0.0));
+    atomicAdd(A.sums + b, (data - model + own) / ah * wgt);
+    atomicAdd(A.wsum + b, wgt);
+}
+
 // Keys splat of a, a d and a d^2 of every hole on three padded pixel grids (FP64, grid k at
 // grid + k hp wp); a hole whose 4 x 4 taps leave the grid is dropped (page.py:371-397).
 __global__ void outline_far_splat_kernel(const double* c, const double* a, const double* delta, int n, int mx, int my,
@@ -602,6 +662,23 @@ int outline_window_launch(const float* image, int64_t H, int64_t W, int64_t pitc
     WindowArgs A{image, H, W, pitch, page, tmpl, N, U, R, idx, n, c, a, level, delta, radius_fine,
                  bin, valid, data, model, own, v0};
     outline_window_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int outline_stack_launch(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,
+                         const float* tmpl, int N, int U, int R, const int32_t* idx, int n, const uint8_t* good,
+                         const double* c, const double* a, const double* level, const double* delta, double nv0,
+                         double nv1, double radius_fine, int cleaned, double* sums, double* wsum, hipStream_t stream) {
+    if (n == 0) return SPX_OK;
+    const int64_t total = (int64_t)n * (2 * R + 1) * (2 * R + 1);
+    if ((total + 255) / 256 >= ((int64_t)1 << 32)) {
+        set_error("outline_stack: too many (hole, pixel) pairs for one launch");
+        return SPX_E_ARG;
+    }
+    StackArgs A{image, H, W, pitch, page, tmpl, N, U, R, idx, n, good, c, a, level, delta, nv0, nv1, radius_fine,
+                cleaned, sums, wsum};
+    outline_stack_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A);
     SPX_LAUNCHED();
     return SPX_OK;
 }

@@ -124,6 +124,10 @@ int outline_window_launch(const float* image, int64_t H, int64_t W, int64_t pitc
                           hipStream_t stream);
 int outline_far_splat_launch(const double* c, const double* a, const double* delta, int n, int mx, int my, int hp,
                              int wp, double* grid, hipStream_t stream);
+int outline_stack_launch(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page,
+                         const float* tmpl, int N, int U, int R, const int32_t* idx, int n, const uint8_t* good,
+                         const double* c, const double* a, const double* level, const double* delta, double nv0,
+                         double nv1, double radius_fine, int cleaned, double* sums, double* wsum, hipStream_t stream);
 
 }  // namespace spx
 
@@ -544,6 +548,22 @@ int spx_outline_far_splat(const double* c, const double* a, const double* delta,
     SPX_REQUIRE(n >= 0 && hp > 0 && wp > 0, "spx_outline_far_splat: bad sizes");
     SPX_REQUIRE(n == 0 || (c && a && delta && grid), "spx_outline_far_splat: null pointer");
     return spx::outline_far_splat_launch(c, a, delta, n, mx, my, hp, wp, grid, spx::as_stream(stream));
+}
+
+int spx_outline_stack(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page, const float* tmpl,
+                      int32_t N, int32_t U, int32_t R, const int32_t* idx, int32_t n, const uint8_t* good,
+                      const double* c, const double* a, const double* level, const double* delta, double nv0,
+                      double nv1, double radius_fine, int32_t cleaned, double* sums, double* wsum,
+                      spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(n >= 0 && H > 0 && W > 0 && pitch >= W, "spx_outline_stack: bad sizes");
+    SPX_REQUIRE(R >= 0 && R <= 255 && N >= 8 && N <= 4096 && U >= 1, "spx_outline_stack: bad window or grid");
+    SPX_REQUIRE(n == 0 || (image && idx && c && sums && wsum), "spx_outline_stack: null pointer");
+    SPX_REQUIRE(!cleaned || (page && tmpl && a && level && delta), "spx_outline_stack: the cleaned stack needs the "
+                "page model, template and hole state");
+    SPX_REQUIRE(!cleaned || nv0 > 0, "spx_outline_stack: noise variance line must be positive");
+    return spx::outline_stack_launch(image, H, W, pitch, page, tmpl, N, U, R, idx, n, good, c, a, level, delta, nv0,
+                                     nv1, radius_fine, cleaned ? 1 : 0, sums, wsum, spx::as_stream(stream));
 }
 
 }  // extern "C"

@@ -159,20 +159,29 @@ class OutlineTemplate:
         return base * mod, dbase * mod + base * dmod
 
     def _columns(self, thetas):
-        """Per-outline parameters of a batch of thetas (B, n_params) as (B, 1) device columns."""
+        """Per-outline parameters of a batch of thetas (B, n_params) as (B, 1) device columns (one
+        host-to-device copy): a, b, p, phi, the harmonic pairs, and the blur's variances / weight."""
         th = np.atleast_2d(np.asarray(thetas, float))
-        dev = self.device
-        col = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(dev)[:, None]  # noqa: E731
+        nk = len(self.ks)
         coeffs = th[:, 4:self.n_shape].reshape(th.shape[0], -1, 2)
-        return dict(a=col(np.exp(th[:, 0])), b=col(np.exp(th[:, 1])), p=col(1.0 + np.exp(th[:, 2])), phi=col(th[:, 3]),
-                    c=[col(coeffs[:, i, 0]) for i in range(len(self.ks))],
-                    s=[col(coeffs[:, i, 1]) for i in range(len(self.ks))], blur=th[:, self.n_shape:])
+        bp = th[:, self.n_shape:]
+        cols = [np.exp(th[:, 0]), np.exp(th[:, 1]), 1.0 + np.exp(th[:, 2]), th[:, 3]]
+        cols += [coeffs[:, i, 0] for i in range(nk)] + [coeffs[:, i, 1] for i in range(nk)]
+        if self.blur == "gauss2":
+            s = np.exp(bp[:, :4])
+            cols += [s[:, 0] ** 2, s[:, 1] ** 2, s[:, 2] ** 2, s[:, 3] ** 2, 1.0 / (1.0 + np.exp(-bp[:, 4]))]
+        else:
+            s = np.exp(bp[:, :2])
+            cols += [s[:, 0] ** 2, s[:, 1] ** 2] + ([np.exp(bp[:, 2])] if self.blur == "lens" else [])
+        m = torch.from_numpy(np.ascontiguousarray(np.stack(cols, axis=1), dtype=np.float64)).to(self.device)
+        c = [m[:, j:j + 1] for j in range(m.shape[1])]
+        return dict(a=c[0], b=c[1], p=c[2], phi=c[3], c=c[4:4 + nk], s=c[4 + nk:4 + 2 * nk], blur=c[4 + 2 * nk:])
 
-    def edge_tables(self, thetas, delta=0.0) -> torch.Tensor:
+    def edge_tables(self, thetas, delta=0.0, columns=None) -> torch.Tensor:
         """The outlines' radii, moved `delta` px along the normal, on the 8192-entry angle table
         (B, 8192) FP64 on the device: the formulas of radius_at, evaluated where the coverage
         reads them."""
-        q = self._columns(thetas)
+        q = self._columns(thetas) if columns is None else columns
         if not hasattr(self, "_table_d"):
             self._table_d = torch.from_numpy(self.table_angles).to(self.device)[None, :]
         ang = self._table_d
@@ -196,9 +205,9 @@ class OutlineTemplate:
     def edge_table(self, theta, delta=0.0) -> torch.Tensor:
         return self.edge_tables(theta, delta)[0]
 
-    def indicator_batch(self, thetas, delta=0.0) -> torch.Tensor:
+    def indicator_batch(self, thetas, delta=0.0, columns=None) -> torch.Tensor:
         """Fine-pixel coverage of each outline moved `delta` px along its normal (B, N, N)."""
-        edge = self.edge_tables(thetas, delta).contiguous()
+        edge = self.edge_tables(thetas, delta, columns).contiguous()
         nb = int(edge.shape[0])
         out = torch.empty((nb, self.N, self.N), dtype=torch.float64, device=self.device)
         _lib.call("spx_outline_coverage", _ptr(edge), _ptr(self.rho_d), _ptr(self.ang_d), self.N, self.aa,
@@ -210,22 +219,19 @@ class OutlineTemplate:
         return self.indicator_batch(theta, delta)[0]
 
     # blur
-    def otf_batch(self, thetas) -> torch.Tensor:
+    def otf_batch(self, thetas, columns=None) -> torch.Tensor:
         """Blur transfer functions (B, N, N) of a batch of thetas."""
-        bp = np.atleast_2d(np.asarray(thetas, float))[:, self.n_shape:]
-        col = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to(self.device)[:, None, None]  # noqa: E731
+        q = self._columns(thetas) if columns is None else columns
+        bl = [v[:, :, None] for v in q["blur"]]
         fx2, fy2 = (self.fx * self.fx)[None], (self.fy * self.fy)[None]
         k = -2.0 * math.pi ** 2
         if self.blur == "gauss2":
-            s = np.exp(bp[:, :4])
-            w2 = col(1.0 / (1.0 + np.exp(-bp[:, 4])))
-            g1 = torch.exp(k * (col(s[:, 0] ** 2) * fx2 + col(s[:, 1] ** 2) * fy2))
-            g2 = torch.exp(k * (col(s[:, 2] ** 2) * fx2 + col(s[:, 3] ** 2) * fy2))
-            return (1.0 - w2) * g1 + w2 * g2
-        s = np.exp(bp[:, :2])
-        g = torch.exp(k * (col(s[:, 0] ** 2) * fx2 + col(s[:, 1] ** 2) * fy2))
+            g1 = torch.exp(k * (bl[0] * fx2 + bl[1] * fy2))
+            g2 = torch.exp(k * (bl[2] * fx2 + bl[3] * fy2))
+            return (1.0 - bl[4]) * g1 + bl[4] * g2
+        g = torch.exp(k * (bl[0] * fx2 + bl[1] * fy2))
         if self.blur == "lens":
-            sc = torch.clamp(self.nu[None] / col(np.exp(bp[:, 2])), 0.0, 1.0)
+            sc = torch.clamp(self.nu[None] / bl[2], 0.0, 1.0)
             g = g * (2.0 / math.pi) * (torch.acos(sc) - sc * torch.sqrt(1.0 - sc * sc))
         return g
 
@@ -257,8 +263,9 @@ class OutlineTemplate:
         """T of each theta on the device (B, N, N) FP64: coverage -> spectrum x sharp-edge
         correction x blur x pixel box [x an extra box of box_px] -> back (model.py:200-205)."""
         dims = (-2, -1)
-        spec = torch.fft.fft2(torch.fft.ifftshift(self.indicator_batch(thetas, delta), dim=dims))
-        factor = (self.edge_fix * self.pixel_otf)[None] * self.otf_batch(thetas)
+        q = self._columns(thetas)
+        spec = torch.fft.fft2(torch.fft.ifftshift(self.indicator_batch(thetas, delta, q), dim=dims))
+        factor = (self.edge_fix * self.pixel_otf)[None] * self.otf_batch(thetas, q)
         if box_px > 0:
             factor = factor * (torch.sinc(self.fx * box_px) * torch.sinc(self.fy * box_px))[None]
         return torch.fft.fftshift(torch.fft.ifft2(spec * factor).real, dim=dims)
@@ -655,17 +662,27 @@ class OutlinePage:
         return page.reshape(self.h, self.w) + self.far_field()
 
     # ------------------------------------------------------------------------------------ start
-    def plain_stack(self) -> np.ndarray:
+    def _stack(self, *, cleaned, page=None, good=None, radius_px=None):
+        """(sums, wsum) (N*N) host arrays of the plain or neighbour-cleaned stack (spx_outline_stack)."""
         t = self.template
-        win = self.windows(outputs=("data", "bin"))
-        b = win["bin"].reshape(-1).long()
-        ok = b >= 0
         nn = t.N * t.N
-        sums = torch.zeros(nn, dtype=torch.float64, device=self.device).index_add_(0, b[ok], win["data"].reshape(-1)[ok])
-        cnt = torch.zeros(nn, dtype=torch.float64, device=self.device).index_add_(
-            0, b[ok], torch.ones_like(win["data"].reshape(-1)[ok]))
+        acc = torch.zeros((2, nn), dtype=torch.float64, device=self.device)
+        g = None if good is None else torch.from_numpy(np.asarray(good, dtype=np.uint8)).to(self.device)
+        v0, v1 = self.var_line
+        _lib.call("spx_outline_stack", _ptr(self.img), self.h, self.w, self.w, _ptr(page),
+                  _ptr(self.tmpl) if cleaned else None, t.N, t.U, self.R, _ptr(self.all_idx), self.n, _ptr(g),
+                  _ptr(self.c), _ptr(self.a), _ptr(self.level), _ptr(self.delta), float(v0), float(v1),
+                  0.0 if radius_px is None else float(radius_px) * t.U, int(bool(cleaned)), _ptr(acc[0]), _ptr(acc[1]),
+                  stream_handle())
+        acc = acc.cpu().numpy()
+        return acc[0], acc[1]
+
+    def plain_stack(self) -> np.ndarray:
+        """Mean grey level per fine bin of every window pixel's offset from its hole (page.py:488-505)."""
+        t = self.template
+        sums, cnt = self._stack(cleaned=False)
         with np.errstate(invalid="ignore", divide="ignore"):
-            return (sums.cpu().numpy() / cnt.cpu().numpy()).reshape(t.N, t.N)
+            return (sums / cnt).reshape(t.N, t.N)
 
     def initialise(self):
         """Round start from the plain stack's half height; brightness and level by a 2 x 2 least
@@ -850,19 +867,7 @@ class OutlinePage:
         """(data - page + own - level) / a binned at each pixel's fine offset from its hole,
         weighted by a^2 / variance; (stack, weight) (N, N) host arrays (page.py:732-766)."""
         t = self.template
-        win = self.windows(page=page, outputs=("data", "model", "own", "bin"),
-                           radius_fine=0.0 if radius_px is None else float(radius_px) * t.U)
-        a = self.a[:, None]
-        v0, v1 = self.var_line
-        cleaned = (win["data"] - win["model"] + win["own"]) / a
-        wgt = a * a / (v0 + v1 * torch.clamp(win["model"], min=0.0))
-        g = torch.from_numpy(np.asarray(good, dtype=bool)).to(self.device)
-        sel = (win["bin"] >= 0) & g[:, None]
-        b = win["bin"][sel].long()
-        nn = t.N * t.N
-        sums = torch.zeros(nn, dtype=torch.float64, device=self.device).index_add_(0, b, (cleaned * wgt)[sel])
-        wsum = torch.zeros(nn, dtype=torch.float64, device=self.device).index_add_(0, b, wgt[sel])
-        sums, wsum = sums.cpu().numpy(), wsum.cpu().numpy()
+        sums, wsum = self._stack(cleaned=True, page=page.reshape(-1).contiguous(), good=good, radius_px=radius_px)
         with np.errstate(invalid="ignore", divide="ignore"):
             stack = np.where(wsum > 0, sums / wsum, np.nan)
         return stack.reshape(t.N, t.N), wsum.reshape(t.N, t.N)

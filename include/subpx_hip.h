@@ -421,6 +421,19 @@ int spx_outline_windows(const float* image, int64_t H, int64_t W, int64_t pitch,
                         double radius_fine, int32_t* bin, uint8_t* valid, double* data, double* model, double* own,
                         double* v0, spx_stream_t stream);
 
+/* Stacked average hole on the fine grid (page.py:488-505 plain, 732-766 neighbour-cleaned): every
+ * window pixel of the holes idx[0..n) with good[h] != 0 (good NULL: all) inside the frame adds to
+ * bin rint(sy) N + rint(sx) of its fine offset (inside the grid; within radius_fine fine units of
+ * the centre when radius_fine > 0): cleaned = 0: sums += grey level, wsum += 1; cleaned = 1:
+ * sums += w (data - page - level + own) / a, wsum += w with w = a^2 / (nv0 + nv1 max(page +
+ * level, 0)) and own = a (T0 + d T1 + d^2 T2). sums, wsum (N x N FP64) accumulate (zero them
+ * first); FP64 atomics, so the sums' last bits depend on the order. */
+int spx_outline_stack(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page, const float* tmpl,
+                      int32_t N, int32_t U, int32_t R, const int32_t* idx, int32_t n, const uint8_t* good,
+                      const double* c, const double* a, const double* level, const double* delta, double nv0,
+                      double nv1, double radius_fine, int32_t cleaned, double* sums, double* wsum,
+                      spx_stream_t stream);
+
 /* Keys splat of a_i, a_i d_i and a_i d_i^2 at c_i + (mx, my) on three padded FP64 grids
  * (grid + k hp wp, k = 0, 1, 2; holes whose taps leave the grid are dropped): the far light is
  * the sum of the three convolved with the far kernels of T0, T1 and T2 (page.py:341-397). */
