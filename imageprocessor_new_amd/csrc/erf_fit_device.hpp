@@ -652,24 +652,39 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
                 masked = masked || (id >= 0 && id < a.nfeat);
             }
         // Gather: lane-strided over the flattened patch, so each load instruction reads a
-        // few contiguous row segments of the image.
+        // few contiguous row segments of the image. Every lane loads from a clamped in-image
+        // address, so the loads carry no branch and issue back to back (one wait, not one per
+        // slot); row and column advance by the uniform quotient and remainder of 64 / side
+        // instead of a division per slot.
         Patch<NPL> pt;
         pt.lds = lds_words;
         pt.nslots = (npx + kWave - 1) / kWave;
         pt.valid = 0ull;
         float cnt = 0.0f;
-#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
-        for (int n = 0; n < NPL; ++n) {
-            const int pix = lane + kWave * n;
-            const int row = pix / side;
-            const int col = pix - row * side;
-            const int64_t gx = ox + col;
-            const int64_t gy = oy + row;
-            const bool ok = pix < npx && gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
-            const uint32_t v = ok ? (uint32_t)a.img[gy * a.pitch + gx] : 0u;
-            pt.set(n, pack_px(ok ? col : 0, ok ? row : 0, v));
-            pt.valid |= ok ? (1ull << n) : 0ull;
-            cnt += ok ? 1.0f : 0.0f;
+        {
+            const int dq = kWave / side, dr = kWave - dq * side;
+            int row = lane / side;
+            int col = lane - row * side;
+            const int64_t W1 = a.W - 1, H1 = a.H - 1;
+#pragma unroll(Patch<NPL>::kLds ? 8 : NPL)
+            for (int n = 0; n < NPL; ++n) {
+                const int pix = lane + kWave * n;
+                const int64_t gx = ox + col, gy = oy + row;
+                const bool ok = pix < npx && gx >= 0 && gx <= W1 && gy >= 0 && gy <= H1;
+                const int64_t qx = gx < 0 ? 0 : (gx > W1 ? W1 : gx);
+                const int64_t qy = gy < 0 ? 0 : (gy > H1 ? H1 : gy);
+                const uint32_t v = (uint32_t)a.img[qy * a.pitch + qx];
+                // masked by arithmetic, not a select, so the load stays unconditional
+                pt.set(n, pack_px(col, row, v) & (0u - (uint32_t)ok));
+                pt.valid |= ok ? (1ull << n) : 0ull;
+                cnt += ok ? 1.0f : 0.0f;
+                col += dr;
+                row += dq;
+                if (col >= side) {
+                    col -= side;
+                    ++row;
+                }
+            }
         }
         if (masked) {
             // restrict to the Voronoi cell (the cell's 64 FP64 registers live only here)
@@ -678,9 +693,8 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
             cnt = 0.0f;
 #pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
             for (int n = 0; n < NPL; ++n) {
-                const int pix = lane + kWave * n;
-                const int row = pix / side;
-                const int col = pix - row * side;
+                const uint32_t wd = pt.word(n);
+                const int col = (int)(wd & 0xffu), row = (int)((wd >> 8) & 0xffu);
                 const bool in = ((pt.valid >> n) & 1ull) && cell_in(cell, (double)(ox + col), (double)(oy + row));
                 if (!in) pt.valid &= ~(1ull << n);
                 cnt += in ? 1.0f : 0.0f;
