@@ -292,22 +292,26 @@ SPX_DEV_INLINE bool damped_solve(const float (&S)[Layout<P>::kN], double npix, d
         const double d = T[a * (a + 1) / 2 + a];
         T[a * (a + 1) / 2 + a] = d + lambda * fmax(d, floor_d);
     }
-    double D[P];
+    // LDL^T keeping both L (below the diagonal of T) and W = L D (the values before the
+    // division by the pivot), so every elimination term is one FMA; the pivots' reciprocals
+    // serve the back substitution too.
+    double D[P], iD[P], Wm[kT];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const double mjj = T[j * (j + 1) / 2 + j];
         double d = mjj;
 #pragma unroll
-        for (int kk = 0; kk < j; ++kk) d -= T[j * (j + 1) / 2 + kk] * T[j * (j + 1) / 2 + kk] * D[kk];
+        for (int kk = 0; kk < j; ++kk) d = fma(-T[j * (j + 1) / 2 + kk], Wm[j * (j + 1) / 2 + kk], d);
         if (!(d > 1e-13 * mjj) || !(d > 0.0)) return false;
         D[j] = d;
         const double inv = hw_rcp64(d);
+        iD[j] = inv;
 #pragma unroll
         for (int i = j + 1; i < P; ++i) {
             double sacc = T[i * (i + 1) / 2 + j];
 #pragma unroll
-            for (int kk = 0; kk < j; ++kk)
-                sacc -= T[i * (i + 1) / 2 + kk] * T[j * (j + 1) / 2 + kk] * D[kk];
+            for (int kk = 0; kk < j; ++kk) sacc = fma(-T[i * (i + 1) / 2 + kk], Wm[j * (j + 1) / 2 + kk], sacc);
+            Wm[i * (i + 1) / 2 + j] = sacc;
             T[i * (i + 1) / 2 + j] = sacc * inv;
         }
     }
@@ -316,14 +320,14 @@ SPX_DEV_INLINE bool damped_solve(const float (&S)[Layout<P>::kN], double npix, d
     for (int i = 0; i < P; ++i) {
         double sacc = (double)S[L::kRhs + i];
 #pragma unroll
-        for (int kk = 0; kk < i; ++kk) sacc -= T[i * (i + 1) / 2 + kk] * y[kk];
+        for (int kk = 0; kk < i; ++kk) sacc = fma(-T[i * (i + 1) / 2 + kk], y[kk], sacc);
         y[i] = sacc;
     }
 #pragma unroll
     for (int i = P - 1; i >= 0; --i) {
-        double sacc = y[i] * hw_rcp64(D[i]);
+        double sacc = y[i] * iD[i];
 #pragma unroll
-        for (int kk = i + 1; kk < P; ++kk) sacc -= T[kk * (kk + 1) / 2 + i] * delta[kk];
+        for (int kk = i + 1; kk < P; ++kk) sacc = fma(-T[kk * (kk + 1) / 2 + i], delta[kk], sacc);
         delta[i] = sacc;
     }
     bool finite = true;
