@@ -673,8 +673,10 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
 #pragma unroll(Patch<NPL>::kLds ? 8 : NPL)
             for (int n = 0; n < NPL; ++n) {
                 const int pix = lane + kWave * n;
-                const int64_t gx = ox + col, gy = oy + row;
-                const bool ok = pix < npx && gx >= 0 && gx <= W1 && gy >= 0 && gy <= H1;
+                const bool in_patch = pix < npx;
+                // lanes past the patch's end read its first row (lines the wave fetches anyway)
+                const int64_t gx = ox + col, gy = oy + (in_patch ? row : 0);
+                const bool ok = in_patch && gx >= 0 && gx <= W1 && gy >= 0 && gy <= H1;
                 const int64_t qx = gx < 0 ? 0 : (gx > W1 ? W1 : gx);
                 const int64_t qy = gy < 0 ? 0 : (gy > H1 ? H1 : gy);
                 const uint32_t v = (uint32_t)a.img[qy * a.pitch + qx];
