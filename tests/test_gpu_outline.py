@@ -237,3 +237,46 @@ def test_fit_outline_reads_the_size_at_slope_one():
     ref = fit_outline(page0, K["seeds"], rounds=2)
     assert inv.outline_cd_px == pytest.approx(ref.outline_cd_px, abs=1e-4)
     np.testing.assert_allclose(inv.level, page0.max() - ref.level, atol=1e-3)
+
+
+def test_fit_outline_lens_blur_reads_its_outline():
+    """The second blur family (Gaussian x circular pupil): a page drawn with it is read back."""
+    from imageprocessor_new_amd import fit_outline
+
+    rng = np.random.default_rng(31)
+    t = O.Template(upsample=8, size_px=48, blur="lens")
+    th = t.pack(2.5, 2.3, 2.5, -0.15, [[0.02, 0.01], [0.0, 0.01]], t.blur_pack(sx=0.9, sy=0.85, cutoff=0.55))
+    X, Y = np.meshgrid(np.arange(7, 170, 11.0), np.arange(7, 120, 11.0))
+    c = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.4, 0.4, (X.size, 2))
+    a = rng.uniform(900, 1100, len(c))
+    page = O.render_model_page(t, th, c, a, 100.0, np.zeros(len(c)), (120, 170), noise=3.0, seed=4)
+    fit = fit_outline(page.astype(np.float32), c + rng.uniform(-0.4, 0.4, c.shape), blur="lens", rounds=6)
+    assert fit.blur == "lens" and fit.template.shape == (384, 384)
+    assert fit.outline_cd_px == pytest.approx(t.cd_px(th), abs=0.05)
+    truth_xy = c + np.array(t.centroid_px(th))
+    ok = fit.good
+    assert np.sqrt(np.mean(np.sum((fit.centers_xy[ok] - truth_xy[ok]) ** 2, axis=1))) < 0.03
+
+
+def test_fit_outline_with_holes_cut_by_the_frame():
+    """Seeds on and beyond the frame (their light is part of the page model): every result is
+    finite and the interior holes read the outline."""
+    from imageprocessor_new_amd import fit_outline
+
+    rng = np.random.default_rng(8)
+    t = O.Template(upsample=8, size_px=24)
+    th = t.pack(2.4, 2.4, 2.0, 0.0, np.zeros((2, 2)), t.blur_pack(s1x=1.1, s1y=1.1, s2x=2.2, s2y=2.2, w2=0.05))
+    X, Y = np.meshgrid(np.arange(-2, 125, 10.0), np.arange(-3, 95, 10.0))
+    c = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.3, 0.3, (X.size, 2))
+    a = rng.uniform(950, 1050, len(c))
+    page = O.render_model_page(t, th, c, a, 120.0, np.zeros(len(c)), (90, 120), noise=3.0, seed=2)
+    fit = fit_outline(page.astype(np.float32), c + rng.uniform(-0.3, 0.3, c.shape), rounds=4)
+    assert np.isfinite(fit.centers_xy).all() and np.isfinite(fit.cd_px).all() and np.isfinite(fit.level).all()
+    assert fit.rays_px.shape == (len(c), 128)
+    assert fit.outline_cd_px == pytest.approx(t.cd_px(th), abs=0.05)
+    inner = (c[:, 0] > 12) & (c[:, 0] < 108) & (c[:, 1] > 12) & (c[:, 1] < 78)
+    truth_xy = c + np.array(t.centroid_px(th))
+    assert np.abs(fit.centers_xy[inner] - truth_xy[inner]).max() < 0.05
+    # holes whose windows leave the frame (window half width 8 px here) are never stacked
+    off = (truth_xy[:, 0] < 8) | (truth_xy[:, 0] > 111) | (truth_xy[:, 1] < 8) | (truth_xy[:, 1] > 81)
+    assert off.sum() > 10 and not fit.good[off].any() and fit.good[inner].mean() > 0.9
