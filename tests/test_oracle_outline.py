@@ -169,3 +169,15 @@ def test_fit_outline_checks_its_input():
         fit_outline(np.zeros((0, 4)), seeds)
     with pytest.raises(ValueError, match="pixel_size"):
         fit_outline(img, seeds, pixel_size=0.0)
+
+
+def test_product_greedy_colours_equal_the_oracle_rule():
+    from imageprocessor_new_amd.outline import greedy_colours
+
+    rng = np.random.default_rng(12)
+    for n, span, reach in ((400, 120.0, 15.0), (1000, 300.0, 12.5)):
+        c = rng.uniform(0, span, (n, 2))
+        assert np.array_equal(greedy_colours(c, reach), O.greedy_colours(c, reach))
+    X, Y = np.meshgrid(np.arange(20) * 8.5, np.arange(15) * 8.5)
+    c = np.stack([X.ravel(), Y.ravel()], 1) + rng.normal(0, 0.05, (300, 2))
+    assert np.array_equal(greedy_colours(c, 1.5 * 8.5), O.greedy_colours(c, 1.5 * 8.5))
