@@ -270,7 +270,7 @@ struct RenderArgs {
 };
 
 // Offsets of the render window inside the light's reach, ox^2 + oy^2 <= rl2, row by row: row
-// oy holds ox in [-w, w] with w the largest such |ox| (none when oy^2 > rl2).
+// oy holds ox in [-w, w] with w the largest such |ox|; w = -1 (no offset) when oy^2 > rl2.
 __host__ __device__ inline int reach_half_width(int oy, int Rr, double rl2) {
     if ((double)(oy * oy) > rl2) return -1;
     int w = 0;
@@ -283,13 +283,14 @@ __global__ __launch_bounds__(256) void hole_render_kernel(RenderArgs A, int Pn) 
     __shared__ int row_start[512];
     __shared__ int row_w[512];
     const int side = 2 * A.Rr + 1;
+    // the rows' half widths in parallel, then their running sum by one thread
+    for (int r = threadIdx.x; r < side; r += blockDim.x) row_w[r] = reach_half_width(r - A.Rr, A.Rr, A.rl2);
+    __syncthreads();
     if (threadIdx.x == 0) {
         int acc = 0;
         for (int r = 0; r < side; ++r) {
-            const int w = reach_half_width(r - A.Rr, A.Rr, A.rl2);
             row_start[r] = acc;
-            row_w[r] = w;
-            acc += 2 * w + 1;
+            acc += row_w[r] >= 0 ? 2 * row_w[r] + 1 : 0;  // a row beyond the reach holds no offset
         }
         row_start[side] = acc;
     }
@@ -729,7 +730,10 @@ int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const f
     if (n == 0) return SPX_OK;
     RenderArgs A{page, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta};
     int Pn = 0;
-    for (int oy = -Rr; oy <= Rr; ++oy) Pn += 2 * reach_half_width(oy, Rr, rl2) + 1;
+    for (int oy = -Rr; oy <= Rr; ++oy) {
+        const int w = reach_half_width(oy, Rr, rl2);
+        if (w >= 0) Pn += 2 * w + 1;
+    }
     if (Pn == 0) return SPX_OK;
     const int64_t total = (int64_t)n * Pn;
     if (total >= ((int64_t)1 << 32)) {

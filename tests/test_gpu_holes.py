@@ -185,3 +185,39 @@ def test_select_kth_matches_sorting():
     assert pg_stats[0] == np.median(np.abs(x[:, 2]))
     assert pg_stats[1] == np.percentile(np.abs(x[:, 0]), 99)
     assert pg_stats[2] == np.percentile(np.abs(x[:, 1]), 99)
+
+
+def test_render_covers_every_offset_of_the_reach():
+    """One hole's light over every offset inside its reach (ox^2 + oy^2 <= rl2, the reach's edge
+    rows included) against the oracle's render on the same FP32 template values: the same
+    offsets are lit, and the cached light agrees."""
+    from imageprocessor_new_amd import _lib, holes
+    from imageprocessor_new_amd.backends import gpu_device, stream_handle
+
+    h = w = 64
+    for x, y in ((31.49, 30.51), (31.51, 30.49), (32.0, 32.0), (31.3, 31.7)):
+        c = np.array([[x, y], [5.0, 5.0]])
+        pg = holes.HolePage(np.zeros((h, w), np.float32), c, noise=1.0, brightness=[1000.0, 1000.0])
+        pg.set_template(2.0, 2.5)
+        N, U, Rr = pg.tN, pg.tU, pg.Rr
+        rl2 = float((pg.r_light + 0.75) ** 2)
+        P = (2 * Rr + 1) ** 2
+        with gpu_device(0) as dev:
+            page = torch.zeros(h * w, dtype=torch.float64, device=dev)
+            cache = torch.zeros(P, dtype=torch.float32, device=dev)
+            idx = torch.zeros(1, dtype=torch.int32, device=dev)
+            c_old = pg.c[:1].clone()
+            _lib.call("spx_hole_render", int(page.data_ptr()), h, w, int(cache.data_ptr()), int(pg.tmpl.data_ptr()), N,
+                      U, Rr, rl2, int(idx.data_ptr()), 1, int(c_old.data_ptr()), int(pg.c.data_ptr()),
+                      int(pg.a.data_ptr()), int(pg.delta.data_ptr()), stream_handle())
+            got_cache = cache.cpu().numpy()
+            got_page = page.cpu().numpy().reshape(h, w)
+        g3 = pg.tmpl.cpu().numpy().reshape(N, N, 8).astype(np.float64)[..., [0, 1, 2]]
+        ref_page = np.zeros((h, w))
+        ref_cache = np.zeros((1, P), dtype=np.float32)
+        O.render_scatter(ref_page, ref_cache, g3, np.array([0]), c[:1].copy(), c.copy(), np.array([1000.0, 1000.0]),
+                         np.zeros(2), U, Rr, rl2)
+        ref_cache = ref_cache[0]
+        assert np.array_equal(got_cache != 0, ref_cache != 0), (x, y)
+        assert np.all(np.abs(got_cache - ref_cache) <= 1e-6 * np.abs(ref_cache).max() + 1e-3 * np.abs(ref_cache))
+        assert np.abs(got_page - ref_page).max() <= 1e-6 * np.abs(ref_page).max()
