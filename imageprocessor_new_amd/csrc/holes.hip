@@ -598,26 +598,38 @@ __global__ __launch_bounds__(1024) void select_kth_kernel(const double* x, int64
         for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
         __syncthreads();
         const uint64_t prefix = s_prefix;
-        for (int64_t base = 0; base < n; base += blockDim.x) {
-            const int64_t i = base + threadIdx.x;
-            bool part = false;
-            unsigned int d = 0;
-            if (i < n) {
-                const uint64_t key = order_key(xc[i * stride]);
-                if ((key & mask) == prefix) {
-                    part = true;
-                    d = (unsigned int)((key >> shift) & 255u);
-                }
+        // eight values per thread per step, their loads issued together (clamped indices keep
+        // them unconditional), then binned
+        constexpr int kU = 8;
+        for (int64_t base = 0; base < n; base += (int64_t)blockDim.x * kU) {
+            double vals[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+                vals[u] = xc[(i < n ? i : n - 1) * stride];
             }
-            // one LDS atomic per distinct digit of the wave (the high digits are mostly equal)
-            uint64_t pending = __ballot(part);
-            while (pending) {
-                const int leader = __ffsll((unsigned long long)pending) - 1;
-                const unsigned int dl = __shfl(d, leader, 64);
-                const uint64_t same = __ballot(part && d == dl);
-                if (lane == leader) atomicAdd(&hist[dl], (unsigned int)__popcll(same));
-                if (part && d == dl) part = false;
-                pending &= ~same;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int64_t i = base + (int64_t)u * blockDim.x + threadIdx.x;
+                bool part = false;
+                unsigned int d = 0;
+                if (i < n) {
+                    const uint64_t key = order_key(vals[u]);
+                    if ((key & mask) == prefix) {
+                        part = true;
+                        d = (unsigned int)((key >> shift) & 255u);
+                    }
+                }
+                // one LDS atomic per distinct digit of the wave (the high digits are mostly equal)
+                uint64_t pending = __ballot(part);
+                while (pending) {
+                    const int leader = __ffsll((unsigned long long)pending) - 1;
+                    const unsigned int dl = __shfl(d, leader, 64);
+                    const uint64_t same = __ballot(part && d == dl);
+                    if (lane == leader) atomicAdd(&hist[dl], (unsigned int)__popcll(same));
+                    if (part && d == dl) part = false;
+                    pending &= ~same;
+                }
             }
         }
         __syncthreads();
