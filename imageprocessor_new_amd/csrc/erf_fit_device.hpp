@@ -368,53 +368,109 @@ struct FitArgs {
 // Absent neighbours get a = b = 0, c = 1e300 (never decisive).
 constexpr int kMaxNb = 8;
 
+// The exact rules in FP64 without contraction and in the oracle's operation order, from the
+// seeds in memory: a_j = 2 (s0x - s_jx), b_j = 2 (s0y - s_jy), c_j = (s_jx^2 + s_jy^2) -
+// (s0x^2 + s0y^2), f_j = (a_j qx + b_j qy) + c_j; inside when every f_j > 0 (or = 0 towards a
+// higher index); on the boundary when some g_j = f_j - max(|a_j|, |b_j|) < 0 (or = 0 towards a
+// lower index). Absent neighbours (id < 0 or >= K) never decide.
+SPX_DEV_INLINE bool cell_exact(const FitArgs& a, int f, double qx, double qy, bool edge) {
+#pragma clang fp contract(off)
+    const double x0 = (double)a.seeds[2 * (int64_t)f], y0 = (double)a.seeds[2 * (int64_t)f + 1];
+    const double n0 = x0 * x0 + y0 * y0;
+    bool in = true, e = false;
+    for (int j = 0; j < a.nb && j < kMaxNb; ++j) {
+        const int id = a.nbr[(int64_t)f * a.nb + j];
+        if (id < 0 || id >= a.nfeat) continue;
+        const double xj = (double)a.seeds[2 * (int64_t)id], yj = (double)a.seeds[2 * (int64_t)id + 1];
+        const double aj = 2.0 * (x0 - xj), bj = 2.0 * (y0 - yj);
+        const double fj = (aj * qx + bj * qy) + ((xj * xj + yj * yj) - n0);
+        const bool lower = id < f;
+        in = in && (fj > 0.0 || (fj == 0.0 && !lower));
+        const double gj = fj - fmax(fabs(aj), fabs(bj));
+        e = e || gj < 0.0 || (gj == 0.0 && lower);
+    }
+    return edge ? e : in;
+}
+
+// A first look in FP32 with the same forms centred on the seed: f_j = u_j x' + v_j y' + w_j
+// with (x', y') = q - s0, u_j = a_j, v_j = b_j, w_j = |s_j - s0|^2 (g_j subtracts
+// max(|u_j|, |v_j|)). Their rounding error is below `margin`; a pixel whose every |f_j| (or
+// |g_j|) exceeds it takes the FP32 decision (the FP64 form, off the exact value by ~1e-6 there,
+// has the same sign); any other pixel takes the exact rule above.
 struct CellForm {
-    double a[kMaxNb], b[kMaxNb], c[kMaxNb], m[kMaxNb];
-    bool lower[kMaxNb];
+    float u[kMaxNb], v[kMaxNb], w[kMaxNb];
+    float x0, y0, margin;
+    uint32_t act;  // bit j: neighbour j present
 };
 
+SPX_DEV_INLINE void make_cell(float x0, float y0, const float (&xs)[kMaxNb], const float (&ys)[kMaxNb],
+                              const bool (&active)[kMaxNb], CellForm& cf) {
+    cf.x0 = x0;
+    cf.y0 = y0;
+    cf.act = 0u;
+    double mg = 1e-3;
+#pragma unroll
+    for (int j = 0; j < kMaxNb; ++j) {
+        const bool act = active[j];
+        cf.act |= act ? (1u << j) : 0u;
+        const double xj = act ? (double)xs[j] : (double)x0, yj = act ? (double)ys[j] : (double)y0;
+        const double aj = 2.0 * ((double)x0 - xj), bj = 2.0 * ((double)y0 - yj);
+        const double dx = xj - (double)x0, dy = yj - (double)y0;
+        cf.u[j] = (float)aj;
+        cf.v[j] = (float)bj;
+        cf.w[j] = (float)(dx * dx + dy * dy);
+        // |x'|, |y'| <= 64 for any patch; FP32 rounding of u, v, w, x', y' and of the two FMAs
+        const double mj = 1e-5 * (fabs(aj) + fabs(bj) + 1.0) * 64.0 + 1e-6 * (dx * dx + dy * dy) + 1e-3;
+        mg = act ? fmax(mg, mj) : mg;
+    }
+    cf.margin = (float)mg;
+}
+
 SPX_DEV_INLINE void load_cell(const FitArgs& a, int f, CellForm& cf) {
-#pragma clang fp contract(off)
-    const double x0 = (double)spx_uniform(a.seeds[2 * (int64_t)f]);
-    const double y0 = (double)spx_uniform(a.seeds[2 * (int64_t)f + 1]);
-    const double n0 = x0 * x0 + y0 * y0;
+    const float x0 = spx_uniform(a.seeds[2 * (int64_t)f]);
+    const float y0 = spx_uniform(a.seeds[2 * (int64_t)f + 1]);
+    float xs[kMaxNb], ys[kMaxNb];
+    bool act[kMaxNb];
 #pragma unroll
     for (int j = 0; j < kMaxNb; ++j) {
         int id = -1;
         if (j < a.nb) id = spx_uniform_i(a.nbr[(int64_t)f * a.nb + j]);
-        const bool act = id >= 0 && id < a.nfeat;
-        double xj = 0.0, yj = 0.0;
-        if (act) {
-            xj = (double)spx_uniform(a.seeds[2 * (int64_t)id]);
-            yj = (double)spx_uniform(a.seeds[2 * (int64_t)id + 1]);
+        act[j] = id >= 0 && id < a.nfeat;
+        xs[j] = 0.0f;
+        ys[j] = 0.0f;
+        if (act[j]) {
+            xs[j] = spx_uniform(a.seeds[2 * (int64_t)id]);
+            ys[j] = spx_uniform(a.seeds[2 * (int64_t)id + 1]);
         }
-        cf.a[j] = act ? 2.0 * (x0 - xj) : 0.0;
-        cf.b[j] = act ? 2.0 * (y0 - yj) : 0.0;
-        cf.c[j] = act ? (xj * xj + yj * yj) - n0 : 1e300;
-        cf.m[j] = fmax(fabs(cf.a[j]), fabs(cf.b[j]));
-        cf.lower[j] = act && id < f;
     }
+    make_cell(x0, y0, xs, ys, act, cf);
 }
 
-SPX_DEV_INLINE bool cell_in(const CellForm& cf, double qx, double qy) {
-#pragma clang fp contract(off)
-    bool in = true;
+SPX_DEV_INLINE bool cell_in_fast(const FitArgs& a, int f, const CellForm& cf, double qx, double qy) {
+    const float px = (float)(qx - (double)cf.x0), py = (float)(qy - (double)cf.y0);
+    bool in = true, sure = true;
 #pragma unroll
     for (int j = 0; j < kMaxNb; ++j) {
-        const double fj = (cf.a[j] * qx + cf.b[j] * qy) + cf.c[j];
-        in = in && (fj > 0.0 || (fj == 0.0 && !cf.lower[j]));
+        if (!((cf.act >> j) & 1u)) continue;
+        const float fj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j]));
+        in = in && fj > 0.0f;
+        sure = sure && fabsf(fj) > cf.margin;
     }
+    if (!sure) in = cell_exact(a, f, qx, qy, false);
     return in;
 }
 
-SPX_DEV_INLINE bool cell_edge(const CellForm& cf, double qx, double qy) {
-#pragma clang fp contract(off)
-    bool e = false;
+SPX_DEV_INLINE bool cell_edge_fast(const FitArgs& a, int f, const CellForm& cf, double qx, double qy) {
+    const float px = (float)(qx - (double)cf.x0), py = (float)(qy - (double)cf.y0);
+    bool e = false, sure = true;
 #pragma unroll
     for (int j = 0; j < kMaxNb; ++j) {
-        const double gj = ((cf.a[j] * qx + cf.b[j] * qy) + cf.c[j]) - cf.m[j];
-        e = e || gj < 0.0 || (gj == 0.0 && cf.lower[j]);
+        if (!((cf.act >> j) & 1u)) continue;
+        const float gj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j])) - fmaxf(fabsf(cf.u[j]), fabsf(cf.v[j]));
+        e = e || gj < 0.0f;
+        sure = sure && fabsf(gj) > cf.margin;
     }
+    if (!sure) e = cell_exact(a, f, qx, qy, true);
     return e;
 }
 
@@ -476,7 +532,7 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
                         const int col = (int)(w & 0xffu), row = (int)((w >> 8) & 0xffu);
                         const float val = (float)(w >> 16);
                         bool ring = row == 0 || row == side - 1 || col == 0 || col == side - 1;
-                        if (!ring && masked) ring = cell_edge(cell, ox + (double)col, oy + (double)row);
+                        if (!ring && masked) ring = cell_edge_fast(a, f, cell, ox + (double)col, oy + (double)row);
                         if (ring) {
                             bsum += val;
                             bcnt += 1.0f;
@@ -701,7 +757,7 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
             for (int n = 0; n < NPL; ++n) {
                 const uint32_t wd = pt.word(n);
                 const int col = (int)(wd & 0xffu), row = (int)((wd >> 8) & 0xffu);
-                const bool in = ((pt.valid >> n) & 1ull) && cell_in(cell, (double)(ox + col), (double)(oy + row));
+                const bool in = ((pt.valid >> n) & 1ull) && cell_in_fast(a, f, cell, (double)(ox + col), (double)(oy + row));
                 if (!in) pt.valid &= ~(1ull << n);
                 cnt += in ? 1.0f : 0.0f;
             }

@@ -89,3 +89,46 @@ def test_fit_device_logic_under_wave_emulation():
                           text=True, timeout=600)
     assert proc.returncode != 3, "lanes diverged in the wave emulation:\n" + proc.stderr
     assert proc.returncode == 0, proc.stdout + proc.stderr
+
+
+@pytest.mark.parametrize("integer_seeds", [True, False])
+def test_fp32_cell_first_look_takes_the_exact_decisions(integer_seeds):
+    """The masked gather and the start's boundary ring decide cell membership with an FP32 form
+    centred on the seed and fall back to the exact FP64 rule within its rounding margin: the
+    decisions equal the FP64 rule at every pixel of every patch, integer seeds (exact ties)
+    and fractional ones, neighbours near and far."""
+    from scipy.spatial import cKDTree
+
+    from oracle import erfdisk
+
+    _build()
+    lib = ctypes.CDLL(str(OUT))
+    rng = np.random.default_rng(7 if integer_seeds else 8)
+    X, Y = np.meshgrid(np.arange(40, 2000, 39.0), np.arange(40, 2000, 39.0 * 0.866))
+    X[1::2] += 19.5
+    seeds = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-6, 6, (X.size, 2))
+    seeds = (np.rint(seeds) if integer_seeds else seeds).astype(np.float32)
+    # some seeds at 16-bit-page coordinates, where the FP64 form's constant is large
+    seeds[:50] += np.float32(30000.0)
+    K = len(seeds)
+    _, nb_idx = cKDTree(seeds).query(seeds, k=9)
+    nbr = nb_idx[:, 1:].astype(np.int32)
+    nbr[::5, 6:] = -1  # absent neighbours
+    off = np.arange(-31, 32)
+    oy, ox = np.meshgrid(off, off, indexing="ij")
+    n = ox.size
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    outs = [np.zeros(n, np.uint8) for _ in range(4)]
+    mism = 0
+    for f in rng.choice(K, 300, replace=False):
+        qx = np.ascontiguousarray(np.rint(seeds[f, 0]) + ox.ravel(), dtype=np.float64)
+        qy = np.ascontiguousarray(np.rint(seeds[f, 1]) + oy.ravel(), dtype=np.float64)
+        row = np.ascontiguousarray(nbr[f])
+        lib.emu_cell_decisions(P(seeds), ctypes.c_int32(K), ctypes.c_int32(int(f)), P(row), ctypes.c_int32(8),
+                               P(qx), P(qy), ctypes.c_int64(n), *[P(o) for o in outs])
+        mism += int((outs[0] != outs[1]).sum() + (outs[2] != outs[3]).sum())
+        assert outs[1].any() and (~outs[1].astype(bool)).any()
+        # the exact rule is the oracle's
+        assert np.array_equal(outs[1].astype(bool), erfdisk.cell_mask(qx, qy, seeds, int(f), row))
+        assert np.array_equal(outs[3].astype(bool), erfdisk.cell_edge(qx, qy, seeds, int(f), row))
+    assert mism == 0

@@ -213,3 +213,36 @@ extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t 
     return emu_fit_batch_cells(img, H, W, pitch, seeds, half, model, init, opt, K, nullptr, 0, params, rms, iters,
                                status, timeout_s);
 }
+
+// Voronoi-cell decisions of one feature f at the points (qx[i], qy[i]): the FP32 first look
+// (cell_in_fast / cell_edge_fast) and the exact FP64 rule (cell_exact), for a test that they
+// agree everywhere. No wave operations: the cell is built by make_cell directly.
+extern "C" void emu_cell_decisions(const float* seeds, int32_t K, int32_t f, const int32_t* nbr, int32_t nb,
+                                   const double* qx, const double* qy, int64_t n, uint8_t* in_fast,
+                                   uint8_t* in_exact, uint8_t* edge_fast, uint8_t* edge_exact) {
+    using namespace spx::fitdev;
+    float xs[kMaxNb], ys[kMaxNb];
+    bool act[kMaxNb];
+    for (int j = 0; j < kMaxNb; ++j) {
+        const int id = j < nb ? nbr[j] : -1;
+        act[j] = id >= 0 && id < K;
+        xs[j] = act[j] ? seeds[2 * id] : 0.0f;
+        ys[j] = act[j] ? seeds[2 * id + 1] : 0.0f;
+    }
+    CellForm cf;
+    make_cell(seeds[2 * f], seeds[2 * f + 1], xs, ys, act, cf);
+    // the row of neighbours stands at row f of a K x nb table as the kernel reads it
+    std::vector<int32_t> table((size_t)K * nb, -1);
+    for (int j = 0; j < nb; ++j) table[(size_t)f * nb + j] = nbr[j];
+    FitArgs a{};
+    a.seeds = seeds;
+    a.nbr = table.data();
+    a.nb = nb;
+    a.nfeat = K;
+    for (int64_t i = 0; i < n; ++i) {
+        in_fast[i] = cell_in_fast(a, f, cf, qx[i], qy[i]);
+        in_exact[i] = cell_exact(a, f, qx[i], qy[i], false);
+        edge_fast[i] = cell_edge_fast(a, f, cf, qx[i], qy[i]);
+        edge_exact[i] = cell_exact(a, f, qx[i], qy[i], true);
+    }
+}
