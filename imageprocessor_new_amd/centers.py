@@ -201,10 +201,11 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
     cap = int(min(max(65536, img_t.shape[0] * img_t.shape[1] // 256), 1 << 24))
     if morph_open > 0 or morph_close > 0:
         mask = binary_mask_device(img_t, thresh, invert=invert, morph_open=morph_open, morph_close=morph_close)
-        num, labels, stats, cents = threshold_components_device(mask, 0, connectivity=connectivity, capacity=cap)
+        num, _, stats, cents = threshold_components_device(mask, 0, connectivity=connectivity, capacity=cap,
+                                                           labels_out=False)
     else:
-        num, labels, stats, cents = threshold_components_device(img_t, thresh, invert=invert,
-                                                                connectivity=connectivity, capacity=cap)
+        num, _, stats, cents = threshold_components_device(img_t, thresh, invert=invert, connectivity=connectivity,
+                                                           capacity=cap, labels_out=False)
     e0 = mark("components", e0)
     dev = img_t.device
     area = stats[:, 4]

@@ -64,11 +64,12 @@ def binary_mask_device(img: torch.Tensor, thresh: int, *, invert: bool = False, 
 
 
 def threshold_components_device(img: torch.Tensor, thresh: int, *, invert: bool = False,
-                                connectivity: int = 8, capacity: int | None = None):
+                                connectivity: int = 8, capacity: int | None = None, labels_out: bool = True):
     """Foreground (g > thresh, or g <= thresh with invert) + components.
 
     Returns (num_labels, labels (H, W) int32 device tensor, stats (num, 5) int32 device,
     centroids (num, 2) float64 device). Rows follow OpenCV's layout, row 0 = background.
+    ``labels_out=False`` skips the label image (labels is then None; the stats are the same).
     """
     if connectivity not in (4, 8):
         raise ValueError("connectivity must be 4 or 8")
@@ -76,7 +77,7 @@ def threshold_components_device(img: torch.Tensor, thresh: int, *, invert: bool 
         raise ValueError("img must be a 2-D uint16 tensor with unit column stride")
     H, W = img.shape
     dev = img.device
-    labels = torch.empty((H, W), dtype=torch.int32, device=dev)
+    labels = torch.empty((H, W), dtype=torch.int32, device=dev) if labels_out else None
     cap = int(capacity or DEFAULT_CAPACITY)
     for _ in range(2):
         stats = torch.empty((cap, 5), dtype=torch.int32, device=dev)
@@ -84,7 +85,7 @@ def threshold_components_device(img: torch.Tensor, thresh: int, *, invert: bool 
         n = ctypes.c_int32(0)
         try:
             _lib.call("spx_threshold_cc", int(img.data_ptr()), H, W, img.stride(0), int(thresh),
-                      int(bool(invert)), int(connectivity), int(labels.data_ptr()),
+                      int(bool(invert)), int(connectivity), None if labels is None else int(labels.data_ptr()),
                       int(stats.data_ptr()), int(cents.data_ptr()), cap, ctypes.byref(n),
                       stream_handle())
         except _lib.SpxError as e:

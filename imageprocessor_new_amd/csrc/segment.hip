@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void cc_label_kernel(CCGeom g, const int32_t* 
                 const int64_t y = ny * BS + dy, x = nx * BS + dx;
                 if (y >= g.H || x >= g.W) continue;
                 const bool fg = (b >> (BS == 2 ? (2 * dy + dx) : 0)) & 1u;
-                labels[y * g.W + x] = fg ? lab : 0;
+                if (labels != nullptr) labels[y * g.W + x] = fg ? lab : 0;
                 if (fg) {
                     xmin = min(xmin, (int32_t)x);
                     ymin = min(ymin, (int32_t)y);

@@ -407,7 +407,7 @@ int spx_threshold_cc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, i
                      int32_t invert, int32_t connectivity, int32_t* labels, int32_t* stats,
                      double* centroids, int32_t cap, int32_t* n_out, spx_stream_t stream) {
     spx::clear_error();
-    SPX_REQUIRE(img != nullptr && labels != nullptr && n_out != nullptr, "spx_threshold_cc: null pointer");
+    SPX_REQUIRE(img != nullptr && n_out != nullptr, "spx_threshold_cc: null pointer");
     SPX_REQUIRE(cap == 0 || (stats != nullptr && centroids != nullptr), "spx_threshold_cc: null stats");
     SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_threshold_cc: bad image shape or pitch");
     SPX_REQUIRE(H < ((int64_t)1 << 24) && W < ((int64_t)1 << 24), "spx_threshold_cc: image side of 2^24 px or more");

@@ -162,8 +162,9 @@ int spx_morph_cross(const uint16_t* in, int64_t H, int64_t W, int64_t pitch, int
  * connected_components_stats_gpu (subpx/_gpu/components.py:64-134).
  * connectivity 4 or 8. Labels: 0 background, 1..n-1 components numbered in the order of
  * their first pixel in a raster scan over 2x2 blocks (8-connectivity) or pixels
- * (4-connectivity). img, labels (H*W int32, row pitch W), stats ((cap,5) int32
- * [x, y, w, h, area], row 0 = background), centroids ((cap,2) f64): device.
+ * (4-connectivity). img, labels (H*W int32, row pitch W; NULL: no label image, the stats
+ * only), stats ((cap,5) int32 [x, y, w, h, area], row 0 = background), centroids ((cap,2)
+ * f64): device.
  * n_out (host) receives the label count including the background; SPX_E_CAPACITY when it
  * exceeds cap (n_out still holds the true count). H and W below 2^24. */
 int spx_threshold_cc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,

@@ -195,3 +195,17 @@ def test_detect_centers_triangle_and_morphology_match_oracle():
     np.testing.assert_allclose(res.centers_xy, want, atol=1e-9, rtol=0)
     with pytest.raises(ValueError, match="Unsupported threshold"):
         detect_centers(arr.image, backend="gpu", threshold="mean")
+
+
+def test_components_without_the_label_image_give_the_same_stats():
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.components import threshold_components_device
+
+    rng = np.random.default_rng(17)
+    img = (rng.random((700, 900)) * 1000).astype(np.uint16)
+    with gpu_device(0) as dev:
+        t = torch.from_numpy(img.astype(np.int32)).to(torch.uint16).to(dev)
+        n1, lab, st1, c1 = threshold_components_device(t, 600)
+        n2, none, st2, c2 = threshold_components_device(t, 600, labels_out=False)
+        assert none is None and lab is not None and n1 == n2 > 100
+        assert torch.equal(st1, st2) and torch.equal(c1, c2)
