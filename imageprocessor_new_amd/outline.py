@@ -316,9 +316,11 @@ def _level_bins(res: torch.Tensor, lev: torch.Tensor):
     """Twelve level bins between the 0, 8.3, ..., 100th percentiles of lev (lo <= lev < hi);
     per bin with more than 200 pixels the mean level and the squared robust spread
     (1.4826 x median absolute deviation) of res, by NumPy's interpolation rules (page.py:515-524)."""
+    m = lev.numel()
+    if m == 0:
+        return [], []
     lev_sorted, perm = torch.sort(lev)
     res_by_lev = res[perm]
-    m = lev_sorted.numel()
     pos = np.linspace(0, 100, 13) / 100.0 * (m - 1)
     lo_i, hi_i = np.floor(pos).astype(np.int64), np.ceil(pos).astype(np.int64)
     frac = torch.from_numpy(pos - lo_i).to(lev.device)
