@@ -35,7 +35,7 @@ def __getattr__(name):
         from . import edge_erf
 
         return getattr(edge_erf, name)
-    if name in ("fit_outline", "ellipse_widths", "outline_rays"):
+    if name in ("fit_outline", "ellipse_widths", "outline_rays", "blur_second", "fit_summary", "fit_line"):
         from . import outline
 
         return getattr(outline, name)

@@ -1064,3 +1064,51 @@ def render_outline_page(theta, centers, brightness, shape, *, level: float = 500
             g.manual_seed(int(seed))
             page += float(noise) * torch.randn(page.shape, generator=g, dtype=torch.float64, device=dev)
         return page
+
+
+# ------------------------------------------------------------------------------------- reporting
+def blur_second(theta, blur: str) -> str:
+    """The blur's second part of a fitted theta in words, in pixels ('' for a form without one)
+    (reference outline.py:496-507): the second Gaussian's widths and light share, or the pupil's
+    cut-off."""
+    th = np.asarray(theta, dtype=float)
+    if blur == "gauss2":
+        s2x, s2y = np.exp(th[-3:-1])
+        return f"second Gaussian {s2x:.1f} x {s2y:.1f} px, {100 / (1 + math.exp(-th[-1])):.0f} % of the light"
+    if blur == "lens":
+        return f"pupil cut-off {math.exp(th[-1]):.2f} cycles/px"
+    return ""
+
+
+def fit_summary(fit_meta: dict | None, pixel_size: float = 1.0, unit: str = "px") -> dict | None:
+    """The page readings of a fit's summary ({"fitted": True, **OutlineFitResult.summary}, or
+    {"fitted": False, "reason": ...}) in `unit` (reference outline.py:306-318)."""
+    if not fit_meta:
+        return None
+    if not fit_meta.get("fitted"):
+        return {"fitted": False, "reason": fit_meta.get("reason", "")}
+    k = float(pixel_size)
+    keys = ("rounds_run", "rounds_max", "converged", "per_hole_size", "blur_family", "seconds", "holes_fitted",
+            "at_clip")
+    out = {"fitted": True, "outline_cd": fit_meta["outline_cd_px"] * k,
+           "blur_fwhm": [v * k for v in fit_meta["blur_fwhm_px"]], "p": fit_meta["superellipse_p"]}
+    out.update({("rounds" if n == "rounds_run" else n): fit_meta[n] for n in keys})
+    out["unit"] = unit
+    return out
+
+
+def fit_line(summary: dict | None, ascii_only: bool = False) -> str:
+    """One line on a page's outline fit from `fit_summary` (reference outline.py:321-334)."""
+    if not summary:
+        return ""
+    if not summary["fitted"]:
+        return f"not fitted ({summary['reason']})"
+    u, x = summary["unit"], "x" if ascii_only else "×"
+    state = (f"converged after {summary['rounds']} rounds" if summary["converged"]
+             else f"NOT converged in {summary['rounds']} rounds (the outline was still moving)")
+    size = ("a size per hole" if summary["per_hole_size"]
+            else "per-hole size off: holes too small for a size each at this noise, every hole reads the image's CD")
+    return (f"outline CD {summary['outline_cd']:.2f} {u}, blur FWHM {summary['blur_fwhm'][0]:.1f} {x} "
+            f"{summary['blur_fwhm'][1]:.1f} {u} ({summary['blur_family']}), exponent p {summary['p']:.2f}; {state}; "
+            f"{size}; {summary['holes_fitted']:,} holes fitted, {summary['at_clip']} at the edge-offset bound; "
+            f"{summary['seconds']:.0f} s")
