@@ -29,7 +29,7 @@ if ROOT not in sys.path:
 FLOP_PER_PIXEL_PASS = {6: 90.0, 7: 110.0}  # model + Jacobian + J^T J + J^T r + cost
 PEAK_FP32_TFLOPS = 157.3                    # MI355X vendor FP32 peak (packed)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "erf_fit_c1_kernel_stats.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "erf_fit_square8192_kernel_stats.md")
 
 
 def measured_traffic():
@@ -156,7 +156,7 @@ def main() -> int:
         return 1
 
     t0 = time.perf_counter()
-    arr = synthetic.make_config("c1", seed=args.seed, H=args.size, W=args.size)
+    arr = synthetic.make_workload("square8192", seed=args.seed, H=args.size, W=args.size)
     log(f"generated {arr.image.shape} with {arr.centers_xy.shape[0]} holes in {time.perf_counter() - t0:.1f}s")
 
     with gpu_device(0) as dev:
@@ -552,7 +552,7 @@ PEAK_BF16_DENSE_TFLOPS = 2500.0
 
 
 def run_cnn(args) -> int:
-    """The learned start's network alone: 65,536 32 x 32 BF16 patches prepared from the c1 array
+    """The learned start's network alone: 65,536 32 x 32 BF16 patches prepared from the 8192 x 8192 square array
     (one per hole, 31 x 31 fit patch resampled), then the fused inference kernel; one step =
     one forward over all patches (weights: the initialisation, which does not change the work)."""
     import torch
@@ -561,7 +561,7 @@ def run_cnn(args) -> int:
     from imageprocessor_new_amd.backends import gpu_device
     from imageprocessor_new_amd.cnn import LearnedStart
 
-    arr = synthetic.make_config("c1", seed=args.seed, H=args.size, W=args.size)
+    arr = synthetic.make_workload("square8192", seed=args.seed, H=args.size, W=args.size)
     K = arr.centers_xy.shape[0]
     steps = max(1, min(args.steps, 50))
     with gpu_device(0) as dev:
@@ -602,7 +602,7 @@ def run_cnn(args) -> int:
         "vs_baseline": None,
         "dtype": "bf16 (fp32 accumulate)",
         "data": "synthetic",
-        "config": {"workload": f"CNN forward over {K} patches of the {args.size}^2 c1 array (conv 16-32-64-64 s2, "
+        "config": {"workload": f"CNN forward over {K} patches of the {args.size}^2 square array (conv 16-32-64-64 s2, "
                                "fc 128 -> 7)", "patches": K, "prepare_ms": prep_ms},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_DENSE_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_DENSE_TFLOPS, "traffic": None},
@@ -625,7 +625,7 @@ def run_detect(args) -> int:
     from imageprocessor_new_amd.backends import gpu_device
     from imageprocessor_new_amd.centers import detect_centers
 
-    arr = synthetic.make_config("c0", seed=args.seed)
+    arr = synthetic.make_workload("square2048", seed=args.seed)
     steps = max(1, min(args.steps, 50))
     kw = dict(backend="gpu", threshold="otsu", refine="logquad", area_min=20, area_max=5000, pad=3)
     legs = {}

@@ -5,8 +5,8 @@
 //   hole i      data(p) = P(p) + level_i on its (2R+1)^2 window
 //
 // Follows the reference's GPU hole pass (subpx/_gpu/outline_onepass.py:252-397, 399-427):
-// the same normal equations, weights, step bounds and render rules. Layout and work split are
-// this build's:
+// the same normal equations, weights, step bounds and render rules. The layout and the work
+// split differ:
 //   - templates are one interleaved FP32 grid of 8 floats per fine point,
 //     [T0, T1, T2, Tx, Ty, 0, 0, 0], so a tap is one 16-byte load (+ one dword for Ty);
 //   - `hole_step_kernel` gives each hole a group of G lanes (a wave64). The lanes stride over

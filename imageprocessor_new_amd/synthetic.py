@@ -132,23 +132,23 @@ def render_hole_array(shape, centers, *, radius, sigma, q=1.0, amplitude=3000.0,
 
 
 # Workloads of the contact-hole benchmarks (sizes in pixels).
-CONFIGS = {
+WORKLOADS = {
     # 2048^2 square array, pitch 32, r = 8, s = 1.2 -> 4,096 holes (logquad detection).
-    "c0": dict(H=2048, W=2048, layout="square", pitch=32.0, r=(8.0, 8.0), s=(1.2, 1.2),
+    "square2048": dict(H=2048, W=2048, layout="square", pitch=32.0, r=(8.0, 8.0), s=(1.2, 1.2),
                ellipse_frac=0.0),
     # 8192^2 square array, pitch 32, r ~ U[6, 10], s ~ U[0.8, 1.6] -> 65,536 holes.
-    "c1": dict(H=8192, W=8192, layout="square", pitch=32.0, r=(6.0, 10.0), s=(0.8, 1.6),
+    "square8192": dict(H=8192, W=8192, layout="square", pitch=32.0, r=(6.0, 10.0), s=(0.8, 1.6),
                ellipse_frac=0.0),
     # 16384^2 staggered array, ~200k holes, r ~ U[4, 20], 10 % ellipses q ~ U[0.8, 1].
-    "c2": dict(H=16384, W=16384, layout="hex", pitch=39.37, r=(4.0, 20.0), s=(0.8, 1.6),
+    "hex16384": dict(H=16384, W=16384, layout="hex", pitch=39.37, r=(4.0, 20.0), s=(0.8, 1.6),
                ellipse_frac=0.1),
 }
 
 
-def make_config(name: str, seed: int = 0, *, H: int | None = None, W: int | None = None,
+def make_workload(name: str, seed: int = 0, *, H: int | None = None, W: int | None = None,
                 noise: bool = True, jitter: float = 0.3) -> HoleArray:
     """Render one benchmark workload; H/W override the size (same pitch and draws)."""
-    cfg = dict(CONFIGS[name])
+    cfg = dict(WORKLOADS[name])
     H = int(H or cfg["H"])
     W = int(W or cfg["W"])
     lat = square_lattice(H, W, cfg["pitch"]) if cfg["layout"] == "square" else hex_lattice(H, W, cfg["pitch"])

@@ -29,7 +29,7 @@ typedef void* spx_stream_t; /* a hipStream_t; NULL = null stream */
 #define SPX_E_ARG (-1)         /* bad argument: shape, size, null pointer, option     */
 #define SPX_E_NOMEM (-2)       /* device workspace could not be allocated             */
 #define SPX_E_HIP (-3)         /* a HIP runtime call failed                            */
-#define SPX_E_UNSUPPORTED (-4) /* valid request this build does not implement         */
+#define SPX_E_UNSUPPORTED (-4) /* valid request the library does not implement        */
 #define SPX_E_CAPACITY (-5)    /* more components than the caller's capacity          */
 
 /* ---- per-feature fit status (status_out bytes) ----------------------------------- */

@@ -35,7 +35,7 @@ sys.path.insert(0, sys.argv[1])
 from imageprocessor_new_amd import synthetic, _lib
 from oracle import erfdisk
 lib = ctypes.CDLL(sys.argv[2])
-arr = synthetic.make_config("c2", seed=21, H=160, W=160)
+arr = synthetic.make_workload("hex16384", seed=21, H=160, W=160)
 sel = np.arange(min(6, arr.centers_xy.shape[0]))
 seeds = np.rint(arr.centers_xy[sel]).astype(np.float32)
 seeds = np.concatenate([seeds, [[1.0, 80.0]]]).astype(np.float32)  # clipped border patch

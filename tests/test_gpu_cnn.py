@@ -52,7 +52,7 @@ def test_gemm_bf16_lane_layout(M, N, K, transA, transB):
 def test_prepare_matches_reference():
     from imageprocessor_new_amd.cnn import LearnedStart
 
-    arr = synthetic.make_config("c2", seed=31, H=300, W=300)
+    arr = synthetic.make_workload("hex16384", seed=31, H=300, W=300)
     seeds = np.concatenate([arr.centers_xy[:20], [[2.0, 3.0], [297.0, 150.0]]]).astype(np.float32)
     half = np.array([5, 7, 15, 20, 31, 12, 3, 9, 10, 11, 13, 14, 16, 17, 18, 19, 21, 22, 25, 30, 15, 15], np.int16)
     with _dev() as dev:
@@ -164,7 +164,7 @@ def test_training_reduces_loss():
 def test_fused_proposal_equals_prepare_then_forward():
     from imageprocessor_new_amd.cnn import LearnedStart
 
-    arr = synthetic.make_config("c2", seed=32, H=300, W=300)
+    arr = synthetic.make_workload("hex16384", seed=32, H=300, W=300)
     seeds = np.concatenate([arr.centers_xy[:35], [[2.0, 3.0], [297.0, 150.0]]]).astype(np.float32)  # K = 37
     half = np.resize(np.array([5, 7, 15, 20, 31, 12, 3, 9], np.int16), len(seeds))
     with _dev() as dev:

@@ -40,7 +40,7 @@ def _compare(res, ref_params, ref_status, tol_px=1e-3, status_agree=0.99):
 
 
 def test_erf_disk_matches_fp64_oracle_c1_shape():
-    arr = synthetic.make_config("c1", seed=3, H=512, W=512)
+    arr = synthetic.make_workload("square8192", seed=3, H=512, W=512)
     seeds = np.rint(arr.centers_xy)
     res = _fit_gpu(arr.image, seeds, 15, "disk")
     ref_p, ref_rms, ref_it, ref_st = erfdisk.fit_batch(arr.image, seeds, 15, 0)
@@ -62,7 +62,7 @@ def test_noise_free_recovery():
 
 
 def test_mixed_disks_and_ellipses():
-    arr = synthetic.make_config("c2", seed=5, H=600, W=600)
+    arr = synthetic.make_workload("hex16384", seed=5, H=600, W=600)
     seeds = np.rint(arr.centers_xy)
     model = arr.ellipse.astype(np.uint8)
     half = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int) + 2, 7, 31)
@@ -74,7 +74,7 @@ def test_mixed_disks_and_ellipses():
 
 
 def test_border_and_degenerate_patches():
-    arr = synthetic.make_config("c1", seed=7, H=256, W=256)
+    arr = synthetic.make_workload("square8192", seed=7, H=256, W=256)
     seeds = np.concatenate([np.rint(arr.centers_xy), [[0.0, 0.0], [255.0, 3.0], [128.0, 255.0]]])
     res = _fit_gpu(arr.image, seeds, 15, "disk")
     ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, seeds, 15, 0)
@@ -85,7 +85,7 @@ def test_border_and_degenerate_patches():
 
 
 def test_deterministic():
-    arr = synthetic.make_config("c1", seed=11, H=384, W=384)
+    arr = synthetic.make_workload("square8192", seed=11, H=384, W=384)
     seeds = np.rint(arr.centers_xy)
     a = _fit_gpu(arr.image, seeds, 15, "disk")
     b = _fit_gpu(arr.image, seeds, 15, "disk")
@@ -94,7 +94,7 @@ def test_deterministic():
 
 
 def test_init_start_is_used():
-    arr = synthetic.make_config("c1", seed=2, H=256, W=256)
+    arr = synthetic.make_workload("square8192", seed=2, H=256, W=256)
     seeds = np.rint(arr.centers_xy)
     first = _fit_gpu(arr.image, seeds, 15, "disk")
     again = _fit_gpu(arr.image, seeds, 15, "disk", init=first.params)
@@ -127,7 +127,7 @@ def test_cell_masked_fits_match_oracle_c2_shape(seed):
     # patch restricted to its Voronoi cell among the 8 nearest seeds
     from imageprocessor_new_amd.fit import cell_radius
 
-    arr = synthetic.make_config("c2", seed=seed, H=500, W=500)
+    arr = synthetic.make_workload("hex16384", seed=seed, H=500, W=500)
     seeds = np.rint(arr.centers_xy).astype(np.float32)
     model = arr.ellipse.astype(np.uint8)
     half = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int) + 2, 7, 31)
@@ -154,7 +154,7 @@ def test_cells_are_a_no_op_when_patches_sit_inside_them():
     # rounded seeds end up 30 px apart (a bisector row of ties goes to the lower index)
     from imageprocessor_new_amd.fit import cell_radius
 
-    arr = synthetic.make_config("c1", seed=4, H=320, W=320)
+    arr = synthetic.make_workload("square8192", seed=4, H=320, W=320)
     seeds = np.rint(arr.centers_xy).astype(np.float32)
     nbr = erfdisk.seed_neighbours(seeds, cell_radius(15), 8)
     inside = np.zeros(len(seeds), bool)
@@ -175,7 +175,7 @@ def test_launch_configurations_give_identical_fits_and_autotune_picks_one():
     from imageprocessor_new_amd.backends import gpu_device
     from imageprocessor_new_amd.fit import LAUNCH_CANDIDATES, _TUNED, autotune_launch, fit_erf_disks_device
 
-    arr = synthetic.make_config("c2", seed=9, H=600, W=600)
+    arr = synthetic.make_workload("hex16384", seed=9, H=600, W=600)
     K = arr.centers_xy.shape[0]
     hs = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int) + 2, 7, 31).astype(np.int16)
     with gpu_device(0) as dev:
@@ -204,7 +204,7 @@ def test_bucket_mask_skips_only_the_masked_classes():
     from imageprocessor_new_amd.backends import gpu_device
     from imageprocessor_new_amd.fit import fit_erf_disks_device
 
-    arr = synthetic.make_config("c2", seed=10, H=600, W=600)
+    arr = synthetic.make_workload("hex16384", seed=10, H=600, W=600)
     hs = np.clip(np.ceil(arr.radius + 3 * arr.sigma).astype(int) + 2, 7, 31).astype(np.int16)
     md = arr.ellipse.astype(np.uint8)
     full = _lib.bucket_mask_of(hs, md)

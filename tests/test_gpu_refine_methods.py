@@ -167,7 +167,7 @@ def test_detect_centers_methods_match_oracle_pipeline(refine):
 def test_tiled_default_logquad_matches_oracle():
     from imageprocessor_new_amd.legacy import detect_centers_tiled
 
-    arr = synthetic.make_config("c0", seed=8, H=1024, W=384)
+    arr = synthetic.make_workload("square2048", seed=8, H=1024, W=384)
     with pytest.warns(DeprecationWarning):
         res = detect_centers_tiled(arr.image, tile_h=300, overlap=64, backend="gpu", threshold="otsu",
                                    refine="logquad", area_min=20, area_max=2000)

@@ -22,7 +22,7 @@ def _dev():
 def test_otsu_matches_oracle(seed, size, step):
     from imageprocessor_new_amd.components import otsu_threshold_device
 
-    arr = synthetic.make_config("c0", seed=seed, H=size, W=size + 17)
+    arr = synthetic.make_workload("square2048", seed=seed, H=size, W=size + 17)
     with _dev() as dev:
         t = otsu_threshold_device(torch.from_numpy(arr.image).to(dev), step)
     assert t == segment.otsu_u16(arr.image, step)
@@ -76,7 +76,7 @@ def test_components_invert_and_capacity_retry():
 def test_logquad_and_pipeline_match_oracle_c0():
     from imageprocessor_new_amd.centers import detect_centers
 
-    arr = synthetic.make_config("c0", seed=4, H=768, W=640)
+    arr = synthetic.make_workload("square2048", seed=4, H=768, W=640)
     res = detect_centers(arr.image, backend="gpu", threshold="otsu", refine="logquad", area_min=20,
                          area_max=2000, pad=3, logquad_variant="cpu")
     ref, rows, t = segment.detect_centers_logquad(arr.image, area_min=20, area_max=2000, pad=3)
@@ -91,7 +91,7 @@ def test_logquad_and_pipeline_match_oracle_c0():
 def test_logquad_border_and_small_rois():
     from imageprocessor_new_amd.centers import detect_centers
 
-    arr = synthetic.make_config("c0", seed=6, H=200, W=200)
+    arr = synthetic.make_workload("square2048", seed=6, H=200, W=200)
     # holes cut by the border and a tiny speck; area filters off
     img = arr.image.copy()
     img[5:7, 100:102] = 4000
@@ -112,7 +112,7 @@ def test_empty_image_gives_no_centers():
 def test_tiled_matches_oracle():
     from imageprocessor_new_amd.legacy import detect_centers_tiled
 
-    arr = synthetic.make_config("c0", seed=8, H=1024, W=384)
+    arr = synthetic.make_workload("square2048", seed=8, H=1024, W=384)
     with pytest.warns(DeprecationWarning):
         res = detect_centers_tiled(arr.image, tile_h=300, overlap=64, backend="gpu", threshold="otsu",
                                    refine="logquad", area_min=20, area_max=2000, logquad_variant="cpu")
@@ -131,7 +131,7 @@ def test_tiled_overlap_error():
 def test_erf_disk_refine_through_detect_centers():
     from imageprocessor_new_amd.centers import detect_centers
 
-    arr = synthetic.make_config("c1", seed=12, H=384, W=384)
+    arr = synthetic.make_workload("square8192", seed=12, H=384, W=384)
     res = detect_centers(arr.image, backend="gpu", threshold="otsu", refine="erf_disk", area_min=20,
                          area_max=2000, fit_half=15)
     d = np.hypot(*(res.centers_xy[:, None, :] - arr.centers_xy[None, :, :]).transpose(2, 0, 1)).min(axis=1)
@@ -164,7 +164,7 @@ def test_triangle_matches_oracle(seed):
 def test_triangle_rejects_16_bit_images():
     from imageprocessor_new_amd.components import triangle_threshold_device
 
-    arr = synthetic.make_config("c0", seed=1, H=128, W=128)
+    arr = synthetic.make_workload("square2048", seed=1, H=128, W=128)
     with _dev() as dev, pytest.raises(ValueError, match="8-bit"):
         triangle_threshold_device(torch.from_numpy(arr.image).to(dev))
 

@@ -48,7 +48,7 @@ def test_lattice_sum_is_disk_area():
 
 
 def test_integer_shift_and_mirror_equivariance():
-    arr = synthetic.make_config("c1", seed=9, H=160, W=160)
+    arr = synthetic.make_workload("square8192", seed=9, H=160, W=160)
     img = arr.image
     seeds = np.rint(arr.centers_xy[:6])
     p0, _, _, s0 = erfdisk.fit_batch(img, seeds, 12, 0)

@@ -31,7 +31,7 @@ def test_product_dedupe_equals_reference_rule():
 
 
 def test_tiling_with_one_strip_is_untiled():
-    arr = synthetic.make_config("c0", seed=1, H=300, W=256)
+    arr = synthetic.make_workload("square2048", seed=1, H=300, W=256)
     t = segment.otsu_u16(arr.image, 4)
     ref, _, _ = segment.detect_centers_logquad(arr.image, threshold_value=int(np.floor(t * 0.8)),
                                                area_min=20, area_max=2000)
