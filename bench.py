@@ -75,8 +75,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(img, seeds, half, gpu_params, gpu_status, sample: int, rng_seed: int = 0):
-    """FP64 oracle on a random sample of the features (1 host thread)."""
+def cpu_baseline(img, seeds, half, gpu_params, gpu_centers, gpu_status, sample: int, rng_seed: int = 0):
+    """FP64 oracle on a random sample of the features (1 host thread); the centres are compared
+    in FP64 (gpu_centers), the other parameters as the kernel's FP32 outputs."""
     from oracle import erfdisk
 
     K = seeds.shape[0]
@@ -86,9 +87,11 @@ def cpu_baseline(img, seeds, half, gpu_params, gpu_status, sample: int, rng_seed
     ref_p, _, _, ref_st = erfdisk.fit_batch(img, seeds[idx], half, 0)
     dt = time.perf_counter() - t0
     both = (ref_st <= 1) & (gpu_status[idx] <= 1)
-    err = np.abs(gpu_params[idx][:, :2].astype(np.float64) - ref_p[:, :2]).max(axis=1)
+    gp = gpu_params[idx].astype(np.float64)
+    gp[:, :2] = gpu_centers[idx]
+    err = np.abs(gp[:, :2] - ref_p[:, :2]).max(axis=1)
     within = both & (err <= 1e-3)
-    per_out = np.abs(gpu_params[idx][both].astype(np.float64) - ref_p[both]).max(axis=0) if both.any() else None
+    per_out = np.abs(gp[both] - ref_p[both]).max(axis=0) if both.any() else None
     return {
         "features_per_s": idx.size / dt,
         "seconds": dt,
@@ -215,6 +218,7 @@ def main() -> int:
         stats = np.zeros(4, dtype=np.uint64)
         _lib.call("spx_fit_stats", int(stats.ctypes.data), int(stream.cuda_stream))
         params = out["params"].cpu().numpy()
+        centers = out["centers"].cpu().numpy()
         status = out["status"].cpu().numpy()
         iters = out["iters"].cpu().numpy()
 
@@ -235,11 +239,11 @@ def main() -> int:
                 res_l, tm = learned.fit_learned(net, img_t, seeds_t, half_t, model_t, lm_iters=4)
                 times.append(tm.seconds)
                 parts.append((tm.propose_s, tm.refine_s, tm.fallback_s, tm.n_fallback))
-            lp = res_l["params"].cpu().numpy()
+            lc = res_l["centers"].cpu().numpy()
             ls = res_l["status"].cpu().numpy()
             li = res_l["iters"].cpu().numpy()
             both = (ls <= 1) & (status <= 1)
-            agree = float(np.abs(lp[both, :2].astype(np.float64) - params[both, :2]).max()) if both.any() else None
+            agree = float(np.abs(lc[both] - centers[both]).max()) if both.any() else None
             learned_leg = {
                 "ms_per_step": 1e3 * float(np.mean(times)),
                 "propose_ms": 1e3 * float(np.mean([p[0] for p in parts])),
@@ -263,13 +267,13 @@ def main() -> int:
     # match fits to the true holes (nearest within 2 px) for the accuracy against truth
     from scipy.spatial import cKDTree
 
-    d_true, _ = cKDTree(arr.centers_xy).query(params[ok, :2].astype(np.float64))
+    d_true, _ = cKDTree(arr.centers_xy).query(centers[ok])
     log(f"centre error vs truth: median {np.median(d_true):.4f} px, 99% {np.percentile(d_true, 99):.4f} px")
     cpu = None
     acc_frac = float(ok.mean())
     if not args.no_cpu_baseline:
         log(f"cpu baseline: FP64 oracle on {args.cpu_sample} sampled features (1 thread)")
-        cpu = cpu_baseline(arr.image, seeds_np, 15, params, status, args.cpu_sample)
+        cpu = cpu_baseline(arr.image, seeds_np, 15, params, centers, status, args.cpu_sample)
         log(f"cpu baseline: {cpu}")
         acc_frac = cpu["frac_within_1e-3px"] * float(ok.mean())
     fitted_per_s = K * acc_frac / (ms_per_step * 1e-3)

@@ -109,7 +109,7 @@ def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit
         extra = {"params": params, "status": status, "rms": res["rms"].cpu().numpy(),
                  "iterations": res["iters"].cpu().numpy()}
         ok = (status <= 1) & np.isfinite(params[:, :2]).all(axis=1)
-        return params[:, :2].astype(np.float64), ok, extra
+        return res["centers"].cpu().numpy(), ok, extra
     raise ValueError(f"Unsupported refine method: {method}")
 
 
@@ -229,7 +229,7 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
         nbr = seed_neighbours_device(seeds, img_t.shape, cell_radius(max(hmax, 7)))
     if net is None:
         res = fit_erf_disks_device(img_t, seeds, half, model, options=fit_options, neighbours=nbr)
-        params, status = res["params"], res["status"]
+        params, status, centers = res["params"], res["status"], res["centers"]
     else:
         init = net.propose(img_t, torch.nan_to_num(seeds, nan=0.0).contiguous(), half)
         e0 = mark("propose", e0)
@@ -242,12 +242,13 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
         again = fit_erf_disks_device(img_t, seeds, torch.where(redo, half, torch.zeros_like(half)).contiguous(),
                                      model, options=fit_options, neighbours=nbr)
         params = torch.where(redo[:, None], again["params"], res["params"])
+        centers = torch.where(redo[:, None], again["centers"], res["centers"])
         status = torch.where(redo, again["status"], res["status"])
         if fallbacks is not None:
             fallbacks.append(redo.sum())
     ok = valid & (status <= 1) & torch.isfinite(params[:, :2]).all(dim=1)
     mark("refine", e0)
-    return params[:, :2].to(torch.float64), ok
+    return centers, ok
 
 
 def detect_centers(

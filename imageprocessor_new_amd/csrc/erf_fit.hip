@@ -139,8 +139,8 @@ __global__ void zero_i32(int32_t* p, int n) {
 // see K / 256 atomics instead of K.
 __global__ __launch_bounds__(256) void bucket_kernel(const int16_t* half, const uint8_t* model, int K,
                                                      int32_t* counts, int32_t* order, float* params,
-                                                     float* rms, int16_t* iters, uint8_t* status,
-                                                     int32_t mask) {
+                                                     double* centers, float* rms, int16_t* iters,
+                                                     uint8_t* status, int32_t mask) {
     __shared__ int32_t s_cnt[kKernBuckets];
     __shared__ int32_t s_base[kKernBuckets];
     if (threadIdx.x < kKernBuckets) s_cnt[threadIdx.x] = 0;
@@ -155,6 +155,7 @@ __global__ __launch_bounds__(256) void bucket_kernel(const int16_t* half, const 
         const int bk = known ? 2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0) : 0;
         if (!known || (mask != 0 && !((mask >> bk) & 1))) {
             for (int i = 0; i < 7; ++i) params[(int64_t)f * 7 + i] = __builtin_nanf("");
+            if (centers != nullptr) centers[2 * (int64_t)f] = centers[2 * (int64_t)f + 1] = __builtin_nan("");
             rms[f] = __builtin_nanf("");
             iters[f] = 0;
             status[f] = SPX_FIT_TOO_FEW;
@@ -232,7 +233,7 @@ int resident_blocks(int dev, int b, int threads) {
 int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                      const int16_t* half, const uint8_t* model, const float* init,
                      const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
-                     float* rms, int16_t* iters, uint8_t* status, hipStream_t stream) {
+                     double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream) {
     if (K == 0) return SPX_OK;
     // Workspace: stats u64[4] | counts[10] | queues[10] | order[10][K]
     void* ws = nullptr;
@@ -246,8 +247,8 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
     g_last_stats = stats;
     zero_i32<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(stats), 8 + 2 * kKernBuckets);
     SPX_LAUNCHED();
-    bucket_kernel<<<(K + 255) / 256, 256, 0, stream>>>(half, model, K, counts, order, params, rms,
-                                                        iters, status, opt.bucket_mask);
+    bucket_kernel<<<(K + 255) / 256, 256, 0, stream>>>(half, model, K, counts, order, params, centers,
+                                                        rms, iters, status, opt.bucket_mask);
     SPX_LAUNCHED();
 
     int dev = 0;
@@ -257,7 +258,7 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
     const int waves_per_block = opt.tile_waves > 0 ? (opt.tile_waves < 4 ? opt.tile_waves : 4) : 4;
     const int threads = waves_per_block * fitdev::kWave;
     FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, nullptr, nullptr, nullptr,
-              stats, params, rms, iters, status, nbr, nb, K};
+              stats, params, centers, rms, iters, status, nbr, nb, K};
     for (int b = 0; b < kKernBuckets; ++b) {
         if (opt.bucket_mask != 0 && !((opt.bucket_mask >> b) & 1)) continue;
         a.order = order + (int64_t)b * K;

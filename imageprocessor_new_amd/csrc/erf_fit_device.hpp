@@ -349,6 +349,7 @@ struct FitArgs {
     int32_t* queue;             // work-queue head of this bucket (device, starts at 0)
     unsigned long long* stats;  // [pixel evaluations P=6, P=7, pixel passes, solves]
     float* params;
+    double* centers;     // (K, 2) FP64 absolute centres, or null
     float* rms;
     int16_t* iters;
     uint8_t* status;
@@ -665,6 +666,9 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
         outv = (lane == i) ? pv : outv;
     }
     if (lane < 7) a.params[(int64_t)f * 7 + lane] = outv;
+    // The FP64 centre: an FP32 coordinate near 16384 px rounds by up to 1e-3 px.
+    if (a.centers != nullptr && (lane == 8 || lane == 9))
+        a.centers[(int64_t)f * 2 + (lane - 8)] = lane == 8 ? cur.p[0] + (double)cx : cur.p[1] + (double)cy;
     if (lane == 7) {
         const float dof = fmaxf(npix - (float)P, 1.0f);
         a.rms[f] = (status == SPX_FIT_TOO_FEW || status == SPX_FIT_BAD_START) ? __builtin_nanf("")

@@ -52,7 +52,7 @@ int workspace(size_t bytes, void** out) {
 int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                      const int16_t* half, const uint8_t* model, const float* init,
                      const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
-                     float* rms, int16_t* iters, uint8_t* status, hipStream_t stream);
+                     double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream);
 
 int fit_stats_copy(uint64_t* out, hipStream_t stream);
 int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
@@ -362,16 +362,17 @@ const char* spx_version(void) { return "subpx-hip 0.1 gfx950"; }
 
 int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                   const int16_t* half, const uint8_t* model, const float* init,
-                  const SpxFitOptions* opt, int32_t K, float* params_out, float* rms_out,
-                  int16_t* iters_out, uint8_t* status_out, spx_stream_t stream) {
+                  const SpxFitOptions* opt, int32_t K, float* params_out, double* centers_out,
+                  float* rms_out, int16_t* iters_out, uint8_t* status_out, spx_stream_t stream) {
     return spx_fit_batch_cells(img, H, W, pitch, seeds, half, model, init, opt, K, nullptr, 0, params_out,
-                               rms_out, iters_out, status_out, stream);
+                               centers_out, rms_out, iters_out, status_out, stream);
 }
 
 int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                         const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
-                        int32_t K, const int32_t* neighbours, int32_t nb, float* params_out, float* rms_out,
-                        int16_t* iters_out, uint8_t* status_out, spx_stream_t stream) {
+                        int32_t K, const int32_t* neighbours, int32_t nb, float* params_out,
+                        double* centers_out, float* rms_out, int16_t* iters_out, uint8_t* status_out,
+                        spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(opt != nullptr, "spx_fit_batch: opt is NULL");
     SPX_REQUIRE(K >= 0, "spx_fit_batch: K < 0");
@@ -390,7 +391,7 @@ int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch
     SPX_REQUIRE(opt->init_sigma > 0.0f, "spx_fit_batch: init_sigma <= 0");
     SPX_REQUIRE(neighbours == nullptr || (nb >= 1 && nb <= 8), "spx_fit_batch_cells: nb must lie in [1, 8]");
     return spx::fit_batch_launch(img, H, W, pitch, seeds, half, model, init, *opt, K, neighbours,
-                                 neighbours ? nb : 0, params_out, rms_out, iters_out, status_out,
+                                 neighbours ? nb : 0, params_out, centers_out, rms_out, iters_out, status_out,
                                  spx::as_stream(stream));
 }
 

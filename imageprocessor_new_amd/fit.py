@@ -108,7 +108,8 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
 
     img (H, W) uint16, seeds (K, 2) float32, half (K,) int16, model (K,) uint8,
     init (K, 7) float32 or None, neighbours (K, nb) int32 or None. Returns dict of device
-    tensors params/rms/iters/status.
+    tensors params/centers/rms/iters/status (``centers`` (K, 2) float64: the centres unrounded;
+    an ``out`` dict without it leaves them unwritten).
     """
     if img.dim() != 2 or img.dtype != torch.uint16:
         raise ValueError("img must be a 2-D uint16 tensor")
@@ -126,6 +127,7 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
     if out is None:
         out = {
             "params": torch.empty((K, 7), dtype=torch.float32, device=dev),
+            "centers": torch.empty((K, 2), dtype=torch.float64, device=dev),
             "rms": torch.empty((K,), dtype=torch.float32, device=dev),
             "iters": torch.empty((K,), dtype=torch.int16, device=dev),
             "status": torch.empty((K,), dtype=torch.uint8, device=dev),
@@ -137,9 +139,13 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
             opts = _with_launch(opts, *tuned)
     H, W = img.shape
     pitch = img.stride(0)
+    centers = out.get("centers")
+    if centers is not None and (centers.shape != (K, 2) or centers.dtype != torch.float64
+                                or centers.device != dev or not centers.is_contiguous()):
+        raise ValueError("out['centers'] must be a contiguous (K, 2) float64 tensor on the image's device")
     if neighbours is None:
         _lib.call("spx_fit_batch", _ptr(img), H, W, pitch, _ptr(seeds), _ptr(half), _ptr(model),
-                  _ptr(init), ctypes.byref(opts), K, _ptr(out["params"]), _ptr(out["rms"]),
+                  _ptr(init), ctypes.byref(opts), K, _ptr(out["params"]), _ptr(centers), _ptr(out["rms"]),
                   _ptr(out["iters"]), _ptr(out["status"]), stream_handle(stream))
         return out
     if (neighbours.dim() != 2 or neighbours.shape[0] != K or neighbours.dtype != torch.int32
@@ -147,7 +153,7 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
         raise ValueError("neighbours must be a contiguous (K, nb) int32 tensor on the image's device")
     _lib.call("spx_fit_batch_cells", _ptr(img), H, W, pitch, _ptr(seeds), _ptr(half), _ptr(model),
               _ptr(init), ctypes.byref(opts), K, _ptr(neighbours), int(neighbours.shape[1]),
-              _ptr(out["params"]), _ptr(out["rms"]), _ptr(out["iters"]), _ptr(out["status"]),
+              _ptr(out["params"]), _ptr(centers), _ptr(out["rms"]), _ptr(out["iters"]), _ptr(out["status"]),
               stream_handle(stream))
     return out
 
@@ -185,7 +191,7 @@ def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: i
         if K == 0:
             empty = np.zeros((0, 7), np.float32)
             return ErfFitResult(empty, np.zeros(0, np.float32), np.zeros(0, np.int16),
-                                np.zeros(0, np.uint8), model_np)
+                                np.zeros(0, np.uint8), model_np, centers=np.zeros((0, 2)))
         seeds_t = torch.from_numpy(seeds_np).to(dev)
         half_t = torch.from_numpy(half_np).to(dev)
         model_t = torch.from_numpy(model_np).to(dev)
@@ -200,4 +206,5 @@ def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: i
         return ErfFitResult(out["params"].cpu().numpy(), out["rms"].cpu().numpy(),
                             out["iters"].cpu().numpy(), out["status"].cpu().numpy(), model_np,
                             backend="gpu",
-                            meta={"max_iter": int(max_iter), "step_tol": float(step_tol), "cells": bool(cells)})
+                            meta={"max_iter": int(max_iter), "step_tol": float(step_tol), "cells": bool(cells)},
+                            centers=out["centers"].cpu().numpy())

@@ -113,7 +113,7 @@ def fit_learned(net: LearnedStart, img, seeds, half, model, *, lm_iters: int = 4
     if nb:
         sub = plain_fit(img, seeds[bad].contiguous(), half[bad].contiguous(), model[bad].contiguous(),
                         step_tol=step_tol)
-        for k in ("params", "rms", "iters", "status"):
+        for k in ("params", "centers", "rms", "iters", "status"):
             res[k][bad] = sub[k]
     sync()
     t3 = time.perf_counter()

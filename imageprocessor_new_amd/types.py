@@ -36,7 +36,9 @@ class ErfFitResult:
 
     ``params`` rows are [x0, y0, r, sigma, A, B, q] in image pixels (q = 1 for disks);
     ``status`` holds SPX_FIT_* codes (0 converged, 1 iteration cap, 2 singular,
-    3 diverged, 4 too few pixels, 5 centre left the patch, 6 flat patch).
+    3 diverged, 4 too few pixels, 5 centre left the patch, 6 flat patch). ``centers``
+    (K, 2) float64 holds x0, y0 unrounded (``params`` is float32, whose spacing near
+    16384 px is 2e-3 px); ``centers_xy`` reads it when present.
     """
 
     params: np.ndarray
@@ -46,6 +48,7 @@ class ErfFitResult:
     model: np.ndarray
     backend: str = "gpu"
     meta: dict[str, Any] = field(default_factory=dict)
+    centers: np.ndarray | None = None
 
     @property
     def ok(self) -> np.ndarray:
@@ -53,6 +56,8 @@ class ErfFitResult:
 
     @property
     def centers_xy(self) -> np.ndarray:
+        if self.centers is not None:
+            return np.asarray(self.centers, dtype=np.float64)
         return self.params[:, :2].astype(np.float64)
 
     @property

@@ -231,12 +231,14 @@ int spx_edge_profile_erf(const double* prof, int32_t K, int32_t L, const double*
  * subpx/_cpu/invariant_cd.py:165-246). Patch = (2*half+1)^2 pixels centred on
  * rint(seed); pixels outside the image are left out. seeds (K,2) f32 [x, y], half (K)
  * int16 in [2, 31], model (K) uint8 SPX_MODEL_*, init (K,7) f32 or NULL (moment start),
- * params_out (K,7) f32 [x0, y0, r, sigma, A, B, q] (q = 1 for disks), rms_out (K) f32,
+ * params_out (K,7) f32 [x0, y0, r, sigma, A, B, q] (q = 1 for disks), centers_out (K,2)
+ * f64 [x0, y0] or NULL (the same centres unrounded: an f32 coordinate of a 16384-px page
+ * rounds by up to 1e-3 px, of a 32768-px page by up to 2e-3 px), rms_out (K) f32,
  * iters_out (K) int16, status_out (K) uint8: device. opt: host. */
 int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                   const int16_t* half, const uint8_t* model, const float* init,
-                  const SpxFitOptions* opt, int32_t K, float* params_out, float* rms_out,
-                  int16_t* iters_out, uint8_t* status_out, spx_stream_t stream);
+                  const SpxFitOptions* opt, int32_t K, float* params_out, double* centers_out,
+                  float* rms_out, int16_t* iters_out, uint8_t* status_out, spx_stream_t stream);
 
 /* spx_fit_batch with a neighbour (Voronoi cell) mask: a patch pixel takes part only when it
  * is nearer to the feature's own seed than to each of its listed neighbour seeds (ties to
@@ -248,8 +250,9 @@ int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, cons
  * (subpx/_cpu/cells.py:8-84). */
 int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                         const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
-                        int32_t K, const int32_t* neighbours, int32_t nb, float* params_out, float* rms_out,
-                        int16_t* iters_out, uint8_t* status_out, spx_stream_t stream);
+                        int32_t K, const int32_t* neighbours, int32_t nb, float* params_out,
+                        double* centers_out, float* rms_out, int16_t* iters_out, uint8_t* status_out,
+                        spx_stream_t stream);
 
 /* The nb nearest other seeds of every seed within `radius` px (ordered by distance, ties by
  * index; -1 where fewer exist), by a uniform-grid hash on the device. seeds (K, 2) f32 in an

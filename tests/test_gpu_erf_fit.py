@@ -31,6 +31,7 @@ def _compare(res, ref_params, ref_status, tol_px=1e-3, status_agree=0.99):
     both = (st == 0) & (ref_status == 0)
     assert both.mean() > 0.9
     d = np.abs(res.params[both].astype(np.float64) - ref_params[both])
+    d[:, :2] = np.abs(res.centers_xy[both] - ref_params[both, :2])
     print("max |gpu - fp64 oracle| per output x0 y0 r s A B q:", d.max(axis=0))
     assert d[:, 0].max() <= tol_px and d[:, 1].max() <= tol_px
     assert d[:, 2].max() <= tol_px and d[:, 3].max() <= tol_px
@@ -193,7 +194,7 @@ def test_launch_configurations_give_identical_fits_and_autotune_picks_one():
         assert _TUNED
         again = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(img, seeds, half, model).items()}
     for o in outs[1:] + [again]:
-        for k in ("params", "rms", "iters", "status"):
+        for k in ("params", "centers", "rms", "iters", "status"):
             np.testing.assert_array_equal(o[k], outs[0][k])
 
 
@@ -223,4 +224,24 @@ def test_bucket_mask_skips_only_the_masked_classes():
     for k in ref:
         np.testing.assert_array_equal(part[k][inside], ref[k][inside])
     assert np.all(part["status"][~inside] == _lib.FIT_TOO_FEW)
-    assert np.all(np.isnan(part["params"][~inside]))
+    assert np.all(np.isnan(part["params"][~inside])) and np.all(np.isnan(part["centers"][~inside]))
+
+
+def test_fp64_centres_keep_their_precision_far_from_the_origin():
+    """Near x = 16400 an FP32 coordinate is spaced 2e-3 px apart; the FP64 centres agree with
+    the oracle as closely as near the origin, and the FP32 parameters are those centres
+    rounded to FP32."""
+    rng = np.random.default_rng(17)
+    n = 8
+    c = np.stack([16390.0 + 40.0 * np.arange(n) + rng.uniform(-0.5, 0.5, n), 32.0 + rng.uniform(-0.5, 0.5, n)], 1)
+    arr = synthetic.render_hole_array((64, 16448 + 320), c, radius=rng.uniform(6, 10, n),
+                                      sigma=rng.uniform(0.8, 1.6, n), noise=True, seed=5)
+    seeds = np.rint(c)
+    res = _fit_gpu(arr.image, seeds, 15, "disk")
+    ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, seeds, 15, 0)
+    both = (res.status == 0) & (ref_st == 0)
+    assert both.all(), (res.status, ref_st)
+    err = np.abs(res.centers_xy - ref_p[:, :2]).max()
+    print("max |FP64 centre - oracle| near x = 16400:", err)
+    assert err <= 1e-4
+    np.testing.assert_array_equal(res.params[:, :2], res.centers_xy.astype(np.float32))

@@ -189,7 +189,7 @@ extern "C" int emu_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, in
         int32_t queue = 0;
         if (count == 0) continue;
         FitArgs a{img, H, W, pitch, seeds, half, model, init, *opt, order[b].data(), &count, &queue,
-                  stats, params, rms, iters, status, nbr, nb, K};
+                  stats, params, nullptr, rms, iters, status, nbr, nb, K};
         switch (b) {
             case 0: run_wave<4, 6>(a, timeout_s); break;
             case 1: run_wave<4, 7>(a, timeout_s); break;
