@@ -87,3 +87,30 @@ def test_patches_at_the_size_bounds_and_past_the_frame():
     assert both.sum() >= 5
     assert np.abs(res.centers_xy[both] - ref_p[both, :2]).max() <= 1e-3
     assert np.all(res.status[seeds[:, 0] < 0] == ref_st[seeds[:, 0] < 0])
+
+
+@pytest.mark.parametrize("refine", REFINES)
+@pytest.mark.parametrize("tile_h", [None, 48])
+def test_a_copy_far_from_the_origin_gives_the_same_centres(refine, tile_h):
+    """The same 64 x 256 patch of holes at x = 0 and at x = 32512 of a 64 x 32768 page: every
+    method returns the second copy's centres as the first's moved by exactly 32512 px (to 1e-9
+    px: FP32 coordinates there are 4e-3 px apart)."""
+    from imageprocessor_new_amd.centers import detect_centers
+
+    rng = np.random.default_rng(8)
+    c = np.stack([20.0 + 30.0 * np.arange(8) + rng.uniform(-0.4, 0.4, 8), 32.0 + rng.uniform(-0.4, 0.4, 8)], 1)
+    kw = dict(radius=8.0, sigma=1.2, noise=True, seed=4)
+    kw.update(HOLE.get(refine, {}))
+    small = synthetic.render_hole_array((64, 256), c, **kw).image
+    bg = np.uint16(np.median(small))
+    page = np.full((64, 32768), bg, np.uint16)
+    off = 32768 - 256
+    page[:, :256] = small
+    page[:, off:] = small
+    res = detect_centers(page, backend="gpu", threshold="otsu", refine=refine, area_min=5, area_max=2000,
+                         tile_h=tile_h, overlap=16)
+    xy = res.centers_xy
+    a, b = xy[xy[:, 0] < 256], xy[xy[:, 0] >= off]
+    assert a.shape[0] >= 6 and a.shape == b.shape
+    a, b = a[np.lexsort(a.T)], b[np.lexsort(b.T)]
+    assert np.abs(b - [off, 0] - a).max() <= 1e-9
