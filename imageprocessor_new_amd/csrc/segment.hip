@@ -216,8 +216,12 @@ __device__ __forceinline__ void uf_union(int32_t* parent, int32_t a, int32_t b) 
     }
 }
 
-// Union-find inside one 16 x 16-node tile in LDS (same lock-free rule as the global one).
-constexpr int kUfTile = 16;
+// Union-find inside one 32 x 32-node tile in LDS (same lock-free rule as the global one);
+// 256 threads, each holding the nodes of rows ly, ly + 8, ly + 16 and ly + 24 of its column, so
+// that a wave reads 128 contiguous bytes of every pixel row.
+constexpr int kUfTile = 32;
+constexpr int kUfThreads = 256;
+constexpr int kUfPer = kUfTile * kUfTile / kUfThreads;
 
 __device__ __forceinline__ int uf_find_lds(const int32_t* lp, int a) {
     int p = lp[a];
@@ -260,26 +264,35 @@ __device__ __forceinline__ uint32_t cc_links(uint32_t b, uint32_t l, uint32_t u,
     return m;
 }
 
-// Foreground bits of every node, then the union-find of the edges inside each 16 x 16-node
+// Foreground bits of every node, then the union-find of the edges inside each 32 x 32-node
 // tile in LDS: parent[n] = the node index of n's tile-local root (the smallest node index of
-// its component within the tile), -1 for background. Only edges between tiles are left for
-// cc_merge_kernel, so the global atomics run on tile borders only.
+// its component within the tile; tile-local order is raster order), -1 for background. Only
+// edges between tiles are left for cc_merge_kernel, so the global atomics run on tile borders.
 template <int BS>
-__global__ __launch_bounds__(256) void cc_local_kernel(const uint16_t* img, CCGeom g, int32_t thresh, int invert,
-                                                       int32_t* parent, uint8_t* bits, int64_t tiles_x) {
+__global__ __launch_bounds__(kUfThreads) void cc_local_kernel(const uint16_t* img, CCGeom g, int32_t thresh,
+                                                              int invert, int32_t* parent, uint8_t* bits,
+                                                              int64_t tiles_x) {
     __shared__ int32_t lp[kUfTile * kUfTile];
     __shared__ uint8_t lb[kUfTile * kUfTile];
     const int64_t ty = blockIdx.x / tiles_x, tx = blockIdx.x - (blockIdx.x / tiles_x) * tiles_x;
-    const int i = threadIdx.x, ly = i / kUfTile, lx = i % kUfTile;
-    const int64_t ny = ty * kUfTile + ly, nx = tx * kUfTile + lx;
-    const bool in = ny < g.NH && nx < g.NW;
-    const int64_t n = ny * g.NW + nx;
-    const uint32_t b = in ? node_bits<BS>(img, g, ny, nx, thresh, invert) : 0u;
-    if (in) bits[n] = (uint8_t)b;
-    lb[i] = (uint8_t)b;
-    lp[i] = b ? i : -1;
+    uint32_t bk[kUfPer];
+#pragma unroll
+    for (int k = 0; k < kUfPer; ++k) {
+        const int i = threadIdx.x + k * kUfThreads, ly = i / kUfTile, lx = i % kUfTile;
+        const int64_t ny = ty * kUfTile + ly, nx = tx * kUfTile + lx;
+        const bool in = ny < g.NH && nx < g.NW;
+        const uint32_t b = in ? node_bits<BS>(img, g, ny, nx, thresh, invert) : 0u;
+        if (in) bits[ny * g.NW + nx] = (uint8_t)b;
+        bk[k] = b;
+        lb[i] = (uint8_t)b;
+        lp[i] = b ? i : -1;
+    }
     __syncthreads();
-    if (b) {
+#pragma unroll
+    for (int k = 0; k < kUfPer; ++k) {
+        const uint32_t b = bk[k];
+        if (!b) continue;
+        const int i = threadIdx.x + k * kUfThreads, ly = i / kUfTile, lx = i % kUfTile;
         const uint32_t l = lx > 0 ? lb[i - 1] : 0u;
         const uint32_t u = ly > 0 ? lb[i - kUfTile] : 0u;
         const uint32_t ul = (lx > 0 && ly > 0) ? lb[i - kUfTile - 1] : 0u;
@@ -291,21 +304,27 @@ __global__ __launch_bounds__(256) void cc_local_kernel(const uint16_t* img, CCGe
         if (m & 8u) uf_union_lds(lp, i, i - kUfTile + 1);
     }
     __syncthreads();
-    if (in) {
-        if (b) {
+#pragma unroll
+    for (int k = 0; k < kUfPer; ++k) {
+        const int i = threadIdx.x + k * kUfThreads, ly = i / kUfTile, lx = i % kUfTile;
+        const int64_t ny = ty * kUfTile + ly, nx = tx * kUfTile + lx;
+        if (ny >= g.NH || nx >= g.NW) continue;
+        int32_t out = -1;
+        if (bk[k]) {
             const int r = uf_find_lds(lp, i);
-            parent[n] = (int32_t)((ty * kUfTile + r / kUfTile) * g.NW + tx * kUfTile + r % kUfTile);
-        } else {
-            parent[n] = -1;
+            out = (int32_t)((ty * kUfTile + r / kUfTile) * g.NW + tx * kUfTile + r % kUfTile);
         }
+        parent[ny * g.NW + nx] = out;
     }
 }
 
-// The edges between tiles, by the global lock-free union-find: one block of 64 threads per
-// tile, thread t taking one of the tile's 46 border nodes (left column, top row without its
-// first node, right column without its first node); interior nodes have no such edge.
+// The edges between tiles, by the global lock-free union-find: one block of kUfMergeThreads
+// threads per tile, thread t taking one of the tile's 3 kUfTile - 2 border nodes (left column,
+// top row without its first node, right column without its first node); interior nodes have
+// no such edge.
+constexpr int kUfMergeThreads = (3 * kUfTile - 2 + 63) / 64 * 64;
 template <int BS>
-__global__ __launch_bounds__(64) void cc_merge_kernel(CCGeom g, int32_t* parent, const uint8_t* bits,
+__global__ __launch_bounds__(kUfMergeThreads) void cc_merge_kernel(CCGeom g, int32_t* parent, const uint8_t* bits,
                                                       int64_t tiles_x) {
     const int t = threadIdx.x;
     if (t >= 3 * kUfTile - 2) return;
@@ -605,10 +624,10 @@ int threshold_cc_impl(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, 
     const int T = 256;
     const int64_t uf_tiles_x = (g.NW + kUfTile - 1) / kUfTile;
     const int64_t uf_tiles = uf_tiles_x * ((g.NH + kUfTile - 1) / kUfTile);
-    cc_local_kernel<BS><<<(unsigned)uf_tiles, kUfTile * kUfTile, 0, stream>>>(img, g, thresh, invert, parent, bits,
-                                                                              uf_tiles_x);
+    cc_local_kernel<BS><<<(unsigned)uf_tiles, kUfThreads, 0, stream>>>(img, g, thresh, invert, parent, bits,
+                                                                       uf_tiles_x);
     SPX_LAUNCHED();
-    cc_merge_kernel<BS><<<(unsigned)uf_tiles, 64, 0, stream>>>(g, parent, bits, uf_tiles_x);
+    cc_merge_kernel<BS><<<(unsigned)uf_tiles, kUfMergeThreads, 0, stream>>>(g, parent, bits, uf_tiles_x);
     SPX_LAUNCHED();
     cc_compress_kernel<<<grid_for(nn, T), T, 0, stream>>>(nn, parent, flag);
     SPX_LAUNCHED();
