@@ -8,7 +8,8 @@ Drop-in for the reference ``subpx`` centre path on ``backend="gpu"``: ``detect_c
 gfx950 HIP kernels in libsubpx_hip.so (C-ABI: include/subpx_hip.h).
 """
 
-from .types import CenterResult, ComponentStatsResult, ErfFitResult, OutlineFitResult  # noqa: F401
+from .types import (CenterResult, ComponentStatsResult, ErfFitResult, OutlineFastResult,  # noqa: F401
+                    OutlineFitResult)
 
 __version__ = "0.1.0"
 
@@ -35,7 +36,8 @@ def __getattr__(name):
         from . import edge_erf
 
         return getattr(edge_erf, name)
-    if name in ("fit_outline", "ellipse_widths", "outline_rays", "blur_second", "fit_summary", "fit_line"):
+    if name in ("fit_outline", "ellipse_widths", "outline_rays", "blur_second", "fit_summary", "fit_line",
+                "fit_outline_fast"):
         from . import outline
 
         return getattr(outline, name)

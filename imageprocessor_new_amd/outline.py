@@ -1112,3 +1112,28 @@ def fit_line(summary: dict | None, ascii_only: bool = False) -> str:
             f"{summary['blur_fwhm'][1]:.1f} {u} ({summary['blur_family']}), exponent p {summary['p']:.2f}; {state}; "
             f"{size}; {summary['holes_fitted']:,} holes fitted, {summary['at_clip']} at the edge-offset bound; "
             f"{summary['seconds']:.0f} s")
+
+
+def fit_outline_fast(image, seeds_xy=None, *, pixel_size: float = 1.0, unit: str = "px", blur: str = "gauss2",
+                     invert: bool = False, n_rays: int = 128, backend: str = "auto", device: int = 0, log=None,
+                     trace: dict | None = None, redrawn: bool = False):
+    """The reference's one-pass outline fit (outline.py:361-493) is not built here: its
+    argument checks run with the reference's error texts, then NotImplementedError names
+    ``fit_outline``, which fits the same outline and blur model with a size per hole."""
+    img = np.asarray(image)
+    if img.ndim != 2 or img.size == 0 or not np.issubdtype(img.dtype, np.number) or not np.isfinite(img).all():
+        raise ValueError("image must be a nonempty 2D array of finite intensities")
+    if seeds_xy is not None:
+        seeds = np.asarray(seeds_xy, dtype=np.float64).reshape(-1, 2)
+        if len(seeds) < MIN_HOLES or not np.isfinite(seeds).all():
+            raise ValueError(f"the outline fit needs at least {MIN_HOLES} finite seeds")
+    if blur not in BLUR_FAMILIES:
+        raise ValueError("blur must be one of " + ", ".join(BLUR_FAMILIES))
+    if int(n_rays) != n_rays or n_rays < 16:
+        raise ValueError("n_rays must be an integer of at least 16")
+    if not (np.isfinite(pixel_size) and pixel_size > 0):
+        raise ValueError("pixel_size must be positive")
+    if backend not in ("auto", "gpu", "cpu"):
+        raise ValueError("backend must be 'auto', 'gpu' or 'cpu'")
+    raise NotImplementedError("fit_outline_fast (the one-pass outline fit) is not implemented; fit_outline fits "
+                              "the same outline and blur model with a size per hole")

@@ -103,3 +103,33 @@ class OutlineFitResult:
     summary: dict[str, Any] = field(default_factory=dict)
     pixel_size: float = 1.0
     unit: str = "px"
+
+
+@dataclass
+class OutlineFastResult:
+    """Result type of the one-pass outline fit (reference ``fit_outline_fast``,
+    subpx/types.py:204-240): one outline and one blur per image, no size per hole. Per hole:
+    ``centers_xy``, ``brightness``, ``level``, ``rms``, ``good``; per page: ``outline_cd_px``,
+    ``cd_x_px``, ``cd_y_px``, ``rays_px``, ``blur_fwhm_px``, ``theta``, ``blur``, ``stack``,
+    ``stack_weight``, ``upsample``, ``summary``. Lengths in pixels; ``pixel_size`` converts
+    them to ``unit``. This package does not compute it (``fit_outline_fast`` raises
+    NotImplementedError); the type keeps imports of the reference's result types working."""
+
+    centers_xy: np.ndarray
+    brightness: np.ndarray
+    level: np.ndarray
+    rms: np.ndarray
+    good: np.ndarray
+    outline_cd_px: float
+    cd_x_px: float
+    cd_y_px: float
+    rays_px: np.ndarray
+    blur_fwhm_px: tuple[float, float]
+    theta: np.ndarray
+    stack: np.ndarray
+    stack_weight: np.ndarray
+    upsample: int
+    blur: str
+    summary: dict[str, Any] = field(default_factory=dict)
+    pixel_size: float = 1.0
+    unit: str = "px"

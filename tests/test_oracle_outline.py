@@ -204,3 +204,20 @@ def test_outline_reporting_helpers():
     assert fit_line(fit_summary({"fitted": False, "reason": "fewer than 10 holes"})) == \
         "not fitted (fewer than 10 holes)"
     assert fit_summary(None) is None and fit_line(None) == ""
+
+
+def test_fit_outline_fast_checks_its_arguments_then_says_it_is_not_built():
+    from imageprocessor_new_amd import OutlineFastResult, fit_outline_fast  # noqa: F401
+
+    img = np.ones((32, 32))
+    seeds = np.stack([np.arange(12.0), np.arange(12.0)], 1)
+    with pytest.raises(ValueError, match="nonempty 2D"):
+        fit_outline_fast(np.zeros((0, 4)), seeds)
+    with pytest.raises(ValueError, match="at least 10 finite seeds"):
+        fit_outline_fast(img, seeds[:3])
+    with pytest.raises(ValueError, match="blur must be one of"):
+        fit_outline_fast(img, seeds, blur="box")
+    with pytest.raises(ValueError, match="n_rays"):
+        fit_outline_fast(img, seeds, n_rays=8)
+    with pytest.raises(NotImplementedError, match="fit_outline fits"):
+        fit_outline_fast(img, seeds)
