@@ -114,11 +114,15 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-start leg")
     ap.add_argument("--train-tiles", type=int, default=1 << 18)
-    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM counters)")
-    ap.add_argument("--workload", choices=("fit", "tiled", "holes", "cnn", "detect", "outline"), default="fit",
-                    help="fit: the batched fit of the 8192^2 array (default); tiled: end-to-end "
+    ap.add_argument("--train-patches", type=int, default=1_000_000)
+    ap.add_argument("--workload", choices=("learned", "uneven", "fit", "tiled", "holes", "cnn", "detect", "outline"),
+                    default="learned",
+                    help="learned (default): CNN start + <=4 LM + fallback vs plain LM on the 16384^2 staggered "
+                         "array after training on 1M patches; uneven: the plain batched LM over that array's "
+                         "15..63 px patches; fit: the batched fit of the 8192^2 array; tiled: end-to-end "
                          "detect_centers over strips of a 32768^2 mixed page (erf-disk refinement); "
                          "holes: outline hole-stage sweeps on a 2048x1378 page of 39,042 holes; "
                          "cnn: the fused CNN proposal on the 65,536 patches of the 8192^2 array; "
@@ -138,6 +142,10 @@ def main() -> int:
 
     import torch
 
+    if args.workload == "learned":
+        return run_learned(args)
+    if args.workload == "uneven":
+        return run_uneven(args)
     if args.workload == "tiled":
         return run_tiled(args)
     if args.workload == "holes":
@@ -334,6 +342,290 @@ def main() -> int:
                       f"({cpu['seconds']:.1f} s)",
             "accuracy": {k: cpu[k] for k in ("frac_within_1e-3px", "status_agree", "max_err_vs_fp64")},
         },
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def timed(fn, steps: int, stream=None):
+    """Run fn() `steps` times back to back; returns (mean event ms per call on the launch
+    stream, wall seconds for all calls, last return value). Synchronises before and after."""
+    import torch
+
+    stream = torch.cuda.current_stream() if stream is None else stream
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    out = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        out = fn()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)])), wall, out
+
+
+def fit_flops(stats) -> float:
+    return float(stats[0]) * FLOP_PER_PIXEL_PASS[6] + float(stats[1]) * FLOP_PER_PIXEL_PASS[7]
+
+
+def read_fit_stats(stream=None):
+    import torch
+
+    from imageprocessor_new_amd import _lib
+
+    stats = np.zeros(4, dtype=np.uint64)
+    s = torch.cuda.current_stream() if stream is None else stream
+    _lib.call("spx_fit_stats", int(stats.ctypes.data), int(s.cuda_stream))
+    return stats
+
+
+OUTPUT_NAMES = ["x0", "y0", "r", "sigma", "A", "B", "q"]
+
+
+def oracle_sample(img_host, seeds_np, half_np, model_np, nbr_np, legs: dict, sample: int, rng_seed: int = 0):
+    """FP64 oracle fits (one host thread) of a seeded random sample of the features with the same
+    patches, models and neighbour lists; every leg's results (dict name -> (params (K, 7) f32,
+    centres (K, 2) f64, status (K,))) are compared with them feature by feature. Returns the
+    oracle's rate and, per leg, the fraction within 1e-3 px and the max error per output."""
+    from oracle import erfdisk
+
+    K = seeds_np.shape[0]
+    rng = np.random.default_rng(rng_seed)
+    idx = np.sort(rng.choice(K, size=min(sample, K), replace=False))
+    t0 = time.perf_counter()
+    ref_p, _, _, ref_st = erfdisk.fit_batch(img_host, seeds_np, half_np, model_np, neighbours=nbr_np, only=idx)
+    dt = time.perf_counter() - t0
+    ref_p, ref_st = ref_p[idx], ref_st[idx]
+    out = {"features_per_s": idx.size / dt, "seconds": dt, "n": int(idx.size),
+           "oracle_converged": float((ref_st == 0).mean()), "legs": {}}
+    for name, (params, centers, status) in legs.items():
+        gp = params[idx].astype(np.float64)
+        gp[:, :2] = centers[idx]
+        st = status[idx]
+        both = (ref_st <= 1) & (st <= 1)
+        err = np.abs(gp[:, :2] - ref_p[:, :2]).max(axis=1)
+        within = both & (err <= 1e-3)
+        leg = {"frac_within_1e-3px": float(within.sum() / max(1, (ref_st <= 1).sum())),
+               "status_agree": float((ref_st == st).mean())}
+        if both.any():
+            e = np.abs(gp[both] - ref_p[both])
+            leg["max_err_vs_fp64"] = dict(zip(OUTPUT_NAMES, [float(v) for v in np.nanmax(e, axis=0)]))
+            leg["median_err_vs_fp64"] = dict(zip(OUTPUT_NAMES, [float(v) for v in np.nanmedian(e, axis=0)]))
+        out["legs"][name] = leg
+    return out
+
+
+def bucket_table(img_t, seeds_t, half_t, model_t, nbr_t, half_np, model_np, repeats: int = 5):
+    """Per (patch class, model) bucket: features, kernel ms (HIP events around a launch of that
+    bucket alone), counted TFLOP/s and its fraction of the FP32 peaks."""
+    from imageprocessor_new_amd import _lib
+    from imageprocessor_new_amd.fit import fit_erf_disks_device
+
+    out = {}
+    mask_all = _lib.bucket_mask_of(half_np, model_np)
+    npl = ((2 * half_np.astype(np.int64) + 1) ** 2 + 63) // 64
+    cls = np.select([npl <= 4, npl <= 8, npl <= 16, npl <= 32], [0, 1, 2, 3], 4)
+    bid = 2 * cls + (model_np == 1)
+    res = None
+    for b in range(10):
+        if not (mask_all >> b) & 1:
+            continue
+        opts = _lib.fit_options(bucket_mask=1 << b)
+        res = fit_erf_disks_device(img_t, seeds_t, half_t, model_t, options=opts, neighbours=nbr_t, out=res)
+        ms, _, _ = timed(lambda: fit_erf_disks_device(img_t, seeds_t, half_t, model_t, options=opts,
+                                                      neighbours=nbr_t, out=res), repeats)
+        st = read_fit_stats()
+        fl = fit_flops(st)
+        name = f"npl{4 << (b // 2)}_{'ellipse7' if b & 1 else 'disk6'}"
+        out[name] = {"features": int((bid == b).sum()), "kernel_ms": ms, "tflops": fl / (ms * 1e-3) / 1e12,
+                     "frac": fl / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                     "frac_unpacked": fl / (ms * 1e-3) / 1e12 / PEAK_FP32_UNPACKED_TFLOPS,
+                     "mean_passes": float(st[2]) / max(1, int((bid == b).sum()))}
+    return out
+
+
+def run_uneven(args) -> int:
+    """Uneven batch: the 16384^2 staggered array (~200k holes, r U[4, 20], 10 % ellipses with q
+    U[0.8, 1], Poisson + N(0, 30^2) noise) rendered on the GPU; patches 15..63 px from each hole's
+    radius and edge width, Voronoi-cell neighbour mask. One step = one batched fit of every hole
+    (one launch of each (patch class, model) bucket's persistent kernel)."""
+    import torch
+
+    from imageprocessor_new_amd import _lib, learned, synthetic
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.fit import autotune_launch, fit_erf_disks_device
+
+    with gpu_device(0) as dev:
+        t0 = time.perf_counter()
+        page = synthetic.render_workload_device("hex16384", seed=args.seed)
+        f = learned.page_features(page, dev)
+        torch.cuda.synchronize()
+        K = int(f.seeds.shape[0])
+        half_np, model_np = f.half.cpu().numpy(), f.model.cpu().numpy()
+        log(f"rendered hex16384 page seed {args.seed}: {K} holes, sides {2 * half_np.min() + 1}..{2 * half_np.max() + 1}"
+            f", {int(model_np.sum())} ellipses, in {time.perf_counter() - t0:.1f}s")
+        mask = _lib.bucket_mask_of(half_np, model_np)
+        tune = autotune_launch(page.image, f.seeds, f.half, f.model, neighbours=f.nbr,
+                               options=_lib.fit_options(bucket_mask=mask))
+        log(f"launch autotune: {tune}")
+        opts = _lib.fit_options(bucket_mask=mask)
+        out = fit_erf_disks_device(page.image, f.seeds, f.half, f.model, options=opts, neighbours=f.nbr)
+        for _ in range(args.warmup):
+            fit_erf_disks_device(page.image, f.seeds, f.half, f.model, options=opts, neighbours=f.nbr, out=out)
+        kernel_ms, wall, _ = timed(lambda: fit_erf_disks_device(page.image, f.seeds, f.half, f.model, options=opts,
+                                                                neighbours=f.nbr, out=out), args.steps)
+        stats = read_fit_stats()
+        params, centers = out["params"].cpu().numpy(), out["centers"].cpu().numpy()
+        status, iters = out["status"].cpu().numpy(), out["iters"].cpu().numpy()
+        buckets = bucket_table(page.image, f.seeds, f.half, f.model, f.nbr, half_np, model_np)
+        log(f"buckets: {buckets}")
+        img_host = page.image.cpu().numpy()
+        seeds_np, nbr_np = f.seeds.cpu().numpy(), f.nbr.cpu().numpy()
+    ms_per_step = 1e3 * wall / args.steps
+    flops = fit_flops(stats)
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    ok = status <= 1
+    from scipy.spatial import cKDTree
+
+    d_true, _ = cKDTree(page.centers_xy).query(centers[ok])
+    log(f"K={K} ms/step={ms_per_step:.3f} kernel_ms={kernel_ms:.3f} status={np.bincount(status, minlength=7).tolist()} "
+        f"mean iters={iters.mean():.2f} achieved={achieved:.2f} TFLOP/s; median err vs truth {np.median(d_true):.4f}")
+    acc = None
+    acc_frac = float(ok.mean())
+    if not args.no_cpu_baseline:
+        acc = oracle_sample(img_host, seeds_np, half_np, model_np, nbr_np, {"plain": (params, centers, status)},
+                            args.cpu_sample)
+        log(f"oracle sample: {acc}")
+        acc_frac = acc["legs"]["plain"]["frac_within_1e-3px"] * float(ok.mean())
+    value = K * acc_frac / (ms_per_step * 1e-3)
+    gather = float((((2 * half_np.astype(np.int64) + 1) ** 2) * 2 + 32).sum())
+    result = {
+        "metric": "features fitted/sec at <=1e-3 px vs FP64 oracle; learned+batched vs plain LM",
+        "value": value, "unit": "features/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": f"uneven batch: plain batched LM over a 16384x16384 uint16 staggered array, {K} holes, "
+                        "r U[4, 20], 10% ellipses (7 params), patches 15..63 px, Voronoi-cell mask, FP32 J / FP64 "
+                        "solve, <=30 iters, step<1e-6 px",
+            "features": K, "ellipses": int(model_np.sum()),
+            "seeds": "true centres rounded to the pixel grid (patch placement); moment start",
+            "counted_fraction": acc_frac, "mean_iterations": float(iters.mean()),
+            "status_counts": np.bincount(status, minlength=7).tolist(),
+            "launch_autotune": {"best": list(tune["best"]), "ms": tune["ms"]},
+            "buckets": buckets, "median_err_vs_truth_px": float(np.median(d_true)),
+            "accuracy": None if acc is None else acc["legs"]["plain"],
+        },
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
+                     "traffic": None, "algorithmic_bytes_per_launch": gather, "kernel_ms": kernel_ms,
+                     "algorithmic_flop_per_launch": flops},
+        "cpu_baseline": None if acc is None else {
+            "value": acc["features_per_s"], "unit": "features/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 NumPy oracle LM on {acc['n']} randomly sampled holes of the same page "
+                      f"({acc['seconds']:.1f} s)"},
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def run_learned(args) -> int:
+    """Learned initial guess against the plain LM on the uneven-batch page. The CNN is trained
+    first (untimed) on 1,000,000 plain-LM-labelled patches from staggered pages rendered with
+    page seeds from 10,000 on (disjoint from the evaluation page's seed), 20 epochs of Adam at
+    lr 1e-3, batch 4096, by the library's own BF16 MFMA kernels. Evaluation on the held-out
+    page (seed --seed, ~200k holes): plain leg = batched LM from the moment start; learned leg =
+    CNN proposal + at most 4 LM iterations, features not converged then refitted by the plain
+    LM from the moment start. One step = one leg over every hole of the page."""
+    import torch
+
+    from imageprocessor_new_amd import _lib, learned, synthetic
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.fit import autotune_launch, fit_erf_disks_device
+
+    with gpu_device(0) as dev:
+        log(f"training the CNN: {args.train_patches} patches, {args.train_epochs} epochs")
+        net, tinfo = learned.train_on_pages(n_patches=args.train_patches, epochs=args.train_epochs, batch=4096,
+                                            lr=1e-3, seed=10_000, device=dev)
+        log(f"trained: {tinfo['samples']} patches from {tinfo['pages']} pages, data {tinfo['data_seconds']:.1f}s, "
+            f"train {tinfo['train_seconds']:.1f}s, loss {tinfo['losses'][0]:.5f} -> {tinfo['losses'][-1]:.5f}")
+        page = synthetic.render_workload_device("hex16384", seed=args.seed)
+        f = learned.page_features(page, dev)
+        K = int(f.seeds.shape[0])
+        half_np, model_np = f.half.cpu().numpy(), f.model.cpu().numpy()
+        mask = _lib.bucket_mask_of(half_np, model_np)
+        tune = autotune_launch(page.image, f.seeds, f.half, f.model, neighbours=f.nbr,
+                               options=_lib.fit_options(bucket_mask=mask))
+        opts = _lib.fit_options(bucket_mask=mask)
+        out = fit_erf_disks_device(page.image, f.seeds, f.half, f.model, options=opts, neighbours=f.nbr)
+
+        def plain_step():
+            return fit_erf_disks_device(page.image, f.seeds, f.half, f.model, options=opts, neighbours=f.nbr, out=out)
+
+        for _ in range(args.warmup):
+            plain_step()
+        plain_kernel_ms, plain_wall, _ = timed(plain_step, args.steps)
+        plain_stats = read_fit_stats()
+        p_params, p_centers = out["params"].cpu().numpy(), out["centers"].cpu().numpy()
+        p_status, p_iters = out["status"].cpu().numpy(), out["iters"].cpu().numpy()
+
+        for _ in range(args.warmup):
+            learned.fit_learned(net, page.image, f.seeds, f.half, f.model, lm_iters=4, neighbours=f.nbr)
+        walls, parts = [], []
+        for _ in range(args.steps):
+            res_l, tm = learned.fit_learned(net, page.image, f.seeds, f.half, f.model, lm_iters=4, neighbours=f.nbr)
+            walls.append(tm.seconds)
+            parts.append((tm.propose_s, tm.refine_s, tm.fallback_s, tm.n_fallback))
+        l_params, l_centers = res_l["params"].cpu().numpy(), res_l["centers"].cpu().numpy()
+        l_status, l_iters = res_l["status"].cpu().numpy(), res_l["iters"].cpu().numpy()
+        img_host = page.image.cpu().numpy()
+        seeds_np, nbr_np = f.seeds.cpu().numpy(), f.nbr.cpu().numpy()
+    plain_ms = 1e3 * plain_wall / args.steps
+    learned_ms = 1e3 * float(np.mean(walls))
+    acc = oracle_sample(img_host, seeds_np, half_np, model_np, nbr_np,
+                        {"plain": (p_params, p_centers, p_status), "learned": (l_params, l_centers, l_status)},
+                        args.cpu_sample)
+    log(f"oracle sample: {acc}")
+    plain_frac = acc["legs"]["plain"]["frac_within_1e-3px"] * float((p_status <= 1).mean())
+    learned_frac = acc["legs"]["learned"]["frac_within_1e-3px"] * float((l_status <= 1).mean())
+    plain_per_s = K * plain_frac / (plain_ms * 1e-3)
+    learned_per_s = K * learned_frac / (learned_ms * 1e-3)
+    flops = fit_flops(plain_stats)
+    achieved = flops / (plain_kernel_ms * 1e-3) / 1e12
+    result = {
+        "metric": "features fitted/sec at <=1e-3 px vs FP64 oracle; learned+batched vs plain LM",
+        "value": learned_per_s, "unit": "features/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": learned_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": f"learned initial guess: CNN proposal (32x32 BF16, conv 16-32-64-64 s2, fc 128 -> 7) + <=4 LM "
+                        f"iterations + plain-LM fallback vs plain batched LM, on {K} held-out holes of a 16384x16384 "
+                        "staggered array (r U[4, 20], 10% ellipses, patches 15..63 px, Voronoi-cell mask)",
+            "features": K,
+            "learned_features_per_s": learned_per_s, "plain_lm_features_per_s": plain_per_s,
+            "ratio_vs_plain": learned_per_s / plain_per_s,
+            "plain_ms_per_step": plain_ms, "learned_ms_per_step": learned_ms,
+            "learned_parts_ms": {"propose": 1e3 * float(np.mean([p[0] for p in parts])),
+                                 "lm_from_proposal": 1e3 * float(np.mean([p[1] for p in parts])),
+                                 "fallback": 1e3 * float(np.mean([p[2] for p in parts]))},
+            "fallback_features": int(parts[-1][3]), "fallback_rate": float(parts[-1][3]) / K,
+            "mean_iterations": {"plain": float(p_iters.mean()), "learned_incl_fallback": float(l_iters.mean())},
+            "accuracy": acc["legs"],
+            "train": {k: tinfo[k] for k in ("samples", "pages", "page_seeds", "holes_seen", "data_seconds",
+                                            "train_seconds", "epochs", "batch", "lr")},
+            "train_loss_first_last": [tinfo["losses"][0], tinfo["losses"][-1]],
+            "launch_autotune": {"best": list(tune["best"]), "ms": tune["ms"]},
+        },
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
+                     "traffic": None, "kernel": "plain batched LM launch", "kernel_ms": plain_kernel_ms,
+                     "algorithmic_flop_per_launch": flops},
+        "cpu_baseline": {"value": acc["features_per_s"], "unit": "features/s", "cores": 1, "kind": "port",
+                         "sample": f"FP64 NumPy oracle LM on {acc['n']} randomly sampled holes of the same page "
+                                   f"({acc['seconds']:.1f} s)"},
     }
     print(json.dumps(result), flush=True)
     return 0

@@ -51,7 +51,7 @@ class SpxFitOptions(ctypes.Structure):
         ("tile_waves", ctypes.c_int32),
         ("persistent", ctypes.c_int32),
         ("bucket_mask", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("learned_iters", ctypes.c_int32),
     ]
 
 
@@ -142,6 +142,7 @@ SIGNATURES = {
     "spx_threshold_cc": (_I32, [_P, _I64, _I64, _I64, _I32, _I32, _I32, _P, _P, _P, _I32, _P, _P]),
     "spx_refine_logquad": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _P, _P, _P]),
     "spx_fit_stats": (_I32, [_P, _P]),
+    "spx_fit_counters": (_I32, [_P, _I32, _P]),
     "spx_cnn_param_count": (_I64, []),
     "spx_gemm_bf16": (_I32, [_P, _I64, _I64, _P, _I64, _I64, _P, _I32, _I32, _I32, _P]),
     "spx_cnn_prepare": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _P, _P, _P]),
@@ -234,10 +235,10 @@ def call(name: str, *args) -> int:
 
 
 def fit_options(*, max_iter=30, step_tol=1e-6, lambda0=1e-3, lambda_up=10.0, lambda_down=0.1,
-                init_sigma=1.0, tile_waves=0, persistent=1, bucket_mask=0) -> SpxFitOptions:
+                init_sigma=1.0, tile_waves=0, persistent=1, bucket_mask=0, learned_iters=0) -> SpxFitOptions:
     return SpxFitOptions(int(max_iter), float(step_tol), float(lambda0), float(lambda_up),
                          float(lambda_down), float(init_sigma), int(tile_waves), int(persistent),
-                         int(bucket_mask), 0)
+                         int(bucket_mask), int(learned_iters))
 
 
 def bucket_mask_of(half, model) -> int:

@@ -182,8 +182,8 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
     float64, ok (num,) bool) device tensors; `events` collects (stage, start, end) CUDA events.
     The only host synchronisation is the component count. With ``net`` (a cnn.LearnedStart)
     the fit starts from the network's proposal for at most ``lm_iters`` steps, and the rows not
-    converged then are fitted again from the moment start (``fallbacks`` collects their count
-    per strip as device tensors)."""
+    converged then are fitted again from the moment start by the same wave in the same launch
+    (``fallbacks`` collects their count per strip as device tensors)."""
     from .fit import cell_radius, fit_erf_disks_device, seed_neighbours_device
 
     def mark(name, e0):
@@ -234,18 +234,13 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
         init = net.propose(img_t, torch.nan_to_num(seeds, nan=0.0).contiguous(), half)
         e0 = mark("propose", e0)
         o = fit_options
-        short = _lib.fit_options(max_iter=int(lm_iters), step_tol=o.step_tol, lambda0=o.lambda0, lambda_up=o.lambda_up,
+        fused = _lib.fit_options(max_iter=o.max_iter, step_tol=o.step_tol, lambda0=o.lambda0, lambda_up=o.lambda_up,
                                  lambda_down=o.lambda_down, init_sigma=o.init_sigma, tile_waves=o.tile_waves,
-                                 persistent=o.persistent, bucket_mask=o.bucket_mask)
-        res = fit_erf_disks_device(img_t, seeds, half, model, init=init, options=short, neighbours=nbr)
-        redo = valid & (res["status"] != _lib.FIT_CONVERGED)
-        again = fit_erf_disks_device(img_t, seeds, torch.where(redo, half, torch.zeros_like(half)).contiguous(),
-                                     model, options=fit_options, neighbours=nbr)
-        params = torch.where(redo[:, None], again["params"], res["params"])
-        centers = torch.where(redo[:, None], again["centers"], res["centers"])
-        status = torch.where(redo, again["status"], res["status"])
+                                 persistent=o.persistent, bucket_mask=o.bucket_mask, learned_iters=int(lm_iters))
+        res = fit_erf_disks_device(img_t, seeds, half, model, init=init, options=fused, neighbours=nbr)
+        params, status, centers = res["params"], res["status"], res["centers"]
         if fallbacks is not None:
-            fallbacks.append(redo.sum())
+            fallbacks.append((valid & (res["iters"] > int(lm_iters))).sum())
     ok = valid & (status <= 1) & torch.isfinite(params[:, :2]).all(dim=1)
     mark("refine", e0)
     return centers, ok

@@ -419,7 +419,7 @@ int64_t max_act(int Bn) {
 }
 constexpr int kSplits = 16;
 
-int cnn_buffers(int Bn, bool train, CnnBuffers& b) {
+int cnn_buffers(int Bn, bool train, CnnBuffers& b, hipStream_t stream) {
     auto al = [](int64_t x) { return (size_t)((x + 255) & ~(int64_t)255); };
     size_t total = 0;
     size_t off_col[kLayers], off_act[kLayers];
@@ -453,7 +453,7 @@ int cnn_buffers(int Bn, bool train, CnnBuffers& b) {
         total += al(4 * (int64_t)Bn * 7);
     }
     void* ws = nullptr;
-    int rc = workspace(total, &ws);
+    int rc = workspace(total, &ws, stream);
     if (rc != SPX_OK) return rc;
     char* base = static_cast<char*>(ws);
     for (int l = 0; l < kLayers; ++l) {
@@ -774,7 +774,7 @@ int infer_weights(const float* theta, const uint16_t* theta_bf, InferWeights& iw
         total += (int64_t)kNet[l].n * iw.kp[l];
     }
     void* ws = nullptr;
-    int rc = workspace(sizeof(uint16_t) * (size_t)total, &ws);
+    int rc = workspace(sizeof(uint16_t) * (size_t)total, &ws, stream);
     if (rc != SPX_OK) return rc;
     uint16_t* wt = static_cast<uint16_t*>(ws);
     iw.wt = wt;
@@ -817,7 +817,7 @@ int cnn_train_step_launch(const uint16_t* x, const float* t, const float* w, int
                           uint16_t* theta_bf, float* adam_m, float* adam_v, int step, float lr, float* loss_out,
                           float* grad_out, hipStream_t stream) {
     CnnBuffers b;
-    int rc = cnn_buffers(Bn, true, b);
+    int rc = cnn_buffers(Bn, true, b, stream);
     if (rc != SPX_OK) return rc;
     rc = forward(x, Bn, theta_bf, theta, b, stream);
     if (rc != SPX_OK) return rc;

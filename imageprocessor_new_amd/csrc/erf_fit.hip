@@ -3,7 +3,9 @@
 // primitives it is written against (DPP/bpermute shuffles, hardware transcendentals).
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
+#include <vector>
 
 #include "spx_internal.hpp"
 
@@ -133,56 +135,80 @@ __global__ void zero_i32(int32_t* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Assigns every feature to its patch-size bucket. Features with an invalid half side or
-// model get status SPX_FIT_TOO_FEW and are not fitted. Slots are reserved per workgroup
-// (LDS counters, then one global atomic per bucket and workgroup), so the bucket heads
-// see K / 256 atomics instead of K.
-__global__ __launch_bounds__(256) void bucket_kernel(const int16_t* half, const uint8_t* model, int K,
-                                                     int32_t* counts, int32_t* order, float* params,
-                                                     double* centers, float* rms, int16_t* iters,
-                                                     uint8_t* status, int32_t mask) {
+// Bucket of feature f, or -1 for a feature with an invalid half side or model or whose
+// bucket the mask leaves out (it gets status SPX_FIT_TOO_FEW and NaNs and is not fitted).
+__device__ inline int feature_bucket(const int16_t* half, const uint8_t* model, int f, int32_t mask) {
+    const int hs = half[f];
+    const int md = model[f];
+    if (hs < 2 || hs > kMaxHalf || (md != SPX_MODEL_ERF_DISK6 && md != SPX_MODEL_ERF_ELLIPSE7)) return -1;
+    const int bk = 2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0);
+    return (mask != 0 && !((mask >> bk) & 1)) ? -1 : bk;
+}
+
+// Pass 1 of the bucketing: the per-bucket feature counts (LDS counters, then one global
+// atomic per bucket and workgroup), and the outputs of features that are not fitted.
+__global__ __launch_bounds__(256) void bucket_count_kernel(const int16_t* half, const uint8_t* model, int K,
+                                                           int32_t* counts, float* params, double* centers,
+                                                           float* rms, int16_t* iters, uint8_t* status,
+                                                           int32_t mask) {
     __shared__ int32_t s_cnt[kKernBuckets];
-    __shared__ int32_t s_base[kKernBuckets];
     if (threadIdx.x < kKernBuckets) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    int b = -1;
-    int local = 0;
     if (f < K) {
-        const int hs = half[f];
-        const int md = model[f];
-        const bool known = !(hs < 2 || hs > kMaxHalf || (md != SPX_MODEL_ERF_DISK6 && md != SPX_MODEL_ERF_ELLIPSE7));
-        const int bk = known ? 2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0) : 0;
-        if (!known || (mask != 0 && !((mask >> bk) & 1))) {
+        const int b = feature_bucket(half, model, f, mask);
+        if (b < 0) {
             for (int i = 0; i < 7; ++i) params[(int64_t)f * 7 + i] = __builtin_nanf("");
             if (centers != nullptr) centers[2 * (int64_t)f] = centers[2 * (int64_t)f + 1] = __builtin_nan("");
             rms[f] = __builtin_nanf("");
             iters[f] = 0;
             status[f] = SPX_FIT_TOO_FEW;
         } else {
-            b = bk;
-            local = atomicAdd(&s_cnt[b], 1);
+            atomicAdd(&s_cnt[b], 1);
         }
     }
     __syncthreads();
-    if (threadIdx.x < kKernBuckets) {
-        const int c = s_cnt[threadIdx.x];
-        s_base[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0;
-    }
-    __syncthreads();
-    if (b >= 0) order[(int64_t)b * K + s_base[b] + local] = f;
+    if (threadIdx.x < kKernBuckets && s_cnt[threadIdx.x] != 0) atomicAdd(&counts[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
+// Pass 2: every bucket's ids back to back in one K-entry array, bucket b after the ids of
+// buckets 0 .. b-1. Each workgroup reserves its slots with one atomic per bucket.
+__global__ __launch_bounds__(256) void bucket_scatter_kernel(const int16_t* half, const uint8_t* model, int K,
+                                                             const int32_t* counts, int32_t* cursors,
+                                                             int32_t* order, int32_t mask) {
+    __shared__ int32_t s_cnt[kKernBuckets];
+    __shared__ int32_t s_base[kKernBuckets];
+    if (threadIdx.x < kKernBuckets) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = -1, local = 0;
+    if (f < K) {
+        b = feature_bucket(half, model, f, mask);
+        if (b >= 0) local = atomicAdd(&s_cnt[b], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kKernBuckets) {
+        int start = 0;
+        for (int j = 0; j < (int)threadIdx.x; ++j) start += counts[j];
+        const int c = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = start + (c ? atomicAdd(&cursors[threadIdx.x], c) : 0);
+    }
+    __syncthreads();
+    if (b >= 0) order[s_base[b] + local] = f;
+}
+
+// Counters of the last fit issued on this thread: where they live and the stream that
+// writes them.
 thread_local unsigned long long* g_last_stats = nullptr;
+thread_local hipStream_t g_last_stats_stream = nullptr;
 
 }  // namespace
 
-int fit_stats_copy(uint64_t* out, hipStream_t stream) {
-    if (g_last_stats == nullptr) {
-        for (int i = 0; i < 4; ++i) out[i] = 0;
-        return SPX_OK;
-    }
-    SPX_HIP(hipMemcpyAsync(out, g_last_stats, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+int fit_stats_copy(uint64_t* out, int n, hipStream_t stream) {
+    for (int i = 0; i < n; ++i) out[i] = 0;
+    if (g_last_stats == nullptr) return SPX_OK;
+    SPX_HIP(hipStreamSynchronize(g_last_stats_stream));
+    SPX_HIP(hipMemcpyAsync(out, g_last_stats, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
     SPX_HIP(hipStreamSynchronize(stream));
     return SPX_OK;
 }
@@ -230,25 +256,34 @@ int resident_blocks(int dev, int b, int threads) {
 
 }  // namespace
 
-int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
-                     const int16_t* half, const uint8_t* model, const float* init,
-                     const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
-                     double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream) {
-    if (K == 0) return SPX_OK;
-    // Workspace: stats u64[4] | counts[10] | queues[10] | order[10][K]
-    void* ws = nullptr;
-    const size_t bytes = 32 + sizeof(int32_t) * (2 * kKernBuckets + (size_t)kKernBuckets * K);
-    int rc = workspace(bytes, &ws);
-    if (rc != SPX_OK) return rc;
-    unsigned long long* stats = static_cast<unsigned long long*>(ws);
-    int32_t* counts = reinterpret_cast<int32_t*>(stats + 4);
-    int32_t* queues = counts + kKernBuckets;
-    int32_t* order = queues + kKernBuckets;
-    g_last_stats = stats;
-    zero_i32<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(stats), 8 + 2 * kKernBuckets);
+// Layout of a fit's scratch: counters u64[8] | counts[10] | cursors[10] | queues[10] | ids.
+constexpr size_t kFitHead = 256;
+constexpr int kFitHeadWords = (64 + 3 * kKernBuckets * 4) / 4;
+
+size_t fit_scratch_bytes(int32_t K) { return kFitHead + sizeof(int32_t) * (size_t)K; }
+
+// Enqueues the fit on `scratch` (fit_scratch_bytes(K) bytes of device memory) with its
+// counters at `stats` (8 x u64, may lie inside scratch).
+int fit_batch_enqueue(void* scratch, unsigned long long* stats, const uint16_t* img, int64_t H, int64_t W,
+                      int64_t pitch, const float* seeds, const int16_t* half, const uint8_t* model, const float* init,
+                      const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
+                      double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream) {
+    int32_t* head = static_cast<int32_t*>(scratch);
+    int32_t* counts = head + 16;
+    int32_t* cursors = counts + kKernBuckets;
+    int32_t* queues = cursors + kKernBuckets;
+    int32_t* order = reinterpret_cast<int32_t*>(static_cast<char*>(scratch) + kFitHead);
+    zero_i32<<<1, 64, 0, stream>>>(head, kFitHeadWords);
     SPX_LAUNCHED();
-    bucket_kernel<<<(K + 255) / 256, 256, 0, stream>>>(half, model, K, counts, order, params, centers,
-                                                        rms, iters, status, opt.bucket_mask);
+    zero_i32<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(stats), 16);
+    SPX_LAUNCHED();
+    g_last_stats = stats;
+    g_last_stats_stream = stream;
+    const int blocks256 = (K + 255) / 256;
+    bucket_count_kernel<<<blocks256, 256, 0, stream>>>(half, model, K, counts, params, centers, rms, iters, status,
+                                                       opt.bucket_mask);
+    SPX_LAUNCHED();
+    bucket_scatter_kernel<<<blocks256, 256, 0, stream>>>(half, model, K, counts, cursors, order, opt.bucket_mask);
     SPX_LAUNCHED();
 
     int dev = 0;
@@ -257,11 +292,11 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
     SPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const int waves_per_block = opt.tile_waves > 0 ? (opt.tile_waves < 4 ? opt.tile_waves : 4) : 4;
     const int threads = waves_per_block * fitdev::kWave;
-    FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, nullptr, nullptr, nullptr,
-              stats, params, centers, rms, iters, status, nbr, nb, K};
+    FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, order, nullptr, nullptr,
+              stats, params, centers, rms, iters, status, nbr, nb, K, counts, 0};
     for (int b = 0; b < kKernBuckets; ++b) {
         if (opt.bucket_mask != 0 && !((opt.bucket_mask >> b) & 1)) continue;
-        a.order = order + (int64_t)b * K;
+        a.bucket = b;
         a.count = counts + b;
         a.queue = queues + b;
         // Persistent grid: the resident capacity of the chip, never more than features.
@@ -274,6 +309,21 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
         SPX_LAUNCHED();
     }
     return SPX_OK;
+}
+
+int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                     const int16_t* half, const uint8_t* model, const float* init,
+                     const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
+                     double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream) {
+    if (K == 0) return SPX_OK;
+    void* ws = nullptr;
+    int rc = workspace(fit_scratch_bytes(K), &ws, stream);
+    if (rc != SPX_OK) return rc;
+    unsigned long long* stats = nullptr;
+    rc = fit_stats_buffer(&stats, stream);
+    if (rc != SPX_OK) return rc;
+    return fit_batch_enqueue(ws, stats, img, H, W, pitch, seeds, half, model, init, opt, K, nbr, nb, params, centers,
+                             rms, iters, status, stream);
 }
 
 }  // namespace spx

@@ -356,6 +356,10 @@ struct FitArgs {
     const int32_t* nbr;  // (K, nb) neighbour feature ids, -1 = none; null = no cell mask
     int32_t nb;
     int32_t nfeat;       // K: ids outside [0, K) are ignored
+    // When set, `order` is the start of one array holding every bucket's ids back to back and
+    // this bucket's list starts after the ids of buckets 0 .. bucket - 1 (their counts here).
+    const int32_t* bucket_counts;
+    int32_t bucket;
 };
 
 // Voronoi cell of a feature against its listed neighbour seeds. A pixel q belongs to the
@@ -478,19 +482,27 @@ SPX_DEV_INLINE bool cell_edge_fast(const FitArgs& a, int f, const CellForm& cf, 
 // Work counters of one wave, flushed to FitArgs::stats once when the wave leaves its
 // queue (one atomic per counter per wave instead of per feature).
 struct WaveStats {
-    unsigned long long evals;   // pixel passes x patch pixels
-    unsigned long long passes;  // pixel passes
-    unsigned long long steps;   // LM step attempts
+    unsigned long long evals;      // pixel passes x patch pixels
+    unsigned long long passes;     // pixel passes
+    unsigned long long steps;      // LM step attempts
+    unsigned long long fallbacks;  // learned starts not converged within their budget
 };
 
 // Fit of one feature by the whole wave. The patch registers are filled by the caller;
-// `masked`: the patch is restricted to the feature's Voronoi cell.
+// `masked`: the patch is restricted to the feature's Voronoi cell. `from_init`: start from
+// the caller's parameters (a.init), else from the patch moments. With a learned budget
+// (a.opt.learned_iters > 0) a fit from the init that has not converged within that many step
+// attempts writes nothing and returns its attempt count: the caller fits the feature again
+// from the moment start (the plain-start fallback), passing that count as `it_before` so that
+// iters_out counts the attempts of both. Returns -1 otherwise.
 template <int NPL, int P>
-SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>& pt,
-                                float cx, float cy, bool masked, WaveStats& ws) {
+SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>& pt,
+                               float cx, float cy, bool masked, bool from_init, int it_before,
+                               WaveStats& ws) {
     using L = Layout<P>;
     const int lane = spx_lane();
     const float npix = pt.npix;
+    const bool learned = a.opt.learned_iters > 0;
     Params<P> cur;
     uint8_t status = SPX_FIT_CONVERGED;
     int it = 0;
@@ -502,7 +514,7 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
         for (int i = 0; i < P; ++i) cur.p[i] = __builtin_nan("");
     } else {
         bool start_ok = true;
-        if (a.init != nullptr) {
+        if (from_init) {
             const float* in = a.init + (int64_t)f * 7;
             cur.p[0] = (double)spx_uniform(in[0]) - cx;
             cur.p[1] = (double)spx_uniform(in[1]) - cy;
@@ -598,7 +610,8 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
             status = SPX_FIT_MAX_ITER;
             if (!isfinite(cost)) status = SPX_FIT_DIVERGED;
             bool any_solve = false;
-            while (status == SPX_FIT_MAX_ITER && it < a.opt.max_iter) {
+            const int max_iter = (from_init && learned) ? a.opt.learned_iters : a.opt.max_iter;
+            while (status == SPX_FIT_MAX_ITER && it < max_iter) {
                 ++it;
                 double delta[P];
                 if (!damped_solve<P>(S, (double)npix, lambda, delta)) {
@@ -653,6 +666,15 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
         }
     }
 
+    ws.evals += (unsigned long long)passes * (unsigned long long)npix;
+    ws.passes += (unsigned long long)passes;
+    ws.steps += (unsigned long long)it;
+    if (from_init && learned && status != SPX_FIT_CONVERGED && status != SPX_FIT_TOO_FEW) {
+        ++ws.fallbacks;
+        return it;
+    }
+    it += it_before;
+
     // lanes 0..6 write one parameter each, lane 7 the fit summary
     float outv = 0.0f;
 #pragma unroll
@@ -676,9 +698,7 @@ SPX_DEV_INLINE void fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL
         a.iters[f] = (int16_t)it;
         a.status[f] = status;
     }
-    ws.evals += (unsigned long long)passes * (unsigned long long)npix;
-    ws.passes += (unsigned long long)passes;
-    ws.steps += (unsigned long long)it;
+    return -1;
 }
 
 // Body of the persistent fit kernel for one wave: pull features of one (patch size,
@@ -689,18 +709,35 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
     const int lane = spx_lane();
     const int count = spx_uniform_i(*a.count);
     if (count == 0) return;
-    WaveStats ws{0ull, 0ull, 0ull};
+    const int32_t* order = a.order;
+    if (a.bucket_counts != nullptr) {
+        int off = 0;
+        for (int j = 0; j < a.bucket; ++j) off += a.bucket_counts[j];
+        order += spx_uniform_i(off);
+    }
+    WaveStats ws{0ull, 0ull, 0ull, 0ull};
     // Features are claimed kGrab at a time: one queue atomic per kGrab features.
     int base = 0, left = 0;
+    // A feature whose learned start needs the fallback is gathered again and fitted from the
+    // moment start in the next turn of this loop (from the L2 lines it has just read).
+    int redo = -1, redo_iters = 0;
     for (;;) {
-        if (left == 0) {
-            base = spx_queue_grab(a.queue, kGrab);
-            left = kGrab;
+        int f;
+        const bool from_init = a.init != nullptr && redo < 0;
+        const int it_before = redo < 0 ? 0 : redo_iters;
+        if (redo >= 0) {
+            f = redo;
+            redo = -1;
+        } else {
+            if (left == 0) {
+                base = spx_queue_grab(a.queue, kGrab);
+                left = kGrab;
+            }
+            const int slot = base + (kGrab - left);
+            --left;
+            if (slot >= count) break;
+            f = spx_uniform_i(order[slot]);
         }
-        const int slot = base + (kGrab - left);
-        --left;
-        if (slot >= count) break;
-        const int f = spx_uniform_i(a.order[slot]);
         const int hs = spx_uniform_i(a.half[f]);
         const int side = 2 * hs + 1;
         const int npx = side * side;
@@ -767,11 +804,16 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
             }
         }
         pt.npix = spx_wave_sum(cnt);
-        fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked, ws);
+        const int again = fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked, from_init, it_before, ws);
+        if (again >= 0) {
+            redo = f;
+            redo_iters = again;
+        }
     }
     if (lane == 8) spx_atomic_add(&a.stats[P == 7 ? 1 : 0], ws.evals);
     if (lane == 9) spx_atomic_add(&a.stats[2], ws.passes);
     if (lane == 10) spx_atomic_add(&a.stats[3], ws.steps);
+    if (lane == 11 && ws.fallbacks != 0ull) spx_atomic_add(&a.stats[4], ws.fallbacks);
 }
 
 // Patch-size bucket of a half side: NPL = 4, 8, 16, 32, 64 pixels per lane.

@@ -112,7 +112,7 @@ int seed_neighbours_launch(const float* seeds, int32_t K, int64_t H, int64_t W, 
     const size_t head = sizeof(int32_t) * (3 * (size_t)ncell + (size_t)K);
     const size_t head_al = (head + 255) & ~(size_t)255;
     void* ws = nullptr;
-    int rc = workspace(head_al + scan_bytes, &ws);
+    int rc = workspace(head_al + scan_bytes, &ws, stream);
     if (rc != SPX_OK) return rc;
     int32_t* cnt = static_cast<int32_t*>(ws);
     int32_t* start = cnt + ncell;

@@ -6,8 +6,8 @@
 // disks / ellipses of the fits, point-sampled at pixel centres (same model as
 // imageprocessor_new_amd/synthetic.py):
 //   I = B + sum_h A erfc((rho_h - r_h) / (sqrt(2) s_h)) / 2,
-// with photon noise approximated by its normal limit N(I, I) (I >= 500 everywhere) plus
-// N(0, read_sigma^2), rounded half-even and clipped to uint16.
+// with photon noise Poisson(I) plus N(0, read_sigma^2), rounded half-even and clipped to
+// uint16 (the noise model of reference subpx/synthetic_arrays.py:327-350).
 // Draws come from a counter-based generator (splitmix64 of seed, tile, hole, field), so
 // a tile is the same whatever else is rendered with it.
 #include <hipcub/hipcub.hpp>
@@ -34,6 +34,41 @@ __device__ inline double normal(uint64_t seed, uint64_t a, uint64_t b, uint64_t 
     const double u1 = uni(seed, a, b, 2 * c);
     const double u2 = uni(seed, a, b, 2 * c + 1);
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// Poisson(lam) from the counter-based stream (seed, a, b, 16 + k), k = 0, 1, ...
+// lam < 12: inversion by sequential search of the CDF with one uniform.
+// lam >= 12: Hoermann's transformed rejection with squeeze, PTRS (W. Hoermann, "The
+// transformed rejection method for generating Poisson random variables", Insurance:
+// Mathematics and Economics 12 (1993) 39-45); its acceptance rate is above 0.9, and the loop
+// gives up after 64 candidates (never reached in practice) with the rounded mean.
+__device__ inline double poisson(uint64_t seed, uint64_t a, uint64_t b, double lam) {
+    if (!(lam > 0.0)) return 0.0;
+    if (lam < 12.0) {
+        const double u = uni(seed, a, b, 16);
+        double p = exp(-lam), F = p, k = 0.0;
+        while (u > F && k < 400.0) {
+            k += 1.0;
+            p *= lam / k;
+            F += p;
+        }
+        return k;
+    }
+    const double slam = sqrt(lam), loglam = log(lam);
+    const double bb = 0.931 + 2.53 * slam;
+    const double aa = -0.059 + 0.02483 * bb;
+    const double inv_alpha = 1.1239 + 1.1328 / (bb - 3.4);
+    const double vr = 0.9277 - 3.6224 / (bb - 2.0);
+    for (int it = 0; it < 64; ++it) {
+        const double U = uni(seed, a, b, 16 + 2 * it) - 0.5;
+        const double V = uni(seed, a, b, 17 + 2 * it);
+        const double us = 0.5 - fabs(U);
+        const double k = floor((2.0 * aa / us + bb) * U + lam + 0.43);
+        if (us >= 0.07 && V <= vr) return k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (log(V) + log(inv_alpha) - log(aa / (us * us) + bb) <= -lam + k * loglam - lgamma(k + 1.0)) return k;
+    }
+    return rint(lam);
 }
 
 constexpr int kMaxHoles = 9;
@@ -105,7 +140,7 @@ __global__ void tile_pixels_kernel(uint64_t seed, int n, int T, SpxTileSpec spec
         I += (double)o[5] * 0.5 * erfc((rho - o[2]) / (1.4142135623730951 * o[3]));
     }
     const uint64_t key = (uint64_t)p;
-    double v = I + sqrt(fmax(I, 0.0)) * normal(seed ^ 0x5bd1e995ull, t, key, 0);
+    double v = poisson(seed ^ 0x5bd1e995ull, t, key, I);
     if (spec.read_sigma > 0.0f) v += spec.read_sigma * normal(seed ^ 0x1b873593ull, t, key, 0);
     v = rint(v);
     v = fmin(fmax(v, 0.0), 65535.0);
@@ -115,7 +150,8 @@ __global__ void tile_pixels_kernel(uint64_t seed, int n, int T, SpxTileSpec spec
 
 // Whole pages: a hole list (x, y, r, s, q, A per hole, image coordinates) binned into a grid
 // of kPageCell-px cells by a counting sort; each pixel adds the holes of its 3 x 3 cells
-// whose support (r + 8 s, where erfc/2 < 1e-15) reaches it, then the tile renderer's noise.
+// whose support (r + 8 s, where erfc/2 < 1e-15) reaches it, then the tile renderer's noise
+// (Poisson(I) + N(0, read_sigma^2)).
 constexpr int kPageCell = 64;
 
 __global__ void page_count_kernel(const float* holes, int n, int gw, int gh, int32_t* cnt) {
@@ -159,7 +195,7 @@ __global__ void page_pixels_kernel(uint64_t seed, const float* holes, const int3
             }
         }
     }
-    double v = I + sqrt(fmax(I, 0.0)) * normal(seed ^ 0x5bd1e995ull, (uint64_t)py, (uint64_t)px, 0);
+    double v = poisson(seed ^ 0x5bd1e995ull, (uint64_t)py, (uint64_t)px, I);
     if (read_sigma > 0.0) v += read_sigma * normal(seed ^ 0x1b873593ull, (uint64_t)py, (uint64_t)px, 0);
     v = rint(v);
     v = fmin(fmax(v, 0.0), 65535.0);
@@ -178,7 +214,7 @@ int render_page_launch(uint64_t seed, const float* holes, int n, int64_t H, int6
     const size_t head = sizeof(int32_t) * (3 * (size_t)ncell + (size_t)(n > 0 ? n : 1));
     const size_t head_al = (head + 255) & ~(size_t)255;
     void* ws = nullptr;
-    int rc = workspace(head_al + scan_bytes, &ws);
+    int rc = workspace(head_al + scan_bytes, &ws, stream);
     if (rc != SPX_OK) return rc;
     int32_t* cnt = static_cast<int32_t*>(ws);
     int32_t* start = cnt + ncell;
@@ -207,7 +243,7 @@ int render_tiles_launch(uint64_t seed, int n, int T, const SpxTileSpec& spec, ui
                         float* truth, hipStream_t stream) {
     if (n == 0) return SPX_OK;
     void* ws = nullptr;
-    int rc = workspace(sizeof(float) * (size_t)n * kMaxHoles * 6, &ws);
+    int rc = workspace(sizeof(float) * (size_t)n * kMaxHoles * 6, &ws, stream);
     if (rc != SPX_OK) return rc;
     float* holes = static_cast<float*>(ws);
     tile_holes_kernel<<<(n + 255) / 256, 256, 0, stream>>>(seed, n, T, spec, holes, truth);

@@ -610,7 +610,7 @@ int threshold_cc_impl(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, 
     const size_t o_scan = o_bg + align(sizeof(CCAcc) * tiles);
     const size_t total = o_scan + align(scan_bytes) + 256;
     void* ws = nullptr;
-    int rc = workspace(total, &ws);
+    int rc = workspace(total, &ws, stream);
     if (rc != SPX_OK) return rc;
     char* base = static_cast<char*>(ws);
     int32_t* parent = reinterpret_cast<int32_t*>(base + o_parent);
@@ -668,7 +668,7 @@ int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_
     const int64_t Hs = (H + step - 1) / step;
     const int64_t Ws = (W + step - 1) / step;
     void* ws = nullptr;
-    int rc = workspace(sizeof(uint32_t) * kHistBins + 256, &ws);
+    int rc = workspace(sizeof(uint32_t) * kHistBins + 256, &ws, stream);
     if (rc != SPX_OK) return rc;
     uint32_t* hist = static_cast<uint32_t*>(ws);
     int32_t* dev_out = reinterpret_cast<int32_t*>(hist + kHistBins);
@@ -784,7 +784,7 @@ int triangle_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, in
     const int64_t Hs = (H + step - 1) / step;
     const int64_t Ws = (W + step - 1) / step;
     void* ws = nullptr;
-    int rc = workspace(sizeof(uint32_t) * kHistBins + 256, &ws);
+    int rc = workspace(sizeof(uint32_t) * kHistBins + 256, &ws, stream);
     if (rc != SPX_OK) return rc;
     uint32_t* hist = static_cast<uint32_t*>(ws);
     int32_t* dev_out = reinterpret_cast<int32_t*>(hist + kHistBins);

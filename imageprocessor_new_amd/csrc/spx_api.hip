@@ -1,7 +1,9 @@
 // C-ABI entry points of libsubpx_hip.so: lifecycle, error text, workspaces and argument
 // checks. Kernels live in the other translation units.
+#include <map>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "spx_internal.hpp"
@@ -15,23 +17,25 @@ struct DeviceWorkspace {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
+// Workspaces and fit counters by (device ordinal, stream).
+using StreamKey = std::pair<int, hipStream_t>;
 std::mutex g_ws_mutex;
-std::vector<DeviceWorkspace> g_ws;  // indexed by device ordinal
+std::map<StreamKey, DeviceWorkspace> g_ws;
+std::map<StreamKey, unsigned long long*> g_fit_stats;
 }  // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
-int workspace(size_t bytes, void** out) {
+int workspace(size_t bytes, void** out, hipStream_t stream) {
     int dev = 0;
     SPX_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-    DeviceWorkspace& w = g_ws[dev];
+    DeviceWorkspace& w = g_ws[StreamKey(dev, stream)];
     if (w.bytes < bytes) {
         if (w.ptr != nullptr) {
-            // Work already queued may still read the old buffer.
-            SPX_HIP(hipDeviceSynchronize());
+            // Work already queued on this stream may still read the old buffer.
+            SPX_HIP(hipStreamSynchronize(stream));
             SPX_HIP(hipFree(w.ptr));
             w.ptr = nullptr;
             w.bytes = 0;
@@ -49,12 +53,30 @@ int workspace(size_t bytes, void** out) {
     return SPX_OK;
 }
 
+int fit_stats_buffer(unsigned long long** out, hipStream_t stream) {
+    int dev = 0;
+    SPX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    unsigned long long*& p = g_fit_stats[StreamKey(dev, stream)];
+    if (p == nullptr) {
+        void* m = nullptr;
+        if (hipMalloc(&m, 8 * sizeof(unsigned long long)) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("fit counters: hipMalloc failed");
+            return SPX_E_NOMEM;
+        }
+        p = static_cast<unsigned long long*>(m);
+    }
+    *out = p;
+    return SPX_OK;
+}
+
 int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                      const int16_t* half, const uint8_t* model, const float* init,
                      const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
                      double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream);
 
-int fit_stats_copy(uint64_t* out, hipStream_t stream);
+int fit_stats_copy(uint64_t* out, int n, hipStream_t stream);
 int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
                 int32_t* thresh_out, hipStream_t stream);
 int threshold_cc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
@@ -327,7 +349,14 @@ int spx_cnn_to_bf16(const float* theta, int64_t n, uint16_t* theta_bf, spx_strea
 int spx_fit_stats(uint64_t* out, spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(out != nullptr, "spx_fit_stats: out is NULL");
-    return spx::fit_stats_copy(out, spx::as_stream(stream));
+    return spx::fit_stats_copy(out, 4, spx::as_stream(stream));
+}
+
+int spx_fit_counters(uint64_t* out, int32_t n, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(out != nullptr, "spx_fit_counters: out is NULL");
+    SPX_REQUIRE(n >= 1 && n <= 8, "spx_fit_counters: n must lie in [1, 8]");
+    return spx::fit_stats_copy(out, n, spx::as_stream(stream));
 }
 
 
@@ -345,14 +374,22 @@ void spx_release(void) {
     std::lock_guard<std::mutex> lock(spx::g_ws_mutex);
     int cur = 0;
     if (hipGetDevice(&cur) != hipSuccess) return;
-    for (size_t d = 0; d < spx::g_ws.size(); ++d) {
-        if (spx::g_ws[d].ptr == nullptr) continue;
-        if (hipSetDevice((int)d) == hipSuccess) {
-            (void)hipDeviceSynchronize();
-            (void)hipFree(spx::g_ws[d].ptr);
+    for (auto& kv : spx::g_ws) {
+        if (kv.second.ptr == nullptr) continue;
+        if (hipSetDevice(kv.first.first) == hipSuccess) {
+            (void)hipStreamSynchronize(kv.first.second);
+            (void)hipFree(kv.second.ptr);
         }
-        spx::g_ws[d] = spx::DeviceWorkspace{};
     }
+    spx::g_ws.clear();
+    for (auto& kv : spx::g_fit_stats) {
+        if (kv.second == nullptr) continue;
+        if (hipSetDevice(kv.first.first) == hipSuccess) {
+            (void)hipStreamSynchronize(kv.first.second);
+            (void)hipFree(kv.second);
+        }
+    }
+    spx::g_fit_stats.clear();
     (void)hipSetDevice(cur);
 }
 
@@ -368,28 +405,39 @@ int spx_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, cons
                                centers_out, rms_out, iters_out, status_out, stream);
 }
 
+namespace {
+int check_fit_args(const char* name, const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                   const int16_t* half, const uint8_t* model, const SpxFitOptions* opt, int32_t K,
+                   const int32_t* neighbours, int32_t nb, const float* params_out, const float* rms_out,
+                   const int16_t* iters_out, const uint8_t* status_out) {
+    const std::string n(name);
+    SPX_REQUIRE(opt != nullptr, n + ": opt is NULL");
+    SPX_REQUIRE(K >= 0, n + ": K < 0");
+    if (K == 0) return SPX_OK;
+    SPX_REQUIRE(img != nullptr && seeds != nullptr && half != nullptr && model != nullptr, n + ": null input pointer");
+    SPX_REQUIRE(params_out != nullptr && rms_out != nullptr && iters_out != nullptr && status_out != nullptr,
+                n + ": null output pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, n + ": bad image shape or pitch");
+    SPX_REQUIRE(opt->max_iter >= 0 && opt->max_iter <= 10000, n + ": max_iter out of range");
+    SPX_REQUIRE(opt->learned_iters >= 0 && opt->learned_iters <= 1000, n + ": learned_iters out of range");
+    SPX_REQUIRE(opt->step_tol >= 0.0f, n + ": step_tol < 0");
+    SPX_REQUIRE(opt->lambda0 > 0.0f && opt->lambda_up > 1.0f && opt->lambda_down > 0.0f && opt->lambda_down < 1.0f,
+                n + ": damping options out of range");
+    SPX_REQUIRE(opt->init_sigma > 0.0f, n + ": init_sigma <= 0");
+    SPX_REQUIRE(neighbours == nullptr || (nb >= 1 && nb <= 8), n + ": nb must lie in [1, 8]");
+    return SPX_OK;
+}
+}  // namespace
+
 int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                         const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
                         int32_t K, const int32_t* neighbours, int32_t nb, float* params_out,
                         double* centers_out, float* rms_out, int16_t* iters_out, uint8_t* status_out,
                         spx_stream_t stream) {
     spx::clear_error();
-    SPX_REQUIRE(opt != nullptr, "spx_fit_batch: opt is NULL");
-    SPX_REQUIRE(K >= 0, "spx_fit_batch: K < 0");
-    if (K == 0) return SPX_OK;
-    SPX_REQUIRE(img != nullptr && seeds != nullptr && half != nullptr && model != nullptr,
-                "spx_fit_batch: null input pointer");
-    SPX_REQUIRE(params_out != nullptr && rms_out != nullptr && iters_out != nullptr &&
-                    status_out != nullptr,
-                "spx_fit_batch: null output pointer");
-    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_fit_batch: bad image shape or pitch");
-    SPX_REQUIRE(opt->max_iter >= 0 && opt->max_iter <= 10000, "spx_fit_batch: max_iter out of range");
-    SPX_REQUIRE(opt->step_tol >= 0.0f, "spx_fit_batch: step_tol < 0");
-    SPX_REQUIRE(opt->lambda0 > 0.0f && opt->lambda_up > 1.0f && opt->lambda_down > 0.0f &&
-                    opt->lambda_down < 1.0f,
-                "spx_fit_batch: damping options out of range");
-    SPX_REQUIRE(opt->init_sigma > 0.0f, "spx_fit_batch: init_sigma <= 0");
-    SPX_REQUIRE(neighbours == nullptr || (nb >= 1 && nb <= 8), "spx_fit_batch_cells: nb must lie in [1, 8]");
+    int rc = check_fit_args("spx_fit_batch", img, H, W, pitch, seeds, half, model, opt, K, neighbours, nb, params_out,
+                            rms_out, iters_out, status_out);
+    if (rc != SPX_OK || K == 0) return rc;
     return spx::fit_batch_launch(img, H, W, pitch, seeds, half, model, init, *opt, K, neighbours,
                                  neighbours ? nb : 0, params_out, centers_out, rms_out, iters_out, status_out,
                                  spx::as_stream(stream));

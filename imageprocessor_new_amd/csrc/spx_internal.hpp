@@ -44,9 +44,14 @@ inline hipStream_t as_stream(spx_stream_t s) { return reinterpret_cast<hipStream
 
 constexpr int kWave = 64;
 
-// Per-device scratch buffer owned by the library, grown on demand and freed by spx_release.
-// Returned memory is only valid for work ordered on the same stream before the next call
-// that grows it; callers synchronise the stream before growing.
-int workspace(size_t bytes, void** out);
+// Scratch buffer owned by the library for one (device, stream) pair, grown on demand and freed
+// by spx_release. Work on other streams never sees it, so calls on different streams do not
+// overwrite each other's scratch. A call that grows it waits for that stream's queued work
+// first; the memory stays valid for work ordered on the stream until the next call on it.
+int workspace(size_t bytes, void** out, hipStream_t stream);
+
+// The fit's own counters (8 x u64) for one (device, stream) pair: a small buffer that no other
+// entry point writes, kept until spx_release.
+int fit_stats_buffer(unsigned long long** out, hipStream_t stream);
 
 }  // namespace spx

@@ -51,8 +51,12 @@ def seed_neighbours_device(seeds: torch.Tensor, shape, radius: float, nb: int = 
 
 
 def cell_radius(half_max: int) -> float:
-    """Seeds farther than this cannot move the bisector into a patch of half side half_max."""
-    return 2.0 * float(np.sqrt(2.0)) * float(half_max) + 1.0
+    """Seeds farther than this cannot move the bisector into a patch of half side half_max, nor
+    into the 4-neighbours the moment start tests around it. A patch pixel q lies within
+    sqrt(2) (h + 0.5) of the seed s (the patch is centred on rint(s)) and its 4-neighbours within
+    sqrt(2) (h + 1.5); a seed t can take q only when |t - s| <= |t - q| + |q - s| <= 2 |q - s|.
+    The 1e-3 px covers the float32 rounding of the radius."""
+    return 2.0 * float(np.sqrt(2.0)) * (float(half_max) + 1.5) + 1e-3
 
 
 # Launch configurations chosen by autotune_launch, per (device, K class).
@@ -67,7 +71,7 @@ def _tune_key(dev: torch.device, K: int):
 def _with_launch(o: _lib.SpxFitOptions, tile_waves: int, persistent: int) -> _lib.SpxFitOptions:
     return _lib.fit_options(max_iter=o.max_iter, step_tol=o.step_tol, lambda0=o.lambda0, lambda_up=o.lambda_up,
                             lambda_down=o.lambda_down, init_sigma=o.init_sigma, tile_waves=int(tile_waves),
-                            persistent=int(persistent), bucket_mask=o.bucket_mask)
+                            persistent=int(persistent), bucket_mask=o.bucket_mask, learned_iters=o.learned_iters)
 
 
 def autotune_launch(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor, model: torch.Tensor, *,

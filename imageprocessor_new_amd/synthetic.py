@@ -145,9 +145,11 @@ WORKLOADS = {
 }
 
 
-def make_workload(name: str, seed: int = 0, *, H: int | None = None, W: int | None = None,
-                noise: bool = True, jitter: float = 0.3) -> HoleArray:
-    """Render one benchmark workload; H/W override the size (same pitch and draws)."""
+def workload_holes(name: str, seed: int = 0, *, H: int | None = None, W: int | None = None,
+                   jitter: float = 0.3) -> dict:
+    """Hole parameters of one benchmark workload, drawn from ``default_rng(seed)`` in the order
+    centre jitter, radii, edge sigmas, ellipse flags, axis ratios (the noise is drawn after
+    them). Returns dict(H, W, centers (N, 2), radius, sigma, q, ellipse, rng)."""
     cfg = dict(WORKLOADS[name])
     H = int(H or cfg["H"])
     W = int(W or cfg["W"])
@@ -160,28 +162,76 @@ def make_workload(name: str, seed: int = 0, *, H: int | None = None, W: int | No
     ell = rng.random(n) < cfg["ellipse_frac"]
     qv = rng.uniform(0.8, 1.0, size=n)
     q = np.where(ell, qv, 1.0)
-    arr = render_hole_array((H, W), centers, radius=radius, sigma=sigma, q=q, seed=None,
-                            noise=noise, rng=rng)
-    arr.ellipse = ell
-    arr.meta.update(config=name, seed=seed, H=H, W=W, pitch=cfg["pitch"], layout=cfg["layout"])
+    return dict(H=H, W=W, centers=centers, radius=radius, sigma=sigma, q=q, ellipse=ell, rng=rng, cfg=cfg)
+
+
+def make_workload(name: str, seed: int = 0, *, H: int | None = None, W: int | None = None,
+                  noise: bool = True, jitter: float = 0.3) -> HoleArray:
+    """Render one benchmark workload on the host; H/W override the size (same pitch and draws)."""
+    d = workload_holes(name, seed, H=H, W=W, jitter=jitter)
+    arr = render_hole_array((d["H"], d["W"]), d["centers"], radius=d["radius"], sigma=d["sigma"], q=d["q"],
+                            seed=None, noise=noise, rng=d["rng"])
+    arr.ellipse = d["ellipse"]
+    arr.meta.update(config=name, seed=seed, H=d["H"], W=d["W"], pitch=d["cfg"]["pitch"], layout=d["cfg"]["layout"])
     return arr
+
+
+@dataclass
+class DevicePage:
+    """A workload rendered on the GPU: ``image`` (H, W) uint16 device tensor plus the exact hole
+    parameters (host arrays)."""
+    image: Any
+    centers_xy: np.ndarray
+    radius: np.ndarray
+    sigma: np.ndarray
+    q: np.ndarray
+    ellipse: np.ndarray
+    amplitude: float
+    background: float
+    meta: dict[str, Any] = field(default_factory=dict)
+
+    def patch_half(self, lo: int = 7, hi: int = 31) -> np.ndarray:
+        """Half side of each hole's fit patch: side = 2 ceil(r_max + 3 s) + 1 clamped to
+        [2 lo + 1, 2 hi + 1] (15..63 px by default), r_max the larger semi-axis."""
+        reach = self.radius * np.maximum(self.q, 1.0) + 3.0 * self.sigma
+        return np.clip(np.ceil(reach), lo, hi).astype(np.int16)
+
+
+def render_workload_device(name: str, seed: int = 0, *, H: int | None = None, W: int | None = None,
+                           jitter: float = 0.3, amplitude: float = 3000.0, background: float = 500.0,
+                           read_sigma: float = 30.0, device: int = 0) -> DevicePage:
+    """The workload's holes (``workload_holes``, same draws as ``make_workload``) rendered on the
+    GPU by spx_render_page: point-sampled erf disks / ellipses, Poisson(I) + N(0, read_sigma^2)
+    noise from a counter-based stream keyed by ``seed`` (not numpy's noise stream)."""
+    d = workload_holes(name, seed, H=H, W=W, jitter=jitter)
+    n = d["centers"].shape[0]
+    holes = np.column_stack([d["centers"], d["radius"], d["sigma"], d["q"], np.full(n, amplitude)])
+    img = render_page(holes, d["H"], d["W"], background=background, read_sigma=read_sigma, seed=seed,
+                      device=device)
+    return DevicePage(img, d["centers"], d["radius"], d["sigma"], d["q"], d["ellipse"], float(amplitude),
+                      float(background), meta=dict(config=name, seed=seed, H=d["H"], W=d["W"],
+                                                   pitch=d["cfg"]["pitch"], layout=d["cfg"]["layout"]))
 
 
 # ---------------------------------------------------------------------------------------
 # Large pages (32768^2, ~1.1M holes): hole lists drawn here, pixels rendered on the GPU.
 
 def mixed_page_holes(seed: int = 0, H: int = 32768, W: int = 32768, region: int = 2048,
-                     pitch=(24.0, 44.0), r_frac=(0.2, 0.3), sigma=(0.8, 1.6), jitter: float = 0.3,
+                     pitch=(24.0, 48.0), r_frac=(0.2, 0.3), sigma=(0.8, 1.6), jitter: float = 0.3,
                      amplitude: float = 3000.0):
     """Hole list of a page tiled into ``region`` x ``region`` blocks, each a square or a
-    staggered array with its own pitch ~ U[pitch] (the mixed large-page workload). Holes keep
-    half a pitch from their block's edges; r ~ U[r_frac] * pitch, sigma ~ U[sigma], centre
-    jitter N(0, jitter^2). Returns (holes (N, 6) float32 [x, y, r, sigma, q = 1, A],
-    layouts (nby, nbx) 0 square / 1 staggered, pitches (nby, nbx))."""
+    staggered array with its own pitch in [pitch[0], pitch[1]] (the mixed large-page workload:
+    32768^2, pitch 24-48 px, ~1.2M holes). The pitch is drawn so that the hole density
+    1 / pitch^2 is uniform over its range (pitch = U[1/48^2, 1/24^2]^(-1/2)): a uniform pitch
+    would put ~1.0M holes on the page. Holes keep half a pitch from their block's edges;
+    r ~ U[r_frac] * pitch, sigma ~ U[sigma], centre jitter N(0, jitter^2). Returns (holes (N, 6)
+    float32 [x, y, r, sigma, q = 1, A], layouts (nby, nbx) 0 square / 1 staggered, pitches
+    (nby, nbx))."""
     rng = np.random.default_rng(seed)
     nby, nbx = -(-H // region), -(-W // region)
     layouts = rng.integers(0, 2, size=(nby, nbx))
-    pitches = rng.uniform(pitch[0], pitch[1], size=(nby, nbx))
+    dens = rng.uniform(1.0 / pitch[1] ** 2, 1.0 / pitch[0] ** 2, size=(nby, nbx))
+    pitches = 1.0 / np.sqrt(dens)
     out = []
     for by in range(nby):
         for bx in range(nbx):
@@ -204,7 +254,7 @@ def mixed_page_holes(seed: int = 0, H: int = 32768, W: int = 32768, region: int 
 def render_page(holes, H: int, W: int, *, background: float = 500.0, read_sigma: float = 30.0,
                 seed: int = 0, device: int = 0):
     """Render a hole list (N, 6) [x, y, r, sigma, q, A] into an (H, W) uint16 device tensor
-    (spx_render_page: point-sampled erf disks, normal-limit photon noise + read noise)."""
+    (spx_render_page: point-sampled erf disks, Poisson photon noise + read noise)."""
     import torch
 
     from . import _lib

@@ -63,7 +63,10 @@ typedef struct SpxFitOptions {
                              / 64 rounded up), m = 1 for ellipses. Features of buckets not
                              launched are not fitted (NaN, status too few). 0 = all ten,
                              so callers that know their patch sizes skip empty launches  */
-    int32_t reserved;
+    int32_t learned_iters; /* with init: > 0 = the init is a learned proposal; a feature whose fit
+                             from it has not converged within this many step attempts is fitted
+                             again from the moment start (max_iter attempts) in the same launch,
+                             and iters_out counts the attempts of both. 0 = init is a plain start */
 } SpxFitOptions;
 
 /* Options of the edge-erf refinement (defaults of the reference's
@@ -264,9 +267,14 @@ int spx_seed_neighbours(const float* seeds, int32_t K, int64_t H, int64_t W, flo
 /* Work counters of the last spx_fit_batch call issued on this thread (no reference
  * counterpart; used to state algorithmic FLOP rates): out[0] = sum over features of pixel
  * passes x patch pixels for 6-parameter fits, out[1] the same for 7-parameter fits,
- * out[2] = pixel passes, out[3] = LM step attempts. out: host uint64[4]. Synchronises
- * the stream. */
+ * out[2] = pixel passes, out[3] = LM step attempts. out: host uint64[4]. Waits for the
+ * stream of that fit, then copies on `stream` and synchronises it. */
 int spx_fit_stats(uint64_t* out, spx_stream_t stream);
+
+/* The first n (1..8) counters of the last fit issued on this thread: those of spx_fit_stats,
+ * then out[4] = features whose learned start did not converge within opt->learned_iters and
+ * were fitted again from the moment start; out[5..7] = 0. out: host uint64[n]. */
+int spx_fit_counters(uint64_t* out, int32_t n, spx_stream_t stream);
 
 /* ---- learned initial guess (no reference counterpart; task-defined CNN) ------------------ */
 /* Parameter count of the CNN: conv3x3 1->16->32->64->64 (stride 2) + fc 256->128->7, each
@@ -316,7 +324,7 @@ int spx_cnn_grad(const uint16_t* x, const float* target, const float* weight, in
 
 /* Whole synthetic page: holes (n, 6) f32 [x, y, r, s, q, A] in image coordinates (device),
  * each I += A erfc((rho - r) / (sqrt2 s)) / 2 with rho = sqrt(dx^2 + (dy / q)^2), on a constant
- * background, then N(I, I) photon noise (normal limit) + N(0, read_sigma^2), rounded and
+ * background, then Poisson(I) photon noise + N(0, read_sigma^2), rounded and
  * clipped to uint16 (device image, row pitch in elements). No reference counterpart: the
  * generator of the large benchmark pages, rendered on the device. */
 int spx_render_page(uint64_t seed, const float* holes, int32_t n, int64_t H, int64_t W, float background,
@@ -324,7 +332,7 @@ int spx_render_page(uint64_t seed, const float* holes, int32_t n, int64_t H, int
 
 /* Render n synthetic tiles of side `tile` into a mosaic (tile t at column t % cols, row
  * t / cols): one hole at the tile centre plus its array neighbours, erf-disk / ellipse
- * model, photon noise in its normal limit + read noise, uint16. truth_out (n, 7)
+ * model, Poisson photon noise + read noise, uint16. truth_out (n, 7)
  * [x0, y0, r, s, A, B, q] of the centre hole in mosaic coordinates. img, truth: device.
  * No reference counterpart (training data for the learned start). */
 int spx_render_tiles(uint64_t seed, int32_t n, int32_t tile, const SpxTileSpec* spec, uint16_t* img,

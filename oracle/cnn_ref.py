@@ -7,8 +7,10 @@ layer l has a (K_l x N_l) weight matrix (row index (ky * 3 + kx) * Cin + c for a
 convolution) followed by N_l biases; activations are NHWC.
 
 ``bf16_round`` reproduces the GPU's round-to-nearest-even BF16 so a test can feed both
-sides the same operands. Pinned by central finite differences of the loss
-(tests/test_oracle_cnn.py).
+sides the same operands. Pinned by central finite differences of the loss, torch's BF16
+rounding, a hand-summed convolution output, and for ``prepare`` by torch's bilinear
+resampling (align_corners=False), the exact resample of an affine ramp with its closed-form
+ring mean and range, and a flat patch mapping to zero (tests/test_oracle_cnn.py).
 """
 
 from __future__ import annotations

@@ -271,9 +271,10 @@ def to_output(p_internal, ellipse, cx, cy):
     return np.array([p[0] + cx, p[1] + cy, p[2], p[3], p[4], p[-1], q], dtype=np.float64)
 
 
-def fit_batch(img, seeds, half, model, init=None, neighbours=None, **opts):
+def fit_batch(img, seeds, half, model, init=None, neighbours=None, only=None, **opts):
     """Oracle counterpart of spx_fit_batch (spx_fit_batch_cells with ``neighbours`` (K, nb)):
-    (params (K,7), rms, iterations, status)."""
+    (params (K,7), rms, iterations, status). ``only``: indices of the features to fit (the
+    others stay NaN with status TOO_FEW); the cell masks still see every seed."""
     img = np.asarray(img)
     seeds = np.asarray(seeds, dtype=np.float64).reshape(-1, 2)
     K = seeds.shape[0]
@@ -283,7 +284,11 @@ def fit_batch(img, seeds, half, model, init=None, neighbours=None, **opts):
     rms = np.full(K, np.nan)
     iters = np.zeros(K, dtype=np.int64)
     status = np.zeros(K, dtype=np.int64)
-    for k in range(K):
+    todo = range(K) if only is None else np.asarray(only, dtype=np.int64).ravel()
+    if only is not None:
+        status[:] = TOO_FEW
+    for k in todo:
+        k = int(k)
         h = int(half[k])
         ell = bool(model[k] == 1)
         if h < 2 or h > 31 or model[k] not in (0, 1):  # the kernel's accepted range

@@ -7,8 +7,10 @@ tile_h - overlap, global Otsu on img[::ds, ::ds] times thr_scale, per-strip fixe
 threshold, centres kept in [y0 + overlap//2, y1 - overlap//2) except at the first / last
 strip) and dedupe_centers (reference subpx/centers.py:335-378: O(N^2) radius union-find
 with union by rank and path halving, clusters replaced by their means, ordered by root).
-Pinned by: a strip size covering the image reduces to the untiled oracle; dedupe of
-well-separated points is the identity and of exact duplicates their mean
+Pinned by: a strip size covering the image reduces to the untiled oracle; three strips
+over a page whose holes lie inside the strips that own them equal the untiled oracle, and
+holes placed exactly on the band boundaries are kept once each (before the dedupe); dedupe
+of well-separated points is the identity and of exact duplicates their mean
 (tests/test_oracle_tiling.py).
 """
 
@@ -61,7 +63,9 @@ def dedupe_centers(centers_xy, eps: float = 1.5) -> np.ndarray:
 
 def detect_centers_tiled_logquad(img, *, tile_h, overlap=128, otsu_downsample=4, thr_scale=0.8,
                                  area_min=1, area_max=50, pad=3, connectivity=8, invert=False,
-                                 dedupe_eps=1.5, refine="logquad", logquad_variant="cpu"):
+                                 dedupe_eps=1.5, refine="logquad", logquad_variant="cpu", raw=False):
+    """Centres of the tiled path after the dedupe; ``raw=True`` returns the centres the strips
+    kept, before the dedupe."""
     g = np.asarray(img)
     H = g.shape[0]
     ds = max(1, int(otsu_downsample))
@@ -88,4 +92,5 @@ def detect_centers_tiled_logquad(img, *, tile_h, overlap=128, otsu_downsample=4,
         y0 += step
     if not out:
         return np.zeros((0, 2))
-    return dedupe_centers(np.vstack(out), eps=dedupe_eps)
+    kept = np.vstack(out)
+    return kept if raw else dedupe_centers(kept, eps=dedupe_eps)

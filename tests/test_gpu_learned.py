@@ -43,12 +43,21 @@ def test_training_gains_over_moment_start():
         res, timing = learned.fit_learned(net, ev.img, ev.seeds, ev.half, ev.model, lm_iters=4)
         p0, s0, i0 = plain["params"].cpu().numpy(), plain["status"].cpu().numpy(), plain["iters"].cpu().numpy()
         p1, s1, i1 = res["params"].cpu().numpy(), res["status"].cpu().numpy(), res["iters"].cpu().numpy()
+        fb = res["fallback"].cpu().numpy()
     both = (s0 == 0) & (s1 == 0)
     assert both.mean() > 0.97
     # same minimum: centres agree to 1e-3 px
     assert np.abs(p0[both, :2] - p1[both, :2]).max() < 1e-3
-    # the trained proposal needs fewer LM steps than the moment start
-    print("mean iterations: moment start", i0.mean(), "learned", i1.mean(), "fallback", timing.n_fallback)
-    assert i1[both].mean() < i0[both].mean()
-    # after this short training a part of the features still needs the plain fallback
-    assert timing.n_fallback < 0.9 * len(s1)
+    # the fused launch counts the fallbacks it refitted, and iters of those include the learned attempts
+    assert timing.n_fallback == int(fb.sum())
+    assert (i1[fb] > 4).all()
+    # where the trained proposal converges within its budget it needs fewer LM steps than the
+    # moment start on the same features
+    keep = both & ~fb
+    print("mean iterations on proposal-converged features: moment start", i0[keep].mean(), "learned",
+          i1[keep].mean(), "fallback", timing.n_fallback)
+    assert keep.sum() > 0.1 * len(s1)
+    assert i1[keep].mean() < i0[keep].mean()
+    # a fallback feature equals the plain fit from the moment start bit for bit
+    fbb = fb & (s0 == 0)
+    assert np.array_equal(p1[fbb], p0[fbb])

@@ -57,3 +57,51 @@ def test_conv_layout_against_direct_sum():
     Wt = layers[0][0].reshape(3, 3, 1, 16).permute(3, 2, 0, 1)
     out = torch.nn.functional.conv2d(torch.tensor(x).reshape(1, 1, 32, 32), Wt, layers[0][1], stride=2, padding=1)
     assert abs(out[0, 3, 0, 1].item() - acc) < 1e-12
+
+
+def _interp_reference(P):
+    """The 32 x 32 bilinear resample of a square patch by torch's own routine (align_corners
+    False: source = (dst + 0.5) * side / 32 - 0.5, clamped at the borders), an implementation
+    independent of the oracle's loops."""
+    t = torch.tensor(np.asarray(P, dtype=np.float64))[None, None]
+    return torch.nn.functional.interpolate(t, size=(32, 32), mode="bilinear", align_corners=False)[0, 0].numpy()
+
+
+def test_prepare_matches_library_bilinear_resample():
+    rng = np.random.default_rng(4)
+    img = rng.integers(0, 4000, size=(160, 170)).astype(np.uint16)
+    for seed, half in (((80.3, 70.6), 15), ((40.0, 50.0), 7), ((100.5, 90.4), 31), ((60.0, 64.0), 4)):
+        x, norm = cnn_ref.prepare(img, seed, half)
+        cx, cy, side, B0, A0 = norm
+        P = img[int(cy) - half:int(cy) + half + 1, int(cx) - half:int(cx) + half + 1].astype(np.float64)
+        assert P.shape == (side, side)
+        ring = np.concatenate([P[0], P[-1], P[1:-1, 0], P[1:-1, -1]])
+        assert B0 == ring.mean() and A0 == max(P.max() - B0, 1.0)
+        np.testing.assert_allclose(x, (_interp_reference(P) - B0) / A0, rtol=0, atol=1e-12)
+
+
+def test_prepare_reproduces_an_affine_ramp():
+    # bilinear resampling is exact on affine data: at output (i, j) the patch coordinate is
+    # (j + 0.5) side / 32 - 0.5 (clamped to [0, side - 1]); the ring mean of a ramp is its
+    # centre value and its maximum lies half a side off in both directions
+    H, W = 200, 220
+    a, bx, by = 1000.0, 3.0, -2.0
+    yy, xx = np.mgrid[0:H, 0:W]
+    img = np.rint(a + bx * xx + by * yy + 1000).astype(np.uint16)  # integer-valued, exact
+    for (sx, sy), half in (((100.2, 90.7), 15), ((60.0, 60.0), 31), ((30.4, 150.0), 9)):
+        x, norm = cnn_ref.prepare(img, (sx, sy), half)
+        cx, cy = np.rint(sx), np.rint(sy)
+        side = 2 * half + 1
+        centre = a + 1000 + bx * cx + by * cy
+        A0 = (abs(bx) + abs(by)) * half
+        p = np.clip((np.arange(32) + 0.5) * side / 32.0 - 0.5, 0, side - 1)
+        expect = (bx * (p[None, :] - half) + by * (p[:, None] - half)) / A0
+        assert norm[3] == centre and norm[4] == A0
+        np.testing.assert_allclose(x, expect, rtol=0, atol=1e-12)
+
+
+def test_prepare_of_a_flat_patch_is_zero():
+    img = np.full((64, 64), 777, np.uint16)
+    x, norm = cnn_ref.prepare(img, (32.0, 30.0), 12)
+    assert norm[3] == 777.0 and norm[4] == 1.0
+    assert np.array_equal(x, np.zeros((32, 32)))

@@ -133,3 +133,25 @@ def test_cell_edge_is_a_four_neighbour_outside():
         for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
             out |= ~erfdisk.cell_mask(gx + dx, gy + dy, seeds, f, others)
         np.testing.assert_array_equal(inside & erfdisk.cell_edge(gx, gy, seeds, f, others), inside & out)
+
+
+def test_cell_radius_lists_every_seed_that_can_cut_the_patch():
+    # seed (99.51, 99.51): patch centred on (100, 100), its far corner (100 + h, 100 + h) lies
+    # sqrt(2) (h + 0.49) from the seed. A neighbour on that diagonal at 2 sqrt(2) h + 1.2 is nearer
+    # to the corner than the seed is, so the corner leaves the cell: the neighbour must be listed.
+    from imageprocessor_new_amd.fit import cell_radius
+
+    for h in (7, 15, 31):
+        s0 = np.array([99.51, 99.51])
+        D = 2 * np.sqrt(2) * h + 1.2
+        s1 = s0 + D / np.sqrt(2)
+        seeds = np.array([s0, s1], dtype=np.float32)
+        nbr = erfdisk.seed_neighbours(seeds, cell_radius(h), 8)
+        assert nbr[0, 0] == 1, (h, nbr[0])
+        corner = erfdisk.cell_mask(np.array([100.0 + h]), np.array([100.0 + h]), seeds, 0, nbr[0])
+        assert not corner[0]
+        # the seed that the radius leaves out can not take any patch pixel or 4-neighbour
+        far = s0 + (cell_radius(h) + 0.01) / np.sqrt(2)
+        seeds2 = np.array([s0, far], dtype=np.float32)
+        gy, gx = np.mgrid[100 - h - 1:100 + h + 2, 100 - h - 1:100 + h + 2]
+        assert erfdisk.cell_mask(gx.astype(float), gy.astype(float), seeds2, 0, np.array([1])).all()

@@ -37,3 +37,33 @@ def test_tiling_with_one_strip_is_untiled():
                                                area_min=20, area_max=2000)
     one = tiling.detect_centers_tiled_logquad(arr.image, tile_h=400, overlap=64, area_min=20, area_max=2000)
     np.testing.assert_allclose(np.sort(one, axis=0), np.sort(tiling.dedupe_centers(ref), axis=0))
+
+
+def _boundary_page():
+    # 600 rows in strips of 256 stepping 192 (overlap 64): strips [0, 256), [192, 448),
+    # [384, 600); bands kept [0, 224), [224, 416), [416, 600). Noise-free Gaussian spots whose
+    # log-quadratic centre is exact, some exactly on the band boundaries y = 224 and 416 and one
+    # in the first / last band's outer 32 rows.
+    H, W = 600, 160
+    ys = [20.0, 60.3, 150.0, 200.0, 224.0, 224.0, 260.7, 330.0, 416.0, 416.0, 470.2, 530.0, 585.0]
+    xs = [30.0, 80.0, 40.0, 120.0, 25.0, 95.0, 60.0, 130.0, 40.0, 110.0, 75.0, 20.0, 100.0]
+    yy, xx = np.mgrid[0:H, 0:W]
+    g = np.full((H, W), 500.0)
+    for x, y in zip(xs, ys):
+        g += 3000.0 * np.exp(-((xx - x) ** 2 + (yy - y) ** 2) / (2 * 2.5 ** 2))
+    return np.rint(g).astype(np.uint16), np.column_stack([xs, ys])
+
+
+def test_three_strips_equal_the_untiled_path_and_boundary_holes_are_kept_once():
+    img, truth = _boundary_page()
+    kw = dict(area_min=5, area_max=2000, pad=3)
+    t = segment.otsu_u16(img, 4)
+    untiled, _, _ = segment.detect_centers_logquad(img, threshold_value=int(np.floor(t * 0.8)), **kw)
+    raw = tiling.detect_centers_tiled_logquad(img, tile_h=256, overlap=64, raw=True, **kw)
+    # every hole exactly once before the dedupe: none lost at a cut, none kept by two strips
+    assert raw.shape[0] == truth.shape[0]
+    d = np.hypot(*(raw[:, None, :] - truth[None, :, :]).transpose(2, 0, 1))
+    assert (d.min(axis=0) < 0.05).all() and (d.min(axis=1) < 0.05).all()
+    tiled = tiling.detect_centers_tiled_logquad(img, tile_h=256, overlap=64, **kw)
+    key = lambda a: a[np.lexsort((a[:, 0], np.round(a[:, 1], 6)))]
+    np.testing.assert_allclose(key(tiled), key(untiled), rtol=0, atol=1e-9)

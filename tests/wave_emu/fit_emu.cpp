@@ -183,7 +183,7 @@ extern "C" int emu_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, in
         }
         order[2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0)].push_back(f);
     }
-    unsigned long long stats[4] = {0, 0, 0, 0};
+    unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = 0; b < kKB; ++b) {
         int32_t count = (int32_t)order[b].size();
         int32_t queue = 0;
