@@ -326,4 +326,61 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
                              rms, iters, status, stream);
 }
 
+// ---- the allocation-free fit -------------------------------------------------------------
+// Scratch of spx_fit_batch_noalloc: a file-scope device array, one copy per GPU at its own
+// address. Host code never passes the array's name to a kernel (on the host it names a
+// stand-in, not device memory): it passes the address hipGetSymbolAddress returns for the
+// current device, looked up once per device at spx_init or at the first call.
+static __device__ __align__(256) unsigned char scratch[kFitHead + sizeof(int32_t) * SPX_FIT_NOALLOC_MAX_FEATURES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // by device ordinal
+}  // namespace
+
+int fit_scratch_init(int dev) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (g_scratch_ptr[dev] != nullptr) return SPX_OK;
+    void* p = nullptr;
+    SPX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) {
+        set_error("fit scratch: hipGetSymbolAddress returned no address");
+        return SPX_E_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+    // The launch shapes of every bucket kernel are looked up now, so that calls of the
+    // allocation-free fit add no host-side cache entries either.
+    int cus = 0;
+    SPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    for (int b = 0; b < kKernBuckets; ++b)
+        for (int w = 1; w <= 4; ++w) (void)resident_blocks(dev, b, w * fitdev::kWave);
+    return SPX_OK;
+}
+
+int fit_batch_noalloc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                             const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions& opt,
+                             int32_t K, const int32_t* nbr, int32_t nb, float* params, double* centers, float* rms,
+                             int16_t* iters, uint8_t* status, hipStream_t stream) {
+    if (K > SPX_FIT_NOALLOC_MAX_FEATURES) {
+        set_error("spx_fit_batch_noalloc: " + std::to_string(K) + " features exceed the capacity of its fixed scratch ("
+                  + std::to_string(SPX_FIT_NOALLOC_MAX_FEATURES) + ")");
+        return SPX_E_ARG;
+    }
+    if (K == 0) return SPX_OK;
+    int dev = 0;
+    SPX_HIP(hipGetDevice(&dev));
+    int rc = fit_scratch_init(dev);
+    if (rc != SPX_OK) return rc;
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = g_scratch_ptr[dev];
+    }
+    // the counters sit in the head's first 64 bytes; counts, cursors and queues follow them
+    unsigned long long* stats = static_cast<unsigned long long*>(base);
+    return fit_batch_enqueue(base, stats, img, H, W, pitch, seeds, half, model, init, opt, K, nbr, nb, params,
+                             centers, rms, iters, status, stream);
+}
+
 }  // namespace spx

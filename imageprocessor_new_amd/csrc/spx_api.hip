@@ -77,6 +77,11 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
                      double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream);
 
 int fit_stats_copy(uint64_t* out, int n, hipStream_t stream);
+int fit_scratch_init(int dev);
+int fit_batch_noalloc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                             const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions& opt,
+                             int32_t K, const int32_t* nbr, int32_t nb, float* params, double* centers, float* rms,
+                             int16_t* iters, uint8_t* status, hipStream_t stream);
 int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
                 int32_t* thresh_out, hipStream_t stream);
 int threshold_cc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
@@ -367,7 +372,7 @@ int spx_init(int device) {
     SPX_REQUIRE(device >= 0 && device < n, "spx_init: device ordinal out of range");
     SPX_HIP(hipSetDevice(device));
     SPX_HIP(hipFree(nullptr));  // creates the context
-    return SPX_OK;
+    return spx::fit_scratch_init(device);
 }
 
 void spx_release(void) {
@@ -441,6 +446,19 @@ int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch
     return spx::fit_batch_launch(img, H, W, pitch, seeds, half, model, init, *opt, K, neighbours,
                                  neighbours ? nb : 0, params_out, centers_out, rms_out, iters_out, status_out,
                                  spx::as_stream(stream));
+}
+
+int spx_fit_batch_noalloc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                          const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
+                          int32_t K, const int32_t* neighbours, int32_t nb, float* params_out, double* centers_out,
+                          float* rms_out, int16_t* iters_out, uint8_t* status_out, spx_stream_t stream) {
+    spx::clear_error();
+    int rc = check_fit_args("spx_fit_batch_noalloc", img, H, W, pitch, seeds, half, model, opt, K, neighbours, nb,
+                            params_out, rms_out, iters_out, status_out);
+    if (rc != SPX_OK) return rc;
+    return spx::fit_batch_noalloc_launch(img, H, W, pitch, seeds, half, model, init, *opt, K, neighbours,
+                                         neighbours ? nb : 0, params_out, centers_out, rms_out, iters_out, status_out,
+                                         spx::as_stream(stream));
 }
 
 int spx_otsu_u16(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,

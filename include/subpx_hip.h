@@ -257,6 +257,22 @@ int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch
                         double* centers_out, float* rms_out, int16_t* iters_out, uint8_t* status_out,
                         spx_stream_t stream);
 
+/* Largest batch of spx_fit_batch_noalloc. */
+#define SPX_FIT_NOALLOC_MAX_FEATURES (1 << 20)
+
+/* spx_fit_batch_cells (neighbours may be NULL) that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Its bucket counters, work queues,
+ * feature lists and work counters live in a fixed device array of 4 MiB + 256 B per GPU, held
+ * until the process ends, so K is at most SPX_FIT_NOALLOC_MAX_FEATURES (SPX_E_ARG beyond).
+ * Every call on a GPU shares that array and clears what must start at zero: calls that
+ * overlap on one GPU (other streams or threads) race and must be serialised by the caller.
+ * Results equal spx_fit_batch_cells's; spx_fit_stats reads the array's counters. */
+int spx_fit_batch_noalloc(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                          const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
+                          int32_t K, const int32_t* neighbours, int32_t nb, float* params_out, double* centers_out,
+                          float* rms_out, int16_t* iters_out, uint8_t* status_out, spx_stream_t stream);
+
 /* The nb nearest other seeds of every seed within `radius` px (ordered by distance, ties by
  * index; -1 where fewer exist), by a uniform-grid hash on the device. seeds (K, 2) f32 in an
  * H x W image, out (K, nb) int32: device. nb in [1, 8]. A seed with a non-finite coordinate
