@@ -118,6 +118,9 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--train-patches", type=int, default=1_000_000)
+    ap.add_argument("--learned-iters", type=int, default=4, help="LM attempts after the CNN proposal (<= 4)")
+    ap.add_argument("--budget-sweep", default="1,2,3,4",
+                    help="learned workload: also time the learned leg at these attempt budgets (reported only)")
     ap.add_argument("--workload", choices=("learned", "uneven", "fit", "tiled", "holes", "cnn", "detect", "outline"),
                     default="learned",
                     help="learned (default): CNN start + <=4 LM + fallback vs plain LM on the 16384^2 staggered "
@@ -573,26 +576,56 @@ def run_learned(args) -> int:
         p_status, p_iters = out["status"].cpu().numpy(), out["iters"].cpu().numpy()
 
         for _ in range(args.warmup):
-            learned.fit_learned(net, page.image, f.seeds, f.half, f.model, lm_iters=4, neighbours=f.nbr)
+            learned.fit_learned(net, page.image, f.seeds, f.half, f.model, lm_iters=args.learned_iters,
+                                neighbours=f.nbr, bucket_mask=mask)
         walls, parts = [], []
         for _ in range(args.steps):
-            res_l, tm = learned.fit_learned(net, page.image, f.seeds, f.half, f.model, lm_iters=4, neighbours=f.nbr)
+            res_l, tm = learned.fit_learned(net, page.image, f.seeds, f.half, f.model, lm_iters=args.learned_iters,
+                                            neighbours=f.nbr, bucket_mask=mask)
             walls.append(tm.seconds)
             parts.append((tm.propose_s, tm.refine_s, tm.fallback_s, tm.n_fallback))
+        learned_stats = read_fit_stats()
         l_params, l_centers = res_l["params"].cpu().numpy(), res_l["centers"].cpu().numpy()
         l_status, l_iters = res_l["status"].cpu().numpy(), res_l["iters"].cpu().numpy()
+        l_fb = res_l["fallback"].cpu().numpy()
+
+        sweep = {}
+        for b in [int(v) for v in args.budget_sweep.split(",") if v.strip()]:
+            ws = []
+            for _ in range(max(2, args.steps)):
+                rb, tb = learned.fit_learned(net, page.image, f.seeds, f.half, f.model, lm_iters=b, neighbours=f.nbr,
+                                             bucket_mask=mask)
+                ws.append(tb.seconds)
+            sweep[str(b)] = {"ms": 1e3 * float(np.median(ws)), "fit_ms": 1e3 * tb.refine_s,
+                             "fallback_rate": tb.n_fallback / K,
+                             "mean_iterations": float(rb["iters"].float().mean().item())}
+        log(f"budget sweep: {sweep}")
+        # control: the same budget, judge and fallback from the moment start instead of the CNN
+        ctrl_opts = _lib.fit_options(bucket_mask=mask, learned_iters=args.learned_iters)
+        ctrl = fit_erf_disks_device(page.image, f.seeds, f.half, f.model, options=ctrl_opts, neighbours=f.nbr)
+        ctrl_ms, ctrl_wall, _ = timed(lambda: fit_erf_disks_device(page.image, f.seeds, f.half, f.model,
+                                                                   options=ctrl_opts, neighbours=f.nbr, out=ctrl),
+                                      args.steps)
+        ctrl_counts = np.zeros(5, dtype=np.uint64)
+        _lib.call("spx_fit_counters", int(ctrl_counts.ctypes.data), 5, int(torch.cuda.current_stream().cuda_stream))
+        c_params, c_centers = ctrl["params"].cpu().numpy(), ctrl["centers"].cpu().numpy()
+        c_status = ctrl["status"].cpu().numpy()
         img_host = page.image.cpu().numpy()
         seeds_np, nbr_np = f.seeds.cpu().numpy(), f.nbr.cpu().numpy()
     plain_ms = 1e3 * plain_wall / args.steps
     learned_ms = 1e3 * float(np.mean(walls))
     acc = oracle_sample(img_host, seeds_np, half_np, model_np, nbr_np,
-                        {"plain": (p_params, p_centers, p_status), "learned": (l_params, l_centers, l_status)},
+                        {"plain": (p_params, p_centers, p_status), "learned": (l_params, l_centers, l_status),
+                         "control_moment_start_same_judge": (c_params, c_centers, c_status)},
                         args.cpu_sample)
     log(f"oracle sample: {acc}")
     plain_frac = acc["legs"]["plain"]["frac_within_1e-3px"] * float((p_status <= 1).mean())
     learned_frac = acc["legs"]["learned"]["frac_within_1e-3px"] * float((l_status <= 1).mean())
+    ctrl_frac = acc["legs"]["control_moment_start_same_judge"]["frac_within_1e-3px"] * float((c_status <= 1).mean())
     plain_per_s = K * plain_frac / (plain_ms * 1e-3)
     learned_per_s = K * learned_frac / (learned_ms * 1e-3)
+    ctrl_per_s = K * ctrl_frac / (1e3 * ctrl_wall / args.steps * 1e-3)
+    lflops = fit_flops(learned_stats)
     flops = fit_flops(plain_stats)
     achieved = flops / (plain_kernel_ms * 1e-3) / 1e12
     result = {
@@ -612,7 +645,21 @@ def run_learned(args) -> int:
                                  "lm_from_proposal": 1e3 * float(np.mean([p[1] for p in parts])),
                                  "fallback": 1e3 * float(np.mean([p[2] for p in parts]))},
             "fallback_features": int(parts[-1][3]), "fallback_rate": float(parts[-1][3]) / K,
-            "mean_iterations": {"plain": float(p_iters.mean()), "learned_incl_fallback": float(l_iters.mean())},
+            "mean_iterations": {"plain": float(p_iters.mean()), "learned_incl_fallback": float(l_iters.mean()),
+                                "learned_accepted": float(l_iters[~l_fb].mean()) if (~l_fb).any() else None,
+                                "plain_on_learned_accepted": float(p_iters[~l_fb].mean()) if (~l_fb).any() else None},
+            "pixel_passes_per_feature": {"plain": float(plain_stats[2]) / K, "learned": float(learned_stats[2]) / K},
+            "learned_fit_kernel_tflops": lflops / (1e3 * float(np.mean([p[1] for p in parts])) * 1e-3) / 1e12,
+            "judge": "after <=4 attempts from the proposal the fit is accepted when the Gauss-Newton model predicts "
+                     "the minimum's cost within 1e-6 of the current cost (that step then applied), else refitted "
+                     "from the moment start in the same launch",
+            "learned_iters": args.learned_iters,
+            "budget_sweep": sweep,
+            "control_moment_start_same_judge": {
+                "features_per_s": ctrl_per_s, "ms_per_step": 1e3 * ctrl_wall / args.steps, "kernel_ms": ctrl_ms,
+                "fallback_features": int(ctrl_counts[4]),
+                "note": "the learned leg's budget, judge and fallback from the moment start: the part of the "
+                        "learned leg's gain that the CNN proposal itself brings is learned / control"},
             "accuracy": acc["legs"],
             "train": {k: tinfo[k] for k in ("samples", "pages", "page_seeds", "holes_seen", "data_seconds",
                                             "train_seconds", "epochs", "batch", "lr")},

@@ -174,7 +174,7 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
 
 def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity, fit_half, fit_options,
                       model_code, fit_cells, morph_open=0, morph_close=0, events=None, net=None, lm_iters=4,
-                      fallbacks=None):
+                      fallbacks=None, capture=None):
     """Segmentation -> CC -> area filter -> erf fit of one strip, kept on the device: the rows of
     _detect_single_tile's erf group (label order, same seeds, half sides, neighbour lists and
     fit), with every component fitted in one launch and the filtered-out rows (background,
@@ -183,7 +183,9 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
     The only host synchronisation is the component count. With ``net`` (a cnn.LearnedStart)
     the fit starts from the network's proposal for at most ``lm_iters`` steps, and the rows not
     converged then are fitted again from the moment start by the same wave in the same launch
-    (``fallbacks`` collects their count per strip as device tensors)."""
+    (``fallbacks`` collects their count per strip as device tensors). ``capture``: a list that
+    receives the strip's fit inputs and results as host arrays (seeds, half, model, nbr, params,
+    centers, status, ok; synchronising), for checking the fits against a reference."""
     from .fit import cell_radius, fit_erf_disks_device, seed_neighbours_device
 
     def mark(name, e0):
@@ -243,6 +245,10 @@ def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity
             fallbacks.append((valid & (res["iters"] > int(lm_iters))).sum())
     ok = valid & (status <= 1) & torch.isfinite(params[:, :2]).all(dim=1)
     mark("refine", e0)
+    if capture is not None:
+        capture.append({"seeds": seeds.cpu().numpy(), "half": half.cpu().numpy(), "model": model.cpu().numpy(),
+                        "nbr": None if nbr is None else nbr.cpu().numpy(), "params": params.cpu().numpy(),
+                        "centers": centers.cpu().numpy(), "status": status.cpu().numpy(), "ok": ok.cpu().numpy()})
     return centers, ok
 
 
@@ -280,6 +286,7 @@ def detect_centers(
     logquad_variant: str = "gpu",
     learned_start=None,
     learned_iters: int = 4,
+    fit_inputs_out: list | None = None,
 ) -> CenterResult:
     """Detect and refine centres (reference subpx/centers.py:130-220).
 
@@ -293,7 +300,9 @@ def detect_centers(
     ("gpu", default) or the CPU backend's ("cpu") log-quadratic algorithm.
     ``learned_start`` (a trained cnn.LearnedStart, erf methods only) starts every fit from the
     network's proposal for at most ``learned_iters`` LM steps and refits the features not
-    converged then from the moment start.
+    converged then from the moment start. ``fit_inputs_out`` (a list; erf refinement with a
+    global threshold) receives, per strip, its row offset ``y0`` and its fits' inputs and results
+    as host arrays: a diagnostic for checking the fits against a reference.
     """
     b = resolve_backend(backend)
     if logquad_variant not in ("gpu", "cpu"):
@@ -381,7 +390,9 @@ def detect_centers(
                     area_max=area_max, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options,
                     model_code=_ERF_METHODS[refine], fit_cells=bool(fit_cells), morph_open=int(morph_open),
                     morph_close=int(morph_close), events=events, net=learned_start, lm_iters=int(learned_iters),
-                    fallbacks=fallbacks)
+                    fallbacks=fallbacks, capture=fit_inputs_out)
+                if fit_inputs_out is not None:
+                    fit_inputs_out[-1]["y0"] = y0
                 n_tiles += 1
                 half = int(overlap) // 2
                 keep_lo = y0 if y0 == 0 else (y0 + half)

@@ -488,21 +488,30 @@ struct WaveStats {
     unsigned long long fallbacks;  // learned starts not converged within their budget
 };
 
+// Residual judge of a budgeted fit (below): the fit is accepted when the Gauss-Newton model
+// at its last point predicts that the minimum's cost lies within this fraction of the current
+// cost, the FP32 rounding floor of the cost sums (kAcceptRtol), so its residual does not
+// exceed the converged fit's by more than the sums can resolve.
+constexpr double kJudgeRtol = 1.0e-6;
+
 // Fit of one feature by the whole wave. The patch registers are filled by the caller;
 // `masked`: the patch is restricted to the feature's Voronoi cell. `from_init`: start from
-// the caller's parameters (a.init), else from the patch moments. With a learned budget
-// (a.opt.learned_iters > 0) a fit from the init that has not converged within that many step
-// attempts writes nothing and returns its attempt count: the caller fits the feature again
-// from the moment start (the plain-start fallback), passing that count as `it_before` so that
-// iters_out counts the attempts of both. Returns -1 otherwise.
+// the caller's parameters (a.init), else from the patch moments. `budget`: a learned-start
+// pass (a.opt.learned_iters > 0) of at most learned_iters step attempts with the residual
+// judge (judge_accepts) at the start point and after every accepted step: at the first point
+// it accepts, its Gauss-Newton step is applied (as the plain rule applies its last
+// sub-tolerance step) and the fit is converged; the step-size rule still ends a fit as well.
+// A budgeted fit that is not converged then writes nothing and returns its attempt count: the
+// caller fits the feature again from the moment start without a budget (the plain-start
+// fallback), passing that count as `it_before`, so that iters_out counts the attempts of both.
+// Returns -1 otherwise.
 template <int NPL, int P>
 SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>& pt,
-                               float cx, float cy, bool masked, bool from_init, int it_before,
+                               float cx, float cy, bool masked, bool from_init, bool budget, int it_before,
                                WaveStats& ws) {
     using L = Layout<P>;
     const int lane = spx_lane();
     const float npix = pt.npix;
-    const bool learned = a.opt.learned_iters > 0;
     Params<P> cur;
     uint8_t status = SPX_FIT_CONVERGED;
     int it = 0;
@@ -610,25 +619,44 @@ SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>
             status = SPX_FIT_MAX_ITER;
             if (!isfinite(cost)) status = SPX_FIT_DIVERGED;
             bool any_solve = false;
-            const int max_iter = (from_init && learned) ? a.opt.learned_iters : a.opt.max_iter;
-            while (status == SPX_FIT_MAX_ITER && it < max_iter) {
-                ++it;
+            const int max_iter = budget ? a.opt.learned_iters : a.opt.max_iter;
+            bool fresh = budget;  // S belongs to a point the judge has not seen yet
+            // One solve per turn: a judging turn (fresh sums of a budgeted fit) solves the
+            // undamped system for the judge and counts no attempt; the others take an LM step.
+            while (status == SPX_FIT_MAX_ITER && (it < max_iter || fresh)) {
+                const bool judging = fresh;
+                fresh = false;
+                if (!judging) ++it;
                 double delta[P];
-                if (!damped_solve<P>(S, (double)npix, lambda, delta)) {
-                    lambda *= (double)a.opt.lambda_up;
+                if (!damped_solve<P>(S, (double)npix, judging ? 0.0 : lambda, delta)) {
+                    if (!judging) lambda *= (double)a.opt.lambda_up;
                     continue;
                 }
-                any_solve = true;
                 Params<P> tr;
 #pragma unroll
                 for (int i = 0; i < P; ++i) tr.p[i] = spx_uniform_d(cur.p[i] + delta[i]);
+                bool inside = fabs(tr.p[0]) <= lim && fabs(tr.p[1]) <= lim && tr.p[2] > 0.25 &&
+                              tr.p[2] < 2.0 * lim && tr.p[3] > 0.05 && tr.p[3] < lim;
+                if constexpr (P == 7) inside = inside && tr.p[5] > 0.25 && tr.p[5] < 4.0;
+                if (judging) {
+                    // residual judge: the Gauss-Newton step predicts the cost decrease
+                    // delta . J^T r; at most kJudgeRtol of the cost -> apply it and stop
+                    double gain = 0.0;
+#pragma unroll
+                    for (int i = 0; i < P; ++i) gain = fma(delta[i], (double)S[L::kRhs + i], gain);
+                    gain = spx_uniform_d(gain);
+                    if (inside && gain >= 0.0 && gain <= kJudgeRtol * (double)cost) {
+                        cur = tr;
+                        status = SPX_FIT_CONVERGED;
+                        break;
+                    }
+                    continue;
+                }
+                any_solve = true;
                 double step = fmax(fmax(fabs(delta[0]), fabs(delta[1])),
                                    fmax(fabs(delta[2]), fabs(delta[3])));
                 if constexpr (P == 7) step = fmax(step, fabs(delta[5]) * fabs(cur.p[2]));
                 step = spx_uniform_d(step);
-                bool inside = fabs(tr.p[0]) <= lim && fabs(tr.p[1]) <= lim && tr.p[2] > 0.25 &&
-                              tr.p[2] < 2.0 * lim && tr.p[3] > 0.05 && tr.p[3] < lim;
-                if constexpr (P == 7) inside = inside && tr.p[5] > 0.25 && tr.p[5] < 4.0;
                 if (!inside) {
                     lambda *= (double)a.opt.lambda_up;
                     continue;
@@ -654,6 +682,7 @@ SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>
                     S[L::kCost] = cost_t;
                     cost = cost_t;
                     lambda = fmax(lambda * (double)a.opt.lambda_down, 1e-12);
+                    fresh = budget;
                 } else {
                     lambda *= (double)a.opt.lambda_up;
                 }
@@ -669,7 +698,7 @@ SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>
     ws.evals += (unsigned long long)passes * (unsigned long long)npix;
     ws.passes += (unsigned long long)passes;
     ws.steps += (unsigned long long)it;
-    if (from_init && learned && status != SPX_FIT_CONVERGED && status != SPX_FIT_TOO_FEW) {
+    if (budget && status != SPX_FIT_CONVERGED && status != SPX_FIT_TOO_FEW) {
         ++ws.fallbacks;
         return it;
     }
@@ -718,12 +747,13 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
     WaveStats ws{0ull, 0ull, 0ull, 0ull};
     // Features are claimed kGrab at a time: one queue atomic per kGrab features.
     int base = 0, left = 0;
-    // A feature whose learned start needs the fallback is gathered again and fitted from the
+    // A feature whose budgeted fit needs the fallback is gathered again and fitted from the
     // moment start in the next turn of this loop (from the L2 lines it has just read).
     int redo = -1, redo_iters = 0;
     for (;;) {
         int f;
         const bool from_init = a.init != nullptr && redo < 0;
+        const bool budget = a.opt.learned_iters > 0 && redo < 0;
         const int it_before = redo < 0 ? 0 : redo_iters;
         if (redo >= 0) {
             f = redo;
@@ -804,7 +834,7 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
             }
         }
         pt.npix = spx_wave_sum(cnt);
-        const int again = fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked, from_init, it_before, ws);
+        const int again = fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked, from_init, budget, it_before, ws);
         if (again >= 0) {
             redo = f;
             redo_iters = again;

@@ -165,7 +165,7 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
 def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: int = 30,
                   step_tol: float = 1e-6, lambda0: float = 1e-3, lambda_up: float = 10.0,
                   lambda_down: float = 0.1, init_sigma: float = 1.0, cells: bool = False,
-                  backend: str = "gpu", device: int = 0) -> ErfFitResult:
+                  learned_iters: int = 0, backend: str = "gpu", device: int = 0) -> ErfFitResult:
     """Fit an erf-edged disk (6 parameters) or ellipse (7) to a patch around each seed.
 
     ``seeds`` (K, 2) [x, y]; the patch is (2 half + 1)^2 pixels centred on rint(seed)
@@ -174,6 +174,9 @@ def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: i
     Fits stop after ``max_iter`` step attempts or when a step below ``step_tol`` px is
     taken. ``cells=True`` restricts every patch to the feature's Voronoi cell among the
     seeds (the neighbour policy above).
+    ``learned_iters`` > 0 makes ``init`` a learned proposal (SpxFitOptions.learned_iters): at
+    most that many attempts from it, the residual judge, and the plain fit from the moment
+    start for the features it does not accept.
     """
     resolve_backend(backend)
     seeds_np = np.asarray(seeds, dtype=np.float32).reshape(-1, 2)
@@ -189,7 +192,7 @@ def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: i
         model_np = np.broadcast_to(np.asarray(model, dtype=np.uint8), (K,)).copy()
     opts = _lib.fit_options(max_iter=max_iter, step_tol=step_tol, lambda0=lambda0,
                             lambda_up=lambda_up, lambda_down=lambda_down, init_sigma=init_sigma,
-                            bucket_mask=_lib.bucket_mask_of(half_np, model_np))
+                            bucket_mask=_lib.bucket_mask_of(half_np, model_np), learned_iters=learned_iters)
     with gpu_device(device) as dev:
         img_t = to_device_u16(image, dev)
         if K == 0:

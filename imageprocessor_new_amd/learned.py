@@ -149,11 +149,11 @@ class FitTiming:
 
 
 def fit_learned(net: LearnedStart, img, seeds, half, model, *, lm_iters: int = 4, step_tol: float = 1e-6,
-                neighbours=None, tile_waves: int = 0):
+                neighbours=None, tile_waves: int = 0, bucket_mask: int = 0):
     """Learned start + <= lm_iters LM steps, plain-LM fallback for the rest: the CNN proposal
     (spx_cnn_propose), then one fit launch (spx_fit_batch_cells with learned_iters) in which a
-    feature not converged after lm_iters attempts from the proposal is fitted again from the
-    moment start by the same wave.
+    feature that the residual judge does not accept after lm_iters attempts from the proposal is
+    fitted again from the moment start by the same wave.
 
     Returns (result dict like fit_erf_disks_device plus ``fallback`` (K,) bool = iters >
     lm_iters, FitTiming). Synchronises the device. ``iters`` of a fallback feature counts the
@@ -165,7 +165,8 @@ def fit_learned(net: LearnedStart, img, seeds, half, model, *, lm_iters: int = 4
     init = net.propose(img, seeds, half)
     sync()
     t1 = time.perf_counter()
-    opts = _lib.fit_options(max_iter=30, step_tol=step_tol, learned_iters=int(lm_iters), tile_waves=tile_waves)
+    opts = _lib.fit_options(max_iter=30, step_tol=step_tol, learned_iters=int(lm_iters), tile_waves=tile_waves,
+                            bucket_mask=bucket_mask)
     res = fit_erf_disks_device(img, seeds, half, model, init, options=opts, neighbours=neighbours)
     sync()
     t2 = time.perf_counter()

@@ -63,10 +63,15 @@ typedef struct SpxFitOptions {
                              / 64 rounded up), m = 1 for ellipses. Features of buckets not
                              launched are not fitted (NaN, status too few). 0 = all ten,
                              so callers that know their patch sizes skip empty launches  */
-    int32_t learned_iters; /* with init: > 0 = the init is a learned proposal; a feature whose fit
-                             from it has not converged within this many step attempts is fitted
-                             again from the moment start (max_iter attempts) in the same launch,
-                             and iters_out counts the attempts of both. 0 = init is a plain start */
+    int32_t learned_iters; /* > 0: budgeted fit of a learned start (init; without init the moment
+                             start, as a control): at most this many step attempts, and a
+                             residual judge at the start and after every accepted step accepts
+                             the fit when the Gauss-Newton model there predicts the minimum's
+                             cost within 1e-6 of the current cost (that step is then applied);
+                             a feature not accepted within the budget is fitted again from the
+                             moment start (max_iter attempts) by the same wave in the same
+                             launch, and iters_out counts the attempts of both.
+                             0: plain fit (init, if given, is just the start) */
 } SpxFitOptions;
 
 /* Options of the edge-erf refinement (defaults of the reference's

@@ -196,8 +196,13 @@ def ldl_solve(M, g):
 
 def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, cell_edge=None, max_iter=30, step_tol=1e-6,
                  lambda0=1e-3, lambda_up=10.0, lambda_down=0.1, init_sigma=1.0,
-                 accept_rtol=1e-6):
-    """Fit one patch. Returns (internal params or NaN, cost, iterations, status, npix)."""
+                 accept_rtol=1e-6, budget=False, judge_rtol=1e-6):
+    """Fit one patch. Returns (internal params or NaN, cost, iterations, status, npix).
+
+    ``budget``: a budgeted (learned-start) pass of at most max_iter attempts with the residual
+    judge at the start point and after every accepted step: there the undamped Gauss-Newton
+    step d (J^T J d = J^T r) predicts the cost decrease d . J^T r; at most judge_rtol of the
+    cost (and d keeps the point in bounds) -> d is applied and the fit is converged."""
     P = 7 if ellipse else 6
     npix = int(valid.sum())
     side = 2 * half + 1
@@ -221,7 +226,27 @@ def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, cell_edge=None, ma
     status = MAX_ITER if np.isfinite(cost) else DIVERGED
     it = 0
     any_solve = False
+
+    def judge(p, JTJ, JTr, cost):
+        d = ldl_solve(JTJ.copy(), JTr)
+        if d is None:
+            return None
+        tr = p + d
+        inside = (abs(tr[0]) <= lim and abs(tr[1]) <= lim and 0.25 < tr[2] < 2.0 * lim and 0.05 < tr[3] < lim)
+        if ellipse:
+            inside = inside and 0.25 < tr[5] < 4.0
+        gain = float(d @ JTr)
+        return tr if inside and 0.0 <= gain <= judge_rtol * cost else None
+
+    fresh = budget
     while status == MAX_ITER and it < max_iter:
+        if fresh:
+            fresh = False
+            tj = judge(p, JTJ, JTr, cost)
+            if tj is not None:
+                p = tj
+                status = CONVERGED
+                break
         it += 1
         M = JTJ.copy()
         dg = np.diag(M).copy()
@@ -253,8 +278,14 @@ def lm_fit_patch(v, valid, half, ellipse=False, init=None, *, cell_edge=None, ma
         if cost_t <= cost * (1.0 + accept_rtol):
             p, JTJ, JTr, cost = tr, JTJ_t, JTr_t, cost_t
             lam = max(lam * lambda_down, 1e-12)
+            fresh = budget
         else:
             lam *= lambda_up
+    if fresh and status == MAX_ITER and np.isfinite(cost):
+        tj = judge(p, JTJ, JTr, cost)
+        if tj is not None:
+            p = tj
+            status = CONVERGED
     if not any_solve and status == MAX_ITER:
         status = SINGULAR
     if not np.isfinite(cost):
@@ -271,10 +302,13 @@ def to_output(p_internal, ellipse, cx, cy):
     return np.array([p[0] + cx, p[1] + cy, p[2], p[3], p[4], p[-1], q], dtype=np.float64)
 
 
-def fit_batch(img, seeds, half, model, init=None, neighbours=None, only=None, **opts):
+def fit_batch(img, seeds, half, model, init=None, neighbours=None, only=None, learned_iters=0, **opts):
     """Oracle counterpart of spx_fit_batch (spx_fit_batch_cells with ``neighbours`` (K, nb)):
     (params (K,7), rms, iterations, status). ``only``: indices of the features to fit (the
-    others stay NaN with status TOO_FEW); the cell masks still see every seed."""
+    others stay NaN with status TOO_FEW); the cell masks still see every seed.
+    ``learned_iters`` > 0: the budgeted fit of SpxFitOptions.learned_iters (from ``init``, or
+    from the moment start without it) and, when it is not converged, the plain fit from the
+    moment start; iterations count both."""
     img = np.asarray(img)
     seeds = np.asarray(seeds, dtype=np.float64).reshape(-1, 2)
     K = seeds.shape[0]
@@ -310,7 +344,14 @@ def fit_batch(img, seeds, half, model, init=None, neighbours=None, only=None, **
             if ell:
                 p0.append(row[6])
             p0.append(row[5])
-        p, cost, it, st, npix = lm_fit_patch(v, valid, h, ell, p0, cell_edge=edge_px, **opts)
+        if learned_iters > 0:
+            kw = dict(opts, max_iter=int(learned_iters))
+            p, cost, it, st, npix = lm_fit_patch(v, valid, h, ell, p0, cell_edge=edge_px, budget=True, **kw)
+            if st not in (CONVERGED, TOO_FEW):
+                p, cost, it2, st, npix = lm_fit_patch(v, valid, h, ell, None, cell_edge=edge_px, **opts)
+                it += it2
+        else:
+            p, cost, it, st, npix = lm_fit_patch(v, valid, h, ell, p0, cell_edge=edge_px, **opts)
         params[k] = to_output(p, ell, cx, cy)
         P = 7 if ell else 6
         rms[k] = np.sqrt(cost / max(npix - P, 1)) if st not in (TOO_FEW, BAD_START) else np.nan

@@ -61,3 +61,39 @@ def test_training_gains_over_moment_start():
     # a fallback feature equals the plain fit from the moment start bit for bit
     fbb = fb & (s0 == 0)
     assert np.array_equal(p1[fbb], p0[fbb])
+
+
+def test_learned_start_on_the_staggered_page_matches_the_fp64_oracle():
+    # the learned path of the uneven-batch workload (pages of the 16384^2 staggered generator:
+    # r U[4, 20], 10 % ellipses, 15..63 px patches, Voronoi cells): CNN proposal, <= 4 LM attempts,
+    # residual judge, in-launch fallback; a seeded sample of holes of a held-out page against the
+    # FP64 oracle's plain fit (centres, r, sigma to 1e-3 px; A, B to 1e-3 relative)
+    from imageprocessor_new_amd import learned, synthetic
+    from oracle import erfdisk
+
+    with _dev() as dev:
+        net, info = learned.train_on_pages(n_patches=100_000, epochs=3, batch=4096, lr=1e-3, seed=10_000,
+                                           device=dev)
+        assert info["losses"][-1] < info["losses"][0]
+        page = synthetic.render_workload_device("hex16384", seed=7, H=4096, W=4096)
+        f = learned.page_features(page, dev)
+        res, timing = learned.fit_learned(net, page.image, f.seeds, f.half, f.model, lm_iters=4, neighbours=f.nbr)
+        params, centers = res["params"].cpu().numpy(), res["centers"].cpu().numpy()
+        status, fb = res["status"].cpu().numpy(), res["fallback"].cpu().numpy()
+        img = page.image.cpu().numpy()
+        seeds, half, model, nbr = (t.cpu().numpy() for t in (f.seeds, f.half, f.model, f.nbr))
+    K = seeds.shape[0]
+    print(f"{K} holes, {timing.n_fallback} fallbacks, statuses {np.bincount(status)}")
+    assert (status == 0).mean() > 0.98
+    idx = np.sort(np.random.default_rng(0).choice(K, 300, replace=False))
+    ref, _, _, ref_st = erfdisk.fit_batch(img, seeds, half, model, neighbours=nbr, only=idx)
+    both = idx[(ref_st[idx] == 0) & (status[idx] == 0)]
+    assert both.size > 0.97 * idx.size
+    err = np.abs(params[both].astype(np.float64) - ref[both])
+    err[:, :2] = np.abs(centers[both] - ref[both, :2])
+    print("max |learned - fp64 oracle| x0 y0 r s A B q:", err.max(axis=0))
+    assert err[:, :4].max() <= 1e-3
+    assert (err[:, 4:6] / np.abs(ref[both, 4:6])).max() <= 1e-3
+    assert err[:, 6].max() <= 1e-3
+    # the sample holds accepted learned fits as well as fallbacks
+    assert (~fb[both]).sum() > 0.3 * both.size

@@ -155,3 +155,59 @@ def test_cell_radius_lists_every_seed_that_can_cut_the_patch():
         seeds2 = np.array([s0, far], dtype=np.float32)
         gy, gx = np.mgrid[100 - h - 1:100 + h + 2, 100 - h - 1:100 + h + 2]
         assert erfdisk.cell_mask(gx.astype(float), gy.astype(float), seeds2, 0, np.array([1])).all()
+
+
+def _noisy_patches(seed=0):
+    c = np.array([[40.3, 41.7], [100.45, 37.2], [70.1, 90.9], [120.6, 110.2]])
+    arr = synthetic.render_hole_array((150, 150), c, radius=[8.0, 7.3, 9.1, 6.5], sigma=[1.2, 0.9, 1.5, 1.0],
+                                      q=[1.0, 0.85, 1.0, 1.0], seed=seed)
+    truth = np.c_[c, arr.radius, arr.sigma, np.full(4, 3000.0), np.full(4, 500.0), arr.q]
+    return arr.image, np.rint(c), np.array([0, 1, 0, 0]), truth
+
+
+def test_budget_judge_accepts_a_start_near_the_minimum():
+    # a start 2e-3 px off the converged plain fit: the one allowed attempt steps (its step is
+    # above the 1e-6 px tolerance), then the judge predicts a further decrease far below 1e-6 of
+    # the cost, applies the Gauss-Newton step and accepts: the result lands on the minimum
+    img, seeds, model, _ = _noisy_patches(1)
+    ref, _, _, st0 = erfdisk.fit_batch(img, seeds, 15, model)
+    assert (st0 == 0).all()
+    start = ref.copy()
+    start[:, 0] += 2e-3
+    start[:, 1] -= 2e-3
+    p, _, it, st = erfdisk.fit_batch(img, seeds, 15, model, init=start, learned_iters=1)
+    assert (st == 0).all() and (it == 1).all()
+    assert np.abs(p[:, :2] - ref[:, :2]).max() < 1e-6
+    # without the judge the same budget leaves these fits unconverged
+    p2, _, _, st2 = erfdisk.fit_batch(img, seeds, 15, model, init=start, max_iter=1)
+    assert (st2 == erfdisk.MAX_ITER).all()
+
+
+def test_budget_judge_refuses_a_far_start_and_falls_back_to_the_plain_fit():
+    # a start 2 px off with a one-attempt budget cannot meet the judge: the fit is redone from
+    # the moment start and equals the plain fit exactly, with one more attempt counted
+    img, seeds, model, truth = _noisy_patches(2)
+    ref, rref, itref, st0 = erfdisk.fit_batch(img, seeds, 15, model)
+    far = truth.copy()
+    far[:, 0] += 2.0
+    p, r, it, st = erfdisk.fit_batch(img, seeds, 15, model, init=far, learned_iters=1)
+    assert np.array_equal(p, ref) and np.array_equal(st, st0)
+    assert np.array_equal(it, itref + 1)
+
+
+def test_budget_judge_predicts_the_cost_decrease():
+    # near the minimum the Gauss-Newton prediction d . J^T r matches the true decrease of the
+    # cost when d is applied (to the model's curvature error), so a sign or scale slip in the
+    # judge shows here
+    img, seeds, model, _ = _noisy_patches(3)
+    ref, _, _, _ = erfdisk.fit_batch(img, seeds, 15, model)
+    v, valid, _, _, cx, cy = erfdisk.extract_patch(img, seeds[0], 15)
+    fy, fx = np.meshgrid(np.arange(31) - 15, np.arange(31) - 15, indexing="ij")
+    p = np.array([ref[0, 0] - cx + 0.05, ref[0, 1] - cy - 0.03, ref[0, 2] + 0.02, ref[0, 3], ref[0, 4], ref[0, 5]])
+    cost = lambda q: float(((v[valid] - erfdisk.model_jac(q, fx[valid], fy[valid], False)[0]) ** 2).sum())
+    mod, J = erfdisk.model_jac(p, fx[valid], fy[valid], False)
+    g = J.T @ (v[valid] - mod)
+    d = erfdisk.ldl_solve(J.T @ J, g)
+    pred = float(d @ g)
+    true = cost(p) - cost(p + d)
+    assert pred > 0 and abs(pred - true) < 0.02 * true
