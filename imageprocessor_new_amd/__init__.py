@@ -36,8 +36,7 @@ def __getattr__(name):
         from . import edge_erf
 
         return getattr(edge_erf, name)
-    if name in ("fit_outline", "ellipse_widths", "outline_rays", "blur_second", "fit_summary", "fit_line",
-                "fit_outline_fast"):
+    if name in ("fit_outline", "ellipse_widths", "outline_rays", "fit_outline_fast"):
         from . import outline
 
         return getattr(outline, name)

@@ -22,12 +22,13 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+// FP32 -> BF16, round to nearest even: gfx950's v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+// Two FP32 values -> two BF16 in one 32-bit word (lo = a), one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2v{a, b}), bf16x2v));
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -229,12 +230,19 @@ __global__ void to_bf16_kernel(const float* x, int64_t n, uint16_t* y) {
 constexpr int kPrepWaves = 4;
 constexpr int kMaxSide = 63;
 
+// Position of input pixel (i, j) of patch slot `slot` in the fused kernels' LDS input: the
+// four stride-2 phases of the 32 x 32 plane apart (see the fused inference below).
+__device__ __forceinline__ int input_phase_idx(int slot, int i, int j) {
+    return (slot * 4 + (((i & 1) << 1) | (j & 1))) * 256 + (i >> 1) * 16 + (j >> 1);
+}
+
 // One wave prepares one patch: gather into the u16 staging area P (side^2 values), ring
-// mean and maximum, bilinear resampling to 32 x 32 written as BF16 to xdst (global memory or
+// mean and maximum, bilinear resampling to 32 x 32 written as BF16 to xdst: row-major
+// (slot < 0; global memory) or at input_phase_idx(slot, ., .) (slot >= 0; the fused kernels'
 // LDS). Returns {B0, A0} through b0 / a0 (every lane).
 __device__ __forceinline__ void prep_patch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, float cx,
-                                           float cy, int hs, int lane, uint16_t* P, uint16_t* xdst, float& b0,
-                                           float& a0) {
+                                           float cy, int hs, int lane, uint16_t* P, uint16_t* xdst, int slot,
+                                           float& b0, float& a0) {
     const int side = 2 * hs + 1;
     const int64_t ox = (int64_t)cx - hs, oy = (int64_t)cy - hs;
     float rsum = 0.0f, rcnt = 0.0f, vmax = -3.0e38f;
@@ -271,7 +279,7 @@ __device__ __forceinline__ void prep_patch(const uint16_t* img, int64_t H, int64
         const int x1 = min(x0 + 1, side - 1), y1 = min(y0 + 1, side - 1);
         const float v = (1.0f - fy) * ((1.0f - fx) * (float)P[y0 * side + x0] + fx * (float)P[y0 * side + x1]) +
                         fy * ((1.0f - fx) * (float)P[y1 * side + x0] + fx * (float)P[y1 * side + x1]);
-        xdst[o] = f2bf((v - B0) / A0);
+        xdst[slot < 0 ? o : input_phase_idx(slot, i, j)] = f2bf((v - B0) / A0);
     }
     b0 = B0;
     a0 = A0;
@@ -287,7 +295,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(const uint16_t* img, int64
     const int hs = __builtin_amdgcn_readfirstlane(max(1, min((int)half[f], (kMaxSide - 1) / 2)));
     const float cx = rintf(seeds[2 * (int64_t)f]), cy = rintf(seeds[2 * (int64_t)f + 1]);
     float B0, A0;
-    prep_patch(img, H, W, pitch, cx, cy, hs, lane, patch[wave], x_out + (int64_t)f * 1024, B0, A0);
+    prep_patch(img, H, W, pitch, cx, cy, hs, lane, patch[wave], x_out + (int64_t)f * 1024, -1, B0, A0);
     if (lane == 0) {
         norm[5 * (int64_t)f + 0] = cx;
         norm[5 * (int64_t)f + 1] = cy;
@@ -501,20 +509,34 @@ int forward(const uint16_t* x, int Bn, const uint16_t* wbf, const float* theta, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Fused inference: one workgroup (4 waves) takes kIP patches through all six layers, the
-// activations staying in LDS (NHWC, BF16), the weights read as MFMA B fragments from a
-// transposed, K-padded BF16 copy (Wt[n][k], k padded to a multiple of 16, zeros beyond K).
-// Each layer is an implicit GEMM on v_mfma_f32_16x16x16bf16_1k tiles: rows = (patch, output
-// pixel), columns = output channels, k = (ky, kx, c). A wave owns one column block and a group
-// of up to kGM row blocks, so one B fragment feeds kGM MFMAs. Bias + ReLU + BF16 rounding in
-// the epilogue, as the GEMM path. The 128 -> 7 head runs on the VALU in FP32.
-typedef short bf16x4s __attribute__((ext_vector_type(4)));
+// Fused inference: one workgroup of 4 waves takes kNP = 4 patches through all six layers,
+// activations staying in LDS (48 KB, three workgroups per CU). conv1 (1 -> 16, 74 kflop per
+// patch) runs on the VALU, one wave per patch, its 160 parameters wave-uniform. The other conv
+// layers and fc1 are implicit GEMMs on gfx950's
+// v_mfma_f32_16x16x32_bf16 (rows = (patch, output pixel), columns = output channels,
+// k = (ky, kx, c)); a wave owns one 16-column block and G row blocks, so each B fragment
+// (8 BF16 weights per lane, one 16-byte load from a transposed, K-padded copy W^T[n][k]) feeds
+// G MFMAs. Bias + ReLU + BF16 rounding in the epilogue, as the GEMM path of training.
+//
+// LDS layout of a layer input of side S and C channels (C >= 8): the four stride-2 phases of
+// the plane apart (phase (y & 1, x & 1), sub-pixel (y >> 1, x >> 1)), so the 16 rows of an A
+// fragment (16 consecutive output pixels) read consecutive sub-pixels for every tap, and the
+// 8-channel chunks of a sub-pixel sp XOR-permuted by sp, so those 16 reads of 16 bytes do not
+// pile onto the same banks:
+//   idx(pt, y, x, c) = ((pt * 4 + ph) * (S / 2)^2 + sp) * C + (((c >> 3) ^ (sp & (C / 8 - 1))) << 3 | (c & 7)).
+// The 32 x 32 input (C = 1) uses the phase split alone; the fc1 input (the 2 x 2 x 64 NHWC
+// flatten, 256 values per patch) permutes its 8-value chunks by the patch index.
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int kIP = 4;   // patches per workgroup
-constexpr int kGM = 8;   // row blocks per wave item (B fragment reuse)
+constexpr int kNP = 4;          // patches per workgroup
+constexpr int kIW = 4;          // waves per workgroup (one per patch in the preparation and conv1)
+constexpr int kBufX = kNP * 2048;  // BF16 values: input, L2 out, L4 out
+constexpr int kBufY = kNP * 4096;  // BF16 values: raw patch staging, L1 out, L3 out, fc1 out
+static_assert(kNP * 1024 <= kBufX && kNP * 8 * 8 * 32 <= kBufX, "bufX");
+static_assert(kNP * 16 * 16 * 16 <= kBufY, "bufY");
 
 struct InferWeights {
-    const uint16_t* wt;      // transposed, K-padded BF16 weights, layers 0..4
+    const uint16_t* wt;      // W^T per layer 0..4: [n][kp], kp = K padded to a multiple of 32, zeros beyond K
     int64_t off[5];          // element offset of layer l in wt
     int kp[5];               // padded K of layer l
     const float* theta;      // FP32 parameters (biases; K x N layout)
@@ -537,141 +559,203 @@ __global__ void transpose_weights_kernel(const uint16_t* wbf, InferWeights iw, u
     }
 }
 
-// One conv layer (3x3, stride 2, pad 1) or the first fc layer (as a 1x1 "conv" on the NHWC
-// flatten: Hout = 1, Cin = 256) from LDS `in` to LDS `out`.
-template <int CIN, int COUT, int HIN, bool FC>
-__device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
-                                            const uint16_t* __restrict__ wt, int kp, const float* __restrict__ bias,
-                                            int wave, int lane) {
-    constexpr int HOUT = FC ? 1 : HIN / 2;
-    constexpr int PIX = HOUT * HOUT;
-    constexpr int M = kIP * PIX;
-    constexpr int MB = (M + 15) / 16;
-    constexpr int NB = COUT / 16;
-    constexpr int MG = (MB + kGM - 1) / kGM;
-    constexpr int KIN = FC ? CIN : 9 * CIN;
-    const int kc_n = kp / 16;
-    const int li = lane & 15, lq = lane >> 4;
-    for (int item = wave; item < NB * MG; item += 4) {
-        const int nb = item % NB, mg = item / NB;
-        const int mb0 = mg * kGM;
-        const int nmb = MB - mb0 < kGM ? MB - mb0 : kGM;
-        f32x4 acc[kGM];
+// Index of element (pt, y, x, c) of a phase-split layer input of side S, C channels.
+template <int S, int C>
+__device__ __forceinline__ int act_idx(int pt, int y, int x, int c) {
+    constexpr int HS = S / 2;
+    const int sp = (y >> 1) * HS + (x >> 1);
+    const int ph = ((y & 1) << 1) | (x & 1);
+    if constexpr (C == 1) {
+        static_assert(S == 32 || C != 1, "the C = 1 layout is the 32 x 32 input's");
+        return (pt * 4 + ph) * HS * HS + sp;  // = input_phase_idx(pt, y, x)
+    } else {
+        constexpr int CM = C / 8 - 1;
+        return ((pt * 4 + ph) * HS * HS + sp) * C + ((((c >> 3) ^ (sp & CM)) << 3) | (c & 7));
+    }
+}
+// Index of value k of patch pt in the fc1 input (2 x 2 x 64 NHWC flatten).
+__device__ __forceinline__ int fc_idx(int pt, int k) { return pt * 256 + ((((k >> 3) ^ (pt & 31)) << 3) | (k & 7)); }
+
+__device__ __forceinline__ bf16x8v load8(const uint16_t* p) { return *reinterpret_cast<const bf16x8v*>(p); }
+
+// One 3x3 stride-2 pad-1 conv layer of the kNP patches, `in` (side HIN, CIN channels) to `out`
+// (the next layer's input layout; NEXT_FC: the fc1 input). Wave w takes column block
+// w % NB and G consecutive row blocks. The k loop is unrolled, so a chunk's tap, phase and
+// sub-pixel shift are constants (for CIN >= 32): tap (ky, kx) reads phase ((ky + 1) & 1,
+// (kx + 1) & 1) at the output pixel's sub-pixel moved up / left by one for ky = 0 / kx = 0,
+// outside the plane only there and only for the first output row / column.
+template <int CIN, int COUT, int HIN, int G, bool NEXT_FC>
+__device__ __forceinline__ void conv_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                           const uint16_t* __restrict__ wt, const float* __restrict__ bias, int wave,
+                                           int lane) {
+    constexpr int HOUT = HIN / 2, PIX = HOUT * HOUT, M = kNP * PIX, MB = M / 16, NB = COUT / 16;
+    constexpr int KIN = 9 * CIN, KC = (KIN + 31) / 32, KP = KC * 32;
+    constexpr int HS = HIN / 2, PLANE = HS * HS * CIN, CM = CIN / 8 - 1;
+    static_assert(MB * NB == kIW * G, "every wave takes G row blocks of one column block");
+    static_assert(CIN % 8 == 0 && HS == HOUT, "8-channel chunks; the output pixel indexes the sub-plane");
+    const int li = lane & 15, g4 = lane >> 4;
+    const int nb = wave % NB, rb0 = (wave / NB) * G;
+    const uint16_t* wcol = wt + (int64_t)(nb * 16 + li) * KP + 8 * g4;
+    int base[G], sp0[G];
+    bool top[G], left[G];
 #pragma unroll
-        for (int g = 0; g < kGM; ++g) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        const uint16_t* wcol = wt + (int64_t)(nb * 16 + li) * kp;
-        if constexpr (!FC && CIN >= 16) {
-            // tap-major: the row geometry once per item, the tap offset and bounds per tap
-            constexpr int CPT = CIN / 16;  // 16-wide k chunks per tap
-            int base[kGM], y0[kGM], x0[kGM];
+    for (int g = 0; g < G; ++g) {
+        const int m = (rb0 + g) * 16 + li;
+        const int pt = m / PIX, pix = m - (m / PIX) * PIX;
+        base[g] = pt * 4 * PLANE;
+        sp0[g] = pix;  // oy * HS + ox
+        top[g] = pix < HOUT;
+        left[g] = (pix % HOUT) == 0;
+    }
+    f32x4 acc[G];
 #pragma unroll
-            for (int g = 0; g < kGM; ++g) {
-                const int row = (mb0 + g) * 16 + li;
-                const int pt = row / PIX, pix = row - pt * PIX;
-                const int oy = pix / HOUT, ox = pix - (pix / HOUT) * HOUT;
-                const bool live = g < nmb && row < M;
-                base[g] = pt * HIN * HIN * CIN;
-                y0[g] = live ? 2 * oy - 1 : -1000;  // an absent row fails every bounds test
-                x0[g] = 2 * ox - 1;
-            }
+    for (int g = 0; g < G; ++g) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int tap = 0; tap < 9; ++tap) {
-                const int ky = tap / 3, kx = tap % 3;
+    for (int kc = 0; kc < KC; ++kc) {
+        const bf16x8v b = load8(wcol + kc * 32);
+        const int k0 = kc * 32 + 8 * g4;
+        const int tap = k0 / CIN, chunk = (k0 % CIN) >> 3;
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+        const int off = ((((ky + 1) & 1) << 1) | ((kx + 1) & 1)) * PLANE;
+        const int dsp = (ky == 0 ? -HS : 0) + (kx == 0 ? -1 : 0);
+        const bool live = k0 < KIN;
 #pragma unroll
-                for (int cc = 0; cc < CPT; ++cc) {
-                    const int c0 = cc * 16 + 4 * lq;
-                    const bf16x4s b = *reinterpret_cast<const bf16x4s*>(wcol + (tap * CPT + cc) * 16 + 4 * lq);
-#pragma unroll
-                    for (int g = 0; g < kGM; ++g) {
-                        if (g >= nmb) break;
-                        const int iy = y0[g] + ky, ix = x0[g] + kx;
-                        bf16x4s a = bf16x4s{0, 0, 0, 0};
-                        if ((unsigned)iy < (unsigned)HIN && (unsigned)ix < (unsigned)HIN)
-                            a = *reinterpret_cast<const bf16x4s*>(in + base[g] + (iy * HIN + ix) * CIN + c0);
-                        acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[g], 0, 0, 0);
-                    }
-                }
-            }
-        } else
-        for (int kc = 0; kc < kc_n; ++kc) {
-            const int k0 = kc * 16 + 4 * lq;
-            const bf16x4s b = *reinterpret_cast<const bf16x4s*>(wcol + k0);
-#pragma unroll
-            for (int g = 0; g < kGM; ++g) {
-                if (g >= nmb) break;
-                const int row = (mb0 + g) * 16 + li;
-                bf16x4s a = bf16x4s{0, 0, 0, 0};
-                if (row < M) {
-                    const int pt = row / PIX, pix = row - pt * PIX;
-                    if constexpr (FC) {
-                        a = *reinterpret_cast<const bf16x4s*>(in + pt * CIN + k0);
-                    } else if constexpr (CIN >= 4) {
-                        const int oy = pix / HOUT, ox = pix - oy * HOUT;
-                        const int tap = k0 / CIN, c0 = k0 - tap * CIN;
-                        const int iy = 2 * oy + tap / 3 - 1, ix = 2 * ox + tap % 3 - 1;
-                        if (k0 < KIN && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN)
-                            a = *reinterpret_cast<const bf16x4s*>(in + ((pt * HIN + iy) * HIN + ix) * CIN + c0);
-                    } else {
-                        const int oy = pix / HOUT, ox = pix - oy * HOUT;
-#pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            const int k = k0 + t;  // CIN == 1: k = tap
-                            const int iy = 2 * oy + k / 3 - 1, ix = 2 * ox + k % 3 - 1;
-                            if (k < KIN && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN)
-                                a[t] = (short)in[(pt * HIN + iy) * HIN + ix];
-                        }
-                    }
-                }
-                acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[g], 0, 0, 0);
-            }
+        for (int g = 0; g < G; ++g) {
+            const int sp = sp0[g] + dsp;
+            bf16x8v a = bf16x8v{};
+            if (live && !(ky == 0 && top[g]) && !(kx == 0 && left[g]))
+                a = load8(in + base[g] + off + sp * CIN + ((chunk ^ (sp & CM)) << 3));
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[g], 0, 0, 0);
         }
-        // D(i, j): i = 4 (lane / 16) + r, j = lane % 16
-        const int n = nb * 16 + li;
-        const float bn = bias[n];
+    }
+    // D(i, j): i = 4 (lane / 16) + r, j = lane % 16
+    const int n = nb * 16 + li;
+    const float bn = bias[n];
 #pragma unroll
-        for (int g = 0; g < kGM; ++g) {
-            if (g >= nmb) break;
+    for (int g = 0; g < G; ++g) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = (mb0 + g) * 16 + 4 * lq + r;
-                if (row < M) out[row * COUT + n] = f2bf(fmaxf(acc[g][r] + bn, 0.0f));
-            }
+        for (int r = 0; r < 4; ++r) {
+            const int m = (rb0 + g) * 16 + 4 * g4 + r;
+            const int p = m / PIX, pix = m - (m / PIX) * PIX;
+            const int y = pix / HOUT, x = pix - (pix / HOUT) * HOUT;
+            const uint16_t v = f2bf(fmaxf(acc[g][r] + bn, 0.0f));
+            if constexpr (NEXT_FC) out[fc_idx(p, pix * COUT + n)] = v;
+            else out[act_idx<HOUT, COUT>(p, y, x, n)] = v;
         }
     }
 }
 
-// Layers 1-5 on the patches in bufB (NHWC BF16, 32 x 32 x 1 each), the head, then the
-// outputs of patches p0 .. p0 + kIP - 1 (< K): decoded with nrm[patch][5] when `decode`, raw
-// otherwise.
-__device__ __forceinline__ void infer_tail(uint16_t* bufA, uint16_t* bufB, float* head, const InferWeights& iw,
-                                           int K, int p0, bool decode, const float* nrm, float* out) {
+// conv1 (1 -> 16, 3x3 stride 2 pad 1, 32 x 32 -> 16 x 16) of patch `wave` on the VALU: lane l
+// takes output pixels l, l + 64, l + 128, l + 192, holds their 36 input taps and forms the 16
+// channels two at a time in FP32 (BF16 weights and inputs, so every product is exact), then
+// bias, ReLU and BF16 rounding. w1: the 9 x 16 weights as FP32 and the 16 biases, in LDS.
+__device__ __forceinline__ void conv1_valu(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                           const float* __restrict__ w1, int wave, int lane) {
+    float v[4][9];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int p = lane + 64 * j, oy = p >> 4, ox = p & 15;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int iy = 2 * oy + t / 3 - 1, ix = 2 * ox + t % 3 - 1;
+            v[j][t] = (iy >= 0 && ix >= 0) ? bf2f(in[act_idx<32, 1>(wave, iy, ix, 0)]) : 0.0f;
+        }
+    }
+    uint32_t packed[4][8];
+#pragma unroll 2
+    for (int c = 0; c < 16; c += 2) {
+        float wa[9], wb[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            wa[t] = w1[t * 16 + c];
+            wb[t] = w1[t * 16 + c + 1];
+        }
+        const float ba = w1[144 + c], bb = w1[144 + c + 1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                sa = fmaf(v[j][t], wa[t], sa);
+                sb = fmaf(v[j][t], wb[t], sb);
+            }
+            packed[j][c >> 1] = f2bf2(fmaxf(sa + ba, 0.0f), fmaxf(sb + bb, 0.0f));
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int p = lane + 64 * j, oy = p >> 4, ox = p & 15;
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            *reinterpret_cast<uint4*>(out + act_idx<16, 16>(wave, oy, ox, 8 * q)) =
+                make_uint4(packed[j][4 * q], packed[j][4 * q + 1], packed[j][4 * q + 2], packed[j][4 * q + 3]);
+    }
+}
+
+// fc1 (256 -> 128, ReLU) of the kNP patches: rows 0 .. kNP-1 of one 16-row block; the 8
+// column blocks go round the waves, the 8 k chunks alternating between two accumulators.
+__device__ __forceinline__ void fc1_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                          const uint16_t* __restrict__ wt, const float* __restrict__ bias, int wave,
+                                          int lane) {
+    const int li = lane & 15, g4 = lane >> 4;
+    for (int nb = wave; nb < 8; nb += kIW) {
+        const uint16_t* wcol = wt + (int64_t)(nb * 16 + li) * 256 + 8 * g4;
+        f32x4 acc0 = f32x4{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
+#pragma unroll
+        for (int kc = 0; kc < 8; kc += 2) {
+            const bf16x8v b0 = load8(wcol + kc * 32), b1 = load8(wcol + kc * 32 + 32);
+            bf16x8v a0 = bf16x8v{}, a1 = bf16x8v{};
+            if (li < kNP) {
+                a0 = load8(in + fc_idx(li, kc * 32 + 8 * g4));
+                a1 = load8(in + fc_idx(li, kc * 32 + 32 + 8 * g4));
+            }
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc1, 0, 0, 0);
+        }
+        const int n = nb * 16 + li;
+        const float bn = bias[n];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int p = 4 * g4 + r;
+            if (p < kNP) out[p * 128 + n] = f2bf(fmaxf(acc0[r] + acc1[r] + bn, 0.0f));
+        }
+    }
+}
+
+// conv1's BF16 weights (as FP32) and biases into LDS; read after the first barrier.
+__device__ __forceinline__ void stage_conv1(const InferWeights& iw, float* w1) {
+    const int tid = threadIdx.x;
+    if (tid < 144) w1[tid] = bf2f(iw.wbf[param_offset_dev(0) + tid]);
+    else if (tid < 160) w1[tid] = iw.theta[param_offset_dev(0) + 144 + (tid - 144)];
+}
+
+// Layers 1-6 on the kNP patches in bufX (input layout), then the outputs of patches p0 ..
+// p0 + kNP - 1 (< K): decoded with nrm[patch][5] when `decode`, raw otherwise.
+__device__ __forceinline__ void infer_net(uint16_t* bufX, uint16_t* bufY, float* head, const float* w1,
+                                          const InferWeights& iw, int K, int p0, bool decode, const float* nrm,
+                                          float* out) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const float* th = iw.theta;
-    infer_layer<1, 16, 32, false>(bufB, bufA, iw.wt + iw.off[0], iw.kp[0], th + param_offset_dev(0) + 9 * 16, wave,
-                                  lane);
+    conv1_valu(bufX, bufY, w1, wave, lane);
     __syncthreads();
-    infer_layer<16, 32, 16, false>(bufA, bufB, iw.wt + iw.off[1], iw.kp[1], th + param_offset_dev(1) + 144 * 32,
-                                   wave, lane);
+    conv_layer<16, 32, 16, 8, false>(bufY, bufX, iw.wt + iw.off[1], th + param_offset_dev(1) + 144 * 32, wave, lane);
     __syncthreads();
-    infer_layer<32, 64, 8, false>(bufB, bufA, iw.wt + iw.off[2], iw.kp[2], th + param_offset_dev(2) + 288 * 64,
-                                  wave, lane);
+    conv_layer<32, 64, 8, 4, false>(bufX, bufY, iw.wt + iw.off[2], th + param_offset_dev(2) + 288 * 64, wave, lane);
     __syncthreads();
-    infer_layer<64, 64, 4, false>(bufA, bufB, iw.wt + iw.off[3], iw.kp[3], th + param_offset_dev(3) + 576 * 64,
-                                  wave, lane);
+    conv_layer<64, 64, 4, 1, true>(bufY, bufX, iw.wt + iw.off[3], th + param_offset_dev(3) + 576 * 64, wave, lane);
     __syncthreads();
-    infer_layer<256, 128, 1, true>(bufB, bufA, iw.wt + iw.off[4], iw.kp[4], th + param_offset_dev(4) + 256 * 128,
-                                   wave, lane);
+    fc1_layer(bufX, bufY, iw.wt + iw.off[4], th + param_offset_dev(4) + 256 * 128, wave, lane);
     __syncthreads();
     // head 128 -> 7 (FP32 on the VALU): thread (patch, output)
-    if (tid < kIP * 7) {
+    if (tid < kNP * 7) {
         const int pt = tid / 7, j = tid - pt * 7;
         const int64_t wo = param_offset_dev(5);
         float acc = 0.0f;
-        for (int k = 0; k < 128; ++k) acc = fmaf(bf2f(bufA[pt * 128 + k]), bf2f(iw.wbf[wo + k * 7 + j]), acc);
+        for (int k = 0; k < 128; ++k) acc = fmaf(bf2f(bufY[pt * 128 + k]), bf2f(iw.wbf[wo + k * 7 + j]), acc);
         head[pt * 8 + j] = acc + th[wo + 128 * 7 + j];
     }
     __syncthreads();
-    if (tid < kIP && p0 + tid < K) {
+    if (tid < kNP && p0 + tid < K) {
         const int64_t k = p0 + tid;
         if (decode) {
             decode_one(head + tid * 8, nrm + 5 * tid, out + 7 * k);
@@ -682,45 +766,49 @@ __device__ __forceinline__ void infer_tail(uint16_t* bufA, uint16_t* bufB, float
     }
 }
 
-__global__ __launch_bounds__(256) void cnn_infer_kernel(const uint16_t* __restrict__ x, int K, InferWeights iw,
+__global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(3))) void cnn_infer_kernel(const uint16_t* __restrict__ x, int K, InferWeights iw,
                                                         const float* __restrict__ norm, float* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) uint16_t bufA[kIP * 16 * 16 * 16];  // L1 out, L3 out, fc1 out
-    __shared__ __attribute__((aligned(16))) uint16_t bufB[kIP * 8 * 8 * 32];    // input, L2 out, L4 out
-    __shared__ float head[kIP * 8];
+    __shared__ __attribute__((aligned(16))) uint16_t bufX[kBufX];
+    __shared__ __attribute__((aligned(16))) uint16_t bufY[kBufY];
+    __shared__ float head[kNP * 8];
+    __shared__ float w1[160];
     const int tid = threadIdx.x;
-    const int p0 = blockIdx.x * kIP;
-    // input patches (2 KB each) -> bufB, 16 bytes per thread per step; missing patches read 0
-    for (int q = tid; q < kIP * 128; q += 256) {
-        const int pt = q >> 7, e = (q & 127) * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (p0 + pt < K) v = *reinterpret_cast<const uint4*>(x + (int64_t)(p0 + pt) * 1024 + e);
-        *reinterpret_cast<uint4*>(bufB + pt * 1024 + e) = v;
+    const int p0 = blockIdx.x * kNP;
+    stage_conv1(iw, w1);
+    // input patches (row-major 32 x 32 BF16) into the phase-split layout; missing patches read 0
+    for (int q = tid; q < kNP * 1024; q += 64 * kIW) {
+        const int pt = q >> 10, e = q & 1023;
+        const uint16_t v = p0 + pt < K ? x[(int64_t)(p0 + pt) * 1024 + e] : (uint16_t)0;
+        bufX[act_idx<32, 1>(pt, e >> 5, e & 31, 0)] = v;
     }
     __syncthreads();
-    infer_tail(bufA, bufB, head, iw, K, p0, norm != nullptr, norm ? norm + 5 * (int64_t)p0 : nullptr, out);
+    infer_net(bufX, bufY, head, w1, iw, K, p0, norm != nullptr, norm ? norm + 5 * (int64_t)p0 : nullptr, out);
 }
 
-// Proposal in one launch: each wave prepares its patch straight into LDS (the u16 staging
-// area aliases bufA, free until the first layer writes it), then the network and the decode.
-static_assert(kIP * kMaxSide * kMaxSide <= kIP * 16 * 16 * 16, "patch staging must fit in bufA");
-static_assert(kIP == 256 / 64, "one wave per patch");
+// Proposal in one launch: wave w prepares patch p0 + w (raw pixels staged in bufY, free until
+// the first layer writes it), writing the resampled input straight into bufX's layout, then
+// the network and the decode.
+static_assert(kNP * kMaxSide * kMaxSide <= kBufY, "patch staging must fit in bufY");
+static_assert(kNP == kIW, "one wave per patch");
 
-__global__ __launch_bounds__(256) void cnn_propose_kernel(const uint16_t* __restrict__ img, int64_t H, int64_t W,
+__global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(3))) void cnn_propose_kernel(const uint16_t* __restrict__ img, int64_t H, int64_t W,
                                                           int64_t pitch, const float* __restrict__ seeds,
                                                           const int16_t* __restrict__ half, int K, InferWeights iw,
                                                           float* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) uint16_t bufA[kIP * 16 * 16 * 16];
-    __shared__ __attribute__((aligned(16))) uint16_t bufB[kIP * 8 * 8 * 32];
-    __shared__ float head[kIP * 8];
-    __shared__ float s_norm[kIP * 5];
+    __shared__ __attribute__((aligned(16))) uint16_t bufX[kBufX];
+    __shared__ __attribute__((aligned(16))) uint16_t bufY[kBufY];
+    __shared__ float head[kNP * 8];
+    __shared__ float s_norm[kNP * 5];
+    __shared__ float w1[160];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int p0 = blockIdx.x * kIP;
+    const int p0 = blockIdx.x * kNP;
+    stage_conv1(iw, w1);
     const int f = p0 + wave;
     if (f < K) {
         const int hs = __builtin_amdgcn_readfirstlane(max(1, min((int)half[f], (kMaxSide - 1) / 2)));
         const float cx = rintf(seeds[2 * (int64_t)f]), cy = rintf(seeds[2 * (int64_t)f + 1]);
         float B0, A0;
-        prep_patch(img, H, W, pitch, cx, cy, hs, lane, bufA + wave * kMaxSide * kMaxSide, bufB + wave * 1024, B0, A0);
+        prep_patch(img, H, W, pitch, cx, cy, hs, lane, bufY + wave * kMaxSide * kMaxSide, bufX, wave, B0, A0);
         if (lane == 0) {
             s_norm[5 * wave + 0] = cx;
             s_norm[5 * wave + 1] = cy;
@@ -729,10 +817,10 @@ __global__ __launch_bounds__(256) void cnn_propose_kernel(const uint16_t* __rest
             s_norm[5 * wave + 4] = A0;
         }
     } else {
-        for (int o = lane; o < 1024; o += 64) bufB[wave * 1024 + o] = 0;
+        for (int o = lane; o < 1024; o += 64) bufX[act_idx<32, 1>(wave, o >> 5, o & 31, 0)] = 0;
     }
     __syncthreads();
-    infer_tail(bufA, bufB, head, iw, K, p0, true, s_norm, out);
+    infer_net(bufX, bufY, head, w1, iw, K, p0, true, s_norm, out);
 }
 
 }  // namespace
@@ -769,7 +857,7 @@ int infer_weights(const float* theta, const uint16_t* theta_bf, InferWeights& iw
     iw = InferWeights{};
     int64_t total = 0;
     for (int l = 0; l < 5; ++l) {
-        iw.kp[l] = (kNet[l].kd + 15) / 16 * 16;
+        iw.kp[l] = (kNet[l].kd + 31) / 32 * 32;
         iw.off[l] = total;
         total += (int64_t)kNet[l].n * iw.kp[l];
     }
@@ -792,7 +880,7 @@ int cnn_forward_launch(const uint16_t* x, int K, const float* theta, const uint1
     InferWeights iw;
     int rc = infer_weights(theta, theta_bf, iw, stream);
     if (rc != SPX_OK) return rc;
-    cnn_infer_kernel<<<(K + kIP - 1) / kIP, 256, 0, stream>>>(x, K, iw, norm, out);
+    cnn_infer_kernel<<<(K + kNP - 1) / kNP, 64 * kIW, 0, stream>>>(x, K, iw, norm, out);
     SPX_LAUNCHED();
     return SPX_OK;
 }
@@ -805,7 +893,7 @@ int cnn_propose_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
     InferWeights iw;
     int rc = infer_weights(theta, theta_bf, iw, stream);
     if (rc != SPX_OK) return rc;
-    cnn_propose_kernel<<<(K + kIP - 1) / kIP, 256, 0, stream>>>(img, H, W, pitch, seeds, half, K, iw, params);
+    cnn_propose_kernel<<<(K + kNP - 1) / kNP, 64 * kIW, 0, stream>>>(img, H, W, pitch, seeds, half, K, iw, params);
     SPX_LAUNCHED();
     return SPX_OK;
 }

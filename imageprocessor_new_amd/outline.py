@@ -58,6 +58,51 @@ def _np_median(x: torch.Tensor) -> float:
 
 
 # ------------------------------------------------------------------------------------- template
+# The shape and blur parameters as the solver moves them: (name, transform, lower, upper) with
+# the bounds in the solver's coordinate. "log": a length or width (log of px); "log1": the
+# superellipse exponent p as log(p - 1); "logit": a light share as log(w / (1 - w)); "lin": the
+# turn phi (rad) and the harmonic amplitudes. Shape rows come first, then the blur's.
+def _shape_rows(ks):
+    rows = [("a", "log", math.log(0.2), math.log(8.0)), ("b", "log", math.log(0.2), math.log(8.0)),
+            ("p", "log1", math.log(0.3), math.log(30.0)), ("phi", "lin", -math.pi / 4, math.pi / 4)]
+    for k in ks:
+        rows += [(f"c{k}", "lin", -0.3, 0.3), (f"s{k}", "lin", -0.3, 0.3)]
+    return rows
+
+
+_WIDTH = (math.log(0.15), math.log(4.0))
+_BLUR_ROWS = {
+    "gauss2": [("s1x", "log") + _WIDTH, ("s1y", "log") + _WIDTH, ("s2x", "log") + _WIDTH, ("s2y", "log") + _WIDTH,
+               ("w2", "logit", math.log(0.005 / 0.995), math.log(0.6 / 0.4))],
+    "lens": [("sx", "log", math.log(0.1), math.log(4.0)), ("sy", "log", math.log(0.1), math.log(4.0)),
+             ("cutoff", "log", math.log(0.2), math.log(8.0))],
+    "gauss1": [("sx", "log") + _WIDTH, ("sy", "log") + _WIDTH],
+}
+# physical defaults of the blur parameters (px, light share, cycles / px)
+_BLUR_DEFAULTS = {"s1x": 1.0, "s1y": 1.0, "s2x": 2.0, "s2y": 2.0, "w2": 0.05, "sx": 1.0, "sy": 1.0, "cutoff": 0.45}
+
+
+def _to_solver(kind, value):
+    if kind == "log":
+        return math.log(value)
+    if kind == "log1":
+        return math.log(value - 1.0)
+    if kind == "logit":
+        return math.log(value / (1.0 - value))
+    return float(value)
+
+
+def _from_solver(kind, u):
+    if kind == "log":
+        return math.exp(u)
+    if kind == "log1":
+        return 1.0 + math.exp(u)
+    if kind == "logit":
+        return 1.0 / (1.0 + math.exp(-u))
+    return float(u)
+
+
+# ------------------------------------------------------------------------------------- template
 class OutlineTemplate:
     """The page's hole template on the device (model.py:25-267). theta = [log a, log b,
     log(p - 1), phi, (c_k, s_k) per harmonic k, blur parameters]; lengths in px."""
@@ -90,57 +135,51 @@ class OutlineTemplate:
         self.pixel_otf = torch.sinc(self.fx) * torch.sinc(self.fy)
         self.edge_fix = 1.0 / (torch.sinc(self.fx / u) * torch.sinc(self.fy / u) * torch.sinc(self.nu / (u * a)))
 
-    # parameters
+    # parameters: one row per entry of theta (_shape_rows + _BLUR_ROWS)
+    @property
+    def rows(self):
+        return _shape_rows(self.ks) + _BLUR_ROWS[self.blur]
+
+    def index(self, name):
+        return [r[0] for r in self.rows].index(name)
+
     @property
     def n_shape(self):
-        return 4 + 2 * len(self.ks)
+        return len(_shape_rows(self.ks))
 
     @property
     def n_blur(self):
-        return {"gauss2": 5, "lens": 3, "gauss1": 2}[self.blur]
+        return len(_BLUR_ROWS[self.blur])
 
     @property
     def n_params(self):
         return self.n_shape + self.n_blur
 
     def pack(self, a, b, p, phi, coeffs, blur_params):
-        return np.concatenate([[math.log(a), math.log(b), math.log(p - 1.0), phi],
-                               np.asarray(coeffs, float).ravel(), np.asarray(blur_params, float)])
+        """theta from the outline's physical values and a blur vector from blur_pack."""
+        shape = [_to_solver(kind, v) for (_, kind, _, _), v in zip(_shape_rows(()), (a, b, p, phi))]
+        return np.concatenate([shape, np.asarray(coeffs, float).ravel(), np.asarray(blur_params, float)])
 
-    def blur_pack(self, **kw):
-        if self.blur == "gauss2":
-            w2 = kw.get("w2", 0.05)
-            return [math.log(kw.get("s1x", 1.0)), math.log(kw.get("s1y", 1.0)), math.log(kw.get("s2x", 2.0)),
-                    math.log(kw.get("s2y", 2.0)), math.log(w2 / (1.0 - w2))]
-        sxy = [math.log(kw.get("sx", 1.0)), math.log(kw.get("sy", 1.0))]
-        return sxy + [math.log(kw.get("cutoff", 0.45))] if self.blur == "lens" else sxy
+    def blur_pack(self, **physical):
+        """The blur part of theta from physical values (missing ones take _BLUR_DEFAULTS)."""
+        return [_to_solver(kind, physical.get(name, _BLUR_DEFAULTS[name])) for name, kind, _, _ in _BLUR_ROWS[self.blur]]
 
     def unpack(self, theta):
+        """(a, b, p, phi, harmonic pairs (k, 2), blur part of theta) of a theta."""
         th = np.asarray(theta, float)
-        return (math.exp(th[0]), math.exp(th[1]), 1.0 + math.exp(th[2]), th[3], th[4:self.n_shape].reshape(-1, 2),
-                th[self.n_shape:])
+        a, b, p, phi = (_from_solver(kind, u) for (_, kind, _, _), u in zip(_shape_rows(()), th[:4]))
+        return a, b, p, phi, th[4:self.n_shape].reshape(-1, 2), th[self.n_shape:]
 
     def start(self, half_width_px, blur_sigma_px=1.0):
-        s = blur_sigma_px
+        """A round outline of the given half width under a blur of the given width (the second
+        Gaussian twice as wide)."""
+        s = float(blur_sigma_px)
         return self.pack(half_width_px, half_width_px, 2.0, 0.0, np.zeros((len(self.ks), 2)),
                          self.blur_pack(s1x=s, s1y=s, s2x=2 * s, s2y=2 * s, sx=s, sy=s))
 
     def bounds(self):
-        lo, hi = np.full(self.n_params, -np.inf), np.full(self.n_params, np.inf)
-        lo[0:2], hi[0:2] = math.log(0.2), math.log(8.0)
-        lo[2], hi[2] = math.log(0.3), math.log(30.0)
-        lo[3], hi[3] = -np.pi / 4, np.pi / 4
-        lo[4:self.n_shape], hi[4:self.n_shape] = -0.3, 0.3
-        j = self.n_shape
-        if self.blur == "gauss2":
-            lo[j:j + 4], hi[j:j + 4] = math.log(0.15), math.log(4.0)
-            lo[j + 4], hi[j + 4] = math.log(0.005 / 0.995), math.log(0.6 / 0.4)
-        elif self.blur == "lens":
-            lo[j:j + 2], hi[j:j + 2] = math.log(0.1), math.log(4.0)
-            lo[j + 2], hi[j + 2] = math.log(0.2), math.log(8.0)
-        else:
-            lo[j:j + 2], hi[j:j + 2] = math.log(0.15), math.log(4.0)
-        return lo, hi
+        rows = self.rows
+        return np.array([r[2] for r in rows], float), np.array([r[3] for r in rows], float)
 
     # outline (host: a few thousand angles)
     def radius_at(self, theta, angles):
@@ -341,24 +380,49 @@ def _level_bins(res: torch.Tensor, lev: torch.Tensor):
 
 
 # ---------------------------------------------------------------------------------- shape stage
-def stage_map(t, stage):
-    """Free parameters and ties of a stage (model.py:270-290): round outline with isotropic
-    blur; superellipse with the full blur; everything."""
-    j = t.n_shape
+# The shape fit releases the model in three stages. "round": a circle (b tied to a) under an
+# isotropic blur (every y width tied to its x width); "super": the superellipse and its turn
+# with the whole blur; "full": the harmonics as well. Parameters by name, ties as {copy: source}.
+_ROUND_BLUR = {"gauss2": (("s1x", "s2x", "w2"), {"s1y": "s1x", "s2y": "s2x"}),
+               "lens": (("sx", "cutoff"), {"sy": "sx"}),
+               "gauss1": (("sx",), {"sy": "sx"})}
+
+
+def _stage_parameters(t, stage):
+    """(free theta indices in ascending order, ties {index: source index}) of a stage."""
+    names = [r[0] for r in t.rows]
+    blur_names = names[t.n_shape:]
     if stage == "round":
-        free, ties = [0, j], {1: 0}
-        if t.blur == "gauss2":
-            free += [j + 2, j + 4]
-            ties.update({j + 1: j, j + 3: j + 2})
-        elif t.blur == "lens":
-            free += [j + 2]
-            ties.update({j + 1: j})
-        else:
-            ties.update({j + 1: j})
-        return free, ties
-    if stage == "super":
-        return [0, 1, 2, 3] + list(range(j, t.n_params)), {}
-    return list(range(t.n_params)), {}
+        free_blur, blur_ties = _ROUND_BLUR[t.blur]
+        free = ("a",) + free_blur
+        ties = dict(blur_ties, b="a")
+    elif stage == "super":
+        free, ties = ("a", "b", "p", "phi") + tuple(blur_names), {}
+    else:
+        free, ties = tuple(names), {}
+    return sorted(names.index(n) for n in free), {names.index(k): names.index(v) for k, v in ties.items()}
+
+
+def _round_start_grid(t, base):
+    """The round stage's start candidates: the base theta at 15 outline sizes (0.4 - 1.8 of its
+    half width) times 12 blur widths (0.45 - 2.2 px; the second Gaussian twice as wide with 5 %
+    of the light, the pupil cut-off at 0.5 cycles / px), sizes outermost."""
+    sizes = math.exp(base[0]) * np.linspace(0.4, 1.8, 15)
+    widths = np.linspace(0.45, 2.2, 12)
+    pairs = [(math.log(a), math.log(w), math.log(2.0 * w)) for a in sizes for w in widths]
+    A, S, S2 = (np.array(v) for v in zip(*pairs))
+    cand = np.repeat(np.asarray(base, float)[None], A.size, axis=0)
+    idx = t.index
+    cand[:, idx("a")] = cand[:, idx("b")] = A
+    first = "s1" if t.blur == "gauss2" else "s"
+    cand[:, idx(first + "x")] = cand[:, idx(first + "y")] = S
+    if t.blur == "gauss2":
+        cand[:, idx("s2x")] = cand[:, idx("s2y")] = S2
+        cand[:, idx("w2")] = math.log(0.05 / 0.95)
+    elif t.blur == "lens":
+        cand[:, idx("cutoff")] = math.log(0.5)
+    lo, hi = t.bounds()
+    return np.clip(cand, lo + 1e-6, hi - 1e-6)
 
 
 def _fd_steps(x0, f0, lo, hi):
@@ -403,28 +467,17 @@ def fit_stages(t, render_for, y, sw, theta0, stages=("round", "super", "full"), 
 
     chi2, scale, const = float("nan"), 1.0, 0.0
     for stage in stages:
-        free, ties = stage_map(t, stage)
+        free, ties = _stage_parameters(t, stage)
         base = theta.copy()
         if stage == "round":
-            base[1], base[2], base[3] = base[0], 0.0, 0.0
+            # a circle without harmonics, the best of the start grid when asked
+            base[t.index("b")] = base[t.index("a")]
+            base[t.index("p")] = 0.0
+            base[t.index("phi")] = 0.0
             base[4:t.n_shape] = 0.0
             if grid:
-                j = t.n_shape
-                a0 = math.exp(base[0])
-                cands = []
-                for a in a0 * np.linspace(0.4, 1.8, 15):
-                    for s in np.linspace(0.45, 2.2, 12):
-                        q = base.copy()
-                        q[0] = q[1] = math.log(a)
-                        q[j] = q[j + 1] = math.log(s)
-                        if t.blur == "gauss2":
-                            q[j + 2] = q[j + 3] = math.log(2 * s)
-                            q[j + 4] = math.log(0.05 / 0.95)
-                        elif t.blur == "lens":
-                            q[j + 2] = math.log(0.5)
-                        cands.append(np.clip(q, lo_all + 1e-6, hi_all - 1e-6))
-                r = resid_batch(np.stack(cands))[0]
-                chi = np.mean(r * r, axis=1)
+                cands = _round_start_grid(t, base)
+                chi = np.mean(resid_batch(cands)[0] ** 2, axis=1)
                 base = cands[int(np.argmin(chi))]
 
         def expand(q, base=base, free=free, ties=ties):
@@ -851,28 +904,37 @@ class OutlinePage:
         return ((c[:, 0] >= self.R + 1) & (c[:, 0] <= self.w - 2 - self.R) & (c[:, 1] >= self.R + 1)
                 & (c[:, 1] <= self.h - 2 - self.R))
 
-    def good_holes(self, page):
-        """Window inside the page, usual brightness and edge offset, usual leftover (page.py:784-803)."""
-        inside = self.inside()
-        a, delta = self.a.cpu().numpy(), self.delta.cpu().numpy()
-        med = np.median(a[inside]) if inside.any() else np.median(a)
-        dmed = np.median(delta[inside] if inside.any() else delta)
-        usual = (a > 0.4 * med) & (a < 2.5 * med) & (np.abs(delta - dmed) < 0.6)
+    def window_rms(self, page) -> torch.Tensor:
+        """Leftover rms of every hole's window against the page model (device, n)."""
         win = self.windows(page=page, outputs=("valid", "data", "model"))
-        res = torch.where(win["valid"], win["data"] - win["model"], 0.0)
-        cnt = win["valid"].sum(1).clamp(min=1).to(torch.float64)
-        rms = torch.sqrt((res * res).sum(1) / cnt).cpu().numpy()
-        typical = np.median(rms[inside]) if inside.any() else np.median(rms)
-        return inside & usual & (rms < 2.0 * typical), rms
+        left = torch.where(win["valid"], win["data"] - win["model"], 0.0)
+        return torch.sqrt((left * left).sum(1) / win["valid"].sum(1).clamp(min=1).to(torch.float64))
+
+    def good_holes(self, page):
+        """(good, rms) host arrays: the holes the shape stage may stack. A good hole's window lies
+        inside the page, its brightness is within 0.4 - 2.5 times and its edge offset within
+        0.6 px of the typical hole's (the medians over the holes inside), and its leftover rms
+        below twice the typical one (page.py:784-803)."""
+        rms = self.window_rms(page)
+        ins = torch.from_numpy(self.inside()).to(self.device)
+        pool = ins if bool(ins.any()) else torch.ones_like(ins)
+        a_typ = _np_median(self.a[pool])
+        d_typ = _np_median(self.delta[pool])
+        r_typ = _np_median(rms[pool])
+        ok = ins & (self.a > 0.4 * a_typ) & (self.a < 2.5 * a_typ) & ((self.delta - d_typ).abs() < 0.6)
+        ok &= rms < 2.0 * r_typ
+        return ok.cpu().numpy(), rms.cpu().numpy()
 
     def cleaned_stack(self, page, good, radius_px=None):
-        """(data - page + own - level) / a binned at each pixel's fine offset from its hole,
-        weighted by a^2 / variance; (stack, weight) (N, N) host arrays (page.py:732-766)."""
+        """The neighbour-cleaned average hole on the fine grid and its weight, (N, N) host arrays
+        (NaN where no pixel fell): every good hole's window pixels without the other holes' light
+        and the level, over its brightness, weighted by a^2 / variance (page.py:732-766)."""
         t = self.template
-        sums, wsum = self._stack(cleaned=True, page=page.reshape(-1).contiguous(), good=good, radius_px=radius_px)
-        with np.errstate(invalid="ignore", divide="ignore"):
-            stack = np.where(wsum > 0, sums / wsum, np.nan)
-        return stack.reshape(t.N, t.N), wsum.reshape(t.N, t.N)
+        total, weight = self._stack(cleaned=True, page=page.reshape(-1).contiguous(), good=good,
+                                    radius_px=radius_px)
+        mean = np.full(total.shape, np.nan)
+        np.divide(total, weight, out=mean, where=weight > 0)
+        return mean.reshape(t.N, t.N), weight.reshape(t.N, t.N)
 
     def fit_shape(self, stack, weight, stages):
         t = self.template
@@ -884,92 +946,143 @@ class OutlinePage:
                           stages=stages, log=self.log.append)
 
     # -------------------------------------------------------------------------------------- run
+    def _take_proposal(self, proposal, scale, weight):
+        """Move theta `weight` of the way to the shape stage's proposal and the brightnesses by
+        the matching power of its scale; returns the proposal's outline-CD change (px)."""
+        t = self.template
+        change = t.cd_px(proposal) - t.cd_px(self.theta)
+        self.a = self.a * scale ** weight
+        self.set_theta(self.theta + weight * (proposal - self.theta))
+        return change
+
+    def _choose_size(self, note):
+        sd, radius = self.delta_information()
+        self.size_on = sd < 0.3 * radius
+        note(f"  edge offsets per hole {'fitted' if self.size_on else 'held at zero'}: their typical "
+             f"standard error is {sd:.3f} px for an outline radius of {radius:.2f} px")
+
     def run(self, rounds=14, iterations=5, log=None, relax=0.5, tol_px=0.08 / 21.0, release_px=0.3 / 21.0,
             per_hole_size="auto"):
-        """Alternate the hole and shape stages (page.py:806-896)."""
-        started = time.perf_counter()
-        say = log if log is not None else (lambda s: None)
+        """The page fit: alternate hole stage and shape stage under _RoundPlan for at most
+        `rounds` rounds, then a last hole stage; returns the readings (_page_readings)."""
+        t_fit = time.perf_counter()
+        note = log if log is not None else (lambda s: None)
         self.initialise()
-        all_stages = ("round", "super", "full")
-        level, chi2_prev = 0, None
+        plan = _RoundPlan(iterations, relax, tol_px, release_px)
         self.size_on = per_hole_size is True
         for rnd in range(int(rounds)):
-            t_round = time.perf_counter()
-            first = rnd == 0
+            t_rnd = time.perf_counter()
             if rnd == 1 and per_hole_size == "auto":
-                sd, radius = self.delta_information()
-                self.size_on = sd < 0.3 * radius
-                say(f"  per-hole edge offset: median standard error {sd:.3f} px against the outline radius "
-                    f"{radius:.2f} px -> per-hole size {'on' if self.size_on else 'off'}")
-            page = self.fit_holes(iterations=iterations + 3 if rnd < 3 else max(3, iterations // 2 + 1),
-                                  size=self.size_on and not first)
+                self._choose_size(note)
+            page = self.fit_holes(iterations=plan.sweeps(rnd), size=self.size_on and rnd > 0)
             self.fit_noise(page)
-            good, rms = self.good_holes(page)
-            stack, weight = self.cleaned_stack(page, good, radius_px=0.55 * self.spacing if first else None)
-            stages = all_stages[:level + 1]
+            good, _ = self.good_holes(page)
+            stack, weight = self.cleaned_stack(page, good, radius_px=plan.stack_radius(rnd, self.spacing))
+            stages = plan.stages()
             proposal, scale, const, chi2 = self.fit_shape(stack, weight, stages)
-            t = self.template
-            change_px = t.cd_px(proposal) - t.cd_px(self.theta)
-            kappa = 1.0 if first else relax
-            self.a = self.a * scale ** kappa
-            self.set_theta(self.theta + kappa * (proposal - self.theta))
-            t = self.template
-            self.history.append(dict(round=rnd + 1, stages=list(stages), outline_cd_px=t.cd_px(self.theta), chi2=chi2,
-                                     proposal_change_px=change_px, const=const, good=int(np.sum(good)),
+            moved = self._take_proposal(proposal, scale, plan.step_weight(rnd))
+            cd = self.template.cd_px(self.theta)
+            self.history.append(dict(round=rnd + 1, stages=list(stages), outline_cd_px=cd, chi2=chi2,
+                                     proposal_change_px=moved, const=const, good=int(good.sum()),
                                      var_line=self.var_line, mean_delta_px=float(self.delta.mean()),
-                                     seconds=time.perf_counter() - t_round))
-            say(f"  round {rnd + 1} ({stages[-1]}): outline CD {t.cd_px(self.theta):.4f} px, proposal "
-                f"{change_px:+.4f} px, stack chi2 {chi2:.3f}, good {int(np.sum(good))}")
+                                     seconds=time.perf_counter() - t_rnd))
+            note(f"  [{rnd + 1}] {stages[-1]} model: CD {cd:.4f} px after a {moved:+.4f} px proposal, "
+                 f"stack chi2 {chi2:.3f} over {int(good.sum())} holes")
             for line in self.log:
-                say(line)
+                note(line)
             self.log = []
-            settled = chi2_prev is not None and abs(chi2 - chi2_prev) < 0.05 * chi2_prev
-            chi2_prev = chi2
-            if not first and settled and abs(change_px) < release_px and level < len(all_stages) - 1:
-                level += 1
-                chi2_prev = None
-                say(f"  {all_stages[level]} stage released")
-                continue
-            if level == len(all_stages) - 1 and settled and abs(change_px) < tol_px:
+            verdict = plan.decide(rnd, chi2, moved)
+            if verdict == "release":
+                note(f"  the {plan.stages()[-1]} model is free from the next round on")
+            elif verdict == "done":
                 self.converged = True
-                say(f"  converged after round {rnd + 1}")
+                note(f"  outline settled in round {rnd + 1}")
                 break
         page = self.fit_holes(iterations=2 * iterations, size=self.size_on)
         if not self.size_on:
             self.delta_information()
         good, rms = self.good_holes(page)
         stack, weight = self.cleaned_stack(page, good)
-        self.timings["total"] = time.perf_counter() - started
-        out = self.results(good, rms)
+        self.timings["total"] = time.perf_counter() - t_fit
+        out = _page_readings(self, good, rms)
         t = self.template
         out.update(stack=stack, stack_weight=weight, template=self.t0_full.cpu().numpy(),
                    template_box=t.render(self.theta, box_px=1.0 / t.U).cpu().numpy())
         return out
 
-    def results(self, good, rms):
-        """Per-hole results (page.py:898-937): centres moved to the outline's centroid, edge
-        offsets from the good holes' mean, CD of the offset outline from its exact area."""
-        t = self.template
-        cx0, cy0 = t.centroid_px(self.theta)
-        g = np.asarray(good, dtype=bool)
-        delta_raw = self.delta.cpu().numpy()
-        delta_mean = float(np.mean(delta_raw[g])) if g.any() else 0.0
-        delta = delta_raw - delta_mean
-        r, dr = t.radius_at(self.theta, t.angles)
-        gr = np.hypot(r, dr) / r
-        area0, perim, quad = math.pi * np.mean(r * r), 2 * math.pi * np.mean(r * gr), math.pi * np.mean(gr * gr)
-        area = area0 + perim * delta + quad * delta ** 2
-        delta_sd = self.delta_sd.cpu().numpy()
-        return dict(
-            centers_xy=self.c.cpu().numpy() + np.array([cx0, cy0]), brightness=self.a.cpu().numpy(),
-            level=self.level.cpu().numpy(), edge_offset_px=delta,
-            cd_px=2.0 * np.sqrt(np.maximum(area, 1e-6) / math.pi), cd_outline_px=t.cd_px(self.theta),
-            theta=self.theta.copy(), delta_sd_px=delta_sd,
-            cd_sd_px=delta_sd * (perim + 2 * quad * delta) / np.sqrt(math.pi * np.maximum(area, 1e-6)),
-            per_hole_size=self.size_on, edge_offset_mean_px=delta_mean, clip_px=self.clip_px,
-            at_clip=np.abs(delta_raw) >= 0.99 * (self.clip_px or np.inf),
-            describe=t.describe(self.theta, 1.0), good=g, rms=rms, var_line=self.var_line, history=self.history,
-            seconds=self.timings.get("total"), spacing_px=self.spacing, window_px=self.R, converged=self.converged)
+
+class _RoundPlan:
+    """The round schedule of the page fit (the reference's PageFit.run, page.py:806-896, keeps
+    the same numbers): `iterations` + 3 hole sweeps in the first three rounds and
+    max(3, iterations // 2 + 1) after them; the first round stacks only the core (0.55 of the
+    spacing) and takes the whole proposal, later rounds `relax` of it. The model starts round and
+    is released one stage further ("super", then "full") once the stack's chi2 holds within 5 %
+    over two rounds and the proposal moves the outline CD by less than `release_px`; with every
+    stage free, the same steadiness and a move below `tol_px` end the fit."""
+
+    ORDER = ("round", "super", "full")
+
+    def __init__(self, iterations, relax, tol_px, release_px):
+        self.iterations, self.relax = int(iterations), float(relax)
+        self.tol_px, self.release_px = float(tol_px), float(release_px)
+        self.level = 0
+        self.last_chi2 = None
+
+    def sweeps(self, rnd):
+        return self.iterations + 3 if rnd < 3 else max(3, self.iterations // 2 + 1)
+
+    def stack_radius(self, rnd, spacing):
+        return 0.55 * spacing if rnd == 0 else None
+
+    def step_weight(self, rnd):
+        return 1.0 if rnd == 0 else self.relax
+
+    def stages(self):
+        return self.ORDER[:self.level + 1]
+
+    def decide(self, rnd, chi2, moved):
+        """'release', 'done' or 'go' after round `rnd` (0-based)."""
+        steady = self.last_chi2 is not None and abs(chi2 - self.last_chi2) < 0.05 * self.last_chi2
+        self.last_chi2 = chi2
+        last = len(self.ORDER) - 1
+        if rnd > 0 and steady and abs(moved) < self.release_px and self.level < last:
+            self.level += 1
+            self.last_chi2 = None
+            return "release"
+        if self.level == last and steady and abs(moved) < self.tol_px:
+            return "done"
+        return "go"
+
+
+def _offset_area_coefficients(t, theta):
+    """(c0, c1, c2) with area(d) = c0 + c1 d + c2 d^2 the exact area of the outline moved d px
+    along its normal: in polar form r + d g with g = |(r, r')| / r, so the area
+    (1/2) integral (r + d g)^2 over the angle is quadratic in d."""
+    r, dr = t.radius_at(theta, t.angles)
+    g = np.hypot(r, dr) / r
+    return math.pi * np.mean(r * r), 2.0 * math.pi * np.mean(r * g), math.pi * np.mean(g * g)
+
+
+def _page_readings(pg, good, rms):
+    """The per-hole and page readings of a finished fit (the fields of PageFit.results,
+    page.py:898-937): centres at the outline's centroid, edge offsets about the good holes'
+    mean, each hole's CD (and its standard error) from the exact area of its offset outline."""
+    t, theta = pg.template, pg.theta
+    good = np.asarray(good, dtype=bool)
+    d_raw = pg.delta.cpu().numpy()
+    d_mean = float(np.mean(d_raw[good])) if good.any() else 0.0
+    d = d_raw - d_mean
+    c0, c1, c2 = _offset_area_coefficients(t, theta)
+    area = np.maximum(c0 + d * (c1 + c2 * d), 1e-6)
+    d_sd = pg.delta_sd.cpu().numpy()
+    return dict(
+        centers_xy=pg.c.cpu().numpy() + np.asarray(t.centroid_px(theta)), brightness=pg.a.cpu().numpy(),
+        level=pg.level.cpu().numpy(), edge_offset_px=d, cd_px=2.0 * np.sqrt(area / math.pi),
+        cd_outline_px=t.cd_px(theta), theta=np.array(theta, copy=True), delta_sd_px=d_sd,
+        cd_sd_px=d_sd * (c1 + 2.0 * c2 * d) / np.sqrt(math.pi * area), per_hole_size=pg.size_on,
+        edge_offset_mean_px=d_mean, clip_px=pg.clip_px, at_clip=np.abs(d_raw) >= 0.99 * (pg.clip_px or np.inf),
+        describe=t.describe(theta, 1.0), good=good, rms=np.asarray(rms), var_line=pg.var_line, history=pg.history,
+        seconds=pg.timings.get("total"), spacing_px=pg.spacing, window_px=pg.R, converged=pg.converged)
 
 
 # ------------------------------------------------------------------------------------ public API
@@ -1064,54 +1177,6 @@ def render_outline_page(theta, centers, brightness, shape, *, level: float = 500
             g.manual_seed(int(seed))
             page += float(noise) * torch.randn(page.shape, generator=g, dtype=torch.float64, device=dev)
         return page
-
-
-# ------------------------------------------------------------------------------------- reporting
-def blur_second(theta, blur: str) -> str:
-    """The blur's second part of a fitted theta in words, in pixels ('' for a form without one)
-    (reference outline.py:496-507): the second Gaussian's widths and light share, or the pupil's
-    cut-off."""
-    th = np.asarray(theta, dtype=float)
-    if blur == "gauss2":
-        s2x, s2y = np.exp(th[-3:-1])
-        return f"second Gaussian {s2x:.1f} x {s2y:.1f} px, {100 / (1 + math.exp(-th[-1])):.0f} % of the light"
-    if blur == "lens":
-        return f"pupil cut-off {math.exp(th[-1]):.2f} cycles/px"
-    return ""
-
-
-def fit_summary(fit_meta: dict | None, pixel_size: float = 1.0, unit: str = "px") -> dict | None:
-    """The page readings of a fit's summary ({"fitted": True, **OutlineFitResult.summary}, or
-    {"fitted": False, "reason": ...}) in `unit` (reference outline.py:306-318)."""
-    if not fit_meta:
-        return None
-    if not fit_meta.get("fitted"):
-        return {"fitted": False, "reason": fit_meta.get("reason", "")}
-    k = float(pixel_size)
-    keys = ("rounds_run", "rounds_max", "converged", "per_hole_size", "blur_family", "seconds", "holes_fitted",
-            "at_clip")
-    out = {"fitted": True, "outline_cd": fit_meta["outline_cd_px"] * k,
-           "blur_fwhm": [v * k for v in fit_meta["blur_fwhm_px"]], "p": fit_meta["superellipse_p"]}
-    out.update({("rounds" if n == "rounds_run" else n): fit_meta[n] for n in keys})
-    out["unit"] = unit
-    return out
-
-
-def fit_line(summary: dict | None, ascii_only: bool = False) -> str:
-    """One line on a page's outline fit from `fit_summary` (reference outline.py:321-334)."""
-    if not summary:
-        return ""
-    if not summary["fitted"]:
-        return f"not fitted ({summary['reason']})"
-    u, x = summary["unit"], "x" if ascii_only else "×"
-    state = (f"converged after {summary['rounds']} rounds" if summary["converged"]
-             else f"NOT converged in {summary['rounds']} rounds (the outline was still moving)")
-    size = ("a size per hole" if summary["per_hole_size"]
-            else "per-hole size off: holes too small for a size each at this noise, every hole reads the image's CD")
-    return (f"outline CD {summary['outline_cd']:.2f} {u}, blur FWHM {summary['blur_fwhm'][0]:.1f} {x} "
-            f"{summary['blur_fwhm'][1]:.1f} {u} ({summary['blur_family']}), exponent p {summary['p']:.2f}; {state}; "
-            f"{size}; {summary['holes_fitted']:,} holes fitted, {summary['at_clip']} at the edge-offset bound; "
-            f"{summary['seconds']:.0f} s")
 
 
 def fit_outline_fast(image, seeds_xy=None, *, pixel_size: float = 1.0, unit: str = "px", blur: str = "gauss2",

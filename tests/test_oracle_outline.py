@@ -183,29 +183,6 @@ def test_product_greedy_colours_equal_the_oracle_rule():
     assert np.array_equal(greedy_colours(c, 1.5 * 8.5), O.greedy_colours(c, 1.5 * 8.5))
 
 
-def test_outline_reporting_helpers():
-    from imageprocessor_new_amd.outline import blur_second, fit_line, fit_summary
-
-    t = O.Template(upsample=8, size_px=24)
-    th = t.pack(2.0, 2.0, 2.0, 0.0, np.zeros((2, 2)), t.blur_pack(s1x=1.0, s1y=1.0, s2x=2.5, s2y=2.0, w2=0.1))
-    assert blur_second(th, "gauss2") == "second Gaussian 2.5 x 2.0 px, 10 % of the light"
-    tl = O.Template(upsample=8, size_px=48, blur="lens")
-    assert blur_second(tl.pack(2.0, 2.0, 2.0, 0.0, np.zeros((2, 2)), tl.blur_pack(cutoff=0.5)), "lens") == \
-        "pupil cut-off 0.50 cycles/px"
-    assert blur_second(th, "gauss1") == ""
-    meta = {"fitted": True, "outline_cd_px": 4.0, "blur_fwhm_px": [3.0, 2.5], "superellipse_p": 2.4, "rounds_run": 10,
-            "rounds_max": 14, "converged": True, "per_hole_size": True, "blur_family": "gauss2", "seconds": 2.4,
-            "holes_fitted": 39042, "at_clip": 3}
-    sm = fit_summary(meta, pixel_size=21.0, unit="nm")
-    assert sm["outline_cd"] == pytest.approx(84.0) and sm["blur_fwhm"] == pytest.approx([63.0, 52.5])
-    assert sm["rounds"] == 10 and sm["unit"] == "nm"
-    line = fit_line(sm, ascii_only=True)
-    assert line.startswith("outline CD 84.00 nm, blur FWHM 63.0 x 52.5 nm (gauss2)") and "39,042 holes" in line
-    assert fit_line(fit_summary({"fitted": False, "reason": "fewer than 10 holes"})) == \
-        "not fitted (fewer than 10 holes)"
-    assert fit_summary(None) is None and fit_line(None) == ""
-
-
 def test_fit_outline_fast_checks_its_arguments_then_says_it_is_not_built():
     from imageprocessor_new_amd import OutlineFastResult, fit_outline_fast  # noqa: F401
 
