@@ -536,24 +536,28 @@ static_assert(kNP * 1024 <= kBufX && kNP * 8 * 8 * 32 <= kBufX, "bufX");
 static_assert(kNP * 16 * 16 * 16 <= kBufY, "bufY");
 
 struct InferWeights {
-    const uint16_t* wt;      // W^T per layer 0..4: [n][kp], kp = K padded to a multiple of 32, zeros beyond K
-    int64_t off[5];          // element offset of layer l in wt
-    int kp[5];               // padded K of layer l
+    const uint16_t* wt;      // W^T per layer 0..5: [n][kp], kp = K padded to a multiple of 32, zeros beyond K;
+                             // the head's 7 columns padded to 16 zero rows
+    int64_t off[6];          // element offset of layer l in wt
+    int kp[6];               // padded K of layer l
     const float* theta;      // FP32 parameters (biases; K x N layout)
     const uint16_t* wbf;     // BF16 K x N weights (head)
 };
+
+// Columns of layer l in its transposed copy (the head's 7 padded to one 16-column block).
+__host__ __device__ constexpr int padded_cols(int l) { return kNet[l].n < 16 ? 16 : kNet[l].n; }
 
 __global__ void transpose_weights_kernel(const uint16_t* wbf, InferWeights iw, uint16_t* wt) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t base = 0;
 #pragma unroll
-    for (int l = 0; l < 5; ++l) {
+    for (int l = 0; l < 6; ++l) {
         const int K = kNet[l].kd, N = kNet[l].n, Kp = iw.kp[l];
-        const int64_t cnt = (int64_t)N * Kp;
+        const int64_t cnt = (int64_t)padded_cols(l) * Kp;
         if (t >= base && t < base + cnt) {
             const int64_t q = t - base;
             const int n = (int)(q / Kp), k = (int)(q - (int64_t)n * Kp);
-            wt[iw.off[l] + q] = k < K ? wbf[param_offset_dev(l) + (int64_t)k * N + n] : (uint16_t)0;
+            wt[iw.off[l] + q] = (k < K && n < N) ? wbf[param_offset_dev(l) + (int64_t)k * N + n] : (uint16_t)0;
         }
         base += cnt;
     }
@@ -746,13 +750,25 @@ __device__ __forceinline__ void infer_net(uint16_t* bufX, uint16_t* bufY, float*
     __syncthreads();
     fc1_layer(bufX, bufY, iw.wt + iw.off[4], th + param_offset_dev(4) + 256 * 128, wave, lane);
     __syncthreads();
-    // head 128 -> 7 (FP32 on the VALU): thread (patch, output)
-    if (tid < kNP * 7) {
-        const int pt = tid / 7, j = tid - pt * 7;
+    // head 128 -> 7: one wave, rows = patches, columns = outputs (7 of a 16-column block), 4 k
+    // chunks; FP32 outputs, no ReLU
+    if (wave == 0) {
+        const int li = lane & 15, g4 = lane >> 4;
+        const uint16_t* wcol = iw.wt + iw.off[5] + li * 128 + 8 * g4;
+        f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) {
+            const bf16x8v b = load8(wcol + kc * 32);
+            bf16x8v a = bf16x8v{};
+            if (li < kNP) a = load8(bufY + li * 128 + kc * 32 + 8 * g4);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        }
         const int64_t wo = param_offset_dev(5);
-        float acc = 0.0f;
-        for (int k = 0; k < 128; ++k) acc = fmaf(bf2f(bufY[pt * 128 + k]), bf2f(iw.wbf[wo + k * 7 + j]), acc);
-        head[pt * 8 + j] = acc + th[wo + 128 * 7 + j];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int pt = 4 * g4 + r;
+            if (pt < kNP && li < 7) head[pt * 8 + li] = acc[r] + th[wo + 128 * 7 + li];
+        }
     }
     __syncthreads();
     if (tid < kNP && p0 + tid < K) {
@@ -856,10 +872,10 @@ namespace {
 int infer_weights(const float* theta, const uint16_t* theta_bf, InferWeights& iw, hipStream_t stream) {
     iw = InferWeights{};
     int64_t total = 0;
-    for (int l = 0; l < 5; ++l) {
+    for (int l = 0; l < 6; ++l) {
         iw.kp[l] = (kNet[l].kd + 31) / 32 * 32;
         iw.off[l] = total;
-        total += (int64_t)kNet[l].n * iw.kp[l];
+        total += (int64_t)padded_cols(l) * iw.kp[l];
     }
     void* ws = nullptr;
     int rc = workspace(sizeof(uint16_t) * (size_t)total, &ws, stream);

@@ -256,6 +256,7 @@ __global__ __launch_bounds__(256) void hole_step_kernel(HoleArgs A) {
 
 struct RenderArgs {
     double* page;
+    unsigned long long* acc;  // (H, W) fixed-point light changes, zero before and after a render
     int64_t H, W;
     float* cache;
     const float* tmpl;
@@ -276,6 +277,50 @@ __host__ __device__ inline int reach_half_width(int oy, int Rr, double rl2) {
     int w = 0;
     while (w < Rr && (double)((w + 1) * (w + 1) + oy * oy) <= rl2) ++w;
     return w;
+}
+
+// Order-independent sums: a value v enters a 64-bit integer accumulator as llrint(v * scale)
+// (two's complement), and integer addition does not depend on the order of the atomics, so a
+// sum is the same bit for bit in every run. The accumulator is folded into its FP64 / FP32
+// destination afterwards by one addition per element.
+constexpr double kFixPage = 4294967296.0;  // 2^32 per count: page light below 2^31 counts
+constexpr double kFixGrid = 4294967296.0;  // far-light splats
+constexpr double kFixStack = 16777216.0;   // 2^24: stack sums below 2^39
+
+__device__ __forceinline__ void fix_add(unsigned long long* acc, double v, double scale) {
+    atomicAdd(acc, (unsigned long long)(long long)llrint(v * scale));
+}
+
+// dst[i] += acc[i] / scale, acc[i] = 0, for n contiguous elements.
+template <typename T>
+__global__ void fix_fold_kernel(unsigned long long* acc, T* dst, int64_t n, double scale) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long v = (long long)acc[i];
+    if (v != 0) {
+        dst[i] = (T)((double)dst[i] + (double)v / scale);
+        acc[i] = 0ull;
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// The render thread's offset (ox, oy) of hole i inside the reach (row table in LDS).
+__device__ __forceinline__ void reach_offset(const int* row_start, const int* row_w, int side, int Rr, int q, int& ox,
+                                             int& oy) {
+    int lo = 0, hi = side;  // row r with row_start[r] <= q < row_start[r + 1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (row_start[mid] <= q)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    oy = lo - Rr;
+    ox = q - row_start[lo] - row_w[lo];
 }
 
 // One thread per (hole, offset inside the reach); the block builds the row table in LDS.
@@ -299,15 +344,8 @@ __global__ __launch_bounds__(256) void hole_render_kernel(RenderArgs A, int Pn) 
     if (t >= (unsigned)A.n * (unsigned)Pn) return;
     const int i = (int)(t / (unsigned)Pn);
     const int q = (int)(t - (unsigned)i * (unsigned)Pn);
-    int lo = 0, hi = side;  // row r with row_start[r] <= q < row_start[r + 1]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (row_start[mid] <= q)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    const int oy = lo - A.Rr, ox = q - row_start[lo] - row_w[lo];
+    int ox, oy;
+    reach_offset(row_start, row_w, side, A.Rr, q, ox, oy);
     const int64_t slot = (int64_t)i * side * side + (int64_t)(oy + A.Rr) * side + (ox + A.Rr);
     const int h = A.idx[i];
     const float old = A.cache[slot];
@@ -324,19 +362,60 @@ __global__ __launch_bounds__(256) void hole_render_kernel(RenderArgs A, int Pn) 
         const double de = A.delta[h];
         light = (float)(A.a[h] * ((double)v[0] + de * ((double)v[1] + de * (double)v[2])));
     }
+    // old light out, new light in, as fixed-point integers: the cached value leaves exactly
+    // the integer it entered as
     if (pxo == px && pyo == py) {
         // same pixel (the rounded centre did not move): one atomic for old out + new in
-        if (light != old) atomicAdd(A.page + py * A.W + px, (double)light - (double)old);
+        if (light != old)
+            atomicAdd(A.acc + py * A.W + px, (unsigned long long)(llrint((double)light * kFixPage) -
+                                                                  llrint((double)old * kFixPage)));
     } else {
-        if (old != 0.0f) atomicAdd(A.page + pyo * A.W + pxo, -(double)old);
-        if (inside) atomicAdd(A.page + py * A.W + px, (double)light);
+        if (old != 0.0f) fix_add(A.acc + pyo * A.W + pxo, -(double)old, kFixPage);
+        if (inside) fix_add(A.acc + py * A.W + px, (double)light, kFixPage);
     }
     A.cache[slot] = light;
 }
 
+// After hole_render_kernel: every pixel the render touched takes its accumulated change into
+// the page once (whichever thread exchanges it first) and leaves the accumulator at zero.
+__global__ __launch_bounds__(256) void hole_fold_kernel(RenderArgs A, int Pn) {
+    __shared__ int row_start[512];
+    __shared__ int row_w[512];
+    const int side = 2 * A.Rr + 1;
+    for (int r = threadIdx.x; r < side; r += blockDim.x) row_w[r] = reach_half_width(r - A.Rr, A.Rr, A.rl2);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int r = 0; r < side; ++r) {
+            row_start[r] = acc;
+            acc += row_w[r] >= 0 ? 2 * row_w[r] + 1 : 0;
+        }
+        row_start[side] = acc;
+    }
+    __syncthreads();
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)A.n * (unsigned)Pn) return;
+    const int i = (int)(t / (unsigned)Pn);
+    const int q = (int)(t - (unsigned)i * (unsigned)Pn);
+    int ox, oy;
+    reach_offset(row_start, row_w, side, A.Rr, q, ox, oy);
+    const int h = A.idx[i];
+    const int64_t pxo = (int64_t)rint(A.c_old[2 * (int64_t)i]) + ox;
+    const int64_t pyo = (int64_t)rint(A.c_old[2 * (int64_t)i + 1]) + oy;
+    const int64_t px = (int64_t)rint(A.c[2 * (int64_t)h]) + ox, py = (int64_t)rint(A.c[2 * (int64_t)h + 1]) + oy;
+    if (pxo >= 0 && pxo < A.W && pyo >= 0 && pyo < A.H) {
+        const long long v = (long long)atomicExch(A.acc + pyo * A.W + pxo, 0ull);
+        if (v != 0) A.page[pyo * A.W + pxo] += (double)v / kFixPage;
+    }
+    if (px >= 0 && px < A.W && py >= 0 && py < A.H) {
+        const long long v = (long long)atomicExch(A.acc + py * A.W + px, 0ull);
+        if (v != 0) A.page[py * A.W + px] += (double)v / kFixPage;
+    }
+}
+
 // Keys splat of every hole's far-light amplitude a (1 + 4 delta / cd0) on the padded pixel grid.
 __global__ void far_splat_kernel(const double* c, const double* a, const double* delta, double cd0, int n, int mx,
-                                 int my, int hp, int wp, float* grid) {
+                                 int my, int hp, int wp, unsigned long long* acc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double x = c[2 * (int64_t)i] + mx, y = c[2 * (int64_t)i + 1] + my;
@@ -352,7 +431,7 @@ __global__ void far_splat_kernel(const double* c, const double* a, const double*
     for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int p = 0; p < 4; ++p)
-            atomicAdd(grid + (int64_t)(iy + q - 1) * wp + (ix + p - 1), (float)(am * wy[q] * wx[p]));
+            fix_add(acc + (int64_t)(iy + q - 1) * wp + (ix + p - 1), (double)(float)(am * wy[q] * wx[p]), kFixGrid);
 }
 
 // Analytic near templates of an erf-edged disk 1/2 erfc((rho - r - d) / (sqrt2 s)) and its far kernel.
@@ -502,8 +581,8 @@ struct StackArgs {
     const double* delta;
     double nv0, nv1, radius_fine;
     int cleaned;
-    double* sums;
-    double* wsum;
+    unsigned long long* sums;  // fixed-point accumulators (N x N each), folded after the launch
+    unsigned long long* wsum;
 };
 
 __global__ __launch_bounds__(256) void outline_stack_kernel(StackArgs A) {
@@ -526,8 +605,8 @@ __global__ __launch_bounds__(256) void outline_stack_kernel(StackArgs A) {
     const int64_t b = (int64_t)by * A.N + (int64_t)bx;
     const double data = (double)A.image[py * A.pitch + px];
     if (!A.cleaned) {
-        atomicAdd(A.sums + b, data);
-        atomicAdd(A.wsum + b, 1.0);
+        fix_add(A.sums + b, data, kFixStack);
+        fix_add(A.wsum + b, 1.0, kFixStack);
         return;
     }
     float v[3];
@@ -536,14 +615,14 @@ __global__ __launch_bounds__(256) void outline_stack_kernel(StackArgs A) {
     const double own = ah * ((double)v[0] + de * ((double)v[1] + de * (double)v[2]));
     const double model = A.page[py * A.W + px] + A.level[h];
     const double wgt = ah * ah / (A.nv0 + A.nv1 * fmax(model, 0.0));
-    atomicAdd(A.sums + b, (data - model + own) / ah * wgt);
-    atomicAdd(A.wsum + b, wgt);
+    fix_add(A.sums + b, (data - model + own) / ah * wgt, kFixStack);
+    fix_add(A.wsum + b, wgt, kFixStack);
 }
 
 // Keys splat of a, a d and a d^2 of every hole on three padded pixel grids (FP64, grid k at
 // grid + k hp wp); a hole whose 4 x 4 taps leave the grid is dropped (page.py:371-397).
 __global__ void outline_far_splat_kernel(const double* c, const double* a, const double* delta, int n, int mx, int my,
-                                         int hp, int wp, double* grid) {
+                                         int hp, int wp, unsigned long long* acc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double x = c[2 * (int64_t)i] + mx, y = c[2 * (int64_t)i + 1] + my;
@@ -562,7 +641,7 @@ __global__ void outline_far_splat_kernel(const double* c, const double* a, const
             const double w = wy[q] * wx[p];
             const int64_t o = (int64_t)(iy + q - 1) * wp + (ix + p - 1);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) atomicAdd(grid + k * plane + o, m[k] * w);
+            for (int k = 0; k < 3; ++k) fix_add(acc + k * plane + o, m[k] * w, kFixGrid);
         }
 }
 
@@ -651,6 +730,17 @@ __global__ __launch_bounds__(1024) void select_kth_kernel(const double* x, int64
 
 }  // namespace
 
+// n zeroed 64-bit accumulators in the stream's workspace.
+int zeroed_accumulator(int64_t n, unsigned long long** out, hipStream_t stream) {
+    void* ws = nullptr;
+    int rc = workspace(sizeof(unsigned long long) * (size_t)n, &ws, stream);
+    if (rc != SPX_OK) return rc;
+    *out = static_cast<unsigned long long*>(ws);
+    zero_u64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(*out, n);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
 int outline_coverage_launch(const double* edge, const double* rho, const int32_t* ang, int N, int aa, double scale,
                             int nbatch, double* out, hipStream_t stream) {
     const int64_t nn = (int64_t)N * N;
@@ -689,9 +779,17 @@ int outline_stack_launch(const float* image, int64_t H, int64_t W, int64_t pitch
         set_error("outline_stack: too many (hole, pixel) pairs for one launch");
         return SPX_E_ARG;
     }
+    const int64_t nn = (int64_t)N * N;
+    unsigned long long* acc = nullptr;
+    int rc = zeroed_accumulator(2 * nn, &acc, stream);
+    if (rc != SPX_OK) return rc;
     StackArgs A{image, H, W, pitch, page, tmpl, N, U, R, idx, n, good, c, a, level, delta, nv0, nv1, radius_fine,
-                cleaned, sums, wsum};
+                cleaned, acc, acc + nn};
     outline_stack_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A);
+    SPX_LAUNCHED();
+    fix_fold_kernel<double><<<(unsigned)((nn + 255) / 256), 256, 0, stream>>>(acc, sums, nn, kFixStack);
+    SPX_LAUNCHED();
+    fix_fold_kernel<double><<<(unsigned)((nn + 255) / 256), 256, 0, stream>>>(acc + nn, wsum, nn, kFixStack);
     SPX_LAUNCHED();
     return SPX_OK;
 }
@@ -699,7 +797,13 @@ int outline_stack_launch(const float* image, int64_t H, int64_t W, int64_t pitch
 int outline_far_splat_launch(const double* c, const double* a, const double* delta, int n, int mx, int my, int hp,
                              int wp, double* grid, hipStream_t stream) {
     if (n == 0) return SPX_OK;
-    outline_far_splat_kernel<<<(n + 255) / 256, 256, 0, stream>>>(c, a, delta, n, mx, my, hp, wp, grid);
+    const int64_t cells = 3 * (int64_t)hp * wp;
+    unsigned long long* acc = nullptr;
+    int rc = zeroed_accumulator(cells, &acc, stream);
+    if (rc != SPX_OK) return rc;
+    outline_far_splat_kernel<<<(n + 255) / 256, 256, 0, stream>>>(c, a, delta, n, mx, my, hp, wp, acc);
+    SPX_LAUNCHED();
+    fix_fold_kernel<double><<<(unsigned)((cells + 255) / 256), 256, 0, stream>>>(acc, grid, cells, kFixGrid);
     SPX_LAUNCHED();
     return SPX_OK;
 }
@@ -736,11 +840,11 @@ int hole_step_launch(const float* image, int64_t H, int64_t W, int64_t pitch, co
     return SPX_OK;
 }
 
-int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int N, int U, int Rr,
-                       double rl2, const int32_t* idx, int n, const double* c_old, const double* c, const double* a,
-                       const double* delta, hipStream_t stream) {
+int hole_render_launch(double* page, unsigned long long* acc, int64_t H, int64_t W, float* cache, const float* tmpl,
+                       int N, int U, int Rr, double rl2, const int32_t* idx, int n, const double* c_old, const double* c,
+                       const double* a, const double* delta, hipStream_t stream) {
     if (n == 0) return SPX_OK;
-    RenderArgs A{page, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta};
+    RenderArgs A{page, acc, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta};
     int Pn = 0;
     for (int oy = -Rr; oy <= Rr; ++oy) {
         const int w = reach_half_width(oy, Rr, rl2);
@@ -754,13 +858,21 @@ int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const f
     }
     hole_render_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A, Pn);
     SPX_LAUNCHED();
+    hole_fold_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(A, Pn);
+    SPX_LAUNCHED();
     return SPX_OK;
 }
 
 int far_splat_launch(const double* c, const double* a, const double* delta, double cd0, int n, int mx, int my, int hp,
                      int wp, float* grid, hipStream_t stream) {
     if (n == 0) return SPX_OK;
-    far_splat_kernel<<<(n + 255) / 256, 256, 0, stream>>>(c, a, delta, cd0, n, mx, my, hp, wp, grid);
+    const int64_t cells = (int64_t)hp * wp;
+    unsigned long long* acc = nullptr;
+    int rc = zeroed_accumulator(cells, &acc, stream);
+    if (rc != SPX_OK) return rc;
+    far_splat_kernel<<<(n + 255) / 256, 256, 0, stream>>>(c, a, delta, cd0, n, mx, my, hp, wp, acc);
+    SPX_LAUNCHED();
+    fix_fold_kernel<float><<<(unsigned)((cells + 255) / 256), 256, 0, stream>>>(acc, grid, cells, kFixGrid);
     SPX_LAUNCHED();
     return SPX_OK;
 }

@@ -133,9 +133,10 @@ int hole_step_launch(const float* image, int64_t H, int64_t W, int64_t pitch, co
                      int N, int U, int R, const int32_t* idx, int n, double* c, double* a, double* level,
                      double* delta, double* c_old, double nv0, double nv1, double rscale, int size, double max_move,
                      double max_delta, double a_med, double relax, double* stat, hipStream_t stream);
-int hole_render_launch(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int N, int U, int Rr,
-                       double rl2, const int32_t* idx, int n, const double* c_old, const double* c, const double* a,
-                       const double* delta, hipStream_t stream);
+int hole_render_launch(double* page, unsigned long long* acc, int64_t H, int64_t W, float* cache, const float* tmpl,
+                       int N, int U, int Rr, double rl2, const int32_t* idx, int n, const double* c_old, const double* c,
+                       const double* a, const double* delta, hipStream_t stream);
+int zeroed_accumulator(int64_t n, unsigned long long** out, hipStream_t stream);
 int select_kth_launch(const double* x, int64_t n, int64_t stride, const int64_t* cols, const int64_t* ks, int nk,
                       double* out, hipStream_t stream);
 int far_splat_launch(const double* c, const double* a, const double* delta, double cd0, int n, int mx, int my, int hp,
@@ -531,8 +532,12 @@ int spx_hole_render(double* page, int64_t H, int64_t W, float* cache, const floa
     SPX_REQUIRE(n >= 0 && H > 0 && W > 0 && Rr >= 0 && Rr <= 255 && N >= 8 && U >= 1, "spx_hole_render: bad sizes");
     SPX_REQUIRE(n == 0 || (page && cache && tmpl && idx && c_old && c && a && delta), "spx_hole_render: null pointer");
     SPX_REQUIRE((int64_t)n * (2 * Rr + 1) * (2 * Rr + 1) < ((int64_t)1 << 40), "spx_hole_render: too many offsets");
-    return spx::hole_render_launch(page, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta,
-                                   spx::as_stream(stream));
+    if (n == 0) return SPX_OK;
+    unsigned long long* acc = nullptr;
+    hipStream_t s = spx::as_stream(stream);
+    int rc = spx::zeroed_accumulator(H * W, &acc, s);
+    if (rc != SPX_OK) return rc;
+    return spx::hole_render_launch(page, acc, H, W, cache, tmpl, N, U, Rr, rl2, idx, n, c_old, c, a, delta, s);
 }
 
 int spx_hole_sweep(const float* image, int64_t H, int64_t W, int64_t pitch, double* page, const float* tmpl, int32_t N,
@@ -551,6 +556,11 @@ int spx_hole_sweep(const float* image, int64_t H, int64_t W, int64_t pitch, doub
                 "spx_hole_sweep: null pointer");
     const int64_t P = (int64_t)(2 * Rr + 1) * (2 * Rr + 1);
     hipStream_t s = spx::as_stream(stream);
+    unsigned long long* acc = nullptr;
+    if (n > 0) {
+        int rc = spx::zeroed_accumulator(H * W, &acc, s);
+        if (rc != SPX_OK) return rc;
+    }
     for (int j = 0; j < ncolour; ++j) {
         const int32_t b = start[j], m = start[j + 1] - start[j];
         if (m == 0) continue;
@@ -558,7 +568,7 @@ int spx_hole_sweep(const float* image, int64_t H, int64_t W, int64_t pitch, doub
                                        c_old + 2 * (int64_t)b, opt->nv0, opt->nv1, opt->rscale, opt->size ? 1 : 0,
                                        opt->max_move, opt->max_delta, opt->a_med, opt->relax, stat + 4 * (int64_t)b, s);
         if (rc != SPX_OK) return rc;
-        rc = spx::hole_render_launch(page, H, W, cache + P * b, tmpl, N, U, Rr, rl2, order + b, m,
+        rc = spx::hole_render_launch(page, acc, H, W, cache + P * b, tmpl, N, U, Rr, rl2, order + b, m,
                                      c_old + 2 * (int64_t)b, c, a, delta, s);
         if (rc != SPX_OK) return rc;
     }

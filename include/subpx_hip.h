@@ -401,7 +401,9 @@ int spx_hole_step(const float* image, int64_t H, int64_t W, int64_t pitch, const
 
 /* Take the holes' cached light (cache (n, (2Rr+1)^2) FP32, at offsets around rint(c_old))
  * out of the page model and put their light at the current state in; offsets with
- * ox^2 + oy^2 > rl2 are skipped (outline_onepass.py:365-397). */
+ * ox^2 + oy^2 > rl2 are skipped (outline_onepass.py:365-397). The changes meet in 64-bit
+ * fixed-point integers (2^-32 counts) and each touched pixel takes its total in one FP64
+ * addition, so the page is the same bit for bit whatever order the threads run in. */
 int spx_hole_render(double* page, int64_t H, int64_t W, float* cache, const float* tmpl, int32_t N,
                     int32_t U, int32_t Rr, double rl2, const int32_t* idx, int32_t n, const double* c_old,
                     const double* c, const double* a, const double* delta, spx_stream_t stream);
@@ -423,9 +425,9 @@ int spx_hole_sweep(const float* image, int64_t H, int64_t W, int64_t pitch, doub
 int spx_select_kth(const double* x, int64_t n, int64_t stride, const int64_t* cols, const int64_t* ks, int32_t nk,
                    double* out, spx_stream_t stream);
 
-/* Keys splat of a_i (1 + 4 delta_i / cd0) at (c_i + (mx, my)) on the padded FP32 grid
+/* Keys splat of a_i (1 + 4 delta_i / cd0) at (c_i + (mx, my)) added to the padded FP32 grid
  * (hp, wp) (outline_onepass.py:399-427); the far light is this grid convolved with the far
- * kernel. */
+ * kernel. The taps meet in 64-bit fixed-point sums (order-independent). */
 int spx_far_splat(const double* c, const double* a, const double* delta, double cd0, int32_t n, int32_t mx,
                   int32_t my, int32_t hp, int32_t wp, float* grid, spx_stream_t stream);
 
@@ -460,16 +462,18 @@ int spx_outline_windows(const float* image, int64_t H, int64_t W, int64_t pitch,
  * the centre when radius_fine > 0): cleaned = 0: sums += grey level, wsum += 1; cleaned = 1:
  * sums += w (data - page - level + own) / a, wsum += w with w = a^2 / (nv0 + nv1 max(page +
  * level, 0)) and own = a (T0 + d T1 + d^2 T2). sums, wsum (N x N FP64) accumulate (zero them
- * first); FP64 atomics, so the sums' last bits depend on the order. */
+ * first); the pixels' terms are summed as 64-bit fixed-point integers (2^-24 per unit), so the
+ * sums do not depend on the order of the atomics. */
 int spx_outline_stack(const float* image, int64_t H, int64_t W, int64_t pitch, const double* page, const float* tmpl,
                       int32_t N, int32_t U, int32_t R, const int32_t* idx, int32_t n, const uint8_t* good,
                       const double* c, const double* a, const double* level, const double* delta, double nv0,
                       double nv1, double radius_fine, int32_t cleaned, double* sums, double* wsum,
                       spx_stream_t stream);
 
-/* Keys splat of a_i, a_i d_i and a_i d_i^2 at c_i + (mx, my) on three padded FP64 grids
+/* Keys splat of a_i, a_i d_i and a_i d_i^2 at c_i + (mx, my) added to three padded FP64 grids
  * (grid + k hp wp, k = 0, 1, 2; holes whose taps leave the grid are dropped): the far light is
- * the sum of the three convolved with the far kernels of T0, T1 and T2 (page.py:341-397). */
+ * the sum of the three convolved with the far kernels of T0, T1 and T2 (page.py:341-397). The
+ * taps meet in 64-bit fixed-point sums (order-independent). */
 int spx_outline_far_splat(const double* c, const double* a, const double* delta, int32_t n, int32_t mx,
                           int32_t my, int32_t hp, int32_t wp, double* grid, spx_stream_t stream);
 

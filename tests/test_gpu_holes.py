@@ -221,3 +221,54 @@ def test_render_covers_every_offset_of_the_reach():
         assert np.array_equal(got_cache != 0, ref_cache != 0), (x, y)
         assert np.all(np.abs(got_cache - ref_cache) <= 1e-6 * np.abs(ref_cache).max() + 1e-3 * np.abs(ref_cache))
         assert np.abs(got_page - ref_page).max() <= 1e-6 * np.abs(ref_page).max()
+
+
+def test_hole_pass_is_bit_reproducible():
+    """The page model and the far light are summed as fixed-point integers, so a second run of
+    the same sweeps gives the same page, centres, brightnesses and sweep log bit for bit (holes
+    packed densely enough that their render windows overlap)."""
+    from imageprocessor_new_amd import holes
+
+    rng = np.random.default_rng(21)
+    K = _page(rng, n_side=14, spacing=8.5, radius=2.4, sigma=1.2, noise=3.0)
+    n = len(K["c"])
+    c0 = K["c"] + rng.uniform(-0.1, 0.1, K["c"].shape)
+    runs = []
+    for _ in range(2):
+        pg = holes.HolePage(K["image"], c0, noise=3.0, brightness=K["a"], level=np.full(n, K["bg"]))
+        pg.set_template(K["radius"], K["sigma"])
+        keep = {}
+        log = pg.fit_holes(sweeps=4, size=True, far=True, keep=keep)
+        res = pg.result(log)
+        runs.append((keep["light"].cpu().numpy(), res, log))
+    (l0, r0, g0), (l1, r1, g1) = runs
+    assert np.array_equal(l0, l1)
+    for f in ("centers_xy", "brightness", "level", "edge_offset_px"):
+        assert np.array_equal(getattr(r0, f), getattr(r1, f)), f
+    assert [e["rms"] for e in g0] == [e["rms"] for e in g1]
+
+
+def test_outline_fit_is_bit_reproducible():
+    """Two fit_outline runs on one page: the same rounds, outline and per-hole results bit for bit
+    (every device sum of the fit is order-independent)."""
+    from imageprocessor_new_amd import outline as G
+    from imageprocessor_new_amd.backends import gpu_device
+
+    sp, nx, ny = 8.5, 16, 12
+    rng = np.random.default_rng(3)
+    X, Y = np.meshgrid(10.0 + sp * np.arange(nx), 10.0 + sp * np.arange(ny))
+    truth = np.stack([X.ravel(), Y.ravel()], 1) + rng.normal(0, 0.05, (nx * ny, 2))
+    with gpu_device(0):
+        t = G.OutlineTemplate()
+        theta = t.pack(2.0, 1.9, 2.4, 0.1, [[0.02, -0.01], [0.01, 0.0]],
+                       t.blur_pack(s1x=1.1, s1y=1.05, s2x=2.2, s2y=2.2, w2=0.08))
+        page = G.render_outline_page(theta, truth, 3000.0 * rng.uniform(0.95, 1.05, len(truth)),
+                                     (int(20 + sp * ny), int(20 + sp * nx)), level=500.0, noise=30.0,
+                                     edge_offset=rng.normal(0, 0.05, len(truth)), seed=3).cpu().numpy()
+    seeds = truth + rng.uniform(-0.3, 0.3, truth.shape)
+    a = G.fit_outline(page, seeds, rounds=4)
+    b = G.fit_outline(page, seeds, rounds=4)
+    assert a.rounds == b.rounds and a.converged == b.converged
+    assert np.array_equal(a.theta, b.theta)
+    for f in ("centers_xy", "cd_px", "brightness", "level", "edge_offset_px"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
