@@ -211,3 +211,23 @@ def test_budget_judge_predicts_the_cost_decrease():
     pred = float(d @ g)
     true = cost(p) - cost(p + d)
     assert pred > 0 and abs(pred - true) < 0.02 * true
+
+
+def test_cpu_restatement_matches_the_oracle():
+    # oracle/cpu_fit.cpp (the all-cores CPU baseline) against the NumPy oracle, with and without
+    # the Voronoi-cell mask, disks and ellipses, on several threads
+    from oracle import cpu_fit
+
+    arr = synthetic.make_workload("hex16384", seed=17, H=400, W=400)
+    seeds = np.rint(arr.centers_xy).astype(np.float32)
+    half = np.clip(np.ceil(arr.radius + 3 * arr.sigma), 7, 31).astype(np.int16)
+    model = arr.ellipse.astype(np.uint8)
+    nbr = erfdisk.seed_neighbours(seeds, 2 * np.sqrt(2) * (half.max() + 1.5), 8)
+    for nb in (None, nbr):
+        ref, rref, itref, stref = erfdisk.fit_batch(arr.image, seeds, half, model, neighbours=nb)
+        got, rgot, itgot, stgot, used = cpu_fit.fit_batch(arr.image, seeds, half, model, neighbours=nb, threads=4)
+        assert used == 4
+        assert np.array_equal(stgot, stref) and np.array_equal(itgot, itref)
+        ok = stref <= 1
+        assert np.abs(got[ok] - ref[ok]).max(axis=0)[:4].max() < 1e-8
+        assert np.abs(rgot[ok] / rref[ok] - 1).max() < 1e-9
