@@ -459,32 +459,46 @@ def cpu_lm_baseline(img_host, seeds_np, half_np, model_np, nbr_np, single_sample
             "cpu_model": cpu_model(), "converged": float((st == 0).mean())}
 
 
+def fit_counters(n: int = 18):
+    """spx_fit_counters of the last fit on the current stream: work counters, fallbacks,
+    overflow, then the features per bucket at 8 + b."""
+    import torch
+
+    from imageprocessor_new_amd import _lib
+
+    c = np.zeros(n, dtype=np.uint64)
+    _lib.call("spx_fit_counters", int(c.ctypes.data), n, int(torch.cuda.current_stream().cuda_stream))
+    return c
+
+
 def bucket_table(img_t, seeds_t, half_t, model_t, nbr_t, half_np, model_np, repeats: int = 5):
-    """Per (patch class, model) bucket: features, kernel ms (HIP events around a launch of that
-    bucket alone), counted TFLOP/s and its fraction of the FP32 peaks."""
+    """Per (patch class, model) bucket (the class from the bound of the feature's fitted pixels):
+    features, kernel ms (HIP events around a fit of that bucket alone), counted TFLOP/s and its
+    fraction of the FP32 peaks. Buckets and counts come from the kernel's own counters."""
     from imageprocessor_new_amd import _lib
     from imageprocessor_new_amd.fit import fit_erf_disks_device
 
     out = {}
     mask_all = _lib.bucket_mask_of(half_np, model_np)
-    npl = ((2 * half_np.astype(np.int64) + 1) ** 2 + 63) // 64
-    cls = np.select([npl <= 4, npl <= 8, npl <= 16, npl <= 32], [0, 1, 2, 3], 4)
-    bid = 2 * cls + (model_np == 1)
-    res = None
+    res = fit_erf_disks_device(img_t, seeds_t, half_t, model_t, options=_lib.fit_options(bucket_mask=mask_all),
+                               neighbours=nbr_t)
+    per_bucket = fit_counters()[8:18]
     for b in range(10):
-        if not (mask_all >> b) & 1:
+        if not (mask_all >> b) & 1 or per_bucket[b] == 0:
             continue
         opts = _lib.fit_options(bucket_mask=1 << b)
         res = fit_erf_disks_device(img_t, seeds_t, half_t, model_t, options=opts, neighbours=nbr_t, out=res)
         ms, _, _ = timed(lambda: fit_erf_disks_device(img_t, seeds_t, half_t, model_t, options=opts,
                                                       neighbours=nbr_t, out=res), repeats)
-        st = read_fit_stats()
+        st = fit_counters()
         fl = fit_flops(st)
+        n = int(st[8 + b])
         name = f"npl{4 << (b // 2)}_{'ellipse7' if b & 1 else 'disk6'}"
-        out[name] = {"features": int((bid == b).sum()), "kernel_ms": ms, "tflops": fl / (ms * 1e-3) / 1e12,
+        out[name] = {"features": n, "kernel_ms": ms, "tflops": fl / (ms * 1e-3) / 1e12,
                      "frac": fl / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
                      "frac_unpacked": fl / (ms * 1e-3) / 1e12 / PEAK_FP32_UNPACKED_TFLOPS,
-                     "mean_passes": float(st[2]) / max(1, int((bid == b).sum()))}
+                     "mean_passes": float(st[2]) / max(1, n),
+                     "mean_pixels_per_pass": float(st[0] + st[1]) / max(1, int(st[2]))}
     return out
 
 
@@ -1026,15 +1040,38 @@ def run_detect(args) -> int:
     log(f"detect: {legs}")
     cpu = None
     if not args.no_cpu_baseline:
-        from oracle import pipeline
+        from oracle import cpu_detect, cpu_fit, pipeline
 
+        def med(fn, n=5):
+            fn()
+            walls = []
+            for _ in range(n):
+                t = time.perf_counter()
+                r = fn()
+                walls.append(time.perf_counter() - t)
+            return float(np.median(walls)), r
+
+        cores = cpu_fit.host_cores()
+        kw_cpu = dict(area_min=20, area_max=5000, pad=3)
+        dt_all, (c_all, _) = med(lambda: cpu_detect.detect_centers_logquad(arr.image, threads=cores, **kw_cpu))
+        dt_one, (c_one, _) = med(lambda: cpu_detect.detect_centers_logquad(arr.image, threads=1, **kw_cpu))
         t = time.perf_counter()
         ref, _ = pipeline.detect_centers(arr.image, area_min=20, area_max=5000, pad=3, refine="logquad",
                                          logquad_variant="cpu", threshold="otsu")
-        dt = time.perf_counter() - t
-        cpu = {"value": ref.shape[0] / dt, "unit": "features/s", "cores": 1, "kind": "port",
-               "sample": f"the whole image through the oracle's NumPy restatement of the CPU detect path "
-                         f"(Otsu, components, per-label log-quadratic), {ref.shape[0]} centres in {dt:.2f} s"}
+        dt_np = time.perf_counter() - t
+        from scipy.spatial import cKDTree
+
+        d, _ = cKDTree(c_all).query(ref) if c_all.shape[0] and ref.shape[0] else (np.array([np.inf]), None)
+        cpu = {"value": c_all.shape[0] / dt_all, "unit": "features/s", "cores": int(cores), "kind": "port",
+               "sample": f"the whole image through the CPU detect path restated in C++ (oracle/cpu_detect.cpp, "
+                         f"OpenMP: Otsu, 8-connected components, per-label log-quadratic), {c_all.shape[0]} "
+                         f"centres in {dt_all * 1e3:.1f} ms on {cores} threads, median of 5",
+               "single_core": {"value": c_one.shape[0] / dt_one, "unit": "features/s", "cores": 1,
+                               "ms": dt_one * 1e3},
+               "numpy_one_thread": {"value": ref.shape[0] / dt_np, "unit": "features/s", "cores": 1,
+                                    "sample": f"the oracle's NumPy restatement, {ref.shape[0]} centres in "
+                                              f"{dt_np:.2f} s"},
+               "max_diff_vs_numpy_oracle_px": float(d.max()), "cpu_model": cpu_model()}
     # the reference's CPU log-quadratic is the path of this configuration; the GPU backend's
     # window fit (logquad_variant="gpu") refuses these 17 px flat-topped disks at its gates,
     # as the reference's GPU algorithm does (the oracle's restatement returns none either)

@@ -244,14 +244,27 @@ def fit_options(*, max_iter=30, step_tol=1e-6, lambda0=1e-3, lambda_up=10.0, lam
 
 
 def bucket_mask_of(half, model) -> int:
-    """Bucket mask (SpxFitOptions.bucket_mask) covering every (patch class, model) present in
-    host arrays of half sides and models."""
+    """Bucket mask (SpxFitOptions.bucket_mask) covering every bucket the features of host arrays
+    of half sides and models can fall in: a feature's fitted pixels (inside the image and its
+    cell) number at most its (2 half + 1)^2 patch pixels, so its bucket's patch class is at
+    most that of the whole patch."""
     import numpy as np
 
     h = np.asarray(half, dtype=np.int64).ravel()
     m = np.asarray(model, dtype=np.int64).ravel()
     ok = (h >= 2) & (h <= 31) & ((m == MODEL_ERF_DISK6) | (m == MODEL_ERF_ELLIPSE7))
-    npl = ((2 * h[ok] + 1) ** 2 + 63) // 64
-    cls = np.select([npl <= 4, npl <= 8, npl <= 16, npl <= 32], [0, 1, 2, 3], 4)
-    bits = np.unique(2 * cls + (m[ok] == MODEL_ERF_ELLIPSE7))
-    return int(np.bitwise_or.reduce(1 << bits)) if bits.size else 0
+    mask = 0
+    for md in (MODEL_ERF_DISK6, MODEL_ERF_ELLIPSE7):
+        sel = ok & (m == md)
+        if not sel.any():
+            continue
+        top = patch_class(int(((2 * h[sel].max() + 1) ** 2)))
+        for c in range(top + 1):
+            mask |= 1 << (2 * c + (md == MODEL_ERF_ELLIPSE7))
+    return mask
+
+
+def patch_class(npix: int) -> int:
+    """Patch class of a pixel count: 0..4 for at most 4, 8, 16, 32, 64 pixels per lane."""
+    npl = (int(npix) + 63) // 64
+    return 0 if npl <= 4 else 1 if npl <= 8 else 2 if npl <= 16 else 3 if npl <= 32 else 4

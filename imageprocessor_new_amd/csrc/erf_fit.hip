@@ -97,6 +97,17 @@ SPX_DEV_INLINE int spx_queue_grab(int32_t* q, int n) {
 SPX_DEV_INLINE unsigned long long spx_atomic_add(unsigned long long* p, unsigned long long v) {
     return atomicAdd(p, v);
 }
+SPX_DEV_INLINE uint64_t spx_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+SPX_DEV_INLINE int spx_lanes_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// A wave's LDS operations complete in order; the fences keep the compiler from moving LDS
+// accesses across this point, so words another lane wrote before it are read after it.
+SPX_DEV_INLINE void spx_wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 SPX_DEV_INLINE float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 SPX_DEV_INLINE float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 SPX_DEV_INLINE float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
@@ -118,13 +129,20 @@ namespace {
 
 using namespace fitdev;
 
-// LDS patches take NPL x 64 words per wave from dynamic LDS sized at launch for the
+// Every wave compacts its patches in NPL x 64 words of dynamic LDS sized at launch for the
 // workgroup's waves, so small workgroups do not reserve LDS for absent waves.
 template <int NPL, int P>
 __global__ __launch_bounds__(256) void erf_fit_kernel(FitArgs a) {
-    constexpr bool kLds = Patch<NPL>::kLds;
     extern __shared__ uint32_t s_words[];
-    erf_fit_wave<NPL, P>(a, kLds ? s_words + (threadIdx.x >> 6) * NPL * spx::fitdev::kWave : nullptr);
+    erf_fit_wave<NPL, P>(a, s_words + (threadIdx.x >> 6) * NPL * spx::fitdev::kWave);
+}
+
+// The LDS-resident disk buckets need a few registers more than three waves per SIMD allow
+// and run faster at three waves with a few dwords spilled outside the pixel loop than at two.
+template <int NPL, int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void erf_fit_kernel_w3(FitArgs a) {
+    extern __shared__ uint32_t s_words[];
+    erf_fit_wave<NPL, P>(a, s_words + (threadIdx.x >> 6) * NPL * spx::fitdev::kWave);
 }
 
 // Bucket id: patch-size class x model (disk 6 / ellipse 7 parameters).
@@ -135,56 +153,73 @@ __global__ void zero_i32(int32_t* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Bucket of feature f, or -1 for a feature with an invalid half side or model or whose
-// bucket the mask leaves out (it gets status SPX_FIT_TOO_FEW and NaNs and is not fitted).
-__device__ inline int feature_bucket(const int16_t* half, const uint8_t* model, int f, int32_t mask) {
-    const int hs = half[f];
-    const int md = model[f];
-    if (hs < 2 || hs > kMaxHalf || (md != SPX_MODEL_ERF_DISK6 && md != SPX_MODEL_ERF_ELLIPSE7)) return -1;
-    const int bk = 2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0);
+// Bucket of a feature with a valid half side and model from its pixel bound, or -1 when the
+// mask leaves that bucket out.
+__device__ inline int feature_bucket(int md, int bound, int32_t mask) {
+    const int bk = 2 * bucket_of_pixels(bound) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0);
     return (mask != 0 && !((mask >> bk) & 1)) ? -1 : bk;
 }
 
-// Pass 1 of the bucketing: the per-bucket feature counts (LDS counters, then one global
-// atomic per bucket and workgroup), and the outputs of features that are not fitted.
-__global__ __launch_bounds__(256) void bucket_count_kernel(const int16_t* half, const uint8_t* model, int K,
-                                                           int32_t* counts, float* params, double* centers,
-                                                           float* rms, int16_t* iters, uint8_t* status,
-                                                           int32_t mask) {
+__device__ inline bool feature_valid(const int16_t* half, const uint8_t* model, int f) {
+    const int hs = half[f];
+    const int md = model[f];
+    return hs >= 2 && hs <= kMaxHalf && (md == SPX_MODEL_ERF_DISK6 || md == SPX_MODEL_ERF_ELLIPSE7);
+}
+
+// Pass 1 of the bucketing: every feature's pixel bound (kept in its iters entry until the
+// fit writes it; -1 for a feature that is not fitted), the per-bucket feature counts (LDS
+// counters, then one global atomic per bucket and workgroup), and the outputs of features
+// that are not fitted: invalid half side or model, or a bucket the mask leaves out (status
+// SPX_FIT_TOO_FEW and NaNs).
+__global__ __launch_bounds__(256) void bucket_count_kernel(FitArgs a, int32_t* counts) {
+    // counts[b] steers the launch; stats[8 + b] keeps a copy for the caller
     __shared__ int32_t s_cnt[kKernBuckets];
     if (threadIdx.x < kKernBuckets) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < K) {
-        const int b = feature_bucket(half, model, f, mask);
+    if (f < a.nfeat) {
+        int b = -1;
+        if (feature_valid(a.half, a.model, f)) {
+            const int bound = patch_pixel_bound(a.seeds, a.half, a.nbr, a.nb, a.nfeat, a.H, a.W, f);
+            b = feature_bucket(a.model[f], bound, a.opt.bucket_mask);
+            a.iters[f] = (int16_t)(b < 0 ? -1 : bound);
+        }
         if (b < 0) {
-            for (int i = 0; i < 7; ++i) params[(int64_t)f * 7 + i] = __builtin_nanf("");
-            if (centers != nullptr) centers[2 * (int64_t)f] = centers[2 * (int64_t)f + 1] = __builtin_nan("");
-            rms[f] = __builtin_nanf("");
-            iters[f] = 0;
-            status[f] = SPX_FIT_TOO_FEW;
+            for (int i = 0; i < 7; ++i) a.params[(int64_t)f * 7 + i] = __builtin_nanf("");
+            if (a.centers != nullptr) a.centers[2 * (int64_t)f] = a.centers[2 * (int64_t)f + 1] = __builtin_nan("");
+            a.rms[f] = __builtin_nanf("");
+            a.iters[f] = -1;
+            a.status[f] = SPX_FIT_TOO_FEW;
         } else {
             atomicAdd(&s_cnt[b], 1);
         }
     }
     __syncthreads();
-    if (threadIdx.x < kKernBuckets && s_cnt[threadIdx.x] != 0) atomicAdd(&counts[threadIdx.x], s_cnt[threadIdx.x]);
+    if (threadIdx.x < kKernBuckets && s_cnt[threadIdx.x] != 0) {
+        atomicAdd(&counts[threadIdx.x], s_cnt[threadIdx.x]);
+        atomicAdd(&a.stats[8 + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+    }
 }
 
 // Pass 2: every bucket's ids back to back in one K-entry array, bucket b after the ids of
-// buckets 0 .. b-1. Each workgroup reserves its slots with one atomic per bucket.
-__global__ __launch_bounds__(256) void bucket_scatter_kernel(const int16_t* half, const uint8_t* model, int K,
-                                                             const int32_t* counts, int32_t* cursors,
-                                                             int32_t* order, int32_t mask) {
+// buckets 0 .. b-1. Each workgroup reserves its slots with one atomic per bucket. The iters
+// entries of features that are not fitted go back to 0.
+__global__ __launch_bounds__(256) void bucket_scatter_kernel(FitArgs a, const int32_t* counts, int32_t* cursors,
+                                                             int32_t* order) {
     __shared__ int32_t s_cnt[kKernBuckets];
     __shared__ int32_t s_base[kKernBuckets];
     if (threadIdx.x < kKernBuckets) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     int b = -1, local = 0;
-    if (f < K) {
-        b = feature_bucket(half, model, f, mask);
-        if (b >= 0) local = atomicAdd(&s_cnt[b], 1);
+    if (f < a.nfeat) {
+        const int bound = a.iters[f];
+        if (bound < 0) {
+            a.iters[f] = 0;
+        } else {
+            b = feature_bucket(a.model[f], bound, a.opt.bucket_mask);
+            local = atomicAdd(&s_cnt[b], 1);
+        }
     }
     __syncthreads();
     if (threadIdx.x < kKernBuckets) {
@@ -221,7 +256,8 @@ template <int B>
 constexpr KernelFn kernel_of() {
     constexpr int npl = 4 << (B / 2);
     constexpr int P = (B & 1) ? 7 : 6;
-    return &erf_fit_kernel<npl, P>;
+    if constexpr (P == 6 && npl > fitdev::kUnrollMax) return &erf_fit_kernel_w3<npl, P>;
+    else return &erf_fit_kernel<npl, P>;
 }
 
 constexpr KernelFn kKernels[kKernBuckets] = {kernel_of<0>(), kernel_of<1>(), kernel_of<2>(), kernel_of<3>(),
@@ -233,7 +269,7 @@ constexpr KernelFn kKernels[kKernBuckets] = {kernel_of<0>(), kernel_of<1>(), ker
 // Dynamic LDS of bucket b's kernel at `threads` per workgroup (the LDS slab of large patches).
 size_t lds_bytes(int b, int threads) {
     const int npl = 4 << (b / 2);
-    return npl > fitdev::kUnrollMax ? (size_t)(threads / fitdev::kWave) * npl * fitdev::kWave * sizeof(uint32_t) : 0;
+    return (size_t)(threads / fitdev::kWave) * npl * fitdev::kWave * sizeof(uint32_t);
 }
 
 int resident_blocks(int dev, int b, int threads) {
@@ -256,9 +292,11 @@ int resident_blocks(int dev, int b, int threads) {
 
 }  // namespace
 
-// Layout of a fit's scratch: counters u64[8] | counts[10] | cursors[10] | queues[10] | ids.
-constexpr size_t kFitHead = 256;
-constexpr int kFitHeadWords = (64 + 3 * kKernBuckets * 4) / 4;
+// Layout of a fit's scratch: counters u64[kFitCounters] | counts[10] | cursors[10] |
+// queues[10] | ids.
+constexpr size_t kFitHead = 512;
+constexpr int kFitHeadWords = (kFitCounters * 8 + 3 * kKernBuckets * 4) / 4;
+static_assert(kFitHeadWords * 4 <= (int)kFitHead, "fit scratch head");
 
 size_t fit_scratch_bytes(int32_t K) { return kFitHead + sizeof(int32_t) * (size_t)K; }
 
@@ -269,21 +307,22 @@ int fit_batch_enqueue(void* scratch, unsigned long long* stats, const uint16_t* 
                       const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
                       double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream) {
     int32_t* head = static_cast<int32_t*>(scratch);
-    int32_t* counts = head + 16;
+    int32_t* counts = head + 2 * kFitCounters;
     int32_t* cursors = counts + kKernBuckets;
     int32_t* queues = cursors + kKernBuckets;
     int32_t* order = reinterpret_cast<int32_t*>(static_cast<char*>(scratch) + kFitHead);
-    zero_i32<<<1, 64, 0, stream>>>(head, kFitHeadWords);
+    zero_i32<<<(kFitHeadWords + 63) / 64, 64, 0, stream>>>(head, kFitHeadWords);
     SPX_LAUNCHED();
-    zero_i32<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(stats), 16);
+    zero_i32<<<(2 * kFitCounters + 63) / 64, 64, 0, stream>>>(reinterpret_cast<int32_t*>(stats), 2 * kFitCounters);
     SPX_LAUNCHED();
     g_last_stats = stats;
     g_last_stats_stream = stream;
+    FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, order, nullptr, nullptr,
+              stats, params, centers, rms, iters, status, nbr, nb, K, counts, 0};
     const int blocks256 = (K + 255) / 256;
-    bucket_count_kernel<<<blocks256, 256, 0, stream>>>(half, model, K, counts, params, centers, rms, iters, status,
-                                                       opt.bucket_mask);
+    bucket_count_kernel<<<blocks256, 256, 0, stream>>>(a, counts);
     SPX_LAUNCHED();
-    bucket_scatter_kernel<<<blocks256, 256, 0, stream>>>(half, model, K, counts, cursors, order, opt.bucket_mask);
+    bucket_scatter_kernel<<<blocks256, 256, 0, stream>>>(a, counts, cursors, order);
     SPX_LAUNCHED();
 
     int dev = 0;
@@ -292,8 +331,6 @@ int fit_batch_enqueue(void* scratch, unsigned long long* stats, const uint16_t* 
     SPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const int waves_per_block = opt.tile_waves > 0 ? (opt.tile_waves < 4 ? opt.tile_waves : 4) : 4;
     const int threads = waves_per_block * fitdev::kWave;
-    FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, order, nullptr, nullptr,
-              stats, params, centers, rms, iters, status, nbr, nb, K, counts, 0};
     for (int b = 0; b < kKernBuckets; ++b) {
         if (opt.bucket_mask != 0 && !((opt.bucket_mask >> b) & 1)) continue;
         a.bucket = b;
@@ -377,7 +414,7 @@ int fit_batch_noalloc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t 
         std::lock_guard<std::mutex> lock(g_scratch_mu);
         base = g_scratch_ptr[dev];
     }
-    // the counters sit in the head's first 64 bytes; counts, cursors and queues follow them
+    // the counters sit at the head's start; counts, cursors and queues follow them
     unsigned long long* stats = static_cast<unsigned long long*>(base);
     return fit_batch_enqueue(base, stats, img, H, W, pitch, seeds, half, model, init, opt, K, nbr, nb, params,
                              centers, rms, iters, status, stream);

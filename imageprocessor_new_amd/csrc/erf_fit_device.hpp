@@ -8,6 +8,9 @@
 //   spx_atomic_add(unsigned long long*, unsigned long long),
 //   spx_pair_sum32 / spx_pair_sum16 / spx_pair_dpp<CTRL, BIT> / spx_self_dpp<CTRL> /
 //   spx_readlane: the butterfly steps of wave_sum_many,
+//   spx_ballot(bool): the wave's 64-bit predicate mask (uniform), spx_lanes_below(mask): set
+//   bits of the mask in lanes below this one, spx_wave_lds_sync(): LDS writes of every lane
+//   visible to every lane of the wave,
 //   hw_rcp / hw_rsq / hw_exp2 (float) and hw_rcp64 (double).
 // Every value that steers control flow is wave-uniform by construction, so the compiler
 // sees uniform (scalar) branches only. (A lane-dependent branch around a shuffle inside the
@@ -24,8 +27,10 @@
 // Gaussian LM kernel `lm_fit` (reference subpx/_gpu/deblend.py:55-152: butterfly totals
 // :32-39/:107, LU solve :113-130, accept/reject :141-149) and the disk LM `_fit_disks`
 // (reference subpx/_cpu/invariant_cd.py:165-246), redesigned for CDNA4:
-//   * the patch is gathered once from HBM into VGPRs, one 32-bit register per pixel
-//     (column, row and grey level packed), NPL pixels per lane;
+//   * the patch is gathered once from HBM and compacted to the pixels the fit uses (inside
+//     the image and the cell), one 32-bit word per pixel (column, row and grey level
+//     packed), at most NPL words per lane, held in VGPRs (or LDS for large patches); the
+//     feature's bucket (NPL) comes from an upper bound of that pixel count;
 //   * residual and Jacobian are FP32 on the VALU, with hardware rsq / rcp / exp2;
 //   * J^T J, J^T r and the cost are reduced over the wave with DPP (no LDS, no atomics),
 //     so a result does not depend on scheduling;
@@ -89,11 +94,12 @@ SPX_DEV_INLINE uint32_t pack_px(int col, int row, uint32_t v) {
     return (uint32_t)col | ((uint32_t)row << 8) | (v << 16);
 }
 
-// Work of one patch: NPL words per lane, lane l holding flattened pixels l, l + 64, ...;
-// slots up to `nslots` are weighted by `valid` (bit n = slot n), later slots are skipped.
-// Patches of up to kUnrollMax slots per lane keep their words in VGPRs (fully unrolled
-// loops); larger ones in the wave's LDS slab (word n of lane l at n * 64 + l, conflict
-// free), so the LM state keeps its registers.
+// Work of one patch: the fitted pixels compacted in patch order, pixel k in slot k / 64 of
+// lane k % 64, so every slot below `nslots` - 1 is full; slots are weighted by `valid` (bit
+// n = slot n), later slots are skipped. The words are compacted in the wave's LDS slab (word
+// n of lane l at n * 64 + l, conflict free); patches of up to kUnrollMax slots per lane then
+// copy them into VGPRs (fully unrolled loops), larger ones read them from the slab, so the
+// LM state keeps its registers.
 template <int NPL>
 struct Patch {
     static constexpr bool kLds = NPL > kUnrollMax;
@@ -105,10 +111,6 @@ struct Patch {
     SPX_DEV_INLINE uint32_t word(int n) const {
         if constexpr (kLds) return lds[n * kWave + spx_lane()];
         else return px[n];
-    }
-    SPX_DEV_INLINE void set(int n, uint32_t w) {
-        if constexpr (kLds) lds[n * kWave + spx_lane()] = w;
-        else px[n] = w;
     }
 };
 
@@ -154,7 +156,7 @@ SPX_DEV_INLINE void pixel_pass(const Patch<NPL>& pt, int hs, const Params<P>& pp
 
     // Register patches: a uniform guard keeps the loop fully unrolled with px[] in VGPRs;
     // LDS patches run a rolled loop over their slots.
-#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
+#pragma unroll(Patch<NPL>::kLds ? 4 : NPL)
     for (int n = 0; n < NPL; ++n) {
         if constexpr (Patch<NPL>::kLds) {
             if (n >= pt.nslots) break;
@@ -403,10 +405,25 @@ SPX_DEV_INLINE bool cell_exact(const FitArgs& a, int f, double qx, double qy, bo
 // |g_j|) exceeds it takes the FP32 decision (the FP64 form, off the exact value by ~1e-6 there,
 // has the same sign); any other pixel takes the exact rule above.
 struct CellForm {
-    float u[kMaxNb], v[kMaxNb], w[kMaxNb];
+    float u[kMaxNb], v[kMaxNb], w[kMaxNb], m[kMaxNb];  // m_j = max(|u_j|, |v_j|)
     float x0, y0, margin;
     uint32_t act;  // bit j: neighbour j present
 };
+
+// One neighbour's FP32 form and its rounding margin (inactive: all zero, margin 1e-3).
+SPX_DEV_INLINE void cell_term(float x0, float y0, float xs, float ys, bool act, float& u, float& v, float& w,
+                              float& m, double& margin) {
+    const double xj = act ? (double)xs : (double)x0, yj = act ? (double)ys : (double)y0;
+    const double aj = 2.0 * ((double)x0 - xj), bj = 2.0 * ((double)y0 - yj);
+    const double dx = xj - (double)x0, dy = yj - (double)y0;
+    u = (float)aj;
+    v = (float)bj;
+    w = (float)(dx * dx + dy * dy);
+    m = fmaxf(fabsf(u), fabsf(v));
+    // |x'|, |y'| <= 64 for any patch; FP32 rounding of u, v, w, x', y' and of the two FMAs
+    const double mj = 1e-5 * (fabs(aj) + fabs(bj) + 1.0) * 64.0 + 1e-6 * (dx * dx + dy * dy) + 1e-3;
+    margin = act ? mj : 1e-3;
+}
 
 SPX_DEV_INLINE void make_cell(float x0, float y0, const float (&xs)[kMaxNb], const float (&ys)[kMaxNb],
                               const bool (&active)[kMaxNb], CellForm& cf) {
@@ -416,53 +433,83 @@ SPX_DEV_INLINE void make_cell(float x0, float y0, const float (&xs)[kMaxNb], con
     double mg = 1e-3;
 #pragma unroll
     for (int j = 0; j < kMaxNb; ++j) {
-        const bool act = active[j];
-        cf.act |= act ? (1u << j) : 0u;
-        const double xj = act ? (double)xs[j] : (double)x0, yj = act ? (double)ys[j] : (double)y0;
-        const double aj = 2.0 * ((double)x0 - xj), bj = 2.0 * ((double)y0 - yj);
-        const double dx = xj - (double)x0, dy = yj - (double)y0;
-        cf.u[j] = (float)aj;
-        cf.v[j] = (float)bj;
-        cf.w[j] = (float)(dx * dx + dy * dy);
-        // |x'|, |y'| <= 64 for any patch; FP32 rounding of u, v, w, x', y' and of the two FMAs
-        const double mj = 1e-5 * (fabs(aj) + fabs(bj) + 1.0) * 64.0 + 1e-6 * (dx * dx + dy * dy) + 1e-3;
-        mg = act ? fmax(mg, mj) : mg;
+        cf.act |= active[j] ? (1u << j) : 0u;
+        double mj;
+        cell_term(x0, y0, xs[j], ys[j], active[j], cf.u[j], cf.v[j], cf.w[j], cf.m[j], mj);
+        mg = fmax(mg, mj);
     }
     cf.margin = (float)mg;
 }
 
+// The cell form of feature f built lane-parallel: lane j < 8 takes neighbour j (the same
+// terms as make_cell), and the wave reads them back as uniform values.
 SPX_DEV_INLINE void load_cell(const FitArgs& a, int f, CellForm& cf) {
+    const int lane = spx_lane();
     const float x0 = spx_uniform(a.seeds[2 * (int64_t)f]);
     const float y0 = spx_uniform(a.seeds[2 * (int64_t)f + 1]);
-    float xs[kMaxNb], ys[kMaxNb];
-    bool act[kMaxNb];
+    int id = -1;
+    if (lane < a.nb && lane < kMaxNb) id = a.nbr[(int64_t)f * a.nb + lane];
+    const bool act = id >= 0 && id < a.nfeat;
+    float xs = 0.0f, ys = 0.0f;
+    if (act) {
+        xs = a.seeds[2 * (int64_t)id];
+        ys = a.seeds[2 * (int64_t)id + 1];
+    }
+    float u, v, w, m;
+    double mj;
+    cell_term(x0, y0, xs, ys, act, u, v, w, m, mj);
+    cf.x0 = x0;
+    cf.y0 = y0;
+    cf.act = (uint32_t)(spx_ballot(act) & 0xffull);
+    // rounding to FP32 is monotonic, so the largest rounded margin is the largest margin rounded
+    cf.margin = spx_wave_max((float)mj);
 #pragma unroll
     for (int j = 0; j < kMaxNb; ++j) {
-        int id = -1;
-        if (j < a.nb) id = spx_uniform_i(a.nbr[(int64_t)f * a.nb + j]);
-        act[j] = id >= 0 && id < a.nfeat;
-        xs[j] = 0.0f;
-        ys[j] = 0.0f;
-        if (act[j]) {
-            xs[j] = spx_uniform(a.seeds[2 * (int64_t)id]);
-            ys[j] = spx_uniform(a.seeds[2 * (int64_t)id + 1]);
-        }
+        cf.u[j] = spx_readlane(u, j);
+        cf.v[j] = spx_readlane(v, j);
+        cf.w[j] = spx_readlane(w, j);
+        cf.m[j] = spx_readlane(m, j);
     }
-    make_cell(x0, y0, xs, ys, act, cf);
 }
 
-SPX_DEV_INLINE bool cell_in_fast(const FitArgs& a, int f, const CellForm& cf, double qx, double qy) {
-    const float px = (float)(qx - (double)cf.x0), py = (float)(qy - (double)cf.y0);
-    bool in = true, sure = true;
+// Cell membership and boundary flags of the pixel q = (qx, qy), at the FP32 offset (px, py) =
+// q - seed (rounded once from the exact difference): the FP32 first look, and the exact rule
+// wherever a form lies within the margin. The boundary flag is decided for member pixels only.
+SPX_DEV_INLINE void cell_flags(const FitArgs& a, int f, const CellForm& cf, float px, float py, double qx,
+                               double qy, bool& in, bool& edge) {
+    bool i = true, e = false, sure_i = true, sure_e = true;
 #pragma unroll
     for (int j = 0; j < kMaxNb; ++j) {
         if (!((cf.act >> j) & 1u)) continue;
         const float fj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j]));
-        in = in && fj > 0.0f;
+        const float gj = fj - cf.m[j];
+        i = i && fj > 0.0f;
+        sure_i = sure_i && fabsf(fj) > cf.margin;
+        e = e || gj < 0.0f;
+        sure_e = sure_e && fabsf(gj) > cf.margin;
+    }
+    if (!sure_i) i = cell_exact(a, f, qx, qy, false);
+    if (i && !sure_e) e = cell_exact(a, f, qx, qy, true);
+    in = i;
+    edge = i && e;
+}
+
+// Cell membership alone (cell_flags without the boundary flag).
+SPX_DEV_INLINE bool cell_in(const FitArgs& a, int f, const CellForm& cf, float px, float py, double qx, double qy) {
+    bool i = true, sure = true;
+#pragma unroll
+    for (int j = 0; j < kMaxNb; ++j) {
+        if (!((cf.act >> j) & 1u)) continue;
+        const float fj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j]));
+        i = i && fj > 0.0f;
         sure = sure && fabsf(fj) > cf.margin;
     }
-    if (!sure) in = cell_exact(a, f, qx, qy, false);
-    return in;
+    if (!sure) i = cell_exact(a, f, qx, qy, false);
+    return i;
+}
+
+SPX_DEV_INLINE bool cell_in_fast(const FitArgs& a, int f, const CellForm& cf, double qx, double qy) {
+    return cell_in(a, f, cf, (float)(qx - (double)cf.x0), (float)(qy - (double)cf.y0), qx, qy);
 }
 
 SPX_DEV_INLINE bool cell_edge_fast(const FitArgs& a, int f, const CellForm& cf, double qx, double qy) {
@@ -471,12 +518,73 @@ SPX_DEV_INLINE bool cell_edge_fast(const FitArgs& a, int f, const CellForm& cf, 
 #pragma unroll
     for (int j = 0; j < kMaxNb; ++j) {
         if (!((cf.act >> j) & 1u)) continue;
-        const float gj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j])) - fmaxf(fabsf(cf.u[j]), fabsf(cf.v[j]));
+        const float gj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j])) - cf.m[j];
         e = e || gj < 0.0f;
         sure = sure && fabsf(gj) > cf.margin;
     }
     if (!sure) e = cell_exact(a, f, qx, qy, true);
     return e;
+}
+
+// Upper bound of the pixels the fit of feature f uses: those of its patch inside the image
+// and, with a neighbour list, inside its cell. Row by row each neighbour's rule f_j > 0 bounds
+// the column (f_j = a_j qx + (b_j qy + c_j) is linear in it); the bound is relaxed by far more
+// than the FP64 rounding of f_j and of the FP32 first look, so every pixel the exact rule
+// keeps is counted. It chooses the feature's bucket before the fit. Plain per-thread code.
+// A seed too far out for an integer patch origin gets the whole patch as its bound.
+SPX_DEV_INLINE int patch_pixel_bound(const float* seeds, const int16_t* half, const int32_t* nbr, int nb,
+                                     int nfeat, int64_t H, int64_t W, int f) {
+    const int hs = half[f];
+    const int side = 2 * hs + 1;
+    const float sx = seeds[2 * (int64_t)f], sy = seeds[2 * (int64_t)f + 1];
+    if (!(fabsf(sx) < 1.0e15f) || !(fabsf(sy) < 1.0e15f)) return side * side;
+    const int64_t ox = (int64_t)rintf(sx) - hs, oy = (int64_t)rintf(sy) - hs;
+    const int64_t c0 = ox < 0 ? -ox : 0, r0 = oy < 0 ? -oy : 0;
+    const int64_t c1 = (W - 1 - ox) < (int64_t)(side - 1) ? (W - 1 - ox) : (int64_t)(side - 1);
+    const int64_t r1 = (H - 1 - oy) < (int64_t)(side - 1) ? (H - 1 - oy) : (int64_t)(side - 1);
+    if (c1 < c0 || r1 < r0) return 0;
+    if (nbr == nullptr || nb <= 0) return (int)((c1 - c0 + 1) * (r1 - r0 + 1));
+    const double x0 = (double)sx, y0 = (double)sy;
+    const double n0 = x0 * x0 + y0 * y0;
+    double ca[kMaxNb], cb[kMaxNb], cc[kMaxNb];
+    int m = 0;
+    for (int j = 0; j < nb && j < kMaxNb; ++j) {
+        const int id = nbr[(int64_t)f * nb + j];
+        if (id < 0 || id >= nfeat) continue;
+        const double xj = (double)seeds[2 * (int64_t)id], yj = (double)seeds[2 * (int64_t)id + 1];
+        const double aj = 2.0 * (x0 - xj), bj = 2.0 * (y0 - yj), cj = (xj * xj + yj * yj) - n0;
+        // a rule with a non-finite coefficient rejects every pixel: leaving it out keeps a bound
+        if (!isfinite(aj) || !isfinite(bj) || !isfinite(cj)) continue;
+        ca[m] = aj;
+        cb[m] = bj;
+        cc[m] = cj;
+        ++m;
+    }
+    const double span = fabs((double)ox) + (double)side;
+    int count = 0;
+    for (int64_t row = r0; row <= r1; ++row) {
+        const double qy = (double)(oy + row);
+        double lo = (double)(ox + c0), hi = (double)(ox + c1);
+        bool empty = false;
+        for (int k = 0; k < m; ++k) {
+            const double g = cb[k] * qy + cc[k];
+            const double e = 1e-9 * (fabs(ca[k]) * span + fabs(cb[k] * qy) + fabs(cc[k])) + 1e-9;
+            // the pixels with a_k qx + g >= -e
+            if (ca[k] > 0.0) lo = fmax(lo, (-g - e) / ca[k]);
+            else if (ca[k] < 0.0) hi = fmin(hi, (-g - e) / ca[k]);
+            else if (g < -e) empty = true;
+        }
+        if (empty) continue;
+        const double cl = ceil(lo - 1e-6), ch = floor(hi + 1e-6);
+        if (ch >= cl) count += (int)(ch - cl + 1.0);
+    }
+    return count;
+}
+
+// Patch class of a pixel count: NPL = 4, 8, 16, 32, 64 words per lane.
+SPX_DEV_INLINE int bucket_of_pixels(int npix) {
+    const int npl = (npix + kWave - 1) / kWave;
+    return npl <= 4 ? 0 : npl <= 8 ? 1 : npl <= 16 ? 2 : npl <= 32 ? 3 : 4;
 }
 
 // Work counters of one wave, flushed to FitArgs::stats once when the wave leaves its
@@ -486,6 +594,14 @@ struct WaveStats {
     unsigned long long passes;     // pixel passes
     unsigned long long steps;      // LM step attempts
     unsigned long long fallbacks;  // learned starts not converged within their budget
+    unsigned long long overflow;   // features with more pixels than their bucket holds
+};
+
+// This lane's partial sums for the moment start, taken during the gather: grey-level sum and
+// count of the ring pixels (the patch's outer ring and, with a cell, the cell's boundary) and
+// the maximum over the fitted pixels.
+struct StartSums {
+    float bsum, bcnt, vmax;
 };
 
 // Residual judge of a budgeted fit (below): the fit is accepted when the Gauss-Newton model
@@ -494,11 +610,11 @@ struct WaveStats {
 // exceed the converged fit's by more than the sums can resolve.
 constexpr double kJudgeRtol = 1.0e-6;
 
-// Fit of one feature by the whole wave. The patch registers are filled by the caller;
-// `masked`: the patch is restricted to the feature's Voronoi cell. `from_init`: start from
-// the caller's parameters (a.init), else from the patch moments. `budget`: a learned-start
-// pass (a.opt.learned_iters > 0) of at most learned_iters step attempts with the residual
-// judge (judge_accepts) at the start point and after every accepted step: at the first point
+// Fit of one feature by the whole wave. The patch and the moment start's ring sums `ss` come
+// from the caller's gather. `from_init`: start from the caller's parameters (a.init), else
+// from the patch moments. `budget`: a learned-start pass (a.opt.learned_iters > 0) of at most
+// learned_iters step attempts with the residual judge at the start point and after every
+// accepted step: at the first point
 // it accepts, its Gauss-Newton step is applied (as the plain rule applies its last
 // sub-tolerance step) and the fit is converged; the step-size rule still ends a fit as well.
 // A budgeted fit that is not converged then writes nothing and returns its attempt count: the
@@ -506,9 +622,8 @@ constexpr double kJudgeRtol = 1.0e-6;
 // fallback), passing that count as `it_before`, so that iters_out counts the attempts of both.
 // Returns -1 otherwise.
 template <int NPL, int P>
-SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>& pt,
-                               float cx, float cy, bool masked, bool from_init, bool budget, int it_before,
-                               WaveStats& ws) {
+SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>& pt, const StartSums& ss,
+                               float cx, float cy, bool from_init, bool budget, int it_before, WaveStats& ws) {
     using L = Layout<P>;
     const int lane = spx_lane();
     const float npix = pt.npix;
@@ -538,35 +653,11 @@ SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>
             // as the reference's _extract_voronoi_rois takes, subpx/_cpu/cells.py:72-80);
             // in-set = pixels above half the peak; centre = weighted centroid of the in-set;
             // radius from its area; amplitude = its mean; edge sigma = init_sigma; q = 1.
-            const int side = 2 * hs + 1;
-            const double ox = (double)cx - (double)hs, oy = (double)cy - (double)hs;
-            float bsum = 0.0f, bcnt = 0.0f, vmax = -3.0e38f;
-            CellForm cell;
-            if (masked) load_cell(a, f, cell);
-#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
-            for (int n = 0; n < NPL; ++n) {
-                if constexpr (Patch<NPL>::kLds) {
-                    if (n >= pt.nslots) break;
-                }
-                if (n < pt.nslots) {
-                    if ((pt.valid >> n) & 1ull) {
-                        const uint32_t w = pt.word(n);
-                        const int col = (int)(w & 0xffu), row = (int)((w >> 8) & 0xffu);
-                        const float val = (float)(w >> 16);
-                        bool ring = row == 0 || row == side - 1 || col == 0 || col == side - 1;
-                        if (!ring && masked) ring = cell_edge_fast(a, f, cell, ox + (double)col, oy + (double)row);
-                        if (ring) {
-                            bsum += val;
-                            bcnt += 1.0f;
-                        }
-                        vmax = fmaxf(vmax, val);
-                    }
-                }
-            }
-            bsum = spx_wave_sum(bsum);
-            bcnt = spx_wave_sum(bcnt);
+            // The ring's sum and count and the maximum were taken during the gather.
+            const float bsum = spx_wave_sum(ss.bsum);
+            const float bcnt = spx_wave_sum(ss.bcnt);
             const float B0 = bcnt > 0.0f ? bsum / bcnt : 0.0f;
-            vmax = spx_wave_max(vmax) - B0;
+            const float vmax = spx_wave_max(ss.vmax) - B0;
             const float half_peak = 0.5f * vmax;
             float s0 = 0.0f, sx = 0.0f, sy = 0.0f, nin = 0.0f;
 #pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
@@ -730,9 +821,9 @@ SPX_DEV_INLINE int fit_feature(const FitArgs& a, int f, int hs, const Patch<NPL>
     return -1;
 }
 
-// Body of the persistent fit kernel for one wave: pull features of one (patch size,
+// Body of the persistent fit kernel for one wave: pull features of one (patch class,
 // model) bucket from the work queue until it is empty. lds_words: the wave's NPL x 64-word
-// LDS slab for patches held in LDS (unused otherwise).
+// LDS slab, where every patch is compacted.
 template <int NPL, int P>
 SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
     const int lane = spx_lane();
@@ -744,7 +835,7 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
         for (int j = 0; j < a.bucket; ++j) off += a.bucket_counts[j];
         order += spx_uniform_i(off);
     }
-    WaveStats ws{0ull, 0ull, 0ull, 0ull};
+    WaveStats ws{0ull, 0ull, 0ull, 0ull, 0ull};
     // Features are claimed kGrab at a time: one queue atomic per kGrab features.
     int base = 0, left = 0;
     // A feature whose budgeted fit needs the fallback is gathered again and fitted from the
@@ -782,59 +873,105 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
                 const int id = spx_uniform_i(a.nbr[(int64_t)f * a.nb + j]);
                 masked = masked || (id >= 0 && id < a.nfeat);
             }
-        // Gather: lane-strided over the flattened patch, so each load instruction reads a
-        // few contiguous row segments of the image. Every lane loads from a clamped in-image
-        // address, so the loads carry no branch and issue back to back (one wait, not one per
-        // slot); row and column advance by the uniform quotient and remainder of 64 / side
-        // instead of a division per slot.
+        // Gather and compaction: the wave walks the patch 64 pixels at a time, lane-strided
+        // over the flattened patch, so each load instruction reads a few contiguous row
+        // segments of the image. The loads of kBatch steps are issued back to back before
+        // their pixels are tested (every lane loads from a clamped in-image address, so they
+        // carry no branch). The pixels inside the image (and the cell) are appended to the LDS
+        // slab in patch order: kept pixel k is word k, slot k / 64 of lane k % 64. Row and
+        // column advance by the uniform quotient and remainder of 64 / side instead of a
+        // division per step. The moment start's ring sums and maximum are taken on the way.
+        constexpr int kCap = NPL * kWave;
+        constexpr int kBatch = 8;
         Patch<NPL> pt;
         pt.lds = lds_words;
-        pt.nslots = (npx + kWave - 1) / kWave;
-        pt.valid = 0ull;
-        float cnt = 0.0f;
+        StartSums ss{0.0f, 0.0f, -3.0e38f};
+        int kept = 0;  // uniform
         {
+            CellForm cell;
+            if (masked) load_cell(a, f, cell);
+            // offsets from the seed: (col - hs) + (cx - x0), one rounding of the exact value
+            const float dcx = cx - spx_uniform(a.seeds[2 * (int64_t)f]);
+            const float dcy = cy - spx_uniform(a.seeds[2 * (int64_t)f + 1]);
+            spx_wave_lds_sync();  // every lane is done with the previous feature's words
+            const int raw = (npx + kWave - 1) / kWave;
             const int dq = kWave / side, dr = kWave - dq * side;
             int row = lane / side;
             int col = lane - row * side;
             const int64_t W1 = a.W - 1, H1 = a.H - 1;
-#pragma unroll(Patch<NPL>::kLds ? 8 : NPL)
-            for (int n = 0; n < NPL; ++n) {
-                const int pix = lane + kWave * n;
-                const bool in_patch = pix < npx;
-                // lanes past the patch's end read its first row (lines the wave fetches anyway)
-                const int64_t gx = ox + col, gy = oy + (in_patch ? row : 0);
-                const bool ok = in_patch && gx >= 0 && gx <= W1 && gy >= 0 && gy <= H1;
-                const int64_t qx = gx < 0 ? 0 : (gx > W1 ? W1 : gx);
-                const int64_t qy = gy < 0 ? 0 : (gy > H1 ? H1 : gy);
-                const uint32_t v = (uint32_t)a.img[qy * a.pitch + qx];
-                // masked by arithmetic, not a select, so the load stays unconditional
-                pt.set(n, pack_px(col, row, v) & (0u - (uint32_t)ok));
-                pt.valid |= ok ? (1ull << n) : 0ull;
-                cnt += ok ? 1.0f : 0.0f;
-                col += dr;
-                row += dq;
-                if (col >= side) {
-                    col -= side;
-                    ++row;
+            for (int n0 = 0; n0 < raw; n0 += kBatch) {
+                uint32_t v[kBatch];
+                int cs[kBatch], rs[kBatch];
+                bool inside[kBatch];
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    const int pix = lane + kWave * (n0 + j);
+                    const bool in_patch = pix < npx;
+                    // lanes past the patch's end read its first row (lines the wave fetches anyway)
+                    const int64_t gx = ox + col, gy = oy + (in_patch ? row : 0);
+                    inside[j] = in_patch && gx >= 0 && gx <= W1 && gy >= 0 && gy <= H1;
+                    const int64_t qx = gx < 0 ? 0 : (gx > W1 ? W1 : gx);
+                    const int64_t qy = gy < 0 ? 0 : (gy > H1 ? H1 : gy);
+                    v[j] = (uint32_t)a.img[qy * a.pitch + qx];
+                    cs[j] = col;
+                    rs[j] = row;
+                    col += dr;
+                    row += dq;
+                    if (col >= side) {
+                        col -= side;
+                        ++row;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    if (n0 + j >= raw) break;
+                    bool ok = inside[j];
+                    bool ring = rs[j] == 0 || rs[j] == side - 1 || cs[j] == 0 || cs[j] == side - 1;
+                    if (masked && ok) {
+                        const float px = (float)(cs[j] - hs) + dcx, py = (float)(rs[j] - hs) + dcy;
+                        const double qx = (double)(ox + cs[j]), qy = (double)(oy + rs[j]);
+                        if (from_init) {
+                            ok = cell_in(a, f, cell, px, py, qx, qy);
+                        } else {
+                            bool edge;
+                            cell_flags(a, f, cell, px, py, qx, qy, ok, edge);
+                            ring = ring || edge;
+                        }
+                    }
+                    if (!from_init && ok) {
+                        const float val = (float)v[j];
+                        ss.vmax = fmaxf(ss.vmax, val);
+                        if (ring) {
+                            ss.bsum += val;
+                            ss.bcnt += 1.0f;
+                        }
+                    }
+                    const uint64_t keep = spx_ballot(ok);
+                    const int dst = kept + spx_lanes_below(keep);
+                    if (ok && dst < kCap) lds_words[dst] = pack_px(cs[j], rs[j], v[j]);
+                    kept += __builtin_popcountll(keep);
                 }
             }
         }
-        if (masked) {
-            // restrict to the Voronoi cell (the cell's 64 FP64 registers live only here)
-            CellForm cell;
-            load_cell(a, f, cell);
-            cnt = 0.0f;
-#pragma unroll(Patch<NPL>::kLds ? 2 : NPL)
-            for (int n = 0; n < NPL; ++n) {
-                const uint32_t wd = pt.word(n);
-                const int col = (int)(wd & 0xffu), row = (int)((wd >> 8) & 0xffu);
-                const bool in = ((pt.valid >> n) & 1ull) && cell_in_fast(a, f, cell, (double)(ox + col), (double)(oy + row));
-                if (!in) pt.valid &= ~(1ull << n);
-                cnt += in ? 1.0f : 0.0f;
-            }
+        if (kept > kCap) {  // the bucket's bound was too small: never by construction
+            ++ws.overflow;
+            kept = kCap;
         }
-        pt.npix = spx_wave_sum(cnt);
-        const int again = fit_feature<NPL, P>(a, f, hs, pt, cx, cy, masked, from_init, budget, it_before, ws);
+        pt.nslots = (kept + kWave - 1) / kWave;
+        // the last slot's empty words are zeroed (its valid bits exclude them as well)
+        if (lane < pt.nslots * kWave - kept) lds_words[kept + lane] = 0u;
+        spx_wave_lds_sync();
+        {
+            const int full = kept / kWave;
+            const bool part = full < kWave && lane < kept - full * kWave;
+            pt.valid = (full >= kWave ? ~0ull : ((1ull << full) - 1ull)) | (part ? (1ull << full) : 0ull);
+        }
+        if constexpr (!Patch<NPL>::kLds) {
+#pragma unroll
+            for (int n = 0; n < NPL; ++n) pt.px[n] = lds_words[n * kWave + lane];
+        }
+        pt.npix = (float)kept;
+        const int again = fit_feature<NPL, P>(a, f, hs, pt, ss, cx, cy, from_init, budget, it_before, ws);
         if (again >= 0) {
             redo = f;
             redo_iters = again;
@@ -844,13 +981,7 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
     if (lane == 9) spx_atomic_add(&a.stats[2], ws.passes);
     if (lane == 10) spx_atomic_add(&a.stats[3], ws.steps);
     if (lane == 11 && ws.fallbacks != 0ull) spx_atomic_add(&a.stats[4], ws.fallbacks);
-}
-
-// Patch-size bucket of a half side: NPL = 4, 8, 16, 32, 64 pixels per lane.
-SPX_DEV_INLINE int bucket_of(int hs) {
-    const int npx = (2 * hs + 1) * (2 * hs + 1);
-    const int npl = (npx + kWave - 1) / kWave;
-    return npl <= 4 ? 0 : npl <= 8 ? 1 : npl <= 16 ? 2 : npl <= 32 ? 3 : 4;
+    if (lane == 12 && ws.overflow != 0ull) spx_atomic_add(&a.stats[5], ws.overflow);
 }
 
 }  // namespace fitdev

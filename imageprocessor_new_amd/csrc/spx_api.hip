@@ -60,7 +60,7 @@ int fit_stats_buffer(unsigned long long** out, hipStream_t stream) {
     unsigned long long*& p = g_fit_stats[StreamKey(dev, stream)];
     if (p == nullptr) {
         void* m = nullptr;
-        if (hipMalloc(&m, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        if (hipMalloc(&m, kFitCounters * sizeof(unsigned long long)) != hipSuccess) {
             (void)hipGetLastError();
             set_error("fit counters: hipMalloc failed");
             return SPX_E_NOMEM;
@@ -361,7 +361,7 @@ int spx_fit_stats(uint64_t* out, spx_stream_t stream) {
 int spx_fit_counters(uint64_t* out, int32_t n, spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(out != nullptr, "spx_fit_counters: out is NULL");
-    SPX_REQUIRE(n >= 1 && n <= 8, "spx_fit_counters: n must lie in [1, 8]");
+    SPX_REQUIRE(n >= 1 && n <= 18, "spx_fit_counters: n must lie in [1, 18]");
     return spx::fit_stats_copy(out, n, spx::as_stream(stream));
 }
 

@@ -50,8 +50,10 @@ constexpr int kWave = 64;
 // first; the memory stays valid for work ordered on the stream until the next call on it.
 int workspace(size_t bytes, void** out, hipStream_t stream);
 
-// The fit's own counters (8 x u64) for one (device, stream) pair: a small buffer that no other
-// entry point writes, kept until spx_release.
+// The fit's own counters (kFitCounters x u64: the work counters, then the features per bucket
+// at 8 + b) for one (device, stream) pair: a small buffer that no other entry point writes,
+// kept until spx_release.
+constexpr int kFitCounters = 32;
 int fit_stats_buffer(unsigned long long** out, hipStream_t stream);
 
 }  // namespace spx

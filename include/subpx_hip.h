@@ -59,8 +59,10 @@ typedef struct SpxFitOptions {
     int32_t persistent;   /* 1 = grid of the chip's resident capacity pulling from the
                              work queue, 0 = one wave per feature                        */
     int32_t bucket_mask;  /* bit 2 c + m: launch the (patch class c, model m) kernel;
-                             c = 0..4 for 4, 8, 16, 32, 64 pixels per lane ((2 half + 1)^2
-                             / 64 rounded up), m = 1 for ellipses. Features of buckets not
+                             c = 0..4 for at most 4, 8, 16, 32, 64 pixels per lane, from an
+                             upper bound of the feature's fitted pixels (its patch's pixels
+                             inside the image and, with neighbours, inside its cell) / 64
+                             rounded up; m = 1 for ellipses. Features of buckets not
                              launched are not fitted (NaN, status too few). 0 = all ten,
                              so callers that know their patch sizes skip empty launches  */
     int32_t learned_iters; /* > 0: budgeted fit of a learned start (init; without init the moment
@@ -268,7 +270,7 @@ int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch
 /* spx_fit_batch_cells (neighbours may be NULL) that allocates nothing: no counterpart in the
  * reference; for callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc). Its bucket counters, work queues,
- * feature lists and work counters live in a fixed device array of 4 MiB + 256 B per GPU, held
+ * feature lists and work counters live in a fixed device array of 4 MiB + 512 B per GPU, held
  * until the process ends, so K is at most SPX_FIT_NOALLOC_MAX_FEATURES (SPX_E_ARG beyond).
  * Every call on a GPU shares that array and clears what must start at zero: calls that
  * overlap on one GPU (other streams or threads) race and must be serialised by the caller.
@@ -292,9 +294,12 @@ int spx_seed_neighbours(const float* seeds, int32_t K, int64_t H, int64_t W, flo
  * stream of that fit, then copies on `stream` and synchronises it. */
 int spx_fit_stats(uint64_t* out, spx_stream_t stream);
 
-/* The first n (1..8) counters of the last fit issued on this thread: those of spx_fit_stats,
+/* The first n (1..18) counters of the last fit issued on this thread: those of spx_fit_stats,
  * then out[4] = features whose learned start did not converge within opt->learned_iters and
- * were fitted again from the moment start; out[5..7] = 0. out: host uint64[n]. */
+ * were fitted again from the moment start; out[5] = features with more fitted pixels than
+ * their bucket's bound (0 unless the bound is wrong; their extra pixels are left out);
+ * out[6..7] = 0; out[8 + b] = features in bucket b = 2 c + m (see bucket_mask), b = 0..9.
+ * out: host uint64[n]. */
 int spx_fit_counters(uint64_t* out, int32_t n, spx_stream_t stream);
 
 /* ---- learned initial guess (no reference counterpart; task-defined CNN) ------------------ */

@@ -51,6 +51,7 @@ P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
 rc = lib.emu_fit_batch(P(img), ctypes.c_int64(160), ctypes.c_int64(160), ctypes.c_int64(160), P(seeds),
                        P(half), P(model), None, ctypes.byref(opts), ctypes.c_int32(K), P(params), P(rms),
                        P(iters), P(st), ctypes.c_int(240))
+assert rc == 0, rc  # no patch overflowed its bucket
 ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, seeds, half, model)
 assert (st == ref_st).all(), (st, ref_st)
 ok = st <= 1
@@ -73,6 +74,7 @@ rc = lib.emu_fit_batch_cells(P(img), ctypes.c_int64(160), ctypes.c_int64(160), c
                              P(half_a), P(model_a), None, ctypes.byref(opts), ctypes.c_int32(Ka),
                              P(np.ascontiguousarray(nbr_all)), ctypes.c_int32(8), P(params), P(rms), P(iters), P(st),
                              ctypes.c_int(240))
+assert rc == 0, rc
 ref_p, _, _, ref_st = erfdisk.fit_batch(arr.image, allseeds, half_a, model_a, neighbours=nbr_all)
 assert (st[only] == ref_st[only]).all(), (st[only], ref_st[only])
 ok = only & (st <= 1)
@@ -93,6 +95,7 @@ iters = np.zeros(K, np.int16); st = np.zeros(K, np.uint8)
 rc = lib.emu_fit_batch(P(img), ctypes.c_int64(160), ctypes.c_int64(160), ctypes.c_int64(160), P(seeds),
                        P(half), P(model), P(init), ctypes.byref(bopts), ctypes.c_int32(K), P(params), P(rms),
                        P(iters), P(st), ctypes.c_int(240))
+assert rc == 0, rc
 ref_p, _, ref_it, ref_st = erfdisk.fit_batch(arr.image, seeds, half, model, init=init, learned_iters=2)
 assert (st == ref_st).all(), (st, ref_st)
 assert (iters == ref_it).all(), (iters, ref_it)
@@ -153,3 +156,42 @@ def test_fp32_cell_first_look_takes_the_exact_decisions(integer_seeds):
         assert np.array_equal(outs[1].astype(bool), erfdisk.cell_mask(qx, qy, seeds, int(f), row))
         assert np.array_equal(outs[3].astype(bool), erfdisk.cell_edge(qx, qy, seeds, int(f), row))
     assert mism == 0
+
+
+def test_pixel_bound_covers_every_fitted_pixel():
+    """The bucketing's bound of a feature's fitted pixels (patch_pixel_bound) is never below
+    the pixels the oracle's patch keeps (inside the image and the Voronoi cell) and at most two
+    columns per row above them, on a page with integer seeds (exact ties), fractional seeds,
+    patches past every border and features without neighbours."""
+    from imageprocessor_new_amd import synthetic
+    from oracle import erfdisk
+
+    _build()
+    lib = ctypes.CDLL(str(OUT))
+    rng = np.random.default_rng(5)
+    H, W = 150, 170
+    base = synthetic.make_workload("hex16384", seed=4, H=H, W=W).centers_xy
+    for frac in (False, True):
+        seeds = np.rint(base) if not frac else base + rng.uniform(-0.5, 0.5, base.shape)
+        seeds = np.concatenate([seeds, [[1.0, 2.0], [168.7, 149.2], [-3.0, 60.0], [85.0, 160.0]]]).astype(np.float32)
+        K = seeds.shape[0]
+        half = rng.integers(2, 32, size=K).astype(np.int16)
+        nbr = erfdisk.seed_neighbours(seeds, 2 * np.sqrt(2) * 32.5 + 1e-3, 8).astype(np.int32)
+        nbr[::7] = -1  # some features without neighbours
+        bound = np.zeros(K, np.int32)
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        lib.emu_pixel_bound(P(seeds), P(half), P(np.ascontiguousarray(nbr)), ctypes.c_int32(8), ctypes.c_int32(K),
+                            ctypes.c_int64(H), ctypes.c_int64(W), P(bound))
+        img = np.zeros((H, W), np.uint16)
+        for f in range(K):
+            h = int(half[f])
+            _, valid, _, _, cx, cy = erfdisk.extract_patch(img, seeds[f], h)
+            if (nbr[f] >= 0).any():
+                side = 2 * h + 1
+                gy, gx = np.meshgrid(np.arange(side) + (cy - h), np.arange(side) + (cx - h), indexing="ij")
+                valid = valid & erfdisk.cell_mask(gx, gy, seeds.astype(np.float64), f, nbr[f])
+            exact = int(valid.sum())
+            assert bound[f] >= exact, (f, bound[f], exact)
+            assert bound[f] <= exact + 2 * (2 * h + 1), (f, bound[f], exact)
+            if not (nbr[f] >= 0).any():
+                assert bound[f] == exact

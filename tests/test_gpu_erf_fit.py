@@ -210,16 +210,24 @@ def test_bucket_mask_skips_only_the_masked_classes():
     md = arr.ellipse.astype(np.uint8)
     full = _lib.bucket_mask_of(hs, md)
     assert full != 0
+    # a feature's bucket: the patch class of its in-image patch pixels (no neighbours here)
+    seeds_np = np.rint(arr.centers_xy).astype(np.int64)
+    side = 2 * hs.astype(np.int64) + 1
+    lo = seeds_np - hs[:, None]
+    cols = np.clip(lo[:, 0] + side, 0, 600) - np.clip(lo[:, 0], 0, 600)
+    rows = np.clip(lo[:, 1] + side, 0, 600) - np.clip(lo[:, 1], 0, 600)
+    bucket = np.array([2 * _lib.patch_class(int(n)) + int(m) for n, m in zip(cols * rows, md)])
+    assert np.all(((full >> bucket) & 1) == 1)
+    low = 1 << int(bucket.min())  # only the lowest occupied bucket
+    inside = bucket == int(bucket.min())
     with gpu_device(0) as dev:
         args = (torch.from_numpy(arr.image).to(dev), torch.from_numpy(np.rint(arr.centers_xy).astype(np.float32)).to(dev),
                 torch.from_numpy(hs).to(dev), torch.from_numpy(md).to(dev))
         ref = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(*args).items()}
         got = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(*args, options=_lib.fit_options(bucket_mask=full)).items()}
-        low = full & -full  # only the lowest present bucket
         part = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(*args, options=_lib.fit_options(bucket_mask=low)).items()}
     for k in ref:
         np.testing.assert_array_equal(got[k], ref[k])
-    inside = np.array([_lib.bucket_mask_of([h], [m]) == low for h, m in zip(hs, md)])
     assert inside.any() and (~inside).any()
     for k in ref:
         np.testing.assert_array_equal(part[k][inside], ref[k][inside])

@@ -20,6 +20,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _buckets(half, model):
+    """(patch class of the whole patch, model): the strata of the oracle sample."""
     npl = ((2 * half.astype(np.int64) + 1) ** 2 + 63) // 64
     cls = np.select([npl <= 4, npl <= 8, npl <= 16, npl <= 32], [0, 1, 2, 3], 4)
     return 2 * cls + (model == 1)
@@ -38,8 +39,27 @@ def uneven():
         out = fit_erf_disks_device(page.image, f.seeds, f.half, f.model, neighbours=f.nbr)
         torch.cuda.synchronize()
         res = {k: v.cpu().numpy() for k, v in out.items()}
+        res["counters"] = _counters()
         host = {k: getattr(f, k).cpu().numpy() for k in ("seeds", "half", "model", "nbr")}
     return page, host, res
+
+
+def _counters():
+    from imageprocessor_new_amd import _lib
+    from imageprocessor_new_amd.backends import stream_handle
+
+    c = np.zeros(8, dtype=np.uint64)
+    _lib.call("spx_fit_counters", int(c.ctypes.data), 8, stream_handle())
+    return c
+
+
+def test_uneven_page_patches_fit_their_buckets(uneven):
+    """Every feature's fitted pixels fit the bucket its pixel bound chose (the overflow counter
+    stays 0), and the counted pixel evaluations are those of the compacted patches."""
+    _, f, res = uneven
+    c = res["counters"]
+    assert int(c[5]) == 0
+    assert int(c[2]) > 0 and int(c[0]) + int(c[1]) > 0
 
 
 def _refit(img_t, seeds, half, model):

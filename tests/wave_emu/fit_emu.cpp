@@ -116,6 +116,25 @@ inline float spx_wave_max(float v) {
     for (int off = 32; off >= 1; off >>= 1) v = std::max(v, spx_shfl_xor(v, off));
     return spx_uniform(v);
 }
+inline uint64_t spx_ballot(bool p) {
+    WaveState& w = *t_wave;
+    w.shuffles[t_lane]++;
+    w.iv[t_lane] = p ? 1 : 0;
+    w.bar.arrive_and_wait();
+    uint64_t m = 0;
+    for (int l = 0; l < 64; ++l) m |= (uint64_t)(w.iv[l] & 1) << l;
+    w.bar.arrive_and_wait();
+    return m;
+}
+inline int spx_lanes_below(uint64_t m) {
+    return t_lane == 0 ? 0 : __builtin_popcountll(m & ((1ull << t_lane) - 1ull));
+}
+// every lane's slab writes before the barrier are seen by every lane after it
+inline void spx_wave_lds_sync() {
+    WaveState& w = *t_wave;
+    w.shuffles[t_lane]++;
+    w.bar.arrive_and_wait();
+}
 inline int spx_queue_grab(int32_t* q, int n) {
     const int mine = t_lane == 0 ? __atomic_fetch_add(q, n, __ATOMIC_SEQ_CST) : 0;
     return spx_shfl(mine, 0);
@@ -181,7 +200,8 @@ extern "C" int emu_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, in
             status[f] = SPX_FIT_TOO_FEW;
             continue;
         }
-        order[2 * bucket_of(hs) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0)].push_back(f);
+        const int bound = patch_pixel_bound(seeds, half, nbr, nb, K, H, W, f);
+        order[2 * bucket_of_pixels(bound) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0)].push_back(f);
     }
     unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = 0; b < kKB; ++b) {
@@ -203,7 +223,13 @@ extern "C" int emu_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, in
             default: run_wave<64, 7>(a, timeout_s); break;
         }
     }
-    return 0;
+    return (int)stats[5];  // features with more pixels than their bucket holds
+}
+
+// The bucketing's pixel bound of every feature (patch_pixel_bound, plain per-thread code).
+extern "C" void emu_pixel_bound(const float* seeds, const int16_t* half, const int32_t* nbr, int32_t nb, int32_t K,
+                                int64_t H, int64_t W, int32_t* out) {
+    for (int f = 0; f < K; ++f) out[f] = patch_pixel_bound(seeds, half, nbr, nb, K, H, W, f);
 }
 
 extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
