@@ -244,20 +244,45 @@ __device__ __forceinline__ void prep_patch(const uint16_t* img, int64_t H, int64
                                            float cy, int hs, int lane, uint16_t* P, uint16_t* xdst, int slot,
                                            float& b0, float& a0) {
     const int side = 2 * hs + 1;
+    const int npx = side * side;
     const int64_t ox = (int64_t)cx - hs, oy = (int64_t)cy - hs;
     float rsum = 0.0f, rcnt = 0.0f, vmax = -3.0e38f;
-    for (int p = lane; p < side * side; p += 64) {
-        const int row = p / side, col = p - (p / side) * side;
-        const int64_t gx = min(max(ox + col, (int64_t)0), W - 1);
-        const int64_t gy = min(max(oy + row, (int64_t)0), H - 1);
-        const uint16_t raw = img[gy * pitch + gx];
-        const float v = (float)raw;
-        P[p] = raw;
-        if (row == 0 || row == side - 1 || col == 0 || col == side - 1) {
-            rsum += v;
-            rcnt += 1.0f;
+    // Lane-strided over the flattened patch, kB loads issued back to back before their values
+    // are used (every address clamped into the image, so the loads carry no branch); row and
+    // column advance by the quotient and remainder of 64 / side.
+    constexpr int kB = 8;
+    const int dq = 64 / side, dr = 64 - dq * side;
+    int row = lane / side, col = lane - row * side;
+    for (int p0 = 0; p0 < npx; p0 += 64 * kB) {
+        uint16_t raw[kB];
+        int rr[kB], cc[kB];
+#pragma unroll
+        for (int j = 0; j < kB; ++j) {
+            const int64_t gx = min(max(ox + col, (int64_t)0), W - 1);
+            const int64_t gy = min(max(oy + row, (int64_t)0), H - 1);
+            raw[j] = img[gy * pitch + gx];
+            rr[j] = row;
+            cc[j] = col;
+            col += dr;
+            row += dq;
+            if (col >= side) {
+                col -= side;
+                ++row;
+            }
         }
-        vmax = fmaxf(vmax, v);
+#pragma unroll
+        for (int j = 0; j < kB; ++j) {
+            const int p = p0 + 64 * j + lane;
+            if (p < npx) {
+                const float v = (float)raw[j];
+                P[p] = raw[j];
+                if (rr[j] == 0 || rr[j] == side - 1 || cc[j] == 0 || cc[j] == side - 1) {
+                    rsum += v;
+                    rcnt += 1.0f;
+                }
+                vmax = fmaxf(vmax, v);
+            }
+        }
     }
     for (int off = 32; off >= 1; off >>= 1) {
         rsum += __shfl_xor(rsum, off, 64);
