@@ -97,6 +97,14 @@ SPX_DEV_INLINE int spx_queue_grab(int32_t* q, int n) {
 SPX_DEV_INLINE unsigned long long spx_atomic_add(unsigned long long* p, unsigned long long v) {
     return atomicAdd(p, v);
 }
+SPX_DEV_INLINE int spx_shfl(int v, int src) { return __shfl(v, src, 64); }
+SPX_DEV_INLINE int spx_readlane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+SPX_DEV_INLINE double spx_readlane_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 SPX_DEV_INLINE uint64_t spx_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 SPX_DEV_INLINE int spx_lanes_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -166,32 +174,45 @@ __device__ inline bool feature_valid(const int16_t* half, const uint8_t* model, 
     return hs >= 2 && hs <= kMaxHalf && (md == SPX_MODEL_ERF_DISK6 || md == SPX_MODEL_ERF_ELLIPSE7);
 }
 
-// Pass 1 of the bucketing: every feature's pixel bound (kept in its iters entry until the
-// fit writes it; -1 for a feature that is not fitted), the per-bucket feature counts (LDS
-// counters, then one global atomic per bucket and workgroup), and the outputs of features
-// that are not fitted: invalid half side or model, or a bucket the mask leaves out (status
-// SPX_FIT_TOO_FEW and NaNs).
-__global__ __launch_bounds__(256) void bucket_count_kernel(FitArgs a, int32_t* counts) {
+// Pass 1 of the bucketing, one wave per feature: a bound of the feature's fitted pixels (lane
+// r bounds patch row r; kept in its iters entry until the fit writes it; -1 for a feature that
+// is not fitted), the per-bucket feature counts (LDS counters, then one global atomic per
+// bucket and workgroup), and the outputs of features that are not fitted: invalid half side
+// or model, or a bucket the mask leaves out (status SPX_FIT_TOO_FEW and NaNs).
+constexpr int kCountWaves = 4;
+constexpr int kCountBlocks = 2048;
+
+__global__ __launch_bounds__(64 * kCountWaves) void bucket_count_kernel(FitArgs a, int32_t* counts) {
     // counts[b] steers the launch; stats[8 + b] keeps a copy for the caller
     __shared__ int32_t s_cnt[kKernBuckets];
     if (threadIdx.x < kKernBuckets) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < a.nfeat) {
+    const int lane = threadIdx.x & 63;
+    // a grid of at most kCountBlocks workgroups strides over the features, so the per-bucket
+    // totals take few global atomics on the same ten addresses
+    for (int f = blockIdx.x * kCountWaves + (threadIdx.x >> 6); f < a.nfeat; f += gridDim.x * kCountWaves) {
         int b = -1;
         if (feature_valid(a.half, a.model, f)) {
-            const int bound = patch_pixel_bound(a.seeds, a.half, a.nbr, a.nb, a.nfeat, a.H, a.W, f);
-            b = feature_bucket(a.model[f], bound, a.opt.bucket_mask);
-            a.iters[f] = (int16_t)(b < 0 ? -1 : bound);
+            const PatchSpan p = patch_span(a, f);
+            int cnt = p.side * p.side;
+            if (!p.whole) {
+                cnt = patch_row_upper(lane_terms(a, f), p, lane);
+                if (lane >= p.side) cnt = 0;
+                for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+            }
+            b = feature_bucket(a.model[f], cnt, a.opt.bucket_mask);
+            if (lane == 0) a.iters[f] = (int16_t)(b < 0 ? -1 : cnt);
         }
-        if (b < 0) {
-            for (int i = 0; i < 7; ++i) a.params[(int64_t)f * 7 + i] = __builtin_nanf("");
-            if (a.centers != nullptr) a.centers[2 * (int64_t)f] = a.centers[2 * (int64_t)f + 1] = __builtin_nan("");
-            a.rms[f] = __builtin_nanf("");
-            a.iters[f] = -1;
-            a.status[f] = SPX_FIT_TOO_FEW;
-        } else {
-            atomicAdd(&s_cnt[b], 1);
+        if (lane == 0) {
+            if (b < 0) {
+                for (int i = 0; i < 7; ++i) a.params[(int64_t)f * 7 + i] = __builtin_nanf("");
+                if (a.centers != nullptr) a.centers[2 * (int64_t)f] = a.centers[2 * (int64_t)f + 1] = __builtin_nan("");
+                a.rms[f] = __builtin_nanf("");
+                a.iters[f] = -1;
+                a.status[f] = SPX_FIT_TOO_FEW;
+            } else {
+                atomicAdd(&s_cnt[b], 1);
+            }
         }
     }
     __syncthreads();
@@ -320,7 +341,9 @@ int fit_batch_enqueue(void* scratch, unsigned long long* stats, const uint16_t* 
     FitArgs a{img, H, W, pitch, seeds, half, model, init, opt, order, nullptr, nullptr,
               stats, params, centers, rms, iters, status, nbr, nb, K, counts, 0};
     const int blocks256 = (K + 255) / 256;
-    bucket_count_kernel<<<blocks256, 256, 0, stream>>>(a, counts);
+    const int count_blocks = (K + kCountWaves - 1) / kCountWaves < kCountBlocks ? (K + kCountWaves - 1) / kCountWaves
+                                                                               : kCountBlocks;
+    bucket_count_kernel<<<count_blocks, 64 * kCountWaves, 0, stream>>>(a, counts);
     SPX_LAUNCHED();
     bucket_scatter_kernel<<<blocks256, 256, 0, stream>>>(a, counts, cursors, order);
     SPX_LAUNCHED();

@@ -8,6 +8,8 @@
 //   spx_atomic_add(unsigned long long*, unsigned long long),
 //   spx_pair_sum32 / spx_pair_sum16 / spx_pair_dpp<CTRL, BIT> / spx_self_dpp<CTRL> /
 //   spx_readlane: the butterfly steps of wave_sum_many,
+//   spx_shfl(int v, int src): lane src's v, spx_readlane_i / spx_readlane_d(v, lane): lane's v
+//   as a uniform value,
 //   spx_ballot(bool): the wave's 64-bit predicate mask (uniform), spx_lanes_below(mask): set
 //   bits of the mask in lanes below this one, spx_wave_lds_sync(): LDS writes of every lane
 //   visible to every lane of the wave,
@@ -30,7 +32,7 @@
 //   * the patch is gathered once from HBM and compacted to the pixels the fit uses (inside
 //     the image and the cell), one 32-bit word per pixel (column, row and grey level
 //     packed), at most NPL words per lane, held in VGPRs (or LDS for large patches); the
-//     feature's bucket (NPL) comes from an upper bound of that pixel count;
+//     feature's bucket (NPL) comes from that pixel count, found before the fit;
 //   * residual and Jacobian are FP32 on the VALU, with hardware rsq / rcp / exp2;
 //   * J^T J, J^T r and the cost are reduced over the wave with DPP (no LDS, no atomics),
 //     so a result does not depend on scheduling;
@@ -369,17 +371,44 @@ struct FitArgs {
 // to the lower feature index (reference subpx/_cpu/cells.py:8-33). The test uses the linear
 // form f_j(q) = |q - s_j|^2 - |q - s0|^2 = (a_j qx + b_j qy) + c_j with a_j = 2 (s0x - s_jx),
 // b_j = 2 (s0y - s_jy), c_j = (s_jx^2 + s_jy^2) - (s0x^2 + s0y^2), in FP64 without contraction
-// and in the oracle's operation order, so both take the same decisions. A pixel of the cell is
-// on its boundary (a 4-neighbour lies outside) when some g_j = f_j(q) - max(|a_j|, |b_j|) is
-// negative, or zero towards a lower index: the 4-neighbours shift f_j by +-a_j or +-b_j.
-// Absent neighbours get a = b = 0, c = 1e300 (never decisive).
+// and in the oracle's operation order, so both take the same decisions: inside when every
+// f_j > 0 (or = 0 towards a higher index). A pixel of the cell is on its boundary (a
+// 4-neighbour lies outside) when some g_j = f_j(q) - max(|a_j|, |b_j|) is negative, or zero
+// towards a lower index: the 4-neighbours shift f_j by +-a_j or +-b_j. Absent neighbours
+// (id < 0 or >= K) never decide; only the first kMaxNb listed neighbours count.
 constexpr int kMaxNb = 8;
 
-// The exact rules in FP64 without contraction and in the oracle's operation order, from the
-// seeds in memory: a_j = 2 (s0x - s_jx), b_j = 2 (s0y - s_jy), c_j = (s_jx^2 + s_jy^2) -
-// (s0x^2 + s0y^2), f_j = (a_j qx + b_j qy) + c_j; inside when every f_j > 0 (or = 0 towards a
-// higher index); on the boundary when some g_j = f_j - max(|a_j|, |b_j|) < 0 (or = 0 towards a
-// lower index). Absent neighbours (id < 0 or >= K) never decide.
+struct CellTerm {
+    double a, b, c, m;  // m = max(|a|, |b|)
+    double inv;         // 1 / a (0 when a = 0): only to estimate where a rule changes
+    bool lower;         // the neighbour's index is below the feature's
+};
+
+SPX_DEV_INLINE CellTerm cell_term(double x0, double y0, double n0, double xj, double yj, bool lower) {
+#pragma clang fp contract(off)
+    CellTerm t;
+    t.a = 2.0 * (x0 - xj);
+    t.b = 2.0 * (y0 - yj);
+    t.c = (xj * xj + yj * yj) - n0;
+    t.m = fmax(fabs(t.a), fabs(t.b));
+    t.inv = t.a != 0.0 ? 1.0 / t.a : 0.0;
+    t.lower = lower;
+    return t;
+}
+
+// f_j at (qx, qy) given by = b_j qy, in the operation order above.
+SPX_DEV_INLINE double cell_f(const CellTerm& t, double qx, double by) {
+#pragma clang fp contract(off)
+    return (t.a * qx + by) + t.c;
+}
+SPX_DEV_INLINE bool cell_keeps(const CellTerm& t, double fj) { return fj > 0.0 || (fj == 0.0 && !t.lower); }
+SPX_DEV_INLINE bool cell_interior(const CellTerm& t, double fj) {
+#pragma clang fp contract(off)
+    const double g = fj - t.m;
+    return !(g < 0.0 || (g == 0.0 && t.lower));
+}
+
+// The exact rules at one pixel, from the seeds in memory (edge: the boundary flag).
 SPX_DEV_INLINE bool cell_exact(const FitArgs& a, int f, double qx, double qy, bool edge) {
 #pragma clang fp contract(off)
     const double x0 = (double)a.seeds[2 * (int64_t)f], y0 = (double)a.seeds[2 * (int64_t)f + 1];
@@ -388,196 +417,234 @@ SPX_DEV_INLINE bool cell_exact(const FitArgs& a, int f, double qx, double qy, bo
     for (int j = 0; j < a.nb && j < kMaxNb; ++j) {
         const int id = a.nbr[(int64_t)f * a.nb + j];
         if (id < 0 || id >= a.nfeat) continue;
-        const double xj = (double)a.seeds[2 * (int64_t)id], yj = (double)a.seeds[2 * (int64_t)id + 1];
-        const double aj = 2.0 * (x0 - xj), bj = 2.0 * (y0 - yj);
-        const double fj = (aj * qx + bj * qy) + ((xj * xj + yj * yj) - n0);
-        const bool lower = id < f;
-        in = in && (fj > 0.0 || (fj == 0.0 && !lower));
-        const double gj = fj - fmax(fabs(aj), fabs(bj));
-        e = e || gj < 0.0 || (gj == 0.0 && lower);
+        const CellTerm t = cell_term(x0, y0, n0, (double)a.seeds[2 * (int64_t)id],
+                                     (double)a.seeds[2 * (int64_t)id + 1], id < f);
+        const double fj = cell_f(t, qx, t.b * qy);
+        in = in && cell_keeps(t, fj);
+        e = e || !cell_interior(t, fj);
     }
     return edge ? e : in;
 }
 
-// A first look in FP32 with the same forms centred on the seed: f_j = u_j x' + v_j y' + w_j
-// with (x', y') = q - s0, u_j = a_j, v_j = b_j, w_j = |s_j - s0|^2 (g_j subtracts
-// max(|u_j|, |v_j|)). Their rounding error is below `margin`; a pixel whose every |f_j| (or
-// |g_j|) exceeds it takes the FP32 decision (the FP64 form, off the exact value by ~1e-6 there,
-// has the same sign); any other pixel takes the exact rule above.
-struct CellForm {
-    float u[kMaxNb], v[kMaxNb], w[kMaxNb], m[kMaxNb];  // m_j = max(|u_j|, |v_j|)
-    float x0, y0, margin;
-    uint32_t act;  // bit j: neighbour j present
+// Along one row every rule is monotone in the column: f_j is a chain of FP64 operations each
+// monotone in qx (non-decreasing for a_j > 0, non-increasing for a_j < 0, constant for 0),
+// so the columns a neighbour keeps (or leaves interior) are a prefix or a suffix of the row.
+// Its end is found from the real root of f_j (or g_j) and then settled by evaluating the
+// exact rule at the columns next to it, moving at most side steps, so the result is exact
+// whatever the root's rounding. The cell's part of the row is then the intersection of the
+// neighbours' intervals: members in [lo, hi], interior members in [elo, ehi] (empty when lo >
+// hi); the columns are patch-local, 0 .. side - 1, at qx = ox + column.
+struct CellRow {
+    int lo, hi, elo, ehi;
 };
 
-// One neighbour's FP32 form and its rounding margin (inactive: all zero, margin 1e-3).
-SPX_DEV_INLINE void cell_term(float x0, float y0, float xs, float ys, bool act, float& u, float& v, float& w,
-                              float& m, double& margin) {
-    const double xj = act ? (double)xs : (double)x0, yj = act ? (double)ys : (double)y0;
-    const double aj = 2.0 * ((double)x0 - xj), bj = 2.0 * ((double)y0 - yj);
-    const double dx = xj - (double)x0, dy = yj - (double)y0;
-    u = (float)aj;
-    v = (float)bj;
-    w = (float)(dx * dx + dy * dy);
-    m = fmaxf(fabsf(u), fabsf(v));
-    // |x'|, |y'| <= 64 for any patch; FP32 rounding of u, v, w, x', y' and of the two FMAs
-    const double mj = 1e-5 * (fabs(aj) + fabs(bj) + 1.0) * 64.0 + 1e-6 * (dx * dx + dy * dy) + 1e-3;
-    margin = act ? mj : 1e-3;
+SPX_DEV_INLINE int imax(int a, int b) { return a > b ? a : b; }
+SPX_DEV_INLINE int imin(int a, int b) { return a < b ? a : b; }
+
+// first column in [0, side) where the monotone (false, then true) predicate holds, else side
+template <class Pred>
+SPX_DEV_INLINE int first_true(const Pred& p, double est, int side) {
+    int c = (est != est) ? 0 : (int)fmin(fmax(ceil(est), 0.0), (double)side);
+    for (int k = 0; k <= side && c > 0 && p(c - 1); ++k) --c;
+    for (int k = 0; k <= side && c < side && !p(c); ++k) ++c;
+    return c;
+}
+// last column in [0, side) where the monotone (true, then false) predicate holds, else -1
+template <class Pred>
+SPX_DEV_INLINE int last_true(const Pred& p, double est, int side) {
+    int c = (est != est) ? side - 1 : (int)fmin(fmax(floor(est), -1.0), (double)(side - 1));
+    for (int k = 0; k <= side && c < side - 1 && p(c + 1); ++k) ++c;
+    for (int k = 0; k <= side && c >= 0 && !p(c); ++k) --c;
+    return c;
 }
 
-SPX_DEV_INLINE void make_cell(float x0, float y0, const float (&xs)[kMaxNb], const float (&ys)[kMaxNb],
-                              const bool (&active)[kMaxNb], CellForm& cf) {
-    cf.x0 = x0;
-    cf.y0 = y0;
-    cf.act = 0u;
-    double mg = 1e-3;
-#pragma unroll
-    for (int j = 0; j < kMaxNb; ++j) {
-        cf.act |= active[j] ? (1u << j) : 0u;
-        double mj;
-        cell_term(x0, y0, xs[j], ys[j], active[j], cf.u[j], cf.v[j], cf.w[j], cf.m[j], mj);
-        mg = fmax(mg, mj);
+// The neighbour terms of feature f, term j read from memory (plain per-thread code).
+struct MemTerms {
+    const FitArgs& a;
+    int f;
+    double x0, y0, n0;
+    SPX_DEV_INLINE MemTerms(const FitArgs& args, int feat) : a(args), f(feat) {
+#pragma clang fp contract(off)
+        x0 = (double)a.seeds[2 * (int64_t)f];
+        y0 = (double)a.seeds[2 * (int64_t)f + 1];
+        n0 = x0 * x0 + y0 * y0;
     }
-    cf.margin = (float)mg;
-}
+    SPX_DEV_INLINE bool get(int j, CellTerm& t) const {
+        if (j >= a.nb) return false;
+        const int id = a.nbr[(int64_t)f * a.nb + j];
+        if (id < 0 || id >= a.nfeat) return false;
+        t = cell_term(x0, y0, n0, (double)a.seeds[2 * (int64_t)id], (double)a.seeds[2 * (int64_t)id + 1], id < f);
+        return true;
+    }
+};
 
-// The cell form of feature f built lane-parallel: lane j < 8 takes neighbour j (the same
-// terms as make_cell), and the wave reads them back as uniform values.
-SPX_DEV_INLINE void load_cell(const FitArgs& a, int f, CellForm& cf) {
+// The same terms spread over the wave: lane j < kMaxNb holds neighbour j's term, which every
+// lane reads back with readlane (one pair of loads per lane instead of eight in turn).
+struct LaneTerms {
+    CellTerm mine;
+    uint32_t act;  // bit j: neighbour j present
+    SPX_DEV_INLINE bool get(int j, CellTerm& t) const {
+        if (!((act >> j) & 1u)) return false;
+        t.a = spx_readlane_d(mine.a, j);
+        t.b = spx_readlane_d(mine.b, j);
+        t.c = spx_readlane_d(mine.c, j);
+        t.m = spx_readlane_d(mine.m, j);
+        t.inv = spx_readlane_d(mine.inv, j);
+        t.lower = spx_readlane_i(mine.lower ? 1 : 0, j) != 0;
+        return true;
+    }
+};
+
+SPX_DEV_INLINE LaneTerms lane_terms(const FitArgs& a, int f) {
+#pragma clang fp contract(off)
     const int lane = spx_lane();
-    const float x0 = spx_uniform(a.seeds[2 * (int64_t)f]);
-    const float y0 = spx_uniform(a.seeds[2 * (int64_t)f + 1]);
+    const double x0 = (double)spx_uniform(a.seeds[2 * (int64_t)f]);
+    const double y0 = (double)spx_uniform(a.seeds[2 * (int64_t)f + 1]);
+    const double n0 = x0 * x0 + y0 * y0;
     int id = -1;
     if (lane < a.nb && lane < kMaxNb) id = a.nbr[(int64_t)f * a.nb + lane];
     const bool act = id >= 0 && id < a.nfeat;
-    float xs = 0.0f, ys = 0.0f;
+    double xj = x0, yj = y0;
     if (act) {
-        xs = a.seeds[2 * (int64_t)id];
-        ys = a.seeds[2 * (int64_t)id + 1];
+        xj = (double)a.seeds[2 * (int64_t)id];
+        yj = (double)a.seeds[2 * (int64_t)id + 1];
     }
-    float u, v, w, m;
-    double mj;
-    cell_term(x0, y0, xs, ys, act, u, v, w, m, mj);
-    cf.x0 = x0;
-    cf.y0 = y0;
-    cf.act = (uint32_t)(spx_ballot(act) & 0xffull);
-    // rounding to FP32 is monotonic, so the largest rounded margin is the largest margin rounded
-    cf.margin = spx_wave_max((float)mj);
+    LaneTerms L;
+    L.mine = cell_term(x0, y0, n0, xj, yj, act && id < f);
+    L.act = (uint32_t)(spx_ballot(act) & 0xffull);
+    return L;
+}
+
+template <class Terms>
+SPX_DEV_INLINE CellRow cell_row(const Terms& terms, int64_t ox, int64_t oy, int side, int row) {
+#pragma clang fp contract(off)
+    CellRow r{0, side - 1, 0, side - 1};
+    const double qy = (double)(oy + row);
+    const double oxd = (double)ox;
 #pragma unroll
     for (int j = 0; j < kMaxNb; ++j) {
-        cf.u[j] = spx_readlane(u, j);
-        cf.v[j] = spx_readlane(v, j);
-        cf.w[j] = spx_readlane(w, j);
-        cf.m[j] = spx_readlane(m, j);
-    }
-}
-
-// Cell membership and boundary flags of the pixel q = (qx, qy), at the FP32 offset (px, py) =
-// q - seed (rounded once from the exact difference): the FP32 first look, and the exact rule
-// wherever a form lies within the margin. The boundary flag is decided for member pixels only.
-SPX_DEV_INLINE void cell_flags(const FitArgs& a, int f, const CellForm& cf, float px, float py, double qx,
-                               double qy, bool& in, bool& edge) {
-    bool i = true, e = false, sure_i = true, sure_e = true;
-#pragma unroll
-    for (int j = 0; j < kMaxNb; ++j) {
-        if (!((cf.act >> j) & 1u)) continue;
-        const float fj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j]));
-        const float gj = fj - cf.m[j];
-        i = i && fj > 0.0f;
-        sure_i = sure_i && fabsf(fj) > cf.margin;
-        e = e || gj < 0.0f;
-        sure_e = sure_e && fabsf(gj) > cf.margin;
-    }
-    if (!sure_i) i = cell_exact(a, f, qx, qy, false);
-    if (i && !sure_e) e = cell_exact(a, f, qx, qy, true);
-    in = i;
-    edge = i && e;
-}
-
-// Cell membership alone (cell_flags without the boundary flag).
-SPX_DEV_INLINE bool cell_in(const FitArgs& a, int f, const CellForm& cf, float px, float py, double qx, double qy) {
-    bool i = true, sure = true;
-#pragma unroll
-    for (int j = 0; j < kMaxNb; ++j) {
-        if (!((cf.act >> j) & 1u)) continue;
-        const float fj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j]));
-        i = i && fj > 0.0f;
-        sure = sure && fabsf(fj) > cf.margin;
-    }
-    if (!sure) i = cell_exact(a, f, qx, qy, false);
-    return i;
-}
-
-SPX_DEV_INLINE bool cell_in_fast(const FitArgs& a, int f, const CellForm& cf, double qx, double qy) {
-    return cell_in(a, f, cf, (float)(qx - (double)cf.x0), (float)(qy - (double)cf.y0), qx, qy);
-}
-
-SPX_DEV_INLINE bool cell_edge_fast(const FitArgs& a, int f, const CellForm& cf, double qx, double qy) {
-    const float px = (float)(qx - (double)cf.x0), py = (float)(qy - (double)cf.y0);
-    bool e = false, sure = true;
-#pragma unroll
-    for (int j = 0; j < kMaxNb; ++j) {
-        if (!((cf.act >> j) & 1u)) continue;
-        const float gj = fmaf(cf.u[j], px, fmaf(cf.v[j], py, cf.w[j])) - cf.m[j];
-        e = e || gj < 0.0f;
-        sure = sure && fabsf(gj) > cf.margin;
-    }
-    if (!sure) e = cell_exact(a, f, qx, qy, true);
-    return e;
-}
-
-// Upper bound of the pixels the fit of feature f uses: those of its patch inside the image
-// and, with a neighbour list, inside its cell. Row by row each neighbour's rule f_j > 0 bounds
-// the column (f_j = a_j qx + (b_j qy + c_j) is linear in it); the bound is relaxed by far more
-// than the FP64 rounding of f_j and of the FP32 first look, so every pixel the exact rule
-// keeps is counted. It chooses the feature's bucket before the fit. Plain per-thread code.
-// A seed too far out for an integer patch origin gets the whole patch as its bound.
-SPX_DEV_INLINE int patch_pixel_bound(const float* seeds, const int16_t* half, const int32_t* nbr, int nb,
-                                     int nfeat, int64_t H, int64_t W, int f) {
-    const int hs = half[f];
-    const int side = 2 * hs + 1;
-    const float sx = seeds[2 * (int64_t)f], sy = seeds[2 * (int64_t)f + 1];
-    if (!(fabsf(sx) < 1.0e15f) || !(fabsf(sy) < 1.0e15f)) return side * side;
-    const int64_t ox = (int64_t)rintf(sx) - hs, oy = (int64_t)rintf(sy) - hs;
-    const int64_t c0 = ox < 0 ? -ox : 0, r0 = oy < 0 ? -oy : 0;
-    const int64_t c1 = (W - 1 - ox) < (int64_t)(side - 1) ? (W - 1 - ox) : (int64_t)(side - 1);
-    const int64_t r1 = (H - 1 - oy) < (int64_t)(side - 1) ? (H - 1 - oy) : (int64_t)(side - 1);
-    if (c1 < c0 || r1 < r0) return 0;
-    if (nbr == nullptr || nb <= 0) return (int)((c1 - c0 + 1) * (r1 - r0 + 1));
-    const double x0 = (double)sx, y0 = (double)sy;
-    const double n0 = x0 * x0 + y0 * y0;
-    double ca[kMaxNb], cb[kMaxNb], cc[kMaxNb];
-    int m = 0;
-    for (int j = 0; j < nb && j < kMaxNb; ++j) {
-        const int id = nbr[(int64_t)f * nb + j];
-        if (id < 0 || id >= nfeat) continue;
-        const double xj = (double)seeds[2 * (int64_t)id], yj = (double)seeds[2 * (int64_t)id + 1];
-        const double aj = 2.0 * (x0 - xj), bj = 2.0 * (y0 - yj), cj = (xj * xj + yj * yj) - n0;
-        // a rule with a non-finite coefficient rejects every pixel: leaving it out keeps a bound
-        if (!isfinite(aj) || !isfinite(bj) || !isfinite(cj)) continue;
-        ca[m] = aj;
-        cb[m] = bj;
-        cc[m] = cj;
-        ++m;
-    }
-    const double span = fabs((double)ox) + (double)side;
-    int count = 0;
-    for (int64_t row = r0; row <= r1; ++row) {
-        const double qy = (double)(oy + row);
-        double lo = (double)(ox + c0), hi = (double)(ox + c1);
-        bool empty = false;
-        for (int k = 0; k < m; ++k) {
-            const double g = cb[k] * qy + cc[k];
-            const double e = 1e-9 * (fabs(ca[k]) * span + fabs(cb[k] * qy) + fabs(cc[k])) + 1e-9;
-            // the pixels with a_k qx + g >= -e
-            if (ca[k] > 0.0) lo = fmax(lo, (-g - e) / ca[k]);
-            else if (ca[k] < 0.0) hi = fmin(hi, (-g - e) / ca[k]);
-            else if (g < -e) empty = true;
+        CellTerm t;
+        if (!terms.get(j, t)) continue;
+        const double by = t.b * qy;
+        const auto keeps = [&](int c) { return cell_keeps(t, cell_f(t, oxd + (double)c, by)); };
+        const auto interior = [&](int c) { return cell_interior(t, cell_f(t, oxd + (double)c, by)); };
+        if (t.a > 0.0) {
+            r.lo = imax(r.lo, first_true(keeps, -(by + t.c) * t.inv - oxd, side));
+            r.elo = imax(r.elo, first_true(interior, (t.m - by - t.c) * t.inv - oxd, side));
+        } else if (t.a < 0.0) {
+            r.hi = imin(r.hi, last_true(keeps, -(by + t.c) * t.inv - oxd, side));
+            r.ehi = imin(r.ehi, last_true(interior, (t.m - by - t.c) * t.inv - oxd, side));
+        } else {
+            // a constant rule (a = 0, or not a number): the whole row or none of it
+            if (!keeps(0)) {
+                r.lo = side;
+                r.hi = -1;
+            }
+            if (!interior(0)) {
+                r.elo = side;
+                r.ehi = -1;
+            }
         }
-        if (empty) continue;
-        const double cl = ceil(lo - 1e-6), ch = floor(hi + 1e-6);
-        if (ch >= cl) count += (int)(ch - cl + 1.0);
     }
+    return r;
+}
+
+// Row bounds packed for a shuffle: lo + 1, hi + 1, elo + 1, ehi + 1 in bytes 0..3.
+SPX_DEV_INLINE uint32_t pack_row(const CellRow& r) {
+    const auto b = [](int v) { return (uint32_t)(imin(imax(v, -1), 254) + 1); };
+    return b(r.lo) | (b(r.hi) << 8) | (b(r.elo) << 16) | (b(r.ehi) << 24);
+}
+
+// The pixels the fit of feature f uses: those of its patch inside the image and, with a
+// neighbour list, inside its cell (the exact rows above). An upper bound of their count
+// chooses the feature's bucket before the fit. A seed too far out for an integer patch origin
+// gets the whole patch as its count (also an upper bound: the fit keeps what its gather finds).
+struct PatchSpan {
+    int64_t ox, oy;          // patch origin
+    int64_t c0, c1, r0, r1;  // columns and rows of the patch inside the image
+    int side;
+    bool cells;  // the feature has a listed neighbour
+    bool whole;  // no integer origin: the count is side^2
+};
+
+SPX_DEV_INLINE PatchSpan patch_span(const FitArgs& a, int f) {
+    PatchSpan p{};
+    const int hs = a.half[f];
+    p.side = 2 * hs + 1;
+    const float sx = a.seeds[2 * (int64_t)f], sy = a.seeds[2 * (int64_t)f + 1];
+    p.whole = !(fabsf(sx) < 1.0e15f) || !(fabsf(sy) < 1.0e15f);
+    if (p.whole) return p;
+    p.ox = (int64_t)rintf(sx) - hs;
+    p.oy = (int64_t)rintf(sy) - hs;
+    p.c0 = p.ox < 0 ? -p.ox : 0;
+    p.r0 = p.oy < 0 ? -p.oy : 0;
+    p.c1 = (a.W - 1 - p.ox) < (int64_t)(p.side - 1) ? (a.W - 1 - p.ox) : (int64_t)(p.side - 1);
+    p.r1 = (a.H - 1 - p.oy) < (int64_t)(p.side - 1) ? (a.H - 1 - p.oy) : (int64_t)(p.side - 1);
+    p.cells = false;
+    if (a.nbr != nullptr)
+        for (int j = 0; j < a.nb && j < kMaxNb; ++j) {
+            const int id = a.nbr[(int64_t)f * a.nb + j];
+            p.cells = p.cells || (id >= 0 && id < a.nfeat);
+        }
+    return p;
+}
+
+// Fitted pixels of patch row `row` (0 for a row outside the image). The cell's row is found
+// for every row (the lane-spread terms need every lane of the wave).
+template <class Terms>
+SPX_DEV_INLINE int patch_row_count(const Terms& terms, const PatchSpan& p, int row) {
+    int64_t lo = p.c0, hi = p.c1;
+    if (p.cells) {
+        const CellRow r = cell_row(terms, p.ox, p.oy, p.side, row < p.side ? row : p.side - 1);
+        lo = r.lo > lo ? r.lo : lo;
+        hi = r.hi < hi ? r.hi : hi;
+    }
+    if (row < p.r0 || row > p.r1) return 0;
+    return hi >= lo ? (int)(hi - lo + 1) : 0;
+}
+
+// An upper bound of patch_row_count for the bucketing, from the real roots alone: each
+// neighbour with |a| >= 1e-3 limits the row to its root's column widened by one column (the
+// exact rule's end lies within a column of the root, whose rounding is far below a column);
+// flatter rules are left out. About a tenth of the exact row's work.
+template <class Terms>
+SPX_DEV_INLINE int patch_row_upper(const Terms& terms, const PatchSpan& p, int row) {
+#pragma clang fp contract(off)
+    double lo = (double)p.c0, hi = (double)p.c1;
+    if (p.cells) {
+        const double qy = (double)(p.oy + (row < p.side ? row : p.side - 1));
+        const double oxd = (double)p.ox;
+#pragma unroll
+        for (int j = 0; j < kMaxNb; ++j) {
+            CellTerm t;
+            if (!terms.get(j, t)) continue;
+            if (!(fabs(t.a) >= 1e-3)) continue;
+            const double root = -(t.b * qy + t.c) * t.inv - oxd;
+            if (t.a > 0.0) lo = fmax(lo, floor(root) - 1.0);
+            else hi = fmin(hi, ceil(root) + 1.0);
+        }
+    }
+    if (row < p.r0 || row > p.r1 || !(hi >= lo)) return 0;
+    return (int)(hi - lo + 1.0);
+}
+
+// The exact count by one thread (rows in turn).
+SPX_DEV_INLINE int patch_pixel_count(const FitArgs& a, int f) {
+    const PatchSpan p = patch_span(a, f);
+    if (p.whole) return p.side * p.side;
+    const MemTerms terms(a, f);
+    int count = 0;
+    for (int row = 0; row < p.side; ++row) count += patch_row_count(terms, p, row);
+    return count;
+}
+
+// The bucketing's bound by one thread.
+SPX_DEV_INLINE int patch_pixel_bound(const FitArgs& a, int f) {
+    const PatchSpan p = patch_span(a, f);
+    if (p.whole) return p.side * p.side;
+    const MemTerms terms(a, f);
+    int count = 0;
+    for (int row = 0; row < p.side; ++row) count += patch_row_upper(terms, p, row);
     return count;
 }
 
@@ -880,7 +947,9 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
         // carry no branch). The pixels inside the image (and the cell) are appended to the LDS
         // slab in patch order: kept pixel k is word k, slot k / 64 of lane k % 64. Row and
         // column advance by the uniform quotient and remainder of 64 / side instead of a
-        // division per step. The moment start's ring sums and maximum are taken on the way.
+        // division per step. With a cell, lane r first finds row r's member and interior
+        // columns (cell_row); a pixel then reads its row's bounds by a shuffle. The moment
+        // start's ring sums and maximum are taken on the way.
         constexpr int kCap = NPL * kWave;
         constexpr int kBatch = 8;
         Patch<NPL> pt;
@@ -888,11 +957,8 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
         StartSums ss{0.0f, 0.0f, -3.0e38f};
         int kept = 0;  // uniform
         {
-            CellForm cell;
-            if (masked) load_cell(a, f, cell);
-            // offsets from the seed: (col - hs) + (cx - x0), one rounding of the exact value
-            const float dcx = cx - spx_uniform(a.seeds[2 * (int64_t)f]);
-            const float dcy = cy - spx_uniform(a.seeds[2 * (int64_t)f + 1]);
+            uint32_t rowb = 0u;
+            if (masked) rowb = pack_row(cell_row(lane_terms(a, f), ox, oy, side, lane < side ? lane : side - 1));
             spx_wave_lds_sync();  // every lane is done with the previous feature's words
             const int raw = (npx + kWave - 1) / kWave;
             const int dq = kWave / side, dr = kWave - dq * side;
@@ -927,16 +993,12 @@ SPX_DEV void erf_fit_wave(const FitArgs& a, uint32_t* lds_words) {
                     if (n0 + j >= raw) break;
                     bool ok = inside[j];
                     bool ring = rs[j] == 0 || rs[j] == side - 1 || cs[j] == 0 || cs[j] == side - 1;
-                    if (masked && ok) {
-                        const float px = (float)(cs[j] - hs) + dcx, py = (float)(rs[j] - hs) + dcy;
-                        const double qx = (double)(ox + cs[j]), qy = (double)(oy + rs[j]);
-                        if (from_init) {
-                            ok = cell_in(a, f, cell, px, py, qx, qy);
-                        } else {
-                            bool edge;
-                            cell_flags(a, f, cell, px, py, qx, qy, ok, edge);
-                            ring = ring || edge;
-                        }
+                    if (masked) {
+                        // bytes: lo + 1, hi + 1, elo + 1, ehi + 1 of this pixel's row
+                        const uint32_t rb = (uint32_t)spx_shfl(rowb, rs[j] < kWave ? rs[j] : kWave - 1);
+                        const int c1 = cs[j] + 1;
+                        ok = ok && c1 >= (int)(rb & 0xffu) && c1 <= (int)((rb >> 8) & 0xffu);
+                        ring = ring || c1 < (int)((rb >> 16) & 0xffu) || c1 > (int)(rb >> 24);
                     }
                     if (!from_init && ok) {
                         const float val = (float)v[j];

@@ -116,11 +116,12 @@ def test_fit_device_logic_under_wave_emulation():
 
 
 @pytest.mark.parametrize("integer_seeds", [True, False])
-def test_fp32_cell_first_look_takes_the_exact_decisions(integer_seeds):
-    """The masked gather and the start's boundary ring decide cell membership with an FP32 form
-    centred on the seed and fall back to the exact FP64 rule within its rounding margin: the
-    decisions equal the FP64 rule at every pixel of every patch, integer seeds (exact ties)
-    and fractional ones, neighbours near and far."""
+def test_cell_rows_take_the_exact_decisions(integer_seeds):
+    """The gather decides cell membership and the boundary ring from per-row column bounds
+    (cell_row: each neighbour's rule is monotone along a row, its end settled by the exact rule):
+    the decisions equal the oracle's FP64 rule at every pixel of every patch, with integer
+    seeds (exact ties on the bisectors) and fractional ones, seeds at 30000 px, absent
+    neighbours and patches of every size."""
     from scipy.spatial import cKDTree
 
     from oracle import erfdisk
@@ -136,33 +137,40 @@ def test_fp32_cell_first_look_takes_the_exact_decisions(integer_seeds):
     seeds[:50] += np.float32(30000.0)
     K = len(seeds)
     _, nb_idx = cKDTree(seeds).query(seeds, k=9)
-    nbr = nb_idx[:, 1:].astype(np.int32)
+    nbr = np.ascontiguousarray(nb_idx[:, 1:].astype(np.int32))
     nbr[::5, 6:] = -1  # absent neighbours
-    off = np.arange(-31, 32)
-    oy, ox = np.meshgrid(off, off, indexing="ij")
-    n = ox.size
+    half = np.full(K, 31, np.int16)
+    half[1::3] = rng.integers(2, 31, size=half[1::3].size)
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    outs = [np.zeros(n, np.uint8) for _ in range(4)]
-    mism = 0
-    for f in rng.choice(K, 300, replace=False):
-        qx = np.ascontiguousarray(np.rint(seeds[f, 0]) + ox.ravel(), dtype=np.float64)
-        qy = np.ascontiguousarray(np.rint(seeds[f, 1]) + oy.ravel(), dtype=np.float64)
-        row = np.ascontiguousarray(nbr[f])
-        lib.emu_cell_decisions(P(seeds), ctypes.c_int32(K), ctypes.c_int32(int(f)), P(row), ctypes.c_int32(8),
-                               P(qx), P(qy), ctypes.c_int64(n), *[P(o) for o in outs])
-        mism += int((outs[0] != outs[1]).sum() + (outs[2] != outs[3]).sum())
-        assert outs[1].any() and (~outs[1].astype(bool)).any()
-        # the exact rule is the oracle's
-        assert np.array_equal(outs[1].astype(bool), erfdisk.cell_mask(qx, qy, seeds, int(f), row))
-        assert np.array_equal(outs[3].astype(bool), erfdisk.cell_edge(qx, qy, seeds, int(f), row))
-    assert mism == 0
+    ties = 0
+    for f in np.r_[np.arange(10), rng.choice(K, 300, replace=False)]:
+        h = int(half[f])
+        side = 2 * h + 1
+        cin = np.zeros(side * side, np.uint8)
+        cedge = np.zeros(side * side, np.uint8)
+        lib.emu_cell_rows(P(seeds), P(half), P(nbr), ctypes.c_int32(8), ctypes.c_int32(K), ctypes.c_int32(int(f)),
+                          P(cin), P(cedge))
+        gy, gx = np.meshgrid(np.arange(side) + np.rint(seeds[f, 1]) - h, np.arange(side) + np.rint(seeds[f, 0]) - h,
+                             indexing="ij")
+        s64 = seeds.astype(np.float64)
+        assert np.array_equal(cin.astype(bool), erfdisk.cell_mask(gx.ravel(), gy.ravel(), s64, int(f), nbr[f]))
+        ref_edge = erfdisk.cell_mask(gx.ravel(), gy.ravel(), s64, int(f), nbr[f]) & \
+            erfdisk.cell_edge(gx.ravel(), gy.ravel(), s64, int(f), nbr[f])
+        assert np.array_equal(cedge.astype(bool), ref_edge)
+        for j in nbr[f]:
+            if j >= 0:
+                d = ((gx - s64[j, 0]) ** 2 + (gy - s64[j, 1]) ** 2) - ((gx - s64[f, 0]) ** 2 + (gy - s64[f, 1]) ** 2)
+                ties += int((d == 0).sum())
+    if integer_seeds:
+        assert ties > 0  # exact ties were decided
 
 
-def test_pixel_bound_covers_every_fitted_pixel():
-    """The bucketing's bound of a feature's fitted pixels (patch_pixel_bound) is never below
-    the pixels the oracle's patch keeps (inside the image and the Voronoi cell) and at most two
-    columns per row above them, on a page with integer seeds (exact ties), fractional seeds,
-    patches past every border and features without neighbours."""
+def test_pixel_count_and_bound_of_the_fitted_pixels():
+    """The exact count of a feature's fitted pixels from the cell rows (patch_pixel_count)
+    equals the pixels the oracle's patch keeps (inside the image and the Voronoi cell), and the
+    bucketing's bound (patch_pixel_bound) is never below it and at most four columns per row
+    above it, on a page with integer seeds (exact ties), fractional seeds, patches past every
+    border and features without neighbours."""
     from imageprocessor_new_amd import synthetic
     from oracle import erfdisk
 
@@ -179,9 +187,10 @@ def test_pixel_bound_covers_every_fitted_pixel():
         nbr = erfdisk.seed_neighbours(seeds, 2 * np.sqrt(2) * 32.5 + 1e-3, 8).astype(np.int32)
         nbr[::7] = -1  # some features without neighbours
         bound = np.zeros(K, np.int32)
+        exact = np.zeros(K, np.int32)
         P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
         lib.emu_pixel_bound(P(seeds), P(half), P(np.ascontiguousarray(nbr)), ctypes.c_int32(8), ctypes.c_int32(K),
-                            ctypes.c_int64(H), ctypes.c_int64(W), P(bound))
+                            ctypes.c_int64(H), ctypes.c_int64(W), P(bound), P(exact))
         img = np.zeros((H, W), np.uint16)
         for f in range(K):
             h = int(half[f])
@@ -190,8 +199,8 @@ def test_pixel_bound_covers_every_fitted_pixel():
                 side = 2 * h + 1
                 gy, gx = np.meshgrid(np.arange(side) + (cy - h), np.arange(side) + (cx - h), indexing="ij")
                 valid = valid & erfdisk.cell_mask(gx, gy, seeds.astype(np.float64), f, nbr[f])
-            exact = int(valid.sum())
-            assert bound[f] >= exact, (f, bound[f], exact)
-            assert bound[f] <= exact + 2 * (2 * h + 1), (f, bound[f], exact)
+            n = int(valid.sum())
+            assert exact[f] == n, (f, exact[f], n)
+            assert n <= bound[f] <= n + 4 * (2 * h + 1), (f, bound[f], n)
             if not (nbr[f] >= 0).any():
-                assert bound[f] == exact
+                assert bound[f] == n

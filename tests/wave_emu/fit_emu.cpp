@@ -116,6 +116,16 @@ inline float spx_wave_max(float v) {
     for (int off = 32; off >= 1; off >>= 1) v = std::max(v, spx_shfl_xor(v, off));
     return spx_uniform(v);
 }
+inline int spx_readlane_i(int v, int src) { return spx_shfl(v, src); }
+inline double spx_readlane_d(double v, int src) {
+    WaveState& w = *t_wave;
+    w.shuffles[t_lane]++;
+    w.dv[t_lane] = v;
+    w.bar.arrive_and_wait();
+    double r = w.dv[src];
+    w.bar.arrive_and_wait();
+    return r;
+}
 inline uint64_t spx_ballot(bool p) {
     WaveState& w = *t_wave;
     w.shuffles[t_lane]++;
@@ -200,7 +210,15 @@ extern "C" int emu_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, in
             status[f] = SPX_FIT_TOO_FEW;
             continue;
         }
-        const int bound = patch_pixel_bound(seeds, half, nbr, nb, K, H, W, f);
+        FitArgs pa{};
+        pa.seeds = seeds;
+        pa.half = half;
+        pa.nbr = nbr;
+        pa.nb = nb;
+        pa.nfeat = K;
+        pa.H = H;
+        pa.W = W;
+        const int bound = patch_pixel_bound(pa, f);
         order[2 * bucket_of_pixels(bound) + (md == SPX_MODEL_ERF_ELLIPSE7 ? 1 : 0)].push_back(f);
     }
     unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -226,10 +244,22 @@ extern "C" int emu_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, in
     return (int)stats[5];  // features with more pixels than their bucket holds
 }
 
-// The bucketing's pixel bound of every feature (patch_pixel_bound, plain per-thread code).
+// The bucketing's bound (out) and the exact count (exact) of every feature's fitted pixels
+// (patch_pixel_bound / patch_pixel_count, plain per-thread code).
 extern "C" void emu_pixel_bound(const float* seeds, const int16_t* half, const int32_t* nbr, int32_t nb, int32_t K,
-                                int64_t H, int64_t W, int32_t* out) {
-    for (int f = 0; f < K; ++f) out[f] = patch_pixel_bound(seeds, half, nbr, nb, K, H, W, f);
+                                int64_t H, int64_t W, int32_t* out, int32_t* exact) {
+    FitArgs a{};
+    a.seeds = seeds;
+    a.half = half;
+    a.nbr = nbr;
+    a.nb = nb;
+    a.nfeat = K;
+    a.H = H;
+    a.W = W;
+    for (int f = 0; f < K; ++f) {
+        out[f] = patch_pixel_bound(a, f);
+        exact[f] = patch_pixel_count(a, f);
+    }
 }
 
 extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
@@ -240,35 +270,28 @@ extern "C" int emu_fit_batch(const uint16_t* img, int64_t H, int64_t W, int64_t 
                                status, timeout_s);
 }
 
-// Voronoi-cell decisions of one feature f at the points (qx[i], qy[i]): the FP32 first look
-// (cell_in_fast / cell_edge_fast) and the exact FP64 rule (cell_exact), for a test that they
-// agree everywhere. No wave operations: the cell is built by make_cell directly.
-extern "C" void emu_cell_decisions(const float* seeds, int32_t K, int32_t f, const int32_t* nbr, int32_t nb,
-                                   const double* qx, const double* qy, int64_t n, uint8_t* in_fast,
-                                   uint8_t* in_exact, uint8_t* edge_fast, uint8_t* edge_exact) {
+// Cell decisions over feature f's whole patch (side x side, row-major) from the row bounds
+// the gather uses (cell_row, packed as the kernel packs them): member and boundary flags.
+extern "C" void emu_cell_rows(const float* seeds, const int16_t* half, const int32_t* nbr, int32_t nb, int32_t K,
+                              int32_t f, uint8_t* in_out, uint8_t* edge_out) {
     using namespace spx::fitdev;
-    float xs[kMaxNb], ys[kMaxNb];
-    bool act[kMaxNb];
-    for (int j = 0; j < kMaxNb; ++j) {
-        const int id = j < nb ? nbr[j] : -1;
-        act[j] = id >= 0 && id < K;
-        xs[j] = act[j] ? seeds[2 * id] : 0.0f;
-        ys[j] = act[j] ? seeds[2 * id + 1] : 0.0f;
-    }
-    CellForm cf;
-    make_cell(seeds[2 * f], seeds[2 * f + 1], xs, ys, act, cf);
-    // the row of neighbours stands at row f of a K x nb table as the kernel reads it
-    std::vector<int32_t> table((size_t)K * nb, -1);
-    for (int j = 0; j < nb; ++j) table[(size_t)f * nb + j] = nbr[j];
     FitArgs a{};
     a.seeds = seeds;
-    a.nbr = table.data();
+    a.half = half;
+    a.nbr = nbr;
     a.nb = nb;
     a.nfeat = K;
-    for (int64_t i = 0; i < n; ++i) {
-        in_fast[i] = cell_in_fast(a, f, cf, qx[i], qy[i]);
-        in_exact[i] = cell_exact(a, f, qx[i], qy[i], false);
-        edge_fast[i] = cell_edge_fast(a, f, cf, qx[i], qy[i]);
-        edge_exact[i] = cell_exact(a, f, qx[i], qy[i], true);
+    const int hs = half[f];
+    const int side = 2 * hs + 1;
+    const int64_t ox = (int64_t)rintf(seeds[2 * f]) - hs, oy = (int64_t)rintf(seeds[2 * f + 1]) - hs;
+    for (int r = 0; r < side; ++r) {
+        const uint32_t rb = pack_row(cell_row(MemTerms(a, f), ox, oy, side, r));
+        for (int c = 0; c < side; ++c) {
+            const int c1 = c + 1;
+            const bool in = c1 >= (int)(rb & 0xffu) && c1 <= (int)((rb >> 8) & 0xffu);
+            const bool interior = c1 >= (int)((rb >> 16) & 0xffu) && c1 <= (int)(rb >> 24);
+            in_out[r * side + c] = in;
+            edge_out[r * side + c] = in && !interior;
+        }
     }
 }
