@@ -30,13 +30,14 @@ FLOP_PER_PIXEL_PASS = {6: 90.0, 7: 110.0}  # model + Jacobian + J^T J + J^T r + 
 PEAK_FP32_TFLOPS = 157.3                    # MI355X vendor FP32 peak (packed)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "erf_fit_square8192_kernel_stats.md")
+TRAFFIC_PROFILE_HEX = os.path.join(ROOT, "profiles", "erf_fit_hex16384_kernel_stats.md")
 
 
-def measured_traffic():
-    """HBM bytes per fit launch of the full workload from the committed rocprofv3 counter
-    summary (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated against a 256 MiB copy), or None."""
+def measured_traffic(path: str = TRAFFIC_PROFILE):
+    """HBM bytes per fit launch of the full workload from a committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated against a 256 MiB copy), or None."""
     try:
-        with open(TRAFFIC_PROFILE) as fh:
+        with open(path) as fh:
             for line in fh:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -113,6 +114,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-start leg")
+    ap.add_argument("--no-bucket-table", action="store_true",
+                    help="uneven workload: skip the per-bucket timing (for counter runs)")
     ap.add_argument("--train-tiles", type=int, default=1 << 18)
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -514,6 +517,13 @@ def run_uneven(args) -> int:
     from imageprocessor_new_amd.fit import autotune_launch, fit_erf_disks_device
 
     with gpu_device(0) as dev:
+        if args.calibrate_copy:
+            # a 256 MiB copy whose counter readings scale FETCH_SIZE / WRITE_SIZE to bytes
+            src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+            src.fill_(1)
+            dst = src.clone()
+            torch.cuda.synchronize()
+            del src, dst
         t0 = time.perf_counter()
         page = synthetic.render_workload_device("hex16384", seed=args.seed)
         f = learned.page_features(page, dev)
@@ -535,8 +545,10 @@ def run_uneven(args) -> int:
         stats = read_fit_stats()
         params, centers = out["params"].cpu().numpy(), out["centers"].cpu().numpy()
         status, iters = out["status"].cpu().numpy(), out["iters"].cpu().numpy()
-        buckets = bucket_table(page.image, f.seeds, f.half, f.model, f.nbr, half_np, model_np)
-        log(f"buckets: {buckets}")
+        buckets = None
+        if not args.no_bucket_table:
+            buckets = bucket_table(page.image, f.seeds, f.half, f.model, f.nbr, half_np, model_np)
+            log(f"buckets: {buckets}")
         img_host = page.image.cpu().numpy()
         seeds_np, nbr_np = f.seeds.cpu().numpy(), f.nbr.cpu().numpy()
     ms_per_step = 1e3 * wall / args.steps
@@ -577,8 +589,9 @@ def run_uneven(args) -> int:
         },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
-                     "traffic": None, "algorithmic_bytes_per_launch": gather, "kernel_ms": kernel_ms,
-                     "algorithmic_flop_per_launch": flops},
+                     "traffic": measured_traffic(TRAFFIC_PROFILE_HEX), "algorithmic_bytes_per_launch": gather,
+                     "kernel_ms": kernel_ms, "algorithmic_flop_per_launch": flops,
+                     "kernel": "plain batched LM launch (bucketing + one persistent kernel per bucket)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
@@ -718,8 +731,14 @@ def run_learned(args) -> int:
         },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,
-                     "traffic": None, "kernel": "plain batched LM launch", "kernel_ms": plain_kernel_ms,
-                     "algorithmic_flop_per_launch": flops},
+                     "traffic": measured_traffic(TRAFFIC_PROFILE_HEX),
+                     "algorithmic_bytes_per_launch": float((((2 * half_np.astype(np.int64) + 1) ** 2) * 2 + 32).sum()),
+                     "kernel": "plain batched LM launch (bucketing + one persistent kernel per bucket)",
+                     "kernel_ms": plain_kernel_ms, "algorithmic_flop_per_launch": flops,
+                     "learned_launch": {"kernel_ms": 1e3 * float(np.mean([p[1] for p in parts])),
+                                        "algorithmic_flop": lflops,
+                                        "frac_unpacked": lflops / (float(np.mean([p[1] for p in parts]))) / 1e12
+                                        / PEAK_FP32_UNPACKED_TFLOPS}},
         "cpu_baseline": cpu_lm_baseline(img_host, seeds_np, half_np, model_np, nbr_np),
     }
     print(json.dumps(result), flush=True)

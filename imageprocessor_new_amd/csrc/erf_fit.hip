@@ -321,12 +321,68 @@ static_assert(kFitHeadWords * 4 <= (int)kFitHead, "fit scratch head");
 
 size_t fit_scratch_bytes(int32_t K) { return kFitHead + sizeof(int32_t) * (size_t)K; }
 
+// The bucket kernels run on kForkStreams streams at once: the caller's and helpers that
+// fork from it after the bucketing and join it before the call returns. A persistent kernel
+// holds every CU it fits on until its queue drains, and the buckets of few or slowly
+// converging features (ellipses, small patches) spend most of their time waiting on one
+// feature's chain of dependent solves; side by side, those waits overlap the large buckets'
+// work and tails. Each stream takes a group of buckets in turn.
+constexpr int kForkStreams = 4;
+constexpr int kForkOrder[kKernBuckets] = {6, 8, 4, 2, 7, 5, 9, 3, 1, 0};   // launch order
+constexpr int kForkStream[kKernBuckets] = {3, 3, 1, 3, 1, 2, 0, 2, 0, 2};  // stream of bucket b
+
+struct ForkSet {
+    hipStream_t helper[kForkStreams] = {};  // [0] unused: the caller's stream
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kForkStreams] = {};
+};
+
+namespace {
+std::mutex g_fork_mu;
+std::map<std::pair<int, hipStream_t>, ForkSet> g_forks;
+
+// The helper streams and events of (device, caller stream), created at first use (default
+// priority, non-blocking) and kept until spx_release.
+int fork_set(int dev, hipStream_t stream, ForkSet** out) {
+    std::lock_guard<std::mutex> lock(g_fork_mu);
+    ForkSet& f = g_forks[std::make_pair(dev, stream)];
+    if (f.fork == nullptr) {
+        SPX_HIP(hipEventCreateWithFlags(&f.fork, hipEventDisableTiming));
+        for (int i = 1; i < kForkStreams; ++i) {
+            SPX_HIP(hipStreamCreateWithFlags(&f.helper[i], hipStreamNonBlocking));
+            SPX_HIP(hipEventCreateWithFlags(&f.join[i], hipEventDisableTiming));
+        }
+    }
+    *out = &f;
+    return SPX_OK;
+}
+}  // namespace
+
+void fit_streams_release() {
+    std::lock_guard<std::mutex> lock(g_fork_mu);
+    for (auto& kv : g_forks) {
+        if (hipSetDevice(kv.first.first) != hipSuccess) continue;
+        for (int i = 1; i < kForkStreams; ++i) {
+            if (kv.second.helper[i] != nullptr) {
+                (void)hipStreamSynchronize(kv.second.helper[i]);
+                (void)hipStreamDestroy(kv.second.helper[i]);
+            }
+            if (kv.second.join[i] != nullptr) (void)hipEventDestroy(kv.second.join[i]);
+        }
+        if (kv.second.fork != nullptr) (void)hipEventDestroy(kv.second.fork);
+    }
+    g_forks.clear();
+}
+
 // Enqueues the fit on `scratch` (fit_scratch_bytes(K) bytes of device memory) with its
-// counters at `stats` (8 x u64, may lie inside scratch).
+// counters at `stats` (kFitCounters x u64, may lie inside scratch). With `forks`, the bucket
+// kernels run on the caller's stream and the helpers (see kForkStreams); without, all in
+// turn on the caller's stream. Results are the same either way.
 int fit_batch_enqueue(void* scratch, unsigned long long* stats, const uint16_t* img, int64_t H, int64_t W,
                       int64_t pitch, const float* seeds, const int16_t* half, const uint8_t* model, const float* init,
                       const SpxFitOptions& opt, int32_t K, const int32_t* nbr, int32_t nb, float* params,
-                      double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream) {
+                      double* centers, float* rms, int16_t* iters, uint8_t* status, hipStream_t stream,
+                      ForkSet* forks) {
     int32_t* head = static_cast<int32_t*>(scratch);
     int32_t* counts = head + 2 * kFitCounters;
     int32_t* cursors = counts + kKernBuckets;
@@ -354,7 +410,14 @@ int fit_batch_enqueue(void* scratch, unsigned long long* stats, const uint16_t* 
     SPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const int waves_per_block = opt.tile_waves > 0 ? (opt.tile_waves < 4 ? opt.tile_waves : 4) : 4;
     const int threads = waves_per_block * fitdev::kWave;
-    for (int b = 0; b < kKernBuckets; ++b) {
+    hipStream_t streams[kForkStreams];
+    for (int i = 0; i < kForkStreams; ++i) streams[i] = (forks != nullptr && i > 0) ? forks->helper[i] : stream;
+    if (forks != nullptr) {
+        SPX_HIP(hipEventRecord(forks->fork, stream));
+        for (int i = 1; i < kForkStreams; ++i) SPX_HIP(hipStreamWaitEvent(forks->helper[i], forks->fork, 0));
+    }
+    for (int bi = 0; bi < kKernBuckets; ++bi) {
+        const int b = kForkOrder[bi];
         if (opt.bucket_mask != 0 && !((opt.bucket_mask >> b) & 1)) continue;
         a.bucket = b;
         a.count = counts + b;
@@ -365,9 +428,15 @@ int fit_batch_enqueue(void* scratch, unsigned long long* stats, const uint16_t* 
         const int64_t want = ((int64_t)K + waves_per_block - 1) / waves_per_block;
         const int64_t cap = opt.persistent ? (int64_t)cus * resident_blocks(dev, b, threads) : want;
         const int blocks = (int)(want < cap ? want : cap);
-        hipLaunchKernelGGL(kKernels[b], dim3(blocks), dim3(threads), lds_bytes(b, threads), stream, a);
+        hipLaunchKernelGGL(kKernels[b], dim3(blocks), dim3(threads), lds_bytes(b, threads), streams[kForkStream[b]],
+                           a);
         SPX_LAUNCHED();
     }
+    if (forks != nullptr)
+        for (int i = 1; i < kForkStreams; ++i) {
+            SPX_HIP(hipEventRecord(forks->join[i], forks->helper[i]));
+            SPX_HIP(hipStreamWaitEvent(stream, forks->join[i], 0));
+        }
     return SPX_OK;
 }
 
@@ -382,8 +451,13 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
     unsigned long long* stats = nullptr;
     rc = fit_stats_buffer(&stats, stream);
     if (rc != SPX_OK) return rc;
+    int dev = 0;
+    SPX_HIP(hipGetDevice(&dev));
+    ForkSet* forks = nullptr;
+    rc = fork_set(dev, stream, &forks);
+    if (rc != SPX_OK) return rc;
     return fit_batch_enqueue(ws, stats, img, H, W, pitch, seeds, half, model, init, opt, K, nbr, nb, params, centers,
-                             rms, iters, status, stream);
+                             rms, iters, status, stream, forks);
 }
 
 // ---- the allocation-free fit -------------------------------------------------------------
@@ -439,8 +513,9 @@ int fit_batch_noalloc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t 
     }
     // the counters sit at the head's start; counts, cursors and queues follow them
     unsigned long long* stats = static_cast<unsigned long long*>(base);
+    // no helper streams here: creating them is work this entry point does not do
     return fit_batch_enqueue(base, stats, img, H, W, pitch, seeds, half, model, init, opt, K, nbr, nb, params,
-                             centers, rms, iters, status, stream);
+                             centers, rms, iters, status, stream, nullptr);
 }
 
 }  // namespace spx

@@ -78,6 +78,7 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
 
 int fit_stats_copy(uint64_t* out, int n, hipStream_t stream);
 int fit_scratch_init(int dev);
+void fit_streams_release();
 int fit_batch_noalloc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
                              const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions& opt,
                              int32_t K, const int32_t* nbr, int32_t nb, float* params, double* centers, float* rms,
@@ -396,6 +397,7 @@ void spx_release(void) {
         }
     }
     spx::g_fit_stats.clear();
+    spx::fit_streams_release();
     (void)hipSetDevice(cur);
 }
 
