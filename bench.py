@@ -739,7 +739,8 @@ def run_learned(args) -> int:
                                         "algorithmic_flop": lflops,
                                         "frac_unpacked": lflops / (float(np.mean([p[1] for p in parts]))) / 1e12
                                         / PEAK_FP32_UNPACKED_TFLOPS}},
-        "cpu_baseline": cpu_lm_baseline(img_host, seeds_np, half_np, model_np, nbr_np),
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_lm_baseline(img_host, seeds_np, half_np, model_np,
+                                                                          nbr_np),
     }
     print(json.dumps(result), flush=True)
     return 0

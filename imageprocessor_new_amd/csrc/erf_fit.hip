@@ -195,7 +195,9 @@ __global__ __launch_bounds__(64 * kCountWaves) void bucket_count_kernel(FitArgs 
         if (feature_valid(a.half, a.model, f)) {
             const PatchSpan p = patch_span(a, f);
             int cnt = p.side * p.side;
-            if (!p.whole) {
+            if (!p.whole && !p.cells) {
+                cnt = p.c1 >= p.c0 && p.r1 >= p.r0 ? (int)((p.c1 - p.c0 + 1) * (p.r1 - p.r0 + 1)) : 0;
+            } else if (!p.whole) {
                 cnt = patch_row_upper(lane_terms(a, f), p, lane);
                 if (lane >= p.side) cnt = 0;
                 for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
