@@ -136,8 +136,6 @@ def main() -> int:
                          "outline: fit_outline (page outline + blur, every hole's size) on a 2048x1378 page of "
                          "39,042 holes")
     ap.add_argument("--page-size", type=int, default=32768, help="side of the tiled-workload page")
-    ap.add_argument("--learned", action="store_true",
-                    help="tiled workload: also run the learned start (CNN proposal, <= 4 LM steps, fallback)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -266,7 +264,7 @@ def main() -> int:
                 "fallback_features": int(parts[-1][3]),
                 "mean_iterations": float(li.mean()),
                 "max_centre_diff_vs_plain_px": agree,
-                "train": {k: tinfo[k] for k in ("tiles", "samples", "data_seconds", "train_seconds", "epochs")},
+                "train": {k: tinfo[k] for k in ("samples", "pages", "data_seconds", "train_seconds", "epochs")},
                 "status_ok": float((ls <= 1).mean()),
             }
             log(f"learned leg: {learned_leg}")
@@ -779,13 +777,13 @@ def run_tiled(args) -> int:
             walls.append(time.perf_counter() - t)
             metas.append(res.meta["stage_ms"])
         learned_leg = None
-        if args.learned:
+        if not args.no_learned:
             from imageprocessor_new_amd import learned
 
-            log("learned start: rendering, labelling and training the CNN on the GPU")
-            net, tinfo = learned.train_learned_start(
-                n_tiles=args.train_tiles, epochs=args.train_epochs, batch=4096, lr=1e-3, seed=10_000,
-                device=torch.device("cuda", 0), mix=[(learned.SPEC_SQUARE, 0.5), (learned.SPEC_STAGGERED, 0.5)])
+            log(f"learned start: training the CNN ({args.train_patches} patches of staggered pages from seed 10000, "
+                f"{args.train_epochs} epochs)")
+            net, tinfo = learned.train_on_pages(n_patches=args.train_patches, epochs=args.train_epochs, batch=4096,
+                                                lr=1e-3, seed=10_000, device=torch.device("cuda", 0))
             kl = dict(kw, learned_start=net, learned_iters=4)
             detect_centers(img_t, **kl)
             lwalls, lmetas = [], []
@@ -807,7 +805,7 @@ def run_tiled(args) -> int:
                 "detected": int(res_l.centers_xy.shape[0]),
                 "found_within_0.5px": float((dl <= 0.5).mean()),
                 "max_centre_diff_vs_plain_px": float(dp.max()) if dp.size else None,
-                "train": {k: tinfo[k] for k in ("tiles", "samples", "data_seconds", "train_seconds", "epochs")},
+                "train": {k: tinfo[k] for k in ("samples", "pages", "data_seconds", "train_seconds", "epochs")},
             }
             log(f"learned leg: {learned_leg}")
     wall = float(np.median(walls))
@@ -822,14 +820,17 @@ def run_tiled(args) -> int:
         "components": (strips_px * (2 + 4)) / (stage["components"] * 1e-3) / 1e9,
     }
     log(f"tiled: {found.shape[0]} centres in {1e3 * wall:.1f} ms; stages {stage}")
+    # the configured pipeline is the learned one (CNN proposal + LM + fallback per strip)
+    value_wall = wall if learned_leg is None else learned_leg["ms_per_step"] * 1e-3
+    value_n = found.shape[0] if learned_leg is None else learned_leg["detected"]
     result = {
         "metric": "features fitted/sec at <=1e-3 px vs FP64 oracle; learned+batched vs plain LM",
-        "value": found.shape[0] / wall,
+        "value": value_n / value_wall,
         "unit": "features/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": max(1, min(args.warmup, 2)),
-        "ms_per_step": 1e3 * wall,
+        "ms_per_step": 1e3 * value_wall,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -837,8 +838,12 @@ def run_tiled(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": f"end-to-end tiled detect_centers, {H}x{W} uint16 mixed page "
-                        f"({holes.shape[0]} holes, square / staggered 2048 px blocks, pitch U[24, 44]), "
-                        "strips of 8192 rows, overlap 128, global Otsu, erf-disk LM with cell mask",
+                        f"({holes.shape[0]} holes, square / staggered 2048 px blocks, pitch "
+                        f"{pitches.min():.1f}..{pitches.max():.1f} px), strips of 8192 rows, overlap 128, global "
+                        "Otsu, erf-disk LM with cell mask; value: the learned start (CNN proposal + <= 4 LM "
+                        "iterations + plain fallback) per strip",
+            "plain_ms_per_step": 1e3 * wall,
+            "plain_features_per_s": found.shape[0] / wall,
             "holes": int(holes.shape[0]),
             "detected": int(found.shape[0]),
             "found_within_0.5px": float(hit.mean()),

@@ -654,70 +654,68 @@ __device__ __forceinline__ void conv_layer(const uint16_t* __restrict__ in, uint
             bf16x8v a = bf16x8v{};
             if (live && !(ky == 0 && top[g]) && !(kx == 0 && left[g]))
                 a = load8(in + base[g] + off + sp * CIN + ((chunk ^ (sp & CM)) << 3));
-            acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[g], 0, 0, 0);
+            // the transposed product W^T X^T: the weight fragment is the A operand, the
+            // activation fragment the B operand (the same registers as X W's B and A)
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a, acc[g], 0, 0, 0);
         }
     }
-    // D(i, j): i = 4 (lane / 16) + r, j = lane % 16
-    const int n = nb * 16 + li;
-    const float bn = bias[n];
+    // D^T(i, j): output channel nb * 16 + 4 (lane / 16) + r of output pixel (rb0 + g) * 16 +
+    // lane % 16, so a lane holds four consecutive channels of one pixel: one 8-byte store
+    const int n0 = nb * 16 + 4 * g4;
+    float bn[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bn[r] = bias[n0 + r];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int m = (rb0 + g) * 16 + 4 * g4 + r;
-            const int p = m / PIX, pix = m - (m / PIX) * PIX;
-            const int y = pix / HOUT, x = pix - (pix / HOUT) * HOUT;
-            const uint16_t v = f2bf(fmaxf(acc[g][r] + bn, 0.0f));
-            if constexpr (NEXT_FC) out[fc_idx(p, pix * COUT + n)] = v;
-            else out[act_idx<HOUT, COUT>(p, y, x, n)] = v;
-        }
+        const int m = (rb0 + g) * 16 + li;
+        const int p = m / PIX, pix = m - (m / PIX) * PIX;
+        const int y = pix / HOUT, x = pix - (pix / HOUT) * HOUT;
+        const uint32_t lo = f2bf2(fmaxf(acc[g][0] + bn[0], 0.0f), fmaxf(acc[g][1] + bn[1], 0.0f));
+        const uint32_t hi = f2bf2(fmaxf(acc[g][2] + bn[2], 0.0f), fmaxf(acc[g][3] + bn[3], 0.0f));
+        const int idx = NEXT_FC ? fc_idx(p, pix * COUT + n0) : act_idx<HOUT, COUT>(p, y, x, n0);
+        *reinterpret_cast<uint2*>(out + idx) = make_uint2(lo, hi);
     }
 }
 
-// conv1 (1 -> 16, 3x3 stride 2 pad 1, 32 x 32 -> 16 x 16) of patch `wave` on the VALU: lane l
-// takes output pixels l, l + 64, l + 128, l + 192, holds their 36 input taps and forms the 16
-// channels two at a time in FP32 (BF16 weights and inputs, so every product is exact), then
-// bias, ReLU and BF16 rounding. w1: the 9 x 16 weights as FP32 and the 16 biases, in LDS.
-__device__ __forceinline__ void conv1_valu(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+// conv1 (1 -> 16, 3x3 stride 2 pad 1, 32 x 32 -> 16 x 16) of patch `wave` on the matrix cores:
+// sixteen v_mfma_f32_16x16x16_bf16 products W^T X^T, one per output row j, with the 16
+// channels as rows, the row's 16 output pixels as columns and the 9 taps (padded to 16) as K.
+// Lane l holds taps 4 (l / 16) .. + 3 of output pixel (j, l % 16) as the B operand and the
+// same taps' weights of channel l % 16 as the A operand (fixed for the patch); taps 9..15 are
+// zero. The result gives lane l channels 4 (l / 16) .. + 3 of pixel (j, l % 16): bias, ReLU,
+// BF16 and one 8-byte store. Products of BF16 values are exact; the FP32 sums are the MFMA's.
+// w1: the 9 x 16 weights as FP32 (BF16 values) and the 16 biases, in LDS.
+__device__ __forceinline__ void conv1_mfma(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
                                            const float* __restrict__ w1, int wave, int lane) {
-    float v[4][9];
+    const int li = lane & 15, g4 = lane >> 4;
+    bf16x4 wa;
+    short tap_base[4];
+    bool tap_live[4], tap_top[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int p = lane + 64 * j, oy = p >> 4, ox = p & 15;
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int iy = 2 * oy + t / 3 - 1, ix = 2 * ox + t % 3 - 1;
-            v[j][t] = (iy >= 0 && ix >= 0) ? bf2f(in[act_idx<32, 1>(wave, iy, ix, 0)]) : 0.0f;
-        }
+    for (int i = 0; i < 4; ++i) {
+        const int t = 4 * g4 + i;  // tap index, live when < 9
+        const int ky = t / 3, kx = t - 3 * (t / 3);
+        tap_live[i] = t < 9 && !(kx == 0 && li == 0);
+        tap_top[i] = ky == 0;
+        // input pixel (2 j + ky - 1, 2 li + kx - 1): phase ((ky + 1) & 1, (kx + 1) & 1), sub-pixel
+        // (j - (ky == 0), li - (kx == 0)); the row offset 16 j is added per output row
+        const int ph = (((ky + 1) & 1) << 1) | ((kx + 1) & 1);
+        tap_base[i] = (short)((wave * 4 + ph) * 256 + (ky == 0 ? -16 : 0) + li - (kx == 0 ? 1 : 0));
+        wa[i] = t < 9 ? (short)f2bf(w1[t * 16 + li]) : (short)0;
     }
-    uint32_t packed[4][8];
-#pragma unroll 2
-    for (int c = 0; c < 16; c += 2) {
-        float wa[9], wb[9];
+    float bn[4];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            wa[t] = w1[t * 16 + c];
-            wb[t] = w1[t * 16 + c + 1];
-        }
-        const float ba = w1[144 + c], bb = w1[144 + c + 1];
+    for (int r = 0; r < 4; ++r) bn[r] = w1[144 + 4 * g4 + r];
+#pragma unroll 4
+    for (int j = 0; j < 16; ++j) {
+        bf16x4 xb;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float sa = 0.0f, sb = 0.0f;
-#pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                sa = fmaf(v[j][t], wa[t], sa);
-                sb = fmaf(v[j][t], wb[t], sb);
-            }
-            packed[j][c >> 1] = f2bf2(fmaxf(sa + ba, 0.0f), fmaxf(sb + bb, 0.0f));
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int p = lane + 64 * j, oy = p >> 4, ox = p & 15;
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-            *reinterpret_cast<uint4*>(out + act_idx<16, 16>(wave, oy, ox, 8 * q)) =
-                make_uint4(packed[j][4 * q], packed[j][4 * q + 1], packed[j][4 * q + 2], packed[j][4 * q + 3]);
+        for (int i = 0; i < 4; ++i)
+            xb[i] = (tap_live[i] && !(tap_top[i] && j == 0)) ? (short)in[tap_base[i] + 16 * j] : (short)0;
+        const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa, xb, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+        const uint32_t lo = f2bf2(fmaxf(d[0] + bn[0], 0.0f), fmaxf(d[1] + bn[1], 0.0f));
+        const uint32_t hi = f2bf2(fmaxf(d[2] + bn[2], 0.0f), fmaxf(d[3] + bn[3], 0.0f));
+        *reinterpret_cast<uint2*>(out + act_idx<16, 16>(wave, j, li, 4 * g4)) = make_uint2(lo, hi);
     }
 }
 
@@ -765,7 +763,7 @@ __device__ __forceinline__ void infer_net(uint16_t* bufX, uint16_t* bufY, float*
                                           float* out) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const float* th = iw.theta;
-    conv1_valu(bufX, bufY, w1, wave, lane);
+    conv1_mfma(bufX, bufY, w1, wave, lane);
     __syncthreads();
     conv_layer<16, 32, 16, 8, false>(bufY, bufX, iw.wt + iw.off[1], th + param_offset_dev(1) + 144 * 32, wave, lane);
     __syncthreads();
