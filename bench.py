@@ -503,6 +503,29 @@ def bucket_table(img_t, seeds_t, half_t, model_t, nbr_t, half_np, model_np, repe
     return out
 
 
+def graph_leg(img, f, opts, eager_out, steps: int) -> dict:
+    """The same batch recorded once as a HIP graph (FitGraph) and replayed `steps` times: replay
+    time beside the eager launch, and whether every replay's outputs equal the eager ones."""
+    import torch
+
+    from imageprocessor_new_amd.fit import FitGraph
+
+    t0 = time.perf_counter()
+    g = FitGraph(img, f.seeds, f.half, f.model, options=opts, neighbours=f.nbr)
+    record_ms = 1e3 * (time.perf_counter() - t0)
+    try:
+        g.launch()
+        ms, wall, _ = timed(g.launch, steps)
+        torch.cuda.synchronize()
+        same = all(torch.equal(g.out[k], eager_out[k]) for k in ("iters", "status")) and all(
+            torch.equal(torch.nan_to_num(g.out[k], nan=-1.0), torch.nan_to_num(eager_out[k], nan=-1.0))
+            for k in ("params", "centers", "rms"))
+    finally:
+        g.close()
+    return {"replay_ms": ms, "ms_per_step": 1e3 * wall / steps, "record_ms": record_ms,
+            "equal_to_eager": bool(same)}
+
+
 def run_uneven(args) -> int:
     """Uneven batch: the 16384^2 staggered array (~200k holes, r U[4, 20], 10 % ellipses with q
     U[0.8, 1], Poisson + N(0, 30^2) noise) rendered on the GPU; patches 15..63 px from each hole's
@@ -543,6 +566,8 @@ def run_uneven(args) -> int:
         stats = read_fit_stats()
         params, centers = out["params"].cpu().numpy(), out["centers"].cpu().numpy()
         status, iters = out["status"].cpu().numpy(), out["iters"].cpu().numpy()
+        graph = graph_leg(page.image, f, opts, out, args.steps)
+        log(f"graph-recorded launch: {graph}")
         buckets = None
         if not args.no_bucket_table:
             buckets = bucket_table(page.image, f.seeds, f.half, f.model, f.nbr, half_np, model_np)
@@ -582,7 +607,7 @@ def run_uneven(args) -> int:
             "counted_fraction": acc_frac, "mean_iterations": float(iters.mean()),
             "status_counts": np.bincount(status, minlength=7).tolist(),
             "launch_autotune": {"best": list(tune["best"]), "ms": tune["ms"]},
-            "buckets": buckets, "median_err_vs_truth_px": float(np.median(d_true)),
+            "buckets": buckets, "graph": graph, "median_err_vs_truth_px": float(np.median(d_true)),
             "accuracy": None if acc is None else acc["legs"]["plain"],
         },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

@@ -156,6 +156,10 @@ SIGNATURES = {
                                    _I32, _P, _P, _P, _P, _P, _P]),
     "spx_fit_batch_noalloc": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions), _I32, _P,
                                      _I32, _P, _P, _P, _P, _P, _P]),
+    "spx_fit_graph_create": (_I32, [_P, _I64, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(SpxFitOptions), _I32, _P,
+                                    _I32, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]),
+    "spx_fit_graph_launch": (_I32, [_P, _P]),
+    "spx_fit_graph_destroy": (_I32, [_P]),
     "spx_seed_neighbours": (_I32, [_P, _I32, _I64, _I64, ctypes.c_float, _I32, _P, _P]),
     "spx_triangle_u8": (_I32, [_P, _I64, _I64, _I64, _I32, ctypes.POINTER(ctypes.c_int32), _P]),
     "spx_threshold_mask": (_I32, [_P, _I64, _I64, _I64, _I32, _I32, _P, _P]),

@@ -462,6 +462,119 @@ int fit_batch_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
                              rms, iters, status, stream, forks);
 }
 
+// ---- the graph-recorded fit --------------------------------------------------------------
+// One batch recorded on a capture stream of the graph's own (the null stream cannot be
+// captured), with helper streams of its own joined into the capture by the fork event. The
+// recorded work uses no memset nodes: the scratch head and counters are cleared by zero_i32.
+
+}  // namespace spx
+
+struct SpxFitGraph {
+    int dev = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* scratch = nullptr;
+    unsigned long long* stats = nullptr;
+    hipStream_t capture = nullptr;
+    spx::ForkSet forks;
+};
+
+namespace spx {
+
+namespace {
+void fit_graph_free(SpxFitGraph* g) {
+    if (g == nullptr) return;
+    (void)hipSetDevice(g->dev);
+    if (g->exec != nullptr) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph != nullptr) (void)hipGraphDestroy(g->graph);
+    for (int i = 1; i < kForkStreams; ++i) {
+        if (g->forks.helper[i] != nullptr) {
+            (void)hipStreamSynchronize(g->forks.helper[i]);
+            (void)hipStreamDestroy(g->forks.helper[i]);
+        }
+        if (g->forks.join[i] != nullptr) (void)hipEventDestroy(g->forks.join[i]);
+    }
+    if (g->forks.fork != nullptr) (void)hipEventDestroy(g->forks.fork);
+    if (g->capture != nullptr) {
+        (void)hipStreamSynchronize(g->capture);
+        (void)hipStreamDestroy(g->capture);
+    }
+    if (g->scratch != nullptr) (void)hipFree(g->scratch);
+    if (g->stats != nullptr) (void)hipFree(g->stats);
+    if (g_last_stats == g->stats) {
+        g_last_stats = nullptr;
+        g_last_stats_stream = nullptr;
+    }
+    delete g;
+}
+
+int fit_graph_record(SpxFitGraph* g, const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                     const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions& opt,
+                     int32_t K, const int32_t* nbr, int32_t nb, float* params, double* centers, float* rms,
+                     int16_t* iters, uint8_t* status) {
+    SPX_HIP(hipGetDevice(&g->dev));
+    // the launch shapes are looked up before the capture (host calls only inside it)
+    const int threads = (opt.tile_waves > 0 ? (opt.tile_waves < 4 ? opt.tile_waves : 4) : 4) * fitdev::kWave;
+    for (int b = 0; b < kKernBuckets; ++b) (void)resident_blocks(g->dev, b, threads);
+    SPX_HIP(hipMalloc(&g->scratch, fit_scratch_bytes(K)));
+    SPX_HIP(hipMalloc(reinterpret_cast<void**>(&g->stats), kFitCounters * sizeof(unsigned long long)));
+    SPX_HIP(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
+    SPX_HIP(hipEventCreateWithFlags(&g->forks.fork, hipEventDisableTiming));
+    for (int i = 1; i < kForkStreams; ++i) {
+        SPX_HIP(hipStreamCreateWithFlags(&g->forks.helper[i], hipStreamNonBlocking));
+        SPX_HIP(hipEventCreateWithFlags(&g->forks.join[i], hipEventDisableTiming));
+    }
+    unsigned long long* prev_stats = g_last_stats;
+    hipStream_t prev_stream = g_last_stats_stream;
+    SPX_HIP(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
+    const int rc = fit_batch_enqueue(g->scratch, g->stats, img, H, W, pitch, seeds, half, model, init, opt, K, nbr,
+                                     nb, params, centers, rms, iters, status, g->capture, &g->forks);
+    // the capture ends whatever the recording returned
+    const hipError_t end = hipStreamEndCapture(g->capture, &g->graph);
+    g_last_stats = prev_stats;
+    g_last_stats_stream = prev_stream;
+    if (rc != SPX_OK) return rc;
+    SPX_HIP(end);
+    SPX_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    return SPX_OK;
+}
+}  // namespace
+
+int fit_graph_create(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                     const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions& opt,
+                     int32_t K, const int32_t* nbr, int32_t nb, float* params, double* centers, float* rms,
+                     int16_t* iters, uint8_t* status, SpxFitGraph** out) {
+    *out = nullptr;
+    SpxFitGraph* g = new SpxFitGraph();
+    const int rc = fit_graph_record(g, img, H, W, pitch, seeds, half, model, init, opt, K, nbr, nb, params, centers,
+                                    rms, iters, status);
+    if (rc != SPX_OK) {
+        fit_graph_free(g);
+        return rc;
+    }
+    *out = g;
+    return SPX_OK;
+}
+
+int fit_graph_launch(SpxFitGraph* g, hipStream_t stream) {
+    int dev = 0;
+    SPX_HIP(hipGetDevice(&dev));
+    if (dev != g->dev) {
+        set_error("spx_fit_graph_launch: the graph was recorded on device " + std::to_string(g->dev) +
+                  ", the current device is " + std::to_string(dev));
+        return SPX_E_ARG;
+    }
+    SPX_HIP(hipGraphLaunch(g->exec, stream));
+    g_last_stats = g->stats;
+    g_last_stats_stream = stream;
+    return SPX_OK;
+}
+
+int fit_graph_destroy(SpxFitGraph* g) {
+    fit_graph_free(g);
+    return SPX_OK;
+}
+
 // ---- the allocation-free fit -------------------------------------------------------------
 // Scratch of spx_fit_batch_noalloc: a file-scope device array, one copy per GPU at its own
 // address. Host code never passes the array's name to a kernel (on the host it names a

@@ -83,6 +83,12 @@ int fit_batch_noalloc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t 
                              const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions& opt,
                              int32_t K, const int32_t* nbr, int32_t nb, float* params, double* centers, float* rms,
                              int16_t* iters, uint8_t* status, hipStream_t stream);
+int fit_graph_create(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                     const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions& opt,
+                     int32_t K, const int32_t* nbr, int32_t nb, float* params, double* centers, float* rms,
+                     int16_t* iters, uint8_t* status, SpxFitGraph** out);
+int fit_graph_launch(SpxFitGraph* g, hipStream_t stream);
+int fit_graph_destroy(SpxFitGraph* g);
 int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
                 int32_t* thresh_out, hipStream_t stream);
 int threshold_cc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
@@ -462,6 +468,33 @@ int spx_fit_batch_noalloc(const uint16_t* img, int64_t H, int64_t W, int64_t pit
     return spx::fit_batch_noalloc_launch(img, H, W, pitch, seeds, half, model, init, *opt, K, neighbours,
                                          neighbours ? nb : 0, params_out, centers_out, rms_out, iters_out, status_out,
                                          spx::as_stream(stream));
+}
+
+int spx_fit_graph_create(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                         const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
+                         int32_t K, const int32_t* neighbours, int32_t nb, float* params_out, double* centers_out,
+                         float* rms_out, int16_t* iters_out, uint8_t* status_out, spx_fit_graph_t* graph_out) {
+    spx::clear_error();
+    SPX_REQUIRE(graph_out != nullptr, "spx_fit_graph_create: graph_out is NULL");
+    *graph_out = nullptr;
+    SPX_REQUIRE(K >= 1, "spx_fit_graph_create: K must be at least 1");
+    int rc = check_fit_args("spx_fit_graph_create", img, H, W, pitch, seeds, half, model, opt, K, neighbours, nb,
+                            params_out, rms_out, iters_out, status_out);
+    if (rc != SPX_OK) return rc;
+    return spx::fit_graph_create(img, H, W, pitch, seeds, half, model, init, *opt, K, neighbours,
+                                 neighbours ? nb : 0, params_out, centers_out, rms_out, iters_out, status_out,
+                                 graph_out);
+}
+
+int spx_fit_graph_launch(spx_fit_graph_t graph, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(graph != nullptr, "spx_fit_graph_launch: graph is NULL");
+    return spx::fit_graph_launch(graph, spx::as_stream(stream));
+}
+
+int spx_fit_graph_destroy(spx_fit_graph_t graph) {
+    spx::clear_error();
+    return spx::fit_graph_destroy(graph);
 }
 
 int spx_otsu_u16(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,

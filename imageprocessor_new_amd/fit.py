@@ -102,19 +102,7 @@ def autotune_launch(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor, 
     return {"best": tuple(best), "ms": ms}
 
 
-def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor,
-                         model: torch.Tensor, init: torch.Tensor | None = None, *,
-                         options: _lib.SpxFitOptions | None = None, out: dict | None = None,
-                         stream: torch.cuda.Stream | None = None,
-                         neighbours: torch.Tensor | None = None) -> dict:
-    """Enqueue spx_fit_batch (spx_fit_batch_cells with ``neighbours``) on device tensors; no
-    host synchronisation.
-
-    img (H, W) uint16, seeds (K, 2) float32, half (K,) int16, model (K,) uint8,
-    init (K, 7) float32 or None, neighbours (K, nb) int32 or None. Returns dict of device
-    tensors params/centers/rms/iters/status (``centers`` (K, 2) float64: the centres unrounded;
-    an ``out`` dict without it leaves them unwritten).
-    """
+def _check_fit_inputs(img, seeds, half, model, init, neighbours) -> None:
     if img.dim() != 2 or img.dtype != torch.uint16:
         raise ValueError("img must be a 2-D uint16 tensor")
     K = int(seeds.shape[0])
@@ -128,14 +116,40 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
     for t in (seeds, half, model) + ((init,) if init is not None else ()):
         if t.device != dev or not t.is_contiguous():
             raise ValueError("all inputs must be contiguous tensors on the image's device")
+    if neighbours is not None and (neighbours.dim() != 2 or neighbours.shape[0] != K
+                                   or neighbours.dtype != torch.int32 or neighbours.device != dev
+                                   or not neighbours.is_contiguous()):
+        raise ValueError("neighbours must be a contiguous (K, nb) int32 tensor on the image's device")
+
+
+def _new_fit_outputs(K: int, dev) -> dict:
+    return {
+        "params": torch.empty((K, 7), dtype=torch.float32, device=dev),
+        "centers": torch.empty((K, 2), dtype=torch.float64, device=dev),
+        "rms": torch.empty((K,), dtype=torch.float32, device=dev),
+        "iters": torch.empty((K,), dtype=torch.int16, device=dev),
+        "status": torch.empty((K,), dtype=torch.uint8, device=dev),
+    }
+
+
+def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor,
+                         model: torch.Tensor, init: torch.Tensor | None = None, *,
+                         options: _lib.SpxFitOptions | None = None, out: dict | None = None,
+                         stream: torch.cuda.Stream | None = None,
+                         neighbours: torch.Tensor | None = None) -> dict:
+    """Enqueue spx_fit_batch (spx_fit_batch_cells with ``neighbours``) on device tensors; no
+    host synchronisation.
+
+    img (H, W) uint16, seeds (K, 2) float32, half (K,) int16, model (K,) uint8,
+    init (K, 7) float32 or None, neighbours (K, nb) int32 or None. Returns dict of device
+    tensors params/centers/rms/iters/status (``centers`` (K, 2) float64: the centres unrounded;
+    an ``out`` dict without it leaves them unwritten).
+    """
+    _check_fit_inputs(img, seeds, half, model, init, None)
+    K = int(seeds.shape[0])
+    dev = img.device
     if out is None:
-        out = {
-            "params": torch.empty((K, 7), dtype=torch.float32, device=dev),
-            "centers": torch.empty((K, 2), dtype=torch.float64, device=dev),
-            "rms": torch.empty((K,), dtype=torch.float32, device=dev),
-            "iters": torch.empty((K,), dtype=torch.int16, device=dev),
-            "status": torch.empty((K,), dtype=torch.uint8, device=dev),
-        }
+        out = _new_fit_outputs(K, dev)
     opts = options if options is not None else _lib.fit_options()
     if opts.tile_waves == 0:
         tuned = _TUNED.get(_tune_key(dev, K))
@@ -160,6 +174,64 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
               _ptr(out["params"]), _ptr(centers), _ptr(out["rms"]), _ptr(out["iters"]), _ptr(out["status"]),
               stream_handle(stream))
     return out
+
+
+class FitGraph:
+    """One batched fit recorded as a HIP graph (spx_fit_graph_create) and replayed by
+    ``launch``: bucketing, every bucket's persistent kernel on the four forked streams and the
+    joins in one hipGraphLaunch. The graph is bound to these tensors (it keeps references to
+    them): each launch fits what they hold when it runs, so a loop over batches of the same
+    size writes its next seeds, patch sizes or image into them (``copy_``) and launches again.
+    Results equal ``fit_erf_disks_device``'s on the same inputs. The graph owns its scratch and
+    streams; ``close`` (or garbage collection) waits for its launches and frees it.
+    """
+
+    def __init__(self, img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor, model: torch.Tensor,
+                 init: torch.Tensor | None = None, *, options: _lib.SpxFitOptions | None = None,
+                 out: dict | None = None, neighbours: torch.Tensor | None = None):
+        K = int(seeds.shape[0])
+        if K < 1:
+            raise ValueError("a fit graph needs at least one feature")
+        _check_fit_inputs(img, seeds, half, model, init, neighbours)
+        dev = img.device
+        if out is None:
+            out = _new_fit_outputs(K, dev)
+        opts = options if options is not None else _lib.fit_options()
+        if opts.tile_waves == 0:
+            tuned = _TUNED.get(_tune_key(dev, K))
+            if tuned is not None:
+                opts = _with_launch(opts, *tuned)
+        self._keep = (img, seeds, half, model, init, neighbours)
+        self.out = out
+        self.options = opts
+        self.device = dev
+        self._handle = ctypes.c_void_p()
+        H, W = img.shape
+        nb = int(neighbours.shape[1]) if neighbours is not None else 0
+        with torch.cuda.device(dev):
+            _lib.call("spx_fit_graph_create", _ptr(img), H, W, img.stride(0), _ptr(seeds), _ptr(half),
+                      _ptr(model), _ptr(init), ctypes.byref(opts), K, _ptr(neighbours), nb,
+                      _ptr(out["params"]), _ptr(out.get("centers")), _ptr(out["rms"]), _ptr(out["iters"]),
+                      _ptr(out["status"]), ctypes.byref(self._handle))
+
+    def launch(self, stream: torch.cuda.Stream | None = None) -> dict:
+        """Enqueue one replay on ``stream`` (default: the current stream); returns ``out``."""
+        if not self._handle:
+            raise RuntimeError("the fit graph is closed")
+        with torch.cuda.device(self.device):
+            _lib.call("spx_fit_graph_launch", self._handle, stream_handle(stream))
+        return self.out
+
+    def close(self) -> None:
+        if self._handle:
+            _lib.call("spx_fit_graph_destroy", self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def fit_erf_disks(image, seeds, *, half=15, model="disk", init=None, max_iter: int = 30,

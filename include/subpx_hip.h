@@ -264,6 +264,28 @@ int spx_fit_batch_cells(const uint16_t* img, int64_t H, int64_t W, int64_t pitch
                         double* centers_out, float* rms_out, int16_t* iters_out, uint8_t* status_out,
                         spx_stream_t stream);
 
+/* A batched fit recorded as one HIP graph: the bucketing kernels, every bucket's persistent
+ * kernel on the four forked streams and the joins, replayed by one hipGraphLaunch. No
+ * reference counterpart: the reference launches each CuPy kernel from Python per iteration
+ * (subpx/_gpu/deblend.py:243-268 runs lm_fit once per batch of cells); a caller that fits
+ * batch after batch of the same size with the same buffers (a page loop writing new seeds,
+ * patch sizes or image contents into them) records the batch once and replays it.
+ * spx_fit_graph_create takes spx_fit_batch_cells's arguments (neighbours may be NULL)
+ * except the stream, records without running anything, and binds the graph to these
+ * buffers: a launch reads and writes them as they are when the graph runs. The graph owns
+ * its scratch, counters, capture and helper streams, so its launches never touch the
+ * workspace of other calls; launches of one graph must not overlap (on one stream they are
+ * ordered). Results equal spx_fit_batch_cells's on the same inputs; spx_fit_stats after a
+ * launch reads the graph's counters. K >= 1. */
+typedef struct SpxFitGraph* spx_fit_graph_t;
+int spx_fit_graph_create(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const float* seeds,
+                         const int16_t* half, const uint8_t* model, const float* init, const SpxFitOptions* opt,
+                         int32_t K, const int32_t* neighbours, int32_t nb, float* params_out, double* centers_out,
+                         float* rms_out, int16_t* iters_out, uint8_t* status_out, spx_fit_graph_t* graph_out);
+int spx_fit_graph_launch(spx_fit_graph_t graph, spx_stream_t stream);
+/* Waits for the graph's launches, then frees it (NULL is ignored). */
+int spx_fit_graph_destroy(spx_fit_graph_t graph);
+
 /* Largest batch of spx_fit_batch_noalloc. */
 #define SPX_FIT_NOALLOC_MAX_FEATURES (1 << 20)
 
