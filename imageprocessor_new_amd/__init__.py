@@ -44,6 +44,16 @@ def __getattr__(name):
         from . import holes
 
         return getattr(holes, name)
+    if name in ("TiffKey", "parse_tiff_key", "build_tiff_index", "iter_pages_across_vertical", "iter_pages_canonical",
+                "map_centers_to_global", "true_runs", "cell_runs_to_global_rects", "process_stripe_board",
+                "PageStream"):
+        from . import mosaic
+
+        return getattr(mosaic, name)
+    if name in ("save_centers", "load_centers"):
+        from . import io as centers_io
+
+        return getattr(centers_io, name)
     if name == "refine_centers_logquad_gpu_match_cpu":
         from . import refine
 

@@ -160,6 +160,12 @@ SIGNATURES = {
                                     _I32, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]),
     "spx_fit_graph_launch": (_I32, [_P, _P]),
     "spx_fit_graph_destroy": (_I32, [_P]),
+    "spx_tiff_open": (_I32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "spx_tiff_page_count": (_I32, [_P, ctypes.POINTER(ctypes.c_int32)]),
+    "spx_tiff_page_info": (_I32, [_P, _I32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.POINTER(ctypes.c_int32)]),
+    "spx_tiff_read_page": (_I32, [_P, _I32, _P, _I64]),
+    "spx_tiff_close": (_I32, [_P]),
     "spx_seed_neighbours": (_I32, [_P, _I32, _I64, _I64, ctypes.c_float, _I32, _P, _P]),
     "spx_triangle_u8": (_I32, [_P, _I64, _I64, _I64, _I32, ctypes.POINTER(ctypes.c_int32), _P]),
     "spx_threshold_mask": (_I32, [_P, _I64, _I64, _I64, _I32, _I32, _P, _P]),

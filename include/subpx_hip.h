@@ -504,6 +504,23 @@ int spx_outline_stack(const float* image, int64_t H, int64_t W, int64_t pitch, c
 int spx_outline_far_splat(const double* c, const double* a, const double* delta, int32_t n, int32_t mx,
                           int32_t my, int32_t hp, int32_t wp, double* grid, spx_stream_t stream);
 
+/* On-disk pages. The reference reads stripe pages with tifffile (subpx/mosaic.py:51-79,
+ * iter_pages_across_vertical / iter_pages_canonical: every page of every file, in file order,
+ * page.asarray()); these entry points replace those reads. spx_tiff_open parses a TIFF
+ * (classic or BigTIFF, either byte order) and lists its pages; a page must hold one grey sample
+ * per pixel, 8- or 16-bit unsigned, in strips or tiles, uncompressed, PackBits or Deflate, with
+ * or without the horizontal-differencing predictor (else SPX_E_UNSUPPORTED naming the tag).
+ * spx_tiff_read_page decodes page `page` into host memory dst (rows dst_pitch_bytes apart;
+ * pinned memory lets the caller's upload to the device overlap the next read): the stored
+ * samples, native byte order, no photometric inversion (as tifffile's asarray). Host only; a
+ * handle is used by one thread at a time. */
+typedef struct SpxTiff* spx_tiff_t;
+int spx_tiff_open(const char* path, spx_tiff_t* out);
+int spx_tiff_page_count(spx_tiff_t tiff, int32_t* n_out);
+int spx_tiff_page_info(spx_tiff_t tiff, int32_t page, int64_t* H, int64_t* W, int32_t* bits);
+int spx_tiff_read_page(spx_tiff_t tiff, int32_t page, void* dst, int64_t dst_pitch_bytes);
+int spx_tiff_close(spx_tiff_t tiff);
+
 #ifdef __cplusplus
 }
 #endif
