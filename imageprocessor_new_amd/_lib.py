@@ -160,6 +160,10 @@ SIGNATURES = {
                                     _I32, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]),
     "spx_fit_graph_launch": (_I32, [_P, _P]),
     "spx_fit_graph_destroy": (_I32, [_P]),
+    "spx_voronoi_labels": (_I32, [_P, _I32, _I64, _I64, _P, _P, _P]),
+    "spx_voronoi_rois": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _P]),
+    "spx_radial_symmetry": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P]),
+    "spx_isophote_curvature": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, ctypes.c_double, _P, _P, _P]),
     "spx_tiff_open": (_I32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
     "spx_tiff_page_count": (_I32, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "spx_tiff_page_info": (_I32, [_P, _I32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),

@@ -19,7 +19,10 @@ Refinement methods (reference _detect_single_tile_gpu, subpx/_gpu/centers.py:163
   * "auto": per component, "logquad" when max(w, h) <= small_feature_max, else
     "edge_gradmoment" (reference subpx/_cpu/centers.py:22-30);
   * "erf_disk" / "erf_ellipse": batched erf-edge disk / ellipse LM fits seeded at the
-    component centroid (no reference counterpart; see fit.py).
+    component centroid (no reference counterpart; see fit.py);
+  * "radial_symmetry" / "isophote_curvature": the Voronoi-cell methods (voronoi.py, reference
+    :1545-1630): components off the image border, one cell ROI each, ``meta`` carries the
+    residual / spread and the cell areas.
 Centres are emitted method group by method group in the reference's order (logquad,
 edge_gradmoment, edge_erf, weighted, none), each group in component order, keeping only
 accepted, finite centres.
@@ -115,7 +118,7 @@ def _refine_rows(img_t, labels, rows_np, cents_np, method, *, pad, fit_half, fit
 
 def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, refine, connectivity,
                         pre_threshold, fit_half, fit_options, small_feature_max=12.0, logquad_variant="gpu",
-                        morph_open=0, morph_close=0, fit_cells=True):
+                        morph_open=0, morph_close=0, fit_cells=True, upsample_factor=4):
     """Segmentation -> CC -> area filter -> refinement on one device image (or strip view).
     info["stage_ms"] holds wall times of the stages (each ends with its results on the host)."""
     t0 = time.perf_counter()
@@ -143,6 +146,9 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
     area = stats_np[:, 4]
     keep = np.nonzero((area >= int(area_min)) & (area <= int(area_max)))[0]
     keep = keep[keep >= 1]
+    if refine in _VORONOI_METHODS:
+        return _voronoi_tile(img_t, stats_np, cents_np, keep, refine, pad=pad, upsample_factor=upsample_factor,
+                             info={"threshold_value": thresh, "num_labels": num}, t=(t0, t1, t2))
     rows = np.concatenate([stats_np[keep, :4], keep[:, None]], axis=1).astype(np.int64)
     methods = _row_methods(refine, rows, small_feature_max)
     out = []
@@ -170,6 +176,31 @@ def _detect_single_tile(img_t, *, threshold, invert, area_min, area_max, pad, re
     info["stage_ms"] = {"threshold": 1e3 * (t1 - t0), "components": 1e3 * (t2 - t1), "rows": 1e3 * (t3 - t2),
                         "refine": 1e3 * (t4 - t3)}
     return centers, info
+
+
+def _voronoi_tile(img_t, stats_np, cents_np, keep, refine, *, pad, upsample_factor, info, t):
+    """The Voronoi dispatch of the reference's single-tile path (subpx/_gpu/centers.py:1545-1630):
+    components off the image border seed the cells; centres, the quality measure and the cell
+    areas of the finite centres, in component order."""
+    from .voronoi import QUALITY_KEYS, refine_rows_device
+
+    t0, t1, t2 = t
+    H, W = img_t.shape
+    st = stats_np[keep].astype(np.int64)
+    inside = (st[:, 0] > 0) & (st[:, 1] > 0) & (st[:, 0] + st[:, 2] < W) & (st[:, 1] + st[:, 3] < H)
+    keep = keep[inside]
+    rows = np.concatenate([stats_np[keep, :4].astype(np.float64), cents_np[keep].astype(np.float64)], axis=1)
+    t3 = time.perf_counter()
+    centers, quality, areas = refine_rows_device(img_t, rows, refine, pad=int(pad),
+                                                 upsample_factor=int(upsample_factor))
+    fin = np.isfinite(centers).all(axis=1)
+    t4 = time.perf_counter()
+    info = dict(info)
+    info.update({"n_candidates": int(keep.size), "n_refined": int(fin.sum()), "upsample_factor": int(upsample_factor),
+                 QUALITY_KEYS[refine]: quality[fin], "voronoi_cell_area": areas[fin],
+                 "stage_ms": {"threshold": 1e3 * (t1 - t0), "components": 1e3 * (t2 - t1), "rows": 1e3 * (t3 - t2),
+                              "refine": 1e3 * (t4 - t3)}})
+    return centers[fin], info
 
 
 def _erf_strip_device(img_t, thresh, *, invert, area_min, area_max, connectivity, fit_half, fit_options,
@@ -290,8 +321,9 @@ def detect_centers(
 ) -> CenterResult:
     """Detect and refine centres (reference subpx/centers.py:130-220).
 
-    Same keywords and CenterResult as the reference; ``gpu_batch``, ``use_float64``,
-    ``components_backend`` and ``upsample_factor`` are accepted for signature compatibility
+    Same keywords and CenterResult as the reference; ``upsample_factor`` sets the Voronoi
+    refinements' upsampling; ``gpu_batch``, ``use_float64`` and ``components_backend`` are
+    accepted for signature compatibility
     (the kernels need no batching and always refine in FP64 / FP32-Jacobian + FP64 solve;
     components always come from the GPU kernel, numbered as OpenCV numbers them).
     ``small_feature_max`` splits ``refine="auto"``. ``fit_half``, ``fit_max_iter``,
@@ -307,13 +339,11 @@ def detect_centers(
     b = resolve_backend(backend)
     if logquad_variant not in ("gpu", "cpu"):
         raise ValueError("logquad_variant must be 'gpu' or 'cpu'")
-    if refine in _VORONOI_METHODS:
-        raise NotImplementedError(f"refine='{refine}' (Voronoi methods) is not implemented")
     if background not in (None, "model"):
         raise ValueError("background must be None or 'model'")
     if background == "model":
         raise NotImplementedError("background='model' (deblending) is not implemented")
-    if refine not in _SUPPORTED:
+    if refine not in _SUPPORTED and refine not in _VORONOI_METHODS:
         raise ValueError(f"Unsupported refine method: {refine}")
     if isinstance(image, torch.Tensor):
         shape = tuple(image.shape)
@@ -325,7 +355,8 @@ def detect_centers(
     common = dict(threshold=threshold, invert=invert, area_min=area_min, area_max=area_max, pad=pad,
                   refine=refine, connectivity=connectivity, fit_half=fit_half, fit_options=fit_options,
                   small_feature_max=small_feature_max, logquad_variant=logquad_variant,
-                  morph_open=int(morph_open), morph_close=int(morph_close), fit_cells=bool(fit_cells))
+                  morph_open=int(morph_open), morph_close=int(morph_close), fit_cells=bool(fit_cells),
+                  upsample_factor=int(upsample_factor))
     base_meta = {
         "invert": bool(invert), "area_min": int(area_min), "area_max": int(area_max),
         "morph_open": int(morph_open), "morph_close": int(morph_close), "pad": int(pad),

@@ -89,6 +89,16 @@ int fit_graph_create(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
                      int16_t* iters, uint8_t* status, SpxFitGraph** out);
 int fit_graph_launch(SpxFitGraph* g, hipStream_t stream);
 int fit_graph_destroy(SpxFitGraph* g);
+int voronoi_labels_launch(const double* seeds, int32_t n, int64_t H, int64_t W, int32_t* labels, int32_t* areas,
+                          hipStream_t stream);
+int voronoi_rois_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
+                        const double* rows, int32_t N, int32_t pad, int32_t Hm, int32_t Wm, float* rois,
+                        uint8_t* masks, int32_t* origins, hipStream_t stream);
+int radial_symmetry_launch(const float* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm,
+                           int32_t factor, int32_t margin, double* centers, double* quality, hipStream_t stream);
+int isophote_curvature_launch(const float* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm,
+                              int32_t factor, int32_t margin, double sigma, double* centers, double* quality,
+                              hipStream_t stream);
 int otsu_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,
                 int32_t* thresh_out, hipStream_t stream);
 int threshold_cc_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t thresh,
@@ -495,6 +505,68 @@ int spx_fit_graph_launch(spx_fit_graph_t graph, spx_stream_t stream) {
 int spx_fit_graph_destroy(spx_fit_graph_t graph) {
     spx::clear_error();
     return spx::fit_graph_destroy(graph);
+}
+
+int spx_voronoi_labels(const double* seeds, int32_t n, int64_t H, int64_t W, int32_t* labels, int32_t* areas,
+                       spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(n >= 0 && (n == 0 || seeds != nullptr), "spx_voronoi_labels: bad seeds");
+    SPX_REQUIRE(H > 0 && W > 0, "spx_voronoi_labels: bad image shape");
+    SPX_REQUIRE(labels != nullptr || areas != nullptr, "spx_voronoi_labels: no output");
+    return spx::voronoi_labels_launch(seeds, n, H, W, labels, areas, spx::as_stream(stream));
+}
+
+int spx_voronoi_rois(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
+                     const double* rows, int32_t N, int32_t pad, int32_t Hm, int32_t Wm, float* rois, uint8_t* masks,
+                     int32_t* origins, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(N >= 0 && pad >= 0, "spx_voronoi_rois: N < 0 or pad < 0");
+    if (N == 0) return SPX_OK;
+    SPX_REQUIRE(img != nullptr && labels != nullptr && rows != nullptr && rois != nullptr && masks != nullptr &&
+                    origins != nullptr, "spx_voronoi_rois: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && pitch >= W, "spx_voronoi_rois: bad image shape or pitch");
+    SPX_REQUIRE(Hm >= 1 && Wm >= 1, "spx_voronoi_rois: empty ROI stack");
+    return spx::voronoi_rois_launch(img, H, W, pitch, labels, rows, N, pad, Hm, Wm, rois, masks, origins,
+                                    spx::as_stream(stream));
+}
+
+namespace {
+int check_estimator(const char* name, const float* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm,
+                    int32_t factor, int32_t margin, const double* centers, const double* quality) {
+    const std::string n(name);
+    SPX_REQUIRE(N >= 0, n + ": N < 0");
+    if (N == 0) return SPX_OK;
+    SPX_REQUIRE(rois != nullptr && masks != nullptr && centers != nullptr && quality != nullptr, n + ": null pointer");
+    SPX_REQUIRE(Hm >= 1 && Wm >= 1, n + ": empty ROIs");
+    SPX_REQUIRE(factor >= 1 && factor <= 16, n + ": upsample_factor must lie in [1, 16]");
+    SPX_REQUIRE(margin >= 0, n + ": boundary_margin < 0");
+    SPX_REQUIRE((int64_t)Hm * factor * Wm * factor < ((int64_t)1 << 30), n + ": upsampled ROI too large");
+    return SPX_OK;
+}
+}  // namespace
+
+int spx_radial_symmetry(const float* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm,
+                        int32_t upsample_factor, int32_t boundary_margin, double* centers, double* quality,
+                        spx_stream_t stream) {
+    spx::clear_error();
+    int rc = check_estimator("spx_radial_symmetry", rois, masks, N, Hm, Wm, upsample_factor, boundary_margin, centers,
+                             quality);
+    if (rc != SPX_OK || N == 0) return rc;
+    return spx::radial_symmetry_launch(rois, masks, N, Hm, Wm, upsample_factor, boundary_margin, centers, quality,
+                                       spx::as_stream(stream));
+}
+
+int spx_isophote_curvature(const float* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm,
+                           int32_t upsample_factor, int32_t boundary_margin, double pre_smooth_sigma,
+                           double* centers, double* quality, spx_stream_t stream) {
+    spx::clear_error();
+    int rc = check_estimator("spx_isophote_curvature", rois, masks, N, Hm, Wm, upsample_factor, boundary_margin,
+                             centers, quality);
+    if (rc != SPX_OK || N == 0) return rc;
+    SPX_REQUIRE(pre_smooth_sigma >= 0.0 && pre_smooth_sigma <= 8.0, "spx_isophote_curvature: pre_smooth_sigma "
+                "must lie in [0, 8]");
+    return spx::isophote_curvature_launch(rois, masks, N, Hm, Wm, upsample_factor, boundary_margin, pre_smooth_sigma,
+                                          centers, quality, spx::as_stream(stream));
 }
 
 int spx_otsu_u16(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, int32_t step,

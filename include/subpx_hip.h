@@ -504,6 +504,37 @@ int spx_outline_stack(const float* image, int64_t H, int64_t W, int64_t pitch, c
 int spx_outline_far_splat(const double* c, const double* a, const double* delta, int32_t n, int32_t mx,
                           int32_t my, int32_t hp, int32_t wp, double* grid, spx_stream_t stream);
 
+/* Voronoi-cell refinements of the GPU detect path (refine="radial_symmetry" /
+ * "isophote_curvature", reference subpx/_gpu/centers.py:1545-1630).
+ * spx_voronoi_labels replaces compute_voronoi_labels_gpu (subpx/_gpu/voronoi.py:116-145):
+ * labels (H, W) int32 (device, or NULL) = index of the nearest of the n seeds (device (n, 2)
+ * f64 [x, y]) by FP64 squared distance, exact ties to the lowest index, -1 everywhere without
+ * seeds; areas (n) int32 (device, or NULL) = each seed's pixel count. */
+int spx_voronoi_labels(const double* seeds, int32_t n, int64_t H, int64_t W, int32_t* labels, int32_t* areas,
+                       spx_stream_t stream);
+/* Replaces _extract_voronoi_rois (subpx/_cpu/cells.py:34-84) on the device: for component row j
+ * of rows (device (N, 6) f64 [x, y, w, h, cx, cy]) the ROI bbox +- pad clipped to the image, the
+ * cell = ROI pixels whose label is j, fill = 10th percentile (NumPy linear) of the cell's border
+ * pixels (on the ROI frame or with a 4-neighbour outside the cell); rois (N, Hm, Wm) f32 = grey
+ * levels in the cell, the fill elsewhere (padding too), masks (N, Hm, Wm) u8 = the cell,
+ * origins (N, 2) int32 [x0, y0]; Hm, Wm = the largest ROI's size (the caller's). ROI stacks
+ * beyond about 12,000 px per ROI: SPX_E_UNSUPPORTED. */
+int spx_voronoi_rois(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
+                     const double* rows, int32_t N, int32_t pad, int32_t Hm, int32_t Wm, float* rois, uint8_t* masks,
+                     int32_t* origins, spx_stream_t stream);
+/* Replace _radial_symmetry_batch_gpu / _isophote_curvature_batch_gpu (subpx/_gpu/centers.py:
+ * 978-1230, 1233-1469): centres of the N ROIs of a stack (device rois (N, Hm, Wm) f32, masks u8)
+ * upsampled by upsample_factor (Keys' cubic a = -0.5, edge-aligned; masks nearest), masks eroded
+ * by boundary_margin (capped as the reference caps it); centers (N, 2) f64 ROI px [x, y] (NaN
+ * outside the ROI), quality (N) f64: the radial-symmetry residual / the isophote votes' spread
+ * (inf outside). */
+int spx_radial_symmetry(const float* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm,
+                        int32_t upsample_factor, int32_t boundary_margin, double* centers, double* quality,
+                        spx_stream_t stream);
+int spx_isophote_curvature(const float* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm,
+                           int32_t upsample_factor, int32_t boundary_margin, double pre_smooth_sigma,
+                           double* centers, double* quality, spx_stream_t stream);
+
 /* On-disk pages. The reference reads stripe pages with tifffile (subpx/mosaic.py:51-79,
  * iter_pages_across_vertical / iter_pages_canonical: every page of every file, in file order,
  * page.asarray()); these entry points replace those reads. spx_tiff_open parses a TIFF
