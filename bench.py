@@ -322,7 +322,8 @@ def main() -> int:
             "mean_iterations": float(iters.mean()),
             "front_end_ms": front_ms,
             "launch_autotune": {"best_waves_per_workgroup": tune["best"][0],
-                                "best_persistent": bool(tune["best"][1]), "ms": tune["ms"]},
+                                "best_persistent": bool(tune["best"][1]), "bucket_waves": tune["best"][2],
+                                "ms": tune["ms"], "bucket_ms": tune.get("bucket_ms")},
             "plain_lm_features_per_s": fitted_per_s,
             "learned_start": learned_leg,
             "median_err_vs_truth_px": float(np.median(d_true)),
@@ -609,7 +610,7 @@ def run_uneven(args) -> int:
             "seeds": "true centres rounded to the pixel grid (patch placement); moment start",
             "counted_fraction": acc_frac, "mean_iterations": float(iters.mean()),
             "status_counts": np.bincount(status, minlength=7).tolist(),
-            "launch_autotune": {"best": list(tune["best"]), "ms": tune["ms"]},
+            "launch_autotune": {"best": list(tune["best"]), "ms": tune["ms"], "bucket_ms": tune.get("bucket_ms")},
             "buckets": buckets, "graph": graph, "median_err_vs_truth_px": float(np.median(d_true)),
             "accuracy": None if acc is None else acc["legs"]["plain"],
         },
@@ -753,7 +754,7 @@ def run_learned(args) -> int:
             "train": {k: tinfo[k] for k in ("samples", "pages", "page_seeds", "holes_seen", "data_seconds",
                                             "train_seconds", "epochs", "batch", "lr")},
             "train_loss_first_last": [tinfo["losses"][0], tinfo["losses"][-1]],
-            "launch_autotune": {"best": list(tune["best"]), "ms": tune["ms"]},
+            "launch_autotune": {"best": list(tune["best"]), "ms": tune["ms"], "bucket_ms": tune.get("bucket_ms")},
         },
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS,

@@ -52,6 +52,7 @@ class SpxFitOptions(ctypes.Structure):
         ("persistent", ctypes.c_int32),
         ("bucket_mask", ctypes.c_int32),
         ("learned_iters", ctypes.c_int32),
+        ("bucket_waves", ctypes.c_int32),
     ]
 
 
@@ -251,10 +252,11 @@ def call(name: str, *args) -> int:
 
 
 def fit_options(*, max_iter=30, step_tol=1e-6, lambda0=1e-3, lambda_up=10.0, lambda_down=0.1,
-                init_sigma=1.0, tile_waves=0, persistent=1, bucket_mask=0, learned_iters=0) -> SpxFitOptions:
+                init_sigma=1.0, tile_waves=0, persistent=1, bucket_mask=0, learned_iters=0,
+                bucket_waves=0) -> SpxFitOptions:
     return SpxFitOptions(int(max_iter), float(step_tol), float(lambda0), float(lambda_up),
                          float(lambda_down), float(init_sigma), int(tile_waves), int(persistent),
-                         int(bucket_mask), int(learned_iters))
+                         int(bucket_mask), int(learned_iters), int(bucket_waves))
 
 
 def bucket_mask_of(half, model) -> int:

@@ -139,16 +139,12 @@ using namespace fitdev;
 
 // Every wave compacts its patches in NPL x 64 words of dynamic LDS sized at launch for the
 // workgroup's waves, so small workgroups do not reserve LDS for absent waves.
-template <int NPL, int P>
-__global__ __launch_bounds__(256) void erf_fit_kernel(FitArgs a) {
-    extern __shared__ uint32_t s_words[];
-    erf_fit_wave<NPL, P>(a, s_words + (threadIdx.x >> 6) * NPL * spx::fitdev::kWave);
-}
-
-// The LDS-resident disk buckets need a few registers more than three waves per SIMD allow
-// and run faster at three waves with a few dwords spilled outside the pixel loop than at two.
-template <int NPL, int P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void erf_fit_kernel_w3(FitArgs a) {
+// WPE: the waves per SIMD the register allocation must leave room for (2, 3 or 4). The LM
+// state (FP64 solve, sums) takes most registers whatever the patch size; at two waves per SIMD
+// a small patch's fit waits on its dependent FP64 chains, at four the compiler spills. Each
+// bucket kernel is built for all three and the launch picks one (SpxFitOptions.bucket_waves).
+template <int NPL, int P, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void erf_fit_kernel(FitArgs a) {
     extern __shared__ uint32_t s_words[];
     erf_fit_wave<NPL, P>(a, s_words + (threadIdx.x >> 6) * NPL * spx::fitdev::kWave);
 }
@@ -275,17 +271,31 @@ namespace {
 
 using KernelFn = void (*)(FitArgs);
 
-template <int B>
+constexpr int kWaveVariants = 3;  // 2, 3, 4 waves per SIMD
+
+template <int B, int V>
 constexpr KernelFn kernel_of() {
-    constexpr int npl = 4 << (B / 2);
-    constexpr int P = (B & 1) ? 7 : 6;
-    if constexpr (P == 6 && npl > fitdev::kUnrollMax) return &erf_fit_kernel_w3<npl, P>;
-    else return &erf_fit_kernel<npl, P>;
+    return &erf_fit_kernel<4 << (B / 2), (B & 1) ? 7 : 6, 2 + V>;
 }
 
-constexpr KernelFn kKernels[kKernBuckets] = {kernel_of<0>(), kernel_of<1>(), kernel_of<2>(), kernel_of<3>(),
-                                             kernel_of<4>(), kernel_of<5>(), kernel_of<6>(), kernel_of<7>(),
-                                             kernel_of<8>(), kernel_of<9>()};
+template <int B>
+struct BucketKernels {
+    static constexpr KernelFn fn[kWaveVariants] = {kernel_of<B, 0>(), kernel_of<B, 1>(), kernel_of<B, 2>()};
+};
+
+constexpr const KernelFn* kKernels[kKernBuckets] = {
+    BucketKernels<0>::fn, BucketKernels<1>::fn, BucketKernels<2>::fn, BucketKernels<3>::fn, BucketKernels<4>::fn,
+    BucketKernels<5>::fn, BucketKernels<6>::fn, BucketKernels<7>::fn, BucketKernels<8>::fn, BucketKernels<9>::fn};
+
+// Default variant per bucket (index into 2, 3, 4 waves per SIMD) when the options leave it to
+// the library: three waves for the LDS-resident disk buckets (a few dwords spilled outside the
+// pixel loop cost less than the third wave gains), two elsewhere.
+constexpr int kDefaultVariant[kKernBuckets] = {0, 0, 0, 0, 0, 0, 1, 0, 1, 0};
+
+int wave_variant(const SpxFitOptions& opt, int b) {
+    const int code = (opt.bucket_waves >> (2 * b)) & 3;
+    return code == 0 ? kDefaultVariant[b] : code - 1;
+}
 
 // Resident workgroups per CU of each bucket kernel at a given workgroup size (cached per
 // device): the persistent grid is sized to what the chip holds at once.
@@ -295,15 +305,15 @@ size_t lds_bytes(int b, int threads) {
     return (size_t)(threads / fitdev::kWave) * npl * fitdev::kWave * sizeof(uint32_t);
 }
 
-int resident_blocks(int dev, int b, int threads) {
+int resident_blocks(int dev, int b, int v, int threads) {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int>, int> cache;
+    static std::map<std::tuple<int, int, int, int>, int> cache;
     std::lock_guard<std::mutex> lock(mu);
-    auto key = std::make_tuple(dev, b, threads);
+    auto key = std::make_tuple(dev, b, v, threads);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kKernels[b]),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kKernels[b][v]),
                                                      threads, lds_bytes(b, threads)) != hipSuccess ||
         blocks < 1) {
         (void)hipGetLastError();
@@ -427,11 +437,12 @@ int fit_batch_enqueue(void* scratch, unsigned long long* stats, const uint16_t* 
         // Persistent grid: the resident capacity of the chip, never more than features.
         // Waves of an empty bucket read its count and exit.
         // (persistent = 0: one wave per feature, the queue then hands each wave its chunk)
+        const int v = wave_variant(opt, b);
         const int64_t want = ((int64_t)K + waves_per_block - 1) / waves_per_block;
-        const int64_t cap = opt.persistent ? (int64_t)cus * resident_blocks(dev, b, threads) : want;
+        const int64_t cap = opt.persistent ? (int64_t)cus * resident_blocks(dev, b, v, threads) : want;
         const int blocks = (int)(want < cap ? want : cap);
-        hipLaunchKernelGGL(kKernels[b], dim3(blocks), dim3(threads), lds_bytes(b, threads), streams[kForkStream[b]],
-                           a);
+        hipLaunchKernelGGL(kKernels[b][v], dim3(blocks), dim3(threads), lds_bytes(b, threads),
+                           streams[kForkStream[b]], a);
         SPX_LAUNCHED();
     }
     if (forks != nullptr)
@@ -515,7 +526,7 @@ int fit_graph_record(SpxFitGraph* g, const uint16_t* img, int64_t H, int64_t W, 
     SPX_HIP(hipGetDevice(&g->dev));
     // the launch shapes are looked up before the capture (host calls only inside it)
     const int threads = (opt.tile_waves > 0 ? (opt.tile_waves < 4 ? opt.tile_waves : 4) : 4) * fitdev::kWave;
-    for (int b = 0; b < kKernBuckets; ++b) (void)resident_blocks(g->dev, b, threads);
+    for (int b = 0; b < kKernBuckets; ++b) (void)resident_blocks(g->dev, b, wave_variant(opt, b), threads);
     SPX_HIP(hipMalloc(&g->scratch, fit_scratch_bytes(K)));
     SPX_HIP(hipMalloc(reinterpret_cast<void**>(&g->stats), kFitCounters * sizeof(unsigned long long)));
     SPX_HIP(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
@@ -603,7 +614,8 @@ int fit_scratch_init(int dev) {
     int cus = 0;
     SPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     for (int b = 0; b < kKernBuckets; ++b)
-        for (int w = 1; w <= 4; ++w) (void)resident_blocks(dev, b, w * fitdev::kWave);
+        for (int v = 0; v < kWaveVariants; ++v)
+            for (int w = 1; w <= 4; ++w) (void)resident_blocks(dev, b, v, w * fitdev::kWave);
     return SPX_OK;
 }
 

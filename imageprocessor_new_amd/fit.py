@@ -68,19 +68,41 @@ def _tune_key(dev: torch.device, K: int):
     return (dev.index if dev.index is not None else torch.cuda.current_device(), max(int(K), 1).bit_length())
 
 
-def _with_launch(o: _lib.SpxFitOptions, tile_waves: int, persistent: int) -> _lib.SpxFitOptions:
+def _with_launch(o: _lib.SpxFitOptions, tile_waves: int, persistent: int,
+                 bucket_waves: int | None = None) -> _lib.SpxFitOptions:
     return _lib.fit_options(max_iter=o.max_iter, step_tol=o.step_tol, lambda0=o.lambda0, lambda_up=o.lambda_up,
                             lambda_down=o.lambda_down, init_sigma=o.init_sigma, tile_waves=int(tile_waves),
-                            persistent=int(persistent), bucket_mask=o.bucket_mask, learned_iters=o.learned_iters)
+                            persistent=int(persistent), bucket_mask=o.bucket_mask, learned_iters=o.learned_iters,
+                            bucket_waves=o.bucket_waves if bucket_waves is None else int(bucket_waves))
+
+
+WAVES_PER_SIMD = (2, 3, 4)  # register-allocation variants of every bucket kernel
+
+
+def _median_ms(fn, repeats: int) -> float:
+    times = []
+    for _ in range(max(1, int(repeats))):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    return float(np.median(times))
 
 
 def autotune_launch(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor, model: torch.Tensor, *,
                     options: _lib.SpxFitOptions | None = None, neighbours: torch.Tensor | None = None,
-                    candidates=LAUNCH_CANDIDATES, repeats: int = 3) -> dict:
-    """Time the fit's launch configurations (waves per workgroup, persistent grid or one wave
-    per feature) on these features with HIP events and remember the fastest for this device
-    and batch size class: fit_erf_disks_device uses it when options.tile_waves is 0. Results
-    do not depend on the configuration. Returns {"best": (waves, persistent), "ms": {...}}."""
+                    candidates=LAUNCH_CANDIDATES, repeats: int = 3, tune_buckets: bool = True) -> dict:
+    """Time the fit's launch configurations on these features with HIP events and remember the
+    fastest for this device and batch size class; fit_erf_disks_device (and FitGraph) use it
+    when options.tile_waves is 0. Results do not depend on the configuration.
+
+    First the whole launch over ``candidates`` (waves per workgroup, persistent grid or one
+    wave per feature); then, with ``tune_buckets``, every bucket the features fill on its own
+    (bucket_mask = that bucket) at 2, 3 and 4 waves per SIMD, keeping each bucket's fastest
+    register allocation (SpxFitOptions.bucket_waves). Returns {"best": (waves, persistent,
+    bucket_waves), "ms": {...}, "bucket_ms": {bucket: {waves_per_simd: ms}}}."""
     base = options if options is not None else _lib.fit_options()
     K = int(seeds.shape[0])
     out = None
@@ -88,18 +110,31 @@ def autotune_launch(img: torch.Tensor, seeds: torch.Tensor, half: torch.Tensor, 
     for cand in candidates:
         o = _with_launch(base, *cand)
         out = fit_erf_disks_device(img, seeds, half, model, options=o, out=out, neighbours=neighbours)
-        times = []
-        for _ in range(max(1, int(repeats))):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            fit_erf_disks_device(img, seeds, half, model, options=o, out=out, neighbours=neighbours)
-            e1.record()
-            e1.synchronize()
-            times.append(e0.elapsed_time(e1))
-        ms[f"{cand[0]}w_{'persistent' if cand[1] else 'per_feature'}"] = float(np.median(times))
+        ms[f"{cand[0]}w_{'persistent' if cand[1] else 'per_feature'}"] = _median_ms(
+            lambda: fit_erf_disks_device(img, seeds, half, model, options=o, out=out, neighbours=neighbours),
+            repeats)
     best = min(zip(candidates, ms.values()), key=lambda t: t[1])[0]
-    _TUNED[_tune_key(img.device, K)] = tuple(best)
-    return {"best": tuple(best), "ms": ms}
+    bucket_waves = int(base.bucket_waves)
+    bucket_ms = {}
+    if tune_buckets:
+        mask = int(base.bucket_mask) or _lib.bucket_mask_of(half.cpu().numpy(), model.cpu().numpy())
+        for b in range(10):
+            if not (mask >> b) & 1:
+                continue
+            per = {}
+            for v, wps in enumerate(WAVES_PER_SIMD):
+                o = _with_launch(base, *best, bucket_waves=(bucket_waves & ~(3 << (2 * b))) | ((v + 1) << (2 * b)))
+                o.bucket_mask = 1 << b
+                fit_erf_disks_device(img, seeds, half, model, options=o, out=out, neighbours=neighbours)
+                per[wps] = _median_ms(
+                    lambda: fit_erf_disks_device(img, seeds, half, model, options=o, out=out, neighbours=neighbours),
+                    repeats)
+            pick = min(per, key=per.get)
+            bucket_waves = (bucket_waves & ~(3 << (2 * b))) | ((WAVES_PER_SIMD.index(pick) + 1) << (2 * b))
+            bucket_ms[b] = per
+    chosen = (int(best[0]), int(best[1]), int(bucket_waves))
+    _TUNED[_tune_key(img.device, K)] = chosen
+    return {"best": chosen, "ms": ms, "bucket_ms": bucket_ms}
 
 
 def _check_fit_inputs(img, seeds, half, model, init, neighbours) -> None:
@@ -154,7 +189,7 @@ def fit_erf_disks_device(img: torch.Tensor, seeds: torch.Tensor, half: torch.Ten
     if opts.tile_waves == 0:
         tuned = _TUNED.get(_tune_key(dev, K))
         if tuned is not None:
-            opts = _with_launch(opts, *tuned)
+            opts = _with_launch(opts, *tuned[:2], bucket_waves=tuned[2] if opts.bucket_waves == 0 else None)
     H, W = img.shape
     pitch = img.stride(0)
     centers = out.get("centers")
@@ -200,7 +235,7 @@ class FitGraph:
         if opts.tile_waves == 0:
             tuned = _TUNED.get(_tune_key(dev, K))
             if tuned is not None:
-                opts = _with_launch(opts, *tuned)
+                opts = _with_launch(opts, *tuned[:2], bucket_waves=tuned[2] if opts.bucket_waves == 0 else None)
         self._keep = (img, seeds, half, model, init, neighbours)
         self.out = out
         self.options = opts

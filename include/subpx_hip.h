@@ -74,6 +74,11 @@ typedef struct SpxFitOptions {
                              moment start (max_iter attempts) by the same wave in the same
                              launch, and iters_out counts the attempts of both.
                              0: plain fit (init, if given, is just the start) */
+    int32_t bucket_waves; /* bits 2 b..2 b+1: waves per SIMD the register allocation of bucket
+                             b's kernel leaves room for: 0 = library default, 1, 2, 3 = 2, 3, 4
+                             waves (more waves hide the FP64 solve's latency, fewer keep the
+                             LM state out of scratch); fit.autotune_launch picks them.
+                             Results do not depend on it */
 } SpxFitOptions;
 
 /* Options of the edge-erf refinement (defaults of the reference's

@@ -189,8 +189,18 @@ def test_launch_configurations_give_identical_fits_and_autotune_picks_one():
             o = _lib.fit_options(tile_waves=tw, persistent=pers)
             r = fit_erf_disks_device(img, seeds, half, model, options=o)
             outs.append({k: v.cpu().numpy() for k, v in r.items()})
+        # every bucket kernel at 2, 3 and 4 waves per SIMD (register allocation only)
+        for code in (1, 2, 3):
+            o = _lib.fit_options(tile_waves=4, bucket_waves=sum(code << (2 * b) for b in range(10)))
+            r = fit_erf_disks_device(img, seeds, half, model, options=o)
+            outs.append({k: v.cpu().numpy() for k, v in r.items()})
         tuned = autotune_launch(img, seeds, half, model, repeats=2)
-        assert tuned["best"] in LAUNCH_CANDIDATES and len(tuned["ms"]) == len(LAUNCH_CANDIDATES)
+        assert tuned["best"][:2] in LAUNCH_CANDIDATES and len(tuned["ms"]) == len(LAUNCH_CANDIDATES)
+        mask = _lib.bucket_mask_of(hs, arr.ellipse.astype(np.uint8))
+        assert sorted(tuned["bucket_ms"]) == [b for b in range(10) if (mask >> b) & 1]
+        for b, per in tuned["bucket_ms"].items():
+            code = (tuned["best"][2] >> (2 * b)) & 3
+            assert code in (1, 2, 3) and per[(2, 3, 4)[code - 1]] == min(per.values())
         assert _TUNED
         again = {k: v.cpu().numpy() for k, v in fit_erf_disks_device(img, seeds, half, model).items()}
     for o in outs[1:] + [again]:
