@@ -267,7 +267,8 @@ def main() -> int:
                 "fallback_features": int(parts[-1][3]),
                 "mean_iterations": float(li.mean()),
                 "max_centre_diff_vs_plain_px": agree,
-                "train": {k: tinfo[k] for k in ("samples", "pages", "data_seconds", "train_seconds", "epochs")},
+                "train": {k: tinfo[k] for k in ("samples", "tiles", "pages", "data_seconds", "train_seconds",
+                                                "epochs") if k in tinfo},
                 "status_ok": float((ls <= 1).mean()),
             }
             log(f"learned leg: {learned_leg}")
