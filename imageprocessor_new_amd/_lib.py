@@ -161,7 +161,15 @@ SIGNATURES = {
                                     _I32, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]),
     "spx_fit_graph_launch": (_I32, [_P, _P]),
     "spx_fit_graph_destroy": (_I32, [_P]),
-    "spx_voronoi_labels": (_I32, [_P, _I32, _I64, _I64, _P, _P, _P]),
+    "spx_voronoi_labels": (_I32, [_P, _I32, _I64, _I64, _P, _P, _P, _P]),
+    "spx_deblend_init": (_I32, [_P, _I64, _I64, _P, _P, _P, _I32, _I32, _I32, _P, _P, _P]),
+    "spx_gauss_fit": (_I32, [_P, _P, _I32, _I32, _I32, _P, _I32, _P, _P]),
+    "spx_gauss_render": (_I32, [_P, _P, _I32, _I64, _I64, _P, _P]),
+    "spx_deblend_clean": (_I32, [_P, _P, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I64, _I64, _P, _P]),
+    "spx_tile_medians": (_I32, [_P, _P, _P, _I64, _I64, _I32, ctypes.c_double, ctypes.c_double, _P, _P, _P]),
+    "spx_tile_map": (_I32, [_P, _P, _P, _I32, _I32, _I64, _I64, _P, _P]),
+    "spx_cell_refine": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _P, ctypes.c_double, _P, _P]),
+    "spx_deblend_window": (_I32, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _P, _P, _P]),
     "spx_voronoi_rois": (_I32, [_P, _I64, _I64, _I64, _P, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _P]),
     "spx_radial_symmetry": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P]),
     "spx_isophote_curvature": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, ctypes.c_double, _P, _P, _P]),

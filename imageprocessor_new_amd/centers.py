@@ -342,7 +342,23 @@ def detect_centers(
     if background not in (None, "model"):
         raise ValueError("background must be None or 'model'")
     if background == "model":
-        raise NotImplementedError("background='model' (deblending) is not implemented")
+        from .deblend import DEBLEND_METHODS, refine_deblended
+
+        if refine not in DEBLEND_METHODS:
+            raise NotImplementedError(
+                f"background='model' is not supported for refine='{refine}': edge methods read the full "
+                "image rather than per-cell ROIs. Use one of " + ", ".join(sorted(DEBLEND_METHODS)) + ".")
+        first = detect_centers(
+            image, backend=backend, threshold=threshold, invert=invert, area_min=area_min, area_max=area_max,
+            morph_open=morph_open, morph_close=morph_close, pad=pad, refine=refine, connectivity=connectivity,
+            gpu_batch=gpu_batch, device=device, use_float64=use_float64, components_backend=components_backend,
+            small_feature_max=small_feature_max, upsample_factor=upsample_factor, tile_h=tile_h, overlap=overlap,
+            threshold_mode=threshold_mode, otsu_downsample=otsu_downsample, thr_scale=thr_scale,
+            dedupe_eps=dedupe_eps, logquad_variant=logquad_variant)
+        with gpu_device(device) as dev:
+            return refine_deblended(to_device_u16(image, dev), first, refine=refine, invert=bool(invert), pad=int(pad),
+                                    iterations=int(deblend_iterations), upsample_factor=int(upsample_factor),
+                                    tile_h=tile_h, overlap=int(overlap))
     if refine not in _SUPPORTED and refine not in _VORONOI_METHODS:
         raise ValueError(f"Unsupported refine method: {refine}")
     if isinstance(image, torch.Tensor):

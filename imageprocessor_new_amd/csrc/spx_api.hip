@@ -90,7 +90,27 @@ int fit_graph_create(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, c
 int fit_graph_launch(SpxFitGraph* g, hipStream_t stream);
 int fit_graph_destroy(SpxFitGraph* g);
 int voronoi_labels_launch(const double* seeds, int32_t n, int64_t H, int64_t W, int32_t* labels, int32_t* areas,
-                          hipStream_t stream);
+                          double* dist, hipStream_t stream);
+int deblend_init_launch(const double* image, int64_t H, int64_t W, const uint8_t* masks, const int32_t* origins,
+                        const double* seeds, int32_t N, int32_t Hm, int32_t Wm, double* data, double* params,
+                        hipStream_t stream);
+int gauss_fit_launch(const double* data, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm, const double* init,
+                     int32_t max_iter, double* out, hipStream_t stream);
+int gauss_render_launch(const double* pg, const double* radius, int32_t N, int64_t H, int64_t W, double* model,
+                        hipStream_t stream);
+int deblend_clean_launch(const double* image, const double* background, const double* model, const double* pg,
+                         const double* radius, const uint8_t* masks, const int32_t* origins, int32_t N, int32_t Hm,
+                         int32_t Wm, int64_t H, int64_t W, double* out, hipStream_t stream);
+int tile_medians_launch(const double* image, const double* model, const double* dist, int64_t H, int64_t W,
+                        int32_t tile, double model_below, double dist_atleast, double* medians, int64_t* counts,
+                        hipStream_t stream);
+int tile_map_launch(const double* medians, const double* yc, const double* xc, int32_t ny, int32_t nx, int64_t H,
+                    int64_t W, double* out, hipStream_t stream);
+int cell_refine_launch(const double* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm, int32_t method,
+                       const double* amplitude, double frac, double* centers, hipStream_t stream);
+int deblend_window_launch(const double* cleaned, const uint8_t* masks, const double* seeds_local,
+                          const double* reach, int32_t N, int32_t Hm, int32_t Wm, int32_t pad, float* rois,
+                          uint8_t* out_masks, hipStream_t stream);
 int voronoi_rois_launch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,
                         const double* rows, int32_t N, int32_t pad, int32_t Hm, int32_t Wm, float* rois,
                         uint8_t* masks, int32_t* origins, hipStream_t stream);
@@ -508,12 +528,107 @@ int spx_fit_graph_destroy(spx_fit_graph_t graph) {
 }
 
 int spx_voronoi_labels(const double* seeds, int32_t n, int64_t H, int64_t W, int32_t* labels, int32_t* areas,
-                       spx_stream_t stream) {
+                       double* dist, spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(n >= 0 && (n == 0 || seeds != nullptr), "spx_voronoi_labels: bad seeds");
     SPX_REQUIRE(H > 0 && W > 0, "spx_voronoi_labels: bad image shape");
-    SPX_REQUIRE(labels != nullptr || areas != nullptr, "spx_voronoi_labels: no output");
-    return spx::voronoi_labels_launch(seeds, n, H, W, labels, areas, spx::as_stream(stream));
+    SPX_REQUIRE(labels != nullptr || areas != nullptr || dist != nullptr, "spx_voronoi_labels: no output");
+    return spx::voronoi_labels_launch(seeds, n, H, W, labels, areas, dist, spx::as_stream(stream));
+}
+
+namespace {
+int check_cells(const char* name, int32_t N, int32_t Hm, int32_t Wm) {
+    const std::string n(name);
+    SPX_REQUIRE(N >= 0, n + ": N < 0");
+    SPX_REQUIRE(N == 0 || (Hm >= 1 && Wm >= 1 && (int64_t)Hm * Wm < ((int64_t)1 << 28)), n + ": bad cell stack size");
+    return SPX_OK;
+}
+}  // namespace
+
+int spx_deblend_init(const double* image, int64_t H, int64_t W, const uint8_t* masks, const int32_t* origins,
+                     const double* seeds, int32_t N, int32_t Hm, int32_t Wm, double* data, double* params,
+                     spx_stream_t stream) {
+    spx::clear_error();
+    int rc = check_cells("spx_deblend_init", N, Hm, Wm);
+    if (rc != SPX_OK || N == 0) return rc;
+    SPX_REQUIRE(image != nullptr && masks != nullptr && origins != nullptr && seeds != nullptr && data != nullptr &&
+                    params != nullptr, "spx_deblend_init: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0, "spx_deblend_init: bad image shape");
+    return spx::deblend_init_launch(image, H, W, masks, origins, seeds, N, Hm, Wm, data, params,
+                                    spx::as_stream(stream));
+}
+
+int spx_gauss_fit(const double* data, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm, const double* init,
+                  int32_t max_iter, double* params_out, spx_stream_t stream) {
+    spx::clear_error();
+    int rc = check_cells("spx_gauss_fit", N, Hm, Wm);
+    if (rc != SPX_OK || N == 0) return rc;
+    SPX_REQUIRE(data != nullptr && masks != nullptr && init != nullptr && params_out != nullptr,
+                "spx_gauss_fit: null pointer");
+    SPX_REQUIRE(max_iter >= 0 && max_iter <= 10000, "spx_gauss_fit: max_iter out of range");
+    return spx::gauss_fit_launch(data, masks, N, Hm, Wm, init, max_iter, params_out, spx::as_stream(stream));
+}
+
+int spx_gauss_render(const double* params_global, const double* radius, int32_t N, int64_t H, int64_t W,
+                     double* model, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(N >= 0 && H > 0 && W > 0 && model != nullptr, "spx_gauss_render: bad arguments");
+    SPX_REQUIRE(N == 0 || (params_global != nullptr && radius != nullptr), "spx_gauss_render: null pointer");
+    return spx::gauss_render_launch(params_global, radius, N, H, W, model, spx::as_stream(stream));
+}
+
+int spx_deblend_clean(const double* image, const double* background, const double* model,
+                      const double* params_global, const double* radius, const uint8_t* masks, const int32_t* origins,
+                      int32_t N, int32_t Hm, int32_t Wm, int64_t H, int64_t W, double* out, spx_stream_t stream) {
+    spx::clear_error();
+    int rc = check_cells("spx_deblend_clean", N, Hm, Wm);
+    if (rc != SPX_OK || N == 0) return rc;
+    SPX_REQUIRE(image != nullptr && background != nullptr && model != nullptr && params_global != nullptr &&
+                    radius != nullptr && masks != nullptr && origins != nullptr && out != nullptr,
+                "spx_deblend_clean: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0, "spx_deblend_clean: bad image shape");
+    return spx::deblend_clean_launch(image, background, model, params_global, radius, masks, origins, N, Hm, Wm, H, W,
+                                     out, spx::as_stream(stream));
+}
+
+int spx_tile_medians(const double* image, const double* model, const double* dist, int64_t H, int64_t W, int32_t tile,
+                     double model_below, double dist_atleast, double* medians, int64_t* counts, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(image != nullptr && model != nullptr && dist != nullptr && medians != nullptr && counts != nullptr,
+                "spx_tile_medians: null pointer");
+    SPX_REQUIRE(H > 0 && W > 0 && tile >= 1 && tile <= 4096, "spx_tile_medians: bad image shape or tile");
+    return spx::tile_medians_launch(image, model, dist, H, W, tile, model_below, dist_atleast, medians, counts,
+                                    spx::as_stream(stream));
+}
+
+int spx_tile_map(const double* medians, const double* yc, const double* xc, int32_t ny, int32_t nx, int64_t H,
+                 int64_t W, double* map, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(medians != nullptr && yc != nullptr && xc != nullptr && map != nullptr, "spx_tile_map: null pointer");
+    SPX_REQUIRE(ny >= 1 && nx >= 1 && H > 0 && W > 0, "spx_tile_map: bad sizes");
+    return spx::tile_map_launch(medians, yc, xc, ny, nx, H, W, map, spx::as_stream(stream));
+}
+
+int spx_cell_refine(const double* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm, int32_t method,
+                    const double* amplitude, double frac, double* centers, spx_stream_t stream) {
+    spx::clear_error();
+    int rc = check_cells("spx_cell_refine", N, Hm, Wm);
+    if (rc != SPX_OK || N == 0) return rc;
+    SPX_REQUIRE(rois != nullptr && masks != nullptr && centers != nullptr, "spx_cell_refine: null pointer");
+    SPX_REQUIRE(method == 0 || method == 1, "spx_cell_refine: method must be 0 (weighted) or 1 (logquad)");
+    return spx::cell_refine_launch(rois, masks, N, Hm, Wm, method, amplitude, frac, centers, spx::as_stream(stream));
+}
+
+int spx_deblend_window(const double* cleaned, const uint8_t* masks, const double* seeds_local, const double* reach,
+                       int32_t N, int32_t Hm, int32_t Wm, int32_t pad, float* rois, uint8_t* masks_out,
+                       spx_stream_t stream) {
+    spx::clear_error();
+    int rc = check_cells("spx_deblend_window", N, Hm, Wm);
+    if (rc != SPX_OK || N == 0) return rc;
+    SPX_REQUIRE(cleaned != nullptr && masks != nullptr && seeds_local != nullptr && reach != nullptr &&
+                    rois != nullptr && masks_out != nullptr, "spx_deblend_window: null pointer");
+    return spx::deblend_window_launch(cleaned, masks, seeds_local, reach, N, Hm, Wm, pad, rois, masks_out,
+                                      spx::as_stream(stream));
 }
 
 int spx_voronoi_rois(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, const int32_t* labels,

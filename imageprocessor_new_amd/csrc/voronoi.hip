@@ -77,7 +77,8 @@ __device__ inline void vscan_cell(const double* seeds, const int32_t* start, con
 // among a wave's 64 consecutive pixels.
 __global__ __launch_bounds__(kThreads) void voronoi_label_kernel(const double* seeds, const int32_t* start,
                                                                  const int32_t* ids, SeedGrid g, int64_t H,
-                                                                 int64_t W, int32_t* labels, int32_t* areas) {
+                                                                 int64_t W, int32_t* labels, int32_t* areas,
+                                                                 double* dist) {
 #pragma clang fp contract(off)
     const int64_t HW = H * W;
     const int64_t idx = blockIdx.x * (int64_t)kThreads + threadIdx.x;
@@ -108,6 +109,7 @@ __global__ __launch_bounds__(kThreads) void voronoi_label_kernel(const double* s
             if (covered || (label >= 0 && best < bound * bound)) break;
         }
         if (labels != nullptr) labels[idx] = label;
+        if (dist != nullptr) dist[idx] = sqrt(best);
     }
     if (areas != nullptr) {
         const int lane = threadIdx.x & 63;
@@ -578,7 +580,7 @@ constexpr size_t kLdsBudget = 60 * 1024;
 }  // namespace
 
 int voronoi_labels_launch(const double* seeds, int32_t n, int64_t H, int64_t W, int32_t* labels, int32_t* areas,
-                          hipStream_t stream) {
+                          double* dist, hipStream_t stream) {
     const int64_t HW = H * W;
     if (areas != nullptr && n > 0) {
         vgrid_zero<<<(n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024, 256, 0, stream>>>(areas, n);
@@ -586,6 +588,10 @@ int voronoi_labels_launch(const double* seeds, int32_t n, int64_t H, int64_t W, 
     }
     if (n == 0) {
         if (labels != nullptr) SPX_HIP(hipMemsetAsync(labels, 0xff, sizeof(int32_t) * (size_t)HW, stream));
+        if (dist != nullptr) {
+            set_error("spx_voronoi_labels: distances need at least one seed");
+            return SPX_E_ARG;
+        }
         return SPX_OK;
     }
     SeedGrid g;
@@ -622,7 +628,8 @@ int voronoi_labels_launch(const double* seeds, int32_t n, int64_t H, int64_t W, 
     vgrid_scatter<<<(n + 255) / 256, 256, 0, stream>>>(seeds, n, g, cursor, ids);
     SPX_LAUNCHED();
     const int64_t blocks = (HW + kThreads - 1) / kThreads;
-    voronoi_label_kernel<<<dim3((unsigned)blocks), kThreads, 0, stream>>>(seeds, start, ids, g, H, W, labels, areas);
+    voronoi_label_kernel<<<dim3((unsigned)blocks), kThreads, 0, stream>>>(seeds, start, ids, g, H, W, labels, areas,
+                                                                          dist);
     SPX_LAUNCHED();
     return SPX_OK;
 }

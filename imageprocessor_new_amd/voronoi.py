@@ -36,7 +36,7 @@ def voronoi_labels_device(seeds: torch.Tensor, shape, *, labels: bool = True, ar
     n = int(seeds.shape[0])
     lab = torch.empty((H, W), dtype=torch.int32, device=dev) if labels else None
     ar = torch.empty((n,), dtype=torch.int32, device=dev) if areas else None
-    _lib.call("spx_voronoi_labels", _ptr(seeds.contiguous()), n, H, W, _ptr(lab), _ptr(ar), stream_handle())
+    _lib.call("spx_voronoi_labels", _ptr(seeds.contiguous()), n, H, W, _ptr(lab), _ptr(ar), None, stream_handle())
     return lab, ar
 
 

@@ -514,9 +514,12 @@ int spx_outline_far_splat(const double* c, const double* a, const double* delta,
  * spx_voronoi_labels replaces compute_voronoi_labels_gpu (subpx/_gpu/voronoi.py:116-145):
  * labels (H, W) int32 (device, or NULL) = index of the nearest of the n seeds (device (n, 2)
  * f64 [x, y]) by FP64 squared distance, exact ties to the lowest index, -1 everywhere without
- * seeds; areas (n) int32 (device, or NULL) = each seed's pixel count. */
+ * seeds; areas (n) int32 (device, or NULL) = each seed's pixel count; dist (H, W) f64 (device,
+ * or NULL; needs n >= 1) = the distance to the nearest seed (with seeds at pixel centres: SciPy's
+ * distance_transform_edt of the seed-free pixels, as the deblending's free-pixel rule takes it,
+ * reference subpx/_cpu/deblend.py:445-451). */
 int spx_voronoi_labels(const double* seeds, int32_t n, int64_t H, int64_t W, int32_t* labels, int32_t* areas,
-                       spx_stream_t stream);
+                       double* dist, spx_stream_t stream);
 /* Replaces _extract_voronoi_rois (subpx/_cpu/cells.py:34-84) on the device: for component row j
  * of rows (device (N, 6) f64 [x, y, w, h, cx, cy]) the ROI bbox +- pad clipped to the image, the
  * cell = ROI pixels whose label is j, fill = 10th percentile (NumPy linear) of the cell's border
@@ -539,6 +542,52 @@ int spx_radial_symmetry(const float* rois, const uint8_t* masks, int32_t N, int3
 int spx_isophote_curvature(const float* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm,
                            int32_t upsample_factor, int32_t boundary_margin, double pre_smooth_sigma,
                            double* centers, double* quality, spx_stream_t stream);
+
+/* Deblending behind detect_centers(background="model") (reference subpx/centers.py:180-316;
+ * subpx/_cpu/deblend.py:48-522, whose GPU twin is subpx/_gpu/deblend.py): the image as a
+ * background map plus one 2-D Gaussian per hole [A, x0, y0, u1, u2, u3] (precision L L^T,
+ * L = [[e^u1, 0], [u2, e^u3]]), alternated with neighbour-cleaned refits. All pointers device;
+ * cells are a (N, Hm, Wm) stack with u8 masks and int32 (N, 2) origins [x0, y0]; images
+ * (H, W) f64. Host code (imageprocessor_new_amd/deblend.py) runs the iteration, the status
+ * rules and the tile interpolation.
+ * spx_deblend_init (:152-171, 467-476): data = image window - the cell border's 10th-percentile
+ * grey level on the cell (0 elsewhere) and the start params (N, 6) (peak, seed, core moment).
+ * spx_gauss_fit (:174-254): one LM fit per cell of data on its mask from init (max_iter steps,
+ * damping lam diag + 1e-12, amplitude in [0, 3 max |data|], log-widths in [-log cap, -log 0.5]).
+ * spx_gauss_render (:74-107): model = sum of the Gaussians (global centres) within radius[j] of
+ * their centre in x and y (64-bit fixed-point sums, order-independent).
+ * spx_deblend_clean (:136-149): window(image) - window(background) - (window(model) - own) on
+ * the cell, 0 elsewhere.
+ * spx_tile_medians (:280-291; subpx/_gpu/deblend.py:364-404): per tile x tile block the count
+ * and median (mean of the middle pair) of image - model over pixels with model < model_below
+ * and dist >= dist_atleast; NaN without pixels. counts int64.
+ * spx_tile_map (:294-320): the map from the (ny, nx) medians: np.interp along x between the
+ * tile centres xc (nx), then linear between the tile rows' centres yc (ny), clamped.
+ * spx_cell_refine (subpx/_gpu/centers.py:33-111): every cleaned cell's centre (ROI px) by the
+ * weighted centroid (method 0) or the log-quadratic fit (1) above its 2-px border median, on
+ * the mask, or with amplitude (N) non-NULL on the mask's pixels above frac x peak (peak = the
+ * amplitude when finite and positive, else the cell's maximum; subpx/centers.py:255-260).
+ * spx_deblend_window (subpx/centers.py:241-252): cleaned cells inside reach + pad of their
+ * seed (ROI px), as an f32 stack and u8 mask for the Voronoi estimators. */
+int spx_deblend_init(const double* image, int64_t H, int64_t W, const uint8_t* masks, const int32_t* origins,
+                     const double* seeds, int32_t N, int32_t Hm, int32_t Wm, double* data, double* params,
+                     spx_stream_t stream);
+int spx_gauss_fit(const double* data, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm, const double* init,
+                  int32_t max_iter, double* params_out, spx_stream_t stream);
+int spx_gauss_render(const double* params_global, const double* radius, int32_t N, int64_t H, int64_t W,
+                     double* model, spx_stream_t stream);
+int spx_deblend_clean(const double* image, const double* background, const double* model,
+                      const double* params_global, const double* radius, const uint8_t* masks, const int32_t* origins,
+                      int32_t N, int32_t Hm, int32_t Wm, int64_t H, int64_t W, double* out, spx_stream_t stream);
+int spx_tile_medians(const double* image, const double* model, const double* dist, int64_t H, int64_t W, int32_t tile,
+                     double model_below, double dist_atleast, double* medians, int64_t* counts, spx_stream_t stream);
+int spx_tile_map(const double* medians, const double* yc, const double* xc, int32_t ny, int32_t nx, int64_t H,
+                 int64_t W, double* map, spx_stream_t stream);
+int spx_cell_refine(const double* rois, const uint8_t* masks, int32_t N, int32_t Hm, int32_t Wm, int32_t method,
+                    const double* amplitude, double frac, double* centers, spx_stream_t stream);
+int spx_deblend_window(const double* cleaned, const uint8_t* masks, const double* seeds_local, const double* reach,
+                       int32_t N, int32_t Hm, int32_t Wm, int32_t pad, float* rois, uint8_t* masks_out,
+                       spx_stream_t stream);
 
 /* On-disk pages. The reference reads stripe pages with tifffile (subpx/mosaic.py:51-79,
  * iter_pages_across_vertical / iter_pages_canonical: every page of every file, in file order,
