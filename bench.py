@@ -125,7 +125,7 @@ def main() -> int:
     ap.add_argument("--budget-sweep", default="1,2,3,4",
                     help="learned workload: also time the learned leg at these attempt budgets (reported only)")
     ap.add_argument("--workload", choices=("learned", "uneven", "fit", "tiled", "holes", "cnn", "detect", "outline",
-                                           "pages"),
+                                           "pages", "voronoi", "deblend"),
                     default="learned",
                     help="learned (default): CNN start + <=4 LM + fallback vs plain LM on the 16384^2 staggered "
                          "array after training on 1M patches; uneven: the plain batched LM over that array's "
@@ -135,7 +135,9 @@ def main() -> int:
                          "cnn: the fused CNN proposal on the 65,536 patches of the 8192^2 array; "
                          "detect: detect_centers(refine='logquad') on the 2048^2 array of 4,096 holes; "
                          "outline: fit_outline (page outline + blur, every hole's size) on a 2048x1378 page of "
-                         "39,042 holes; pages: process_stripe_board over a stack of 2048^2 TIFF pages on disk")
+                         "39,042 holes; pages: process_stripe_board over a stack of 2048^2 TIFF pages on disk; "
+                         "voronoi: refine='radial_symmetry' / 'isophote_curvature' on the 2048^2 array; deblend: "
+                         "detect_centers(background='model') on a dense lattice of Gaussian holes")
     ap.add_argument("--page-size", type=int, default=32768, help="side of the tiled-workload page")
     args = ap.parse_args()
 
@@ -161,6 +163,10 @@ def main() -> int:
         return run_outline(args)
     if args.workload == "pages":
         return run_pages(args)
+    if args.workload == "voronoi":
+        return run_voronoi(args)
+    if args.workload == "deblend":
+        return run_deblend(args)
     if args.workload == "detect":
         return run_detect(args)
 
@@ -1144,6 +1150,182 @@ def run_pages(args) -> int:
             "streamed_equals_serial": same,
         },
         "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def _median_wall(fn, steps: int, warmup: int):
+    import torch
+
+    out = None
+    for _ in range(max(1, warmup)):
+        out = fn()
+    walls = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t)
+    return 1e3 * float(np.median(walls)), out
+
+
+def run_voronoi(args) -> int:
+    """The Voronoi-cell refinements on the 2048 x 2048 array of 4,096 holes (pitch 32, r 8,
+    Poisson + N(0, 30^2)): detect_centers(threshold="otsu", refine="radial_symmetry" /
+    "isophote_curvature", area 20..2000) end to end; the estimator kernels timed alone on the same
+    ROI stack (HIP events); the oracle's NumPy estimators on a 256-ROI sample of that stack as
+    the CPU baseline (one thread)."""
+    import torch
+
+    from imageprocessor_new_amd import synthetic
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.centers import detect_centers
+    from imageprocessor_new_amd.voronoi import _estimate
+    from oracle import voronoi_refine as vr
+
+    arr = synthetic.make_workload("square2048", seed=args.seed)
+    steps = max(1, min(args.steps, 20))
+    kw = dict(backend="gpu", threshold="otsu", area_min=20, area_max=2000, pad=3)
+    legs = {}
+    from scipy.spatial import cKDTree
+
+    with gpu_device(0) as dev:
+        img_t = torch.from_numpy(arr.image).to(dev)
+        for refine in ("radial_symmetry", "isophote_curvature"):
+            ms, res = _median_wall(lambda: detect_centers(img_t, refine=refine, **kw), steps, args.warmup)
+            d, _ = cKDTree(arr.centers_xy).query(res.centers_xy)
+            legs[refine] = {"ms": ms, "centres": int(res.centers_xy.shape[0]), "stage_ms": res.meta.get("stage_ms"),
+                            "median_err_vs_truth_px": float(np.median(d))}
+        # the estimators alone on the page's ROI stack
+        rows = _voronoi_rows(arr.image)
+        from imageprocessor_new_amd.voronoi import voronoi_labels_device, _stack_shape
+        from imageprocessor_new_amd import _lib
+        from imageprocessor_new_amd.backends import stream_handle
+
+        rows_t = torch.from_numpy(rows).to(dev)
+        lab, _ = voronoi_labels_device(rows_t[:, 4:6].contiguous(), arr.image.shape, areas=False)
+        K = rows.shape[0]
+        Hm, Wm = _stack_shape(rows, *arr.image.shape, 3)
+        rois = torch.empty((K, Hm, Wm), dtype=torch.float32, device=dev)
+        masks = torch.empty((K, Hm, Wm), dtype=torch.uint8, device=dev)
+        org = torch.empty((K, 2), dtype=torch.int32, device=dev)
+        H, W = arr.image.shape
+        _lib.call("spx_voronoi_rois", int(img_t.data_ptr()), H, W, W, int(lab.data_ptr()), int(rows_t.data_ptr()), K,
+                  3, Hm, Wm, int(rois.data_ptr()), int(masks.data_ptr()), int(org.data_ptr()), stream_handle())
+        kern = {}
+        for refine in ("radial_symmetry", "isophote_curvature"):
+            _estimate(rois, masks, refine, 4, None, 0.5)
+            kms, _, _ = timed(lambda: _estimate(rois, masks, refine, 4, None, 0.5), steps)
+            kern[refine] = {"kernel_ms": kms, "rois": K, "roi_px": [Hm, Wm], "ns_per_roi": 1e6 * kms / K}
+        legs["kernels"] = kern
+        sample = min(256, K)
+        r_np = rois[:sample].cpu().numpy().astype(np.float64)
+        m_np = masks[:sample].cpu().numpy().astype(bool)
+    cpu = {}
+    for refine, fn in (("radial_symmetry", vr.radial_symmetry), ("isophote_curvature", vr.isophote_curvature)):
+        t = time.perf_counter()
+        fn(r_np, m_np)
+        dt = time.perf_counter() - t
+        cpu[refine] = {"rois_per_s": sample / dt, "seconds": dt}
+    log(f"voronoi: {legs} cpu {cpu}")
+    g = legs["radial_symmetry"]
+    result = {
+        "metric": "features detected and refined per second (detect_centers, refine='radial_symmetry')",
+        "value": g["centres"] / (g["ms"] * 1e-3), "unit": "features/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(1, args.warmup), "ms_per_step": g["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16 in / f32 gradients / f64 sums", "data": "synthetic",
+        "config": {"workload": "2048x2048 uint16 square array, 4,096 holes (pitch 32, r 8, sigma 1.2), Otsu + "
+                               "8-connected components, Voronoi cells, 4x upsampled ROIs",
+                   "legs": legs},
+        "roofline": None,
+        "cpu_baseline": {"value": cpu["radial_symmetry"]["rois_per_s"], "unit": "features/s", "cores": 1,
+                         "kind": "port", "sample": f"the oracle's NumPy estimators on {sample} ROIs of the same stack, "
+                                                   f"one thread (estimators only)", "legs": cpu},
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def _voronoi_rows(img):
+    from oracle import segment
+    from oracle import voronoi_refine as vr
+
+    t = segment.otsu_u16(img)
+    num, _, stats, cents = segment.connected_components_fast(segment.binary_mask(img, t, False, 0, 0), 8)
+    keep = [k for k in range(1, num) if 20 <= stats[k, 4] <= 2000]
+    rows = np.concatenate([stats[keep, :4].astype(np.float64), np.asarray(cents)[keep]], axis=1)
+    return np.ascontiguousarray(vr.voronoi_rows(rows, img.shape))
+
+
+def dense_lattice(side: int, ratio: float = 2.0, fwhm: float = 4.0, seed: int = 0):
+    """A staggered lattice of Gaussian holes (FWHM 4 px, amplitude 2500 counts over 1000, pitch
+    ratio x FWHM, 15 px hole-free margin) as uint16 with Poisson counts."""
+    rng = np.random.default_rng(seed)
+    pitch = ratio * fwhm
+    sigma = fwhm / (2 * np.sqrt(2 * np.log(2)))
+    xs = np.arange(15, side - 15, pitch)
+    ys = np.arange(15, side - 15, pitch)
+    xx, yy = np.meshgrid(xs, ys)
+    xx[1::2] += pitch / 2
+    c = np.column_stack((xx.ravel(), yy.ravel()))
+    c = c[c[:, 0] < side - 15]
+    img = np.full((side, side), 1000.0)
+    r = int(np.ceil(4 * sigma))
+    for cx, cy in c:
+        x0, y0 = int(cx) - r, int(cy) - r
+        y, x = np.mgrid[y0:y0 + 2 * r + 2, x0:x0 + 2 * r + 2]
+        img[y0:y0 + 2 * r + 2, x0:x0 + 2 * r + 2] += 2500.0 * np.exp(-0.5 * ((x - cx) ** 2 + (y - cy) ** 2) / sigma ** 2)
+    return np.clip(rng.poisson(img), 0, 65535).astype(np.uint16), c
+
+
+def run_deblend(args) -> int:
+    """detect_centers(refine="logquad", background="model") on a 1024 x 1024 dense staggered
+    lattice of Gaussian holes (pitch 8 px = 2 FWHM, Poisson counts): detection, then the
+    deblending iterations and the refinement of every cleaned cell. The CPU baseline is the
+    oracle's NumPy deblending of a 256 x 256 crop (one thread), per hole."""
+    import torch
+
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.centers import detect_centers
+    from oracle import deblend as db
+
+    img, truth = dense_lattice(1024, seed=args.seed)
+    steps = max(1, min(args.steps, 10))
+    kw = dict(backend="gpu", threshold="otsu", refine="logquad", area_max=400, logquad_variant="cpu")
+    from scipy.spatial import cKDTree
+
+    with gpu_device(0) as dev:
+        img_t = torch.from_numpy(img).to(dev)
+        ms_plain, plain = _median_wall(lambda: detect_centers(img_t, **kw), steps, args.warmup)
+        ms, res = _median_wall(lambda: detect_centers(img_t, background="model", **kw), steps, args.warmup)
+    e_model = cKDTree(truth).query(res.centers_xy)[0]
+    e_plain = cKDTree(truth).query(plain.centers_xy)[0]
+    crop_img, crop_truth = dense_lattice(256, seed=args.seed)
+    seeds = detect_centers(crop_img, **kw).centers_xy
+    t = time.perf_counter()
+    db.deblended_centers(crop_img.astype(np.float64), seeds, "logquad")
+    dt = time.perf_counter() - t
+    log(f"deblend: {ms:.1f} ms for {len(res.centers_xy)} holes (plain detection {ms_plain:.1f} ms); "
+        f"median error {np.median(e_model):.4f} px (plain {np.median(e_plain):.4f}); cpu {len(seeds) / dt:.0f} holes/s")
+    result = {
+        "metric": "holes detected, deblended and refined per second (detect_centers background='model')",
+        "value": len(res.centers_xy) / (ms * 1e-3), "unit": "features/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(1, args.warmup), "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"1024x1024 uint16 staggered lattice of {len(truth)} Gaussian holes (FWHM 4 px, pitch "
+                               f"8 px), Poisson counts; refine='logquad', background='model'",
+                   "holes": int(len(res.centers_xy)), "plain_detection_ms": ms_plain,
+                   "deblend_iterations": res.meta.get("deblend_iterations_used"),
+                   "status_counts": res.meta.get("deblend_status_counts"),
+                   "background_source": res.meta.get("background_source"),
+                   "median_err_vs_truth_px": float(np.median(e_model)),
+                   "plain_median_err_vs_truth_px": float(np.median(e_plain))},
+        "roofline": None,
+        "cpu_baseline": {"value": len(seeds) / dt, "unit": "features/s", "cores": 1, "kind": "port",
+                         "sample": f"the oracle's NumPy deblending and refinement of a 256x256 crop ({len(seeds)} holes), "
+                                   f"one thread, {dt:.1f} s"},
     }
     print(json.dumps(result), flush=True)
     return 0
