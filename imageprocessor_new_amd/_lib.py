@@ -179,6 +179,23 @@ SIGNATURES = {
                                   ctypes.POINTER(ctypes.c_int32)]),
     "spx_tiff_read_page": (_I32, [_P, _I32, _P, _I64]),
     "spx_tiff_close": (_I32, [_P]),
+    "spx_onepass_smooth": (_I32, [_P, _I64, _I64, _P, _I32, _P, _P, _P]),
+    "spx_onepass_peaks": (_I32, [_P, _I64, _I64, _I32, ctypes.c_double, _P, _P, _P]),
+    "spx_onepass_peak_list": (_I32, [_P, _P, _P, _I64, _I64, ctypes.c_double, _P, _P]),
+    "spx_onepass_count_above": (_I32, [_P, _I64, _I64, _I32, ctypes.c_double, _P, _P]),
+    "spx_onepass_nearest": (_I32, [_P, _I32, _P, _P, _I32, _I32, ctypes.c_double, _P, _P]),
+    "spx_onepass_stack": (_I32, [_P, _I64, _I64, _P, _I32, _P, _I32, _I32, _I32, _P, _P, _P]),
+    "spx_onepass_versus": (_I32, [_P, _I64, _I64, _P, _I32, _P, _I32, _I32, _I32, ctypes.c_double, _P, _P]),
+    "spx_onepass_pair_hist": (_I32, [_P, _P, _P, _P, _P, _I32, _P, _P, _I32, _I32, ctypes.c_double, ctypes.c_double,
+                                     _I32, _I32, _I32, _P, _P]),
+    "spx_onepass_pair_crop": (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _P, _P]),
+    "spx_onepass_outline_hat": (_I32, [_P, _P, _P, _I32, _I32, _I32, ctypes.c_double, _P, _P, _I32, _P, _P]),
+    "spx_onepass_coeff": (_I32, [_P, _I32, _P, _I32, _I32, _P, _I32, _P, _P, _P, _I32, _I32, _P, _P]),
+    "spx_onepass_grid_coeff": (_I32, [_P, _I32, _P, _I32, _I32, _I32, _P, _P, _P, _I32, _I32, _P, _P]),
+    "spx_onepass_synth": (_I32, [_P, _I32, _I32, _I32, _P, _P, _P, _I32, _P, _P, _P, _I32, _I32, _I32, _P, _P]),
+    "spx_onepass_project": (_I32, [_P, _P, _P, _I32, _I32, _I32, _P, _P]),
+    "spx_onepass_grid_costs": (_I32, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _P, _P]),
+    "spx_onepass_band": (_I32, [_P, _I32, _I32, _P, _I32, _I32, _I32, _I32, _I32, ctypes.c_double, _P, _P]),
     "spx_seed_neighbours": (_I32, [_P, _I32, _I64, _I64, ctypes.c_float, _I32, _P, _P]),
     "spx_triangle_u8": (_I32, [_P, _I64, _I64, _I64, _I32, ctypes.POINTER(ctypes.c_int32), _P]),
     "spx_threshold_mask": (_I32, [_P, _I64, _I64, _I64, _I32, _I32, _P, _P]),

@@ -1179,15 +1179,40 @@ def render_outline_page(theta, centers, brightness, shape, *, level: float = 500
         return page
 
 
+_reading_templates: dict = {}
+
+
+def _reading_template(blur, device):
+    """The template the page readings (extents, blur widths) are taken on: fit_outline's first box
+    of the blur form, no coverage sub-samples (outline.py:339-349)."""
+    key = (blur, str(device))
+    if key not in _reading_templates:
+        _reading_templates[key] = OutlineTemplate(upsample=UPSAMPLE, size_px=BLUR_FAMILIES[blur], blur=blur, aa=1,
+                                                  device=device)
+    return _reading_templates[key]
+
+
 def fit_outline_fast(image, seeds_xy=None, *, pixel_size: float = 1.0, unit: str = "px", blur: str = "gauss2",
                      invert: bool = False, n_rays: int = 128, backend: str = "auto", device: int = 0, log=None,
                      trace: dict | None = None, redrawn: bool = False):
-    """The reference's one-pass outline fit (outline.py:361-493) is not built here: its
-    argument checks run with the reference's error texts, then NotImplementedError names
-    ``fit_outline``, which fits the same outline and blur model with a size per hole."""
+    """One outline and one blur for the whole image in one pass (reference outline.py:361-493):
+    the holes are averaged as they are (binned at their sub-pixel offsets), the neighbours' light is
+    put into the model of that average, outline and blur are fitted to it; every hole is then
+    centred against the complete page model and the fit repeated with those centres
+    (``onepass.fit_images``). Page values, no size per hole; GPU only.
+
+    image (H, W): bright holes (``invert=True``: dark holes, reflected as max - image). seeds_xy
+    (N, 2) [x, y], or None for the method's own seed finder. blur: 'gauss2' or 'lens'. n_rays >= 16
+    directions of the page's outline in ``rays_px``. ``trace`` receives the per-page record of the
+    fit (``fit_images``' dict and ``image``: inverted, top, own_seeds); ``redrawn`` is accepted for
+    the signature and adds nothing. Returns ``OutlineFastResult`` (lengths in px)."""
+    from .types import OutlineFastResult
+
+    started = time.perf_counter()
     img = np.asarray(image)
     if img.ndim != 2 or img.size == 0 or not np.issubdtype(img.dtype, np.number) or not np.isfinite(img).all():
         raise ValueError("image must be a nonempty 2D array of finite intensities")
+    seeds = None
     if seeds_xy is not None:
         seeds = np.asarray(seeds_xy, dtype=np.float64).reshape(-1, 2)
         if len(seeds) < MIN_HOLES or not np.isfinite(seeds).all():
@@ -1200,5 +1225,54 @@ def fit_outline_fast(image, seeds_xy=None, *, pixel_size: float = 1.0, unit: str
         raise ValueError("pixel_size must be positive")
     if backend not in ("auto", "gpu", "cpu"):
         raise ValueError("backend must be 'auto', 'gpu' or 'cpu'")
-    raise NotImplementedError("fit_outline_fast (the one-pass outline fit) is not implemented; fit_outline fits "
-                              "the same outline and blur model with a size per hole")
+    if backend == "cpu" or not torch.cuda.is_available():
+        raise RuntimeError("fit_outline_fast needs the GPU; it has no CPU path")
+    resolve_backend("gpu")
+    from . import onepass
+
+    signal = img.astype(np.float64)
+    top = float(signal.max())
+    if invert:
+        signal = top - signal
+    with gpu_device(device) as dev:
+        own = seeds is None
+        if own:
+            img_t = torch.from_numpy(signal).to(device=dev, dtype=torch.float32)
+            seeds = onepass.find_seeds_device(img_t).cpu().numpy()
+            if len(seeds) < MIN_HOLES:
+                raise ValueError(f"the outline fit needs at least {MIN_HOLES} holes; {len(seeds)} were found")
+        res = onepass.fit_images([signal], [seeds], blur=blur, device=device, log=log)[0]
+        t = onepass.lib_template(blur, dev)
+        theta = res["theta"]
+        zero = np.zeros(1)
+        cd = float(t.cd_px(theta))
+        cd_x, cd_y = (float(v[0]) for v in ellipse_widths(t, theta, zero))
+        rays = outline_rays(t, theta, zero, int(n_rays))[0]
+        centers = res["centers_xy"] + np.asarray(t.centroid_px(theta))
+        d = _reading_template(blur, dev).describe(theta, 1.0)
+    if trace is not None:
+        trace.update(res)
+        trace["image"] = dict(inverted=bool(invert), top=top, own_seeds=own)
+    level = top - res["level"] if invert else res["level"]
+    fwhm = tuple(float(v) for v in d["blur_fwhm_nm"])
+    summary = {
+        "blur_family": blur, "outline_cd_px": cd,
+        "outline_extent_x_px": float(d["extent_x_nm"]), "outline_extent_y_px": float(d["extent_y_nm"]),
+        "superellipse_p": float(d["superellipse"]["p"]), "outline_turn_deg": float(d["superellipse"]["phi_deg"]),
+        "harmonics": {k: [float(v) for v in pair] for k, pair in d["harmonics"].items()},
+        "blur_fwhm_px": [float(v) for v in fwhm], "blur_rms_px": [float(v) for v in d["blur_rms_nm"]],
+        "holes_fitted": int(len(seeds)), "holes_stacked": int(np.sum(res["good"])),
+        "spacing_px": float(res["spacing_px"]), "window_px": int(res["window_px"]),
+        "pair_reach_px": float(res["pair_reach_px"]), "noise_sd": float(res["noise_sd"]),
+        "leftover_chi2": float(res["chi2"]) / max(int(res["bins"]), 1),
+        "first_cd_px": float(t.cd_px(res["theta_first"])),
+        "fit_iterations": int(res["iterations"]), "converged": bool(res["converged"]),
+        "exchanged": bool(res["exchanged"]), "per_hole_size": False,
+        "seconds": time.perf_counter() - started, "backend": "gpu", "theta": [float(v) for v in theta],
+        "lengths": "px",
+    }
+    return OutlineFastResult(
+        centers_xy=centers, brightness=res["brightness"], level=level, rms=res["rms"],
+        good=np.asarray(res["good"], dtype=bool), outline_cd_px=cd, cd_x_px=cd_x, cd_y_px=cd_y, rays_px=rays,
+        blur_fwhm_px=fwhm, theta=theta, stack=res["stack"], stack_weight=res["stack_weight"],
+        upsample=int(res["upsample"]), blur=blur, summary=summary, pixel_size=float(pixel_size), unit=str(unit))

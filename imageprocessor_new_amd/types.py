@@ -112,8 +112,7 @@ class OutlineFastResult:
     ``centers_xy``, ``brightness``, ``level``, ``rms``, ``good``; per page: ``outline_cd_px``,
     ``cd_x_px``, ``cd_y_px``, ``rays_px``, ``blur_fwhm_px``, ``theta``, ``blur``, ``stack``,
     ``stack_weight``, ``upsample``, ``summary``. Lengths in pixels; ``pixel_size`` converts
-    them to ``unit``. This package does not compute it (``fit_outline_fast`` raises
-    NotImplementedError); the type keeps imports of the reference's result types working."""
+    them to ``unit``."""
 
     centers_xy: np.ndarray
     brightness: np.ndarray

@@ -606,6 +606,101 @@ int spx_tiff_page_info(spx_tiff_t tiff, int32_t page, int64_t* H, int64_t* W, in
 int spx_tiff_read_page(spx_tiff_t tiff, int32_t page, void* dst, int64_t dst_pitch_bytes);
 int spx_tiff_close(spx_tiff_t tiff);
 
+/* ---- One-pass outline fit (fit_outline_fast: subpx/outline.py:361-493; the device work of
+ * subpx/_gpu/outline_onepass.py and subpx/_outline/onepass.py). Lengths in px, FP64 unless
+ * stated; complex arrays hold (re, im) pairs. The hole pass reuses spx_hole_sweep,
+ * spx_hole_render and spx_far_splat above with this fit's templates. ---- */
+
+/* Separable smoothing of image (H, W) FP32 with the symmetric taps taps[0..radius] and clamped
+ * edges, down the columns first (into tmp) then along the rows (into out); each output is
+ * taps[0] x + sum_{k=radius..1} taps[k] (x[-k] + x[+k]) (find_seeds' gaussian_filter,
+ * outline_onepass.py:474). */
+int spx_onepass_smooth(const float* image, int64_t H, int64_t W, const double* taps, int32_t radius, double* tmp,
+                       double* out, spx_stream_t stream);
+/* Local maxima of sm (H, W): flags (H, W) = not on the frame, > thr and equal to the maximum of
+ * the (2 nms + 1)^2 window with clamped edges; row_counts (H) = peaks per row
+ * (outline_onepass.py:475-480). */
+int spx_onepass_peaks(const double* sm, int64_t H, int64_t W, int32_t nms, double thr, uint8_t* flags,
+                      int32_t* row_counts, spx_stream_t stream);
+/* The peaks in row-major order from row_offsets (H, exclusive prefix of the row counts):
+ * seeds (n, 2) [x, y], each refined by a parabola through log(max(v - base, 1e-6)) of the three
+ * samples along each axis, the vertex clipped to +-0.5 px (outline_onepass.py:490-501). */
+int spx_onepass_peak_list(const double* sm, const uint8_t* flags, const int32_t* row_offsets, int64_t H, int64_t W,
+                          double base, double* seeds, spx_stream_t stream);
+/* count (device u64) = pixels of image[::step, ::step] above thr (outline_onepass.py:473). */
+int spx_onepass_count_above(const float* image, int64_t H, int64_t W, int32_t step, double thr,
+                            unsigned long long* count, spx_stream_t stream);
+/* Distance from every hole c (n, 2) to its nearest other hole among the 3 x 3 cells round its
+ * own (cells of side cs, ncx x ncy; order / cell_start: the holes sorted by cell and the cells'
+ * first positions, ncx ncy + 1 entries); 1e15 when none is there (outline_onepass.py:227-250). */
+int spx_onepass_nearest(const double* c, int32_t n, const int32_t* order, const int32_t* cell_start, int32_t ncx,
+                        int32_t ncy, double cs, double* out, spx_stream_t stream);
+/* Plain stack (outline_onepass.py:81-107, 582-592): every window pixel (half side R) of the
+ * holes with use[i] != 0 inside the frame adds its grey level to the bin rint((p - c) U) + Ms / 2
+ * of the Ms x Ms grid; stack = mean per bin (NaN where none fell), wsum = pixels per bin. The
+ * sums are 64-bit fixed-point integers (2^24 per grey level): the same bits every run. */
+int spx_onepass_stack(const float* image, int64_t H, int64_t W, const double* c, int32_t n, const uint8_t* use,
+                      int32_t R, int32_t U, int32_t Ms, double* stack, double* wsum, spx_stream_t stream);
+/* Every hole against the stack (outline_onepass.py:112-171): out (n, 6) = brightness and level
+ * (2 x 2 least squares over the pixels within sqrt(core2) of the hole), leftover rms over the
+ * window, one Gauss-Newton step (dx, dy) of the centre over the core pixels, pixels compared. */
+int spx_onepass_versus(const float* image, int64_t H, int64_t W, const double* c, int32_t n, const double* stack,
+                       int32_t R, int32_t U, int32_t Ms, double core2, double* out, spx_stream_t stream);
+/* Pair maps before the FFT (outline_onepass.py:184-224): hist (nclass, Ng, Ng); every hole i
+ * with g[i] != 0 adds g_i amp_j at c_j - c_i - s_i (cubic B-spline taps, Ug per px, wrapped)
+ * to the map cls[i] for every hole j of the cells within ceil(reach / cs) of its cell with
+ * |c_j - c_i| <= reach per axis (j = i included). Fixed-point sums (2^40 per count). */
+int spx_onepass_pair_hist(const double* c, const double* amp, const double* g, const int32_t* cls, const double* s,
+                          int32_t n, const int32_t* order, const int32_t* cell_start, int32_t ncx, int32_t ncy,
+                          double cs, double reach, int32_t Ug, int32_t Ng, int32_t nclass, double* hist,
+                          spx_stream_t stream);
+/* gamma (nclass, 2 kmax + 1, kmax + 1) complex = spec[q][ky mod Ng][kx] / (centre x fix) for
+ * ky = -kmax .. kmax, kx = 0 .. kmax, f = k / L (outline_onepass.py:688-698); spec (nclass, Ng,
+ * Ng / 2 + 1) complex, the maps' real FFT. */
+int spx_onepass_pair_crop(const double* spec, int32_t nclass, int32_t Ng, int32_t Ug, int32_t L, int32_t kmax,
+                          double* gamma, spx_stream_t stream);
+/* The outline's transform along its contour and its shape derivatives (onepass.py:129-162,
+ * 281-312): geo (4, rows, n) = [x, y, dx/dphi, dy/dphi], gk (rows, n, K) = r dr/dp_k, area (rows);
+ * out (rows, F, 1 + K) complex at the frequencies (fx, fy) (F). K <= 16. */
+int spx_onepass_outline_hat(const double* geo, const double* gk, const double* area, int32_t rows, int32_t n,
+                            int32_t K, double dphi, const double* fx, const double* fy, int32_t F, double* out,
+                            spx_stream_t stream);
+/* Band coefficients of the template and (tm != NULL) its derivatives along a stage's free
+ * directions (onepass.py:314-355): coeff (B, 1 + nf, F) complex from ohat (B, F, 1 + K), the
+ * pages' blur parameters blur (B, nb) (kind 0: two Gaussians, nb 5; 1: Gaussian x pupil, nb 3;
+ * 2: one Gaussian, nb 2), tm (K + nb, nf) and the pixel box pix (F). nf <= 16. */
+int spx_onepass_coeff(const double* ohat, int32_t K, const double* blur, int32_t kind, int32_t nb, const double* tm,
+                      int32_t nf, const double* fx, const double* fy, const double* pix, int32_t B, int32_t F,
+                      double* coeff, spx_stream_t stream);
+/* The grid of starts' coefficients (onepass.py:453-458): coeff (B, ns nw, F) = O_(b,i) OTF_j pix,
+ * ohat (B ns, F, 1) complex, blur (nw, nb). */
+int spx_onepass_grid_coeff(const double* ohat, int32_t ns, const double* blur, int32_t nw, int32_t kind, int32_t nb,
+                           const double* fx, const double* fy, const double* pix, int32_t B, int32_t F, double* coeff,
+                           spx_stream_t stream);
+/* The stack's model by class (onepass.py:357-369): M (B, C, slots, npts, npts) = Re Ey[qy] G ExT[qx]
+ * with G (nky, nkx) = coeff[b][c][fmap] dbl gamma[b][q] (0 where fmap < 0); slots = the U^2
+ * classes (cls_sel NULL) or the page's class cls_sel[b]. Ey (U, npts, nky), ExT (U, nkx, npts),
+ * gamma (B, U^2, nky, nkx) complex. */
+int spx_onepass_synth(const double* coeff, int32_t B, int32_t C, int32_t F, const int32_t* fmap, const double* dbl,
+                      const double* gamma, int32_t U, const int32_t* cls_sel, const double* Ey, const double* ExT,
+                      int32_t npts, int32_t nky, int32_t nkx, double* M, spx_stream_t stream);
+/* Variable projection per page (onepass.py:371-410): out (B, 3 + nf + nf^2) = [cost, scale,
+ * const, g (nf), A (nf x nf)] of M (B, C = 1 + nf, nbins) against y, w (B, nbins); scale and
+ * constant solved exactly, the derivatives projected off them (nf = 0: [cost, scale, const]). */
+int spx_onepass_project(const double* M, const double* y, const double* w, int32_t B, int32_t C, int32_t nbins,
+                        double* out, spx_stream_t stream);
+/* The grid of starts' costs (onepass.py:459-470): table (B, C), the cost of model c on the bins of
+ * the page's class cls_sel[b] with the best scale and constant; M (B, C, nb), y, w (B, ncls, nb). */
+int spx_onepass_grid_costs(const double* M, const double* y, const double* w, const int32_t* cls_sel, int32_t B,
+                           int32_t C, int32_t ncls, int32_t nb, double* table, spx_stream_t stream);
+/* Analytic templates on a band (onepass.py:52-126): for every polygon row of xy (rows, 2, n) and
+ * blur row (rows, nb), the polygon's exact transform x OTF x sinc(fx) sinc(fy) [x sinc(fx box)
+ * sinc(fy box)] x exp(-2 pi i (fx + fy) (N / 2) / Uu), f = k / L, |kx|, |ky| <= kmax, kx >= 0,
+ * written into full (rows, N, N / 2 + 1) complex (N = L Uu; other entries untouched) for the
+ * inverse real FFT. */
+int spx_onepass_band(const double* xy, int32_t rows, int32_t n, const double* blur, int32_t kind, int32_t nb,
+                     int32_t L, int32_t Uu, int32_t kmax, double box_px, double* full, spx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

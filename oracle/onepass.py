@@ -32,7 +32,7 @@ import math
 import numpy as np
 
 from . import holes as H
-from .outline import Template, stage_map
+from .outline import Template, render_model_page, stage_map
 
 STACK_BINS = 4
 PAIR_BINS = 8
@@ -988,3 +988,20 @@ def bspline_weights(t):
     u = 1.0 - t
     return np.stack([u * u * u / 6, (3 * t ** 3 - 6 * t * t + 4) / 6, (-3 * t ** 3 + 3 * t * t + 3 * t + 1) / 6,
                      t ** 3 / 6], axis=-1)
+
+
+def drawn_page(blur="gauss2", seed=5, noise=3.0, shape=(120, 170), pitch=10.0, seed_error=0.15):
+    """A test page of the outline model (known answer): holes of one outline and blur on a square
+    lattice (+-0.3 px jitter), brightness 900-1100, level 100, N(0, noise^2) noise, drawn with the
+    fit_outline template of the blur's box (24 px gauss2, 48 px lens); seeds within +-seed_error
+    px. Returns (template, theta, centres, page, seeds)."""
+    rng = np.random.default_rng(seed)
+    t = Template(upsample=8, size_px=24 if blur == "gauss2" else 48, blur=blur)
+    theta = t.pack(2.3, 2.2, 2.4, 0.1, [[0.02, -0.01], [0.01, 0.0]],
+                   t.blur_pack(s1x=1.2, s1y=1.1, s2x=2.5, s2y=2.5, w2=0.06, sx=1.1, sy=1.0, cutoff=0.45))
+    X, Y = np.meshgrid(np.arange(6, shape[1], pitch), np.arange(6, shape[0], pitch))
+    c = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.3, 0.3, (X.size, 2))
+    a = rng.uniform(900, 1100, len(c))
+    page = render_model_page(t, theta, c, a, 100.0, np.zeros(len(c)), shape, noise=noise, seed=seed)
+    seeds = c + rng.uniform(-seed_error, seed_error, c.shape)
+    return t, theta, c, page, seeds

@@ -16,7 +16,7 @@ import pytest
 
 from oracle import holes as OH
 from oracle import onepass as OP
-from oracle.outline import Template, render_model_page
+from oracle.outline import Template
 
 PIXEL_NM = 21.0
 
@@ -206,22 +206,9 @@ def test_find_seeds_on_gaussian_blobs():
     assert d.max() < 0.1
 
 
-def _drawn_page(blur, seed=5, noise=3.0):
-    rng = np.random.default_rng(seed)
-    t = Template(upsample=8, size_px=24 if blur == "gauss2" else 48, blur=blur)
-    theta = t.pack(2.3, 2.2, 2.4, 0.1, [[0.02, -0.01], [0.01, 0.0]],
-                   t.blur_pack(s1x=1.2, s1y=1.1, s2x=2.5, s2y=2.5, w2=0.06, sx=1.1, sy=1.0, cutoff=0.45))
-    X, Y = np.meshgrid(np.arange(6, 170, 10.0), np.arange(6, 120, 10.0))
-    c = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.3, 0.3, (X.size, 2))
-    a = rng.uniform(900, 1100, len(c))
-    page = render_model_page(t, theta, c, a, 100.0, np.zeros(len(c)), (120, 170), noise=noise, seed=seed)
-    seeds = c + rng.uniform(-0.15, 0.15, c.shape)
-    return t, theta, c, page, seeds
-
-
 @pytest.mark.parametrize("blur", ["gauss2", "lens"])
 def test_one_pass_fit_recovers_the_drawn_outline(blur):
-    t, theta, c, page, seeds = _drawn_page(blur)
+    t, theta, c, page, seeds = OP.drawn_page(blur)
     res = OP.fit_images([page], [seeds], blur=blur)[0]
     cd = t.cd_px(res["theta"])
     assert cd == pytest.approx(t.cd_px(theta), abs=0.01)

@@ -183,7 +183,9 @@ def test_product_greedy_colours_equal_the_oracle_rule():
     assert np.array_equal(greedy_colours(c, 1.5 * 8.5), O.greedy_colours(c, 1.5 * 8.5))
 
 
-def test_fit_outline_fast_checks_its_arguments_then_says_it_is_not_built():
+def test_fit_outline_fast_checks_its_arguments():
+    """The reference's checks and error texts (tests/test_outline_fit_fast.py:115-126); without a
+    GPU (or with backend='cpu') the method says it needs one."""
     from imageprocessor_new_amd import OutlineFastResult, fit_outline_fast  # noqa: F401
 
     img = np.ones((32, 32))
@@ -196,5 +198,5 @@ def test_fit_outline_fast_checks_its_arguments_then_says_it_is_not_built():
         fit_outline_fast(img, seeds, blur="box")
     with pytest.raises(ValueError, match="n_rays"):
         fit_outline_fast(img, seeds, n_rays=8)
-    with pytest.raises(NotImplementedError, match="fit_outline fits"):
-        fit_outline_fast(img, seeds)
+    with pytest.raises(RuntimeError, match="needs the GPU"):
+        fit_outline_fast(img, seeds, backend="cpu")
