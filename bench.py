@@ -125,7 +125,7 @@ def main() -> int:
     ap.add_argument("--budget-sweep", default="1,2,3,4",
                     help="learned workload: also time the learned leg at these attempt budgets (reported only)")
     ap.add_argument("--workload", choices=("learned", "uneven", "fit", "tiled", "holes", "cnn", "detect", "outline",
-                                           "pages", "voronoi", "deblend"),
+                                           "outline_fast", "pages", "voronoi", "deblend"),
                     default="learned",
                     help="learned (default): CNN start + <=4 LM + fallback vs plain LM on the 16384^2 staggered "
                          "array after training on 1M patches; uneven: the plain batched LM over that array's "
@@ -135,10 +135,12 @@ def main() -> int:
                          "cnn: the fused CNN proposal on the 65,536 patches of the 8192^2 array; "
                          "detect: detect_centers(refine='logquad') on the 2048^2 array of 4,096 holes; "
                          "outline: fit_outline (page outline + blur, every hole's size) on a 2048x1378 page of "
-                         "39,042 holes; pages: process_stripe_board over a stack of 2048^2 TIFF pages on disk; "
+                         "39,042 holes; outline_fast: fit_outline_fast (the one-pass fit) on that page; pages: process_stripe_board over a stack of 2048^2 TIFF pages on disk; "
                          "voronoi: refine='radial_symmetry' / 'isophote_curvature' on the 2048^2 array; deblend: "
                          "detect_centers(background='model') on a dense lattice of Gaussian holes")
     ap.add_argument("--page-size", type=int, default=32768, help="side of the tiled-workload page")
+    ap.add_argument("--fast-batch", type=int, default=8,
+                    help="outline_fast workload: pages of the lockstep batch timed beside the single-page call")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -161,6 +163,8 @@ def main() -> int:
         return run_cnn(args)
     if args.workload == "outline":
         return run_outline(args)
+    if args.workload == "outline_fast":
+        return run_outline_fast(args)
     if args.workload == "pages":
         return run_pages(args)
     if args.workload == "voronoi":
@@ -1525,6 +1529,119 @@ def run_outline(args) -> int:
     }
     print(json.dumps(result), flush=True)
     return 0
+
+
+def run_outline_fast(args) -> int:
+    """fit_outline_fast (the one-pass outline fit: seeds' plain stack with the neighbours' light in
+    its model, a page fit, a hole pass, the page fit again) on the reference's published page
+    shape: 2048 x 1378, 39,042 holes (the outline workload's page). One step = one whole call from
+    seeds on one page; a lockstep batch of pages (the reference's page fits of several images in
+    one call) is timed beside it and reported per page."""
+    import torch
+
+    from imageprocessor_new_amd import onepass as P
+    from imageprocessor_new_amd.backends import gpu_device
+    from imageprocessor_new_amd.outline import fit_outline_fast
+
+    steps = max(1, min(args.steps, 5))
+    warmup = max(1, min(args.warmup, 2))
+    batch = max(1, int(args.fast_batch))
+    with gpu_device(0):
+        page, truth, theta, delta, t = outline_page(args.seed)
+        rng = np.random.default_rng(args.seed + 1)
+        seeds = truth + rng.uniform(-0.2, 0.2, truth.shape)
+        for _ in range(warmup):
+            fit_outline_fast(page, seeds)
+        walls, fit = [], None
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fit = fit_outline_fast(page, seeds)
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+        marks = []
+        P.fit_images([page], [seeds], log=marks.append)
+        pages, pseeds = [page], [seeds]
+        for k in range(1, batch):
+            pg_k, tr_k, _, _, _ = outline_page(args.seed + 10 * k)
+            pages.append(pg_k)
+            pseeds.append(tr_k + np.random.default_rng(args.seed + 10 * k + 1).uniform(-0.2, 0.2, tr_k.shape))
+        batch_ms = None
+        if batch > 1:
+            P.fit_images(pages, pseeds)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            many = P.fit_images(pages, pseeds)
+            torch.cuda.synchronize()
+            batch_ms = 1e3 * (time.perf_counter() - t0) / batch
+        true_xy = truth + np.array(t.centroid_px(theta))
+        true_cd = t.cd_px(theta)
+    n = truth.shape[0]
+    wall = float(np.median(walls))
+    err_xy = np.hypot(*(fit.centers_xy - true_xy).T)
+    log(f"outline_fast: {n} holes, {1e3 * wall:.1f} ms per page (batch of {batch}: {batch_ms} ms per page); "
+        f"outline CD {fit.outline_cd_px:.4f} px (true {true_cd:.4f}, mean hole CD "
+        f"{np.mean([t.cd_px(theta, float(d)) for d in delta]):.4f}); steps {marks}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = outline_fast_cpu_baseline(page, seeds)
+    result = {
+        "metric": "one-pass outline fit (fit_outline_fast): holes per second (page outline + blur, every hole "
+                  "centred, brightness and level)",
+        "value": n / wall,
+        "unit": "holes/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 stack, pair maps, page model and solve / f32 hole-stage templates",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fit_outline_fast on a 2048x1378 page of {n} holes (square array pitch 8.5 px, superellipse "
+                        f"outline CD {true_cd:.3f} px, two-Gaussian blur), seeds within 0.2 px",
+            "holes": n,
+            "outline_cd_px": fit.outline_cd_px,
+            "true_outline_cd_px": true_cd,
+            "first_cd_px": fit.summary["first_cd_px"],
+            "holes_stacked": fit.summary["holes_stacked"],
+            "leftover_chi2": fit.summary["leftover_chi2"],
+            "fit_iterations": fit.summary["fit_iterations"],
+            "exchanged": fit.summary["exchanged"],
+            "median_centre_err_px": float(np.median(err_xy[fit.good])),
+            "steps_s": marks,
+            "walls_s": walls,
+            "batch_pages": batch,
+            "batch_ms_per_page": batch_ms,
+            "batch_cd_px": None if batch_ms is None else [float(t.cd_px(m["theta"])) for m in many],
+            "reference_published_ms_per_page": {"one image per call": [120, 230], "20 pages per call": 80},
+            "reference_published_hardware": "RTX 4060 Ti (docs/outline_fit_one_pass.md:147, 300; "
+                                            "docs/outline_fit.md:192-193)",
+        },
+        "roofline": None,
+        "roofline_note": "a chain of small launches (page-fit evaluations read back per Levenberg-Marquardt step, "
+                         "two hole-pass sweeps, FFTs); per-kernel times in profiles/outline_fast_page2048_kernel_stats.md",
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def outline_fast_cpu_baseline(page, seeds):
+    """The oracle's one-pass fit (FP64 NumPy, one thread) on a 320 x 320 crop of the same page."""
+    from oracle import onepass as OP
+
+    x0, y0, S = 600, 400, 320
+    inside = (seeds[:, 0] >= x0) & (seeds[:, 0] < x0 + S) & (seeds[:, 1] >= y0) & (seeds[:, 1] < y0 + S)
+    crop = page[y0:y0 + S, x0:x0 + S]
+    s = seeds[inside] - np.array([x0, y0])
+    t0 = time.perf_counter()
+    OP.fit_images([crop], [s])
+    dt = time.perf_counter() - t0
+    return {"value": len(s) / dt, "unit": "holes/s", "cores": 1, "kind": "port",
+            "sample": f"oracle one-pass fit (steps A to C) of a {S}x{S} crop ({len(s)} holes) in {dt:.1f} s"}
 
 
 def outline_cpu_baseline(page, seeds, rounds):

@@ -227,6 +227,71 @@ __global__ void stack_finish(const unsigned long long* __restrict__ sums, const 
     wsum[b] = cnt;
 }
 
+// The same sums with the bins of one workgroup's holes in LDS (64-bit fixed-point grey levels and
+// 32-bit counts, LDS atomics), each workgroup's totals then stored as its own partial row
+// (partial[g] = [sums (nb), counts (nb)] as u64): no global atomics, and the integer sums do not
+// depend on the order of the threads. Holes [g per, (g + 1) per) of workgroup g.
+constexpr int kStackLdsBytes = 64 * 1024;
+
+__global__ __launch_bounds__(kBlock) void stack_acc_lds(const float* __restrict__ img, int64_t H, int64_t W,
+                                                        const double* __restrict__ c, int n,
+                                                        const uint8_t* __restrict__ use, int R, int U, int Ms, int per,
+                                                        unsigned long long* __restrict__ partial) {
+    extern __shared__ unsigned long long lds_stack[];
+    const int nb = Ms * Ms;
+    unsigned long long* ssum = lds_stack;
+    unsigned int* scnt = reinterpret_cast<unsigned int*>(lds_stack + nb);
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        ssum[i] = 0ull;
+        scnt[i] = 0u;
+    }
+    __syncthreads();
+    const int side = 2 * R + 1;
+    const int h0 = blockIdx.x * per;
+    const int h1 = h0 + per < n ? h0 + per : n;
+    for (int t = threadIdx.x; t < (h1 - h0) * side; t += blockDim.x) {
+        const int i = h0 + t / side;
+        if (!use[i]) continue;
+        const int oy = t % side - R;
+        const double x = c[2 * i], y = c[2 * i + 1];
+        const int64_t rx = (int64_t)rint(x), ry = (int64_t)rint(y);
+        const int64_t py = ry + oy;
+        if (py < 0 || py >= H) continue;
+        const int64_t by = (int64_t)rint(((double)py - y) * U) + Ms / 2;
+        if (by < 0 || by >= Ms) continue;
+        for (int ox = -R; ox <= R; ++ox) {
+            const int64_t px = rx + ox;
+            if (px < 0 || px >= W) continue;
+            const int64_t bx = (int64_t)rint(((double)px - x) * U) + Ms / 2;
+            if (bx < 0 || bx >= Ms) continue;
+            const int b = (int)(by * Ms + bx);
+            atomicAdd(ssum + b, (unsigned long long)llrint((double)img[py * W + px] * kStackFix));
+            atomicAdd(scnt + b, 1u);
+        }
+    }
+    __syncthreads();
+    unsigned long long* out = partial + (int64_t)blockIdx.x * 2 * nb;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        out[i] = ssum[i];
+        out[nb + i] = (unsigned long long)scnt[i];
+    }
+}
+
+// stack and wsum from the G workgroups' partial rows (sums over g in index order).
+__global__ void stack_reduce(const unsigned long long* __restrict__ partial, int G, int64_t nb,
+                             double* __restrict__ stack, double* __restrict__ wsum) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    unsigned long long s = 0ull, cnt = 0ull;
+    for (int g = 0; g < G; ++g) {
+        s += partial[(int64_t)g * 2 * nb + b];
+        cnt += partial[(int64_t)g * 2 * nb + nb + b];
+    }
+    const double cn = (double)cnt;
+    stack[b] = cnt > 0 ? ((double)(long long)s / kStackFix) / cn : nan_d();
+    wsum[b] = cn;
+}
+
 // Filled stack (NaN -> 0) and its central-difference gradient (one-sided at the edges), bins of
 // 1 / U px.
 __device__ inline double filled(const double* S, int64_t b) {
@@ -330,17 +395,19 @@ __global__ void versus_kernel(const float* __restrict__ img, int64_t H, int64_t 
 }
 
 // ------------------------------------------------------------------------------- pair maps
-__device__ inline void bspline4(double t, double* w) {
+// Weight of tap k (at -1, 0, 1, 2) of the cubic B-spline for the fraction t in [0, 1).
+__device__ inline double bspline_tap(double t, int k) {
     const double u = 1.0 - t;
-    w[0] = u * u * u / 6.0;
-    w[1] = (3.0 * t * t * t - 6.0 * t * t + 4.0) / 6.0;
-    w[2] = (-3.0 * t * t * t + 3.0 * t * t + 3.0 * t + 1.0) / 6.0;
-    w[3] = t * t * t / 6.0;
+    if (k == 0) return u * u * u / 6.0;
+    if (k == 1) return (3.0 * t * t * t - 6.0 * t * t + 4.0) / 6.0;
+    if (k == 2) return (-3.0 * t * t * t + 3.0 * t * t + 3.0 * t + 1.0) / 6.0;
+    return t * t * t / 6.0;
 }
 
 // One wave per hole i with g[i] != 0: every hole j of the cells within ceil(reach / cs) of i's
 // cell with |c_j - c_i| <= reach per axis adds g_i amp_j at c_j - c_i - s_i to the map of i's
-// class, as 4 x 4 cubic B-spline taps on Ug points per px, wrapped round the Ng x Ng map.
+// class, as 4 x 4 cubic B-spline taps on Ug points per px, wrapped round the Ng x Ng map. The
+// lanes take one (neighbour, tap) each, 4 neighbours per pass of the wave.
 __global__ void pair_splat(const double* __restrict__ c, const double* __restrict__ amp,
                            const double* __restrict__ g, const int32_t* __restrict__ cls,
                            const double* __restrict__ s, int n, const int32_t* __restrict__ order,
@@ -356,29 +423,22 @@ __global__ void pair_splat(const double* __restrict__ c, const double* __restric
     const int cx = (int)floor(x / cs), cy = (int)floor(y / cs);
     const int r = (int)ceil(reach / cs);
     unsigned long long* h = hist + (int64_t)cls[i] * Ng * Ng;
+    const int tq = (lane >> 2) & 3, tp = lane & 3;  // this lane's tap (row q, column p)
     for (int yy = cy - r; yy <= cy + r; ++yy) {
         if (yy < 0 || yy >= ncy) continue;
         const int x0 = cx - r < 0 ? 0 : cx - r, x1 = cx + r >= ncx ? ncx - 1 : cx + r;
         if (x0 > x1) continue;
-        const int k1 = start[yy * ncx + x1 + 1];
-        for (int k = start[yy * ncx + x0] + lane; k < k1; k += 64) {
+        const int k0 = start[yy * ncx + x0], k1 = start[yy * ncx + x1 + 1];
+        for (int k = k0 + (lane >> 4); k < k1; k += 4) {
             const int j = order[k];
             const double dx = c[2 * j] - x, dy = c[2 * j + 1] - y;
             if (fabs(dx) > reach || fabs(dy) > reach) continue;
             const double w = gi * amp[j];
             const double pxf = (dx - sx) * Ug + (double)(Ng / 2), pyf = (dy - sy) * Ug + (double)(Ng / 2);
             const double fx = floor(pxf), fy = floor(pyf);
-            double wx[4], wy[4];
-            bspline4(pxf - fx, wx);
-            bspline4(pyf - fy, wy);
-            const int ix0 = (int)fx - 1, iy0 = (int)fy - 1;
-            for (int q = 0; q < 4; ++q) {
-                const int ry = ((iy0 + q) % Ng + Ng) % Ng;
-                for (int p = 0; p < 4; ++p) {
-                    const int rxx = ((ix0 + p) % Ng + Ng) % Ng;
-                    fix_add(h + (int64_t)ry * Ng + rxx, w * wy[q] * wx[p], kPairFix);
-                }
-            }
+            const int ry = (((int)fy - 1 + tq) % Ng + Ng) % Ng;
+            const int rxx = (((int)fx - 1 + tp) % Ng + Ng) % Ng;
+            fix_add(h + (int64_t)ry * Ng + rxx, w * bspline_tap(pyf - fy, tq) * bspline_tap(pxf - fx, tp), kPairFix);
         }
     }
 }
@@ -412,16 +472,22 @@ __global__ void pair_crop(const double* __restrict__ spec, int nclass, int Ng, i
 }
 
 // --------------------------------------------------------------------------------- page fit
-// One thread per (outline row, frequency): O^(f) = i dphi / (2 pi |f|^2) sum_m (fx yd_m - fy xd_m)
-// exp(-2 pi i f . x_m) (the area at f = 0) and dO^/dp_k = dphi sum_m g_mk exp(-2 pi i f . x_m).
-// geo = (4, rows, n) [x, y, xd, yd]; gk (rows, n, K); out (rows, F, 1 + K) complex.
+// One wave per (outline row, frequency), the lanes splitting the outline points:
+// O^(f) = i dphi / (2 pi |f|^2) sum_m (fx yd_m - fy xd_m) exp(-2 pi i f . x_m) (the area at f = 0)
+// and dO^/dp_k = dphi sum_m g_mk exp(-2 pi i f . x_m); the lanes' sums meet in a fixed-order wave
+// reduction. geo = (4, rows, n) [x, y, xd, yd]; gk (rows, n, K); out (rows, F, 1 + K) complex.
 constexpr int kMaxShape = 16;
 
+// KK >= 0: the number of derivatives fixed at compile time (the accumulators stay in registers);
+// KK < 0: K read at run time (up to kMaxShape).
+template <int KK>
 __global__ void outline_hat_kernel(const double* __restrict__ geo, const double* __restrict__ gk,
-                                   const double* __restrict__ area, int rows, int n, int K, double dphi,
+                                   const double* __restrict__ area, int rows, int n, int Kr, double dphi,
                                    const double* __restrict__ fxs, const double* __restrict__ fys, int F,
                                    double* __restrict__ out) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = KK >= 0 ? KK : Kr;
+    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     if (t >= (int64_t)rows * F) return;
     const int b = (int)(t / F), f = (int)(t % F);
     const double fx = fxs[f], fy = fys[f];
@@ -431,9 +497,11 @@ __global__ void outline_hat_kernel(const double* __restrict__ geo, const double*
     const double* yd = geo + (3 * (int64_t)rows + b) * n;
     const double* g = gk + (int64_t)b * n * K;
     double re0 = 0, im0 = 0;
-    double re[kMaxShape], im[kMaxShape];
-    for (int k = 0; k < K; ++k) re[k] = im[k] = 0;
-    for (int m = 0; m < n; ++m) {
+    constexpr int KA = KK >= 0 ? (KK > 0 ? KK : 1) : kMaxShape;
+    double re[KA], im[KA];
+#pragma unroll
+    for (int k = 0; k < KA; ++k) re[k] = im[k] = 0;
+    for (int m = lane; m < n; m += 64) {
         const double ph = -kTwoPi * (fx * xs[m] + fy * ys[m]);
         double sn, cs;
         sincos(ph, &sn, &cs);
@@ -446,20 +514,29 @@ __global__ void outline_hat_kernel(const double* __restrict__ geo, const double*
             im[k] += sn * gv;
         }
     }
-    double* o = out + t * (1 + K) * 2;
-    const double f2 = fx * fx + fy * fy;
-    if (f2 == 0.0) {
-        o[0] = area[b];
-        o[1] = 0.0;
-    } else {
-        const double q = dphi / (kTwoPi * f2);
-        o[0] = -im0 * q;
-        o[1] = re0 * q;
-    }
+    re0 = wave::sum_d(re0);
+    im0 = wave::sum_d(im0);
     for (int k = 0; k < K; ++k) {
-        o[2 * (1 + k)] = re[k] * dphi;
-        o[2 * (1 + k) + 1] = im[k] * dphi;
+        re[k] = wave::sum_d(re[k]);
+        im[k] = wave::sum_d(im[k]);
     }
+    double* o = out + t * (1 + K) * 2;
+    if (lane == 0) {
+        const double f2 = fx * fx + fy * fy;
+        if (f2 == 0.0) {
+            o[0] = area[b];
+            o[1] = 0.0;
+        } else {
+            const double q = dphi / (kTwoPi * f2);
+            o[0] = -im0 * q;
+            o[1] = re0 * q;
+        }
+    }
+    for (int k = 0; k < K; ++k)
+        if (lane == k) {
+            o[2 * (1 + k)] = re[k] * dphi;
+            o[2 * (1 + k) + 1] = im[k] * dphi;
+        }
 }
 
 // Blur transfer function at (fx, fy) and (grads) its derivatives along the blur parameters bp.
@@ -538,8 +615,13 @@ __global__ void coeff_kernel(const double* __restrict__ ohat, int K, const doubl
                 fr = o[2 * (1 + k)] * bo[0];
                 fi = o[2 * (1 + k) + 1] * bo[0];
             } else {
-                fr = o[0] * bo[1 + k - K];
-                fi = o[1] * bo[1 + k - K];
+                // bo[1 + k - K] by selects: a register array indexed at run time would live in scratch
+                const int q = 1 + k - K;
+                double bq = bo[1];
+#pragma unroll
+                for (int r = 2; r <= kMaxBlur; ++r) bq = q == r ? bo[r] : bq;
+                fr = o[0] * bq;
+                fi = o[1] * bq;
             }
             re += fr * w;
             im += fi * w;
@@ -670,11 +752,12 @@ __device__ inline void block_sum(double (&v)[M], double* red) {
 // [ws, mw, yw, var, scale, const, cost, -, alpha..., beta...].
 constexpr int kMaxFree = 16;
 constexpr int kStats = 8 + 2 * kMaxFree;
+constexpr int kProjBlock = 1024;  // one page's bins per block: the passes are latency bound
 
-__global__ __launch_bounds__(kBlock) void project_scalars(const double* __restrict__ Mall, const double* __restrict__ y,
+__global__ __launch_bounds__(kProjBlock) void project_scalars(const double* __restrict__ Mall, const double* __restrict__ y,
                                                           const double* __restrict__ w, int C, int nbins,
                                                           double* __restrict__ stats) {
-    __shared__ double red[(kBlock / 64) * kMaxFree];
+    __shared__ double red[(kProjBlock / 64) * kMaxFree];
     const int b = blockIdx.x;
     const double* M = Mall + (int64_t)b * C * nbins;
     const double* yb = y + (int64_t)b * nbins;
@@ -712,7 +795,9 @@ __global__ __launch_bounds__(kBlock) void project_scalars(const double* __restri
         for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
             const double sw = sqrt(wb[i]);
             const double u1 = sw / rws;
-            for (int k = 0; k < nf; ++k) al[k] += M[(int64_t)(1 + k) * nbins + i] * (scale * sw) * u1;
+#pragma unroll
+            for (int k = 0; k < kMaxFree; ++k)
+                if (k < nf) al[k] += M[(int64_t)(1 + k) * nbins + i] * (scale * sw) * u1;
         }
         block_sum<kMaxFree>(al, red);
         double be[kMaxFree];
@@ -720,14 +805,18 @@ __global__ __launch_bounds__(kBlock) void project_scalars(const double* __restri
         for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
             const double sw = sqrt(wb[i]);
             const double u1 = sw / rws, u2 = sw * (M[i] - mw) / rvar;
-            for (int k = 0; k < nf; ++k) be[k] += (M[(int64_t)(1 + k) * nbins + i] * (scale * sw) - al[k] * u1) * u2;
+#pragma unroll
+            for (int k = 0; k < kMaxFree; ++k)
+                if (k < nf) be[k] += (M[(int64_t)(1 + k) * nbins + i] * (scale * sw) - al[k] * u1) * u2;
         }
         block_sum<kMaxFree>(be, red);
-        if (threadIdx.x == 0)
-            for (int k = 0; k < nf; ++k) {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < kMaxFree; ++k) {
                 st[8 + k] = al[k];
                 st[8 + kMaxFree + k] = be[k];
             }
+        }
     }
     if (threadIdx.x == 0) {
         st[0] = ws;
@@ -743,10 +832,10 @@ __global__ __launch_bounds__(kBlock) void project_scalars(const double* __restri
 
 // One block per (page, k): row k of A = D D^T and g_k = D_k . r (r = sqrt(w) res), D projected.
 // out (B, 3 + nf + nf^2) = [cost, scale, const, g, A].
-__global__ __launch_bounds__(kBlock) void project_rows(const double* __restrict__ Mall, const double* __restrict__ y,
+__global__ __launch_bounds__(kProjBlock) void project_rows(const double* __restrict__ Mall, const double* __restrict__ y,
                                                        const double* __restrict__ w, int C, int nbins,
                                                        const double* __restrict__ stats, double* __restrict__ out) {
-    __shared__ double red[(kBlock / 64) * (kMaxFree + 1)];
+    __shared__ double red[(kProjBlock / 64) * (kMaxFree + 1)];
     const int nf = C - 1;
     const int b = blockIdx.x / nf, k = blockIdx.x % nf;
     const double* M = Mall + (int64_t)b * C * nbins;
@@ -756,6 +845,7 @@ __global__ __launch_bounds__(kBlock) void project_rows(const double* __restrict_
     const double ws = st[0], mw = st[1], yw = st[2], var = st[3], scale = st[4];
     const double rws = sqrt(ws), rvar = sqrt(fmax(var, 1e-300));
     double acc[kMaxFree + 1];
+#pragma unroll
     for (int l = 0; l <= kMaxFree; ++l) acc[l] = 0;
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
         const double sw = sqrt(wb[i]);
@@ -763,10 +853,13 @@ __global__ __launch_bounds__(kBlock) void project_rows(const double* __restrict_
         const double u1 = sw / rws, u2 = sw * dm / rvar;
         const double r = sw * (scale * dm - (yb[i] - yw));
         const double dk = M[(int64_t)(1 + k) * nbins + i] * (scale * sw) - st[8 + k] * u1 - st[8 + kMaxFree + k] * u2;
-        for (int l = 0; l < nf; ++l) {
-            const double dl =
-                M[(int64_t)(1 + l) * nbins + i] * (scale * sw) - st[8 + l] * u1 - st[8 + kMaxFree + l] * u2;
-            acc[l] += dk * dl;
+#pragma unroll
+        for (int l = 0; l < kMaxFree; ++l) {
+            if (l < nf) {
+                const double dl =
+                    M[(int64_t)(1 + l) * nbins + i] * (scale * sw) - st[8 + l] * u1 - st[8 + kMaxFree + l] * u2;
+                acc[l] += dk * dl;
+            }
         }
         acc[kMaxFree] += dk * r;
     }
@@ -779,7 +872,9 @@ __global__ __launch_bounds__(kBlock) void project_rows(const double* __restrict_
             o[2] = st[5];
         }
         o[3 + k] = acc[kMaxFree];
-        for (int l = 0; l < nf; ++l) o[3 + nf + k * nf + l] = acc[l];
+#pragma unroll
+        for (int l = 0; l < kMaxFree; ++l)
+            if (l < nf) o[3 + nf + k * nf + l] = acc[l];
     }
 }
 
@@ -965,6 +1060,24 @@ int spx_onepass_stack(const float* image, int64_t H, int64_t W, const double* c,
     hipStream_t st = as_stream(stream);
     const int64_t nb = (int64_t)Ms * Ms;
     void* ws = nullptr;
+    const size_t lds = (size_t)nb * (sizeof(unsigned long long) + sizeof(unsigned int));
+    if (lds <= (size_t)kStackLdsBytes) {
+        // per-workgroup bins in LDS: about 8 (hole, row) items per thread
+        const int64_t items = (int64_t)n * (2 * R + 1);
+        int G = (int)((items + 8 * kBlock - 1) / (8 * kBlock));
+        G = G < 1 ? 1 : (G > 512 ? 512 : G);
+        const int per = (int)((n + G - 1) / G);
+        G = (int)((n + per - 1) / per);
+        int rc = spx::workspace((size_t)G * 2 * nb * sizeof(unsigned long long), &ws, st);
+        if (rc != SPX_OK) return rc;
+        auto* partial = static_cast<unsigned long long*>(ws);
+        hipLaunchKernelGGL(stack_acc_lds, dim3(G), dim3(kBlock), lds, st, image, H, W, c, (int)n, use, (int)R, (int)U,
+                           (int)Ms, per, partial);
+        SPX_LAUNCHED();
+        hipLaunchKernelGGL(stack_reduce, dim3(blocks(nb)), dim3(kBlock), 0, st, partial, G, nb, stack, wsum);
+        SPX_LAUNCHED();
+        return SPX_OK;
+    }
     int rc = spx::workspace((size_t)nb * 2 * sizeof(unsigned long long), &ws, st);
     if (rc != SPX_OK) return rc;
     auto* sums = static_cast<unsigned long long*>(ws);
@@ -1032,8 +1145,17 @@ int spx_onepass_outline_hat(const double* geo, const double* gk, const double* a
     spx::clear_error();
     SPX_REQUIRE(geo && area && fx && fy && out && (K == 0 || gk), "spx_onepass_outline_hat: null pointer");
     SPX_REQUIRE(rows > 0 && n > 0 && F > 0 && K >= 0 && K <= kMaxShape, "spx_onepass_outline_hat: bad sizes");
-    hipLaunchKernelGGL(outline_hat_kernel, dim3(blocks((int64_t)rows * F)), dim3(kBlock), 0, as_stream(stream), geo, gk,
-                       area, (int)rows, (int)n, (int)K, dphi, fx, fy, (int)F, out);
+    const dim3 grid(blocks((int64_t)rows * F * 64));
+    hipStream_t st = as_stream(stream);
+    if (K == 0)
+        hipLaunchKernelGGL(outline_hat_kernel<0>, grid, dim3(kBlock), 0, st, geo, gk, area, (int)rows, (int)n, 0, dphi,
+                           fx, fy, (int)F, out);
+    else if (K == 8)
+        hipLaunchKernelGGL(outline_hat_kernel<8>, grid, dim3(kBlock), 0, st, geo, gk, area, (int)rows, (int)n, 8, dphi,
+                           fx, fy, (int)F, out);
+    else
+        hipLaunchKernelGGL(outline_hat_kernel<-1>, grid, dim3(kBlock), 0, st, geo, gk, area, (int)rows, (int)n, (int)K,
+                           dphi, fx, fy, (int)F, out);
     SPX_LAUNCHED();
     return SPX_OK;
 }
@@ -1115,11 +1237,11 @@ int spx_onepass_project(const double* M, const double* y, const double* w, int32
     int rc = spx::workspace((size_t)B * kStats * sizeof(double), &ws, st);
     if (rc != SPX_OK) return rc;
     auto* stats = static_cast<double*>(ws);
-    hipLaunchKernelGGL(project_scalars, dim3(B), dim3(kBlock), 0, st, M, y, w, (int)C, (int)nbins, stats);
+    hipLaunchKernelGGL(project_scalars, dim3(B), dim3(kProjBlock), 0, st, M, y, w, (int)C, (int)nbins, stats);
     SPX_LAUNCHED();
     if (C > 1) {
-        hipLaunchKernelGGL(project_rows, dim3(B * (C - 1)), dim3(kBlock), 0, st, M, y, w, (int)C, (int)nbins, stats,
-                           out);
+        hipLaunchKernelGGL(project_rows, dim3(B * (C - 1)), dim3(kProjBlock), 0, st, M, y, w, (int)C, (int)nbins,
+                           stats, out);
     } else {
         hipLaunchKernelGGL(project_costs, dim3(blocks(B)), dim3(kBlock), 0, st, stats, (int)B, out);
     }
