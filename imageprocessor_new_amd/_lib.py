@@ -129,6 +129,15 @@ class SpxTileSpec(ctypes.Structure):
     ]
 
 
+class SpxLmStage(ctypes.Structure):
+    """One stage of the one-pass page fit's device Levenberg-Marquardt (all pointers device)."""
+
+    _fields_ = [(name, ctypes.c_int32) for name in ("B", "P", "nf", "nshape", "nb", "npr")] + [
+        (name, ctypes.c_void_p) for name in ("fixed", "tm", "lo", "hi", "PL", "pm", "pw", "pj", "pa", "ev", "q", "qn",
+                                             "theta", "blur", "cost", "scale", "cst", "g", "A", "lam", "nu", "done",
+                                             "alldone")] + [("ftol", ctypes.c_double), ("xtol", ctypes.c_double)]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -180,6 +189,10 @@ SIGNATURES = {
     "spx_tiff_read_page": (_I32, [_P, _I32, _P, _I64]),
     "spx_tiff_close": (_I32, [_P]),
     "spx_onepass_smooth": (_I32, [_P, _I64, _I64, _P, _I32, _P, _P, _P]),
+    "spx_onepass_lm_trial": (_I32, [ctypes.POINTER(SpxLmStage), _I32, _P]),
+    "spx_onepass_lm_geometry": (_I32, [_P, _I32, _I32, _I32, _I32, _P, _P, _P, _P, _P]),
+    "spx_onepass_lm_init": (_I32, [ctypes.POINTER(SpxLmStage), _P]),
+    "spx_onepass_lm_update": (_I32, [ctypes.POINTER(SpxLmStage), _P]),
     "spx_onepass_peaks": (_I32, [_P, _I64, _I64, _I32, ctypes.c_double, _P, _P, _P]),
     "spx_onepass_peak_list": (_I32, [_P, _P, _P, _I64, _I64, ctypes.c_double, _P, _P]),
     "spx_onepass_count_above": (_I32, [_P, _I64, _I64, _I32, ctypes.c_double, _P, _P]),

@@ -979,12 +979,358 @@ __global__ void band_kernel(const double* __restrict__ xy, int rows, int n, cons
     o[1] = vr * sn + vi * cs;
 }
 
+// ------------------------------------------------------------------ device Levenberg-Marquardt
+// The page fit's iterations without a round trip per step (onepass.py:504-623): per page the
+// damped Gauss-Newton step with the bounds as an active set, the trial point's theta and blur
+// row, the outline geometry of every trial theta, and after the trial's evaluation the prior
+// terms, the gain ratio, the accept / reject rule, the stopping rules and the damping update.
+// State and stage constants live on the device; the host reads one flag per iteration.
+struct LmState {
+    int B, P, nf, nshape, nb, npr;
+    const double* fixed;  // (B, P): theta entries no free parameter drives
+    const double* tm;     // (P, nf): free parameter -> theta entries (ties included)
+    const double* lo;     // (nf)
+    const double* hi;     // (nf)
+    const double* PL;     // (npr, P) prior rows
+    const double* pm;     // (npr) prior means
+    const double* pw;     // (npr) prior weights 1 / sd^2
+    const double* pj;     // (npr, nf) = PL tm
+    const double* pa;     // (nf, nf) = pj^T diag(pw) pj
+    const double* ev;     // (B, 3 + nf + nf^2) the trial's evaluation [cost, scale, const, g, A]
+    double* q;            // (B, nf) accepted point
+    double* qn;           // (B, nf) trial point
+    double* theta;        // (B, P) trial theta
+    double* blur;         // (B, nb) trial theta's blur part
+    double* cost;         // (B)
+    double* scale;        // (B)
+    double* cst;          // (B)
+    double* g;            // (B, nf)
+    double* A;            // (B, nf, nf)
+    double* lam;          // (B)
+    double* nu;           // (B)
+    int32_t* done;        // (B)
+    int32_t* alldone;     // (1)
+    double ftol, xtol;
+};
+
+// In-place LU with partial pivoting of M (m x m, row stride kMaxFree) and the solve of M x = b
+// (b overwritten by x). Returns false when a pivot is exactly zero (a singular matrix).
+__device__ inline bool lu_solve(double* M, double* b, int m) {
+    for (int c = 0; c < m; ++c) {
+        int piv = c;
+        double best = fabs(M[c * kMaxFree + c]);
+        for (int r = c + 1; r < m; ++r) {
+            const double v = fabs(M[r * kMaxFree + c]);
+            if (v > best) {
+                best = v;
+                piv = r;
+            }
+        }
+        if (best == 0.0) return false;
+        if (piv != c) {
+            for (int k = 0; k < m; ++k) {
+                const double t = M[c * kMaxFree + k];
+                M[c * kMaxFree + k] = M[piv * kMaxFree + k];
+                M[piv * kMaxFree + k] = t;
+            }
+            const double t = b[c];
+            b[c] = b[piv];
+            b[piv] = t;
+        }
+        const double d = M[c * kMaxFree + c];
+        for (int r = c + 1; r < m; ++r) {
+            const double f = M[r * kMaxFree + c] / d;
+            M[r * kMaxFree + c] = f;
+            for (int k = c + 1; k < m; ++k) M[r * kMaxFree + k] -= f * M[c * kMaxFree + k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int r = m - 1; r >= 0; --r) {
+        double v = b[r];
+        for (int k = r + 1; k < m; ++k) v -= M[r * kMaxFree + k] * b[k];
+        b[r] = v / M[r * kMaxFree + r];
+    }
+    return true;
+}
+
+// One thread per page. first = 1: the stage's start (qn = q); else, for a page not done, the
+// bounded step: parameters on a bound whose downhill direction points outside are frozen, the
+// damped system (diagonal floor 1e-10 of the largest) is solved for the rest, a parameter the
+// step carries past a bound is put on it and the others are solved again (at most nf rounds).
+// Then theta = fixed + tm qn and its blur part.
+__global__ void lm_trial_kernel(LmState S, int first) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= S.B) return;
+    const int nf = S.nf;
+    const double* q = S.q + (int64_t)b * nf;
+    double* qn = S.qn + (int64_t)b * nf;
+    if (first || S.done[b]) {
+        for (int i = 0; i < nf; ++i) qn[i] = q[i];
+    } else {
+        const double* g = S.g + (int64_t)b * nf;
+        const double* A = S.A + (int64_t)b * nf * nf;
+        const double lam = S.lam[b];
+        bool frozen[kMaxFree], kk[kMaxFree];
+        double held[kMaxFree], step[kMaxFree];
+        for (int i = 0; i < nf; ++i) {
+            frozen[i] = (q[i] <= S.lo[i] && g[i] > 0) || (q[i] >= S.hi[i] && g[i] < 0);
+            held[i] = 0.0;
+            step[i] = 0.0;
+        }
+        for (int round = 0; round < nf; ++round) {
+            int idx[kMaxFree], m = 0;
+            for (int i = 0; i < nf; ++i) {
+                kk[i] = !frozen[i];
+                if (kk[i]) idx[m++] = i;
+            }
+            if (m == 0) {
+                for (int i = 0; i < nf; ++i) step[i] = held[i];
+                break;
+            }
+            double dmax = -1e300;
+            for (int a = 0; a < m; ++a) dmax = fmax(dmax, A[idx[a] * nf + idx[a]]);
+            double M[kMaxFree * kMaxFree], rhs[kMaxFree];
+            for (int a = 0; a < m; ++a) {
+                double r = g[idx[a]];
+                for (int j = 0; j < nf; ++j)
+                    if (!kk[j]) r += A[idx[a] * nf + j] * held[j];
+                rhs[a] = -r;
+                for (int c = 0; c < m; ++c) M[a * kMaxFree + c] = A[idx[a] * nf + idx[c]];
+                const double d = fmax(A[idx[a] * nf + idx[a]], 1e-10 * dmax);
+                M[a * kMaxFree + a] += lam * d;
+            }
+            const bool ok = lu_solve(M, rhs, m);
+            for (int i = 0; i < nf; ++i) step[i] = held[i];
+            for (int a = 0; a < m; ++a) {
+                const double v = ok ? rhs[a] : 0.0;
+                step[idx[a]] = isfinite(v) ? v : 0.0;
+            }
+            bool any = false;
+            bool over[kMaxFree], under[kMaxFree];
+            for (int i = 0; i < nf; ++i) {
+                over[i] = kk[i] && (q[i] + step[i] > S.hi[i]);
+                under[i] = kk[i] && (q[i] + step[i] < S.lo[i]);
+                any = any || over[i] || under[i];
+            }
+            if (!any) break;
+            for (int i = 0; i < nf; ++i) {
+                held[i] = over[i] ? S.hi[i] - q[i] : (under[i] ? S.lo[i] - q[i] : held[i]);
+                frozen[i] = frozen[i] || over[i] || under[i];
+            }
+        }
+        for (int i = 0; i < nf; ++i) qn[i] = fmin(fmax(q[i] + step[i], S.lo[i]), S.hi[i]);
+    }
+    double* th = S.theta + (int64_t)b * S.P;
+    for (int p = 0; p < S.P; ++p) {
+        double v = 0.0;
+        for (int i = 0; i < nf; ++i) v += qn[i] * S.tm[p * nf + i];
+        th[p] = S.fixed[(int64_t)b * S.P + p] + v;
+    }
+    for (int k = 0; k < S.nb; ++k) S.blur[(int64_t)b * S.nb + k] = th[S.nshape + k];
+}
+
+// The outline at n equal angles of every page's trial theta (one block per page, threads over
+// the points): geo (4, B, n) = [x, y, dx/dphi, dy/dphi], gk (B, n, 4 + 2 nk) = r dr/d(shape
+// parameter), area (B) = sum r^2 dphi / 2. Superellipse of half axes e^th0, e^th1, power
+// 1 + e^th2, turned by th3, times 1 + sum c_k cos(k phi) + s_k sin(k phi) over the harmonics ks.
+__global__ void lm_geometry_kernel(const double* __restrict__ theta, int B, int P, int n, int nk, int k0, int k1, int k2,
+                                   int k3, double* __restrict__ geo, double* __restrict__ gk, double* __restrict__ area) {
+    __shared__ double red[kBlock / 64];
+    const int b = blockIdx.x;
+    const double* th = theta + (int64_t)b * P;
+    const int K = 4 + 2 * nk;
+    const int ks[4] = {k0, k1, k2, k3};
+    const double a = exp(th[0]), bb = exp(th[1]);
+    const double p = 1.0 + exp(th[2]);
+    const double dphi = kTwoPi / n;
+    double part = 0.0;
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        const double phi = kTwoPi * m / n;
+        const double t = phi - th[3];
+        const double ct = cos(t), st = sin(t);
+        const double ua = fabs(ct / a), ub = fabs(st / bb);
+        const double Aa = pow(ua, p), Bq = pow(ub, p);
+        const double S = Aa + Bq;
+        const double base = pow(S, -1.0 / p);
+        const double sct = (ct > 0) - (ct < 0), sst = (st > 0) - (st < 0);
+        const double dbase = -(1.0 / p) * pow(base, 1.0 + p) *
+                             (p * pow(ua, p - 1.0) * sct * (-st / a) + p * pow(ub, p - 1.0) * sst * (ct / bb));
+        double mod = 1.0, dmod = 0.0;
+        for (int i = 0; i < nk; ++i) {
+            const double ck = th[4 + 2 * i], sk = th[5 + 2 * i];
+            const double kp = ks[i] * phi;
+            mod += ck * cos(kp) + sk * sin(kp);
+            dmod += -ck * ks[i] * sin(kp) + sk * ks[i] * cos(kp);
+        }
+        const double r = base * mod;
+        const double dr = dbase * mod + base * dmod;
+        const double la = ua > 0 ? Aa * log(ua) : 0.0;
+        const double lb = ub > 0 ? Bq * log(ub) : 0.0;
+        double* gm = gk + ((int64_t)b * n + m) * K;
+        gm[0] = base * (Aa / S) * mod * r;
+        gm[1] = base * (Bq / S) * mod * r;
+        gm[2] = (p - 1.0) * base * (log(S) / (p * p) - (la + lb) / (p * S)) * mod * r;
+        gm[3] = -dbase * mod * r;
+        for (int i = 0; i < nk; ++i) {
+            const double kp = ks[i] * phi;
+            gm[4 + 2 * i] = base * cos(kp) * r;
+            gm[5 + 2 * i] = base * sin(kp) * r;
+        }
+        const double cp = cos(phi), sp = sin(phi);
+        geo[(int64_t)b * n + m] = r * cp;
+        geo[((int64_t)B + b) * n + m] = r * sp;
+        geo[(2 * (int64_t)B + b) * n + m] = dr * cp - r * sp;
+        geo[(3 * (int64_t)B + b) * n + m] = dr * sp + r * cp;
+        part += r * r;
+    }
+    part = wave::sum_d(part);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w];
+        area[b] = 0.5 * tot * dphi;
+    }
+}
+
+// The trial's evaluation with the priors: cost + sum pw dev^2, g + pj^T (pw dev), A + pa,
+// dev = PL theta - pm.
+__device__ inline void with_priors(const LmState& S, int b, double& cost, double* g, double* A) {
+    const int nf = S.nf;
+    const double* ev = S.ev + (int64_t)b * (3 + nf + nf * nf);
+    const double* th = S.theta + (int64_t)b * S.P;
+    cost = ev[0];
+    for (int i = 0; i < nf; ++i) g[i] = ev[3 + i];
+    for (int i = 0; i < nf * nf; ++i) A[i] = ev[3 + nf + i] + S.pa[i];
+    for (int r = 0; r < S.npr; ++r) {
+        double dv = -S.pm[r];
+        for (int p = 0; p < S.P; ++p) dv += S.PL[r * S.P + p] * th[p];
+        cost += S.pw[r] * dv * dv;
+        for (int i = 0; i < nf; ++i) g[i] += dv * S.pw[r] * S.pj[r * nf + i];
+    }
+}
+
+// The stage's start: the state from the evaluation of q (one block, threads over pages).
+__global__ void lm_init_kernel(LmState S) {
+    for (int b = threadIdx.x; b < S.B; b += blockDim.x) {
+        const int nf = S.nf;
+        double cost;
+        with_priors(S, b, cost, S.g + (int64_t)b * nf, S.A + (int64_t)b * nf * nf);
+        const double* ev = S.ev + (int64_t)b * (3 + nf + nf * nf);
+        S.cost[b] = cost;
+        S.scale[b] = ev[1];
+        S.cst[b] = ev[2];
+        S.lam[b] = 1e-3;
+        S.nu[b] = 2.0;
+        S.done[b] = 0;
+    }
+    if (threadIdx.x == 0) S.alldone[0] = 0;
+}
+
+// After the trial's evaluation (one block, threads over pages): the gain ratio against the
+// quadratic model, accept when the cost falls, the stopping rules (a small fall or step, a
+// rejected step at damping > 1e7, a predicted fall below ftol cost), Nielsen's damping update;
+// alldone = every page's stage has ended.
+__global__ void lm_update_kernel(LmState S) {
+    __shared__ int open_pages;
+    if (threadIdx.x == 0) open_pages = 0;
+    __syncthreads();
+    for (int b = threadIdx.x; b < S.B; b += blockDim.x) {
+        const int nf = S.nf;
+        double cost2, g2[kMaxFree], A2[kMaxFree * kMaxFree];
+        with_priors(S, b, cost2, g2, A2);
+        const double* ev = S.ev + (int64_t)b * (3 + nf + nf * nf);
+        double* q = S.q + (int64_t)b * nf;
+        const double* qn = S.qn + (int64_t)b * nf;
+        double* g = S.g + (int64_t)b * nf;
+        double* A = S.A + (int64_t)b * nf * nf;
+        double dq[kMaxFree];
+        double gd = 0.0, qmax = 0.0, dmax = 0.0;
+        for (int i = 0; i < nf; ++i) {
+            dq[i] = qn[i] - q[i];
+            gd += g[i] * dq[i];
+            qmax = fmax(qmax, fabs(q[i]));
+            dmax = fmax(dmax, fabs(dq[i]));
+        }
+        double quad = 0.0;
+        for (int i = 0; i < nf; ++i) {
+            double row = 0.0;
+            for (int j = 0; j < nf; ++j) row += A[i * nf + j] * dq[j];
+            quad += dq[i] * row;
+        }
+        const double cost = S.cost[b];
+        const double pred = -(2.0 * gd + quad);
+        const double rho = (cost - cost2) / fmax(pred, 1e-300);
+        const bool was_done = S.done[b] != 0;
+        const bool better = (cost2 < cost) && !was_done;
+        const bool small = ((cost - cost2) <= S.ftol * cost) || (dmax <= S.xtol * (S.xtol + qmax));
+        const double lam = S.lam[b];
+        if (better) {
+            for (int i = 0; i < nf; ++i) {
+                q[i] = qn[i];
+                g[i] = g2[i];
+            }
+            for (int i = 0; i < nf * nf; ++i) A[i] = A2[i];
+            S.scale[b] = ev[1];
+            S.cst[b] = ev[2];
+            S.cost[b] = cost2;
+        }
+        const bool done = was_done || (better && small) || (!better && lam > 1e7) || (pred <= S.ftol * cost);
+        S.done[b] = done ? 1 : 0;
+        const double t = 2.0 * rho - 1.0;
+        const double grow = fmax(1.0 / 3.0, 1.0 - t * t * t);
+        S.lam[b] = better ? lam * grow : lam * S.nu[b];
+        S.nu[b] = better ? 2.0 : S.nu[b] * 2.0;
+        if (!done) atomicAdd(&open_pages, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) S.alldone[0] = open_pages == 0 ? 1 : 0;
+}
+
 inline unsigned blocks(int64_t n, int per = kBlock) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
 }  // namespace spx
 
 using namespace spx;
+
+namespace {
+LmState to_state(const SpxLmStage& t) {
+    LmState S{};
+    S.B = t.B;
+    S.P = t.P;
+    S.nf = t.nf;
+    S.nshape = t.nshape;
+    S.nb = t.nb;
+    S.npr = t.npr;
+    S.fixed = t.fixed;
+    S.tm = t.tm;
+    S.lo = t.lo;
+    S.hi = t.hi;
+    S.PL = t.PL;
+    S.pm = t.pm;
+    S.pw = t.pw;
+    S.pj = t.pj;
+    S.pa = t.pa;
+    S.ev = t.ev;
+    S.q = t.q;
+    S.qn = t.qn;
+    S.theta = t.theta;
+    S.blur = t.blur;
+    S.cost = t.cost;
+    S.scale = t.scale;
+    S.cst = t.cst;
+    S.g = t.g;
+    S.A = t.A;
+    S.lam = t.lam;
+    S.nu = t.nu;
+    S.done = t.done;
+    S.alldone = t.alldone;
+    S.ftol = t.ftol;
+    S.xtol = t.xtol;
+    return S;
+}
+}  // namespace
 
 extern "C" {
 
@@ -1270,6 +1616,47 @@ int spx_onepass_band(const double* xy, int32_t rows, int32_t n, const double* bl
     const int64_t total = (int64_t)rows * (2 * kmax + 1) * (kmax + 1);
     hipLaunchKernelGGL(band_kernel, dim3(blocks(total)), dim3(kBlock), 0, as_stream(stream), xy, (int)rows, (int)n, blur,
                        (int)kind, (int)nb, (int)L, (int)Uu, (int)kmax, box_px, full);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int spx_onepass_lm_trial(const SpxLmStage* st, int32_t first, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(st != nullptr, "spx_onepass_lm_trial: null stage");
+    SPX_REQUIRE(st->B > 0 && st->nf > 0 && st->nf <= kMaxFree && st->P >= st->nf, "spx_onepass_lm_trial: bad sizes");
+    LmState S = to_state(*st);
+    hipLaunchKernelGGL(lm_trial_kernel, dim3(blocks(S.B, 64)), dim3(64), 0, as_stream(stream), S, (int)first);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int spx_onepass_lm_geometry(const double* theta, int32_t B, int32_t P, int32_t n, int32_t nk, const int32_t* ks,
+                            double* geo, double* gk, double* area, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(theta && ks && geo && gk && area, "spx_onepass_lm_geometry: null pointer");
+    SPX_REQUIRE(B > 0 && n > 0 && nk >= 0 && nk <= 4 && P >= 4 + 2 * nk, "spx_onepass_lm_geometry: bad sizes");
+    int k[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nk; ++i) k[i] = ks[i];
+    hipLaunchKernelGGL(lm_geometry_kernel, dim3(B), dim3(kBlock), 0, as_stream(stream), theta, (int)B, (int)P, (int)n,
+                       (int)nk, k[0], k[1], k[2], k[3], geo, gk, area);
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int spx_onepass_lm_init(const SpxLmStage* st, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(st != nullptr, "spx_onepass_lm_init: null stage");
+    SPX_REQUIRE(st->B > 0 && st->nf > 0 && st->nf <= kMaxFree, "spx_onepass_lm_init: bad sizes");
+    hipLaunchKernelGGL(lm_init_kernel, dim3(1), dim3(kBlock), 0, as_stream(stream), to_state(*st));
+    SPX_LAUNCHED();
+    return SPX_OK;
+}
+
+int spx_onepass_lm_update(const SpxLmStage* st, spx_stream_t stream) {
+    spx::clear_error();
+    SPX_REQUIRE(st != nullptr, "spx_onepass_lm_update: null stage");
+    SPX_REQUIRE(st->B > 0 && st->nf > 0 && st->nf <= kMaxFree, "spx_onepass_lm_update: bad sizes");
+    hipLaunchKernelGGL(lm_update_kernel, dim3(1), dim3(kBlock), 0, as_stream(stream), to_state(*st));
     SPX_LAUNCHED();
     return SPX_OK;
 }

@@ -240,67 +240,39 @@ class Fitter:
         mask = torch.from_numpy(self.pad[:, None, :, None] & self.pad[None, :, None, :]).to(grid.device)
         return torch.where(mask, grid[py, px], torch.zeros((), dtype=grid.dtype, device=grid.device)), mask
 
-    # outline geometry (host: n_poly points per page)
-    def outline_geometry(self, th: np.ndarray):
-        """Outline points, tangents, r dr/d(shape parameter) (B, n, n_shape), areas, dphi."""
-        phi = self.phi[None, :]
-        a, b = np.exp(th[:, 0:1]), np.exp(th[:, 1:2])
-        p = 1.0 + np.exp(th[:, 2:3])
-        tt = phi - th[:, 3:4]
-        ct, st = np.cos(tt), np.sin(tt)
-        ua, ub = np.abs(ct / a), np.abs(st / b)
-        A, Bq = ua ** p, ub ** p
-        S = A + Bq
-        base = S ** (-1.0 / p)
-        dbase = -(1.0 / p) * base ** (1.0 + p) * (p * ua ** (p - 1.0) * np.sign(ct) * (-st / a)
-                                                   + p * ub ** (p - 1.0) * np.sign(st) * (ct / b))
-        mod, dmod = np.ones_like(tt), np.zeros_like(tt)
-        for i, k in enumerate(self.ks):
-            ck, sk = th[:, 4 + 2 * i:5 + 2 * i], th[:, 5 + 2 * i:6 + 2 * i]
-            mod = mod + ck * np.cos(k * phi) + sk * np.sin(k * phi)
-            dmod = dmod - ck * k * np.sin(k * phi) + sk * k * np.cos(k * phi)
-        r = base * mod
-        dr = dbase * mod + base * dmod
-        with np.errstate(divide="ignore", invalid="ignore"):
-            la = np.where(ua > 0, A * np.log(np.maximum(ua, 1e-300)), 0.0)
-            lb = np.where(ub > 0, Bq * np.log(np.maximum(ub, 1e-300)), 0.0)
-        g = [base * (A / S) * mod, base * (Bq / S) * mod,
-             (p - 1.0) * base * (np.log(S) / (p * p) - (la + lb) / (p * S)) * mod, -dbase * mod]
-        for k in self.ks:
-            g += [base * np.cos(k * phi) + 0.0 * tt, base * np.sin(k * phi) + 0.0 * tt]
-        g = np.stack(g, axis=-1) * r[..., None]
-        x, y = r * np.cos(phi), r * np.sin(phi)
-        xd, yd = dr * np.cos(phi) - r * np.sin(phi), dr * np.sin(phi) + r * np.cos(phi)
-        dphi = 2.0 * np.pi / self.n_poly
-        return x, y, xd, yd, g, 0.5 * (r * r).sum(axis=1) * dphi, dphi
+    # the model
+    def geometry(self, theta: torch.Tensor):
+        """The outline of every row of theta (rows, P) (device): geo (4, rows, n) = [x, y, dx/dphi,
+        dy/dphi], gk (rows, n, n_shape) = r dr/d(shape parameter), area (rows)."""
+        rows, n = int(theta.shape[0]), self.n_poly
+        geo = torch.empty((4, rows, n), dtype=torch.float64, device=self.device)
+        gk = torch.empty((rows, n, self.n_shape), dtype=torch.float64, device=self.device)
+        area = torch.empty(rows, dtype=torch.float64, device=self.device)
+        ks = (ctypes.c_int32 * len(self.ks))(*self.ks)
+        _lib.call("spx_onepass_lm_geometry", _p(theta), rows, self.P, n, len(self.ks), ks, _p(geo), _p(gk), _p(area),
+                  stream_handle())
+        return geo, gk, area
 
-    def outline_hat(self, th: np.ndarray, grads: bool) -> torch.Tensor:
-        """O^ and (grads) its shape derivatives at the band: device (rows, F, 1 + K, 2) FP64."""
-        th = np.asarray(th, dtype=np.float64)
-        x, y, xd, yd, g, area, dphi = self.outline_geometry(th)
-        rows = th.shape[0]
+    def outline_hat(self, theta: torch.Tensor, grads: bool) -> torch.Tensor:
+        """O^ and (grads) its shape derivatives at the band: (rows, F, 1 + K, 2) FP64."""
+        geo, gk, area = self.geometry(theta)
+        rows = int(theta.shape[0])
         K = self.n_shape if grads else 0
-        dev = self.device
-        geo = _dev(np.stack([x, y, xd, yd]), dev)
-        gk = _dev(g, dev) if K else None
-        ar = _dev(area, dev)
-        out = torch.empty((rows, self.F, 1 + K, 2), dtype=torch.float64, device=dev)
-        _lib.call("spx_onepass_outline_hat", _p(geo), _p(gk), _p(ar), rows, self.n_poly, K, float(dphi), _p(self.fx),
-                  _p(self.fy), self.F, _p(out), stream_handle())
+        out = torch.empty((rows, self.F, 1 + K, 2), dtype=torch.float64, device=self.device)
+        _lib.call("spx_onepass_outline_hat", _p(geo), _p(gk), _p(area), rows, self.n_poly, K,
+                  2.0 * math.pi / self.n_poly, _p(self.fx), _p(self.fy), self.F, _p(out), stream_handle())
         return out
 
-    def coeff(self, th: np.ndarray, tm: np.ndarray | None) -> torch.Tensor:
-        """Band coefficients (B, 1 + nf, F, 2) of the template and its free-direction derivatives."""
-        th = np.asarray(th, dtype=np.float64)
-        B = th.shape[0]
+    def coeff(self, theta: torch.Tensor, blur: torch.Tensor, tm: torch.Tensor | None) -> torch.Tensor:
+        """Band coefficients (B, 1 + nf, F, 2) of the template and its free-direction derivatives
+        (tm (P, nf) device, or None); blur (B, n_blur) the rows' blur parts."""
+        B = int(theta.shape[0])
         grads = tm is not None
-        ohat = self.outline_hat(th, grads)
-        blur = _dev(th[:, self.n_shape:], self.device)
-        tmd = _dev(tm, self.device) if grads else None
-        nf = tm.shape[1] if grads else 0
+        ohat = self.outline_hat(theta, grads)
+        nf = int(tm.shape[1]) if grads else 0
         out = torch.empty((B, 1 + nf, self.F, 2), dtype=torch.float64, device=self.device)
         _lib.call("spx_onepass_coeff", _p(ohat), self.n_shape if grads else 0, _p(blur), self.kind, self.n_blur,
-                  _p(tmd), nf, _p(self.fx), _p(self.fy), _p(self.pix), B, self.F, _p(out), stream_handle())
+                  _p(tm), nf, _p(self.fx), _p(self.fy), _p(self.pix), B, self.F, _p(out), stream_handle())
         return out
 
     def synth(self, coeff: torch.Tensor, gamma: torch.Tensor, cls_sel: torch.Tensor | None = None) -> torch.Tensor:
@@ -314,19 +286,32 @@ class Fitter:
 
     def evaluate(self, th: np.ndarray, data: dict, tm: np.ndarray | None = None):
         """Host (cost, scale, const[, g (B, nf), A (B, nf, nf)]) of parameter vectors th (B, P):
-        scale and constant solved exactly, their directions projected off the derivatives."""
+        scale and constant solved exactly, their directions projected off the derivatives (no
+        priors: the fit adds them)."""
         self.evals += 1
         B = data["B"]
-        M = self.synth(self.coeff(th, tm), data["gamma"])
-        C = int(M.shape[1])
-        nf = C - 1
-        out = torch.empty((B, 3 + nf + nf * nf), dtype=torch.float64, device=self.device)
-        _lib.call("spx_onepass_project", _p(M), _p(data["y"]), _p(data["w"]), B, C, self.nbins, _p(out),
-                  stream_handle())
-        o = out.cpu().numpy()
+        theta = _dev(np.asarray(th, dtype=np.float64), self.device)
+        blur = theta[:, self.n_shape:].contiguous()
+        tmd = None if tm is None else _dev(tm, self.device)
+        ev = self._evaluate_rows(theta, blur, tmd, data)
+        o = ev.cpu().numpy()
         if tm is None:
             return o[:, 0], o[:, 1], o[:, 2]
+        nf = tm.shape[1]
         return o[:, 0], o[:, 1], o[:, 2], o[:, 3:3 + nf], o[:, 3 + nf:].reshape(B, nf, nf)
+
+    def _evaluate_rows(self, theta, blur, tmd, data, out=None) -> torch.Tensor:
+        """spx_onepass_project's (B, 3 + nf + nf^2) for device rows theta (B, P): the outline, the
+        band coefficients, the model by class, the variable projection."""
+        B = data["B"]
+        M = self.synth(self.coeff(theta, blur, tmd), data["gamma"])
+        C = int(M.shape[1])
+        nf = C - 1
+        if out is None:
+            out = torch.empty((B, 3 + nf + nf * nf), dtype=torch.float64, device=self.device)
+        _lib.call("spx_onepass_project", _p(M), _p(data["y"]), _p(data["w"]), B, C, self.nbins, _p(out),
+                  stream_handle())
+        return out
 
     def prepare(self, gamma: torch.Tensor, y: torch.Tensor, w: torch.Tensor) -> dict:
         B = int(y.shape[0])
@@ -353,7 +338,8 @@ class Fitter:
         U2, nb = self.U * self.U, self.npts * self.npts
         wsum = data["w"].reshape(B, U2, nb).sum(dim=2)
         top = torch.argmax(wsum, dim=1).to(torch.int32).contiguous()
-        ohat = self.outline_hat(tha.reshape(B * 15, -1), False)
+        tha_d = _dev(tha.reshape(B * 15, -1), self.device)
+        ohat = self.outline_hat(tha_d, False)
         blur = _dev(ths[:, j:], self.device)
         coeff = torch.empty((B, 180, self.F, 2), dtype=torch.float64, device=self.device)
         _lib.call("spx_onepass_grid_coeff", _p(ohat), 15, _p(blur), 12, self.kind, self.n_blur, _p(self.fx),
@@ -391,9 +377,11 @@ class Fitter:
 
     def fit(self, theta0, gamma, y, w, stages=("round", "super", "full"), grid=True, max_iter=(12, 20, 30), ftol=1e-7,
             xtol=1e-7, keep: dict | None = None):
-        """Staged Levenberg-Marquardt for a batch of pages (onepass.py:504-623). theta0 (B, P) host;
-        gamma (B, U^2, nky, nkx) complex, y, w (B, U, U, npts, npts) on the device, w in 1 / variance
-        units. Returns theta (B, P), scale, const, cost, [(stage, iterations, stopped (B,))]."""
+        """Staged Levenberg-Marquardt for a batch of pages in lockstep (onepass.py:504-623): bounds as
+        an active set, Nielsen's gain-ratio damping, weak priors on the second Gaussian. theta0 (B, P)
+        host; gamma (B, U^2, nky, nkx) complex, y, w (B, U, U, npts, npts) device, w in 1 / variance
+        units. The iterations run on the device (spx_onepass_lm_*); the host reads one flag per
+        iteration. Returns theta (B, P), scale, const, cost (host), [(stage, iterations, stopped (B,))]."""
         data = self.prepare(gamma, y, w)
         theta = np.clip(np.asarray(theta0, float).copy(), self.lo + 1e-6, self.hi - 1e-6)
         B = theta.shape[0]
@@ -419,79 +407,45 @@ class Fitter:
             fixed = base * (1.0 - tm.sum(axis=1))[None, :]
             pj = PL @ tm
             pa = (pj.T * pw[None, :]) @ pj
-
-            def expand(q):
-                return fixed + q @ tm.T
-
-            def evaluate(q):
-                th = expand(q)
-                c, s, k, g, A = self.evaluate(th, data, tm)
-                dev = th @ PL.T - pm[None, :]
-                return c + (pw[None, :] * dev * dev).sum(axis=1), s, k, g + (dev * pw[None, :]) @ pj, A + pa[None]
-
-            q = np.clip(base[:, free], lo, hi)
-            cost, scale, const, g, A = evaluate(q)
-            lam = np.full(B, 1e-3)
-            nu = np.full(B, 2.0)
-            done = np.zeros(B, dtype=bool)
+            q0 = np.clip(base[:, free], lo, hi)
+            st, buf = self._stage(B, fixed, tm, lo, hi, PL, pm, pw, pj, pa, q0, ftol, xtol)
+            _lib.call("spx_onepass_lm_trial", ctypes.byref(st), 1, stream_handle())
+            self._evaluate_rows(buf["theta"], buf["blur"], buf["tm"], data, out=buf["ev"])
+            _lib.call("spx_onepass_lm_init", ctypes.byref(st), stream_handle())
+            self.evals += 1
             its = 0
             for its in range(1, cap + 1):
-                q_new = q.copy()
-                for b in range(B):
-                    if not done[b]:
-                        q_new[b] = np.clip(q[b] + bounded_step(q[b], g[b], A[b], lam[b], lo, hi), lo, hi)
-                dq = q_new - q
-                cost2, scale2, const2, g2, A2 = evaluate(q_new)
-                pred = -(2.0 * (g * dq).sum(axis=1) + np.einsum("bi,bij,bj->b", dq, A, dq))
-                rho = (cost - cost2) / np.maximum(pred, 1e-300)
-                better = (cost2 < cost) & ~done
-                small = (((cost - cost2) <= ftol * cost)
-                         | (np.abs(dq).max(axis=1) <= xtol * (xtol + np.abs(q).max(axis=1))))
-                q = np.where(better[:, None], q_new, q)
-                A = np.where(better[:, None, None], A2, A)
-                g = np.where(better[:, None], g2, g)
-                scale, const = np.where(better, scale2, scale), np.where(better, const2, const)
-                done = done | (better & small) | (~better & (lam > 1e7)) | (pred <= ftol * cost)
-                cost = np.where(better, cost2, cost)
-                grow = np.maximum(1.0 / 3.0, 1.0 - (2.0 * rho - 1.0) ** 3)
-                lam = np.where(better, lam * grow, lam * nu)
-                nu = np.where(better, 2.0, nu * 2.0)
-                if done.all():
+                _lib.call("spx_onepass_lm_trial", ctypes.byref(st), 0, stream_handle())
+                self._evaluate_rows(buf["theta"], buf["blur"], buf["tm"], data, out=buf["ev"])
+                _lib.call("spx_onepass_lm_update", ctypes.byref(st), stream_handle())
+                self.evals += 1
+                if int(buf["alldone"].item()):
                     break
-            theta = expand(q)
-            info.append((stage, its, done.copy()))
+            q = buf["q"].cpu().numpy()
+            theta = fixed + q @ tm.T
+            cost, scale, const = (buf[k].cpu().numpy() for k in ("cost", "scale", "cst"))
+            info.append((stage, its, buf["done"].cpu().numpy().astype(bool)))
         return theta, scale, const, cost, info
 
-
-def bounded_step(q, g, A, lam, lo, hi):
-    """Damped Gauss-Newton step of one page with the bounds as an active set (onepass.py:559-585):
-    a parameter on a bound whose downhill direction points outside takes no step; one that the
-    step would carry past a bound is put on the bound and the others are solved again."""
-    nf = len(q)
-    frozen = ((q <= lo) & (g > 0)) | ((q >= hi) & (g < 0))
-    held = np.zeros(nf)
-    step = held
-    for _ in range(nf):
-        k = ~frozen
-        if not k.any():
-            step = held
-            break
-        Ak = A[np.ix_(k, k)]
-        d = np.maximum(np.diag(Ak), 1e-10 * np.diag(Ak).max())
-        rhs = -(g[k] + A[np.ix_(k, ~k)] @ held[~k])
-        try:
-            sk = np.linalg.solve(Ak + np.diag(lam * d), rhs)
-        except np.linalg.LinAlgError:
-            sk = np.zeros(int(k.sum()))
-        step = held.copy()
-        step[k] = np.where(np.isfinite(sk), sk, 0.0)
-        over = (q + step > hi) & k
-        under = (q + step < lo) & k
-        if not (over | under).any():
-            break
-        held = np.where(over, hi - q, np.where(under, lo - q, held))
-        frozen = frozen | over | under
-    return step
+    def _stage(self, B, fixed, tm, lo, hi, PL, pm, pw, pj, pa, q0, ftol, xtol):
+        """The device state of one stage: constants uploaded, work arrays allocated; returns the
+        SpxLmStage and the tensors behind its pointers (kept alive by the caller)."""
+        dev = self.device
+        nf = tm.shape[1]
+        npr = PL.shape[0]
+        f64 = lambda a: _dev(np.asarray(a, dtype=np.float64), dev)  # noqa: E731
+        z = lambda *shape: torch.zeros(shape, dtype=torch.float64, device=dev)  # noqa: E731
+        buf = dict(fixed=f64(fixed), tm=f64(tm), lo=f64(lo), hi=f64(hi), PL=f64(PL if npr else np.zeros((1, self.P))),
+                   pm=f64(pm if npr else np.zeros(1)), pw=f64(pw if npr else np.zeros(1)),
+                   pj=f64(pj if npr else np.zeros((1, nf))), pa=f64(pa), ev=z(B, 3 + nf + nf * nf), q=f64(q0),
+                   qn=z(B, nf), theta=z(B, self.P), blur=z(B, self.n_blur), cost=z(B), scale=z(B), cst=z(B),
+                   g=z(B, nf), A=z(B, nf, nf), lam=z(B), nu=z(B),
+                   done=torch.zeros(B, dtype=torch.int32, device=dev),
+                   alldone=torch.zeros(1, dtype=torch.int32, device=dev))
+        st = _lib.SpxLmStage(B=B, P=self.P, nf=nf, nshape=self.n_shape, nb=self.n_blur, npr=npr,
+                             ftol=float(ftol), xtol=float(xtol),
+                             **{k: _p(v) for k, v in buf.items()})
+        return st, buf
 
 
 def second_is_narrower(t, theta) -> bool:

@@ -693,6 +693,48 @@ int spx_onepass_project(const double* M, const double* y, const double* w, int32
  * the page's class cls_sel[b] with the best scale and constant; M (B, C, nb), y, w (B, ncls, nb). */
 int spx_onepass_grid_costs(const double* M, const double* y, const double* w, const int32_t* cls_sel, int32_t B,
                            int32_t C, int32_t ncls, int32_t nb, double* table, spx_stream_t stream);
+/* The page fit's Levenberg-Marquardt iterations on the device (onepass.py:504-623): one stage of
+ * B pages with nf free parameters of P (nf <= 16). The caller owns every array (device) and
+ * fills the stage constants; per iteration it calls spx_onepass_lm_trial (first = 0), evaluates
+ * the trial theta (spx_onepass_lm_geometry, spx_onepass_outline_hat, spx_onepass_coeff with the
+ * blur rows, spx_onepass_synth, spx_onepass_project into ev) and calls spx_onepass_lm_update;
+ * *alldone = 1 once every page's stage has ended. The stage starts with q set, lm_trial(first
+ * = 1), the evaluation and spx_onepass_lm_init. */
+typedef struct SpxLmStage {
+    int32_t B, P, nf, nshape, nb, npr;
+    const double* fixed;  /* (B, P) theta entries no free parameter drives */
+    const double* tm;     /* (P, nf) free parameter -> theta entries (ties included) */
+    const double* lo;     /* (nf) bounds of the free parameters */
+    const double* hi;
+    const double* PL;     /* (npr, P) prior rows; pm, pw (npr): means and 1 / sd^2 */
+    const double* pm;
+    const double* pw;
+    const double* pj;     /* (npr, nf) = PL tm */
+    const double* pa;     /* (nf, nf) = pj^T diag(pw) pj */
+    const double* ev;     /* (B, 3 + nf + nf^2): spx_onepass_project's output for the trial */
+    double* q;            /* (B, nf) accepted point */
+    double* qn;           /* (B, nf) trial point */
+    double* theta;        /* (B, P) trial theta */
+    double* blur;         /* (B, nb) trial theta's blur part */
+    double* cost;         /* (B) with the priors */
+    double* scale;
+    double* cst;
+    double* g;            /* (B, nf) */
+    double* A;            /* (B, nf, nf) */
+    double* lam;          /* (B) damping */
+    double* nu;           /* (B) damping growth */
+    int32_t* done;        /* (B) */
+    int32_t* alldone;     /* (1) */
+    double ftol, xtol;
+} SpxLmStage;
+int spx_onepass_lm_trial(const SpxLmStage* stage, int32_t first, spx_stream_t stream);
+/* The outline of every trial theta (B, P): geo (4, B, n) = [x, y, dx/dphi, dy/dphi] at n equal
+ * angles, gk (B, n, 4 + 2 nk) = r dr/d(shape parameter), area (B); ks (host, nk <= 4) the
+ * harmonics (onepass.py:246-279). */
+int spx_onepass_lm_geometry(const double* theta, int32_t B, int32_t P, int32_t n, int32_t nk, const int32_t* ks,
+                            double* geo, double* gk, double* area, spx_stream_t stream);
+int spx_onepass_lm_init(const SpxLmStage* stage, spx_stream_t stream);
+int spx_onepass_lm_update(const SpxLmStage* stage, spx_stream_t stream);
 /* Analytic templates on a band (onepass.py:52-126): for every polygon row of xy (rows, 2, n) and
  * blur row (rows, nb), the polygon's exact transform x OTF x sinc(fx) sinc(fy) [x sinc(fx box)
  * sinc(fy box)] x exp(-2 pi i (fx + fy) (N / 2) / Uu), f = k / L, |kx|, |ky| <= kmax, kx >= 0,
