@@ -753,24 +753,37 @@ __device__ inline void block_sum(double (&v)[M], double* red) {
 constexpr int kMaxFree = 16;
 constexpr int kStats = 8 + 2 * kMaxFree;
 constexpr int kProjBlock = 1024;  // one page's bins per block: the passes are latency bound
+constexpr int kProjScalarBlock = 512;  // 35 FP64 moment accumulators per thread: room for 256 VGPRs
 
-__global__ __launch_bounds__(kProjBlock) void project_scalars(const double* __restrict__ Mall, const double* __restrict__ y,
+__global__ __launch_bounds__(kProjScalarBlock) void project_scalars(const double* __restrict__ Mall, const double* __restrict__ y,
                                                           const double* __restrict__ w, int C, int nbins,
                                                           double* __restrict__ stats) {
-    __shared__ double red[(kProjBlock / 64) * kMaxFree];
+    constexpr int kMom = 3 + 2 * kMaxFree;
+    __shared__ double red[(kProjScalarBlock / 64) * kMom];
     const int b = blockIdx.x;
     const double* M = Mall + (int64_t)b * C * nbins;
     const double* yb = y + (int64_t)b * nbins;
     const double* wb = w + (int64_t)b * nbins;
     const int nf = C - 1;
-    double s3[3] = {0, 0, 0};
+    // first moments: ws, sum w m, sum w y and per derivative sum w M_k, sum w M_k m
+    double mo[kMom];
+#pragma unroll
+    for (int k = 0; k < kMom; ++k) mo[k] = 0.0;
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
-        s3[0] += wb[i];
-        s3[1] += wb[i] * M[i];
-        s3[2] += wb[i] * yb[i];
+        const double wi = wb[i], mi = M[i];
+        mo[0] += wi;
+        mo[1] += wi * mi;
+        mo[2] += wi * yb[i];
+#pragma unroll
+        for (int k = 0; k < kMaxFree; ++k)
+            if (k < nf) {
+                const double wmk = wi * M[(int64_t)(1 + k) * nbins + i];
+                mo[3 + k] += wmk;
+                mo[3 + kMaxFree + k] += wmk * mi;
+            }
     }
-    block_sum<3>(s3, red);
-    const double ws = s3[0], mw = s3[1] / ws, yw = s3[2] / ws;
+    block_sum<kMom>(mo, red);
+    const double ws = mo[0], mw = mo[1] / ws, yw = mo[2] / ws;
     double s2[2] = {0, 0};
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
         const double dm = M[i] - mw, dy = yb[i] - yw;
@@ -781,52 +794,35 @@ __global__ __launch_bounds__(kProjBlock) void project_scalars(const double* __re
     const double var = s2[0];
     const double scale = s2[1] / (var > 0 ? var : 1.0);
     const double cst = yw - scale * mw;
-    double s1[1] = {0};
-    for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
-        const double res = scale * (M[i] - mw) - (yb[i] - yw);
-        s1[0] += wb[i] * res * res;
-    }
-    block_sum<1>(s1, red);
     double* st = stats + (int64_t)b * kStats;
-    if (nf > 0) {
-        const double rws = sqrt(ws), rvar = sqrt(fmax(var, 1e-300));
-        double al[kMaxFree];
-        for (int k = 0; k < kMaxFree; ++k) al[k] = 0;
+    double cost[1] = {0.0};
+    if (nf == 0) {  // without derivatives there is no rows kernel to sum the cost
         for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
-            const double sw = sqrt(wb[i]);
-            const double u1 = sw / rws;
-#pragma unroll
-            for (int k = 0; k < kMaxFree; ++k)
-                if (k < nf) al[k] += M[(int64_t)(1 + k) * nbins + i] * (scale * sw) * u1;
+            const double res = scale * (M[i] - mw) - (yb[i] - yw);
+            cost[0] += wb[i] * res * res;
         }
-        block_sum<kMaxFree>(al, red);
-        double be[kMaxFree];
-        for (int k = 0; k < kMaxFree; ++k) be[k] = 0;
-        for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
-            const double sw = sqrt(wb[i]);
-            const double u1 = sw / rws, u2 = sw * (M[i] - mw) / rvar;
-#pragma unroll
-            for (int k = 0; k < kMaxFree; ++k)
-                if (k < nf) be[k] += (M[(int64_t)(1 + k) * nbins + i] * (scale * sw) - al[k] * u1) * u2;
-        }
-        block_sum<kMaxFree>(be, red);
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int k = 0; k < kMaxFree; ++k) {
-                st[8 + k] = al[k];
-                st[8 + kMaxFree + k] = be[k];
-            }
-        }
+        block_sum<1>(cost, red);
     }
     if (threadIdx.x == 0) {
+        // alpha_k = D_k . u1 = scale sum w M_k / sqrt(ws); beta_k = (D_k - alpha_k u1) . u2 with
+        // D_k . u2 = scale sum w M_k (m - mw) / rvar and u1 . u2 = sum w (m - mw) / (sqrt(ws) rvar)
+        const double rws = sqrt(ws), rvar = sqrt(fmax(var, 1e-300));
+        const double u12 = (mo[1] - mw * ws) / (rws * rvar);
+#pragma unroll
+        for (int k = 0; k < kMaxFree; ++k) {
+            const double al = k < nf ? scale * mo[3 + k] / rws : 0.0;
+            const double du2 = k < nf ? scale * (mo[3 + kMaxFree + k] - mw * mo[3 + k]) / rvar : 0.0;
+            st[8 + k] = al;
+            st[8 + kMaxFree + k] = du2 - al * u12;
+        }
         st[0] = ws;
         st[1] = mw;
         st[2] = yw;
         st[3] = var;
         st[4] = scale;
         st[5] = cst;
-        st[6] = s1[0];
-        st[7] = 0;
+        st[6] = cost[0];
+        st[7] = 0.0;
     }
 }
 
@@ -864,10 +860,18 @@ __global__ __launch_bounds__(kProjBlock) void project_rows(const double* __restr
         acc[kMaxFree] += dk * r;
     }
     block_sum<kMaxFree + 1>(acc, red);
+    double cost[1] = {0.0};
+    if (k == 0) {
+        for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+            const double res = scale * (M[i] - mw) - (yb[i] - yw);
+            cost[0] += wb[i] * res * res;
+        }
+        block_sum<1>(cost, red);
+    }
     if (threadIdx.x == 0) {
         double* o = out + (int64_t)b * (3 + nf + nf * nf);
         if (k == 0) {
-            o[0] = st[6];
+            o[0] = cost[0];
             o[1] = st[4];
             o[2] = st[5];
         }
@@ -1013,120 +1017,155 @@ struct LmState {
     double ftol, xtol;
 };
 
-// In-place LU with partial pivoting of M (m x m, row stride kMaxFree) and the solve of M x = b
-// (b overwritten by x). Returns false when a pivot is exactly zero (a singular matrix).
-__device__ inline bool lu_solve(double* M, double* b, int m) {
-    for (int c = 0; c < m; ++c) {
-        int piv = c;
-        double best = fabs(M[c * kMaxFree + c]);
-        for (int r = c + 1; r < m; ++r) {
-            const double v = fabs(M[r * kMaxFree + c]);
-            if (v > best) {
-                best = v;
-                piv = r;
-            }
-        }
-        if (best == 0.0) return false;
-        if (piv != c) {
-            for (int k = 0; k < m; ++k) {
-                const double t = M[c * kMaxFree + k];
-                M[c * kMaxFree + k] = M[piv * kMaxFree + k];
-                M[piv * kMaxFree + k] = t;
-            }
-            const double t = b[c];
-            b[c] = b[piv];
-            b[piv] = t;
-        }
-        const double d = M[c * kMaxFree + c];
-        for (int r = c + 1; r < m; ++r) {
-            const double f = M[r * kMaxFree + c] / d;
-            M[r * kMaxFree + c] = f;
-            for (int k = c + 1; k < m; ++k) M[r * kMaxFree + k] -= f * M[c * kMaxFree + k];
-            b[r] -= f * b[c];
-        }
-    }
-    for (int r = m - 1; r >= 0; --r) {
-        double v = b[r];
-        for (int k = r + 1; k < m; ++k) v -= M[r * kMaxFree + k] * b[k];
-        b[r] = v / M[r * kMaxFree + r];
-    }
-    return true;
-}
-
-// One thread per page. first = 1: the stage's start (qn = q); else, for a page not done, the
-// bounded step: parameters on a bound whose downhill direction points outside are frozen, the
-// damped system (diagonal floor 1e-10 of the largest) is solved for the rest, a parameter the
+// One 64-thread block per page. first = 1: the stage's start (qn = q); else, for a page not
+// done, the bounded step: parameters on a bound whose downhill direction points outside are
+// frozen, the damped system of the others (diagonal floor 1e-10 of the largest) is solved by LU
+// with partial pivoting (the first largest pivot; a zero pivot gives no step), a parameter the
 // step carries past a bound is put on it and the others are solved again (at most nf rounds).
-// Then theta = fixed + tm qn and its blur part.
-__global__ void lm_trial_kernel(LmState S, int first) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// The system sits in LDS; the lanes build it and eliminate one row each, thread 0 picks the
+// pivots and back-substitutes, so every element sees the same operations in the same order as
+// a serial elimination. Then theta = fixed + tm qn and its blur part.
+__global__ __launch_bounds__(64) void lm_trial_kernel(LmState S, int first) {
+    __shared__ double M[kMaxFree * kMaxFree];
+    __shared__ double rhs[kMaxFree], held[kMaxFree], step[kMaxFree];
+    __shared__ int frozen[kMaxFree], idx[kMaxFree], over[kMaxFree];
+    __shared__ int m_sh, piv_sh, flag_sh, any_sh;
+    __shared__ double dmax_sh;
+    const int b = blockIdx.x, t = threadIdx.x;
     if (b >= S.B) return;
     const int nf = S.nf;
     const double* q = S.q + (int64_t)b * nf;
     double* qn = S.qn + (int64_t)b * nf;
     if (first || S.done[b]) {
-        for (int i = 0; i < nf; ++i) qn[i] = q[i];
+        if (t < nf) qn[t] = q[t];
     } else {
         const double* g = S.g + (int64_t)b * nf;
         const double* A = S.A + (int64_t)b * nf * nf;
         const double lam = S.lam[b];
-        bool frozen[kMaxFree], kk[kMaxFree];
-        double held[kMaxFree], step[kMaxFree];
-        for (int i = 0; i < nf; ++i) {
-            frozen[i] = (q[i] <= S.lo[i] && g[i] > 0) || (q[i] >= S.hi[i] && g[i] < 0);
-            held[i] = 0.0;
-            step[i] = 0.0;
+        if (t < nf) {
+            frozen[t] = (q[t] <= S.lo[t] && g[t] > 0) || (q[t] >= S.hi[t] && g[t] < 0);
+            held[t] = 0.0;
+            step[t] = 0.0;
         }
+        __syncthreads();
         for (int round = 0; round < nf; ++round) {
-            int idx[kMaxFree], m = 0;
-            for (int i = 0; i < nf; ++i) {
-                kk[i] = !frozen[i];
-                if (kk[i]) idx[m++] = i;
+            if (t == 0) {
+                int m = 0;
+                double dmax = -1e300;
+                for (int i = 0; i < nf; ++i)
+                    if (!frozen[i]) {
+                        idx[m++] = i;
+                        dmax = fmax(dmax, A[i * nf + i]);
+                    }
+                m_sh = m;
+                dmax_sh = dmax;
             }
+            __syncthreads();
+            const int m = m_sh;
             if (m == 0) {
-                for (int i = 0; i < nf; ++i) step[i] = held[i];
+                if (t < nf) step[t] = held[t];
                 break;
             }
-            double dmax = -1e300;
-            for (int a = 0; a < m; ++a) dmax = fmax(dmax, A[idx[a] * nf + idx[a]]);
-            double M[kMaxFree * kMaxFree], rhs[kMaxFree];
-            for (int a = 0; a < m; ++a) {
-                double r = g[idx[a]];
+            for (int e = t; e < m * m; e += blockDim.x) {
+                const int a = e / m, c = e % m;
+                const double v = A[idx[a] * nf + idx[c]];
+                M[a * kMaxFree + c] = a == c ? v + lam * fmax(v, 1e-10 * dmax_sh) : v;
+            }
+            if (t < m) {
+                double r = g[idx[t]];
                 for (int j = 0; j < nf; ++j)
-                    if (!kk[j]) r += A[idx[a] * nf + j] * held[j];
-                rhs[a] = -r;
-                for (int c = 0; c < m; ++c) M[a * kMaxFree + c] = A[idx[a] * nf + idx[c]];
-                const double d = fmax(A[idx[a] * nf + idx[a]], 1e-10 * dmax);
-                M[a * kMaxFree + a] += lam * d;
+                    if (frozen[j]) r += A[idx[t] * nf + j] * held[j];
+                rhs[t] = -r;
             }
-            const bool ok = lu_solve(M, rhs, m);
-            for (int i = 0; i < nf; ++i) step[i] = held[i];
-            for (int a = 0; a < m; ++a) {
-                const double v = ok ? rhs[a] : 0.0;
-                step[idx[a]] = isfinite(v) ? v : 0.0;
+            __syncthreads();
+            bool ok = true;
+            for (int c = 0; c < m; ++c) {
+                if (t == 0) {
+                    int piv = c;
+                    double best = fabs(M[c * kMaxFree + c]);
+                    for (int r = c + 1; r < m; ++r) {
+                        const double v = fabs(M[r * kMaxFree + c]);
+                        if (v > best) {
+                            best = v;
+                            piv = r;
+                        }
+                    }
+                    piv_sh = piv;
+                    flag_sh = best == 0.0;
+                }
+                __syncthreads();
+                if (flag_sh) {
+                    ok = false;
+                    break;
+                }
+                const int piv = piv_sh;
+                if (piv != c) {
+                    if (t < m) {
+                        const double tmp = M[c * kMaxFree + t];
+                        M[c * kMaxFree + t] = M[piv * kMaxFree + t];
+                        M[piv * kMaxFree + t] = tmp;
+                    }
+                    if (t == 0) {
+                        const double tmp = rhs[c];
+                        rhs[c] = rhs[piv];
+                        rhs[piv] = tmp;
+                    }
+                }
+                __syncthreads();
+                if (t > c && t < m) {
+                    const double f = M[t * kMaxFree + c] / M[c * kMaxFree + c];
+                    M[t * kMaxFree + c] = f;
+                    for (int k = c + 1; k < m; ++k) M[t * kMaxFree + k] -= f * M[c * kMaxFree + k];
+                    rhs[t] -= f * rhs[c];
+                }
+                __syncthreads();
             }
-            bool any = false;
-            bool over[kMaxFree], under[kMaxFree];
-            for (int i = 0; i < nf; ++i) {
-                over[i] = kk[i] && (q[i] + step[i] > S.hi[i]);
-                under[i] = kk[i] && (q[i] + step[i] < S.lo[i]);
-                any = any || over[i] || under[i];
+            if (ok && t == 0)
+                for (int r = m - 1; r >= 0; --r) {
+                    double v = rhs[r];
+                    for (int k = r + 1; k < m; ++k) v -= M[r * kMaxFree + k] * rhs[k];
+                    rhs[r] = v / M[r * kMaxFree + r];
+                }
+            __syncthreads();
+            if (t < nf) step[t] = held[t];
+            __syncthreads();
+            if (t < m) {
+                const double v = ok ? rhs[t] : 0.0;
+                step[idx[t]] = isfinite(v) ? v : 0.0;
             }
-            if (!any) break;
-            for (int i = 0; i < nf; ++i) {
-                held[i] = over[i] ? S.hi[i] - q[i] : (under[i] ? S.lo[i] - q[i] : held[i]);
-                frozen[i] = frozen[i] || over[i] || under[i];
+            __syncthreads();
+            if (t < nf) {
+                const bool free_t = !frozen[t];
+                const bool up = free_t && (q[t] + step[t] > S.hi[t]);
+                const bool dn = free_t && (q[t] + step[t] < S.lo[t]);
+                over[t] = up ? 1 : (dn ? -1 : 0);
             }
+            __syncthreads();
+            if (t == 0) {
+                int any = 0;
+                for (int i = 0; i < nf; ++i) any |= over[i] != 0;
+                any_sh = any;
+            }
+            __syncthreads();
+            if (!any_sh) break;
+            if (t < nf) {
+                held[t] = over[t] > 0 ? S.hi[t] - q[t] : (over[t] < 0 ? S.lo[t] - q[t] : held[t]);
+                frozen[t] = frozen[t] || over[t] != 0;
+            }
+            __syncthreads();
         }
-        for (int i = 0; i < nf; ++i) qn[i] = fmin(fmax(q[i] + step[i], S.lo[i]), S.hi[i]);
+        __syncthreads();
+        if (t < nf) qn[t] = fmin(fmax(q[t] + step[t], S.lo[t]), S.hi[t]);
     }
-    double* th = S.theta + (int64_t)b * S.P;
-    for (int p = 0; p < S.P; ++p) {
+    __syncthreads();
+    if (b == 0 && t == 0 && !first) S.alldone[0] = 1;
+    if (t < S.P) {
         double v = 0.0;
-        for (int i = 0; i < nf; ++i) v += qn[i] * S.tm[p * nf + i];
-        th[p] = S.fixed[(int64_t)b * S.P + p] + v;
+        for (int i = 0; i < nf; ++i) v += qn[i] * S.tm[t * nf + i];
+        const double th = S.fixed[(int64_t)b * S.P + t] + v;
+        S.theta[(int64_t)b * S.P + t] = th;
+        if (t >= S.nshape) S.blur[(int64_t)b * S.nb + (t - S.nshape)] = th;
     }
-    for (int k = 0; k < S.nb; ++k) S.blur[(int64_t)b * S.nb + k] = th[S.nshape + k];
 }
 
 // The outline at n equal angles of every page's trial theta (one block per page, threads over
@@ -1193,31 +1232,43 @@ __global__ void lm_geometry_kernel(const double* __restrict__ theta, int B, int 
     }
 }
 
-// The trial's evaluation with the priors: cost + sum pw dev^2, g + pj^T (pw dev), A + pa,
-// dev = PL theta - pm.
-__device__ inline void with_priors(const LmState& S, int b, double& cost, double* g, double* A) {
-    const int nf = S.nf;
-    const double* ev = S.ev + (int64_t)b * (3 + nf + nf * nf);
+// The priors at the trial theta: dev = PL theta - pm (npr <= 4 rows).
+constexpr int kMaxPrior = 4;
+
+__device__ inline double prior_dev(const LmState& S, int b, double (&dv)[kMaxPrior]) {
     const double* th = S.theta + (int64_t)b * S.P;
-    cost = ev[0];
-    for (int i = 0; i < nf; ++i) g[i] = ev[3 + i];
-    for (int i = 0; i < nf * nf; ++i) A[i] = ev[3 + nf + i] + S.pa[i];
-    for (int r = 0; r < S.npr; ++r) {
-        double dv = -S.pm[r];
-        for (int p = 0; p < S.P; ++p) dv += S.PL[r * S.P + p] * th[p];
-        cost += S.pw[r] * dv * dv;
-        for (int i = 0; i < nf; ++i) g[i] += dv * S.pw[r] * S.pj[r * nf + i];
+    double c = 0.0;
+#pragma unroll
+    for (int r = 0; r < kMaxPrior; ++r) {
+        dv[r] = 0.0;
+        if (r < S.npr) {
+            double v = -S.pm[r];
+            for (int p = 0; p < S.P; ++p) v += S.PL[r * S.P + p] * th[p];
+            dv[r] = v;
+            c += S.pw[r] * v * v;
+        }
     }
+    return c;
+}
+
+// g_i of the trial with the priors: ev's g_i + sum_r dev_r pw_r pj[r][i].
+__device__ inline double prior_g(const LmState& S, const double* ev, const double (&dv)[kMaxPrior], int i) {
+    double v = ev[3 + i];
+#pragma unroll
+    for (int r = 0; r < kMaxPrior; ++r)
+        if (r < S.npr) v += dv[r] * S.pw[r] * S.pj[r * S.nf + i];
+    return v;
 }
 
 // The stage's start: the state from the evaluation of q (one block, threads over pages).
 __global__ void lm_init_kernel(LmState S) {
     for (int b = threadIdx.x; b < S.B; b += blockDim.x) {
         const int nf = S.nf;
-        double cost;
-        with_priors(S, b, cost, S.g + (int64_t)b * nf, S.A + (int64_t)b * nf * nf);
         const double* ev = S.ev + (int64_t)b * (3 + nf + nf * nf);
-        S.cost[b] = cost;
+        double dv[kMaxPrior];
+        S.cost[b] = ev[0] + prior_dev(S, b, dv);
+        for (int i = 0; i < nf; ++i) S.g[(int64_t)b * nf + i] = prior_g(S, ev, dv, i);
+        for (int i = 0; i < nf * nf; ++i) S.A[(int64_t)b * nf * nf + i] = ev[3 + nf + i] + S.pa[i];
         S.scale[b] = ev[1];
         S.cst[b] = ev[2];
         S.lam[b] = 1e-3;
@@ -1227,64 +1278,67 @@ __global__ void lm_init_kernel(LmState S) {
     if (threadIdx.x == 0) S.alldone[0] = 0;
 }
 
-// After the trial's evaluation (one block, threads over pages): the gain ratio against the
-// quadratic model, accept when the cost falls, the stopping rules (a small fall or step, a
-// rejected step at damping > 1e7, a predicted fall below ftol cost), Nielsen's damping update;
-// alldone = every page's stage has ended.
-__global__ void lm_update_kernel(LmState S) {
-    __shared__ int open_pages;
-    if (threadIdx.x == 0) open_pages = 0;
-    __syncthreads();
-    for (int b = threadIdx.x; b < S.B; b += blockDim.x) {
-        const int nf = S.nf;
-        double cost2, g2[kMaxFree], A2[kMaxFree * kMaxFree];
-        with_priors(S, b, cost2, g2, A2);
-        const double* ev = S.ev + (int64_t)b * (3 + nf + nf * nf);
-        double* q = S.q + (int64_t)b * nf;
-        const double* qn = S.qn + (int64_t)b * nf;
-        double* g = S.g + (int64_t)b * nf;
-        double* A = S.A + (int64_t)b * nf * nf;
-        double dq[kMaxFree];
-        double gd = 0.0, qmax = 0.0, dmax = 0.0;
-        for (int i = 0; i < nf; ++i) {
-            dq[i] = qn[i] - q[i];
-            gd += g[i] * dq[i];
-            qmax = fmax(qmax, fabs(q[i]));
-            dmax = fmax(dmax, fabs(dq[i]));
+// After the trial's evaluation, one 64-lane block per page: the gain ratio against the quadratic
+// model at the accepted point, accept when the cost falls, the stopping rules (a small fall or
+// step, a rejected step at damping > 1e7, a predicted fall below ftol cost), Nielsen's damping
+// update. The lanes take the rows of g . dq and dq^T A dq; lm_trial set *alldone = 1 and an
+// open page clears it.
+__global__ __launch_bounds__(64) void lm_update_kernel(LmState S) {
+    __shared__ int better_sh;
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (b >= S.B) return;
+    const int nf = S.nf;
+    const double* ev = S.ev + (int64_t)b * (3 + nf + nf * nf);
+    double* q = S.q + (int64_t)b * nf;
+    const double* qn = S.qn + (int64_t)b * nf;
+    double* g = S.g + (int64_t)b * nf;
+    double* A = S.A + (int64_t)b * nf * nf;
+    double dv[kMaxPrior];
+    const double cost2 = ev[0] + prior_dev(S, b, dv);
+    double gd = 0.0, quad = 0.0, qmax = 0.0, dmax = 0.0;
+    if (t < nf) {
+        const double di = qn[t] - q[t];
+        gd = g[t] * di;
+        qmax = fabs(q[t]);
+        dmax = fabs(di);
+        double row = 0.0;
+        for (int j = 0; j < nf; ++j) row += A[t * nf + j] * (qn[j] - q[j]);
+        quad = di * row;
+    }
+    gd = wave::sum_d(gd);
+    quad = wave::sum_d(quad);
+    qmax = wave::max_d(qmax);
+    dmax = wave::max_d(dmax);
+    const double cost = S.cost[b];
+    const double pred = -(2.0 * gd + quad);
+    const double rho = (cost - cost2) / fmax(pred, 1e-300);
+    const bool was_done = S.done[b] != 0;
+    const bool better = (cost2 < cost) && !was_done;
+    const bool small = ((cost - cost2) <= S.ftol * cost) || (dmax <= S.xtol * (S.xtol + qmax));
+    const double lam = S.lam[b];
+    if (t == 0) better_sh = better;
+    __syncthreads();  // every lane has read q, g and A before any is overwritten
+    if (better) {
+        if (t < nf) {
+            q[t] = qn[t];
+            g[t] = prior_g(S, ev, dv, t);
         }
-        double quad = 0.0;
-        for (int i = 0; i < nf; ++i) {
-            double row = 0.0;
-            for (int j = 0; j < nf; ++j) row += A[i * nf + j] * dq[j];
-            quad += dq[i] * row;
-        }
-        const double cost = S.cost[b];
-        const double pred = -(2.0 * gd + quad);
-        const double rho = (cost - cost2) / fmax(pred, 1e-300);
-        const bool was_done = S.done[b] != 0;
-        const bool better = (cost2 < cost) && !was_done;
-        const bool small = ((cost - cost2) <= S.ftol * cost) || (dmax <= S.xtol * (S.xtol + qmax));
-        const double lam = S.lam[b];
+        for (int i = t; i < nf * nf; i += blockDim.x) A[i] = ev[3 + nf + i] + S.pa[i];
+    }
+    if (t == 0) {
         if (better) {
-            for (int i = 0; i < nf; ++i) {
-                q[i] = qn[i];
-                g[i] = g2[i];
-            }
-            for (int i = 0; i < nf * nf; ++i) A[i] = A2[i];
             S.scale[b] = ev[1];
             S.cst[b] = ev[2];
             S.cost[b] = cost2;
         }
         const bool done = was_done || (better && small) || (!better && lam > 1e7) || (pred <= S.ftol * cost);
         S.done[b] = done ? 1 : 0;
-        const double t = 2.0 * rho - 1.0;
-        const double grow = fmax(1.0 / 3.0, 1.0 - t * t * t);
+        const double tt = 2.0 * rho - 1.0;
+        const double grow = fmax(1.0 / 3.0, 1.0 - tt * tt * tt);
         S.lam[b] = better ? lam * grow : lam * S.nu[b];
         S.nu[b] = better ? 2.0 : S.nu[b] * 2.0;
-        if (!done) atomicAdd(&open_pages, 1);
+        if (!done) S.alldone[0] = 0;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) S.alldone[0] = open_pages == 0 ? 1 : 0;
 }
 
 inline unsigned blocks(int64_t n, int per = kBlock) { return (unsigned)((n + per - 1) / per); }
@@ -1583,7 +1637,7 @@ int spx_onepass_project(const double* M, const double* y, const double* w, int32
     int rc = spx::workspace((size_t)B * kStats * sizeof(double), &ws, st);
     if (rc != SPX_OK) return rc;
     auto* stats = static_cast<double*>(ws);
-    hipLaunchKernelGGL(project_scalars, dim3(B), dim3(kProjBlock), 0, st, M, y, w, (int)C, (int)nbins, stats);
+    hipLaunchKernelGGL(project_scalars, dim3(B), dim3(kProjScalarBlock), 0, st, M, y, w, (int)C, (int)nbins, stats);
     SPX_LAUNCHED();
     if (C > 1) {
         hipLaunchKernelGGL(project_rows, dim3(B * (C - 1)), dim3(kProjBlock), 0, st, M, y, w, (int)C, (int)nbins,
@@ -1624,8 +1678,9 @@ int spx_onepass_lm_trial(const SpxLmStage* st, int32_t first, spx_stream_t strea
     spx::clear_error();
     SPX_REQUIRE(st != nullptr, "spx_onepass_lm_trial: null stage");
     SPX_REQUIRE(st->B > 0 && st->nf > 0 && st->nf <= kMaxFree && st->P >= st->nf, "spx_onepass_lm_trial: bad sizes");
+    SPX_REQUIRE(st->npr >= 0 && st->npr <= 4 && st->P <= 64, "spx_onepass_lm_trial: at most 4 prior rows, 64 parameters");
     LmState S = to_state(*st);
-    hipLaunchKernelGGL(lm_trial_kernel, dim3(blocks(S.B, 64)), dim3(64), 0, as_stream(stream), S, (int)first);
+    hipLaunchKernelGGL(lm_trial_kernel, dim3(S.B), dim3(64), 0, as_stream(stream), S, (int)first);
     SPX_LAUNCHED();
     return SPX_OK;
 }
@@ -1646,7 +1701,8 @@ int spx_onepass_lm_geometry(const double* theta, int32_t B, int32_t P, int32_t n
 int spx_onepass_lm_init(const SpxLmStage* st, spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(st != nullptr, "spx_onepass_lm_init: null stage");
-    SPX_REQUIRE(st->B > 0 && st->nf > 0 && st->nf <= kMaxFree, "spx_onepass_lm_init: bad sizes");
+    SPX_REQUIRE(st->B > 0 && st->nf > 0 && st->nf <= kMaxFree && st->npr >= 0 && st->npr <= 4,
+                "spx_onepass_lm_init: bad sizes");
     hipLaunchKernelGGL(lm_init_kernel, dim3(1), dim3(kBlock), 0, as_stream(stream), to_state(*st));
     SPX_LAUNCHED();
     return SPX_OK;
@@ -1655,8 +1711,9 @@ int spx_onepass_lm_init(const SpxLmStage* st, spx_stream_t stream) {
 int spx_onepass_lm_update(const SpxLmStage* st, spx_stream_t stream) {
     spx::clear_error();
     SPX_REQUIRE(st != nullptr, "spx_onepass_lm_update: null stage");
-    SPX_REQUIRE(st->B > 0 && st->nf > 0 && st->nf <= kMaxFree, "spx_onepass_lm_update: bad sizes");
-    hipLaunchKernelGGL(lm_update_kernel, dim3(1), dim3(kBlock), 0, as_stream(stream), to_state(*st));
+    SPX_REQUIRE(st->B > 0 && st->nf > 0 && st->nf <= kMaxFree && st->npr >= 0 && st->npr <= 4,
+                "spx_onepass_lm_update: bad sizes");
+    hipLaunchKernelGGL(lm_update_kernel, dim3(st->B), dim3(64), 0, as_stream(stream), to_state(*st));
     SPX_LAUNCHED();
     return SPX_OK;
 }
