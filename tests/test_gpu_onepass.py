@@ -157,3 +157,42 @@ def test_fit_outline_fast_entry(drawn):
     both = dark.good & a.good
     assert both.mean() > 0.6
     np.testing.assert_allclose(dark.level[both], 5000.0 - a.level[both], atol=0.05)
+
+
+def test_sparse_page_takes_the_wide_paths():
+    """Pitch 20 px: spacing > 15 px (the wider cell list), window R = 15 (the stack's bins no
+    longer fit in LDS: the global fixed-point path), and the fit still matches the oracle."""
+    t, theta, c, page, seeds = OP.drawn_page("gauss2", shape=(200, 260), pitch=20.0, seed=11)
+    ref = OP.Page(page, seeds)
+    pg = P.OnePassPage(page, seeds)
+    assert pg.spacing > 15.0 and (pg.R, pg.Ms, pg.L) == (ref.R, ref.Ms, ref.L) and pg.Ms ** 2 * 12 > 64 * 1024
+    inside = ref.interior()
+    with gpu_device(0):
+        stack, wsum = pg.stack_of(inside)
+    want_s, want_w = ref.stack_of(inside)
+    assert np.array_equal(wsum.cpu().numpy(), want_w)
+    fin = np.isfinite(want_s)
+    np.testing.assert_allclose(stack.cpu().numpy()[fin], want_s[fin], rtol=1e-12)
+    want = OP.fit_images([page], [seeds])[0]
+    got = P.fit_images([page], [seeds])[0]
+    print("sparse cd", t.cd_px(got["theta"]), t.cd_px(want["theta"]), t.cd_px(theta), flush=True)
+    assert t.cd_px(got["theta"]) == pytest.approx(t.cd_px(want["theta"]), abs=1e-4)
+    assert t.cd_px(got["theta"]) == pytest.approx(t.cd_px(theta), abs=0.02)
+
+
+def test_pages_in_lockstep_read_as_alone(drawn):
+    """Pages fitted together (one group in lockstep, and a second window size in its own group)
+    give each page the theta and centres it gets alone."""
+    pages, seeds = [], []
+    for seed in (5, 6):
+        _, _, _, page, s = OP.drawn_page("gauss2", seed=seed)
+        pages.append(page)
+        seeds.append(s)
+    _, _, _, sparse, s = OP.drawn_page("gauss2", shape=(200, 260), pitch=20.0, seed=11)
+    pages.append(sparse)
+    seeds.append(s)
+    together = P.fit_images(pages, seeds)
+    for k in range(3):
+        alone = P.fit_images([pages[k]], [seeds[k]])[0]
+        np.testing.assert_array_equal(together[k]["theta"], alone["theta"])
+        np.testing.assert_array_equal(together[k]["centers_xy"], alone["centers_xy"])
