@@ -8,17 +8,20 @@
 //   page fit     the outline's transform along its contour (trapezoid rule) and its shape
 //                derivatives, the blur's transfer function and its derivatives, the stack's
 //                model by class (two separable complex products), the variable-projection
-//                cost, gradient and Gauss-Newton matrix per page, the grid of starts' costs
+//                cost, gradient and Gauss-Newton matrix per page, the grid of starts' costs,
+//                and the Levenberg-Marquardt iterations themselves (bounded step, outline of
+//                the trial point, accept and damping rules) with every page's state resident
 //   templates    the outline polygon's exact transform x blur x pixel box on a band, written
 //                into a half spectrum for the inverse FFT
 //
 // Reference: subpx/_gpu/outline_onepass.py:81-250 (stack_acc, hole_vs_stack, pair_splat,
 // nearest), :463-501 (find_seeds), :663-698 (pair_map); subpx/_outline/onepass.py:44-126 (Spec,
 // Band), :129-162 (outline_hat), :281-410 (otf, template_hat, synth, evaluate), :422-486
-// (grid_start). The reference sums with floating-point atomics and synthesises in single
-// precision; here every sum that several threads share is a 64-bit fixed-point integer
-// (order-independent, the same bits every run), the synthesis and the reductions are FP64 in a
-// fixed order, and the host-side Levenberg-Marquardt loop reads one small result per page.
+// (grid_start), :504-623 (fit). The reference sums with floating-point atomics, synthesises in
+// single precision and runs its Levenberg-Marquardt loop on the host; here every sum that
+// several threads share is a 64-bit fixed-point integer (order-independent, the same bits every
+// run), the synthesis and the reductions are FP64 in a fixed order, and the iterations stay on
+// the device (the host reads one flag per iteration).
 //
 // FP64 without FMA contraction, so that the smoothing reproduces the oracle's sums bit for bit.
 #include <cmath>

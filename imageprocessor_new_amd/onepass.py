@@ -18,8 +18,9 @@ The reference's ``fit_outline_fast`` (subpx/outline.py:361-493) and its driver
   transfer function x pixel box (``spx_onepass_coeff``), the model by class
   (``spx_onepass_synth``), scale and constant solved exactly and the cost, gradient and
   Gauss-Newton matrix of the free parameters per page (``spx_onepass_project``), all FP64 on the
-  device; the host runs Levenberg-Marquardt on those small matrices (bounds as an active set,
-  gain-ratio damping, weak priors on the second Gaussian), pages in lockstep;
+  device; the Levenberg-Marquardt iterations (bounds as an active set, gain-ratio damping, weak
+  priors on the second Gaussian) run there too (``spx_onepass_lm_*``), pages in lockstep, the
+  host reading one flag per iteration;
 - hole pass: the hole-stage kernels of ``holes.HolePage`` with templates synthesised from the
   fitted outline and blur (the polygon's exact transform on a band, ``spx_onepass_band``, and an
   inverse FFT).
@@ -482,7 +483,10 @@ def lib_template(blur: str, device):
 
 class OnePassPage(HolePage):
     """One image through steps A to C (outline_onepass.py:520-935). The hole pass is HolePage's
-    (spx_hole_sweep / spx_hole_render / spx_far_splat) with the fitted templates."""
+    (spx_hole_sweep / spx_hole_render / spx_far_splat) with the fitted templates: this class sets
+    every attribute that pass reads (img, h, w, n, c, a, level, delta, noise, spacing, R, the
+    template and far-light fields of set_template) by the one-pass rules, so HolePage.__init__
+    (its own spacing and start rules) is not called."""
 
     def __init__(self, image, seeds=None, *, blur: str = "gauss2", device: int = 0):
         resolve_backend("gpu")
