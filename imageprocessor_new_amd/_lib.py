@@ -199,8 +199,8 @@ SIGNATURES = {
     "spx_onepass_nearest": (_I32, [_P, _I32, _P, _P, _I32, _I32, ctypes.c_double, _P, _P]),
     "spx_onepass_stack": (_I32, [_P, _I64, _I64, _P, _I32, _P, _I32, _I32, _I32, _P, _P, _P]),
     "spx_onepass_versus": (_I32, [_P, _I64, _I64, _P, _I32, _P, _I32, _I32, _I32, ctypes.c_double, _P, _P]),
-    "spx_onepass_pair_hist": (_I32, [_P, _P, _P, _P, _P, _I32, _P, _P, _I32, _I32, ctypes.c_double, ctypes.c_double,
-                                     _I32, _I32, _I32, _P, _P]),
+    "spx_onepass_pair_hist": (_I32, [_P, _P, ctypes.c_double, _P, _I32, _I32, _P, _P, _I32, _I32, ctypes.c_double,
+                                     ctypes.c_double, _I32, _I32, _P, _P, _P]),
     "spx_onepass_pair_crop": (_I32, [_P, _I32, _I32, _I32, _I32, _I32, _P, _P]),
     "spx_onepass_outline_hat": (_I32, [_P, _P, _P, _I32, _I32, _I32, ctypes.c_double, _P, _P, _I32, _P, _P]),
     "spx_onepass_coeff": (_I32, [_P, _I32, _P, _I32, _I32, _P, _I32, _P, _P, _P, _I32, _I32, _P, _P]),

@@ -407,25 +407,45 @@ __device__ inline double bspline_tap(double t, int k) {
     return t * t * t / 6.0;
 }
 
-// One wave per hole i with g[i] != 0: every hole j of the cells within ceil(reach / cs) of i's
-// cell with |c_j - c_i| <= reach per axis adds g_i amp_j at c_j - c_i - s_i to the map of i's
-// class, as 4 x 4 cubic B-spline taps on Ug points per px, wrapped round the Ng x Ng map. The
-// lanes take one (neighbour, tap) each, 4 neighbours per pass of the wave.
-__global__ void pair_splat(const double* __restrict__ c, const double* __restrict__ amp,
-                           const double* __restrict__ g, const int32_t* __restrict__ cls,
-                           const double* __restrict__ s, int n, const int32_t* __restrict__ order,
-                           const int32_t* __restrict__ start, int ncx, int ncy, double cs, double reach, int Ug,
-                           int Ng, unsigned long long* __restrict__ hist) {
+// Sub-pixel class and residual offset of hole i (outline_onepass.py:670-674): v = -c U,
+// class (mod(rint(v_y), U), mod(rint(v_x), U)), s = (v - rint(v)) / U.
+__device__ inline int pair_class(const double* c, int i, int U, double& sx, double& sy) {
+    const double vx = -c[2 * i] * U, vy = -c[2 * i + 1] * U;
+    const double nx = rint(vx), ny = rint(vy);
+    sx = (vx - nx) / U;
+    sy = (vy - ny) / U;
+    const int qx = (int)(((long long)nx % U + U) % U), qy = (int)(((long long)ny % U + U) % U);
+    return qy * U + qx;
+}
+
+// Stacked holes per class.
+__global__ void pair_counts(const double* __restrict__ c, const uint8_t* __restrict__ use, int n, int U,
+                            unsigned long long* __restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !use[i]) return;
+    double sx, sy;
+    atomicAdd(counts + pair_class(c, i, U, sx, sy), 1ull);
+}
+
+// One wave per stacked hole i (g_i = 1 / the stacked holes of its class): every hole j of the
+// cells within ceil(reach / cs) of i's cell with |c_j - c_i| <= reach per axis adds
+// g_i a_j / amean at c_j - c_i - s_i to the map of i's class, as 4 x 4 cubic B-spline taps on
+// Ug points per px, wrapped round the Ng x Ng map. The lanes take one (neighbour, tap) each,
+// 4 neighbours per pass of the wave.
+__global__ void pair_splat(const double* __restrict__ c, const double* __restrict__ a, double amean,
+                           const uint8_t* __restrict__ use, const unsigned long long* __restrict__ counts, int n,
+                           int U, const int32_t* __restrict__ order, const int32_t* __restrict__ start, int ncx,
+                           int ncy, double cs, double reach, int Ug, int Ng, unsigned long long* __restrict__ hist) {
     const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
-    if (i >= n) return;
-    const double gi = g[i];
-    if (gi == 0.0) return;
+    if (i >= n || !use[i]) return;
+    double sx, sy;
+    const int cls = pair_class(c, i, U, sx, sy);
+    const double gi = 1.0 / fmax((double)counts[cls], 1e-300);
     const double x = c[2 * i], y = c[2 * i + 1];
-    const double sx = s[2 * i], sy = s[2 * i + 1];
     const int cx = (int)floor(x / cs), cy = (int)floor(y / cs);
     const int r = (int)ceil(reach / cs);
-    unsigned long long* h = hist + (int64_t)cls[i] * Ng * Ng;
+    unsigned long long* h = hist + (int64_t)cls * Ng * Ng;
     const int tq = (lane >> 2) & 3, tp = lane & 3;  // this lane's tap (row q, column p)
     for (int yy = cy - r; yy <= cy + r; ++yy) {
         if (yy < 0 || yy >= ncy) continue;
@@ -436,7 +456,7 @@ __global__ void pair_splat(const double* __restrict__ c, const double* __restric
             const int j = order[k];
             const double dx = c[2 * j] - x, dy = c[2 * j + 1] - y;
             if (fabs(dx) > reach || fabs(dy) > reach) continue;
-            const double w = gi * amp[j];
+            const double w = gi * (a[j] / amean);
             const double pxf = (dx - sx) * Ug + (double)(Ng / 2), pyf = (dy - sy) * Ug + (double)(Ng / 2);
             const double fx = floor(pxf), fy = floor(pyf);
             const int ry = (((int)fy - 1 + tq) % Ng + Ng) % Ng;
@@ -1505,27 +1525,36 @@ int spx_onepass_versus(const float* image, int64_t H, int64_t W, const double* c
     return SPX_OK;
 }
 
-int spx_onepass_pair_hist(const double* c, const double* amp, const double* g, const int32_t* cls, const double* s,
-                          int32_t n, const int32_t* order, const int32_t* cell_start, int32_t ncx, int32_t ncy,
-                          double cs, double reach, int32_t Ug, int32_t Ng, int32_t nclass, double* hist,
+int spx_onepass_pair_hist(const double* c, const double* a, double amean, const uint8_t* use, int32_t n, int32_t U,
+                          const int32_t* order, const int32_t* cell_start, int32_t ncx, int32_t ncy, double cs,
+                          double reach, int32_t Ug, int32_t Ng, double* hist, double* class_counts,
                           spx_stream_t stream) {
     spx::clear_error();
-    SPX_REQUIRE(c && amp && g && cls && s && order && cell_start && hist, "spx_onepass_pair_hist: null pointer");
-    SPX_REQUIRE(n > 0 && ncx > 0 && ncy > 0 && cs > 0 && reach >= 0 && Ug > 0 && Ng > 0 && nclass > 0,
+    SPX_REQUIRE(c && a && use && order && cell_start && hist, "spx_onepass_pair_hist: null pointer");
+    SPX_REQUIRE(n > 0 && U > 0 && U <= 16 && ncx > 0 && ncy > 0 && cs > 0 && reach >= 0 && Ug > 0 && Ng > 0 &&
+                    amean > 0,
                 "spx_onepass_pair_hist: bad sizes");
     hipStream_t st = as_stream(stream);
+    const int nclass = U * U;
     const int64_t total = (int64_t)nclass * Ng * Ng;
     void* ws = nullptr;
-    int rc = spx::workspace((size_t)total * sizeof(unsigned long long), &ws, st);
+    int rc = spx::workspace((size_t)(total + nclass) * sizeof(unsigned long long), &ws, st);
     if (rc != SPX_OK) return rc;
     auto* acc = static_cast<unsigned long long*>(ws);
-    rc = zero_launch(acc, total, st);
+    unsigned long long* counts = acc + total;
+    rc = zero_launch(acc, total + nclass, st);
     if (rc != SPX_OK) return rc;
-    hipLaunchKernelGGL(pair_splat, dim3(blocks((int64_t)n * 64)), dim3(kBlock), 0, st, c, amp, g, cls, s, (int)n, order,
-                       cell_start, (int)ncx, (int)ncy, cs, reach, (int)Ug, (int)Ng, acc);
+    hipLaunchKernelGGL(pair_counts, dim3(blocks(n)), dim3(kBlock), 0, st, c, use, (int)n, (int)U, counts);
+    SPX_LAUNCHED();
+    hipLaunchKernelGGL(pair_splat, dim3(blocks((int64_t)n * 64)), dim3(kBlock), 0, st, c, a, amean, use, counts, (int)n,
+                       (int)U, order, cell_start, (int)ncx, (int)ncy, cs, reach, (int)Ug, (int)Ng, acc);
     SPX_LAUNCHED();
     hipLaunchKernelGGL(fix_to_double, dim3(blocks(total)), dim3(kBlock), 0, st, acc, total, kPairFix, hist);
     SPX_LAUNCHED();
+    if (class_counts != nullptr) {
+        hipLaunchKernelGGL(fix_to_double, dim3(1), dim3(kBlock), 0, st, counts, (int64_t)nclass, 1.0, class_counts);
+        SPX_LAUNCHED();
+    }
     return SPX_OK;
 }
 

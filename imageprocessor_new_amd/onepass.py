@@ -578,30 +578,25 @@ class OnePassPage(HolePage):
 
     # pair map
     def pair_map(self, sf: Fitter) -> torch.Tensor:
-        """Gamma_q (U^2, nky, nkx) complex on the fitter's band (outline_onepass.py:663-698)."""
+        """Gamma_q (U^2, nky, nkx) complex on the fitter's band (outline_onepass.py:663-698): the
+        stacked holes' neighbours (brightness over the stacked mean, each hole's class weighted
+        by 1 / its stacked holes) splatted per class, a real FFT, the crop to the band."""
         U, Ug = self.U, self.pair_U
         Ng = self.L * Ug
         with gpu_device(self.device) as dev:
-            c = self.c.cpu().numpy()
-            v = -c * U
-            nb = np.rint(v)
-            q = np.mod(nb, U).astype(np.int64)
-            s = (v - nb) / U
-            cls = q[:, 1] * U + q[:, 0]
-            sw = self.use.astype(np.float64)
-            wq = np.bincount(cls, weights=sw, minlength=U * U)
-            g = np.where(sw > 0, sw / np.maximum(wq[cls], 1e-300), 0.0)
-            amp = self.a_host / float(self.a_host[self.use].mean())
-            hist = torch.empty((U * U, Ng, Ng), dtype=torch.float64, device=dev)
             # bound to names: a temporary freed before the launch could hand its block to the next one
-            amp_d, g_d, cls_d, s_d = _dev(amp, dev), _dev(g, dev), _dev(cls, dev, torch.int32), _dev(s, dev)
-            _lib.call("spx_onepass_pair_hist", _p(self.c), _p(amp_d), _p(g_d), _p(cls_d), _p(s_d), self.n,
-                      _p(self.cells.order), _p(self.cells.start), self.cells.ncx, self.cells.ncy, self.cells.cs,
-                      self.reach, Ug, Ng, U * U, _p(hist), stream_handle())
+            a_d = _dev(self.a_host, dev)
+            use_d = _dev(np.asarray(self.use, dtype=np.uint8), dev, torch.uint8)
+            hist = torch.empty((U * U, Ng, Ng), dtype=torch.float64, device=dev)
+            counts = torch.empty(U * U, dtype=torch.float64, device=dev)
+            amean = float(self.a_host[self.use].mean())
+            _lib.call("spx_onepass_pair_hist", _p(self.c), _p(a_d), amean, _p(use_d), self.n, U, _p(self.cells.order),
+                      _p(self.cells.start), self.cells.ncx, self.cells.ncy, self.cells.cs, self.reach, Ug, Ng,
+                      _p(hist), _p(counts), stream_handle())
             spec = torch.fft.rfft2(hist).contiguous()
             gamma = torch.empty((U * U, sf.nky, sf.nkx), dtype=torch.complex128, device=dev)
             _lib.call("spx_onepass_pair_crop", _p(spec), U * U, Ng, Ug, self.L, sf.kmax, _p(gamma), stream_handle())
-            self.holes_per_class = wq.reshape(U, U)
+            self.holes_per_class = counts.reshape(U, U)
             return gamma
 
     def stack_data(self, sf: Fitter):

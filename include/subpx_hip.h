@@ -646,13 +646,15 @@ int spx_onepass_stack(const float* image, int64_t H, int64_t W, const double* c,
  * window, one Gauss-Newton step (dx, dy) of the centre over the core pixels, pixels compared. */
 int spx_onepass_versus(const float* image, int64_t H, int64_t W, const double* c, int32_t n, const double* stack,
                        int32_t R, int32_t U, int32_t Ms, double core2, double* out, spx_stream_t stream);
-/* Pair maps before the FFT (outline_onepass.py:184-224): hist (nclass, Ng, Ng); every hole i
- * with g[i] != 0 adds g_i amp_j at c_j - c_i - s_i (cubic B-spline taps, Ug per px, wrapped)
- * to the map cls[i] for every hole j of the cells within ceil(reach / cs) of its cell with
- * |c_j - c_i| <= reach per axis (j = i included). Fixed-point sums (2^40 per count). */
-int spx_onepass_pair_hist(const double* c, const double* amp, const double* g, const int32_t* cls, const double* s,
-                          int32_t n, const int32_t* order, const int32_t* cell_start, int32_t ncx, int32_t ncy,
-                          double cs, double reach, int32_t Ug, int32_t Ng, int32_t nclass, double* hist,
+/* Pair maps before the FFT (outline_onepass.py:184-224, 663-687): hist (U^2, Ng, Ng); every hole
+ * i with use[i] != 0, of sub-pixel class q = (mod(rint(-c_y U), U), mod(rint(-c_x U), U)) and
+ * residual s_i = (-c U - rint(-c U)) / U, adds (a_j / amean) / (stacked holes of class q) at
+ * c_j - c_i - s_i (cubic B-spline taps, Ug per px, wrapped) to map q for every hole j of the
+ * cells within ceil(reach / cs) of its cell with |c_j - c_i| <= reach per axis (j = i included).
+ * Fixed-point sums (2^40 per count). class_counts (U^2, may be NULL): stacked holes per class. */
+int spx_onepass_pair_hist(const double* c, const double* a, double amean, const uint8_t* use, int32_t n, int32_t U,
+                          const int32_t* order, const int32_t* cell_start, int32_t ncx, int32_t ncy, double cs,
+                          double reach, int32_t Ug, int32_t Ng, double* hist, double* class_counts,
                           spx_stream_t stream);
 /* gamma (nclass, 2 kmax + 1, kmax + 1) complex = spec[q][ky mod Ng][kx] / (centre x fix) for
  * ky = -kmax .. kmax, kx = 0 .. kmax, f = k / L (outline_onepass.py:688-698); spec (nclass, Ng,
