@@ -441,18 +441,36 @@ struct CellRow {
 SPX_DEV_INLINE int imax(int a, int b) { return a > b ? a : b; }
 SPX_DEV_INLINE int imin(int a, int b) { return a < b ? a : b; }
 
-// first column in [0, side) where the monotone (false, then true) predicate holds, else side
+// How far (in columns) a column must lie from a rule's root estimate for the exact rule's
+// decision there to be the root's side, so that the columns next to the estimate need not be
+// evaluated. The rule's FP64 value at column x, ((a qx + by) + c) [- m] with a, b, qx, qy exact,
+// is within 4 u (|a| (|ox| + side) + |by| + |c| + m) of its real value (u = 2^-53), and the
+// estimate -(by + c) / a - ox (or (m - by - c) / a - ox) within 5 u (|by| + |c| + m) / |a| + u |est|
+// of the real root; a column farther from the estimate than the sum over |a| has the real sign.
+// The margin takes 64 u over those magnitudes (about 7 times the bound) plus 1e-12; a NaN or
+// infinite margin (|a| near 0) never passes, and the exact walk decides.
+SPX_DEV_INLINE double root_margin(const CellTerm& t, double by, double oxd, int side, double est) {
+#pragma clang fp contract(off)
+    constexpr double kU = 1.1102230246251565e-16;
+    const double mag = fabs(t.a) * (fabs(oxd) + (double)side + 1.0) + fabs(by) + fabs(t.c) + t.m;
+    return 64.0 * kU * (mag * fabs(t.inv) + fabs(est) + fabs(oxd) + (double)side) + 1e-12;
+}
+
+// first column in [0, side) where the monotone (false, then true) predicate holds, else side;
+// the exact rule is evaluated only at a checked column within `margin` of the estimate
 template <class Pred>
-SPX_DEV_INLINE int first_true(const Pred& p, double est, int side) {
+SPX_DEV_INLINE int first_true(const Pred& p, double est, int side, double margin) {
     int c = (est != est) ? 0 : (int)fmin(fmax(ceil(est), 0.0), (double)side);
+    if ((c == 0 || est - (double)(c - 1) > margin) && (c == side || (double)c - est > margin)) return c;
     for (int k = 0; k <= side && c > 0 && p(c - 1); ++k) --c;
     for (int k = 0; k <= side && c < side && !p(c); ++k) ++c;
     return c;
 }
 // last column in [0, side) where the monotone (true, then false) predicate holds, else -1
 template <class Pred>
-SPX_DEV_INLINE int last_true(const Pred& p, double est, int side) {
+SPX_DEV_INLINE int last_true(const Pred& p, double est, int side, double margin) {
     int c = (est != est) ? side - 1 : (int)fmin(fmax(floor(est), -1.0), (double)(side - 1));
+    if ((c == side - 1 || (double)(c + 1) - est > margin) && (c == -1 || est - (double)c > margin)) return c;
     for (int k = 0; k <= side && c < side - 1 && p(c + 1); ++k) ++c;
     for (int k = 0; k <= side && c >= 0 && !p(c); ++k) --c;
     return c;
@@ -528,12 +546,16 @@ SPX_DEV_INLINE CellRow cell_row(const Terms& terms, int64_t ox, int64_t oy, int 
         const double by = t.b * qy;
         const auto keeps = [&](int c) { return cell_keeps(t, cell_f(t, oxd + (double)c, by)); };
         const auto interior = [&](int c) { return cell_interior(t, cell_f(t, oxd + (double)c, by)); };
-        if (t.a > 0.0) {
-            r.lo = imax(r.lo, first_true(keeps, -(by + t.c) * t.inv - oxd, side));
-            r.elo = imax(r.elo, first_true(interior, (t.m - by - t.c) * t.inv - oxd, side));
-        } else if (t.a < 0.0) {
-            r.hi = imin(r.hi, last_true(keeps, -(by + t.c) * t.inv - oxd, side));
-            r.ehi = imin(r.ehi, last_true(interior, (t.m - by - t.c) * t.inv - oxd, side));
+        if (t.a != 0.0) {
+            const double ek = -(by + t.c) * t.inv - oxd, ei = (t.m - by - t.c) * t.inv - oxd;
+            const double mk = root_margin(t, by, oxd, side, ek), mi = root_margin(t, by, oxd, side, ei);
+            if (t.a > 0.0) {
+                r.lo = imax(r.lo, first_true(keeps, ek, side, mk));
+                r.elo = imax(r.elo, first_true(interior, ei, side, mi));
+            } else {
+                r.hi = imin(r.hi, last_true(keeps, ek, side, mk));
+                r.ehi = imin(r.ehi, last_true(interior, ei, side, mi));
+            }
         } else {
             // a constant rule (a = 0, or not a number): the whole row or none of it
             if (!keeps(0)) {

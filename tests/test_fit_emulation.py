@@ -165,6 +165,22 @@ def test_cell_rows_take_the_exact_decisions(integer_seeds):
         assert ties > 0  # exact ties were decided
 
 
+def test_row_walk_shortcut_takes_the_walks_decisions():
+    """cell_row skips the exact rule next to a neighbour rule's root estimate when both checked
+    columns lie farther from it than the rule's FP64 error margin (root_margin): over random
+    seed pairs (fractional, quarter- and half-integer, integer and 1e-15 off an integer), the
+    first/last kept and interior columns with the shortcut equal the walk's and a scan of every
+    column, and the walk still runs for the near-integer roots (exact ties)."""
+    _build()
+    lib = ctypes.CDLL(str(OUT))
+    lib.emu_row_walk_check.restype = ctypes.c_int64
+    bad, walked = ctypes.c_int64(), ctypes.c_int64()
+    n = lib.emu_row_walk_check(ctypes.c_uint64(11), ctypes.c_int32(40000), ctypes.byref(bad), ctypes.byref(walked))
+    assert n > 1_000_000
+    assert bad.value == 0
+    assert 0 < walked.value < n // 4  # the ties took the walk; most rows did not
+
+
 def test_pixel_count_and_bound_of_the_fitted_pixels():
     """The exact count of a feature's fitted pixels from the cell rows (patch_pixel_count)
     equals the pixels the oracle's patch keeps (inside the image and the Voronoi cell), and the

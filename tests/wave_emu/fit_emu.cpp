@@ -295,3 +295,79 @@ extern "C" void emu_cell_rows(const float* seeds, const int16_t* half, const int
         }
     }
 }
+
+// The row walk's shortcut against the walk itself and a brute-force scan: for n random pairs
+// of float seeds (fractional, quarter- and half-integer, integer, and 1e-15 off an integer, so
+// that many roots fall on or next to a column), every row of a patch around the first seed,
+// each neighbour rule's first (a > 0) or last (a < 0) kept and interior column, found (1) with
+// the margin, (2) by the walk alone and (3) by scanning every column. Returns the number of
+// decisions; *mismatch counts those where the three differ, *walked those the margin left to
+// the walk.
+extern "C" int64_t emu_row_walk_check(uint64_t seed, int32_t n, int64_t* mismatch, int64_t* walked) {
+    using namespace spx::fitdev;
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+    const auto next = [&]() {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        return s;
+    };
+    const auto unit = [&]() { return (double)(next() >> 11) * 0x1.0p-53 * 2.0 - 1.0; };
+    const double fr[] = {0.0, 0.5, 0.25, 0.125, 1e-15, -1e-15, 1e-9, 0.1, 1.0 / 3.0};
+    int64_t dec = 0, bad = 0, slow = 0;
+    for (int it = 0; it < n; ++it) {
+        const int mode = it % 4;
+        const auto pick = [&](double base) {
+            const double w = base + unit() * 20.0;
+            if (mode == 0) return w;
+            if (mode == 3) return std::round(w);
+            return std::round(w) + fr[next() % 9] * (mode == 2 && unit() < 0.0 ? -1.0 : 1.0);
+        };
+        const double base = unit() * 8000.0 + 8000.0;
+        const float sx = (float)pick(base), sy = (float)pick(base);
+        const float jx = (float)pick(sx), jy = (float)pick(sy);
+        if (sx == jx && sy == jy) continue;
+        const double x0 = sx, y0 = sy;
+        const CellTerm t = cell_term(x0, y0, x0 * x0 + y0 * y0, jx, jy, (it & 1) != 0);
+        if (t.a == 0.0) continue;
+        const int hs = 3 + (int)(next() % 20), side = 2 * hs + 1;
+        const int64_t ox = (int64_t)rintf(sx) - hs, oy = (int64_t)rintf(sy) - hs;
+        const double oxd = (double)ox;
+        for (int r = 0; r < side; ++r) {
+            const double by = t.b * (double)(oy + r);
+            const auto keeps = [&](int c) { return cell_keeps(t, cell_f(t, oxd + (double)c, by)); };
+            const auto interior = [&](int c) { return cell_interior(t, cell_f(t, oxd + (double)c, by)); };
+            const double ek = -(by + t.c) * t.inv - oxd, ei = (t.m - by - t.c) * t.inv - oxd;
+            const double mk = root_margin(t, by, oxd, side, ek), mi = root_margin(t, by, oxd, side, ei);
+            int got[2][3];
+            const bool up = t.a > 0.0;
+            for (int q = 0; q < 2; ++q) {
+                const double est = q ? ei : ek, mg = q ? mi : mk;
+                const auto scan = [&](const auto& p) {
+                    if (up) {
+                        for (int c = 0; c < side; ++c)
+                            if (p(c)) return c;
+                        return side;
+                    }
+                    for (int c = side - 1; c >= 0; --c)
+                        if (p(c)) return c;
+                    return -1;
+                };
+                const auto run = [&](const auto& p) {
+                    got[q][0] = up ? first_true(p, est, side, mg) : last_true(p, est, side, mg);
+                    got[q][1] = up ? first_true(p, est, side, INFINITY) : last_true(p, est, side, INFINITY);
+                    got[q][2] = scan(p);
+                };
+                if (q) run(interior);
+                else run(keeps);
+                const double near = std::fabs(est - std::round(est));
+                slow += !(near > mg);
+                bad += got[q][0] != got[q][1] || got[q][0] != got[q][2];
+                ++dec;
+            }
+        }
+    }
+    *mismatch = bad;
+    *walked = slow;
+    return dec;
+}
