@@ -236,78 +236,162 @@ __device__ __forceinline__ int input_phase_idx(int slot, int i, int j) {
     return (slot * 4 + (((i & 1) << 1) | (j & 1))) * 256 + (i >> 1) * 16 + (j >> 1);
 }
 
-// One wave prepares one patch: gather into the u16 staging area P (side^2 values), ring
-// mean and maximum, bilinear resampling to 32 x 32 written as BF16 to xdst: row-major
-// (slot < 0; global memory) or at input_phase_idx(slot, ., .) (slot >= 0; the fused kernels'
-// LDS). Returns {B0, A0} through b0 / a0 (every lane).
+// Patch preparation by NTHR threads (one or two waves): thread t gathers pixels t, t + NTHR, ...
+// of the flattened patch into the u16 staging area P (side^2 values) and keeps their maximum.
+// A patch inside the image is read without clamps; the loads of kB steps are issued back to
+// back before their values are used (a border patch clamps every address into the image, so
+// its loads carry no branch either). Row and column advance by the quotient and remainder of
+// NTHR / side, the image offset by the matching number of rows and columns.
+template <int NTHR>
+__device__ __forceinline__ void prep_gather(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, float cx,
+                                            float cy, int hs, int t, uint16_t* P, float& vmax) {
+    const int side = 2 * hs + 1;
+    const int npx = side * side;
+    const int64_t ox = (int64_t)cx - hs, oy = (int64_t)cy - hs;
+    constexpr int kB = 8;
+    const int dq = NTHR / side, dr = NTHR - dq * side;
+    int row = t / side, col = t - row * side;
+    uint32_t m = 0u;
+    if (ox >= 0 && oy >= 0 && ox + side <= W && oy + side <= H) {
+        const uint16_t* first = img + oy * pitch + ox;
+        const uint16_t* q = first + (int64_t)row * pitch + col;
+        const int64_t step = (int64_t)dq * pitch + dr, wrap = pitch - side;
+        for (int p0 = 0; p0 < npx; p0 += NTHR * kB) {
+            uint16_t raw[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                // lanes past the patch's end read its first pixel
+                raw[j] = *(p0 + NTHR * j + t < npx ? q : first);
+                q += step;
+                col += dr;
+                if (col >= side) {
+                    col -= side;
+                    q += wrap;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int p = p0 + NTHR * j + t;
+                if (p < npx) P[p] = raw[j];
+                m = max(m, (uint32_t)raw[j]);
+            }
+        }
+    } else {
+        for (int p0 = 0; p0 < npx; p0 += NTHR * kB) {
+            uint16_t raw[kB];
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int64_t gx = min(max(ox + col, (int64_t)0), W - 1);
+                const int64_t gy = min(max(oy + min(row, side - 1), (int64_t)0), H - 1);
+                raw[j] = img[gy * pitch + gx];
+                col += dr;
+                row += dq;
+                if (col >= side) {
+                    col -= side;
+                    ++row;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kB; ++j) {
+                const int p = p0 + NTHR * j + t;
+                if (p < npx) {
+                    P[p] = raw[j];
+                    m = max(m, (uint32_t)raw[j]);
+                }
+            }
+        }
+    }
+    vmax = fmaxf(vmax, (float)m);
+}
+
+// Partial ring sum of the staged patch (its outer rows and columns, 4 side - 4 pixels) over
+// threads t of NTHR. Ring values are integers of at most 65535 and at most 248 of them, so
+// every partial sum is an integer below 2^24: exact in FP32 whichever threads add it up.
+template <int NTHR>
+__device__ __forceinline__ float prep_ring(const uint16_t* P, int side, int t) {
+    float rsum = 0.0f;
+    const int n = 4 * side - 4;
+    for (int r = t; r < n; r += NTHR) {
+        // top row, bottom row, then the left and right columns between them
+        int y, x;
+        if (r < side) {
+            y = 0;
+            x = r;
+        } else if (r < 2 * side) {
+            y = side - 1;
+            x = r - side;
+        } else {
+            const int e = r - 2 * side;
+            y = 1 + (e >> 1);
+            x = (e & 1) ? side - 1 : 0;
+        }
+        rsum += (float)P[y * side + x];
+    }
+    return rsum;
+}
+
+// Wave totals of the partial ring sums and maxima (every lane).
+__device__ __forceinline__ void prep_wave_reduce(float& rsum, float& vmax) {
+    for (int off = 32; off >= 1; off >>= 1) {
+        rsum += __shfl_xor(rsum, off, 64);
+        vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+    }
+}
+
+// B0 = ring mean, A0 = max - B0 (1 when flat).
+__device__ __forceinline__ void prep_levels(float rsum, int side, float vmax, float& b0, float& a0) {
+    b0 = rsum / (float)(4 * side - 4);
+    a0 = (vmax - b0) > 1.0f ? (vmax - b0) : 1.0f;
+}
+
+// Bilinear resampling of the staged patch onto the 32 x 32 grid, written as BF16
+// (v - B0) * (1 / A0) to xdst: row-major (slot < 0; global memory) or at
+// input_phase_idx(slot, ., .) (slot >= 0; the fused kernels' LDS). Thread t takes column
+// j = t % 32 (its x weights fixed) and rows t / 32 + k NTHR / 32. Grid point (j, i) lies at
+// patch coordinate (j + 0.5) side / 32 - 0.5, clamped into the patch; its left / upper
+// neighbour is at most side - 2, so the right / lower one is always the next pixel.
+template <int NTHR>
+__device__ __forceinline__ void prep_resample(const uint16_t* P, int side, float B0, float A0, int t,
+                                              uint16_t* xdst, int slot) {
+    static_assert(NTHR % 32 == 0, "a thread keeps one grid column");
+    constexpr int kStep = NTHR / 32;
+    const float scale = (float)side / 32.0f;
+    const float rA = 1.0f / A0;
+    const int j = t & 31, i0 = t >> 5;
+    const float px = fminf(fmaxf(((float)j + 0.5f) * scale - 0.5f, 0.0f), (float)(side - 1));
+    const int x0 = min((int)px, side - 2);
+    const float fx = px - (float)x0;
+    const int o0 = slot < 0 ? i0 * 32 + j : input_phase_idx(slot, i0, j);
+    constexpr int kDo = 32 * kStep;  // index step per row step, row-major
+    constexpr int kDp = 8 * kStep;   // the same in the phase-split layout (the row's phase is fixed)
+    const int dstep = slot < 0 ? kDo : kDp;
+#pragma unroll 4
+    for (int k = 0; k < 1024 / NTHR; ++k) {
+        const int i = i0 + kStep * k;
+        const float py = fminf(fmaxf(((float)i + 0.5f) * scale - 0.5f, 0.0f), (float)(side - 1));
+        const int y0 = min((int)py, side - 2);
+        const float fy = py - (float)y0;
+        const uint16_t* r0 = P + y0 * side + x0;
+        const uint16_t* r1 = r0 + side;
+        const float v = (1.0f - fy) * ((1.0f - fx) * (float)r0[0] + fx * (float)r0[1]) +
+                        fy * ((1.0f - fx) * (float)r1[0] + fx * (float)r1[1]);
+        xdst[o0 + dstep * k] = f2bf((v - B0) * rA);
+    }
+}
+
+// One wave prepares one patch (gather, ring mean and maximum, resampling); returns {B0, A0}
+// through b0 / a0 (every lane).
 __device__ __forceinline__ void prep_patch(const uint16_t* img, int64_t H, int64_t W, int64_t pitch, float cx,
                                            float cy, int hs, int lane, uint16_t* P, uint16_t* xdst, int slot,
                                            float& b0, float& a0) {
     const int side = 2 * hs + 1;
-    const int npx = side * side;
-    const int64_t ox = (int64_t)cx - hs, oy = (int64_t)cy - hs;
-    float rsum = 0.0f, rcnt = 0.0f, vmax = -3.0e38f;
-    // Lane-strided over the flattened patch, kB loads issued back to back before their values
-    // are used (every address clamped into the image, so the loads carry no branch); row and
-    // column advance by the quotient and remainder of 64 / side.
-    constexpr int kB = 8;
-    const int dq = 64 / side, dr = 64 - dq * side;
-    int row = lane / side, col = lane - row * side;
-    for (int p0 = 0; p0 < npx; p0 += 64 * kB) {
-        uint16_t raw[kB];
-        int rr[kB], cc[kB];
-#pragma unroll
-        for (int j = 0; j < kB; ++j) {
-            const int64_t gx = min(max(ox + col, (int64_t)0), W - 1);
-            const int64_t gy = min(max(oy + row, (int64_t)0), H - 1);
-            raw[j] = img[gy * pitch + gx];
-            rr[j] = row;
-            cc[j] = col;
-            col += dr;
-            row += dq;
-            if (col >= side) {
-                col -= side;
-                ++row;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kB; ++j) {
-            const int p = p0 + 64 * j + lane;
-            if (p < npx) {
-                const float v = (float)raw[j];
-                P[p] = raw[j];
-                if (rr[j] == 0 || rr[j] == side - 1 || cc[j] == 0 || cc[j] == side - 1) {
-                    rsum += v;
-                    rcnt += 1.0f;
-                }
-                vmax = fmaxf(vmax, v);
-            }
-        }
-    }
-    for (int off = 32; off >= 1; off >>= 1) {
-        rsum += __shfl_xor(rsum, off, 64);
-        rcnt += __shfl_xor(rcnt, off, 64);
-        vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
-    }
-    const float B0 = rsum / fmaxf(rcnt, 1.0f);
-    const float A0 = (vmax - B0) > 1.0f ? (vmax - B0) : 1.0f;
+    float vmax = -3.0e38f;
+    prep_gather<64>(img, H, W, pitch, cx, cy, hs, lane, P, vmax);
     __builtin_amdgcn_wave_barrier();
-    const float scale = (float)side / 32.0f;
-    for (int o = lane; o < 1024; o += 64) {
-        const int i = o >> 5, j = o & 31;
-        // grid point (j, i) at patch coordinate (j + 0.5) * side / 32 - 0.5
-        float px = ((float)j + 0.5f) * scale - 0.5f, py = ((float)i + 0.5f) * scale - 0.5f;
-        px = fminf(fmaxf(px, 0.0f), (float)(side - 1));
-        py = fminf(fmaxf(py, 0.0f), (float)(side - 1));
-        const int x0 = min((int)px, side - 2 >= 0 ? side - 2 : 0), y0 = min((int)py, side - 2 >= 0 ? side - 2 : 0);
-        const float fx = px - (float)x0, fy = py - (float)y0;
-        const int x1 = min(x0 + 1, side - 1), y1 = min(y0 + 1, side - 1);
-        const float v = (1.0f - fy) * ((1.0f - fx) * (float)P[y0 * side + x0] + fx * (float)P[y0 * side + x1]) +
-                        fy * ((1.0f - fx) * (float)P[y1 * side + x0] + fx * (float)P[y1 * side + x1]);
-        xdst[slot < 0 ? o : input_phase_idx(slot, i, j)] = f2bf((v - B0) / A0);
-    }
-    b0 = B0;
-    a0 = A0;
+    float rsum = prep_ring<64>(P, side, lane);
+    prep_wave_reduce(rsum, vmax);
+    prep_levels(rsum, side, vmax, b0, a0);
+    prep_resample<64>(P, side, b0, a0, lane, xdst, slot);
 }
 
 __global__ __launch_bounds__(256) void prepare_kernel(const uint16_t* img, int64_t H, int64_t W, int64_t pitch,
@@ -554,15 +638,17 @@ int forward(const uint16_t* x, int Bn, const uint16_t* wbf, const float* theta, 
 typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kNP = 4;          // patches per workgroup
-constexpr int kIW = 4;          // waves per workgroup (one per patch in the preparation and conv1)
+constexpr int kIW = 8;          // waves per workgroup (two per patch in conv1)
+static_assert(kIW % kNP == 0, "conv1 splits each patch's rows evenly over the waves");
 constexpr int kBufX = kNP * 2048;  // BF16 values: input, L2 out, L4 out
+constexpr int kZeroElems = 128;    // the zeroed block that reads outside a plane (or beyond K) go to
 constexpr int kBufY = kNP * 4096;  // BF16 values: raw patch staging, L1 out, L3 out, fc1 out
 static_assert(kNP * 1024 <= kBufX && kNP * 8 * 8 * 32 <= kBufX, "bufX");
 static_assert(kNP * 16 * 16 * 16 <= kBufY, "bufY");
 
 struct InferWeights {
-    const uint16_t* wt;      // W^T per layer 0..5: [n][kp], kp = K padded to a multiple of 32, zeros beyond K;
-                             // the head's 7 columns padded to 16 zero rows
+    const uint16_t* wt;      // W^T per layer 0..5 in MFMA fragment order (16 x 32 blocks of 512 values), K
+                             // padded to a multiple of 32 with zeros; the head's 7 columns padded to 16 zero rows
     int64_t off[6];          // element offset of layer l in wt
     int kp[6];               // padded K of layer l
     const float* theta;      // FP32 parameters (biases; K x N layout)
@@ -580,8 +666,14 @@ __global__ void transpose_weights_kernel(const uint16_t* wbf, InferWeights iw, u
         const int K = kNet[l].kd, N = kNet[l].n, Kp = iw.kp[l];
         const int64_t cnt = (int64_t)padded_cols(l) * Kp;
         if (t >= base && t < base + cnt) {
+            // fragment order: the 16 x 32 block (nb, kc) of W^T is 512 contiguous values, lane
+            // r + 16 c holding W^T[16 nb + r][32 kc + 8 c .. + 7], so a wave reads a block
+            // with one fully coalesced 1 KB load
             const int64_t q = t - base;
-            const int n = (int)(q / Kp), k = (int)(q - (int64_t)n * Kp);
+            const int j = (int)(q & 7), ln = (int)((q >> 3) & 63);
+            const int64_t blk = q >> 9;
+            const int KC = Kp / 32;
+            const int n = (int)(blk / KC) * 16 + (ln & 15), k = (int)(blk % KC) * 32 + 8 * (ln >> 4) + j;
             wt[iw.off[l] + q] = (k < K && n < N) ? wbf[param_offset_dev(l) + (int64_t)k * N + n] : (uint16_t)0;
         }
         base += cnt;
@@ -612,19 +704,22 @@ __device__ __forceinline__ bf16x8v load8(const uint16_t* p) { return *reinterpre
 // w % NB and G consecutive row blocks. The k loop is unrolled, so a chunk's tap, phase and
 // sub-pixel shift are constants (for CIN >= 32): tap (ky, kx) reads phase ((ky + 1) & 1,
 // (kx + 1) & 1) at the output pixel's sub-pixel moved up / left by one for ky = 0 / kx = 0,
-// outside the plane only there and only for the first output row / column.
+// outside the plane only there and only for the first output row / column. A fragment outside
+// the plane (or beyond K) is read from `zb`, a zeroed 16-byte block in LDS, so every load is
+// unconditional: the G loads of a chunk issue back to back ahead of its G MFMAs.
 template <int CIN, int COUT, int HIN, int G, bool NEXT_FC>
 __device__ __forceinline__ void conv_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
                                            const uint16_t* __restrict__ wt, const float* __restrict__ bias, int wave,
-                                           int lane) {
+                                           int lane, const uint16_t* zb) {
     constexpr int HOUT = HIN / 2, PIX = HOUT * HOUT, M = kNP * PIX, MB = M / 16, NB = COUT / 16;
     constexpr int KIN = 9 * CIN, KC = (KIN + 31) / 32, KP = KC * 32;
     constexpr int HS = HIN / 2, PLANE = HS * HS * CIN, CM = CIN / 8 - 1;
-    static_assert(MB * NB == kIW * G, "every wave takes G row blocks of one column block");
+    static_assert((MB * NB) % G == 0 && MB * NB / G <= kIW, "a wave takes G row blocks of one column block");
+    if (wave >= MB * NB / G) return;  // wave-uniform: more waves than blocks
     static_assert(CIN % 8 == 0 && HS == HOUT, "8-channel chunks; the output pixel indexes the sub-plane");
     const int li = lane & 15, g4 = lane >> 4;
     const int nb = wave % NB, rb0 = (wave / NB) * G;
-    const uint16_t* wcol = wt + (int64_t)(nb * 16 + li) * KP + 8 * g4;
+    const uint16_t* wcol = wt + ((int64_t)nb * KC * 64 + lane) * 8;  // fragment-ordered W^T
     int base[G], sp0[G];
     bool top[G], left[G];
 #pragma unroll
@@ -641,23 +736,24 @@ __device__ __forceinline__ void conv_layer(const uint16_t* __restrict__ in, uint
     for (int g = 0; g < G; ++g) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
-        const bf16x8v b = load8(wcol + kc * 32);
+        const bf16x8v b = load8(wcol + kc * 512);
         const int k0 = kc * 32 + 8 * g4;
         const int tap = k0 / CIN, chunk = (k0 % CIN) >> 3;
         const int ky = tap / 3, kx = tap - 3 * (tap / 3);
         const int off = ((((ky + 1) & 1) << 1) | ((kx + 1) & 1)) * PLANE;
         const int dsp = (ky == 0 ? -HS : 0) + (kx == 0 ? -1 : 0);
         const bool live = k0 < KIN;
+        bf16x8v a[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int sp = sp0[g] + dsp;
-            bf16x8v a = bf16x8v{};
-            if (live && !(ky == 0 && top[g]) && !(kx == 0 && left[g]))
-                a = load8(in + base[g] + off + sp * CIN + ((chunk ^ (sp & CM)) << 3));
-            // the transposed product W^T X^T: the weight fragment is the A operand, the
-            // activation fragment the B operand (the same registers as X W's B and A)
-            acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a, acc[g], 0, 0, 0);
+            const bool ok = live && !(ky == 0 && top[g]) && !(kx == 0 && left[g]);
+            a[g] = load8(ok ? in + base[g] + off + sp * CIN + ((chunk ^ (sp & CM)) << 3) : zb);
         }
+        // the transposed product W^T X^T: the weight fragment is the A operand, the
+        // activation fragment the B operand (the same registers as X W's B and A)
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[g], acc[g], 0, 0, 0);
     }
     // D^T(i, j): output channel nb * 16 + 4 (lane / 16) + r of output pixel (rb0 + g) * 16 +
     // lane % 16, so a lane holds four consecutive channels of one pixel: one 8-byte store
@@ -684,10 +780,14 @@ __device__ __forceinline__ void conv_layer(const uint16_t* __restrict__ in, uint
 // same taps' weights of channel l % 16 as the A operand (fixed for the patch); taps 9..15 are
 // zero. The result gives lane l channels 4 (l / 16) .. + 3 of pixel (j, l % 16): bias, ReLU,
 // BF16 and one 8-byte store. Products of BF16 values are exact; the FP32 sums are the MFMA's.
-// w1: the 9 x 16 weights as FP32 (BF16 values) and the 16 biases, in LDS.
+// w1: the 9 x 16 weights as FP32 (BF16 values) and the 16 biases, in LDS. Taps outside the
+// plane read the zeroed block zb; the taps of four output rows are loaded before their four
+// products.
 __device__ __forceinline__ void conv1_mfma(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
-                                           const float* __restrict__ w1, int wave, int lane) {
+                                           const float* __restrict__ w1, int wave, int lane, const uint16_t* zb) {
     const int li = lane & 15, g4 = lane >> 4;
+    constexpr int kRowsPerWave = 16 * kNP / kIW;
+    const int pt = wave % kNP, jb = (wave / kNP) * kRowsPerWave;  // patch and first output row
     bf16x4 wa;
     short tap_base[4];
     bool tap_live[4], tap_top[4];
@@ -700,22 +800,41 @@ __device__ __forceinline__ void conv1_mfma(const uint16_t* __restrict__ in, uint
         // input pixel (2 j + ky - 1, 2 li + kx - 1): phase ((ky + 1) & 1, (kx + 1) & 1), sub-pixel
         // (j - (ky == 0), li - (kx == 0)); the row offset 16 j is added per output row
         const int ph = (((ky + 1) & 1) << 1) | ((kx + 1) & 1);
-        tap_base[i] = (short)((wave * 4 + ph) * 256 + (ky == 0 ? -16 : 0) + li - (kx == 0 ? 1 : 0));
+        tap_base[i] = (short)((pt * 4 + ph) * 256 + (ky == 0 ? -16 : 0) + li - (kx == 0 ? 1 : 0));
         wa[i] = t < 9 ? (short)f2bf(w1[t * 16 + li]) : (short)0;
     }
     float bn[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) bn[r] = w1[144 + 4 * g4 + r];
-#pragma unroll 4
-    for (int j = 0; j < 16; ++j) {
-        bf16x4 xb;
+    // tap i of output row jb + r is read at tp[i] + 16 r: a dead tap reads the zeroed rows zb
+    // (kZeroRows rows of 16), the taps above the plane (ky = 0 for output row 0) read zb too
+    static_assert(16 * kRowsPerWave <= kZeroElems, "the zeroed block covers every row offset");
+    const uint16_t* tp[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            xb[i] = (tap_live[i] && !(tap_top[i] && j == 0)) ? (short)in[tap_base[i] + 16 * j] : (short)0;
-        const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa, xb, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
-        const uint32_t lo = f2bf2(fmaxf(d[0] + bn[0], 0.0f), fmaxf(d[1] + bn[1], 0.0f));
-        const uint32_t hi = f2bf2(fmaxf(d[2] + bn[2], 0.0f), fmaxf(d[3] + bn[3], 0.0f));
-        *reinterpret_cast<uint2*>(out + act_idx<16, 16>(wave, j, li, 4 * g4)) = make_uint2(lo, hi);
+    for (int i = 0; i < 4; ++i) tp[i] = tap_live[i] ? in + tap_base[i] + 16 * jb : zb;
+    constexpr int kRows = 4;
+#pragma unroll
+    for (int j0 = 0; j0 < kRowsPerWave; j0 += kRows) {
+        bf16x4 xb[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint16_t* a = tp[i];
+                if (j0 + r == 0) a = (jb == 0 && tap_top[i]) ? zb : a;
+                xb[r][i] = (short)a[16 * (j0 + r)];
+            }
+        }
+        f32x4 d[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+            d[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa, xb[r], f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            const uint32_t lo = f2bf2(fmaxf(d[r][0] + bn[0], 0.0f), fmaxf(d[r][1] + bn[1], 0.0f));
+            const uint32_t hi = f2bf2(fmaxf(d[r][2] + bn[2], 0.0f), fmaxf(d[r][3] + bn[3], 0.0f));
+            *reinterpret_cast<uint2*>(out + act_idx<16, 16>(pt, jb + j0 + r, li, 4 * g4)) = make_uint2(lo, hi);
+        }
     }
 }
 
@@ -723,19 +842,17 @@ __device__ __forceinline__ void conv1_mfma(const uint16_t* __restrict__ in, uint
 // column blocks go round the waves, the 8 k chunks alternating between two accumulators.
 __device__ __forceinline__ void fc1_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
                                           const uint16_t* __restrict__ wt, const float* __restrict__ bias, int wave,
-                                          int lane) {
+                                          int lane, const uint16_t* zb) {
     const int li = lane & 15, g4 = lane >> 4;
     for (int nb = wave; nb < 8; nb += kIW) {
-        const uint16_t* wcol = wt + (int64_t)(nb * 16 + li) * 256 + 8 * g4;
+        const uint16_t* wcol = wt + ((int64_t)nb * 8 * 64 + lane) * 8;  // fragment-ordered W^T, 8 k blocks
         f32x4 acc0 = f32x4{0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
 #pragma unroll
         for (int kc = 0; kc < 8; kc += 2) {
-            const bf16x8v b0 = load8(wcol + kc * 32), b1 = load8(wcol + kc * 32 + 32);
-            bf16x8v a0 = bf16x8v{}, a1 = bf16x8v{};
-            if (li < kNP) {
-                a0 = load8(in + fc_idx(li, kc * 32 + 8 * g4));
-                a1 = load8(in + fc_idx(li, kc * 32 + 32 + 8 * g4));
-            }
+            const bf16x8v b0 = load8(wcol + kc * 512), b1 = load8(wcol + kc * 512 + 512);
+            const bool row = li < kNP;  // rows beyond the patches read the zeroed block
+            const bf16x8v a0 = load8(row ? in + fc_idx(li, kc * 32 + 8 * g4) : zb);
+            const bf16x8v a1 = load8(row ? in + fc_idx(li, kc * 32 + 32 + 8 * g4) : zb);
             acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc1, 0, 0, 0);
         }
@@ -749,41 +866,45 @@ __device__ __forceinline__ void fc1_layer(const uint16_t* __restrict__ in, uint1
     }
 }
 
-// conv1's BF16 weights (as FP32) and biases into LDS; read after the first barrier.
-__device__ __forceinline__ void stage_conv1(const InferWeights& iw, float* w1) {
+// conv1's BF16 weights (as FP32) and biases into LDS, and the zeroed block that fragments
+// outside a plane read; both read after the first barrier.
+__device__ __forceinline__ void stage_conv1(const InferWeights& iw, float* w1, uint16_t* zb) {
     const int tid = threadIdx.x;
     if (tid < 144) w1[tid] = bf2f(iw.wbf[param_offset_dev(0) + tid]);
     else if (tid < 160) w1[tid] = iw.theta[param_offset_dev(0) + 144 + (tid - 144)];
+    else if (tid < 160 + kZeroElems) zb[tid - 160] = 0;
 }
 
 // Layers 1-6 on the kNP patches in bufX (input layout), then the outputs of patches p0 ..
 // p0 + kNP - 1 (< K): decoded with nrm[patch][5] when `decode`, raw otherwise.
 __device__ __forceinline__ void infer_net(uint16_t* bufX, uint16_t* bufY, float* head, const float* w1,
-                                          const InferWeights& iw, int K, int p0, bool decode, const float* nrm,
-                                          float* out) {
+                                          const uint16_t* zb, const InferWeights& iw, int K, int p0, bool decode,
+                                          const float* nrm, float* out) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const float* th = iw.theta;
-    conv1_mfma(bufX, bufY, w1, wave, lane);
+    conv1_mfma(bufX, bufY, w1, wave, lane, zb);
     __syncthreads();
-    conv_layer<16, 32, 16, 8, false>(bufY, bufX, iw.wt + iw.off[1], th + param_offset_dev(1) + 144 * 32, wave, lane);
+    conv_layer<16, 32, 16, 4, false>(bufY, bufX, iw.wt + iw.off[1], th + param_offset_dev(1) + 144 * 32, wave, lane,
+                                     zb);
     __syncthreads();
-    conv_layer<32, 64, 8, 4, false>(bufX, bufY, iw.wt + iw.off[2], th + param_offset_dev(2) + 288 * 64, wave, lane);
+    conv_layer<32, 64, 8, 2, false>(bufX, bufY, iw.wt + iw.off[2], th + param_offset_dev(2) + 288 * 64, wave, lane,
+                                    zb);
     __syncthreads();
-    conv_layer<64, 64, 4, 1, true>(bufY, bufX, iw.wt + iw.off[3], th + param_offset_dev(3) + 576 * 64, wave, lane);
+    conv_layer<64, 64, 4, 1, true>(bufY, bufX, iw.wt + iw.off[3], th + param_offset_dev(3) + 576 * 64, wave, lane,
+                                   zb);
     __syncthreads();
-    fc1_layer(bufX, bufY, iw.wt + iw.off[4], th + param_offset_dev(4) + 256 * 128, wave, lane);
+    fc1_layer(bufX, bufY, iw.wt + iw.off[4], th + param_offset_dev(4) + 256 * 128, wave, lane, zb);
     __syncthreads();
     // head 128 -> 7: one wave, rows = patches, columns = outputs (7 of a 16-column block), 4 k
     // chunks; FP32 outputs, no ReLU
     if (wave == 0) {
         const int li = lane & 15, g4 = lane >> 4;
-        const uint16_t* wcol = iw.wt + iw.off[5] + li * 128 + 8 * g4;
+        const uint16_t* wcol = iw.wt + iw.off[5] + lane * 8;  // fragment-ordered W^T, 4 k blocks
         f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc) {
-            const bf16x8v b = load8(wcol + kc * 32);
-            bf16x8v a = bf16x8v{};
-            if (li < kNP) a = load8(bufY + li * 128 + kc * 32 + 8 * g4);
+            const bf16x8v b = load8(wcol + kc * 512);
+            const bf16x8v a = load8(li < kNP ? bufY + li * 128 + kc * 32 + 8 * g4 : zb);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         }
         const int64_t wo = param_offset_dev(5);
@@ -805,15 +926,16 @@ __device__ __forceinline__ void infer_net(uint16_t* bufX, uint16_t* bufY, float*
     }
 }
 
-__global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(3))) void cnn_infer_kernel(const uint16_t* __restrict__ x, int K, InferWeights iw,
+__global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(6))) void cnn_infer_kernel(const uint16_t* __restrict__ x, int K, InferWeights iw,
                                                         const float* __restrict__ norm, float* __restrict__ out) {
     __shared__ __attribute__((aligned(16))) uint16_t bufX[kBufX];
     __shared__ __attribute__((aligned(16))) uint16_t bufY[kBufY];
     __shared__ float head[kNP * 8];
     __shared__ float w1[160];
+    __shared__ __attribute__((aligned(16))) uint16_t zb[kZeroElems];
     const int tid = threadIdx.x;
     const int p0 = blockIdx.x * kNP;
-    stage_conv1(iw, w1);
+    stage_conv1(iw, w1, zb);
     // input patches (row-major 32 x 32 BF16) into the phase-split layout; missing patches read 0
     for (int q = tid; q < kNP * 1024; q += 64 * kIW) {
         const int pt = q >> 10, e = q & 1023;
@@ -821,16 +943,15 @@ __global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(3))) v
         bufX[act_idx<32, 1>(pt, e >> 5, e & 31, 0)] = v;
     }
     __syncthreads();
-    infer_net(bufX, bufY, head, w1, iw, K, p0, norm != nullptr, norm ? norm + 5 * (int64_t)p0 : nullptr, out);
+    infer_net(bufX, bufY, head, w1, zb, iw, K, p0, norm != nullptr, norm ? norm + 5 * (int64_t)p0 : nullptr, out);
 }
 
 // Proposal in one launch: wave w prepares patch p0 + w (raw pixels staged in bufY, free until
 // the first layer writes it), writing the resampled input straight into bufX's layout, then
 // the network and the decode.
 static_assert(kNP * kMaxSide * kMaxSide <= kBufY, "patch staging must fit in bufY");
-static_assert(kNP == kIW, "one wave per patch");
 
-__global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(3))) void cnn_propose_kernel(const uint16_t* __restrict__ img, int64_t H, int64_t W,
+__global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(6))) void cnn_propose_kernel(const uint16_t* __restrict__ img, int64_t H, int64_t W,
                                                           int64_t pitch, const float* __restrict__ seeds,
                                                           const int16_t* __restrict__ half, int K, InferWeights iw,
                                                           float* __restrict__ out) {
@@ -838,28 +959,54 @@ __global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(3))) v
     __shared__ __attribute__((aligned(16))) uint16_t bufY[kBufY];
     __shared__ float head[kNP * 8];
     __shared__ float s_norm[kNP * 5];
+    __shared__ float s_part[kIW * 2];
     __shared__ float w1[160];
+    __shared__ __attribute__((aligned(16))) uint16_t zb[kZeroElems];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int p0 = blockIdx.x * kNP;
-    stage_conv1(iw, w1);
-    const int f = p0 + wave;
+    stage_conv1(iw, w1, zb);
+    // waves pt and pt + kNP prepare patch pt together: the gather, a barrier, then the
+    // resampling (the staged pixels in bufY, free until conv1 writes it)
+    static_assert(kIW == 2 * kNP, "two waves per patch in the preparation");
+    const int pt = wave % kNP, h = wave / kNP, t = lane + 64 * h;
+    const int f = p0 + pt;
+    int hs = 1;
+    float cx = 0.0f, cy = 0.0f;
+    uint16_t* P = bufY + pt * kMaxSide * kMaxSide;
+    float vmax = -3.0e38f;
     if (f < K) {
-        const int hs = __builtin_amdgcn_readfirstlane(max(1, min((int)half[f], (kMaxSide - 1) / 2)));
-        const float cx = rintf(seeds[2 * (int64_t)f]), cy = rintf(seeds[2 * (int64_t)f + 1]);
-        float B0, A0;
-        prep_patch(img, H, W, pitch, cx, cy, hs, lane, bufY + wave * kMaxSide * kMaxSide, bufX, wave, B0, A0);
-        if (lane == 0) {
-            s_norm[5 * wave + 0] = cx;
-            s_norm[5 * wave + 1] = cy;
-            s_norm[5 * wave + 2] = (float)(2 * hs + 1);
-            s_norm[5 * wave + 3] = B0;
-            s_norm[5 * wave + 4] = A0;
-        }
-    } else {
-        for (int o = lane; o < 1024; o += 64) bufX[act_idx<32, 1>(wave, o >> 5, o & 31, 0)] = 0;
+        hs = __builtin_amdgcn_readfirstlane(max(1, min((int)half[f], (kMaxSide - 1) / 2)));
+        cx = rintf(seeds[2 * (int64_t)f]);
+        cy = rintf(seeds[2 * (int64_t)f + 1]);
+        prep_gather<128>(img, H, W, pitch, cx, cy, hs, t, P, vmax);
     }
     __syncthreads();
-    infer_net(bufX, bufY, head, w1, iw, K, p0, true, s_norm, out);
+    if (f < K) {
+        float rsum = prep_ring<128>(P, 2 * hs + 1, t);
+        prep_wave_reduce(rsum, vmax);
+        if (lane == 0) {
+            s_part[2 * wave + 0] = rsum;
+            s_part[2 * wave + 1] = vmax;
+        }
+    }
+    __syncthreads();
+    if (f < K) {
+        const int o = 2 * pt, q = 2 * (pt + kNP);
+        float B0, A0;
+        prep_levels(s_part[o] + s_part[q], 2 * hs + 1, fmaxf(s_part[o + 1], s_part[q + 1]), B0, A0);
+        prep_resample<128>(P, 2 * hs + 1, B0, A0, t, bufX, pt);
+        if (t == 0) {
+            s_norm[5 * pt + 0] = cx;
+            s_norm[5 * pt + 1] = cy;
+            s_norm[5 * pt + 2] = (float)(2 * hs + 1);
+            s_norm[5 * pt + 3] = B0;
+            s_norm[5 * pt + 4] = A0;
+        }
+    } else {
+        for (int o = t; o < 1024; o += 128) bufX[act_idx<32, 1>(pt, o >> 5, o & 31, 0)] = 0;
+    }
+    __syncthreads();
+    infer_net(bufX, bufY, head, w1, zb, iw, K, p0, true, s_norm, out);
 }
 
 }  // namespace
