@@ -618,14 +618,17 @@ int forward(const uint16_t* x, int Bn, const uint16_t* wbf, const float* theta, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Fused inference: one workgroup of 4 waves takes kNP = 4 patches through all six layers,
-// activations staying in LDS (48 KB, three workgroups per CU). conv1 (1 -> 16, 74 kflop per
-// patch) runs on the VALU, one wave per patch, its 160 parameters wave-uniform. The other conv
-// layers and fc1 are implicit GEMMs on gfx950's
-// v_mfma_f32_16x16x32_bf16 (rows = (patch, output pixel), columns = output channels,
-// k = (ky, kx, c)); a wave owns one 16-column block and G row blocks, so each B fragment
-// (8 BF16 weights per lane, one 16-byte load from a transposed, K-padded copy W^T[n][k]) feeds
-// G MFMAs. Bias + ReLU + BF16 rounding in the epilogue, as the GEMM path of training.
+// Fused inference: one workgroup of 8 waves takes kNP = 4 patches through all six layers,
+// activations staying in LDS (49 KB, three workgroups per CU). conv1 (1 -> 16, 74 kflop per
+// patch) runs on v_mfma_f32_16x16x16_bf16, two waves per patch. The other conv layers and fc1
+// are implicit GEMMs on gfx950's v_mfma_f32_16x16x32_bf16 (rows = (patch, output pixel),
+// columns = output channels, k = (ky, kx, c)); a wave owns GC 16-column blocks and G row
+// blocks, so each weight fragment feeds G MFMAs and each activation fragment GC. The weights
+// are a transposed, K-padded copy W^T in fragment order: the 16 x 32 block of a (column block,
+// k chunk) is 512 contiguous BF16 values, lane r + 16 c holding row r, k 8 c .. 8 c + 7, so a
+// wave reads a fragment with one coalesced 1 KB load (with rows of W^T apart, each load
+// touched sixteen 128-byte lines for 1 KB and the layers waited on L2: 1.66 -> 1.00 ms for
+// 199,440 patches). Bias + ReLU + BF16 rounding in the epilogue, as the GEMM path of training.
 //
 // LDS layout of a layer input of side S and C channels (C >= 8): the four stride-2 phases of
 // the plane apart (phase (y & 1, x & 1), sub-pixel (y >> 1, x >> 1)), so the 16 rows of an A
@@ -700,26 +703,28 @@ __device__ __forceinline__ int fc_idx(int pt, int k) { return pt * 256 + ((((k >
 __device__ __forceinline__ bf16x8v load8(const uint16_t* p) { return *reinterpret_cast<const bf16x8v*>(p); }
 
 // One 3x3 stride-2 pad-1 conv layer of the kNP patches, `in` (side HIN, CIN channels) to `out`
-// (the next layer's input layout; NEXT_FC: the fc1 input). Wave w takes column block
-// w % NB and G consecutive row blocks. The k loop is unrolled, so a chunk's tap, phase and
+// (the next layer's input layout; NEXT_FC: the fc1 input). Wave w takes column blocks
+// GC (w % (NB / GC)) .. + GC - 1 and G consecutive row blocks. The k loop is unrolled, so a chunk's tap, phase and
 // sub-pixel shift are constants (for CIN >= 32): tap (ky, kx) reads phase ((ky + 1) & 1,
 // (kx + 1) & 1) at the output pixel's sub-pixel moved up / left by one for ky = 0 / kx = 0,
 // outside the plane only there and only for the first output row / column. A fragment outside
-// the plane (or beyond K) is read from `zb`, a zeroed 16-byte block in LDS, so every load is
-// unconditional: the G loads of a chunk issue back to back ahead of its G MFMAs.
-template <int CIN, int COUT, int HIN, int G, bool NEXT_FC>
+// the plane (or beyond K) is read from `zb`, a zeroed block in LDS, so every load is
+// unconditional: the G loads of a chunk issue back to back ahead of its G x GC MFMAs.
+template <int CIN, int COUT, int HIN, int G, int GC, bool NEXT_FC>
 __device__ __forceinline__ void conv_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
                                            const uint16_t* __restrict__ wt, const float* __restrict__ bias, int wave,
                                            int lane, const uint16_t* zb) {
     constexpr int HOUT = HIN / 2, PIX = HOUT * HOUT, M = kNP * PIX, MB = M / 16, NB = COUT / 16;
-    constexpr int KIN = 9 * CIN, KC = (KIN + 31) / 32, KP = KC * 32;
+    constexpr int KIN = 9 * CIN, KC = (KIN + 31) / 32;
     constexpr int HS = HIN / 2, PLANE = HS * HS * CIN, CM = CIN / 8 - 1;
-    static_assert((MB * NB) % G == 0 && MB * NB / G <= kIW, "a wave takes G row blocks of one column block");
-    if (wave >= MB * NB / G) return;  // wave-uniform: more waves than blocks
+    constexpr int NG = NB / GC;  // column-block groups
+    static_assert(NB % GC == 0 && MB % G == 0 && (MB / G) * NG <= kIW,
+                  "a wave takes G row blocks of GC column blocks");
+    if (wave >= (MB / G) * NG) return;  // wave-uniform: more waves than tiles
     static_assert(CIN % 8 == 0 && HS == HOUT, "8-channel chunks; the output pixel indexes the sub-plane");
     const int li = lane & 15, g4 = lane >> 4;
-    const int nb = wave % NB, rb0 = (wave / NB) * G;
-    const uint16_t* wcol = wt + ((int64_t)nb * KC * 64 + lane) * 8;  // fragment-ordered W^T
+    const int nb0 = (wave % NG) * GC, rb0 = (wave / NG) * G;
+    const uint16_t* wcol = wt + ((int64_t)nb0 * KC * 64 + lane) * 8;  // fragment-ordered W^T
     int base[G], sp0[G];
     bool top[G], left[G];
 #pragma unroll
@@ -731,12 +736,16 @@ __device__ __forceinline__ void conv_layer(const uint16_t* __restrict__ in, uint
         top[g] = pix < HOUT;
         left[g] = (pix % HOUT) == 0;
     }
-    f32x4 acc[G];
+    f32x4 acc[G][GC];
 #pragma unroll
-    for (int g = 0; g < G; ++g) acc[g] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int c = 0; c < GC; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
-        const bf16x8v b = load8(wcol + kc * 512);
+        bf16x8v b[GC];
+#pragma unroll
+        for (int c = 0; c < GC; ++c) b[c] = load8(wcol + (c * KC + kc) * 512);
         const int k0 = kc * 32 + 8 * g4;
         const int tap = k0 / CIN, chunk = (k0 % CIN) >> 3;
         const int ky = tap / 3, kx = tap - 3 * (tap / 3);
@@ -753,28 +762,34 @@ __device__ __forceinline__ void conv_layer(const uint16_t* __restrict__ in, uint
         // the transposed product W^T X^T: the weight fragment is the A operand, the
         // activation fragment the B operand (the same registers as X W's B and A)
 #pragma unroll
-        for (int g = 0; g < G; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[g], acc[g], 0, 0, 0);
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < GC; ++c)
+                acc[g][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[c], a[g], acc[g][c], 0, 0, 0);
     }
     // D^T(i, j): output channel nb * 16 + 4 (lane / 16) + r of output pixel (rb0 + g) * 16 +
     // lane % 16, so a lane holds four consecutive channels of one pixel: one 8-byte store
-    const int n0 = nb * 16 + 4 * g4;
-    float bn[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) bn[r] = bias[n0 + r];
+    for (int c = 0; c < GC; ++c) {
+        const int n0 = (nb0 + c) * 16 + 4 * g4;
+        float bn[4];
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const int m = (rb0 + g) * 16 + li;
-        const int p = m / PIX, pix = m - (m / PIX) * PIX;
-        const int y = pix / HOUT, x = pix - (pix / HOUT) * HOUT;
-        const uint32_t lo = f2bf2(fmaxf(acc[g][0] + bn[0], 0.0f), fmaxf(acc[g][1] + bn[1], 0.0f));
-        const uint32_t hi = f2bf2(fmaxf(acc[g][2] + bn[2], 0.0f), fmaxf(acc[g][3] + bn[3], 0.0f));
-        const int idx = NEXT_FC ? fc_idx(p, pix * COUT + n0) : act_idx<HOUT, COUT>(p, y, x, n0);
-        *reinterpret_cast<uint2*>(out + idx) = make_uint2(lo, hi);
+        for (int r = 0; r < 4; ++r) bn[r] = bias[n0 + r];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int m = (rb0 + g) * 16 + li;
+            const int p = m / PIX, pix = m - (m / PIX) * PIX;
+            const int y = pix / HOUT, x = pix - (pix / HOUT) * HOUT;
+            const uint32_t lo = f2bf2(fmaxf(acc[g][c][0] + bn[0], 0.0f), fmaxf(acc[g][c][1] + bn[1], 0.0f));
+            const uint32_t hi = f2bf2(fmaxf(acc[g][c][2] + bn[2], 0.0f), fmaxf(acc[g][c][3] + bn[3], 0.0f));
+            const int idx = NEXT_FC ? fc_idx(p, pix * COUT + n0) : act_idx<HOUT, COUT>(p, y, x, n0);
+            *reinterpret_cast<uint2*>(out + idx) = make_uint2(lo, hi);
+        }
     }
 }
 
-// conv1 (1 -> 16, 3x3 stride 2 pad 1, 32 x 32 -> 16 x 16) of patch `wave` on the matrix cores:
-// sixteen v_mfma_f32_16x16x16_bf16 products W^T X^T, one per output row j, with the 16
+// conv1 (1 -> 16, 3x3 stride 2 pad 1, 32 x 32 -> 16 x 16) of patch `wave % kNP`, output rows split
+// over kIW / kNP waves, on the matrix cores: one v_mfma_f32_16x16x16_bf16 product W^T X^T per output row j, with the 16
 // channels as rows, the row's 16 output pixels as columns and the 9 taps (padded to 16) as K.
 // Lane l holds taps 4 (l / 16) .. + 3 of output pixel (j, l % 16) as the B operand and the
 // same taps' weights of channel l % 16 as the A operand (fixed for the patch); taps 9..15 are
@@ -884,13 +899,13 @@ __device__ __forceinline__ void infer_net(uint16_t* bufX, uint16_t* bufY, float*
     const float* th = iw.theta;
     conv1_mfma(bufX, bufY, w1, wave, lane, zb);
     __syncthreads();
-    conv_layer<16, 32, 16, 4, false>(bufY, bufX, iw.wt + iw.off[1], th + param_offset_dev(1) + 144 * 32, wave, lane,
+    conv_layer<16, 32, 16, 2, 2, false>(bufY, bufX, iw.wt + iw.off[1], th + param_offset_dev(1) + 144 * 32, wave, lane,
                                      zb);
     __syncthreads();
-    conv_layer<32, 64, 8, 2, false>(bufX, bufY, iw.wt + iw.off[2], th + param_offset_dev(2) + 288 * 64, wave, lane,
+    conv_layer<32, 64, 8, 2, 1, false>(bufX, bufY, iw.wt + iw.off[2], th + param_offset_dev(2) + 288 * 64, wave, lane,
                                     zb);
     __syncthreads();
-    conv_layer<64, 64, 4, 1, true>(bufY, bufX, iw.wt + iw.off[3], th + param_offset_dev(3) + 576 * 64, wave, lane,
+    conv_layer<64, 64, 4, 1, 1, true>(bufY, bufX, iw.wt + iw.off[3], th + param_offset_dev(3) + 576 * 64, wave, lane,
                                    zb);
     __syncthreads();
     fc1_layer(bufX, bufY, iw.wt + iw.off[4], th + param_offset_dev(4) + 256 * 128, wave, lane, zb);
