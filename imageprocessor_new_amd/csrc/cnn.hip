@@ -951,11 +951,26 @@ __global__ __launch_bounds__(64 * kIW) __attribute__((amdgpu_waves_per_eu(6))) v
     const int tid = threadIdx.x;
     const int p0 = blockIdx.x * kNP;
     stage_conv1(iw, w1, zb);
-    // input patches (row-major 32 x 32 BF16) into the phase-split layout; missing patches read 0
-    for (int q = tid; q < kNP * 1024; q += 64 * kIW) {
-        const int pt = q >> 10, e = q & 1023;
-        const uint16_t v = p0 + pt < K ? x[(int64_t)(p0 + pt) * 1024 + e] : (uint16_t)0;
-        bufX[act_idx<32, 1>(pt, e >> 5, e & 31, 0)] = v;
+    // input patches (row-major 32 x 32 BF16) into the phase-split layout; missing patches read 0.
+    // With x 16-byte aligned a thread moves 8 values of a row in one load: the even columns go
+    // to one phase plane and the odd ones to the other, 4 consecutive values each.
+    static_assert(kNP * 1024 == 8 * 64 * kIW, "one 8-value piece per thread");
+    if ((reinterpret_cast<uintptr_t>(x) & 15u) == 0u) {
+        const int q = 8 * tid, pt = q >> 10, e = q & 1023, i = e >> 5, j = e & 31;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (p0 + pt < K) v = *reinterpret_cast<const uint4*>(x + (int64_t)(p0 + pt) * 1024 + e);
+        const uint2 even = make_uint2(__builtin_amdgcn_perm(v.y, v.x, 0x05040100u),
+                                      __builtin_amdgcn_perm(v.w, v.z, 0x05040100u));
+        const uint2 odd = make_uint2(__builtin_amdgcn_perm(v.y, v.x, 0x07060302u),
+                                     __builtin_amdgcn_perm(v.w, v.z, 0x07060302u));
+        *reinterpret_cast<uint2*>(bufX + act_idx<32, 1>(pt, i, j, 0)) = even;
+        *reinterpret_cast<uint2*>(bufX + act_idx<32, 1>(pt, i, j + 1, 0)) = odd;
+    } else {
+        for (int q = tid; q < kNP * 1024; q += 64 * kIW) {
+            const int pt = q >> 10, e = q & 1023;
+            const uint16_t v = p0 + pt < K ? x[(int64_t)(p0 + pt) * 1024 + e] : (uint16_t)0;
+            bufX[act_idx<32, 1>(pt, e >> 5, e & 31, 0)] = v;
+        }
     }
     __syncthreads();
     infer_net(bufX, bufY, head, w1, zb, iw, K, p0, norm != nullptr, norm ? norm + 5 * (int64_t)p0 : nullptr, out);
