@@ -860,7 +860,7 @@ __device__ __forceinline__ void conv1_mfma(const uint16_t* __restrict__ in, uint
 #pragma unroll
     for (int r = 0; r < 4; ++r) bn[r] = w1[144 + 4 * g4 + r];
     // tap i of output row jb + r is read at tp[i] + 16 r: a dead tap reads the zeroed rows zb
-    // (kZeroRows rows of 16), the taps above the plane (ky = 0 for output row 0) read zb too
+    // (kZeroElems values: every row offset 16 r stays inside it), the taps above the plane (ky = 0 for output row 0) read zb too
     static_assert(16 * kRowsPerWave <= kZeroElems, "the zeroed block covers every row offset");
     const uint16_t* tp[4];
 #pragma unroll
